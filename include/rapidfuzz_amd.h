@@ -1,0 +1,157 @@
+/*
+ * rapidfuzz_amd.h -- C ABI of the MI355X (gfx950) bit-parallel distance engine.
+ *
+ * The reference (rapidfuzz-cpp 3.3.3) is a header-only C++ template library
+ * with no C FFI; its "operator API" is the scorer classes.  Each entry point
+ * below names the reference interface whose work it takes over.  Plain
+ * pointers and sizes only; no torch or HIP types cross this boundary (the
+ * stream is handed out as an opaque pointer).
+ *
+ * Strings are sequences of unsigned code units of width 1, 2 or 4 bytes,
+ * passed packed: chars[offsets[i] .. offsets[i+1]) is string i.
+ *
+ * Threading: a context owns one HIP stream; calls on one context are
+ * serialized by an internal mutex, calls on different contexts may run
+ * concurrently.  Scorers and string sets are immutable after creation, like
+ * the reference's const scorer objects (README.md:216-236).
+ *
+ * Errors: every call returns an rfa_status; rfa_last_error() gives the
+ * message of the last failed call on the calling thread.  Messages reuse the
+ * reference's exception texts where the reference throws
+ * (details/distance.hpp:426-427, distance/Levenshtein.hpp:338-339,372).
+ */
+#ifndef RAPIDFUZZ_AMD_H
+#define RAPIDFUZZ_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RFA_ABI_VERSION 1
+
+typedef enum {
+    RFA_OK = 0,
+    RFA_ERR_INVALID_ARGUMENT = 1, /* the reference throws std::invalid_argument */
+    RFA_ERR_UNSUPPORTED = 2,      /* outside the accelerated path (e.g. general weights) */
+    RFA_ERR_HIP = 3,              /* HIP runtime error */
+    RFA_ERR_OUT_OF_MEMORY = 4,
+    RFA_ERR_NO_DEVICE = 5
+} rfa_status;
+
+/* Metric of a scorer (the reference's scorer classes). */
+typedef enum {
+    RFA_LEVENSHTEIN = 0,  /* CachedLevenshtein / experimental::MultiLevenshtein */
+    RFA_INDEL = 1,        /* CachedIndel / experimental::MultiIndel */
+    RFA_LCSSEQ = 2,       /* CachedLCSseq / experimental::MultiLCSseq */
+    RFA_OSA = 3,          /* CachedOSA / experimental::MultiOSA */
+    RFA_JARO = 4,         /* CachedJaro / experimental::MultiJaro */
+    RFA_JARO_WINKLER = 5, /* CachedJaroWinkler / experimental::MultiJaroWinkler */
+    RFA_RATIO = 6         /* fuzz::CachedRatio / fuzz::experimental::MultiRatio */
+} rfa_metric;
+
+/* Scorer method (details/distance.hpp:211-301, 326-356). */
+typedef enum {
+    RFA_DISTANCE = 0,
+    RFA_SIMILARITY = 1,
+    RFA_NORMALIZED_DISTANCE = 2,
+    RFA_NORMALIZED_SIMILARITY = 3
+} rfa_kind;
+
+/* Which reference class's result semantics to reproduce. */
+typedef enum {
+    RFA_CACHED = 0, /* Cached*: cutoff applied inside (details/distance.hpp:210-388) */
+    RFA_MULTI = 1   /* experimental::Multi*: full score, cutoff after (details/distance.hpp:390-582) */
+} rfa_semantics;
+
+/* Pair schedules for rfa_pairs (the uneven batch). */
+typedef enum {
+    RFA_SCHEDULE_BUCKETED = 0, /* length-bucketed persistent launch (the default) */
+    RFA_SCHEDULE_PLAIN = 1     /* one pair per lane, all words sequential in the lane */
+} rfa_schedule;
+
+/* LevenshteinWeightTable (details/types.hpp:19-23). */
+typedef struct {
+    uint64_t insert_cost;
+    uint64_t delete_cost;
+    uint64_t replace_cost;
+} rfa_weights;
+
+typedef struct rfa_context rfa_context;
+typedef struct rfa_strings rfa_strings;
+typedef struct rfa_scorer rfa_scorer;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* One HIP stream plus memory on `device`.  No reference counterpart: the
+ * reference runs inline on the caller's thread. */
+rfa_status rfa_context_create(int device, rfa_context** out);
+rfa_status rfa_context_destroy(rfa_context* ctx);
+rfa_status rfa_context_synchronize(rfa_context* ctx);
+/* The context's hipStream_t, for callers that time or order work. */
+void* rfa_context_stream(rfa_context* ctx);
+/* Message of the last failed call on this thread ("" if none). */
+const char* rfa_last_error(void);
+int rfa_abi_version(void);
+
+/* ---- device-resident string sets ---------------------------------------- */
+
+/* Copies `count` packed strings to the device.  char_width is 1, 2 or 4.
+ * Replaces the per-call Range<InputIt> views of s2 (details/Range.hpp:73-186)
+ * with one resident set scored many times. */
+rfa_status rfa_strings_create(rfa_context* ctx, const void* chars, int char_width, const uint64_t* offsets,
+                              uint64_t count, rfa_strings** out);
+rfa_status rfa_strings_destroy(rfa_strings* set);
+uint64_t rfa_strings_count(const rfa_strings* set);
+
+/* ---- scorers ------------------------------------------------------------ */
+
+/* A scorer holding one cached query per string of `queries` -- the shared
+ * state of Cached<Metric><CharT>(s1) (e.g. distance/Levenshtein.hpp:424-432,
+ * Indel.hpp:149-159, LCSseq.hpp:197-206, OSA.hpp:232-241,
+ * JaroWinkler.hpp:168-178, fuzz.hpp:99-118) and of
+ * experimental::Multi<Metric><MaxLen>(count) after count inserts
+ * (Levenshtein.hpp:334-380).  weights may be NULL ({1,1,1}); only Levenshtein
+ * reads it.  prefix_weight is read by Jaro-Winkler only.
+ * Levenshtein weights must be uniform ({w,w,w}) or Indel-like (ins == del,
+ * replace >= ins + del); other weights return RFA_ERR_UNSUPPORTED. */
+rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* queries, const rfa_weights* weights,
+                             double prefix_weight, rfa_scorer** out);
+rfa_status rfa_scorer_destroy(rfa_scorer* scorer);
+
+/* All queries against all choices: out[q * n_choices + c] is
+ *   Cached<Metric>(query q).<kind>(choice c, cutoff)        (semantics RFA_CACHED)
+ *   Multi<Metric>.<kind>(scores, ..., choice c, cutoff)[q]  (semantics RFA_MULTI)
+ * Integer kinds of Levenshtein/Indel/LCSseq/OSA write uint64_t with
+ * cutoff_u (INT64_MAX = the reference default); every other combination
+ * writes double with cutoff_d.  `out` is a device pointer when
+ * out_on_device != 0, else host memory.  Replaces the caller's loop of
+ * scorer.distance(s2) calls (README.md:153-214; distance/Levenshtein.hpp:383-399
+ * for the Multi* per-text call). */
+rfa_status rfa_cdist(rfa_scorer* scorer, const rfa_strings* choices, int kind, int semantics, uint64_t cutoff_u,
+                     double cutoff_d, void* out, int out_on_device);
+
+/* Independent pairs: out[i] = metrics[i](a[i], b[i]).<kind>(cutoff) with
+ * Cached* semantics; metrics[i] is one of Levenshtein (weights {1,1,1}),
+ * Indel, LCSseq, OSA.  Replaces per-pair calls of the free functions
+ * levenshtein_distance / indel_distance (Levenshtein.hpp:140-265,
+ * Indel.hpp:20-60).  `metrics` is host memory; a and b must hold the same
+ * number of strings and the same char width. */
+rfa_status rfa_pairs(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics, int kind,
+                     uint64_t cutoff_u, double cutoff_d, int schedule, void* out, int out_on_device);
+
+/* Top-k extraction (the README's extract() loop, README.md:153-214): for every
+ * query, the k best choices by the scorer's `kind` with cutoff_d, ordered by
+ * (score descending, index ascending).  Only double-valued kinds are
+ * accepted (normalized kinds, Jaro(-Winkler), ratio).  Slots beyond the
+ * number of qualifying choices get index -1 and score 0.
+ * out_index[q * k + j], out_score[q * k + j]: host memory. */
+rfa_status rfa_topk(rfa_scorer* scorer, const rfa_strings* choices, int kind, double cutoff_d, uint32_t k,
+                    int64_t* out_index, double* out_score);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAPIDFUZZ_AMD_H */

@@ -1,0 +1,219 @@
+"""ctypes bindings for the checker libraries under oracle/ (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this module.  It loads
+
+* ``librf_oracle.so`` -- the C restatement of the reference semantics, and
+* ``_ref/librfref.so`` -- the reference's own headers compiled in place
+  (optional: present where it was built from the reference sources).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "librf_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "librfref.so")
+
+LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO = range(7)
+DISTANCE, SIMILARITY, NORM_DISTANCE, NORM_SIMILARITY = range(4)
+INT64_MAX = (1 << 63) - 1
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def to_codes(s) -> np.ndarray:
+    """A string-like value as uint32 code units."""
+    if isinstance(s, np.ndarray):
+        return s.astype(np.uint32, copy=False)
+    if isinstance(s, (bytes, bytearray)):
+        return np.frombuffer(bytes(s), dtype=np.uint8).astype(np.uint32)
+    if isinstance(s, str):
+        return np.array([ord(c) for c in s], dtype=np.uint32)
+    return np.asarray(list(s), dtype=np.uint32)
+
+
+def pack(strings, dtype=np.uint32):
+    """Pack strings into (chars, offsets[n+1])."""
+    arrs = [to_codes(s) for s in strings]
+    lens = np.array([len(a) for a in arrs], dtype=np.uint64)
+    off = np.zeros(len(arrs) + 1, dtype=np.uint64)
+    np.cumsum(lens, out=off[1:])
+    chars = np.concatenate(arrs).astype(dtype) if arrs and off[-1] else np.zeros(1, dtype=dtype)
+    return np.ascontiguousarray(chars), off
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def is_int_result(metric: int, kind: int) -> bool:
+    return metric in (LEVENSHTEIN, INDEL, LCSSEQ, OSA) and kind in (DISTANCE, SIMILARITY)
+
+
+class Oracle:
+    """The C restatement (rf_oracle.c)."""
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run `make -C oracle`)")
+        self.lib = ctypes.CDLL(path)
+        L = self.lib
+        L.orc_cdist.restype = ctypes.c_int
+        L.orc_cdist.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p, _u64p, ctypes.c_size_t,
+                                _u32p, _u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64,
+                                ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
+                                _u64p, _f64p]
+        L.orc_pairs.restype = ctypes.c_int
+        L.orc_pairs.argtypes = [_i32p, ctypes.c_int, _u32p, _u64p, _u32p, _u64p, ctypes.c_size_t,
+                                ctypes.c_uint64, ctypes.c_double, _u64p, _f64p]
+        L.orc_jaro_raw.restype = None
+        L.orc_jaro_raw.argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _u64p, _u64p]
+        for name in ("orc_levenshtein_raw",):
+            getattr(L, name).restype = ctypes.c_uint64
+            getattr(L, name).argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, ctypes.c_uint64,
+                                         ctypes.c_uint64, ctypes.c_uint64]
+        for name in ("orc_lcs_raw", "orc_osa_raw"):
+            getattr(L, name).restype = ctypes.c_uint64
+            getattr(L, name).argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t]
+
+    def cdist(self, metric, kind, queries, choices, weights=(1, 1, 1), prefix_weight=0.1,
+              cutoff_u=INT64_MAX, cutoff_d=None, multi=False):
+        qc, qo = pack(queries)
+        cc, co = pack(choices)
+        nq, nc = len(qo) - 1, len(co) - 1
+        if cutoff_d is None:
+            cutoff_d = default_fp_cutoff(metric, kind)
+        ou = np.zeros(nq * nc, dtype=np.uint64)
+        od = np.zeros(nq * nc, dtype=np.float64)
+        r = self.lib.orc_cdist(metric, kind, int(multi), _p(qc, _u32p), _p(qo, _u64p), nq, _p(cc, _u32p),
+                               _p(co, _u64p), nc, weights[0], weights[1], weights[2], prefix_weight,
+                               cutoff_u, cutoff_d, _p(ou, _u64p), _p(od, _f64p))
+        if r < 0:
+            raise ValueError(f"unsupported metric/kind {metric}/{kind}")
+        return (ou if r == 0 else od).reshape(nq, nc)
+
+    def score(self, metric, kind, s1, s2, **kw):
+        return self.cdist(metric, kind, [s1], [s2], **kw)[0, 0]
+
+    def pairs(self, metrics, a, b, kind=DISTANCE, cutoff_u=INT64_MAX, cutoff_d=1.0):
+        ac, ao = pack(a)
+        bc, bo = pack(b)
+        n = len(ao) - 1
+        m = np.ascontiguousarray(np.asarray(metrics, dtype=np.int32))
+        ou = np.zeros(n, dtype=np.uint64)
+        od = np.zeros(n, dtype=np.float64)
+        r = self.lib.orc_pairs(_p(m, _i32p), kind, _p(ac, _u32p), _p(ao, _u64p), _p(bc, _u32p),
+                               _p(bo, _u64p), n, cutoff_u, cutoff_d, _p(ou, _u64p), _p(od, _f64p))
+        if r < 0:
+            raise ValueError("unsupported")
+        return ou if r == 0 else od
+
+    def jaro_raw(self, p, t):
+        pa, ta = to_codes(p), to_codes(t)
+        pa = np.ascontiguousarray(pa if len(pa) else np.zeros(1, np.uint32))
+        ta = np.ascontiguousarray(ta if len(ta) else np.zeros(1, np.uint32))
+        cc = np.zeros(1, np.uint64)
+        tr = np.zeros(1, np.uint64)
+        self.lib.orc_jaro_raw(_p(pa, _u32p), len(to_codes(p)), _p(ta, _u32p), len(to_codes(t)),
+                              _p(cc, _u64p), _p(tr, _u64p))
+        return int(cc[0]), int(tr[0])
+
+
+def default_fp_cutoff(metric, kind):
+    """The reference's default double cutoff for a method (details/distance.hpp:211-233)."""
+    if kind in (NORM_DISTANCE,):
+        return 1.0
+    if kind == DISTANCE and metric in (JARO, JARO_WINKLER):
+        return 1.0
+    return 0.0
+
+
+class Reference:
+    """The reference's own headers compiled in place (oracle/_ref/librfref.so)."""
+
+    def __init__(self, path: str = REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run `make -C oracle ref`)")
+        self.lib = ctypes.CDLL(path)
+        L = self.lib
+        L.ref_last_error.restype = ctypes.c_char_p
+        L.ref_cdist_cached.restype = ctypes.c_int
+        L.ref_cdist_cached.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p, _u64p,
+                                       ctypes.c_size_t, _u32p, _u64p, ctypes.c_size_t, ctypes.c_uint64,
+                                       ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64,
+                                       ctypes.c_double, _u64p, _f64p]
+        L.ref_cdist_multi.restype = ctypes.c_int
+        L.ref_cdist_multi.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p, _u64p, ctypes.c_size_t,
+                                      _u32p, _u64p, ctypes.c_size_t, ctypes.c_double, ctypes.c_uint64,
+                                      ctypes.c_double, _u64p, _f64p]
+        for name in ("ref_cdist_cached_u8_mt", "ref_cdist_multi_u8_mt"):
+            f = getattr(L, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [ctypes.c_int, ctypes.c_int, _u8p, _u64p, ctypes.c_size_t, _u8p, _u64p,
+                          ctypes.c_size_t, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, _u64p, _f64p,
+                          ctypes.c_int]
+        L.ref_pairs_u32_mt.restype = ctypes.c_int
+        L.ref_pairs_u32_mt.argtypes = [_i32p, _u32p, _u64p, _u32p, _u64p, ctypes.c_size_t, _u64p,
+                                       ctypes.c_int]
+
+    def _err(self):
+        return self.lib.ref_last_error().decode()
+
+    def cdist(self, metric, kind, queries, choices, weights=(1, 1, 1), prefix_weight=0.1,
+              cutoff_u=INT64_MAX, cutoff_d=None, char_width=4, multi_len=None):
+        qc, qo = pack(queries)
+        cc, co = pack(choices)
+        nq, nc = len(qo) - 1, len(co) - 1
+        if cutoff_d is None:
+            cutoff_d = default_fp_cutoff(metric, kind)
+        ou = np.zeros(nq * nc, dtype=np.uint64)
+        od = np.zeros(nq * nc, dtype=np.float64)
+        if multi_len:
+            r = self.lib.ref_cdist_multi(metric, kind, multi_len, _p(qc, _u32p), _p(qo, _u64p), nq,
+                                         _p(cc, _u32p), _p(co, _u64p), nc, prefix_weight, cutoff_u, cutoff_d,
+                                         _p(ou, _u64p), _p(od, _f64p))
+        else:
+            r = self.lib.ref_cdist_cached(metric, kind, char_width, _p(qc, _u32p), _p(qo, _u64p), nq,
+                                          _p(cc, _u32p), _p(co, _u64p), nc, weights[0], weights[1],
+                                          weights[2], prefix_weight, cutoff_u, cutoff_d, _p(ou, _u64p),
+                                          _p(od, _f64p))
+        if r < 0:
+            raise ValueError(f"reference error {r}: {self._err()}")
+        return (ou if r == 0 else od).reshape(nq, nc)
+
+    def cdist_u8_mt(self, metric, kind, qchars, qoff, cchars, coff, cutoff_u=INT64_MAX, cutoff_d=None,
+                    prefix_weight=0.1, threads=None, multi=False):
+        """Threaded u8 cdist (the CPU baseline); inputs already packed as uint8."""
+        nq, nc = len(qoff) - 1, len(coff) - 1
+        if cutoff_d is None:
+            cutoff_d = default_fp_cutoff(metric, kind)
+        threads = threads or os.cpu_count() or 1
+        ou = np.zeros(nq * nc, dtype=np.uint64)
+        od = np.zeros(nq * nc, dtype=np.float64)
+        f = self.lib.ref_cdist_multi_u8_mt if multi else self.lib.ref_cdist_cached_u8_mt
+        r = f(metric, kind, _p(qchars, _u8p), _p(qoff, _u64p), nq, _p(cchars, _u8p), _p(coff, _u64p), nc,
+              prefix_weight, cutoff_u, cutoff_d, _p(ou, _u64p), _p(od, _f64p), threads)
+        if r < 0:
+            raise ValueError(f"reference error {r}: {self._err()}")
+        return (ou if is_int_result(metric, kind) else od).reshape(nq, nc)
+
+    def pairs_mt(self, metrics, achars, aoff, bchars, boff, threads=None):
+        n = len(aoff) - 1
+        threads = threads or os.cpu_count() or 1
+        m = np.ascontiguousarray(np.asarray(metrics, dtype=np.int32))
+        out = np.zeros(n, dtype=np.uint64)
+        self.lib.ref_pairs_u32_mt(_p(m, _i32p), _p(achars, _u32p), _p(aoff, _u64p), _p(bchars, _u32p),
+                                  _p(boff, _u64p), n, _p(out, _u64p), threads)
+        return out
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_SO)

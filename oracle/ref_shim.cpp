@@ -1,0 +1,484 @@
+// ref_shim.cpp -- extern "C" wrapper around the reference's own headers.
+//
+// TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile against the
+// reference sources where they lie (-I <reference root>), output only into
+// oracle/_ref/.  It gives the tests a second, independent checker (the real
+// reference library) and gives bench.py a "reference" CPU baseline.  The
+// product library never links or loads it.
+//
+// Entry points mirror the reference call shapes:
+//   Cached<Metric><CharT>(s1).<kind>(s2, cutoff)         (e.g. Levenshtein.hpp:421-482)
+//   experimental::Multi<Metric><MaxLen>(n).insert / .<kind>(scores, n, s2, cutoff)
+//                                                        (e.g. Levenshtein.hpp:297-419)
+#include <rapidfuzz/distance.hpp>
+#include <rapidfuzz/fuzz.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6 };
+enum { DIST = 0, SIM = 1, NDIST = 2, NSIM = 3 };
+
+thread_local std::string g_err;
+
+template <typename CharT>
+std::vector<CharT> take(const uint32_t* s, size_t n)
+{
+    std::vector<CharT> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = static_cast<CharT>(s[i]);
+    return v;
+}
+
+template <typename Scorer, typename It>
+int run_int_metric(const Scorer& sc, int kind, It b, It e, uint64_t cu, double cd, uint64_t* ou, double* od)
+{
+    switch (kind) {
+    case DIST: *ou = sc.distance(b, e, static_cast<size_t>(cu)); return 0;
+    case SIM: *ou = sc.similarity(b, e, static_cast<size_t>(cu)); return 0;
+    case NDIST: *od = sc.normalized_distance(b, e, cd); return 1;
+    case NSIM: *od = sc.normalized_similarity(b, e, cd); return 1;
+    }
+    return -1;
+}
+
+template <typename Scorer, typename It>
+int run_fp_metric(const Scorer& sc, int kind, It b, It e, double cd, double* od)
+{
+    switch (kind) {
+    case DIST: *od = sc.distance(b, e, cd); return 1;
+    case SIM: *od = sc.similarity(b, e, cd); return 1;
+    case NDIST: *od = sc.normalized_distance(b, e, cd); return 1;
+    case NSIM: *od = sc.normalized_similarity(b, e, cd); return 1;
+    }
+    return -1;
+}
+
+// A cached scorer for one query, dispatching on the metric at run time.
+template <typename CharT>
+struct AnyCached {
+    int metric;
+    std::vector<CharT> s1;
+    rapidfuzz::LevenshteinWeightTable w;
+    double pw;
+    rapidfuzz::CachedLevenshtein<CharT>* lev = nullptr;
+    rapidfuzz::CachedIndel<CharT>* indel = nullptr;
+    rapidfuzz::CachedLCSseq<CharT>* lcs = nullptr;
+    rapidfuzz::CachedOSA<CharT>* osa = nullptr;
+    rapidfuzz::CachedJaro<CharT>* jaro = nullptr;
+    rapidfuzz::CachedJaroWinkler<CharT>* jw = nullptr;
+    rapidfuzz::fuzz::CachedRatio<CharT>* ratio = nullptr;
+
+    AnyCached(int m, std::vector<CharT> q, rapidfuzz::LevenshteinWeightTable w_, double pw_)
+        : metric(m), s1(std::move(q)), w(w_), pw(pw_)
+    {
+        switch (metric) {
+        case LEV: lev = new rapidfuzz::CachedLevenshtein<CharT>(s1, w); break;
+        case INDEL: indel = new rapidfuzz::CachedIndel<CharT>(s1); break;
+        case LCS: lcs = new rapidfuzz::CachedLCSseq<CharT>(s1); break;
+        case OSA: osa = new rapidfuzz::CachedOSA<CharT>(s1); break;
+        case JARO: jaro = new rapidfuzz::CachedJaro<CharT>(s1); break;
+        case JW: jw = new rapidfuzz::CachedJaroWinkler<CharT>(s1, pw); break;
+        case RATIO: ratio = new rapidfuzz::fuzz::CachedRatio<CharT>(s1); break;
+        default: throw std::invalid_argument("unknown metric");
+        }
+    }
+    ~AnyCached()
+    {
+        delete lev;
+        delete indel;
+        delete lcs;
+        delete osa;
+        delete jaro;
+        delete jw;
+        delete ratio;
+    }
+    template <typename It>
+    int score(int kind, It b, It e, uint64_t cu, double cd, uint64_t* ou, double* od) const
+    {
+        switch (metric) {
+        case LEV: return run_int_metric(*lev, kind, b, e, cu, cd, ou, od);
+        case INDEL: return run_int_metric(*indel, kind, b, e, cu, cd, ou, od);
+        case LCS: return run_int_metric(*lcs, kind, b, e, cu, cd, ou, od);
+        case OSA: return run_int_metric(*osa, kind, b, e, cu, cd, ou, od);
+        case JARO: return run_fp_metric(*jaro, kind, b, e, cd, od);
+        case JW: return run_fp_metric(*jw, kind, b, e, cd, od);
+        case RATIO:
+            if (kind != SIM) return -1;
+            *od = ratio->similarity(b, e, cd);
+            return 1;
+        }
+        return -1;
+    }
+};
+
+template <typename CharT>
+int cdist_cached_impl(int metric, int kind, const uint32_t* q, const uint64_t* qoff, size_t nq,
+                      const uint32_t* c, const uint64_t* coff, size_t nc, uint64_t ins, uint64_t del,
+                      uint64_t rep, double pw, uint64_t cu, double cd, uint64_t* ou, double* od)
+{
+    std::vector<std::vector<CharT>> choices(nc);
+    for (size_t j = 0; j < nc; ++j) choices[j] = take<CharT>(c + coff[j], coff[j + 1] - coff[j]);
+    int r = -1;
+    for (size_t i = 0; i < nq; ++i) {
+        AnyCached<CharT> sc(metric, take<CharT>(q + qoff[i], qoff[i + 1] - qoff[i]),
+                            {static_cast<size_t>(ins), static_cast<size_t>(del), static_cast<size_t>(rep)},
+                            pw);
+        for (size_t j = 0; j < nc; ++j) {
+            uint64_t u = 0;
+            double d = 0;
+            r = sc.score(kind, choices[j].data(), choices[j].data() + choices[j].size(), cu, cd, &u, &d);
+            if (r < 0) return r;
+            if (r == 0)
+                ou[i * nc + j] = u;
+            else
+                od[i * nc + j] = d;
+        }
+    }
+    return r;
+}
+
+// ---- experimental::Multi* ----
+template <int MaxLen>
+struct AnyMulti {
+    int metric;
+    size_t n;
+    rapidfuzz::experimental::MultiLevenshtein<MaxLen>* lev = nullptr;
+    rapidfuzz::experimental::MultiIndel<MaxLen>* indel = nullptr;
+    rapidfuzz::experimental::MultiLCSseq<MaxLen>* lcs = nullptr;
+    rapidfuzz::experimental::MultiOSA<MaxLen>* osa = nullptr;
+    rapidfuzz::experimental::MultiJaro<MaxLen>* jaro = nullptr;
+    rapidfuzz::experimental::MultiJaroWinkler<MaxLen>* jw = nullptr;
+    rapidfuzz::fuzz::experimental::MultiRatio<MaxLen>* ratio = nullptr;
+
+    AnyMulti(int m, size_t n_, double pw) : metric(m), n(n_)
+    {
+        switch (metric) {
+        case LEV: lev = new rapidfuzz::experimental::MultiLevenshtein<MaxLen>(n); break;
+        case INDEL: indel = new rapidfuzz::experimental::MultiIndel<MaxLen>(n); break;
+        case LCS: lcs = new rapidfuzz::experimental::MultiLCSseq<MaxLen>(n); break;
+        case OSA: osa = new rapidfuzz::experimental::MultiOSA<MaxLen>(n); break;
+        case JARO: jaro = new rapidfuzz::experimental::MultiJaro<MaxLen>(n); break;
+        case JW: jw = new rapidfuzz::experimental::MultiJaroWinkler<MaxLen>(n, pw); break;
+        case RATIO: ratio = new rapidfuzz::fuzz::experimental::MultiRatio<MaxLen>(n); break;
+        default: throw std::invalid_argument("unknown metric");
+        }
+    }
+    ~AnyMulti()
+    {
+        delete lev;
+        delete indel;
+        delete lcs;
+        delete osa;
+        delete jaro;
+        delete jw;
+        delete ratio;
+    }
+    size_t result_count() const
+    {
+        switch (metric) {
+        case LEV: return lev->result_count();
+        case INDEL: return indel->result_count();
+        case LCS: return lcs->result_count();
+        case OSA: return osa->result_count();
+        case JARO: return jaro->result_count();
+        case JW: return jw->result_count();
+        case RATIO: return ratio->result_count();
+        }
+        return 0;
+    }
+    template <typename S>
+    void insert(const S& s)
+    {
+        switch (metric) {
+        case LEV: lev->insert(s); break;
+        case INDEL: indel->insert(s); break;
+        case LCS: lcs->insert(s); break;
+        case OSA: osa->insert(s); break;
+        case JARO: jaro->insert(s); break;
+        case JW: jw->insert(s); break;
+        case RATIO: ratio->insert(s); break;
+        }
+    }
+    template <typename M, typename It>
+    static int run_int(const M& m, int kind, size_t* su, double* sd, size_t cnt, It b, It e, uint64_t cu,
+                       double cd)
+    {
+        switch (kind) {
+        case DIST: m.distance(su, cnt, b, e, static_cast<size_t>(cu)); return 0;
+        case SIM: m.similarity(su, cnt, b, e, static_cast<size_t>(cu)); return 0;
+        case NDIST: m.normalized_distance(sd, cnt, b, e, cd); return 1;
+        case NSIM: m.normalized_similarity(sd, cnt, b, e, cd); return 1;
+        }
+        return -1;
+    }
+    template <typename M, typename It>
+    static int run_fp(const M& m, int kind, double* sd, size_t cnt, It b, It e, double cd)
+    {
+        switch (kind) {
+        case DIST: m.distance(sd, cnt, b, e, cd); return 1;
+        case SIM: m.similarity(sd, cnt, b, e, cd); return 1;
+        case NDIST: m.normalized_distance(sd, cnt, b, e, cd); return 1;
+        case NSIM: m.normalized_similarity(sd, cnt, b, e, cd); return 1;
+        }
+        return -1;
+    }
+    template <typename It>
+    int score(int kind, size_t* su, double* sd, size_t cnt, It b, It e, uint64_t cu, double cd) const
+    {
+        switch (metric) {
+        case LEV: return run_int(*lev, kind, su, sd, cnt, b, e, cu, cd);
+        case INDEL: return run_int(*indel, kind, su, sd, cnt, b, e, cu, cd);
+        case LCS: return run_int(*lcs, kind, su, sd, cnt, b, e, cu, cd);
+        case OSA: return run_int(*osa, kind, su, sd, cnt, b, e, cu, cd);
+        case JARO: return run_fp(*jaro, kind, sd, cnt, b, e, cd);
+        case JW: return run_fp(*jw, kind, sd, cnt, b, e, cd);
+        case RATIO:
+            if (kind != SIM) return -1;
+            ratio->similarity(sd, cnt, b, e, cd);
+            return 1;
+        }
+        return -1;
+    }
+};
+
+template <int MaxLen, typename CharT>
+int multi_cdist_impl(int metric, int kind, const uint32_t* q, const uint64_t* qoff, size_t nq,
+                     const uint32_t* c, const uint64_t* coff, size_t nc, double pw, uint64_t cu,
+                     double cd, uint64_t* ou, double* od)
+{
+    AnyMulti<MaxLen> m(metric, nq, pw);
+    for (size_t i = 0; i < nq; ++i) m.insert(take<CharT>(q + qoff[i], qoff[i + 1] - qoff[i]));
+    size_t rc = m.result_count();
+    std::vector<size_t> su(rc);
+    std::vector<double> sd(rc);
+    int r = -1;
+    for (size_t j = 0; j < nc; ++j) {
+        auto s2 = take<CharT>(c + coff[j], coff[j + 1] - coff[j]);
+        r = m.score(kind, su.data(), sd.data(), rc, s2.data(), s2.data() + s2.size(), cu, cd);
+        if (r < 0) return r;
+        for (size_t i = 0; i < nq; ++i) {
+            if (r == 0)
+                ou[i * nc + j] = su[i];
+            else
+                od[i * nc + j] = sd[i];
+        }
+    }
+    return r;
+}
+
+// ---- threaded u8 cdist used as the CPU baseline ----
+struct U8Set {
+    const uint8_t* chars;
+    const uint64_t* off;
+    size_t n;
+};
+
+int cdist_cached_u8_threaded(int metric, int kind, U8Set q, U8Set c, double pw, uint64_t cu, double cd,
+                             uint64_t* ou, double* od, int nthreads)
+{
+    std::atomic<size_t> next(0);
+    std::atomic<int> status(0);
+    auto worker = [&]() {
+        for (;;) {
+            size_t i = next.fetch_add(1);
+            if (i >= q.n) break;
+            std::vector<uint8_t> s1(q.chars + q.off[i], q.chars + q.off[i + 1]);
+            AnyCached<uint8_t> sc(metric, s1, {1, 1, 1}, pw);
+            for (size_t j = 0; j < c.n; ++j) {
+                const uint8_t* b = c.chars + c.off[j];
+                const uint8_t* e = c.chars + c.off[j + 1];
+                uint64_t u = 0;
+                double d = 0;
+                int r = sc.score(kind, b, e, cu, cd, &u, &d);
+                if (r < 0) {
+                    status = -1;
+                    return;
+                }
+                if (r == 0)
+                    ou[i * c.n + j] = u;
+                else
+                    od[i * c.n + j] = d;
+            }
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nthreads; ++t) ts.emplace_back(worker);
+    for (auto& t : ts) t.join();
+    return status.load();
+}
+
+// Multi* over length buckets (MaxLen 8/16/32/64), threads split the choices.
+template <int MaxLen>
+void multi_bucket_u8(int metric, int kind, const std::vector<size_t>& idx, U8Set q, U8Set c, double pw,
+                     uint64_t cu, double cd, uint64_t* ou, double* od, int nthreads)
+{
+    if (idx.empty()) return;
+    AnyMulti<MaxLen> m(metric, idx.size(), pw);
+    for (size_t k = 0; k < idx.size(); ++k) {
+        size_t i = idx[k];
+        m.insert(std::vector<uint8_t>(q.chars + q.off[i], q.chars + q.off[i + 1]));
+    }
+    size_t rc = m.result_count();
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+        std::vector<size_t> su(rc);
+        std::vector<double> sd(rc);
+        const size_t chunk = 256;
+        for (;;) {
+            size_t j0 = next.fetch_add(chunk);
+            if (j0 >= c.n) break;
+            size_t j1 = std::min(c.n, j0 + chunk);
+            for (size_t j = j0; j < j1; ++j) {
+                int r = m.score(kind, su.data(), sd.data(), rc, c.chars + c.off[j], c.chars + c.off[j + 1], cu,
+                                cd);
+                for (size_t k = 0; k < idx.size(); ++k) {
+                    if (r == 0)
+                        ou[idx[k] * c.n + j] = su[k];
+                    else
+                        od[idx[k] * c.n + j] = sd[k];
+                }
+            }
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nthreads; ++t) ts.emplace_back(worker);
+    for (auto& t : ts) t.join();
+}
+
+} // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_err.c_str(); }
+
+// Cached* cdist over arbitrary code units; char_width picks CharT (1, 2 or 4).
+int ref_cdist_cached(int metric, int kind, int char_width, const uint32_t* q, const uint64_t* qoff,
+                     size_t nq, const uint32_t* c, const uint64_t* coff, size_t nc, uint64_t ins,
+                     uint64_t del, uint64_t rep, double pw, uint64_t cu, double cd, uint64_t* ou,
+                     double* od)
+{
+    try {
+        if (char_width == 1)
+            return cdist_cached_impl<uint8_t>(metric, kind, q, qoff, nq, c, coff, nc, ins, del, rep, pw, cu,
+                                              cd, ou, od);
+        if (char_width == 2)
+            return cdist_cached_impl<uint16_t>(metric, kind, q, qoff, nq, c, coff, nc, ins, del, rep, pw,
+                                               cu, cd, ou, od);
+        return cdist_cached_impl<uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, ins, del, rep, pw, cu,
+                                           cd, ou, od);
+    }
+    catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    }
+}
+
+// experimental::Multi<Metric><max_len> cdist: all queries inserted into one scorer.
+int ref_cdist_multi(int metric, int kind, int max_len, const uint32_t* q, const uint64_t* qoff, size_t nq,
+                    const uint32_t* c, const uint64_t* coff, size_t nc, double pw, uint64_t cu, double cd,
+                    uint64_t* ou, double* od)
+{
+    try {
+        switch (max_len) {
+        case 8: return multi_cdist_impl<8, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od);
+        case 16:
+            return multi_cdist_impl<16, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od);
+        case 32:
+            return multi_cdist_impl<32, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od);
+        case 64:
+            return multi_cdist_impl<64, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od);
+        }
+        g_err = "max_len must be 8, 16, 32 or 64";
+        return -1;
+    }
+    catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    }
+}
+
+// CPU baseline: Cached* per query, queries spread over nthreads std::threads.
+int ref_cdist_cached_u8_mt(int metric, int kind, const uint8_t* q, const uint64_t* qoff, size_t nq,
+                           const uint8_t* c, const uint64_t* coff, size_t nc, double pw, uint64_t cu,
+                           double cd, uint64_t* ou, double* od, int nthreads)
+{
+    try {
+        return cdist_cached_u8_threaded(metric, kind, {q, qoff, nq}, {c, coff, nc}, pw, cu, cd, ou, od,
+                                        nthreads);
+    }
+    catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    }
+}
+
+// CPU baseline: AVX2 Multi* with queries bucketed by MaxLen 8/16/32/64.
+int ref_cdist_multi_u8_mt(int metric, int kind, const uint8_t* q, const uint64_t* qoff, size_t nq,
+                          const uint8_t* c, const uint64_t* coff, size_t nc, double pw, uint64_t cu,
+                          double cd, uint64_t* ou, double* od, int nthreads)
+{
+    try {
+        std::vector<size_t> b8, b16, b32, b64;
+        for (size_t i = 0; i < nq; ++i) {
+            size_t l = qoff[i + 1] - qoff[i];
+            if (l <= 8)
+                b8.push_back(i);
+            else if (l <= 16)
+                b16.push_back(i);
+            else if (l <= 32)
+                b32.push_back(i);
+            else if (l <= 64)
+                b64.push_back(i);
+            else {
+                g_err = "query longer than 64";
+                return -1;
+            }
+        }
+        U8Set qs{q, qoff, nq}, cs{c, coff, nc};
+        multi_bucket_u8<8>(metric, kind, b8, qs, cs, pw, cu, cd, ou, od, nthreads);
+        multi_bucket_u8<16>(metric, kind, b16, qs, cs, pw, cu, cd, ou, od, nthreads);
+        multi_bucket_u8<32>(metric, kind, b32, qs, cs, pw, cu, cd, ou, od, nthreads);
+        multi_bucket_u8<64>(metric, kind, b64, qs, cs, pw, cu, cd, ou, od, nthreads);
+        return 0;
+    }
+    catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    }
+}
+
+// CPU baseline for independent pairs: free functions per pair.
+int ref_pairs_u32_mt(const int32_t* metrics, const uint32_t* a, const uint64_t* aoff, const uint32_t* b,
+                     const uint64_t* boff, size_t n, uint64_t* out, int nthreads)
+{
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+        const size_t chunk = 64;
+        for (;;) {
+            size_t i0 = next.fetch_add(chunk);
+            if (i0 >= n) break;
+            size_t i1 = std::min(n, i0 + chunk);
+            for (size_t i = i0; i < i1; ++i) {
+                if (metrics[i] == LEV)
+                    out[i] = rapidfuzz::levenshtein_distance(a + aoff[i], a + aoff[i + 1], b + boff[i],
+                                                             b + boff[i + 1]);
+                else
+                    out[i] = rapidfuzz::indel_distance(a + aoff[i], a + aoff[i + 1], b + boff[i], b + boff[i + 1]);
+            }
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nthreads; ++t) ts.emplace_back(worker);
+    for (auto& t : ts) t.join();
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,613 @@
+/*
+ * rf_oracle.c -- CPU restatement of rapidfuzz-cpp scoring (TEST INFRASTRUCTURE).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load
+ * this file's library, and only as a checker.  The product path never does.
+ *
+ * Raw values are naive dynamic programs restated from the reference's own
+ * oracle directory; the API layer restates the result semantics of the
+ * reference's CRTP bases step by step.  Each function names the reference
+ * lines it follows.
+ */
+#include "rf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static uint64_t u64min(uint64_t a, uint64_t b) { return a < b ? a : b; }
+static uint64_t u64max(uint64_t a, uint64_t b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+static double dmax(double a, double b) { return a > b ? a : b; }
+
+/* rapidfuzz/details/intrinsics.hpp ceil_div */
+static uint64_t ceil_div(uint64_t a, uint64_t b) { return a / b + (uint64_t)(a % b != 0); }
+
+/* ------------------------------------------------------------------------ */
+/* raw values                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Wagner-Fischer with weights: rapidfuzz_reference/Levenshtein.hpp:31-56
+ * (column-by-column with one rolling column instead of the full matrix). */
+uint64_t orc_levenshtein_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2,
+                             uint64_t ins, uint64_t del, uint64_t rep)
+{
+    uint64_t* col = (uint64_t*)malloc((l1 + 1) * sizeof(uint64_t));
+    for (size_t i = 0; i <= l1; ++i) col[i] = (uint64_t)i * del;
+    for (size_t j = 0; j < l2; ++j) {
+        uint64_t diag = col[0];
+        col[0] = (uint64_t)(j + 1) * ins;
+        for (size_t i = 0; i < l1; ++i) {
+            uint64_t cost = (s1[i] == s2[j]) ? 0 : rep;
+            uint64_t up = col[i + 1]; /* D[i+1][j] */
+            uint64_t v = up + ins;    /* insertion of s2[j] */
+            uint64_t a = col[i] + del; /* deletion of s1[i] (D[i][j+1]) */
+            uint64_t b = diag + cost;
+            if (a < v) v = a;
+            if (b < v) v = b;
+            diag = up;
+            col[i + 1] = v;
+        }
+    }
+    uint64_t d = col[l1];
+    free(col);
+    return d;
+}
+
+/* rapidfuzz_reference/LCSseq.hpp:9-17: LCS = (len1 + len2 - Indel) / 2 with
+ * Indel = Levenshtein with weights {1, 1, 2} (rapidfuzz_reference/Indel.hpp:11-16). */
+uint64_t orc_lcs_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2)
+{
+    uint64_t indel = orc_levenshtein_raw(s1, l1, s2, l2, 1, 1, 2);
+    return ((uint64_t)l1 + (uint64_t)l2 - indel) / 2;
+}
+
+/* rapidfuzz_reference/OSA.hpp:13-43 (rolling three rows over s2). */
+uint64_t orc_osa_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2)
+{
+    size_t w = l2 + 1;
+    uint64_t* m = (uint64_t*)malloc(3 * w * sizeof(uint64_t));
+    uint64_t *prev2 = m, *prev = m + w, *cur = m + 2 * w;
+    for (size_t j = 0; j <= l2; ++j) prev[j] = j;
+    for (size_t i = 0; i < l1; ++i) {
+        cur[0] = i + 1;
+        for (size_t j = 0; j < l2; ++j) {
+            uint64_t cost = (s1[i] == s2[j]) ? 0 : 1;
+            uint64_t v = prev[j + 1] + 1;
+            if (cur[j] + 1 < v) v = cur[j] + 1;
+            if (prev[j] + cost < v) v = prev[j] + cost;
+            if (i > 0 && j > 0 && s1[i] == s2[j - 1] && s1[i - 1] == s2[j]) {
+                if (prev2[j - 1] + cost < v) v = prev2[j - 1] + cost;
+            }
+            cur[j + 1] = v;
+        }
+        uint64_t* t = prev2;
+        prev2 = prev;
+        prev = cur;
+        cur = t;
+    }
+    uint64_t d = prev[l2];
+    free(m);
+    return d;
+}
+
+/* rapidfuzz_reference/Jaro.hpp:22-56: greedy matching window and
+ * transposition walk.  Writes the common-character count and the number of
+ * mismatched flagged pairs (before the halving of the Jaro formula). */
+void orc_jaro_raw(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, uint64_t* common,
+                  uint64_t* transpositions)
+{
+    *common = 0;
+    *transpositions = 0;
+    if (!Pl || !Tl) return;
+    unsigned char* Pf = (unsigned char*)calloc(Pl + 1, 1);
+    unsigned char* Tf = (unsigned char*)calloc(Tl + 1, 1);
+    size_t Bound = (Pl > Tl ? Pl : Tl) / 2;
+    if (Bound > 0) Bound--;
+    uint64_t cc = 0;
+    for (size_t i = 0; i < Tl; i++) {
+        size_t lowlim = (i >= Bound) ? i - Bound : 0;
+        size_t hilim = (i + Bound <= Pl - 1) ? (i + Bound) : Pl - 1;
+        for (size_t j = lowlim; j <= hilim; j++) {
+            if (!Pf[j] && P[j] == T[i]) {
+                Tf[i] = 1;
+                Pf[j] = 1;
+                cc++;
+                break;
+            }
+        }
+    }
+    uint64_t tr = 0;
+    size_t k = 0;
+    for (size_t i = 0; i < Tl; i++) {
+        if (Tf[i]) {
+            size_t j = k;
+            for (; j < Pl; j++) {
+                if (Pf[j]) {
+                    k = j + 1;
+                    break;
+                }
+            }
+            if (T[i] != P[j]) tr++;
+        }
+    }
+    free(Pf);
+    free(Tf);
+    *common = cc;
+    *transpositions = tr;
+}
+
+/* rapidfuzz/distance/Jaro_impl.hpp:32-42 (operation order kept). */
+static double jaro_formula(size_t Pl, size_t Tl, uint64_t cc, uint64_t tr)
+{
+    tr /= 2;
+    double Sim = 0;
+    Sim += (double)cc / (double)Pl;
+    Sim += (double)cc / (double)Tl;
+    Sim += ((double)cc - (double)tr) / (double)cc;
+    return Sim / 3.0;
+}
+
+/* Library-level Jaro with cutoff: rapidfuzz/distance/Jaro_impl.hpp:409-454
+ * with the filters of :47-71.  The filters are kept literally so the double
+ * results match bit for bit. */
+double orc_jaro_similarity(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, double cutoff)
+{
+    if (cutoff > 1.0) return 0.0;
+    if (!Pl && !Tl) return 1.0;
+    /* jaro_length_filter */
+    if (!Tl || !Pl) return 0.0;
+    {
+        double min_len = (double)(Pl < Tl ? Pl : Tl);
+        double Sim = min_len / (double)Pl + min_len / (double)Tl + 1.0;
+        Sim /= 3.0;
+        if (!(Sim >= cutoff)) return 0.0;
+    }
+    if (Pl == 1 && Tl == 1) return (double)(P[0] == T[0]);
+    uint64_t cc, tr;
+    orc_jaro_raw(P, Pl, T, Tl, &cc, &tr);
+    /* jaro_common_char_filter */
+    if (!cc) return 0.0;
+    {
+        double Sim = 0;
+        Sim += (double)cc / (double)Pl;
+        Sim += (double)cc / (double)Tl;
+        Sim += 1.0;
+        Sim /= 3.0;
+        if (!(Sim >= cutoff)) return 0.0;
+    }
+    double Sim = jaro_formula(Pl, Tl, cc, tr);
+    return (Sim >= cutoff) ? Sim : 0;
+}
+
+static size_t common_prefix4(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl)
+{
+    size_t max_prefix = Pl < Tl ? Pl : Tl;
+    if (max_prefix > 4) max_prefix = 4;
+    size_t prefix = 0;
+    for (; prefix < max_prefix; ++prefix)
+        if (T[prefix] != P[prefix]) break;
+    return prefix;
+}
+
+/* rapidfuzz/distance/JaroWinkler_impl.hpp:41-71 (cached path). */
+double orc_jaro_winkler_similarity(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl,
+                                   double prefix_weight, double cutoff)
+{
+    size_t prefix = common_prefix4(P, Pl, T, Tl);
+    double jaro_cutoff = cutoff;
+    if (jaro_cutoff > 0.7) {
+        double prefix_sim = (double)prefix * prefix_weight;
+        if (prefix_sim >= 1.0)
+            jaro_cutoff = 0.7;
+        else
+            jaro_cutoff = dmax(0.7, (prefix_sim - jaro_cutoff) / (prefix_sim - 1.0));
+    }
+    double Sim = orc_jaro_similarity(P, Pl, T, Tl, jaro_cutoff);
+    if (Sim > 0.7) {
+        Sim += (double)prefix * prefix_weight * (1.0 - Sim);
+        Sim = dmin(Sim, 1.0);
+    }
+    return (Sim >= cutoff) ? Sim : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cached* API layer                                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int metric;
+    const uint32_t *s1, *s2;
+    size_t l1, l2;
+    uint64_t ins, del, rep;
+    double pw;
+} Pair;
+
+/* rapidfuzz/distance/Levenshtein_impl.hpp:118-128 */
+static uint64_t lev_maximum(uint64_t l1, uint64_t l2, uint64_t ins, uint64_t del, uint64_t rep)
+{
+    uint64_t m = l1 * del + l2 * ins;
+    if (l1 >= l2)
+        m = u64min(m, l2 * rep + (l1 - l2) * del);
+    else
+        m = u64min(m, l1 * rep + (l2 - l1) * ins);
+    return m;
+}
+
+/* CachedLCSseq::_similarity -> lcs_seq_similarity (LCSseq_impl.hpp:375-403):
+ * the exact LCS when it reaches the cutoff, else 0. */
+static uint64_t lcs_cached_similarity(const Pair* p, uint64_t c)
+{
+    if (c > p->l1 || c > p->l2) return 0;
+    uint64_t lcs = orc_lcs_raw(p->s1, p->l1, p->s2, p->l2);
+    return lcs >= c ? lcs : 0;
+}
+
+/* detail::indel_distance(PM, s1, s2, cutoff) (Indel_impl.hpp:15-24) and
+ * CachedIndel::_distance (Indel.hpp:171-179). */
+static uint64_t indel_cached_distance(const Pair* p, uint64_t c)
+{
+    uint64_t maximum = (uint64_t)p->l1 + (uint64_t)p->l2;
+    uint64_t lcs_cutoff = (maximum / 2 >= c) ? maximum / 2 - c : 0;
+    uint64_t lcs_sim = lcs_cached_similarity(p, lcs_cutoff);
+    uint64_t dist = maximum - 2 * lcs_sim;
+    return (dist <= c) ? dist : c + 1;
+}
+
+/* uniform_levenshtein_distance result contract (Levenshtein_impl.hpp:852-905):
+ * exact distance when <= the clamped cutoff, else cutoff + 1. */
+static uint64_t lev_uniform_cached(const Pair* p, uint64_t c)
+{
+    c = u64min(c, u64max(p->l1, p->l2));
+    uint64_t d = orc_levenshtein_raw(p->s1, p->l1, p->s2, p->l2, 1, 1, 1);
+    return d <= c ? d : c + 1;
+}
+
+/* CachedLevenshtein::_distance (Levenshtein.hpp:445-477). */
+static uint64_t lev_cached_distance(const Pair* p, uint64_t c)
+{
+    if (p->ins == p->del) {
+        if (p->ins == 0) return 0;
+        if (p->ins == p->rep) {
+            uint64_t nc = ceil_div(c, p->ins);
+            uint64_t dist = lev_uniform_cached(p, nc) * p->ins;
+            return (dist <= c) ? dist : c + 1;
+        }
+        else if (p->rep >= p->ins + p->del) {
+            uint64_t nc = ceil_div(c, p->ins);
+            uint64_t dist = indel_cached_distance(p, nc) * p->ins;
+            return (dist <= c) ? dist : c + 1;
+        }
+    }
+    /* generalized_levenshtein_distance (Levenshtein_impl.hpp:141-152) */
+    uint64_t min_edits = p->l1 > p->l2 ? (p->l1 - p->l2) * p->del : (p->l2 - p->l1) * p->ins;
+    if (min_edits > c) return c + 1;
+    uint64_t d = orc_levenshtein_raw(p->s1, p->l1, p->s2, p->l2, p->ins, p->del, p->rep);
+    return d <= c ? d : c + 1;
+}
+
+/* CachedOSA::_distance (OSA.hpp:253-267). */
+static uint64_t osa_cached_distance(const Pair* p, uint64_t c)
+{
+    uint64_t res;
+    if (p->l1 == 0)
+        res = p->l2;
+    else if (p->l2 == 0)
+        res = p->l1;
+    else {
+        res = orc_osa_raw(p->s1, p->l1, p->s2, p->l2);
+        res = res <= c ? res : c + 1;
+    }
+    return (res <= c) ? res : c + 1;
+}
+
+static uint64_t int_maximum(const Pair* p)
+{
+    switch (p->metric) {
+    case ORC_LEVENSHTEIN: return lev_maximum(p->l1, p->l2, p->ins, p->del, p->rep);
+    case ORC_INDEL: return (uint64_t)p->l1 + (uint64_t)p->l2;
+    case ORC_LCSSEQ:
+    case ORC_OSA: return u64max(p->l1, p->l2);
+    }
+    return 0;
+}
+
+/* derived._distance for distance-type integer metrics */
+static uint64_t int_distance_core(const Pair* p, uint64_t c)
+{
+    switch (p->metric) {
+    case ORC_LEVENSHTEIN: return lev_cached_distance(p, c);
+    case ORC_INDEL: return indel_cached_distance(p, c);
+    case ORC_OSA: return osa_cached_distance(p, c);
+    }
+    return 0;
+}
+
+/* CachedSimilarityBase::_distance for LCSseq (details/distance.hpp:359-369). */
+static uint64_t lcs_cached_distance(const Pair* p, uint64_t c)
+{
+    uint64_t maximum = int_maximum(p);
+    uint64_t cs = (maximum > c) ? maximum - c : 0;
+    uint64_t sim = lcs_cached_similarity(p, cs);
+    uint64_t dist = maximum - sim;
+    return (dist <= c) ? dist : c + 1;
+}
+
+static uint64_t int_api_distance(const Pair* p, uint64_t c)
+{
+    if (p->metric == ORC_LCSSEQ) return lcs_cached_distance(p, c);
+    return int_distance_core(p, c);
+}
+
+static uint64_t int_api_similarity(const Pair* p, uint64_t c)
+{
+    if (p->metric == ORC_LCSSEQ) return lcs_cached_similarity(p, c);
+    /* CachedDistanceBase::_similarity (details/distance.hpp:306-317) */
+    uint64_t maximum = int_maximum(p);
+    if (c > maximum) return 0;
+    uint64_t cd = maximum - c;
+    uint64_t dist = int_distance_core(p, cd);
+    uint64_t sim = maximum - dist;
+    return (sim >= c) ? sim : 0;
+}
+
+/* CachedNormalizedMetricBase::_normalized_distance (details/distance.hpp:240-252). */
+static double int_api_norm_distance(const Pair* p, double c)
+{
+    uint64_t maximum = int_maximum(p);
+    uint64_t cd = (uint64_t)ceil((double)maximum * c);
+    double dist = (double)int_api_distance(p, cd);
+    double norm = (maximum != 0) ? dist / (double)maximum : 0.0;
+    return (norm <= c) ? norm : 1.0;
+}
+
+/* details/common.hpp:50-53 */
+static double normsim_to_normdist(double c) { return dmin(1.0, 1.0 - c + 0.00001); }
+
+/* CachedNormalizedMetricBase::_normalized_similarity (details/distance.hpp:254-262). */
+static double int_api_norm_similarity(const Pair* p, double c)
+{
+    double cs = normsim_to_normdist(c);
+    double nd = int_api_norm_distance(p, cs);
+    double ns = 1.0 - nd;
+    return (ns >= c) ? ns : 0.0;
+}
+
+/* Jaro / JaroWinkler: CachedSimilarityBase<double> with maximum 1.0. */
+static double fp_similarity(const Pair* p, double c)
+{
+    if (p->metric == ORC_JARO) return orc_jaro_similarity(p->s1, p->l1, p->s2, p->l2, c);
+    return orc_jaro_winkler_similarity(p->s1, p->l1, p->s2, p->l2, p->pw, c);
+}
+
+static double fp_distance(const Pair* p, double c)
+{
+    double maximum = 1.0;
+    double cs = (maximum > c) ? maximum - c : 0;
+    double sim = fp_similarity(p, cs);
+    double dist = maximum - sim;
+    return (dist <= c) ? dist : 1.0;
+}
+
+static double fp_norm_distance(const Pair* p, double c)
+{
+    double maximum = 1.0;
+    double cd = ceil(maximum * c);
+    double dist = fp_distance(p, cd);
+    double norm = (maximum != 0) ? dist / maximum : 0.0;
+    return (norm <= c) ? norm : 1.0;
+}
+
+static double fp_norm_similarity(const Pair* p, double c)
+{
+    double cs = normsim_to_normdist(c);
+    double nd = fp_norm_distance(p, cs);
+    double ns = 1.0 - nd;
+    return (ns >= c) ? ns : 0.0;
+}
+
+int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const uint32_t* s2,
+                     size_t l2, uint64_t ins, uint64_t del, uint64_t rep, double prefix_weight,
+                     uint64_t cutoff_u, double cutoff_d, uint64_t* out_u, double* out_d)
+{
+    Pair p = {metric, s1, s2, l1, l2, ins, del, rep, prefix_weight};
+    switch (metric) {
+    case ORC_LEVENSHTEIN:
+    case ORC_INDEL:
+    case ORC_LCSSEQ:
+    case ORC_OSA:
+        if (metric != ORC_LEVENSHTEIN) p.ins = p.del = 1, p.rep = (metric == ORC_INDEL) ? 2 : 1;
+        switch (kind) {
+        case ORC_DISTANCE: *out_u = int_api_distance(&p, cutoff_u); return 0;
+        case ORC_SIMILARITY: *out_u = int_api_similarity(&p, cutoff_u); return 0;
+        case ORC_NORM_DISTANCE: *out_d = int_api_norm_distance(&p, cutoff_d); return 1;
+        case ORC_NORM_SIMILARITY: *out_d = int_api_norm_similarity(&p, cutoff_d); return 1;
+        }
+        return -1;
+    case ORC_JARO:
+    case ORC_JARO_WINKLER:
+        switch (kind) {
+        case ORC_DISTANCE: *out_d = fp_distance(&p, cutoff_d); return 1;
+        case ORC_SIMILARITY: *out_d = fp_similarity(&p, cutoff_d); return 1;
+        case ORC_NORM_DISTANCE: *out_d = fp_norm_distance(&p, cutoff_d); return 1;
+        case ORC_NORM_SIMILARITY: *out_d = fp_norm_similarity(&p, cutoff_d); return 1;
+        }
+        return -1;
+    case ORC_RATIO:
+        /* fuzz::CachedRatio::similarity (fuzz_impl.hpp:42-47) */
+        if (kind != ORC_SIMILARITY) return -1;
+        p.metric = ORC_INDEL;
+        p.ins = p.del = 1;
+        p.rep = 2;
+        *out_d = int_api_norm_similarity(&p, cutoff_d / 100) * 100;
+        return 1;
+    }
+    return -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* experimental::Multi* API layer (details/distance.hpp:390-582)            */
+/* ------------------------------------------------------------------------ */
+
+/* Full (uncut) raw distance of the Multi kernels: Levenshtein/OSA SIMD kernels
+ * report s2.size() for an empty s1 (Levenshtein_impl.hpp:393-394). */
+static uint64_t multi_raw_distance(const Pair* p)
+{
+    switch (p->metric) {
+    case ORC_LEVENSHTEIN:
+        if (p->l1 == 0) return p->l2;
+        return orc_levenshtein_raw(p->s1, p->l1, p->s2, p->l2, 1, 1, 1);
+    case ORC_OSA:
+        if (p->l1 == 0) return p->l2;
+        return orc_osa_raw(p->s1, p->l1, p->s2, p->l2);
+    case ORC_INDEL:
+        /* MultiIndel::_distance (Indel.hpp:118-130) */
+        return (uint64_t)p->l1 + p->l2 - 2 * orc_lcs_raw(p->s1, p->l1, p->s2, p->l2);
+    case ORC_LCSSEQ:
+        /* MultiSimilarityBase::_distance with the full similarity */
+        return u64max(p->l1, p->l2) - orc_lcs_raw(p->s1, p->l1, p->s2, p->l2);
+    }
+    return 0;
+}
+
+static double multi_fp_similarity(const Pair* p, double c)
+{
+    if (p->metric == ORC_JARO) return orc_jaro_similarity(p->s1, p->l1, p->s2, p->l2, c);
+    /* MultiJaroWinkler::_similarity (JaroWinkler.hpp:122-145) */
+    double s = orc_jaro_similarity(p->s1, p->l1, p->s2, p->l2, dmin(0.7, c));
+    if (s > 0.7) {
+        size_t prefix = common_prefix4(p->s1, p->l1, p->s2, p->l2);
+        s += (double)prefix * p->pw * (1.0 - s);
+        s = dmin(s, 1.0);
+    }
+    if (s < c) s = 0.0;
+    return s;
+}
+
+int orc_multi_score(int metric, int kind, const uint32_t* s1, size_t l1, const uint32_t* s2,
+                    size_t l2, double prefix_weight, uint64_t cutoff_u, double cutoff_d,
+                    uint64_t* out_u, double* out_d)
+{
+    Pair p = {metric, s1, s2, l1, l2, 1, 1, 1, prefix_weight};
+    if (metric == ORC_LEVENSHTEIN || metric == ORC_OSA || metric == ORC_INDEL ||
+        metric == ORC_LCSSEQ) {
+        uint64_t maximum = (metric == ORC_INDEL) ? (uint64_t)l1 + l2 : u64max(l1, l2);
+        uint64_t d = multi_raw_distance(&p);
+        if (metric == ORC_LCSSEQ && kind == ORC_SIMILARITY) {
+            uint64_t lcs = maximum - d;
+            *out_u = lcs >= cutoff_u ? lcs : 0;
+            return 0;
+        }
+        switch (kind) {
+        case ORC_DISTANCE: *out_u = d <= cutoff_u ? d : cutoff_u + 1; return 0;
+        case ORC_SIMILARITY: {
+            uint64_t sim = maximum - d;
+            *out_u = sim >= cutoff_u ? sim : 0;
+            return 0;
+        }
+        case ORC_NORM_DISTANCE: {
+            double norm = maximum ? (double)d / (double)maximum : 0.0;
+            *out_d = norm <= cutoff_d ? norm : 1.0;
+            return 1;
+        }
+        case ORC_NORM_SIMILARITY: {
+            double norm = maximum ? (double)d / (double)maximum : 0.0;
+            norm = norm <= 1.0 ? norm : 1.0;
+            double ns = 1.0 - norm;
+            *out_d = ns >= cutoff_d ? ns : 0.0;
+            return 1;
+        }
+        }
+        return -1;
+    }
+    if (metric == ORC_JARO || metric == ORC_JARO_WINKLER) {
+        switch (kind) {
+        case ORC_SIMILARITY: *out_d = multi_fp_similarity(&p, cutoff_d); return 1;
+        case ORC_DISTANCE: {
+            double dist = 1.0 - multi_fp_similarity(&p, 0.0);
+            *out_d = dist <= cutoff_d ? dist : 1.0;
+            return 1;
+        }
+        case ORC_NORM_DISTANCE: {
+            double dist = 1.0 - multi_fp_similarity(&p, 0.0);
+            dist = dist <= 1.0 ? dist : 1.0;
+            *out_d = dist <= cutoff_d ? dist : 1.0;
+            return 1;
+        }
+        case ORC_NORM_SIMILARITY: {
+            double dist = 1.0 - multi_fp_similarity(&p, 0.0);
+            dist = dist <= 1.0 ? dist : 1.0;
+            double ns = 1.0 - dist;
+            *out_d = ns >= cutoff_d ? ns : 0.0;
+            return 1;
+        }
+        }
+        return -1;
+    }
+    if (metric == ORC_RATIO) {
+        /* MultiRatio::similarity (fuzz.hpp:82-89) over MultiIndel */
+        if (kind != ORC_SIMILARITY) return -1;
+        p.metric = ORC_INDEL;
+        uint64_t maximum = (uint64_t)l1 + l2;
+        uint64_t d = multi_raw_distance(&p);
+        double norm = maximum ? (double)d / (double)maximum : 0.0;
+        norm = norm <= 1.0 ? norm : 1.0;
+        double ns = 1.0 - norm;
+        double c = cutoff_d / 100.0;
+        *out_d = (ns >= c ? ns : 0.0) * 100.0;
+        return 1;
+    }
+    return -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* batch helpers                                                            */
+/* ------------------------------------------------------------------------ */
+
+int orc_cdist(int metric, int kind, int multi, const uint32_t* qchars, const uint64_t* qoff,
+              size_t nq, const uint32_t* cchars, const uint64_t* coff, size_t nc, uint64_t ins,
+              uint64_t del, uint64_t rep, double prefix_weight, uint64_t cutoff_u, double cutoff_d,
+              uint64_t* out_u, double* out_d)
+{
+    int r = -1;
+    for (size_t q = 0; q < nq; ++q) {
+        const uint32_t* s1 = qchars + qoff[q];
+        size_t l1 = (size_t)(qoff[q + 1] - qoff[q]);
+        for (size_t c = 0; c < nc; ++c) {
+            const uint32_t* s2 = cchars + coff[c];
+            size_t l2 = (size_t)(coff[c + 1] - coff[c]);
+            uint64_t u = 0;
+            double d = 0;
+            if (multi)
+                r = orc_multi_score(metric, kind, s1, l1, s2, l2, prefix_weight, cutoff_u, cutoff_d, &u,
+                                    &d);
+            else
+                r = orc_cached_score(metric, kind, s1, l1, s2, l2, ins, del, rep, prefix_weight,
+                                     cutoff_u, cutoff_d, &u, &d);
+            if (r < 0) return r;
+            if (r == 0)
+                out_u[q * nc + c] = u;
+            else
+                out_d[q * nc + c] = d;
+        }
+    }
+    return r;
+}
+
+int orc_pairs(const int32_t* metrics, int kind, const uint32_t* achars, const uint64_t* aoff,
+              const uint32_t* bchars, const uint64_t* boff, size_t n, uint64_t cutoff_u,
+              double cutoff_d, uint64_t* out_u, double* out_d)
+{
+    int r = -1;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t u = 0;
+        double d = 0;
+        r = orc_cached_score(metrics[i], kind, achars + aoff[i], (size_t)(aoff[i + 1] - aoff[i]),
+                             bchars + boff[i], (size_t)(boff[i + 1] - boff[i]), 1, 1, 1, 0.1, cutoff_u,
+                             cutoff_d, &u, &d);
+        if (r < 0) return r;
+        if (out_u) out_u[i] = u;
+        if (out_d) out_d[i] = d;
+    }
+    return r;
+}

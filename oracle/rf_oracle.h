@@ -1,0 +1,80 @@
+/*
+ * rf_oracle.h -- CPU restatement of rapidfuzz-cpp's scoring semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links or calls
+ * this code: it exists so that tests/, the smoke check and the CPU-baseline
+ * leg of bench.py have an independent checker for the HIP kernels.
+ *
+ * Two layers:
+ *   1. raw values by naive O(m*n) dynamic programming, restated from the
+ *      reference's own naive oracle (the rapidfuzz_reference directory);
+ *   2. the API layer (distance / similarity / normalized_* with score_cutoff)
+ *      restated operation by operation from rapidfuzz/details/distance.hpp,
+ *      rapidfuzz/details/common.hpp:50-53 and the per-metric Cached* classes.
+ *
+ * All strings are passed as uint32_t code units.  Build with
+ * -ffp-contract=off so that double formulas round exactly like the
+ * reference built with the same flag.
+ */
+#ifndef RF_ORACLE_H
+#define RF_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_LEVENSHTEIN = 0,
+    ORC_INDEL = 1,
+    ORC_LCSSEQ = 2,
+    ORC_OSA = 3,
+    ORC_JARO = 4,
+    ORC_JARO_WINKLER = 5,
+    ORC_RATIO = 6
+};
+
+enum { ORC_DISTANCE = 0, ORC_SIMILARITY = 1, ORC_NORM_DISTANCE = 2, ORC_NORM_SIMILARITY = 3 };
+
+/* ---- raw values (naive DP) ---- */
+uint64_t orc_levenshtein_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2,
+                             uint64_t ins, uint64_t del, uint64_t rep);
+uint64_t orc_lcs_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
+uint64_t orc_osa_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
+void orc_jaro_raw(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, uint64_t* common,
+                  uint64_t* transpositions);
+double orc_jaro_similarity(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, double cutoff);
+double orc_jaro_winkler_similarity(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl,
+                                   double prefix_weight, double cutoff);
+
+/* ---- API layer: Cached<Metric>(s1).<kind>(s2, cutoff) ----
+ * Returns 0 and writes *out_u for integer results, 1 and writes *out_d for
+ * double results, -1 for an unsupported (metric, kind). */
+int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const uint32_t* s2,
+                     size_t l2, uint64_t ins, uint64_t del, uint64_t rep, double prefix_weight,
+                     uint64_t cutoff_u, double cutoff_d, uint64_t* out_u, double* out_d);
+
+/* ---- API layer: experimental::Multi<Metric><MaxLen> for one inserted s1 ----
+ * (the Multi* classes score unbounded first and apply the cutoff after). */
+int orc_multi_score(int metric, int kind, const uint32_t* s1, size_t l1, const uint32_t* s2,
+                    size_t l2, double prefix_weight, uint64_t cutoff_u, double cutoff_d,
+                    uint64_t* out_u, double* out_d);
+
+/* ---- batch helpers (single thread), out[q * nc + c] ----
+ * chars: packed code units; offsets: count+1 entries. */
+int orc_cdist(int metric, int kind, int multi, const uint32_t* qchars, const uint64_t* qoff,
+              size_t nq, const uint32_t* cchars, const uint64_t* coff, size_t nc, uint64_t ins,
+              uint64_t del, uint64_t rep, double prefix_weight, uint64_t cutoff_u, double cutoff_d,
+              uint64_t* out_u, double* out_d);
+
+/* pairs: out[i] = score(a[i], b[i]) with metric[i] */
+int orc_pairs(const int32_t* metrics, int kind, const uint32_t* achars, const uint64_t* aoff,
+              const uint32_t* bchars, const uint64_t* boff, size_t n, uint64_t cutoff_u,
+              double cutoff_d, uint64_t* out_u, double* out_d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
