@@ -85,7 +85,9 @@ class Oracle:
             getattr(L, name).argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t]
 
     def cdist(self, metric, kind, queries, choices, weights=(1, 1, 1), prefix_weight=0.1,
-              cutoff_u=INT64_MAX, cutoff_d=None, multi=False):
+              cutoff_u=None, cutoff_d=None, multi=False):
+        if cutoff_u is None:
+            cutoff_u = default_int_cutoff(kind)
         qc, qo = pack(queries)
         cc, co = pack(choices)
         nq, nc = len(qo) - 1, len(co) - 1
@@ -125,6 +127,11 @@ class Oracle:
         self.lib.orc_jaro_raw(_p(pa, _u32p), len(to_codes(p)), _p(ta, _u32p), len(to_codes(t)),
                               _p(cc, _u64p), _p(tr, _u64p))
         return int(cc[0]), int(tr[0])
+
+
+def default_int_cutoff(kind):
+    """WorstDistance (INT64_MAX) for distance(), WorstSimilarity (0) for similarity()."""
+    return INT64_MAX if kind == DISTANCE else 0
 
 
 def default_fp_cutoff(metric, kind):
@@ -168,7 +175,9 @@ class Reference:
         return self.lib.ref_last_error().decode()
 
     def cdist(self, metric, kind, queries, choices, weights=(1, 1, 1), prefix_weight=0.1,
-              cutoff_u=INT64_MAX, cutoff_d=None, char_width=4, multi_len=None):
+              cutoff_u=None, cutoff_d=None, char_width=4, multi_len=None):
+        if cutoff_u is None:
+            cutoff_u = default_int_cutoff(kind)
         qc, qo = pack(queries)
         cc, co = pack(choices)
         nq, nc = len(qo) - 1, len(co) - 1
@@ -189,9 +198,11 @@ class Reference:
             raise ValueError(f"reference error {r}: {self._err()}")
         return (ou if r == 0 else od).reshape(nq, nc)
 
-    def cdist_u8_mt(self, metric, kind, qchars, qoff, cchars, coff, cutoff_u=INT64_MAX, cutoff_d=None,
+    def cdist_u8_mt(self, metric, kind, qchars, qoff, cchars, coff, cutoff_u=None, cutoff_d=None,
                     prefix_weight=0.1, threads=None, multi=False):
         """Threaded u8 cdist (the CPU baseline); inputs already packed as uint8."""
+        if cutoff_u is None:
+            cutoff_u = default_int_cutoff(kind)
         nq, nc = len(qoff) - 1, len(coff) - 1
         if cutoff_d is None:
             cutoff_d = default_fp_cutoff(metric, kind)

@@ -1,0 +1,639 @@
+// rfa_api.cpp -- the C ABI: contexts, resident string sets, scorers, and the
+// host-side planning that splits a request into kernel launches.
+//
+// Planning (done once per scorer, like the reference's PM construction in
+// the Cached* constructors):
+//   * queries of <= 64 code units are sorted by length (longest first) and cut
+//     into tiles of 64; a tile whose queries all fit in 32 units runs with
+//     32-bit words.  Each tile's distinct-symbol count sizes its LDS PM table.
+//   * longer queries (and tiles whose alphabet does not fit in LDS) are scored
+//     by the block kernel, one wave per pair.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "rfa_internal.hpp"
+
+using namespace rfa;
+
+namespace {
+
+thread_local std::string g_err;
+
+rfa_status fail(rfa_status st, const std::string& msg)
+{
+    g_err = msg;
+    return st;
+}
+
+struct HipError : std::runtime_error {
+    hipError_t e;
+    HipError(hipError_t e_, const char* what) : std::runtime_error(std::string(what) + ": " + hipGetErrorString(e_)), e(e_) {}
+};
+
+#define HIP_CHECK(x)                                                                                               \
+    do {                                                                                                           \
+        hipError_t e_ = (x);                                                                                       \
+        if (e_ != hipSuccess) throw HipError(e_, #x);                                                              \
+    } while (0)
+
+struct ApiError : std::runtime_error {
+    rfa_status st;
+    ApiError(rfa_status s, const std::string& m) : std::runtime_error(m), st(s) {}
+};
+
+// Device buffer that grows on demand (kept by the context between calls).
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes)
+    {
+        if (bytes > cap) {
+            if (p) HIP_CHECK(hipFree(p));
+            p = nullptr;
+            cap = 0;
+            size_t want = std::max<size_t>(bytes, 256);
+            if (hipMalloc(&p, want) != hipSuccess) {
+                (void)hipGetLastError();
+                throw ApiError(RFA_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(want) + " bytes failed");
+            }
+            cap = want;
+        }
+        return p;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+template <typename T>
+T* upload(DevBuf& b, const std::vector<T>& v, hipStream_t s)
+{
+    T* d = (T*)b.get(std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return d;
+}
+
+uint64_t ceil_div_u64(uint64_t a, uint64_t b) { return a / b + (a % b != 0); }
+
+uint32_t log2_ceil(uint64_t x)
+{
+    uint32_t b = 0;
+    while ((1ull << b) < x) ++b;
+    return b;
+}
+
+} // namespace
+
+struct rfa_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    int num_cus = 256;
+    size_t lds_per_cu = 160 * 1024;
+    DevBuf queue, pm_scratch, tflag, items_a, items_b, items_idx, items_swap, out_tmp, misc;
+};
+
+struct rfa_strings {
+    rfa_context* ctx = nullptr;
+    int width = 1;
+    uint64_t count = 0;
+    void* d_chars = nullptr;
+    uint64_t* d_off = nullptr;
+    std::vector<uint8_t> h_chars;
+    std::vector<uint64_t> h_off;
+    uint64_t max_len = 0;
+
+    uint64_t len(uint64_t i) const { return h_off[i + 1] - h_off[i]; }
+    uint32_t unit(uint64_t pos) const
+    {
+        if (width == 1) return h_chars[pos];
+        if (width == 2) return ((const uint16_t*)h_chars.data())[pos];
+        return ((const uint32_t*)h_chars.data())[pos];
+    }
+};
+
+struct TileClass {
+    uint32_t first_tile = 0, ntiles = 0;
+    int word_bits = 64;
+    uint32_t rows_cap = 2;
+};
+
+struct rfa_scorer {
+    rfa_context* ctx = nullptr;
+    int metric = 0;
+    rfa_weights w{1, 1, 1};
+    double prefix_weight = 0.1;
+    rfa_strings* q = nullptr; // owned copy of the queries
+    // tile plan
+    std::vector<uint32_t> tile_order; // query indices, 64 per tile
+    uint32_t* d_tile_order = nullptr;
+    std::vector<TileClass> classes;
+    uint32_t hash_bits = 6;
+    std::vector<uint32_t> block_queries; // scored by the block kernel
+    uint32_t* d_block_queries = nullptr;
+    uint64_t block_max_len = 0;
+};
+
+namespace {
+
+int algo_for(int metric, const rfa_weights& w)
+{
+    switch (metric) {
+    case RFA_LEVENSHTEIN:
+        if (w.insert_cost == w.delete_cost) {
+            if (w.insert_cost == 0 || w.insert_cost == w.replace_cost) return ALGO_LEV;
+            if (w.replace_cost >= w.insert_cost + w.delete_cost) return ALGO_LCS;
+        }
+        return -1;
+    case RFA_INDEL:
+    case RFA_LCSSEQ:
+    case RFA_RATIO: return ALGO_LCS;
+    case RFA_OSA: return ALGO_OSA;
+    case RFA_JARO:
+    case RFA_JARO_WINKLER: return ALGO_JARO;
+    }
+    return -1;
+}
+
+bool is_int_kind(int metric, int kind)
+{
+    return (metric == RFA_LEVENSHTEIN || metric == RFA_INDEL || metric == RFA_LCSSEQ || metric == RFA_OSA) &&
+           (kind == RFA_DISTANCE || kind == RFA_SIMILARITY);
+}
+
+// The distinct-symbol count above which a query tile leaves the LDS path.
+constexpr uint32_t TILE_MAX_ROWS = 255;
+
+void plan_scorer(rfa_scorer* sc)
+{
+    rfa_strings* q = sc->q;
+    std::vector<uint32_t> shortq;
+    for (uint64_t i = 0; i < q->count; ++i) {
+        if (q->len(i) <= 64)
+            shortq.push_back((uint32_t)i);
+        else
+            sc->block_queries.push_back((uint32_t)i);
+    }
+    std::stable_sort(shortq.begin(), shortq.end(), [&](uint32_t a, uint32_t b) { return q->len(a) > q->len(b); });
+    // tiles; a tile whose alphabet is too large for LDS moves to the block path
+    std::vector<uint32_t> kept;
+    std::vector<uint32_t> rows_of_tile;
+    for (size_t t0 = 0; t0 < shortq.size(); t0 += 64) {
+        size_t t1 = std::min(shortq.size(), t0 + 64);
+        std::unordered_set<uint32_t> syms;
+        for (size_t k = t0; k < t1; ++k) {
+            uint32_t qi = shortq[k];
+            for (uint64_t p = q->h_off[qi]; p < q->h_off[qi + 1]; ++p) syms.insert(q->unit(p));
+        }
+        if (syms.size() + 1 > TILE_MAX_ROWS) {
+            for (size_t k = t0; k < t1; ++k) sc->block_queries.push_back(shortq[k]);
+            continue;
+        }
+        for (size_t k = t0; k < t1; ++k) kept.push_back(shortq[k]);
+        rows_of_tile.push_back((uint32_t)syms.size() + 1);
+    }
+    // kept stays sorted by length (whole tiles were removed), so tiles are re-cut identically
+    // only if no tile was removed before a kept one; re-cut to keep tiles full.
+    std::vector<uint32_t> order = kept;
+    std::vector<uint32_t> rows;
+    for (size_t t0 = 0; t0 < order.size(); t0 += 64) {
+        size_t t1 = std::min(order.size(), t0 + 64);
+        std::unordered_set<uint32_t> syms;
+        for (size_t k = t0; k < t1; ++k) {
+            uint32_t qi = order[k];
+            for (uint64_t p = q->h_off[qi]; p < q->h_off[qi + 1]; ++p) syms.insert(q->unit(p));
+        }
+        rows.push_back((uint32_t)syms.size() + 1);
+    }
+    sc->tile_order = order;
+    uint32_t ntiles = (uint32_t)rows.size();
+    uint32_t split = ntiles; // first tile with all queries <= 32 units
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        if (q->len(order[(size_t)t * 64]) <= 32) {
+            split = t;
+            break;
+        }
+    }
+    uint32_t max_rows_all = 2;
+    for (int c = 0; c < 2; ++c) {
+        TileClass tc;
+        tc.first_tile = c == 0 ? 0 : split;
+        tc.ntiles = c == 0 ? split : ntiles - split;
+        tc.word_bits = c == 0 ? 64 : 32;
+        uint32_t mr = 2;
+        for (uint32_t t = tc.first_tile; t < tc.first_tile + tc.ntiles; ++t) mr = std::max(mr, rows[t]);
+        tc.rows_cap = mr;
+        max_rows_all = std::max(max_rows_all, mr);
+        if (tc.ntiles) sc->classes.push_back(tc);
+    }
+    sc->hash_bits = std::max<uint32_t>(6, log2_ceil(2ull * max_rows_all));
+    for (uint32_t qi : sc->block_queries) sc->block_max_len = std::max(sc->block_max_len, q->len(qi));
+    // longest first: the block queue is then longest-processing-time-first
+    std::stable_sort(sc->block_queries.begin(), sc->block_queries.end(),
+                     [&](uint32_t a, uint32_t b) { return q->len(a) > q->len(b); });
+    if (!order.empty()) {
+        HIP_CHECK(hipMalloc(&sc->d_tile_order, order.size() * sizeof(uint32_t)));
+        HIP_CHECK(hipMemcpy(sc->d_tile_order, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    if (!sc->block_queries.empty()) {
+        HIP_CHECK(hipMalloc(&sc->d_block_queries, sc->block_queries.size() * sizeof(uint32_t)));
+        HIP_CHECK(hipMemcpy(sc->d_block_queries, sc->block_queries.data(), sc->block_queries.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice));
+    }
+}
+
+rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const uint64_t* off, uint64_t count)
+{
+    if (width != 1 && width != 2 && width != 4) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "char_width must be 1, 2 or 4");
+    if (!off && count) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "offsets is NULL");
+    auto* s = new rfa_strings();
+    s->ctx = ctx;
+    s->width = width;
+    s->count = count;
+    s->h_off.assign(off, off + count + 1);
+    if (count == 0) s->h_off.assign(1, 0);
+    for (uint64_t i = 0; i < count; ++i) {
+        if (s->h_off[i + 1] < s->h_off[i]) {
+            delete s;
+            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "offsets must be non-decreasing");
+        }
+        s->max_len = std::max(s->max_len, s->h_off[i + 1] - s->h_off[i]);
+    }
+    uint64_t nunits = s->h_off.back();
+    if (nunits && !chars) {
+        delete s;
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "chars is NULL");
+    }
+    s->h_chars.resize(nunits * width);
+    if (nunits) std::memcpy(s->h_chars.data(), chars, nunits * width);
+    size_t cb = std::max<size_t>(nunits * width, 64);
+    if (hipMalloc(&s->d_chars, cb) != hipSuccess || hipMalloc((void**)&s->d_off, s->h_off.size() * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        if (s->d_chars) (void)hipFree(s->d_chars);
+        delete s;
+        throw ApiError(RFA_ERR_OUT_OF_MEMORY, "device allocation for strings failed");
+    }
+    if (nunits) HIP_CHECK(hipMemcpy(s->d_chars, s->h_chars.data(), nunits * width, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->d_off, s->h_off.data(), s->h_off.size() * 8, hipMemcpyHostToDevice));
+    return s;
+}
+
+void free_strings(rfa_strings* s)
+{
+    if (!s) return;
+    if (s->d_chars) (void)hipFree(s->d_chars);
+    if (s->d_off) (void)hipFree(s->d_off);
+    delete s;
+}
+
+ScoreSpec make_spec(const rfa_scorer* sc, int kind, int semantics, uint64_t cu, double cd, int algo)
+{
+    ScoreSpec s{};
+    s.metric = sc->metric;
+    s.kind = kind;
+    s.semantics = semantics;
+    s.algo = algo;
+    s.cutoff_u = cu;
+    s.cutoff_d = cd;
+    s.ins = sc->w.insert_cost;
+    s.del = sc->w.delete_cost;
+    s.rep = sc->w.replace_cost;
+    s.prefix_weight = sc->prefix_weight;
+    s.band_k = ~0ull;
+    if (kind == RFA_DISTANCE && is_int_kind(sc->metric, kind) && sc->metric != RFA_LCSSEQ) {
+        if (sc->metric == RFA_LEVENSHTEIN)
+            s.band_k = s.ins ? ceil_div_u64(cu, s.ins) : ~0ull;
+        else
+            s.band_k = cu;
+    }
+    return s;
+}
+
+void validate_kind(int metric, int kind, int semantics)
+{
+    if (kind < RFA_DISTANCE || kind > RFA_NORMALIZED_SIMILARITY) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown kind");
+    if (semantics != RFA_CACHED && semantics != RFA_MULTI) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown semantics");
+    if (metric == RFA_RATIO && kind != RFA_SIMILARITY)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "ratio only provides similarity");
+}
+
+// Block-kernel launch over a cross product (queries x choices) or explicit items.
+void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_pattern, uint64_t max_text,
+               bool reuse)
+{
+    if (a.nitems == 0) return;
+    hipStream_t s = ctx->stream;
+    const uint32_t W = (uint32_t)((max_pattern + 63) / 64);
+    const uint32_t slots = std::max<uint32_t>(1, (W + 63) / 64);
+    if (slots > 16) throw ApiError(RFA_ERR_UNSUPPORTED, "patterns longer than 65536 code units are not supported");
+    uint64_t alpha = width == 1 ? 256 : (width == 2 ? 65536 : ~0ull);
+    uint64_t max_rows = std::min<uint64_t>(max_pattern, alpha) + 1;
+    uint32_t hash_bits = std::max<uint32_t>(6, log2_ceil(2 * max_rows));
+    size_t arena = block_arena_bytes(width, (uint32_t)max_pattern, (uint32_t)max_rows, hash_bits);
+    a.hash_bits = hash_bits;
+    a.reuse_pattern = reuse ? 1u : 0u;
+    bool lds = arena <= 64 * 1024;
+    uint32_t waves_per_cu = lds ? (uint32_t)std::min<size_t>(16, std::max<size_t>(1, ctx->lds_per_cu / (arena + 512)))
+                                : 8u;
+    uint32_t nblocks = (uint32_t)std::min<uint64_t>(a.nitems, (uint64_t)ctx->num_cus * waves_per_cu);
+    if (!lds) {
+        a.pm_scratch_words = (arena + 7) / 8;
+        a.pm_scratch = (uint64_t*)ctx->pm_scratch.get(a.pm_scratch_words * 8 * nblocks);
+    }
+    if (algo == ALGO_JARO) {
+        a.tflag_words = (max_text + 63) / 64 + 1;
+        a.tflag_scratch = (uint64_t*)ctx->tflag.get(a.tflag_words * 8 * nblocks);
+    }
+    a.queue = (uint32_t*)ctx->queue.get(64);
+    HIP_CHECK(hipMemsetAsync(a.queue, 0, 4, s));
+    HIP_CHECK(launch_block(algo, width, a, nblocks, lds, lds ? arena : 0, slots, s));
+}
+
+void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
+                int out_on_device)
+{
+    rfa_context* ctx = sc->ctx;
+    validate_kind(sc->metric, kind, semantics);
+    if (ch->width != sc->q->width)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
+    const int algo = algo_for(sc->metric, sc->w);
+    hipStream_t s = ctx->stream;
+    const uint64_t nq = sc->q->count, nc = ch->count;
+    const size_t bytes = (size_t)nq * nc * 8;
+    if (nq == 0 || nc == 0) return;
+    if (nc > 0xffffffffull || nq > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 strings");
+    void* dout = out_on_device ? out : ctx->out_tmp.get(bytes);
+    const ScoreSpec spec = make_spec(sc, kind, semantics, cu, cd, algo);
+    const int width = sc->q->width;
+
+    // choices routed to the block kernel for Jaro (texts longer than one word)
+    std::vector<uint32_t> jaro_long;
+    std::vector<uint32_t> tile_choices;
+    bool all_choices = true;
+    if (algo == ALGO_JARO) {
+        for (uint64_t c = 0; c < nc; ++c) {
+            if (ch->len(c) > 64)
+                jaro_long.push_back((uint32_t)c);
+            else
+                tile_choices.push_back((uint32_t)c);
+        }
+        all_choices = jaro_long.empty();
+    }
+    // ---- tile kernel ----
+    if (!sc->classes.empty()) {
+        TileArgs ta{};
+        ta.qchars = sc->q->d_chars;
+        ta.qoff = sc->q->d_off;
+        ta.qorder = sc->d_tile_order;
+        ta.nq = (uint32_t)sc->tile_order.size();
+        ta.cchars = ch->d_chars;
+        ta.coff = ch->d_off;
+        ta.corder = all_choices ? nullptr : upload(ctx->misc, tile_choices, s);
+        ta.ncsel = all_choices ? (uint32_t)nc : (uint32_t)tile_choices.size();
+        ta.out_stride = nc;
+        ta.hash_bits = sc->hash_bits;
+        ta.spec = spec;
+        ta.out = dout;
+        if (ta.ncsel) {
+            for (const TileClass& tc : sc->classes) {
+                ta.tile_base = tc.first_tile;
+                ta.rows_cap = tc.rows_cap;
+                // enough workgroups to fill the chip several times over
+                uint32_t target_blocks = (uint32_t)ctx->num_cus * 12;
+                uint32_t nbx = std::max<uint32_t>(1, (target_blocks + tc.ntiles - 1) / tc.ntiles);
+                uint32_t cpb = std::max<uint32_t>(64, (ta.ncsel + nbx - 1) / nbx);
+                cpb = ((cpb + 3) / 4) * 4;
+                ta.choices_per_block = cpb;
+                nbx = (ta.ncsel + cpb - 1) / cpb;
+                HIP_CHECK(launch_tile(algo, width, tc.word_bits, ta, tc.ntiles, nbx, s));
+            }
+        }
+    }
+    // ---- block kernel: long (or large-alphabet) queries x all choices ----
+    if (!sc->block_queries.empty()) {
+        BlockArgs ba{};
+        ba.achars = sc->q->d_chars;
+        ba.aoff = sc->q->d_off;
+        ba.bchars = ch->d_chars;
+        ba.boff = ch->d_off;
+        ba.item_a = sc->d_block_queries;
+        std::vector<uint32_t> all(nc);
+        std::iota(all.begin(), all.end(), 0u);
+        ba.item_b = upload(ctx->items_b, all, s);
+        ba.cross_nb = (uint32_t)nc;
+        ba.out_stride = nc;
+        uint64_t ni = (uint64_t)sc->block_queries.size() * nc;
+        if (ni > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "too many long-query pairs in one call");
+        ba.nitems = (uint32_t)ni;
+        ba.spec = spec;
+        ba.out = dout;
+        run_block(ctx, algo, width, ba, sc->block_max_len, ch->max_len, true);
+    }
+    // ---- Jaro: short queries x long choices ----
+    if (!jaro_long.empty() && !sc->tile_order.empty()) {
+        BlockArgs ba{};
+        ba.achars = sc->q->d_chars;
+        ba.aoff = sc->q->d_off;
+        ba.bchars = ch->d_chars;
+        ba.boff = ch->d_off;
+        ba.item_a = sc->d_tile_order;
+        ba.item_b = upload(ctx->items_a, jaro_long, s);
+        ba.cross_nb = (uint32_t)jaro_long.size();
+        ba.out_stride = nc;
+        uint64_t ni = (uint64_t)sc->tile_order.size() * jaro_long.size();
+        if (ni > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "too many long-choice pairs in one call");
+        ba.nitems = (uint32_t)ni;
+        ba.spec = spec;
+        ba.out = dout;
+        run_block(ctx, algo, width, ba, 64, ch->max_len, true);
+    }
+    if (!out_on_device) {
+        HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+template <typename F>
+rfa_status guarded(F&& f)
+{
+    try {
+        f();
+        return RFA_OK;
+    }
+    catch (const ApiError& e) {
+        return fail(e.st, e.what());
+    }
+    catch (const HipError& e) {
+        return fail(RFA_ERR_HIP, e.what());
+    }
+    catch (const std::bad_alloc&) {
+        return fail(RFA_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    catch (const std::exception& e) {
+        return fail(RFA_ERR_INVALID_ARGUMENT, e.what());
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int rfa_abi_version(void) { return RFA_ABI_VERSION; }
+
+const char* rfa_last_error(void) { return g_err.c_str(); }
+
+rfa_status rfa_context_create(int device, rfa_context** out)
+{
+    if (!out) return fail(RFA_ERR_INVALID_ARGUMENT, "out is NULL");
+    return guarded([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            (void)hipGetLastError();
+            throw ApiError(RFA_ERR_NO_DEVICE, "no HIP device available");
+        }
+        if (device < 0 || device >= n) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "device index out of range");
+        HIP_CHECK(hipSetDevice(device));
+        auto* c = new rfa_context();
+        c->device = device;
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        if (prop.maxSharedMemoryPerMultiProcessor > 0) c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+        HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        *out = c;
+    });
+}
+
+rfa_status rfa_context_destroy(rfa_context* ctx)
+{
+    if (!ctx) return RFA_OK;
+    return guarded([&] {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (DevBuf* b : {&ctx->queue, &ctx->pm_scratch, &ctx->tflag, &ctx->items_a, &ctx->items_b, &ctx->items_idx,
+                          &ctx->items_swap, &ctx->out_tmp, &ctx->misc})
+            b->release();
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+rfa_status rfa_context_synchronize(rfa_context* ctx)
+{
+    if (!ctx) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    return guarded([&] { HIP_CHECK(hipStreamSynchronize(ctx->stream)); });
+}
+
+void* rfa_context_stream(rfa_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+rfa_status rfa_strings_create(rfa_context* ctx, const void* chars, int char_width, const uint64_t* offsets,
+                              uint64_t count, rfa_strings** out)
+{
+    if (!ctx || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx or out is NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_CHECK(hipSetDevice(ctx->device));
+        *out = make_strings(ctx, chars, char_width, offsets, count);
+    });
+}
+
+rfa_status rfa_strings_destroy(rfa_strings* set)
+{
+    if (!set) return RFA_OK;
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(set->ctx->mu);
+        (void)hipSetDevice(set->ctx->device);
+        (void)hipStreamSynchronize(set->ctx->stream);
+        free_strings(set);
+    });
+}
+
+uint64_t rfa_strings_count(const rfa_strings* set) { return set ? set->count : 0; }
+
+rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* queries, const rfa_weights* weights,
+                             double prefix_weight, rfa_scorer** out)
+{
+    if (!ctx || !queries || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, queries or out is NULL");
+    return guarded([&] {
+        if (metric < RFA_LEVENSHTEIN || metric > RFA_RATIO) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
+        rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
+        if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};
+        if (algo_for(metric, w) < 0)
+            throw ApiError(RFA_ERR_UNSUPPORTED,
+                           "Levenshtein weights other than {w,w,w} or Indel-like {w,w,>=2w} are not accelerated");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_CHECK(hipSetDevice(ctx->device));
+        auto* sc = new rfa_scorer();
+        sc->ctx = ctx;
+        sc->metric = metric;
+        sc->w = w;
+        sc->prefix_weight = prefix_weight;
+        try {
+            sc->q = make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
+            plan_scorer(sc);
+        }
+        catch (...) {
+            free_strings(sc->q);
+            if (sc->d_tile_order) (void)hipFree(sc->d_tile_order);
+            if (sc->d_block_queries) (void)hipFree(sc->d_block_queries);
+            delete sc;
+            throw;
+        }
+        *out = sc;
+    });
+}
+
+rfa_status rfa_scorer_destroy(rfa_scorer* sc)
+{
+    if (!sc) return RFA_OK;
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(sc->ctx->mu);
+        (void)hipSetDevice(sc->ctx->device);
+        (void)hipStreamSynchronize(sc->ctx->stream);
+        free_strings(sc->q);
+        if (sc->d_tile_order) (void)hipFree(sc->d_tile_order);
+        if (sc->d_block_queries) (void)hipFree(sc->d_block_queries);
+        delete sc;
+    });
+}
+
+rfa_status rfa_cdist(rfa_scorer* sc, const rfa_strings* choices, int kind, int semantics, uint64_t cutoff_u,
+                     double cutoff_d, void* out, int out_on_device)
+{
+    if (!sc || !choices) return fail(RFA_ERR_INVALID_ARGUMENT, "scorer or choices is NULL");
+    if (!out && sc->q->count && choices->count) return fail(RFA_ERR_INVALID_ARGUMENT, "out is NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(sc->ctx->mu);
+        HIP_CHECK(hipSetDevice(sc->ctx->device));
+        cdist_impl(sc, choices, kind, semantics, cutoff_u, cutoff_d, out, out_on_device);
+    });
+}
+
+rfa_status rfa_pairs(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics, int kind,
+                     uint64_t cutoff_u, double cutoff_d, int schedule, void* out, int out_on_device)
+{
+    (void)ctx, (void)a, (void)b, (void)metrics, (void)kind, (void)cutoff_u, (void)cutoff_d, (void)schedule, (void)out,
+        (void)out_on_device;
+    return fail(RFA_ERR_UNSUPPORTED, "rfa_pairs: not built yet");
+}
+
+rfa_status rfa_topk(rfa_scorer* sc, const rfa_strings* choices, int kind, double cutoff_d, uint32_t k,
+                    int64_t* out_index, double* out_score)
+{
+    (void)sc, (void)choices, (void)kind, (void)cutoff_d, (void)k, (void)out_index, (void)out_score;
+    return fail(RFA_ERR_UNSUPPORTED, "rfa_topk: not built yet");
+}
+
+} // extern "C"

@@ -1,0 +1,63 @@
+// rfa_internal.hpp -- kernel argument blocks and launch entry points (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rfa_types.hpp"
+
+namespace rfa {
+
+// Query-tile kernel (rfa_tile.hip): tiles of 64 queries of <= 64 units.
+struct TileArgs {
+    const void* qchars;
+    const uint64_t* qoff;
+    const uint32_t* qorder; // query indices, grouped 64 per tile
+    uint32_t nq;            // entries of qorder
+    uint32_t tile_base;     // first tile of this launch
+    const void* cchars;
+    const uint64_t* coff;
+    const uint32_t* corder; // choices to score (nullptr: 0 .. ncsel-1)
+    uint32_t ncsel;
+    uint32_t choices_per_block;
+    uint64_t out_stride; // number of choices in the set (row length of out)
+    uint32_t rows_cap;   // PM rows reserved in LDS (>= distinct symbols of any tile + 1)
+    uint32_t hash_bits;  // log2 slots of the LDS symbol hash (16/32-bit units)
+    ScoreSpec spec;
+    void* out;
+};
+
+size_t tile_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t hash_bits);
+hipError_t launch_tile(int algo, int char_width, int word_bits, const TileArgs& a, uint32_t ntiles,
+                       uint32_t nblocks_x, hipStream_t s);
+
+// Block kernel (rfa_block.hip): one wave per pair, pattern words across lanes.
+// Item i pairs string item_a[i] of set A with item_b[i] of set B; the pattern
+// (bit-vector side) is A's string unless item_swap[i] is set.
+struct BlockArgs {
+    const void* achars;
+    const uint64_t* aoff;
+    const void* bchars;
+    const uint64_t* boff;
+    const uint32_t* item_a;
+    const uint32_t* item_b;
+    const uint8_t* item_swap;  // nullptr: never swap
+    const uint64_t* out_index; // output element per item (explicit mode)
+    uint32_t cross_nb;         // 0: explicit items; else item k = (item_a[k / cross_nb], item_b[k % cross_nb])
+    uint64_t out_stride;       // cross mode: out[a * out_stride + b]
+    uint32_t nitems;
+    uint32_t reuse_pattern;    // consecutive items with the same pattern reuse its PM
+    uint32_t* queue;           // work counter, zero at launch
+    uint64_t* pm_scratch;      // global arenas (when the PM does not fit in LDS)
+    uint64_t pm_scratch_words; // uint64 words per workgroup arena
+    uint64_t* tflag_scratch;   // Jaro text flags, tflag_words per workgroup
+    uint64_t tflag_words;
+    uint32_t hash_bits; // symbol hash size (16/32-bit units)
+    ScoreSpec spec;
+    void* out;
+};
+
+size_t block_arena_bytes(int char_width, uint32_t max_m, uint32_t max_rows, uint32_t hash_bits);
+hipError_t launch_block(int algo, int char_width, const BlockArgs& a, uint32_t nblocks, bool lds, size_t lds_bytes,
+                        uint32_t slots, hipStream_t s);
+
+} // namespace rfa

@@ -1,0 +1,336 @@
+// rfa_semantics.hpp -- device-side result semantics (score_cutoff, normalization).
+//
+// Every kernel computes an exact raw value per pair (edit distance, LCS
+// length, or the Jaro common/transposition counts).  The functions here turn
+// it into the value the reference API returns, following the reference's
+// CRTP bases operation by operation so integer results are identical and
+// double results round identically:
+//
+//   Cached* distance/similarity ........ rapidfuzz/details/distance.hpp:269-388
+//   Cached* normalized_* ................ rapidfuzz/details/distance.hpp:210-267
+//   NormSim_to_NormDist (+1e-5) ......... rapidfuzz/details/common.hpp:50-53
+//   experimental::Multi* (cut after) .... rapidfuzz/details/distance.hpp:390-582
+//   Levenshtein weights dispatch ........ rapidfuzz/distance/Levenshtein.hpp:445-477
+//   Jaro filters and formula ............ rapidfuzz/distance/Jaro_impl.hpp:32-71,409-454
+//   Jaro-Winkler cutoff and boost ....... rapidfuzz/distance/JaroWinkler_impl.hpp:41-71
+//   MultiJaroWinkler .................... rapidfuzz/distance/JaroWinkler.hpp:122-145
+//   fuzz::ratio / CachedRatio ........... rapidfuzz/fuzz_impl.hpp:28-47, fuzz.hpp:82-89
+//
+// Compile with -ffp-contract=off: the JW boost must not become an FMA.
+#pragma once
+#include <stdint.h>
+
+#include "rfa_types.hpp"
+
+namespace rfa {
+
+struct RawResult {
+    uint64_t v;      // distance (Levenshtein/OSA), LCS length (Indel/LCSseq/ratio), common chars (Jaro)
+    uint32_t tr;     // Jaro: mismatched flagged pairs (halved inside the formula)
+    uint32_t prefix; // Jaro-Winkler: common prefix length, at most 4
+};
+
+__device__ __forceinline__ uint64_t u64min(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint64_t u64max(uint64_t a, uint64_t b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ uint64_t ceil_div_u64(uint64_t a, uint64_t b) { return a / b + (uint64_t)(a % b != 0); }
+
+// --- integer metrics -------------------------------------------------------
+
+struct IntPair {
+    const ScoreSpec* s;
+    uint64_t l1, l2;
+    uint64_t raw; // exact distance (LEV uniform, OSA) or exact LCS (INDEL, LCSSEQ, LEV indel-weights)
+};
+
+__device__ __forceinline__ uint64_t int_maximum(const IntPair& p)
+{
+    const ScoreSpec& s = *p.s;
+    switch (s.metric) {
+    case RFA_LEVENSHTEIN: {
+        // Levenshtein_impl.hpp:118-128
+        uint64_t m = p.l1 * s.del + p.l2 * s.ins;
+        if (p.l1 >= p.l2)
+            m = u64min(m, p.l2 * s.rep + (p.l1 - p.l2) * s.del);
+        else
+            m = u64min(m, p.l1 * s.rep + (p.l2 - p.l1) * s.ins);
+        return m;
+    }
+    case RFA_INDEL: return p.l1 + p.l2;
+    default: return u64max(p.l1, p.l2); // LCSseq, OSA
+    }
+}
+
+// exact Indel distance from the LCS length
+__device__ __forceinline__ uint64_t indel_from_lcs(const IntPair& p) { return p.l1 + p.l2 - 2 * p.raw; }
+
+// derived._distance(s2, c) of the distance metrics, from the exact value.
+__device__ __forceinline__ uint64_t int_distance_core(const IntPair& p, uint64_t c)
+{
+    const ScoreSpec& s = *p.s;
+    if (s.metric == RFA_LEVENSHTEIN) {
+        // CachedLevenshtein::_distance: uniform or Indel, scaled by the common weight
+        if (s.ins == s.del) {
+            if (s.ins == 0) return 0;
+            uint64_t nc = ceil_div_u64(c, s.ins);
+            uint64_t d;
+            if (s.ins == s.rep) {
+                nc = u64min(nc, u64max(p.l1, p.l2)); // uniform_levenshtein_distance clamp
+                d = p.raw <= nc ? p.raw : nc + 1;
+            }
+            else {
+                uint64_t exact = indel_from_lcs(p);
+                d = exact <= nc ? exact : nc + 1;
+            }
+            d *= s.ins;
+            return d <= c ? d : c + 1;
+        }
+        return c + 1; // general weights are rejected before launch
+    }
+    uint64_t exact = (s.metric == RFA_INDEL) ? indel_from_lcs(p) : p.raw;
+    return exact <= c ? exact : c + 1;
+}
+
+// LCSseq is a similarity metric: CachedSimilarityBase::_distance
+__device__ __forceinline__ uint64_t lcs_distance(const IntPair& p, uint64_t c)
+{
+    uint64_t maximum = int_maximum(p);
+    uint64_t cs = (maximum > c) ? maximum - c : 0;
+    uint64_t sim = p.raw >= cs ? p.raw : 0;
+    uint64_t dist = maximum - sim;
+    return dist <= c ? dist : c + 1;
+}
+
+__device__ __forceinline__ uint64_t int_api_distance(const IntPair& p, uint64_t c)
+{
+    if (p.s->metric == RFA_LCSSEQ) return lcs_distance(p, c);
+    return int_distance_core(p, c);
+}
+
+__device__ __forceinline__ uint64_t int_api_similarity(const IntPair& p, uint64_t c)
+{
+    if (p.s->metric == RFA_LCSSEQ) return p.raw >= c ? p.raw : 0;
+    uint64_t maximum = int_maximum(p);
+    if (c > maximum) return 0;
+    uint64_t dist = int_distance_core(p, maximum - c);
+    uint64_t sim = maximum - dist;
+    return sim >= c ? sim : 0;
+}
+
+__device__ __forceinline__ double int_api_norm_distance(const IntPair& p, double c)
+{
+    uint64_t maximum = int_maximum(p);
+    uint64_t cd = (uint64_t)ceil((double)maximum * c);
+    double dist = (double)int_api_distance(p, cd);
+    double norm = (maximum != 0) ? dist / (double)maximum : 0.0;
+    return (norm <= c) ? norm : 1.0;
+}
+
+__device__ __forceinline__ double normsim_to_normdist(double c) { return dmin(1.0, 1.0 - c + 0.00001); }
+
+__device__ __forceinline__ double int_api_norm_similarity(const IntPair& p, double c)
+{
+    double nd = int_api_norm_distance(p, normsim_to_normdist(c));
+    double ns = 1.0 - nd;
+    return (ns >= c) ? ns : 0.0;
+}
+
+// Multi* semantics: the full (uncut) distance first, cut after.
+__device__ __forceinline__ uint64_t multi_full_distance(const IntPair& p)
+{
+    switch (p.s->metric) {
+    case RFA_INDEL:
+    case RFA_RATIO: return indel_from_lcs(p);
+    case RFA_LCSSEQ: return u64max(p.l1, p.l2) - p.raw;
+    default: return p.raw; // Levenshtein / OSA (empty s1 already gives s2.size())
+    }
+}
+
+// --- Jaro / Jaro-Winkler ---------------------------------------------------
+
+// jaro_similarity(PM, P, T, cutoff) with the library's filters, from counts.
+__device__ __forceinline__ double jaro_from_counts(uint64_t Pl, uint64_t Tl, uint64_t cc, uint64_t tr, bool eq11,
+                                                   double cutoff)
+{
+    if (cutoff > 1.0) return 0.0;
+    if (!Pl && !Tl) return 1.0;
+    if (!Tl || !Pl) return 0.0;
+    {
+        double min_len = (double)(Pl < Tl ? Pl : Tl);
+        double Sim = min_len / (double)Pl + min_len / (double)Tl + 1.0;
+        Sim /= 3.0;
+        if (!(Sim >= cutoff)) return 0.0;
+    }
+    if (Pl == 1 && Tl == 1) return eq11 ? 1.0 : 0.0;
+    if (!cc) return 0.0;
+    {
+        double Sim = 0;
+        Sim += (double)cc / (double)Pl;
+        Sim += (double)cc / (double)Tl;
+        Sim += 1.0;
+        Sim /= 3.0;
+        if (!(Sim >= cutoff)) return 0.0;
+    }
+    uint64_t t = tr / 2;
+    double Sim = 0;
+    Sim += (double)cc / (double)Pl;
+    Sim += (double)cc / (double)Tl;
+    Sim += ((double)cc - (double)t) / (double)cc;
+    Sim /= 3.0;
+    return (Sim >= cutoff) ? Sim : 0;
+}
+
+struct FpPair {
+    const ScoreSpec* s;
+    uint64_t l1, l2;
+    RawResult r;
+};
+
+__device__ __forceinline__ double fp_similarity(const FpPair& p, double c)
+{
+    const bool eq11 = p.r.v == 1; // 1x1 strings: one common char iff equal
+    if (p.s->metric == RFA_JARO) return jaro_from_counts(p.l1, p.l2, p.r.v, p.r.tr, eq11, c);
+    // JaroWinkler_impl.hpp:41-71
+    double w = p.s->prefix_weight;
+    double jaro_cutoff = c;
+    if (jaro_cutoff > 0.7) {
+        double prefix_sim = (double)p.r.prefix * w;
+        if (prefix_sim >= 1.0)
+            jaro_cutoff = 0.7;
+        else
+            jaro_cutoff = dmax(0.7, (prefix_sim - jaro_cutoff) / (prefix_sim - 1.0));
+    }
+    double Sim = jaro_from_counts(p.l1, p.l2, p.r.v, p.r.tr, eq11, jaro_cutoff);
+    if (Sim > 0.7) {
+        Sim += (double)p.r.prefix * w * (1.0 - Sim);
+        Sim = dmin(Sim, 1.0);
+    }
+    return (Sim >= c) ? Sim : 0;
+}
+
+__device__ __forceinline__ double fp_distance(const FpPair& p, double c)
+{
+    double cs = (1.0 > c) ? 1.0 - c : 0;
+    double dist = 1.0 - fp_similarity(p, cs);
+    return (dist <= c) ? dist : 1.0;
+}
+
+__device__ __forceinline__ double fp_norm_distance(const FpPair& p, double c)
+{
+    double cd = ceil(1.0 * c);
+    double dist = fp_distance(p, cd);
+    double norm = dist / 1.0;
+    return (norm <= c) ? norm : 1.0;
+}
+
+__device__ __forceinline__ double fp_norm_similarity(const FpPair& p, double c)
+{
+    double nd = fp_norm_distance(p, normsim_to_normdist(c));
+    double ns = 1.0 - nd;
+    return (ns >= c) ? ns : 0.0;
+}
+
+__device__ __forceinline__ double multi_fp_similarity(const FpPair& p, double c)
+{
+    const bool eq11 = p.r.v == 1;
+    if (p.s->metric == RFA_JARO) return jaro_from_counts(p.l1, p.l2, p.r.v, p.r.tr, eq11, c);
+    double s = jaro_from_counts(p.l1, p.l2, p.r.v, p.r.tr, eq11, dmin(0.7, c));
+    if (s > 0.7) {
+        s += (double)p.r.prefix * p.s->prefix_weight * (1.0 - s);
+        s = dmin(s, 1.0);
+    }
+    if (s < c) s = 0.0;
+    return s;
+}
+
+// --- dispatch ----------------------------------------------------------------
+
+// Writes the reference API value for one pair into out[idx] (uint64 or double).
+__device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r, void* out,
+                                         uint64_t idx)
+{
+    uint64_t* ou = (uint64_t*)out;
+    double* od = (double*)out;
+    if (s.metric == RFA_JARO || s.metric == RFA_JARO_WINKLER) {
+        FpPair p{&s, l1, l2, r};
+        double v;
+        if (s.semantics == RFA_MULTI) {
+            if (s.kind == RFA_SIMILARITY)
+                v = multi_fp_similarity(p, s.cutoff_d);
+            else {
+                double dist = 1.0 - multi_fp_similarity(p, 0.0);
+                if (s.kind != RFA_DISTANCE) dist = dist <= 1.0 ? dist : 1.0;
+                if (s.kind == RFA_NORMALIZED_SIMILARITY) {
+                    double ns = 1.0 - dist;
+                    v = ns >= s.cutoff_d ? ns : 0.0;
+                }
+                else
+                    v = dist <= s.cutoff_d ? dist : 1.0;
+            }
+        }
+        else {
+            switch (s.kind) {
+            case RFA_DISTANCE: v = fp_distance(p, s.cutoff_d); break;
+            case RFA_SIMILARITY: v = fp_similarity(p, s.cutoff_d); break;
+            case RFA_NORMALIZED_DISTANCE: v = fp_norm_distance(p, s.cutoff_d); break;
+            default: v = fp_norm_similarity(p, s.cutoff_d); break;
+            }
+        }
+        od[idx] = v;
+        return;
+    }
+    IntPair p{&s, l1, l2, r.v};
+    if (s.metric == RFA_RATIO) {
+        // ratio = Indel normalized similarity * 100 (Cached: with the +1e-5 widening)
+        ScoreSpec t = s;
+        t.metric = RFA_INDEL;
+        IntPair q{&t, l1, l2, r.v};
+        if (s.semantics == RFA_MULTI) {
+            uint64_t maximum = l1 + l2;
+            double norm = maximum ? (double)indel_from_lcs(q) / (double)maximum : 0.0;
+            norm = norm <= 1.0 ? norm : 1.0;
+            double ns = 1.0 - norm;
+            double c = s.cutoff_d / 100.0;
+            od[idx] = (ns >= c ? ns : 0.0) * 100.0;
+        }
+        else
+            od[idx] = int_api_norm_similarity(q, s.cutoff_d / 100) * 100;
+        return;
+    }
+    if (s.semantics == RFA_MULTI) {
+        uint64_t maximum = (s.metric == RFA_INDEL) ? l1 + l2 : u64max(l1, l2);
+        uint64_t d = multi_full_distance(p);
+        switch (s.kind) {
+        case RFA_DISTANCE: ou[idx] = d <= s.cutoff_u ? d : s.cutoff_u + 1; return;
+        case RFA_SIMILARITY:
+            if (s.metric == RFA_LCSSEQ)
+                ou[idx] = p.raw >= s.cutoff_u ? p.raw : 0;
+            else {
+                uint64_t sim = maximum - d;
+                ou[idx] = sim >= s.cutoff_u ? sim : 0;
+            }
+            return;
+        case RFA_NORMALIZED_DISTANCE: {
+            double norm = maximum ? (double)d / (double)maximum : 0.0;
+            od[idx] = norm <= s.cutoff_d ? norm : 1.0;
+            return;
+        }
+        default: {
+            double norm = maximum ? (double)d / (double)maximum : 0.0;
+            norm = norm <= 1.0 ? norm : 1.0;
+            double ns = 1.0 - norm;
+            od[idx] = ns >= s.cutoff_d ? ns : 0.0;
+            return;
+        }
+        }
+    }
+    switch (s.kind) {
+    case RFA_DISTANCE: ou[idx] = int_api_distance(p, s.cutoff_u); return;
+    case RFA_SIMILARITY: ou[idx] = int_api_similarity(p, s.cutoff_u); return;
+    case RFA_NORMALIZED_DISTANCE: od[idx] = int_api_norm_distance(p, s.cutoff_d); return;
+    default: od[idx] = int_api_norm_similarity(p, s.cutoff_d); return;
+    }
+}
+
+} // namespace rfa

@@ -1,0 +1,301 @@
+// rfa_tile.hip -- query-tile kernel: 64 short queries (one per lane) against a
+// stream of choices that is uniform across the wave.
+//
+// Shape.  A workgroup of 4 waves owns one tile of up to 64 queries, each at
+// most 64 code units (the reference's single-word paths:
+// Levenshtein_impl.hpp:259-322, LCSseq_impl.hpp:205-260 with N=1,
+// OSA_impl.hpp:34-74, Jaro_impl.hpp:94-132 and 258-276).  The tile's
+// pattern-match vectors PM[symbol][lane] are built into LDS once
+// (PatternMatchVector.hpp:62-134 semantics; symbols compacted to dense rows),
+// then each wave walks its share of the choices.  The current choice's code
+// units are the same for every lane, so PM[row][lane] is one contiguous,
+// conflict-free 64-lane LDS read per text position and the whole wave runs
+// the same trip count.
+//
+// Word width W is 32 bits for tiles whose queries all fit in 32 units and 64
+// bits otherwise (a 64-bit op is two VALU ops on CDNA).
+#include "rfa_device.hpp"
+#include "rfa_internal.hpp"
+#include "rfa_semantics.hpp"
+
+namespace rfa {
+
+template <typename W>
+__device__ __forceinline__ int popc(W x)
+{
+    if constexpr (sizeof(W) == 8)
+        return __popcll(x);
+    else
+        return __popc(x);
+}
+
+// ---- recurrences on one word (one query per lane) -----------------------
+
+// Levenshtein_impl.hpp:282-309
+template <typename W>
+__device__ __forceinline__ void lev_step(W X, W& VP, W& VN, uint32_t& dist, W mask)
+{
+    W D0 = (((X & VP) + VP) ^ VP) | X | VN;
+    W HP = VN | ~(D0 | VP);
+    W HN = D0 & VP;
+    dist += (HP & mask) != 0;
+    dist -= (HN & mask) != 0;
+    HP = (HP << 1) | 1;
+    HN = HN << 1;
+    VP = HN | ~(D0 | HP);
+    VN = HP & D0;
+}
+
+// LCSseq_impl.hpp:219-252 (one word, no carry in)
+template <typename W>
+__device__ __forceinline__ void lcs_step(W X, W& S)
+{
+    W u = S & X;
+    S = (S + u) | (S - u);
+}
+
+// OSA_impl.hpp:48-71
+template <typename W>
+__device__ __forceinline__ void osa_step(W X, W& VP, W& VN, W& D0, W& PMold, uint32_t& dist, W mask)
+{
+    W TR = (((~D0) & X) << 1) & PMold;
+    D0 = (((X & VP) + VP) ^ VP) | X | VN;
+    D0 = D0 | TR;
+    W HP = VN | ~(D0 | VP);
+    W HN = D0 & VP;
+    dist += (HP & mask) != 0;
+    dist -= (HN & mask) != 0;
+    HP = (HP << 1) | 1;
+    HN = HN << 1;
+    VP = HN | ~(D0 | HP);
+    VN = HP & D0;
+    PMold = X;
+}
+
+template <typename CharT>
+struct SymLookup;
+
+// 8-bit units: direct 256-entry row table.
+template <>
+struct SymLookup<uint8_t> {
+    const uint8_t* rowmap;
+    __device__ __forceinline__ uint32_t operator()(uint32_t c) const { return rowmap[c]; }
+};
+
+template <>
+struct SymLookup<uint16_t> {
+    HashMap map;
+    __device__ __forceinline__ uint32_t operator()(uint32_t c) const { return map.find(c); }
+};
+
+template <>
+struct SymLookup<uint32_t> {
+    HashMap map;
+    __device__ __forceinline__ uint32_t operator()(uint32_t c) const { return map.find(c); }
+};
+
+// Builds the tile's PM rows in LDS.  Returns the lookup functor.
+template <typename CharT, typename W>
+__device__ __forceinline__ SymLookup<CharT> build_tile_pm(const CharT* __restrict__ q, uint64_t qo, uint32_t m,
+                                                          W* pm, uint8_t* extra, const TileArgs& a)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    SymLookup<CharT> look;
+    uint32_t R;
+    if constexpr (sizeof(CharT) == 1) {
+        uint32_t* flags = (uint32_t*)extra;         // 256
+        uint8_t* rowmap = (uint8_t*)(flags + 256);  // 256
+        uint32_t* tmp = (uint32_t*)(rowmap + 256);  // 4
+        flags[tid] = 0;
+        __syncthreads();
+        for (uint32_t j = wave; j < m; j += 4) flags[q[qo + j]] = 1;
+        __syncthreads();
+        uint32_t f = flags[tid];
+        uint32_t total;
+        uint32_t ex = block_exclusive_scan256(f, tmp, &total);
+        rowmap[tid] = f ? (uint8_t)(ex + 1) : 0;
+        R = total + 1;
+        look.rowmap = rowmap;
+    }
+    else {
+        const uint32_t H = 1u << a.hash_bits;
+        uint64_t* keys = (uint64_t*)extra;
+        uint16_t* rows = (uint16_t*)(keys + H);
+        uint32_t* tmp = (uint32_t*)(rows + H + 8);
+        for (uint32_t i = tid; i < H; i += 256) keys[i] = EMPTY_KEY;
+        __syncthreads();
+        HashMap map{keys, rows, a.hash_bits};
+        for (uint32_t j = wave; j < m; j += 4) map.insert((uint32_t)q[qo + j]);
+        __syncthreads();
+        uint32_t base = 0;
+        for (uint32_t s0 = 0; s0 < H; s0 += 256) {
+            uint32_t i = s0 + tid;
+            uint32_t f = (i < H && keys[i] != EMPTY_KEY) ? 1u : 0u;
+            uint32_t total;
+            uint32_t ex = block_exclusive_scan256(f, tmp, &total);
+            if (f) rows[i] = (uint16_t)(base + ex + 1);
+            base += total;
+        }
+        R = base + 1;
+        look.map = map;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < R * 64; i += 256) pm[i] = 0;
+    __syncthreads();
+    for (uint32_t j = wave; j < m; j += 4) {
+        uint32_t r = look((uint32_t)q[qo + j]);
+        if constexpr (sizeof(W) == 8)
+            atomicOr((unsigned long long*)&pm[r * 64 + lane], 1ull << j);
+        else
+            atomicOr((unsigned int*)&pm[r * 64 + lane], 1u << j);
+    }
+    __syncthreads();
+    return look;
+}
+
+template <int ALGO, typename CharT, typename W>
+__global__ __launch_bounds__(256) void k_tile(TileArgs a)
+{
+    extern __shared__ uint64_t smem_u64[];
+    W* pm = (W*)smem_u64;
+    uint8_t* extra = (uint8_t*)(pm + (size_t)a.rows_cap * 64);
+    const CharT* __restrict__ q = (const CharT*)a.qchars;
+    const CharT* __restrict__ c = (const CharT*)a.cchars;
+
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t tile = blockIdx.y + a.tile_base;
+    const uint32_t qpos = tile * 64 + lane;
+    const bool qact = qpos < a.nq;
+    const uint32_t qidx = qact ? a.qorder[qpos] : 0;
+    const uint64_t qo = qact ? a.qoff[qidx] : 0;
+    const uint32_t m = qact ? (uint32_t)(a.qoff[qidx + 1] - qo) : 0;
+
+    SymLookup<CharT> look = build_tile_pm<CharT, W>(q, qo, m, pm, extra, a);
+
+    constexpr uint32_t BITS = sizeof(W) * 8;
+    const W mask = m ? ((W)1 << (m - 1)) : (W)0;
+    const uint32_t it0 = blockIdx.x * a.choices_per_block;
+    const uint32_t it1 = min(a.ncsel, it0 + a.choices_per_block);
+
+    for (uint32_t it = it0 + wave; it < it1; it += 4) {
+        const uint32_t cidx = a.corder ? a.corder[it] : it;
+        const uint64_t co = a.coff[cidx];
+        const uint32_t n = (uint32_t)(a.coff[cidx + 1] - co);
+        RawResult res{0, 0, 0};
+
+        if constexpr (ALGO == ALGO_LEV || ALGO == ALGO_OSA || ALGO == ALGO_LCS) {
+            W VP = ~(W)0, VN = 0, D0 = 0, PMold = 0, S = ~(W)0;
+            uint32_t dist = m;
+            for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+                uint32_t row = 0;
+                if (k0 + lane < n) row = look((uint32_t)c[co + k0 + lane]);
+                const uint32_t cnt = min(64u, n - k0);
+                for (uint32_t i = 0; i < cnt; ++i) {
+                    const uint32_t r = readlane(row, i);
+                    const W X = pm[r * 64 + lane];
+                    if constexpr (ALGO == ALGO_LEV)
+                        lev_step<W>(X, VP, VN, dist, mask);
+                    else if constexpr (ALGO == ALGO_OSA)
+                        osa_step<W>(X, VP, VN, D0, PMold, dist, mask);
+                    else
+                        lcs_step<W>(X, S);
+                }
+            }
+            if constexpr (ALGO == ALGO_LCS)
+                res.v = (uint64_t)popc<W>(~S);
+            else
+                res.v = m ? dist : n;
+        }
+        else {
+            // Jaro word path; the host routes only choices with n <= 64 here.
+            uint32_t row = 0;
+            if (lane < n) row = look((uint32_t)c[co + lane]);
+            uint32_t Bound = max(m, n) / 2;
+            if (Bound > 0) Bound--;
+            W BoundMask = (Bound + 1 >= BITS) ? ~(W)0 : (((W)1 << (Bound + 1)) - 1);
+            W Pf = 0;
+            uint64_t Tf = 0;
+            uint32_t pfx = 0;
+            const uint32_t maxpfx = min(min(m, n), 4u);
+            for (uint32_t j = 0; j < n; ++j) {
+                const uint32_t r = readlane(row, j);
+                const W PMr = pm[r * 64 + lane];
+                if (j < maxpfx && pfx == j && ((PMr >> j) & 1)) pfx = j + 1;
+                const W x = PMr & BoundMask & ~Pf;
+                Pf |= x & ((W)0 - x);
+                Tf |= (uint64_t)(x != 0) << j;
+                BoundMask = (j < Bound) ? (W)((BoundMask << 1) | 1) : (W)(BoundMask << 1);
+            }
+            // count_transpositions_word (Jaro_impl.hpp:258-276), walked in text order
+            uint32_t tr = 0;
+            W P2 = Pf;
+            for (uint32_t j = 0; j < n; ++j) {
+                const uint32_t r = readlane(row, j);
+                const W PMr = pm[r * 64 + lane];
+                if ((Tf >> j) & 1) {
+                    const W pmask = P2 & ((W)0 - P2);
+                    tr += (PMr & pmask) == 0;
+                    P2 ^= pmask;
+                }
+            }
+            res.v = (uint64_t)popc<W>(Pf);
+            res.tr = tr;
+            res.prefix = pfx;
+        }
+        if (qact) finalize(a.spec, m, n, res, a.out, (uint64_t)qidx * a.out_stride + cidx);
+    }
+}
+
+template <int ALGO, typename CharT, typename W>
+static void launch_one(const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x, size_t lds, hipStream_t s)
+{
+    dim3 grid(nblocks_x, ntiles);
+    hipLaunchKernelGGL((k_tile<ALGO, CharT, W>), grid, dim3(256), lds, s, a);
+}
+
+template <typename CharT, typename W>
+static void launch_algo(int algo, const TileArgs& a, uint32_t ntiles, uint32_t nbx, size_t lds, hipStream_t s)
+{
+    switch (algo) {
+    case ALGO_LEV: launch_one<ALGO_LEV, CharT, W>(a, ntiles, nbx, lds, s); break;
+    case ALGO_LCS: launch_one<ALGO_LCS, CharT, W>(a, ntiles, nbx, lds, s); break;
+    case ALGO_OSA: launch_one<ALGO_OSA, CharT, W>(a, ntiles, nbx, lds, s); break;
+    default: launch_one<ALGO_JARO, CharT, W>(a, ntiles, nbx, lds, s); break;
+    }
+}
+
+size_t tile_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t hash_bits)
+{
+    size_t pm = (size_t)rows_cap * 64 * (word_bits / 8);
+    size_t extra;
+    if (char_width == 1)
+        extra = 256 * 4 + 256 + 16;
+    else
+        extra = ((size_t)1 << hash_bits) * 10 + 16 + 16;
+    return pm + extra;
+}
+
+hipError_t launch_tile(int algo, int char_width, int word_bits, const TileArgs& a, uint32_t ntiles,
+                       uint32_t nblocks_x, hipStream_t s)
+{
+    size_t lds = tile_lds_bytes(char_width, word_bits, a.rows_cap, a.hash_bits);
+    if (word_bits == 32) {
+        if (char_width == 1)
+            launch_algo<uint8_t, uint32_t>(algo, a, ntiles, nblocks_x, lds, s);
+        else if (char_width == 2)
+            launch_algo<uint16_t, uint32_t>(algo, a, ntiles, nblocks_x, lds, s);
+        else
+            launch_algo<uint32_t, uint32_t>(algo, a, ntiles, nblocks_x, lds, s);
+    }
+    else {
+        if (char_width == 1)
+            launch_algo<uint8_t, uint64_t>(algo, a, ntiles, nblocks_x, lds, s);
+        else if (char_width == 2)
+            launch_algo<uint16_t, uint64_t>(algo, a, ntiles, nblocks_x, lds, s);
+        else
+            launch_algo<uint32_t, uint64_t>(algo, a, ntiles, nblocks_x, lds, s);
+    }
+    return hipGetLastError();
+}
+
+} // namespace rfa

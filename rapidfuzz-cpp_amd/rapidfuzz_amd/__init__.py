@@ -1,0 +1,256 @@
+"""Python bindings (ctypes) for librapidfuzz_amd.so, the gfx950 scoring engine.
+
+Thin mirror of the C ABI in include/rapidfuzz_amd.h, used by the tests and
+bench.py.  There is no CPU fallback: importing this module without the built
+library raises, and every call runs on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "librapidfuzz_amd.so")
+
+LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO = range(7)
+DISTANCE, SIMILARITY, NORMALIZED_DISTANCE, NORMALIZED_SIMILARITY = range(4)
+CACHED, MULTI = 0, 1
+SCHEDULE_BUCKETED, SCHEDULE_PLAIN = 0, 1
+INT64_MAX = (1 << 63) - 1
+
+_STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED", 3: "RFA_ERR_HIP",
+           4: "RFA_ERR_OUT_OF_MEMORY", 5: "RFA_ERR_NO_DEVICE"}
+
+# every symbol include/rapidfuzz_amd.h declares
+EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
+            "rfa_context_synchronize", "rfa_context_stream", "rfa_strings_create", "rfa_strings_destroy",
+            "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_pairs", "rfa_topk"]
+
+
+class RfaError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class InvalidArgument(RfaError, ValueError):
+    pass
+
+
+class Weights(ctypes.Structure):
+    _fields_ = [("insert_cost", ctypes.c_uint64), ("delete_cost", ctypes.c_uint64),
+                ("replace_cost", ctypes.c_uint64)]
+
+
+_vp = ctypes.c_void_p
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run `make -C rapidfuzz-cpp_amd` (or __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    L.rfa_abi_version.restype = ctypes.c_int
+    L.rfa_last_error.restype = ctypes.c_char_p
+    L.rfa_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.rfa_context_destroy.argtypes = [_vp]
+    L.rfa_context_synchronize.argtypes = [_vp]
+    L.rfa_context_stream.argtypes = [_vp]
+    L.rfa_context_stream.restype = _vp
+    L.rfa_strings_create.argtypes = [_vp, _vp, ctypes.c_int, _vp, ctypes.c_uint64, ctypes.POINTER(_vp)]
+    L.rfa_strings_destroy.argtypes = [_vp]
+    L.rfa_strings_count.argtypes = [_vp]
+    L.rfa_strings_count.restype = ctypes.c_uint64
+    L.rfa_scorer_create.argtypes = [_vp, ctypes.c_int, _vp, ctypes.POINTER(Weights), ctypes.c_double,
+                                    ctypes.POINTER(_vp)]
+    L.rfa_scorer_destroy.argtypes = [_vp]
+    L.rfa_cdist.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _vp,
+                            ctypes.c_int]
+    L.rfa_pairs.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+                            _vp, ctypes.c_int]
+    L.rfa_topk.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp]
+    for name in ("rfa_context_create", "rfa_context_destroy", "rfa_context_synchronize", "rfa_strings_create",
+                 "rfa_strings_destroy", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_pairs",
+                 "rfa_topk"):
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        msg = _lib.rfa_last_error().decode()
+        if status == 1:
+            raise InvalidArgument(status, msg)
+        raise RfaError(status, msg)
+
+
+def is_int_result(metric, kind):
+    return metric in (LEVENSHTEIN, INDEL, LCSSEQ, OSA) and kind in (DISTANCE, SIMILARITY)
+
+
+def default_cutoffs(metric, kind):
+    """The reference's default cutoffs for a method (details/distance.hpp:211-301)."""
+    if is_int_result(metric, kind):
+        return (INT64_MAX if kind == DISTANCE else 0), 0.0
+    if kind == NORMALIZED_DISTANCE or (kind == DISTANCE and metric in (JARO, JARO_WINKLER)):
+        return INT64_MAX, 1.0
+    return INT64_MAX, 0.0
+
+
+def _as_units(s, width):
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[width]
+    if isinstance(s, np.ndarray):
+        return s.astype(dt, copy=False)
+    if isinstance(s, (bytes, bytearray)):
+        return np.frombuffer(bytes(s), dtype=np.uint8).astype(dt)
+    if isinstance(s, str):
+        return np.array([ord(c) for c in s], dtype=dt)
+    return np.asarray(list(s), dtype=dt)
+
+
+def pack(strings, width=None):
+    """(chars, offsets, width) for a list of strings; width inferred from the code points."""
+    arrs = []
+    mx = 0
+    for s in strings:
+        if isinstance(s, str):
+            a = np.array([ord(c) for c in s], dtype=np.uint32)
+        elif isinstance(s, (bytes, bytearray)):
+            a = np.frombuffer(bytes(s), dtype=np.uint8)
+        else:
+            a = np.asarray(s)
+        arrs.append(a)
+        if a.size:
+            mx = max(mx, int(a.max()))
+    if width is None:
+        width = 1 if mx < 256 else (2 if mx < 65536 else 4)
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[width]
+    off = np.zeros(len(arrs) + 1, dtype=np.uint64)
+    if arrs:
+        np.cumsum([a.size for a in arrs], out=off[1:])
+    chars = np.concatenate([a.astype(dt) for a in arrs]) if arrs and off[-1] else np.zeros(1, dt)
+    return np.ascontiguousarray(chars), off, width
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        L = load_library()
+        h = _vp()
+        _check(L.rfa_context_create(device, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def stream(self):
+        return _lib.rfa_context_stream(self.h)
+
+    def synchronize(self):
+        _check(_lib.rfa_context_synchronize(self.h))
+
+    def close(self):
+        """Destroys the context; strings and scorers made from it become unusable
+        (their device memory is released with the context's process)."""
+        if self.h:
+            _check(_lib.rfa_context_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def strings(self, strings=None, *, chars=None, offsets=None, width=None):
+        return Strings(self, strings, chars=chars, offsets=offsets, width=width)
+
+
+class Strings:
+    """A device-resident packed string set."""
+
+    def __init__(self, ctx: Context, strings=None, *, chars=None, offsets=None, width=None):
+        self.ctx = ctx
+        if strings is not None:
+            chars, offsets, width = pack(strings, width)
+        assert chars is not None and offsets is not None
+        if width is None:
+            width = chars.dtype.itemsize
+        chars = np.ascontiguousarray(chars)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self.width = width
+        self.count = len(offsets) - 1
+        self.lengths = np.diff(offsets)
+        h = _vp()
+        _check(_lib.rfa_strings_create(ctx.h, chars.ctypes.data, width, offsets.ctypes.data, self.count,
+                                       ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h and self.ctx.h:
+            _check(_lib.rfa_strings_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scorer:
+    """Cached<metric>(q) for every q of `queries` (or Multi<metric> over them)."""
+
+    def __init__(self, ctx: Context, metric: int, queries: Strings, weights=(1, 1, 1), prefix_weight=0.1):
+        self.ctx = ctx
+        self.metric = metric
+        self.nq = queries.count
+        w = Weights(*weights)
+        h = _vp()
+        _check(_lib.rfa_scorer_create(ctx.h, metric, queries.h, ctypes.byref(w), prefix_weight, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h and self.ctx.h:
+            _check(_lib.rfa_scorer_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cdist(self, choices: Strings, kind=DISTANCE, cutoff_u=None, cutoff_d=None, semantics=CACHED,
+              out_device_ptr=None):
+        du, dd = default_cutoffs(self.metric, kind)
+        cu = du if cutoff_u is None else cutoff_u
+        cd = dd if cutoff_d is None else cutoff_d
+        if out_device_ptr is not None:
+            _check(_lib.rfa_cdist(self.h, choices.h, kind, semantics, cu, cd, out_device_ptr, 1))
+            return None
+        dt = np.uint64 if is_int_result(self.metric, kind) else np.float64
+        out = np.zeros((self.nq, choices.count), dtype=dt)
+        _check(_lib.rfa_cdist(self.h, choices.h, kind, semantics, cu, cd, out.ctypes.data, 0))
+        return out
+
+    def topk(self, choices: Strings, k: int, kind=SIMILARITY, cutoff_d=0.0):
+        idx = np.zeros((self.nq, k), dtype=np.int64)
+        sc = np.zeros((self.nq, k), dtype=np.float64)
+        _check(_lib.rfa_topk(self.h, choices.h, kind, cutoff_d, k, idx.ctypes.data, sc.ctypes.data))
+        return idx, sc
+
+
+def pairs(ctx: Context, a: Strings, b: Strings, metrics, kind=DISTANCE, cutoff_u=INT64_MAX, cutoff_d=1.0,
+          schedule=SCHEDULE_BUCKETED, out_device_ptr=None):
+    m = np.ascontiguousarray(np.asarray(metrics, dtype=np.int32))
+    if out_device_ptr is not None:
+        _check(_lib.rfa_pairs(ctx.h, a.h, b.h, m.ctypes.data, kind, cutoff_u, cutoff_d, schedule, out_device_ptr, 1))
+        return None
+    dt = np.uint64 if kind in (DISTANCE, SIMILARITY) else np.float64
+    out = np.zeros(a.count, dtype=dt)
+    _check(_lib.rfa_pairs(ctx.h, a.h, b.h, m.ctypes.data, kind, cutoff_u, cutoff_d, schedule, out.ctypes.data, 0))
+    return out
