@@ -159,7 +159,10 @@ def main():
     cus = hip.device_attribute(hip.ATTR_CU_COUNT)
     clk_khz = hip.device_attribute(hip.ATTR_CLOCK_RATE_KHZ)
     peak_tops = cus * 64 * clk_khz * 1e3 / 1e12
-    word_steps = float(NQ) * float(cl.sum())  # one 64-bit word per query (m <= 64) per text position
+    # 64-cell word-steps: a query of <= 64 units is one 64-bit word per text position;
+    # tiles whose queries fit in 32 units run 32-bit words, counted as half a word-step.
+    n64 = float(np.count_nonzero(ql > 32))
+    word_steps = (n64 + 0.5 * (NQ - n64)) * float(cl.sum())
     ops = OPS[0] * word_steps
     achieved_tops = ops / (ms_per_step / 1e3) / 1e12
     # algorithmic HBM bytes: the results, plus every query tile (64 queries) reading each
@@ -202,7 +205,8 @@ def main():
                 ms = e0.elapsed_ms(e1) / reps
                 scorers[f"{NAMES[metric]}/{mode}"] = {
                     "ms": round(ms, 3), "gcups": round(cells / (ms / 1e3) / 1e9, 1),
-                    "int_tops": round(OPS[metric] * word_steps / (ms / 1e3) / 1e12, 2)}
+                    "int_tops": round(OPS[metric] * word_steps / (ms / 1e3) / 1e12, 2),
+                    "int_frac": round(OPS[metric] * word_steps / (ms / 1e3) / 1e12 / peak_tops, 4)}
             if sc is not lev:
                 sc.close()
 

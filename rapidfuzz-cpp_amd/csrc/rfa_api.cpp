@@ -337,7 +337,7 @@ void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_
     hipStream_t s = ctx->stream;
     const uint32_t W = (uint32_t)((max_pattern + 63) / 64);
     const uint32_t slots = std::max<uint32_t>(1, (W + 63) / 64);
-    if (slots > 16) throw ApiError(RFA_ERR_UNSUPPORTED, "patterns longer than 65536 code units are not supported");
+    if (slots > 32) throw ApiError(RFA_ERR_UNSUPPORTED, "patterns longer than 131072 code units are not supported");
     uint64_t alpha = width == 1 ? 256 : (width == 2 ? 65536 : ~0ull);
     uint64_t max_rows = std::min<uint64_t>(max_pattern, alpha) + 1;
     uint32_t hash_bits = std::max<uint32_t>(6, log2_ceil(2 * max_rows));

@@ -612,8 +612,10 @@ static void launch2(const BlockArgs& a, uint32_t nblocks, bool lds, size_t lds_b
         launch3<ALGO, 4, CharT>(a, nblocks, lds, lds_bytes, s);
     else if (slots <= 8)
         launch3<ALGO, 8, CharT>(a, nblocks, lds, lds_bytes, s);
-    else
+    else if (slots <= 16)
         launch3<ALGO, 16, CharT>(a, nblocks, lds, lds_bytes, s);
+    else
+        launch3<ALGO, 32, CharT>(a, nblocks, lds, lds_bytes, s);
 }
 
 template <typename CharT>

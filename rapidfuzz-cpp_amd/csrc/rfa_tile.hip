@@ -30,46 +30,115 @@ __device__ __forceinline__ int popc(W x)
 }
 
 // ---- recurrences on one word (one query per lane) -----------------------
+//
+// The distance is not tracked per text position: after the last position the
+// vertical deltas of the final column give D[m][n] = n + popcount(VP & M) -
+// popcount(VN & M) with M the m low bits, so the loop body is the bare
+// recurrence.  gfx950's three-input bitop (v_bitop3_b32, truth table over
+// a=0xF0, b=0xCC, c=0xAA) folds the NOR/OR/XOR pairs, and 64-bit words are
+// kept as two 32-bit halves (v_alignbit_b32 carries the shifted-out bit).
 
-// Levenshtein_impl.hpp:282-309
-template <typename W>
-__device__ __forceinline__ void lev_step(W X, W& VP, W& VN, uint32_t& dist, W mask)
+constexpr uint32_t TA = 0xF0, TB = 0xCC, TC = 0xAA;
+constexpr uint32_t T_XOR_OR = ((TA ^ TB) | TC) & 0xFF;  // (a ^ b) | c
+constexpr uint32_t T_OR_NOR = (TA | ~(TB | TC)) & 0xFF; // a | ~(b | c)
+constexpr uint32_t T_OR_ANDN = (TA | (TB & ~TC)) & 0xFF; // a | (b & ~c)
+
+template <uint32_t IMM>
+__device__ __forceinline__ uint32_t bop3(uint32_t a, uint32_t b, uint32_t c)
 {
-    W D0 = (((X & VP) + VP) ^ VP) | X | VN;
-    W HP = VN | ~(D0 | VP);
-    W HN = D0 & VP;
-    dist += (HP & mask) != 0;
-    dist -= (HN & mask) != 0;
+    return __builtin_amdgcn_bitop3_b32(a, b, c, IMM);
+}
+
+// (hi:lo) << 1, high half
+__device__ __forceinline__ uint32_t shl1_hi(uint32_t hi, uint32_t lo) { return __builtin_amdgcn_alignbit(hi, lo, 31); }
+
+// Levenshtein_impl.hpp:282-309 (score handled at the end)
+__device__ __forceinline__ void lev_step(uint32_t X, uint32_t& VP, uint32_t& VN)
+{
+    const uint32_t XV = X | VN;
+    const uint32_t s = (X & VP) + VP;
+    const uint32_t D0 = bop3<T_XOR_OR>(s, VP, XV);
+    uint32_t HP = bop3<T_OR_NOR>(VN, D0, VP);
+    uint32_t HN = D0 & VP;
     HP = (HP << 1) | 1;
     HN = HN << 1;
-    VP = HN | ~(D0 | HP);
+    VP = bop3<T_OR_NOR>(HN, D0, HP);
     VN = HP & D0;
 }
 
-// LCSseq_impl.hpp:219-252 (one word, no carry in)
-template <typename W>
-__device__ __forceinline__ void lcs_step(W X, W& S)
+__device__ __forceinline__ void lev_step(uint64_t X, uint32_t& VPl, uint32_t& VPh, uint32_t& VNl, uint32_t& VNh)
 {
-    W u = S & X;
-    S = (S + u) | (S - u);
+    const uint32_t Xl = (uint32_t)X, Xh = (uint32_t)(X >> 32);
+    const uint32_t XVl = Xl | VNl, XVh = Xh | VNh;
+    const uint64_t t = ((uint64_t)(Xh & VPh) << 32) | (Xl & VPl);
+    const uint64_t s = t + (((uint64_t)VPh << 32) | VPl);
+    const uint32_t D0l = bop3<T_XOR_OR>((uint32_t)s, VPl, XVl);
+    const uint32_t D0h = bop3<T_XOR_OR>((uint32_t)(s >> 32), VPh, XVh);
+    const uint32_t HPl0 = bop3<T_OR_NOR>(VNl, D0l, VPl);
+    const uint32_t HPh0 = bop3<T_OR_NOR>(VNh, D0h, VPh);
+    const uint32_t HNl0 = D0l & VPl, HNh0 = D0h & VPh;
+    const uint32_t HPh = shl1_hi(HPh0, HPl0), HPl = (HPl0 << 1) | 1;
+    const uint32_t HNh = shl1_hi(HNh0, HNl0), HNl = HNl0 << 1;
+    VPl = bop3<T_OR_NOR>(HNl, D0l, HPl);
+    VPh = bop3<T_OR_NOR>(HNh, D0h, HPh);
+    VNl = HPl & D0l;
+    VNh = HPh & D0h;
 }
 
-// OSA_impl.hpp:48-71
-template <typename W>
-__device__ __forceinline__ void osa_step(W X, W& VP, W& VN, W& D0, W& PMold, uint32_t& dist, W mask)
+// LCSseq_impl.hpp:219-252 (one word): S = (S + u) | (S - u), S - u = S & ~X
+__device__ __forceinline__ void lcs_step(uint32_t X, uint32_t& S)
 {
-    W TR = (((~D0) & X) << 1) & PMold;
-    D0 = (((X & VP) + VP) ^ VP) | X | VN;
-    D0 = D0 | TR;
-    W HP = VN | ~(D0 | VP);
-    W HN = D0 & VP;
-    dist += (HP & mask) != 0;
-    dist -= (HN & mask) != 0;
+    const uint32_t u = S & X;
+    S = bop3<T_OR_ANDN>(S + u, S, X);
+}
+
+__device__ __forceinline__ void lcs_step(uint64_t X, uint32_t& Sl, uint32_t& Sh)
+{
+    const uint32_t Xl = (uint32_t)X, Xh = (uint32_t)(X >> 32);
+    const uint64_t S = ((uint64_t)Sh << 32) | Sl;
+    const uint64_t sum = S + (((uint64_t)(Sh & Xh) << 32) | (Sl & Xl));
+    Sl = bop3<T_OR_ANDN>((uint32_t)sum, Sl, Xl);
+    Sh = bop3<T_OR_ANDN>((uint32_t)(sum >> 32), Sh, Xh);
+}
+
+// OSA_impl.hpp:48-71 (score handled at the end)
+__device__ __forceinline__ void osa_step(uint32_t X, uint32_t& VP, uint32_t& VN, uint32_t& D0, uint32_t& PMold)
+{
+    const uint32_t TR = ((X & ~D0) << 1) & PMold;
+    const uint32_t XVT = X | VN | TR;
+    const uint32_t s = (X & VP) + VP;
+    D0 = bop3<T_XOR_OR>(s, VP, XVT);
+    uint32_t HP = bop3<T_OR_NOR>(VN, D0, VP);
+    uint32_t HN = D0 & VP;
     HP = (HP << 1) | 1;
     HN = HN << 1;
-    VP = HN | ~(D0 | HP);
+    VP = bop3<T_OR_NOR>(HN, D0, HP);
     VN = HP & D0;
     PMold = X;
+}
+
+__device__ __forceinline__ void osa_step(uint64_t X, uint32_t& VPl, uint32_t& VPh, uint32_t& VNl, uint32_t& VNh,
+                                         uint32_t& D0l, uint32_t& D0h, uint32_t& PMl, uint32_t& PMh)
+{
+    const uint32_t Xl = (uint32_t)X, Xh = (uint32_t)(X >> 32);
+    const uint32_t al = Xl & ~D0l, ah = Xh & ~D0h;
+    const uint32_t TRl = (al << 1) & PMl, TRh = shl1_hi(ah, al) & PMh;
+    const uint32_t XVTl = Xl | VNl | TRl, XVTh = Xh | VNh | TRh;
+    const uint64_t t = ((uint64_t)(Xh & VPh) << 32) | (Xl & VPl);
+    const uint64_t s = t + (((uint64_t)VPh << 32) | VPl);
+    D0l = bop3<T_XOR_OR>((uint32_t)s, VPl, XVTl);
+    D0h = bop3<T_XOR_OR>((uint32_t)(s >> 32), VPh, XVTh);
+    const uint32_t HPl0 = bop3<T_OR_NOR>(VNl, D0l, VPl);
+    const uint32_t HPh0 = bop3<T_OR_NOR>(VNh, D0h, VPh);
+    const uint32_t HNl0 = D0l & VPl, HNh0 = D0h & VPh;
+    const uint32_t HPh = shl1_hi(HPh0, HPl0), HPl = (HPl0 << 1) | 1;
+    const uint32_t HNh = shl1_hi(HNh0, HNl0), HNl = HNl0 << 1;
+    VPl = bop3<T_OR_NOR>(HNl, D0l, HPl);
+    VPh = bop3<T_OR_NOR>(HNh, D0h, HPh);
+    VNl = HPl & D0l;
+    VNh = HPh & D0h;
+    PMl = Xl;
+    PMh = Xh;
 }
 
 template <typename CharT>
@@ -173,38 +242,56 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
     SymLookup<CharT> look = build_tile_pm<CharT, W>(q, qo, m, pm, extra, a);
 
     constexpr uint32_t BITS = sizeof(W) * 8;
-    const W mask = m ? ((W)1 << (m - 1)) : (W)0;
     const uint32_t it0 = blockIdx.x * a.choices_per_block;
     const uint32_t it1 = min(a.ncsel, it0 + a.choices_per_block);
 
     for (uint32_t it = it0 + wave; it < it1; it += 4) {
-        const uint32_t cidx = a.corder ? a.corder[it] : it;
-        const uint64_t co = a.coff[cidx];
-        const uint32_t n = (uint32_t)(a.coff[cidx + 1] - co);
+        const uint32_t cidx = uniform(a.corder ? a.corder[it] : it);
+        const uint64_t co0 = a.coff[cidx];
+        const uint32_t n = uniform((uint32_t)(a.coff[cidx + 1] - co0));
+        const uint64_t co = ((uint64_t)uniform((uint32_t)(co0 >> 32)) << 32) | uniform((uint32_t)co0);
         RawResult res{0, 0, 0};
 
         if constexpr (ALGO == ALGO_LEV || ALGO == ALGO_OSA || ALGO == ALGO_LCS) {
-            W VP = ~(W)0, VN = 0, D0 = 0, PMold = 0, S = ~(W)0;
-            uint32_t dist = m;
+            // state as 32-bit halves (h* unused for 32-bit words)
+            uint32_t VPl = ~0u, VPh = ~0u, VNl = 0, VNh = 0, D0l = 0, D0h = 0, PMl = 0, PMh = 0;
+            const W* __restrict__ col = pm + lane;
             for (uint32_t k0 = 0; k0 < n; k0 += 64) {
                 uint32_t row = 0;
                 if (k0 + lane < n) row = look((uint32_t)c[co + k0 + lane]);
                 const uint32_t cnt = min(64u, n - k0);
+                W Xn = col[readlane(row, 0) * 64];
                 for (uint32_t i = 0; i < cnt; ++i) {
-                    const uint32_t r = readlane(row, i);
-                    const W X = pm[r * 64 + lane];
-                    if constexpr (ALGO == ALGO_LEV)
-                        lev_step<W>(X, VP, VN, dist, mask);
-                    else if constexpr (ALGO == ALGO_OSA)
-                        osa_step<W>(X, VP, VN, D0, PMold, dist, mask);
-                    else
-                        lcs_step<W>(X, S);
+                    const W X = Xn;
+                    Xn = col[readlane(row, min(i + 1, 63u)) * 64]; // prefetch the next row
+                    if constexpr (sizeof(W) == 4) {
+                        if constexpr (ALGO == ALGO_LEV)
+                            lev_step((uint32_t)X, VPl, VNl);
+                        else if constexpr (ALGO == ALGO_OSA)
+                            osa_step((uint32_t)X, VPl, VNl, D0l, PMl);
+                        else
+                            lcs_step((uint32_t)X, VPl);
+                    }
+                    else {
+                        if constexpr (ALGO == ALGO_LEV)
+                            lev_step((uint64_t)X, VPl, VPh, VNl, VNh);
+                        else if constexpr (ALGO == ALGO_OSA)
+                            osa_step((uint64_t)X, VPl, VPh, VNl, VNh, D0l, D0h, PMl, PMh);
+                        else
+                            lcs_step((uint64_t)X, VPl, VPh);
+                    }
                 }
             }
-            if constexpr (ALGO == ALGO_LCS)
-                res.v = (uint64_t)popc<W>(~S);
-            else
-                res.v = m ? dist : n;
+            const W VP = (sizeof(W) == 8) ? (W)(((uint64_t)VPh << 32) | VPl) : (W)VPl;
+            const W VN = (sizeof(W) == 8) ? (W)(((uint64_t)VNh << 32) | VNl) : (W)VNl;
+            if constexpr (ALGO == ALGO_LCS) {
+                res.v = (uint64_t)popc<W>(~VP); // S is kept in the VP registers
+            }
+            else {
+                // D[m][n] = D[0][n] + sum of the final column's vertical deltas
+                const W M = (m >= BITS) ? ~(W)0 : (((W)1 << m) - 1);
+                res.v = m ? (uint64_t)n + popc<W>(VP & M) - popc<W>(VN & M) : n;
+            }
         }
         else {
             // Jaro word path; the host routes only choices with n <= 64 here.
