@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
@@ -77,11 +78,21 @@ struct DevBuf {
     }
 };
 
+// Host staging of asynchronous uploads: the source vectors stay alive in
+// `keep` until the stream has passed them (see begin_call).
+struct Keep {
+    std::vector<std::shared_ptr<void>> items;
+};
+
 template <typename T>
-T* upload(DevBuf& b, const std::vector<T>& v, hipStream_t s)
+T* upload(Keep& keep, DevBuf& b, const std::vector<T>& v, hipStream_t s)
 {
     T* d = (T*)b.get(std::max<size_t>(v.size(), 1) * sizeof(T));
-    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    if (!v.empty()) {
+        auto copy = std::make_shared<std::vector<T>>(v);
+        HIP_CHECK(hipMemcpyAsync(d, copy->data(), copy->size() * sizeof(T), hipMemcpyHostToDevice, s));
+        keep.items.push_back(copy);
+    }
     return d;
 }
 
@@ -98,12 +109,29 @@ uint32_t log2_ceil(uint64_t x)
 
 struct rfa_context {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // all results are ordered on this stream
+    hipStream_t stream2 = nullptr; // second lane for work that overlaps (pairs buckets)
+    hipEvent_t fork = nullptr, join = nullptr;
     std::mutex mu;
     int num_cus = 256;
     size_t lds_per_cu = 160 * 1024;
     DevBuf queue, pm_scratch, tflag, items_a, items_b, items_idx, items_swap, out_tmp, misc;
+    DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4];
+    size_t lane_arena_bytes_zeroed = 0;
+    uint32_t gen_next = 1;
+    Keep keep; // host sources of in-flight uploads
 };
+
+// Called at the start of every scoring call: releases the host sources of the
+// previous call's uploads once the device has consumed them.
+static void begin_call(rfa_context* ctx)
+{
+    if (!ctx->keep.items.empty()) {
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream2));
+        ctx->keep.items.clear();
+    }
+}
 
 struct rfa_strings {
     rfa_context* ctx = nullptr;
@@ -298,6 +326,30 @@ void free_strings(rfa_strings* s)
     delete s;
 }
 
+ScoreSpec make_spec_m(int metric, const rfa_weights& w, double pw, int kind, int semantics, uint64_t cu, double cd,
+                      int algo)
+{
+    ScoreSpec s{};
+    s.metric = metric;
+    s.kind = kind;
+    s.semantics = semantics;
+    s.algo = algo;
+    s.cutoff_u = cu;
+    s.cutoff_d = cd;
+    s.ins = w.insert_cost;
+    s.del = w.delete_cost;
+    s.rep = w.replace_cost;
+    s.prefix_weight = pw;
+    s.band_k = ~0ull;
+    if (kind == RFA_DISTANCE && is_int_kind(metric, kind) && metric != RFA_LCSSEQ) {
+        if (metric == RFA_LEVENSHTEIN)
+            s.band_k = s.ins ? ceil_div_u64(cu, s.ins) : ~0ull;
+        else
+            s.band_k = cu;
+    }
+    return s;
+}
+
 ScoreSpec make_spec(const rfa_scorer* sc, int kind, int semantics, uint64_t cu, double cd, int algo)
 {
     ScoreSpec s{};
@@ -400,7 +452,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ta.nq = (uint32_t)sc->tile_order.size();
         ta.cchars = ch->d_chars;
         ta.coff = ch->d_off;
-        ta.corder = all_choices ? nullptr : upload(ctx->misc, tile_choices, s);
+        ta.corder = all_choices ? nullptr : upload(ctx->keep, ctx->misc, tile_choices, s);
         ta.ncsel = all_choices ? (uint32_t)nc : (uint32_t)tile_choices.size();
         ta.out_stride = nc;
         ta.hash_bits = sc->hash_bits;
@@ -431,7 +483,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ba.item_a = sc->d_block_queries;
         std::vector<uint32_t> all(nc);
         std::iota(all.begin(), all.end(), 0u);
-        ba.item_b = upload(ctx->items_b, all, s);
+        ba.item_b = upload(ctx->keep, ctx->items_b, all, s);
         ba.cross_nb = (uint32_t)nc;
         ba.out_stride = nc;
         uint64_t ni = (uint64_t)sc->block_queries.size() * nc;
@@ -449,7 +501,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ba.bchars = ch->d_chars;
         ba.boff = ch->d_off;
         ba.item_a = sc->d_tile_order;
-        ba.item_b = upload(ctx->items_a, jaro_long, s);
+        ba.item_b = upload(ctx->keep, ctx->items_a, jaro_long, s);
         ba.cross_nb = (uint32_t)jaro_long.size();
         ba.out_stride = nc;
         uint64_t ni = (uint64_t)sc->tile_order.size() * jaro_long.size();
@@ -459,6 +511,168 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ba.out = dout;
         run_block(ctx, algo, width, ba, 64, ch->max_len, true);
     }
+    if (!out_on_device) {
+        HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+
+// ---- independent pairs (the uneven batch) ---------------------------------
+
+uint64_t* lane_arena(rfa_context* ctx, uint32_t maxw, uint32_t nlanes, uint64_t nitems, uint32_t* gen_base)
+{
+    const size_t words = lane_arena_words(maxw);
+    const size_t bytes = words * 8 * (size_t)nlanes;
+    uint64_t* arena = (uint64_t*)ctx->lane_arena.get(bytes);
+    // generation tags must never repeat against stale slots: zero fresh memory,
+    // and restart the counter (with a fresh zeroing) before it would wrap.
+    const uint64_t per_lane = (nitems + nlanes - 1) / nlanes + 1;
+    if (ctx->lane_arena_bytes_zeroed < bytes || (uint64_t)ctx->gen_next + per_lane >= 0xffffff00ull) {
+        HIP_CHECK(hipMemsetAsync(arena, 0, ctx->lane_arena.cap, ctx->stream2));
+        ctx->lane_arena_bytes_zeroed = ctx->lane_arena.cap;
+        ctx->gen_next = 1;
+    }
+    *gen_base = ctx->gen_next;
+    ctx->gen_next += (uint32_t)per_lane;
+    return arena;
+}
+
+void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics, int kind,
+                uint64_t cu, double cd, int schedule, void* out, int out_on_device)
+{
+    if (a->count != b->count) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "a and b must hold the same number of strings");
+    if (a->width != b->width) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "a and b must have the same char width");
+    if (kind < RFA_DISTANCE || kind > RFA_NORMALIZED_SIMILARITY) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown kind");
+    if (schedule != RFA_SCHEDULE_BUCKETED && schedule != RFA_SCHEDULE_PLAIN)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown schedule");
+    const uint64_t n = a->count;
+    if (n == 0) return;
+    if (!metrics) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "metrics is NULL");
+    for (uint64_t i = 0; i < n; ++i)
+        if (metrics[i] < RFA_LEVENSHTEIN || metrics[i] > RFA_OSA)
+            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "pair metrics must be Levenshtein, Indel, LCSseq or OSA");
+    if (n > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 pairs");
+    hipStream_t s = ctx->stream, s2 = ctx->stream2;
+    const size_t bytes = (size_t)n * 8;
+    void* dout = out_on_device ? out : ctx->out_tmp.get(bytes);
+    const int width = a->width;
+    const rfa_weights unit{1, 1, 1};
+    ScoreSpec base = make_spec_m(RFA_LEVENSHTEIN, unit, 0.1, kind, RFA_CACHED, cu, cd, ALGO_LEV);
+
+    LaneArgs la{};
+    la.achars = a->d_chars;
+    la.aoff = a->d_off;
+    la.bchars = b->d_chars;
+    la.boff = b->d_off;
+    la.spec = base;
+    la.out = dout;
+
+    HIP_CHECK(hipEventRecord(ctx->fork, s));
+    HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
+    if (schedule == RFA_SCHEDULE_PLAIN) {
+        // one pair per lane, input order, all words in the lane
+        uint64_t maxm = 0;
+        for (uint64_t i = 0; i < n; ++i) maxm = std::max(maxm, a->len(i));
+        if (maxm > 4096) throw ApiError(RFA_ERR_UNSUPPORTED, "the plain schedule handles patterns up to 4096 units");
+        std::vector<int32_t> met(metrics, metrics + n);
+        la.item_metric = upload(ctx->keep, ctx->lane_metric, met, s2);
+        la.nitems = n;
+        const uint32_t nlanes = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 4 * 64,
+                                                             ((n + 255) / 256) * 256);
+        la.arena_words = lane_arena_words(64);
+        la.arena = lane_arena(ctx, 64, nlanes, n, &la.gen_base);
+        HIP_CHECK(launch_lane(width, 64, la, nlanes / 256, s2));
+    }
+    else {
+        // bucketed: pattern = the shorter string; <= 64 units -> lane kernel, longer -> block kernel
+        std::vector<uint32_t> short_items, long_items[4];
+        std::vector<uint8_t> swap(n);
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t l1 = a->len(i), l2 = b->len(i);
+            swap[i] = l2 < l1;
+            const uint64_t m = std::min(l1, l2);
+            if (m <= 64)
+                short_items.push_back((uint32_t)i);
+            else
+                long_items[metrics[i]].push_back((uint32_t)i);
+        }
+        // long pairs first (they are the tail), longest-processing-time first
+        for (int met = 0; met < 4; ++met) {
+            auto& li = long_items[met];
+            if (li.empty()) continue;
+            auto cost = [&](uint32_t i) {
+                uint64_t l1 = a->len(i), l2 = b->len(i);
+                uint64_t m = std::min(l1, l2), nn = std::max(l1, l2);
+                return ((m + 63) / 64) * nn;
+            };
+            std::stable_sort(li.begin(), li.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
+            uint64_t maxm = 0, maxn = 0;
+            for (uint32_t i : li) {
+                maxm = std::max(maxm, std::min(a->len(i), b->len(i)));
+                maxn = std::max(maxn, std::max(a->len(i), b->len(i)));
+            }
+            // items: (index, index, swap) packed into the per-metric buffer
+            std::vector<uint32_t> packed(li.size() * 2);
+            std::vector<uint64_t> oidx(li.size());
+            std::vector<uint8_t> sw(li.size());
+            for (size_t k = 0; k < li.size(); ++k) {
+                packed[k] = li[k];
+                packed[li.size() + k] = li[k];
+                oidx[k] = li[k];
+                sw[k] = swap[li[k]];
+            }
+            DevBuf& buf = ctx->block_items[met];
+            const size_t need = packed.size() * 4 + oidx.size() * 8 + sw.size() + 64;
+            uint8_t* base_ptr = (uint8_t*)buf.get(need);
+            auto staged = std::make_shared<std::vector<uint8_t>>(need);
+            std::memcpy(staged->data(), oidx.data(), oidx.size() * 8);
+            std::memcpy(staged->data() + oidx.size() * 8, packed.data(), packed.size() * 4);
+            std::memcpy(staged->data() + oidx.size() * 8 + packed.size() * 4, sw.data(), sw.size());
+            HIP_CHECK(hipMemcpyAsync(base_ptr, staged->data(), need, hipMemcpyHostToDevice, s));
+            ctx->keep.items.push_back(staged);
+            BlockArgs ba{};
+            ba.achars = a->d_chars;
+            ba.aoff = a->d_off;
+            ba.bchars = b->d_chars;
+            ba.boff = b->d_off;
+            ba.out_index = (const uint64_t*)base_ptr;
+            ba.item_a = (const uint32_t*)(base_ptr + oidx.size() * 8);
+            ba.item_b = ba.item_a + li.size();
+            ba.item_swap = base_ptr + oidx.size() * 8 + packed.size() * 4;
+            ba.nitems = (uint32_t)li.size();
+            ba.spec = make_spec_m(met, unit, 0.1, kind, RFA_CACHED, cu, cd, algo_for(met, unit));
+            ba.out = dout;
+            run_block(ctx, algo_for(met, unit), width, ba, maxm, maxn, false);
+        }
+        if (!short_items.empty()) {
+            // group similar text lengths into the same wave
+            std::stable_sort(short_items.begin(), short_items.end(), [&](uint32_t x, uint32_t y) {
+                return std::max(a->len(x), b->len(x)) > std::max(a->len(y), b->len(y));
+            });
+            std::vector<uint8_t> sw(short_items.size());
+            std::vector<int32_t> met(short_items.size());
+            std::vector<uint64_t> oidx(short_items.size());
+            for (size_t k = 0; k < short_items.size(); ++k) {
+                sw[k] = swap[short_items[k]];
+                met[k] = metrics[short_items[k]];
+                oidx[k] = short_items[k];
+            }
+            la.item_a = upload(ctx->keep, ctx->lane_items, short_items, s2);
+            la.item_b = la.item_a;
+            la.item_swap = upload(ctx->keep, ctx->lane_swap, sw, s2);
+            la.item_metric = upload(ctx->keep, ctx->lane_metric, met, s2);
+            la.out_index = upload(ctx->keep, ctx->lane_out, oidx, s2);
+            la.nitems = short_items.size();
+            const uint32_t nlanes = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8 * 64,
+                                                                 ((short_items.size() + 255) / 256) * 256);
+            la.arena_words = lane_arena_words(1);
+            la.arena = lane_arena(ctx, 1, nlanes, short_items.size(), &la.gen_base);
+            HIP_CHECK(launch_lane(width, 1, la, nlanes / 256, s2));
+        }
+    }
+    HIP_CHECK(hipEventRecord(ctx->join, s2));
+    HIP_CHECK(hipStreamWaitEvent(s, ctx->join, 0));
     if (!out_on_device) {
         HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
@@ -512,6 +726,9 @@ rfa_status rfa_context_create(int device, rfa_context** out)
         c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         if (prop.maxSharedMemoryPerMultiProcessor > 0) c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
         HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -522,9 +739,15 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
     return guarded([&] {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream2);
         for (DevBuf* b : {&ctx->queue, &ctx->pm_scratch, &ctx->tflag, &ctx->items_a, &ctx->items_b, &ctx->items_idx,
-                          &ctx->items_swap, &ctx->out_tmp, &ctx->misc})
+                          &ctx->items_swap, &ctx->out_tmp, &ctx->misc, &ctx->lane_arena, &ctx->lane_items,
+                          &ctx->lane_swap, &ctx->lane_metric, &ctx->lane_out, &ctx->block_items[0],
+                          &ctx->block_items[1], &ctx->block_items[2], &ctx->block_items[3]})
             b->release();
+        (void)hipEventDestroy(ctx->fork);
+        (void)hipEventDestroy(ctx->join);
+        (void)hipStreamDestroy(ctx->stream2);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -617,6 +840,7 @@ rfa_status rfa_cdist(rfa_scorer* sc, const rfa_strings* choices, int kind, int s
     return guarded([&] {
         std::lock_guard<std::mutex> lk(sc->ctx->mu);
         HIP_CHECK(hipSetDevice(sc->ctx->device));
+        begin_call(sc->ctx);
         cdist_impl(sc, choices, kind, semantics, cutoff_u, cutoff_d, out, out_on_device);
     });
 }
@@ -624,9 +848,14 @@ rfa_status rfa_cdist(rfa_scorer* sc, const rfa_strings* choices, int kind, int s
 rfa_status rfa_pairs(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics, int kind,
                      uint64_t cutoff_u, double cutoff_d, int schedule, void* out, int out_on_device)
 {
-    (void)ctx, (void)a, (void)b, (void)metrics, (void)kind, (void)cutoff_u, (void)cutoff_d, (void)schedule, (void)out,
-        (void)out_on_device;
-    return fail(RFA_ERR_UNSUPPORTED, "rfa_pairs: not built yet");
+    if (!ctx || !a || !b) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, a or b is NULL");
+    if (!out && a->count) return fail(RFA_ERR_INVALID_ARGUMENT, "out is NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_CHECK(hipSetDevice(ctx->device));
+        begin_call(ctx);
+        pairs_impl(ctx, a, b, metrics, kind, cutoff_u, cutoff_d, schedule, out, out_on_device);
+    });
 }
 
 rfa_status rfa_topk(rfa_scorer* sc, const rfa_strings* choices, int kind, double cutoff_d, uint32_t k,

@@ -60,4 +60,26 @@ size_t block_arena_bytes(int char_width, uint32_t max_m, uint32_t max_rows, uint
 hipError_t launch_block(int algo, int char_width, const BlockArgs& a, uint32_t nblocks, bool lds, size_t lds_bytes,
                         uint32_t slots, hipStream_t s);
 
+// Lane kernel (rfa_lane.hip): one pair per lane, per-lane 128-slot maps per 64-unit block.
+struct LaneArgs {
+    const void* achars;
+    const uint64_t* aoff;
+    const void* bchars;
+    const uint64_t* boff;
+    const uint32_t* item_a;      // nullptr: item index
+    const uint32_t* item_b;      // nullptr: item index
+    const uint8_t* item_swap;    // nullptr: pattern is always A's string
+    const int32_t* item_metric;  // nullptr: spec.metric for every item
+    const uint64_t* out_index;   // nullptr: out[item]
+    uint64_t nitems;
+    uint64_t* arena;             // per-lane maps
+    uint64_t arena_words;        // uint64 words per lane
+    uint32_t gen_base;           // generation tags of this launch start above gen_base
+    ScoreSpec spec;
+    void* out;
+};
+
+size_t lane_arena_words(uint32_t maxw);
+hipError_t launch_lane(int char_width, int maxw, const LaneArgs& a, uint32_t nblocks, hipStream_t s);
+
 } // namespace rfa
