@@ -116,7 +116,7 @@ struct rfa_context {
     int num_cus = 256;
     size_t lds_per_cu = 160 * 1024;
     DevBuf queue, pm_scratch, tflag, items_a, items_b, items_idx, items_swap, out_tmp, misc;
-    DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4];
+    DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4], plan_keys, plan_vals, plan_temp;
     size_t lane_arena_bytes_zeroed = 0;
     uint32_t gen_next = 1;
     Keep keep; // host sources of in-flight uploads
@@ -383,34 +383,38 @@ void validate_kind(int metric, int kind, int semantics)
 
 // Block-kernel launch over a cross product (queries x choices) or explicit items.
 void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_pattern, uint64_t max_text,
-               bool reuse)
+               bool reuse, uint32_t* queue = nullptr)
 {
     if (a.nitems == 0) return;
     hipStream_t s = ctx->stream;
     const uint32_t W = (uint32_t)((max_pattern + 63) / 64);
     const uint32_t slots = std::max<uint32_t>(1, (W + 63) / 64);
     if (slots > 32) throw ApiError(RFA_ERR_UNSUPPORTED, "patterns longer than 131072 code units are not supported");
-    uint64_t alpha = width == 1 ? 256 : (width == 2 ? 65536 : ~0ull);
-    uint64_t max_rows = std::min<uint64_t>(max_pattern, alpha) + 1;
-    uint32_t hash_bits = std::max<uint32_t>(6, log2_ceil(2 * max_rows));
-    size_t arena = block_arena_bytes(width, (uint32_t)max_pattern, (uint32_t)max_rows, hash_bits);
-    a.hash_bits = hash_bits;
+    // patterns whose map + PM rows fit in LDS_BUDGET run from LDS; larger ones from a
+    // global slice of the workgroup (sized for the largest pattern)
+    constexpr uint64_t LDS_BUDGET = 40 * 1024;
+    const uint64_t need_max = block_arena_bytes(width, (uint32_t)max_pattern);
+    a.lds_bytes = (uint32_t)std::min<uint64_t>(need_max, LDS_BUDGET);
     a.reuse_pattern = reuse ? 1u : 0u;
-    bool lds = arena <= 64 * 1024;
-    uint32_t waves_per_cu = lds ? (uint32_t)std::min<size_t>(16, std::max<size_t>(1, ctx->lds_per_cu / (arena + 512)))
-                                : 8u;
-    uint32_t nblocks = (uint32_t)std::min<uint64_t>(a.nitems, (uint64_t)ctx->num_cus * waves_per_cu);
-    if (!lds) {
-        a.pm_scratch_words = (arena + 7) / 8;
+    const uint32_t waves_per_cu =
+        (uint32_t)std::min<uint64_t>(16, std::max<uint64_t>(1, ctx->lds_per_cu / (a.lds_bytes + 512)));
+    const uint32_t nblocks = (uint32_t)std::min<uint64_t>(a.nitems, (uint64_t)ctx->num_cus * waves_per_cu);
+    if (need_max > LDS_BUDGET) {
+        a.pm_scratch_words = (need_max + 7) / 8;
         a.pm_scratch = (uint64_t*)ctx->pm_scratch.get(a.pm_scratch_words * 8 * nblocks);
     }
     if (algo == ALGO_JARO) {
         a.tflag_words = (max_text + 63) / 64 + 1;
         a.tflag_scratch = (uint64_t*)ctx->tflag.get(a.tflag_words * 8 * nblocks);
     }
-    a.queue = (uint32_t*)ctx->queue.get(64);
-    HIP_CHECK(hipMemsetAsync(a.queue, 0, 4, s));
-    HIP_CHECK(launch_block(algo, width, a, nblocks, lds, lds ? arena : 0, slots, s));
+    if (queue) {
+        a.queue = queue; // reset by the planning kernels on this stream
+    }
+    else {
+        a.queue = (uint32_t*)ctx->queue.get(64);
+        HIP_CHECK(hipMemsetAsync(a.queue, 0, 4, s));
+    }
+    HIP_CHECK(launch_block(algo, width, a, nblocks, slots, s));
 }
 
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
@@ -568,9 +572,9 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
     la.spec = base;
     la.out = dout;
 
-    HIP_CHECK(hipEventRecord(ctx->fork, s));
-    HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
     if (schedule == RFA_SCHEDULE_PLAIN) {
+        HIP_CHECK(hipEventRecord(ctx->fork, s));
+        HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
         // one pair per lane, input order, all words in the lane
         uint64_t maxm = 0;
         for (uint64_t i = 0; i < n; ++i) maxm = std::max(maxm, a->len(i));
@@ -585,91 +589,48 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         HIP_CHECK(launch_lane(width, 64, la, nlanes / 256, s2));
     }
     else {
-        // bucketed: pattern = the shorter string; <= 64 units -> lane kernel, longer -> block kernel
-        std::vector<uint32_t> short_items, long_items[4];
-        std::vector<uint8_t> swap(n);
-        for (uint64_t i = 0; i < n; ++i) {
-            const uint64_t l1 = a->len(i), l2 = b->len(i);
-            swap[i] = l2 < l1;
-            const uint64_t m = std::min(l1, l2);
-            if (m <= 64)
-                short_items.push_back((uint32_t)i);
-            else
-                long_items[metrics[i]].push_back((uint32_t)i);
-        }
-        // long pairs first (they are the tail), longest-processing-time first
-        for (int met = 0; met < 4; ++met) {
-            auto& li = long_items[met];
-            if (li.empty()) continue;
-            auto cost = [&](uint32_t i) {
-                uint64_t l1 = a->len(i), l2 = b->len(i);
-                uint64_t m = std::min(l1, l2), nn = std::max(l1, l2);
-                return ((m + 63) / 64) * nn;
-            };
-            std::stable_sort(li.begin(), li.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
-            uint64_t maxm = 0, maxn = 0;
-            for (uint32_t i : li) {
-                maxm = std::max(maxm, std::min(a->len(i), b->len(i)));
-                maxn = std::max(maxn, std::max(a->len(i), b->len(i)));
-            }
-            // items: (index, index, swap) packed into the per-metric buffer
-            std::vector<uint32_t> packed(li.size() * 2);
-            std::vector<uint64_t> oidx(li.size());
-            std::vector<uint8_t> sw(li.size());
-            for (size_t k = 0; k < li.size(); ++k) {
-                packed[k] = li[k];
-                packed[li.size() + k] = li[k];
-                oidx[k] = li[k];
-                sw[k] = swap[li[k]];
-            }
-            DevBuf& buf = ctx->block_items[met];
-            const size_t need = packed.size() * 4 + oidx.size() * 8 + sw.size() + 64;
-            uint8_t* base_ptr = (uint8_t*)buf.get(need);
-            auto staged = std::make_shared<std::vector<uint8_t>>(need);
-            std::memcpy(staged->data(), oidx.data(), oidx.size() * 8);
-            std::memcpy(staged->data() + oidx.size() * 8, packed.data(), packed.size() * 4);
-            std::memcpy(staged->data() + oidx.size() * 8 + packed.size() * 4, sw.data(), sw.size());
-            HIP_CHECK(hipMemcpyAsync(base_ptr, staged->data(), need, hipMemcpyHostToDevice, s));
-            ctx->keep.items.push_back(staged);
+        // bucketed, planned on the device: pattern = the shorter string; <= 64 units -> lane
+        // kernel, longer -> block kernel (longest-processing-time-first queue)
+        std::vector<int32_t> met(metrics, metrics + n);
+        const int32_t* dmet = upload(ctx->keep, ctx->lane_metric, met, s);
+        uint64_t* keys = (uint64_t*)ctx->plan_keys.get(n * 16);
+        uint32_t* vals = (uint32_t*)ctx->plan_vals.get(n * 8);
+        const size_t temp_bytes = plan_temp_bytes(n);
+        void* temp = ctx->plan_temp.get(temp_bytes);
+        uint32_t* counters = (uint32_t*)ctx->queue.get(64); // [0] queue, [1] end of long, [2] end of short-64
+        HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, keys, keys + n, vals, vals + n, temp, temp_bytes, counters,
+                              s));
+        const uint32_t* packed = vals + n;
+        const uint64_t maxm = std::min(a->max_len, b->max_len);
+        const uint64_t maxn = std::max(a->max_len, b->max_len);
+        // the short-pattern kernels (second stream) overlap the long-pattern queue (first stream)
+        HIP_CHECK(hipEventRecord(ctx->fork, s));
+        HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
+        if (maxm > 64) {
             BlockArgs ba{};
             ba.achars = a->d_chars;
             ba.aoff = a->d_off;
             ba.bchars = b->d_chars;
             ba.boff = b->d_off;
-            ba.out_index = (const uint64_t*)base_ptr;
-            ba.item_a = (const uint32_t*)(base_ptr + oidx.size() * 8);
-            ba.item_b = ba.item_a + li.size();
-            ba.item_swap = base_ptr + oidx.size() * 8 + packed.size() * 4;
-            ba.nitems = (uint32_t)li.size();
-            ba.spec = make_spec_m(met, unit, 0.1, kind, RFA_CACHED, cu, cd, algo_for(met, unit));
+            ba.item_packed = packed;
+            ba.nitems_dev = counters + 1;
+            ba.pair_metric = dmet;
+            ba.nitems = (uint32_t)n; // upper bound, for the grid size
+            ba.spec = base;
             ba.out = dout;
-            run_block(ctx, algo_for(met, unit), width, ba, maxm, maxn, false);
+            run_block(ctx, ALGO_MIXED, width, ba, maxm, maxn, false, counters);
         }
-        if (!short_items.empty()) {
-            // group similar text lengths into the same wave
-            std::stable_sort(short_items.begin(), short_items.end(), [&](uint32_t x, uint32_t y) {
-                return std::max(a->len(x), b->len(x)) > std::max(a->len(y), b->len(y));
-            });
-            std::vector<uint8_t> sw(short_items.size());
-            std::vector<int32_t> met(short_items.size());
-            std::vector<uint64_t> oidx(short_items.size());
-            for (size_t k = 0; k < short_items.size(); ++k) {
-                sw[k] = swap[short_items[k]];
-                met[k] = metrics[short_items[k]];
-                oidx[k] = short_items[k];
-            }
-            la.item_a = upload(ctx->keep, ctx->lane_items, short_items, s2);
-            la.item_b = la.item_a;
-            la.item_swap = upload(ctx->keep, ctx->lane_swap, sw, s2);
-            la.item_metric = upload(ctx->keep, ctx->lane_metric, met, s2);
-            la.out_index = upload(ctx->keep, ctx->lane_out, oidx, s2);
-            la.nitems = short_items.size();
-            const uint32_t nlanes = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8 * 64,
-                                                                 ((short_items.size() + 255) / 256) * 256);
-            la.arena_words = lane_arena_words(1);
-            la.arena = lane_arena(ctx, 1, nlanes, short_items.size(), &la.gen_base);
-            HIP_CHECK(launch_lane(width, 1, la, nlanes / 256, s2));
-        }
+        la.item_packed = packed;
+        la.pair_metric = dmet;
+        la.nitems = n;
+        // short-64 range [end_long, end_s64) then short-32 range [end_s64, n)
+        const uint32_t nblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 6, (n + 63) / 64);
+        la.begin_dev = counters + 1;
+        la.end_dev = counters + 2;
+        HIP_CHECK(launch_lane_short(width, 64, la, nblk, s2));
+        la.begin_dev = counters + 2;
+        la.end_dev = nullptr;
+        HIP_CHECK(launch_lane_short(width, 32, la, nblk, s2));
     }
     HIP_CHECK(hipEventRecord(ctx->join, s2));
     HIP_CHECK(hipStreamWaitEvent(s, ctx->join, 0));
@@ -743,7 +704,8 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
         for (DevBuf* b : {&ctx->queue, &ctx->pm_scratch, &ctx->tflag, &ctx->items_a, &ctx->items_b, &ctx->items_idx,
                           &ctx->items_swap, &ctx->out_tmp, &ctx->misc, &ctx->lane_arena, &ctx->lane_items,
                           &ctx->lane_swap, &ctx->lane_metric, &ctx->lane_out, &ctx->block_items[0],
-                          &ctx->block_items[1], &ctx->block_items[2], &ctx->block_items[3]})
+                          &ctx->block_items[1], &ctx->block_items[2], &ctx->block_items[3], &ctx->plan_keys,
+                          &ctx->plan_vals, &ctx->plan_temp})
             b->release();
         (void)hipEventDestroy(ctx->fork);
         (void)hipEventDestroy(ctx->join);

@@ -1,4 +1,4 @@
-// rfa_block.hip -- block kernel: one wave per (pattern, text) pair, the
+// rfa_block_impl.hpp -- block kernel: one wave per (pattern, text) pair, the
 // pattern's 64-bit words spread across the 64 lanes (word w in lane w % 64,
 // register slot w / 64).
 //
@@ -26,6 +26,7 @@
 // a per-workgroup global arena.  Work items are pulled from a device queue by
 // persistent single-wave workgroups (items are sorted longest first by the
 // host, so the queue is a longest-processing-time-first schedule).
+#pragma once
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
 #include "rfa_semantics.hpp"
@@ -100,7 +101,7 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total)
 template <typename CharT, bool LDS>
 struct Pattern {
     ArenaMap<CharT, LDS> map;
-    uint64_t* pm; // [R][Wp]
+    uint64_t* pm; // [R][Wp], Wp = W (reads of lanes past the last word are masked)
     uint32_t R, W, Wp, slots;
 };
 
@@ -114,7 +115,7 @@ __device__ Pattern<CharT, LDS> build_pattern(const CharT* __restrict__ p, uint32
     Pattern<CharT, LDS> P;
     P.W = (m + 63) / 64;
     P.slots = (P.W + 63) / 64;
-    P.Wp = P.slots * 64;
+    P.Wp = P.W;
     uint64_t* pm;
     if constexpr (sizeof(CharT) == 1) {
         uint32_t* rowmap = (uint32_t*)arena;
@@ -205,16 +206,27 @@ __device__ uint64_t run_lev(const Pattern<CharT, LDS>& P, uint32_t m, const Char
         uint32_t row = 0;
         if (k0 + lane < n) row = P.map.find((uint32_t)t[k0 + lane]);
         const uint32_t cnt = min(64u, n - k0);
+        uint64_t Xn[MAXS];
+        {
+            const uint64_t* pmrow = P.pm + (size_t)readlane(row, 0) * P.Wp;
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) Xn[s] = ((uint32_t)s < P.slots && s * 64 + lane < P.W) ? pmrow[s * 64 + lane] : 0;
+        }
         for (uint32_t i = 0; i < cnt; ++i) {
-            const uint32_t r = readlane(row, i);
-            const uint64_t* pmrow = P.pm + (size_t)r * P.Wp;
+            uint64_t Xc[MAXS];
+            const uint64_t* pmnext = P.pm + (size_t)readlane(row, min(i + 1, 63u)) * P.Wp; // prefetch the next row
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) {
+                Xc[s] = Xn[s];
+                Xn[s] = ((uint32_t)s < P.slots && s * 64 + lane < P.W) ? pmnext[s * 64 + lane] : 0;
+            }
             uint64_t hn_c0 = 0, hp_c0 = 1;
 #pragma unroll
             for (int s = 0; s < MAXS; ++s) {
                 if ((uint32_t)s < P.slots) {
                     const uint32_t w = s * 64 + lane;
                     const bool act = w < P.W;
-                    const uint64_t X = pmrow[w];
+                    const uint64_t X = Xc[s];
                     const uint64_t vp = VP[s], vn = VN[s];
                     const uint64_t X1 = X | 1;
                     const uint64_t d0a = (((X & vp) + vp) ^ vp) | X | vn;
@@ -259,9 +271,20 @@ __device__ uint64_t run_lcs(const Pattern<CharT, LDS>& P, const CharT* __restric
         uint32_t row = 0;
         if (k0 + lane < n) row = P.map.find((uint32_t)t[k0 + lane]);
         const uint32_t cnt = min(64u, n - k0);
+        uint64_t Xn[MAXS];
+        {
+            const uint64_t* pmrow = P.pm + (size_t)readlane(row, 0) * P.Wp;
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) Xn[s] = ((uint32_t)s < P.slots && s * 64 + lane < P.W) ? pmrow[s * 64 + lane] : 0;
+        }
         for (uint32_t i = 0; i < cnt; ++i) {
-            const uint32_t r = readlane(row, i);
-            const uint64_t* pmrow = P.pm + (size_t)r * P.Wp;
+            uint64_t Xc[MAXS];
+            const uint64_t* pmnext = P.pm + (size_t)readlane(row, min(i + 1, 63u)) * P.Wp; // prefetch the next row
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) {
+                Xc[s] = Xn[s];
+                Xn[s] = ((uint32_t)s < P.slots && s * 64 + lane < P.W) ? pmnext[s * 64 + lane] : 0;
+            }
             uint64_t c0 = 0;
 #pragma unroll
             for (int s = 0; s < MAXS; ++s) {
@@ -269,7 +292,7 @@ __device__ uint64_t run_lcs(const Pattern<CharT, LDS>& P, const CharT* __restric
                     const uint32_t w = s * 64 + lane;
                     const bool act = w < P.W;
                     const uint64_t Sv = S[s];
-                    const uint64_t u = Sv & pmrow[w];
+                    const uint64_t u = Sv & Xc[s];
                     uint64_t x = Sv + u;
                     const bool g = act && (x < Sv);
                     const bool p = act && !g && (x == ~0ull);
@@ -311,16 +334,27 @@ __device__ uint64_t run_osa(const Pattern<CharT, LDS>& P, uint32_t m, const Char
         uint32_t row = 0;
         if (k0 + lane < n) row = P.map.find((uint32_t)t[k0 + lane]);
         const uint32_t cnt = min(64u, n - k0);
+        uint64_t Xn[MAXS];
+        {
+            const uint64_t* pmrow = P.pm + (size_t)readlane(row, 0) * P.Wp;
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) Xn[s] = ((uint32_t)s < P.slots && s * 64 + lane < P.W) ? pmrow[s * 64 + lane] : 0;
+        }
         for (uint32_t i = 0; i < cnt; ++i) {
-            const uint32_t r = readlane(row, i);
-            const uint64_t* pmrow = P.pm + (size_t)r * P.Wp;
+            uint64_t Xc[MAXS];
+            const uint64_t* pmnext = P.pm + (size_t)readlane(row, min(i + 1, 63u)) * P.Wp; // prefetch the next row
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) {
+                Xc[s] = Xn[s];
+                Xn[s] = ((uint32_t)s < P.slots && s * 64 + lane < P.W) ? pmnext[s * 64 + lane] : 0;
+            }
             uint64_t hn_c0 = 0, hp_c0 = 1, tr_c0 = 0;
 #pragma unroll
             for (int s = 0; s < MAXS; ++s) {
                 if ((uint32_t)s < P.slots) {
                     const uint32_t w = s * 64 + lane;
                     const bool act = w < P.W;
-                    const uint64_t X = pmrow[w];
+                    const uint64_t X = Xc[s];
                     const uint64_t vp = VP[s], vn = VN[s], d0old = D0s[s];
                     // ((~D0_last) & PM_last) >> 63 of the word below in the pattern
                     const uint64_t TRm = ballot(act && ((((~d0old) & X) >> 63) & 1));
@@ -404,7 +438,7 @@ __device__ RawResult run_jaro(const Pattern<CharT, LDS>& P, uint32_t m, const Ch
                         const uint32_t bb = b > 63 ? 63 : (uint32_t)b;
                         wm = (~0ull >> (63 - bb)) & (~0ull << aa);
                     }
-                    const uint64_t x = pmrow[w] & wm & ~Pf[s];
+                    const uint64_t x = (w < P.W ? pmrow[w] : 0) & wm & ~Pf[s];
                     const uint64_t any = ballot(w < P.W && x != 0);
                     if (any) {
                         const uint32_t first = __builtin_ctzll(any);
@@ -444,7 +478,7 @@ __device__ RawResult run_jaro(const Pattern<CharT, LDS>& P, uint32_t m, const Ch
                         uint32_t mis = 0;
                         if (lane == first) {
                             const uint64_t pmask = blsi(Pr[s]);
-                            mis = (pmrow[s * 64 + lane] & pmask) == 0;
+                            mis = ((s * 64 + lane < P.W ? pmrow[s * 64 + lane] : 0) & pmask) == 0;
                             Pr[s] ^= pmask;
                         }
                         tr += readlane(mis, first);
@@ -523,27 +557,89 @@ __device__ __forceinline__ bool length_filter(const ScoreSpec& s, int algo, uint
     return false;
 }
 
+// Arena bytes one pattern of m units needs: symbol map + PM rows (rows bounded by
+// the distinct-symbol count, which is at most min(m, alphabet)).
+template <typename CharT>
+__host__ __device__ inline uint32_t pattern_hash_bits(uint32_t m)
+{
+    const uint64_t alpha = sizeof(CharT) == 1 ? 256 : (sizeof(CharT) == 2 ? 65536 : 0xffffffffull);
+    const uint64_t d = m < alpha ? m : alpha;
+    uint32_t b = 6;
+    while ((1ull << b) < 2 * d) ++b;
+    return b;
+}
+
+template <typename CharT>
+__host__ __device__ inline uint64_t pattern_arena_bytes(uint32_t m)
+{
+    const uint64_t alpha = sizeof(CharT) == 1 ? 256 : (sizeof(CharT) == 2 ? 65536 : 0xffffffffull);
+    const uint64_t rows = (m < alpha ? m : alpha) + 1;
+    const uint64_t W = (m + 63) / 64;
+    const uint64_t map = sizeof(CharT) == 1 ? 1024 : ((1ull << pattern_hash_bits<CharT>(m)) * 12 + 8);
+    return map + rows * W * 8;
+}
+
+// One item with the arena in LDS (LDS = true) or in the workgroup's global slice.
 template <int ALGO, int MAXS, typename CharT, bool LDS>
+__device__ __forceinline__ RawResult block_item(int algo, const CharT* pp, uint32_t m, const CharT* tt, uint32_t n,
+                                                uint8_t* arena, uint64_t* tflag, uint64_t key, bool reuse,
+                                                Pattern<CharT, LDS>& P, uint64_t& prev_key)
+{
+    RawResult res{0, 0, 0};
+    if (key != prev_key || !reuse) {
+        P = build_pattern<CharT, LDS>(pp, m, arena, pattern_hash_bits<CharT>(m));
+        prev_key = key;
+    }
+    if constexpr (ALGO == ALGO_MIXED) {
+        if (algo == ALGO_LEV)
+            res.v = run_lev<MAXS>(P, m, tt, n);
+        else if (algo == ALGO_OSA)
+            res.v = run_osa<MAXS>(P, m, tt, n);
+        else
+            res.v = run_lcs<MAXS>(P, tt, n);
+    }
+    else if constexpr (ALGO == ALGO_LEV)
+        res.v = run_lev<MAXS>(P, m, tt, n);
+    else if constexpr (ALGO == ALGO_LCS)
+        res.v = run_lcs<MAXS>(P, tt, n);
+    else if constexpr (ALGO == ALGO_OSA)
+        res.v = run_osa<MAXS>(P, m, tt, n);
+    else
+        res = run_jaro<MAXS>(P, m, tt, n, tflag);
+    return res;
+}
+
+template <int ALGO, int MAXS, typename CharT>
 __global__ __launch_bounds__(64) void k_block(BlockArgs a)
 {
     extern __shared__ uint64_t smem_u64[];
-    uint8_t* arena = LDS ? (uint8_t*)smem_u64 : (uint8_t*)(a.pm_scratch + (size_t)blockIdx.x * a.pm_scratch_words);
+    uint8_t* lds_arena = (uint8_t*)smem_u64;
+    uint8_t* glb_arena = a.pm_scratch ? (uint8_t*)(a.pm_scratch + (size_t)blockIdx.x * a.pm_scratch_words) : nullptr;
     uint64_t* tflag = a.tflag_scratch ? a.tflag_scratch + (size_t)blockIdx.x * a.tflag_words : nullptr;
     const CharT* __restrict__ A = (const CharT*)a.achars;
     const CharT* __restrict__ B = (const CharT*)a.bchars;
     const uint32_t lane = threadIdx.x & 63;
     __shared__ uint32_t s_item;
-    uint64_t prev_key = ~0ull;
-    Pattern<CharT, LDS> P{};
+    uint64_t keyL = ~0ull, keyG = ~0ull;
+    Pattern<CharT, true> PL{};
+    Pattern<CharT, false> PG{};
     for (;;) {
         if (lane == 0) s_item = atomicAdd(a.queue, 1u);
         __syncthreads();
         const uint32_t item = uniform(s_item);
         __syncthreads();
-        if (item >= a.nitems) break;
+        const uint32_t nitems = a.nitems_dev ? uniform(*a.nitems_dev) : a.nitems;
+        if (item >= nitems) break;
         uint32_t ia, ib;
         uint64_t oidx;
-        if (a.cross_nb) {
+        bool swap = false;
+        if (a.item_packed) {
+            const uint32_t pk = uniform(a.item_packed[item]);
+            ia = ib = pk & 0x7fffffffu;
+            swap = (pk >> 31) != 0;
+            oidx = ia;
+        }
+        else if (a.cross_nb) {
             ia = a.item_a[item / a.cross_nb];
             ib = a.item_b[item % a.cross_nb];
             oidx = (uint64_t)ia * a.out_stride + ib;
@@ -552,94 +648,80 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a)
             ia = a.item_a[item];
             ib = a.item_b[item];
             oidx = a.out_index[item];
+            swap = a.item_swap ? a.item_swap[item] != 0 : false;
         }
-        const bool swap = a.item_swap ? a.item_swap[item] != 0 : false;
+        ScoreSpec spec = a.spec;
+        int algo = ALGO;
+        if constexpr (ALGO == ALGO_MIXED) {
+            spec.metric = uniform((uint32_t)a.pair_metric[ia]);
+            algo = spec.metric == RFA_LEVENSHTEIN ? ALGO_LEV : (spec.metric == RFA_OSA ? ALGO_OSA : ALGO_LCS);
+            if (spec.kind == RFA_DISTANCE && spec.metric != RFA_LCSSEQ) spec.band_k = spec.cutoff_u;
+        }
         const uint64_t oa = a.aoff[ia], ob = a.boff[ib];
         const uint32_t la = (uint32_t)(a.aoff[ia + 1] - oa), lb = (uint32_t)(a.boff[ib + 1] - ob);
         const CharT* pp = swap ? B + ob : A + oa;
         const CharT* tt = swap ? A + oa : B + ob;
         const uint32_t m = swap ? lb : la, n = swap ? la : lb;
         RawResult res{0, 0, 0};
-        if (!length_filter(a.spec, ALGO, la, lb, &res)) {
+        if (!length_filter(spec, algo, la, lb, &res)) {
             if (m == 0) {
-                res.v = (ALGO == ALGO_LCS || ALGO == ALGO_JARO) ? 0 : n;
+                res.v = (algo == ALGO_LCS || algo == ALGO_JARO) ? 0 : n;
             }
             else {
                 const uint64_t key = ((uint64_t)swap << 63) | (swap ? ib : ia);
-                if (key != prev_key || !a.reuse_pattern) {
-                    P = build_pattern<CharT, LDS>(pp, m, arena, a.hash_bits);
-                    prev_key = key;
-                }
-                if constexpr (ALGO == ALGO_LEV)
-                    res.v = run_lev<MAXS>(P, m, tt, n);
-                else if constexpr (ALGO == ALGO_LCS)
-                    res.v = run_lcs<MAXS>(P, tt, n);
-                else if constexpr (ALGO == ALGO_OSA)
-                    res.v = run_osa<MAXS>(P, m, tt, n);
+                if (pattern_arena_bytes<CharT>(m) <= a.lds_bytes)
+                    res = block_item<ALGO, MAXS, CharT, true>(algo, pp, m, tt, n, lds_arena, tflag, key,
+                                                              a.reuse_pattern, PL, keyL);
                 else
-                    res = run_jaro<MAXS>(P, m, tt, n, tflag);
+                    res = block_item<ALGO, MAXS, CharT, false>(algo, pp, m, tt, n, glb_arena, tflag, key,
+                                                               a.reuse_pattern, PG, keyG);
             }
         }
-        if (lane == 0) finalize(a.spec, la, lb, res, a.out, oidx);
+        if (lane == 0) finalize(spec, la, lb, res, a.out, oidx);
     }
-}
-
-size_t block_arena_bytes(int char_width, uint32_t max_m, uint32_t max_rows, uint32_t hash_bits)
-{
-    const uint32_t W = (max_m + 63) / 64;
-    const uint32_t Wp = ((W + 63) / 64) * 64;
-    size_t map = (char_width == 1) ? 1024 : (((size_t)1 << hash_bits) * 12 + 8);
-    return map + (size_t)max_rows * Wp * 8;
 }
 
 template <int ALGO, int MAXS, typename CharT>
-static void launch3(const BlockArgs& a, uint32_t nblocks, bool lds, size_t lds_bytes, hipStream_t s)
+static void launch3(const BlockArgs& a, uint32_t nblocks, hipStream_t s)
 {
-    if (lds)
-        hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT, true>), dim3(nblocks), dim3(64), lds_bytes, s, a);
-    else
-        hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT, false>), dim3(nblocks), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT>), dim3(nblocks), dim3(64), a.lds_bytes, s, a);
 }
 
 template <int ALGO, typename CharT>
-static void launch2(const BlockArgs& a, uint32_t nblocks, bool lds, size_t lds_bytes, uint32_t slots, hipStream_t s)
+static void launch2(const BlockArgs& a, uint32_t nblocks, uint32_t slots, hipStream_t s)
 {
     if (slots <= 1)
-        launch3<ALGO, 1, CharT>(a, nblocks, lds, lds_bytes, s);
+        launch3<ALGO, 1, CharT>(a, nblocks, s);
     else if (slots <= 2)
-        launch3<ALGO, 2, CharT>(a, nblocks, lds, lds_bytes, s);
+        launch3<ALGO, 2, CharT>(a, nblocks, s);
     else if (slots <= 4)
-        launch3<ALGO, 4, CharT>(a, nblocks, lds, lds_bytes, s);
+        launch3<ALGO, 4, CharT>(a, nblocks, s);
     else if (slots <= 8)
-        launch3<ALGO, 8, CharT>(a, nblocks, lds, lds_bytes, s);
+        launch3<ALGO, 8, CharT>(a, nblocks, s);
     else if (slots <= 16)
-        launch3<ALGO, 16, CharT>(a, nblocks, lds, lds_bytes, s);
+        launch3<ALGO, 16, CharT>(a, nblocks, s);
     else
-        launch3<ALGO, 32, CharT>(a, nblocks, lds, lds_bytes, s);
+        launch3<ALGO, 32, CharT>(a, nblocks, s);
 }
 
+// Instantiated once per code-unit width (rfa_block_u8/u16/u32.hip).
 template <typename CharT>
-static void launch1(int algo, const BlockArgs& a, uint32_t nblocks, bool lds, size_t lds_bytes, uint32_t slots,
-                    hipStream_t s)
+void launch_block_w(int algo, const BlockArgs& a, uint32_t nblocks, uint32_t slots, hipStream_t s)
 {
     switch (algo) {
-    case ALGO_LEV: launch2<ALGO_LEV, CharT>(a, nblocks, lds, lds_bytes, slots, s); break;
-    case ALGO_LCS: launch2<ALGO_LCS, CharT>(a, nblocks, lds, lds_bytes, slots, s); break;
-    case ALGO_OSA: launch2<ALGO_OSA, CharT>(a, nblocks, lds, lds_bytes, slots, s); break;
-    default: launch2<ALGO_JARO, CharT>(a, nblocks, lds, lds_bytes, slots, s); break;
+    case ALGO_LEV: launch2<ALGO_LEV, CharT>(a, nblocks, slots, s); break;
+    case ALGO_LCS: launch2<ALGO_LCS, CharT>(a, nblocks, slots, s); break;
+    case ALGO_OSA: launch2<ALGO_OSA, CharT>(a, nblocks, slots, s); break;
+    case ALGO_MIXED: launch2<ALGO_MIXED, CharT>(a, nblocks, slots, s); break;
+    default: launch2<ALGO_JARO, CharT>(a, nblocks, slots, s); break;
     }
 }
 
-hipError_t launch_block(int algo, int char_width, const BlockArgs& a, uint32_t nblocks, bool lds, size_t lds_bytes,
-                        uint32_t slots, hipStream_t s)
+inline uint64_t block_arena_need(int char_width, uint32_t m)
 {
-    if (char_width == 1)
-        launch1<uint8_t>(algo, a, nblocks, lds, lds_bytes, slots, s);
-    else if (char_width == 2)
-        launch1<uint16_t>(algo, a, nblocks, lds, lds_bytes, slots, s);
-    else
-        launch1<uint32_t>(algo, a, nblocks, lds, lds_bytes, slots, s);
-    return hipGetLastError();
+    if (char_width == 1) return pattern_arena_bytes<uint8_t>(m);
+    if (char_width == 2) return pattern_arena_bytes<uint16_t>(m);
+    return pattern_arena_bytes<uint32_t>(m);
 }
 
 } // namespace rfa

@@ -51,14 +51,20 @@ struct BlockArgs {
     uint64_t pm_scratch_words; // uint64 words per workgroup arena
     uint64_t* tflag_scratch;   // Jaro text flags, tflag_words per workgroup
     uint64_t tflag_words;
-    uint32_t hash_bits; // symbol hash size (16/32-bit units)
+    uint32_t hash_bits;  // unused (each pattern sizes its own symbol hash)
+    uint32_t lds_bytes;  // dynamic LDS per workgroup; patterns that need more use the global arena
     ScoreSpec spec;
     void* out;
+    // device-planned mode: item k is item_packed[k] = pair | swap << 31 for k < *nitems_dev,
+    // the metric of a pair is pair_metric[pair] and its result goes to out[pair]
+    const uint32_t* item_packed;
+    const uint32_t* nitems_dev;
+    const int32_t* pair_metric;
 };
 
-size_t block_arena_bytes(int char_width, uint32_t max_m, uint32_t max_rows, uint32_t hash_bits);
-hipError_t launch_block(int algo, int char_width, const BlockArgs& a, uint32_t nblocks, bool lds, size_t lds_bytes,
-                        uint32_t slots, hipStream_t s);
+// arena bytes of the largest pattern (symbol map + PM rows)
+uint64_t block_arena_bytes(int char_width, uint32_t max_m);
+hipError_t launch_block(int algo, int char_width, const BlockArgs& a, uint32_t nblocks, uint32_t slots, hipStream_t s);
 
 // Lane kernel (rfa_lane.hip): one pair per lane, per-lane 128-slot maps per 64-unit block.
 struct LaneArgs {
@@ -77,9 +83,22 @@ struct LaneArgs {
     uint32_t gen_base;           // generation tags of this launch start above gen_base
     ScoreSpec spec;
     void* out;
+    // device-planned mode: items item_packed[*begin_dev .. nitems), pair | swap << 31,
+    // metric pair_metric[pair], result out[pair]
+    const uint32_t* item_packed;
+    const uint32_t* begin_dev;
+    const uint32_t* end_dev; // nullptr: nitems
+    const int32_t* pair_metric;
 };
+
+size_t plan_temp_bytes(uint64_t n);
+// counters[0]: block queue, [1]: end of the long range, [2]: end of the short-64 range
+hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t* metric, uint64_t n,
+                       uint64_t* keys_in, uint64_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp,
+                       size_t temp_bytes, uint32_t* counters, hipStream_t s);
 
 size_t lane_arena_words(uint32_t maxw);
 hipError_t launch_lane(int char_width, int maxw, const LaneArgs& a, uint32_t nblocks, hipStream_t s);
+hipError_t launch_lane_short(int char_width, int word_bits, const LaneArgs& a, uint32_t nblocks, hipStream_t s);
 
 } // namespace rfa

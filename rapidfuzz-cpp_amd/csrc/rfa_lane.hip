@@ -16,6 +16,7 @@
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
 #include "rfa_semantics.hpp"
+#include "rfa_steps.hpp"
 
 namespace rfa {
 
@@ -69,19 +70,33 @@ __global__ __launch_bounds__(256) void k_lane(LaneArgs a)
     uint32_t gen = a.gen_base;
     // per-lane word state (private memory for MAXW > 1)
     uint64_t VP[MAXW], VN[MAXW], D0s[MAXW], PMo[MAXW];
-    for (uint64_t item = g; item < a.nitems; item += nlanes) {
-        const uint32_t ia = a.item_a ? a.item_a[item] : (uint32_t)item;
-        const uint32_t ib = a.item_b ? a.item_b[item] : (uint32_t)item;
-        const bool swap = a.item_swap ? a.item_swap[item] != 0 : false;
+    const uint64_t begin = a.begin_dev ? *a.begin_dev : 0;
+    for (uint64_t item = begin + g; item < a.nitems; item += nlanes) {
+        uint32_t ia, ib;
+        bool swap;
+        uint64_t oidx;
+        if (a.item_packed) {
+            const uint32_t pk = a.item_packed[item];
+            ia = ib = pk & 0x7fffffffu;
+            swap = (pk >> 31) != 0;
+            oidx = ia;
+        }
+        else {
+            ia = a.item_a ? a.item_a[item] : (uint32_t)item;
+            ib = a.item_b ? a.item_b[item] : (uint32_t)item;
+            swap = a.item_swap ? a.item_swap[item] != 0 : false;
+            oidx = a.out_index ? a.out_index[item] : item;
+        }
         const uint64_t oa = a.aoff[ia], ob = a.boff[ib];
         const uint32_t la = (uint32_t)(a.aoff[ia + 1] - oa), lb = (uint32_t)(a.boff[ib + 1] - ob);
         const CharT* __restrict__ p = swap ? B + ob : A + oa;
         const CharT* __restrict__ t = swap ? A + oa : B + ob;
         const uint32_t m = swap ? lb : la, n = swap ? la : lb;
         ScoreSpec spec = a.spec;
-        if (a.item_metric) {
+        if (a.pair_metric)
+            spec.metric = a.pair_metric[ia];
+        else if (a.item_metric)
             spec.metric = a.item_metric[item];
-        }
         const int algo = (spec.metric == RFA_LEVENSHTEIN) ? ALGO_LEV : (spec.metric == RFA_OSA ? ALGO_OSA : ALGO_LCS);
         RawResult res{0, 0, 0};
         if (m == 0) {
@@ -158,8 +173,151 @@ __global__ __launch_bounds__(256) void k_lane(LaneArgs a)
                 res.v = (uint64_t)d;
             }
         }
-        finalize(spec, la, lb, res, a.out, a.out_index ? a.out_index[item] : item);
+        finalize(spec, la, lb, res, a.out, oidx);
     }
+}
+
+// ---- short patterns (<= 64 units): per-lane symbol tables in LDS ------------
+//
+// The bucketed schedule sends pairs whose shorter string has at most 64 units
+// here.  Each lane keeps a 64-slot open-addressing table (symbol -> match
+// mask) for its pattern in LDS, slot s of lane l at [s * 64 + l]; a slot is
+// free when its mask is zero, and after a pair only the occupied slots
+// (tracked in a 64-bit register) are cleared.  W is 32 bits for patterns of at
+// most 32 units and 64 bits otherwise; the planner keeps the two groups (and
+// the metrics) in separate ranges sorted by text length, so the lanes of a
+// wave run the same recurrence for about the same number of steps.
+constexpr uint32_t SHORT_SLOTS = 64;
+
+template <typename CharT, typename W>
+__global__ __launch_bounds__(64) void k_lane_short(LaneArgs a)
+{
+    __shared__ uint32_t keys[SHORT_SLOTS * 64];
+    __shared__ W masks[SHORT_SLOTS * 64];
+    const CharT* __restrict__ A = (const CharT*)a.achars;
+    const CharT* __restrict__ B = (const CharT*)a.bchars;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t s = 0; s < SHORT_SLOTS; ++s) masks[s * 64 + lane] = 0;
+    const uint64_t begin = a.begin_dev ? *a.begin_dev : 0;
+    const uint64_t end = a.end_dev ? *a.end_dev : a.nitems;
+    const uint64_t nl = (uint64_t)gridDim.x * 64;
+    for (uint64_t item = begin + (uint64_t)blockIdx.x * 64 + lane; item < end; item += nl) {
+        const uint32_t pk = a.item_packed[item];
+        const uint32_t pair = pk & 0x7fffffffu;
+        const bool swap = (pk >> 31) != 0;
+        const uint64_t oa = a.aoff[pair], ob = a.boff[pair];
+        const uint32_t la = (uint32_t)(a.aoff[pair + 1] - oa), lb = (uint32_t)(a.boff[pair + 1] - ob);
+        const CharT* __restrict__ p = swap ? B + ob : A + oa;
+        const CharT* __restrict__ t = swap ? A + oa : B + ob;
+        const uint32_t m = swap ? lb : la, n = swap ? la : lb;
+        ScoreSpec spec = a.spec;
+        spec.metric = a.pair_metric[pair];
+        const int algo = (spec.metric == RFA_LEVENSHTEIN) ? ALGO_LEV : (spec.metric == RFA_OSA ? ALGO_OSA : ALGO_LCS);
+        RawResult res{0, 0, 0};
+        if (m == 0) {
+            res.v = (algo == ALGO_LCS) ? 0 : n;
+        }
+        else {
+            uint64_t occ = 0;
+            uint32_t pat[64];
+#pragma unroll
+            for (uint32_t i = 0; i < 64; ++i) pat[i] = (uint32_t)p[min(i, m - 1)]; // independent loads
+#pragma unroll
+            for (uint32_t i = 0; i < 64; ++i) {
+                if (i < m) {
+                    const uint32_t c = pat[i];
+                    uint32_t h = (c * 0x9E3779B1u) >> 26;
+                    while (masks[h * 64 + lane] != 0 && keys[h * 64 + lane] != c) h = (h + 1) & (SHORT_SLOTS - 1);
+                    keys[h * 64 + lane] = c;
+                    masks[h * 64 + lane] |= (W)1 << i;
+                    occ |= 1ull << h;
+                }
+            }
+            uint32_t VPl = ~0u, VPh = ~0u, VNl = 0, VNh = 0, D0l = 0, D0h = 0, PMl = 0, PMh = 0;
+            // the text is read CHUNK units ahead: one batch of independent loads is in
+            // flight while the previous batch is consumed
+            constexpr uint32_t CHUNK = 8;
+            uint32_t nxt[CHUNK];
+            // loads are unconditional (index clamped to the last unit) so the wait
+            // counter can retire them in order instead of draining all of them
+            const uint32_t last = n - 1;
+#pragma unroll
+            for (uint32_t k = 0; k < CHUNK; ++k) nxt[k] = (uint32_t)t[min(k, last)];
+            for (uint32_t j0 = 0; j0 < n; j0 += CHUNK) {
+                uint32_t cur[CHUNK];
+#pragma unroll
+                for (uint32_t k = 0; k < CHUNK; ++k) {
+                    cur[k] = nxt[k];
+                    nxt[k] = (uint32_t)t[min(j0 + CHUNK + k, last)];
+                }
+                const uint32_t cnt = min(CHUNK, n - j0);
+#pragma unroll
+                for (uint32_t k = 0; k < CHUNK; ++k) {
+                if (k < cnt) {
+                const uint32_t c = cur[k];
+                uint32_t h = (c * 0x9E3779B1u) >> 26;
+                W X = masks[h * 64 + lane];
+                uint32_t key = keys[h * 64 + lane];
+                while (X != 0 && key != c) {
+                    h = (h + 1) & (SHORT_SLOTS - 1);
+                    X = masks[h * 64 + lane];
+                    key = keys[h * 64 + lane];
+                }
+                if constexpr (sizeof(W) == 4) {
+                    if (algo == ALGO_LEV)
+                        lev_step((uint32_t)X, VPl, VNl);
+                    else if (algo == ALGO_OSA)
+                        osa_step((uint32_t)X, VPl, VNl, D0l, PMl);
+                    else
+                        lcs_step((uint32_t)X, VPl);
+                }
+                else {
+                    if (algo == ALGO_LEV)
+                        lev_step((uint64_t)X, VPl, VPh, VNl, VNh);
+                    else if (algo == ALGO_OSA)
+                        osa_step((uint64_t)X, VPl, VPh, VNl, VNh, D0l, D0h, PMl, PMh);
+                    else
+                        lcs_step((uint64_t)X, VPl, VPh);
+                }
+                }
+                }
+            }
+            while (occ) {
+                const uint32_t h = __builtin_ctzll(occ);
+                occ &= occ - 1;
+                masks[h * 64 + lane] = 0;
+            }
+            const W VP = (sizeof(W) == 8) ? (W)(((uint64_t)VPh << 32) | VPl) : (W)VPl;
+            const W VN = (sizeof(W) == 8) ? (W)(((uint64_t)VNh << 32) | VNl) : (W)VNl;
+            const W M = (m >= sizeof(W) * 8) ? ~(W)0 : (((W)1 << m) - 1);
+            if (algo == ALGO_LCS)
+                res.v = (uint64_t)popc<W>(~VP & M);
+            else
+                res.v = (uint64_t)n + popc<W>(VP & M) - popc<W>(VN & M);
+        }
+        finalize(spec, la, lb, res, a.out, pair);
+    }
+}
+
+hipError_t launch_lane_short(int char_width, int word_bits, const LaneArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    if (word_bits == 32) {
+        if (char_width == 1)
+            hipLaunchKernelGGL((k_lane_short<uint8_t, uint32_t>), dim3(nblocks), dim3(64), 0, s, a);
+        else if (char_width == 2)
+            hipLaunchKernelGGL((k_lane_short<uint16_t, uint32_t>), dim3(nblocks), dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_lane_short<uint32_t, uint32_t>), dim3(nblocks), dim3(64), 0, s, a);
+    }
+    else {
+        if (char_width == 1)
+            hipLaunchKernelGGL((k_lane_short<uint8_t, uint64_t>), dim3(nblocks), dim3(64), 0, s, a);
+        else if (char_width == 2)
+            hipLaunchKernelGGL((k_lane_short<uint16_t, uint64_t>), dim3(nblocks), dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_lane_short<uint32_t, uint64_t>), dim3(nblocks), dim3(64), 0, s, a);
+    }
+    return hipGetLastError();
 }
 
 size_t lane_arena_words(uint32_t maxw) { return (size_t)maxw * MAP_SLOTS * 2; }

@@ -24,7 +24,8 @@ enum Algo : int32_t {
     ALGO_LEV = 0,  // Hyyro 2003 Levenshtein (Levenshtein_impl.hpp:259-322, 641-850)
     ALGO_LCS = 1,  // Hyyro 2004 LCS (LCSseq_impl.hpp:205-333)
     ALGO_OSA = 2,  // Hyyro 2003 with transpositions (OSA_impl.hpp:34-74, 176-249)
-    ALGO_JARO = 3  // greedy window flagging + transposition walk (Jaro_impl.hpp:94-315)
+    ALGO_JARO = 3, // greedy window flagging + transposition walk (Jaro_impl.hpp:94-315)
+    ALGO_MIXED = 4 // per-item metric (Levenshtein / Indel / LCSseq / OSA), chosen at run time
 };
 
 // A packed string set on the device.
