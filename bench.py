@@ -198,16 +198,183 @@ def workload_uneven(args):
     print(json.dumps(line), flush=True)
 
 
+SERVE_CHOICES = 2_000_000
+SERVE_QPS = 2000.0
+SERVE_K = 5
+SERVE_CUTOFF = 70.0
+
+
+def gen_serving_choices(rng, n=SERVE_CHOICES, vocab_n=5000, zipf_s=1.0):
+    """n choices of 1-12 words (uniform) from a seeded vocabulary of vocab_n lowercase words of 2-12
+    letters, word ranks Zipf(zipf_s), joined by single spaces; a choice shorter than 4 bytes takes one
+    more word, one longer than 128 bytes is cut at 128."""
+    vlen = rng.integers(2, 13, size=vocab_n).astype(np.int64)
+    voff = np.zeros(vocab_n + 1, np.int64)
+    np.cumsum(vlen, out=voff[1:])
+    vchars = rng.integers(97, 123, size=int(voff[-1])).astype(np.uint8)
+    p = 1.0 / np.arange(1, vocab_n + 1) ** zipf_s
+    cdf = np.cumsum(p / p.sum())
+    counts = rng.integers(1, 13, size=n).astype(np.int64)
+    words = np.minimum(np.searchsorted(cdf, rng.random((n, 13)), side="right"), vocab_n - 1)
+    wl = vlen[words]
+    used = np.arange(13)[None, :] < counts[:, None]
+    total = (wl * used).sum(1) + counts - 1
+    counts += (total < 4)
+    used = np.arange(13)[None, :] < counts[:, None]
+    ids = words[used]                      # row-major: choice by choice
+    per_choice = counts
+    lens_w = vlen[ids] + 1                 # word + separator
+    starts = np.zeros(len(ids) + 1, np.int64)
+    np.cumsum(lens_w, out=starts[1:])
+    wi = np.repeat(np.arange(len(ids), dtype=np.int64), lens_w)
+    o = np.arange(int(starts[-1]), dtype=np.int64) - starts[wi]
+    flat = np.where(o < vlen[ids][wi], vchars[np.minimum(voff[ids][wi] + o, len(vchars) - 1)], np.uint8(32))
+    # drop each choice's trailing separator and cut choices at 128 bytes
+    cend_w = np.cumsum(per_choice) - 1     # last word of each choice
+    choice_of_word = np.repeat(np.arange(n, dtype=np.int64), per_choice)
+    cstart = starts[np.concatenate([[0], cend_w[:-1] + 1])]
+    pos_in_choice = np.arange(int(starts[-1]), dtype=np.int64) - cstart[choice_of_word[wi]]
+    clen = np.minimum(starts[cend_w + 1] - cstart - 1, 128)
+    keep = pos_in_choice < clen[choice_of_word[wi]]
+    chars = np.ascontiguousarray(flat[keep])
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(clen, out=off[1:])
+    return chars, off
+
+
+def gen_requests(rng, cc, co, n):
+    """n queries: a uniformly chosen choice with U{0..6} random edits (substitute / insert / delete,
+    lowercase letters), padded with random letters to at least 8 bytes, at most 256."""
+    nc = len(co) - 1
+    picks = rng.integers(0, nc, size=n)
+    out = []
+    for c in picks:
+        s = bytearray(cc[int(co[c]):int(co[c + 1])].tobytes())
+        for _ in range(int(rng.integers(0, 7))):
+            op = int(rng.integers(0, 3))
+            if op == 0 and s:
+                s[int(rng.integers(0, len(s)))] = int(rng.integers(97, 123))
+            elif op == 1:
+                s.insert(int(rng.integers(0, len(s) + 1)), int(rng.integers(97, 123)))
+            elif s:
+                del s[int(rng.integers(0, len(s)))]
+        while len(s) < 8:
+            s.append(int(rng.integers(97, 123)))
+        out.append(np.frombuffer(bytes(s[:256]), dtype=np.uint8))
+    return out
+
+
+def workload_serve(args, emit=True):
+    """Serving: 2,000,000 resident choices, requests Poisson at 2,000 QPS, each the top-5 by
+    fuzz::ratio >= 70 (ordered by score desc, index asc); p50/p99 latency from the scheduled arrival
+    to the answer, through the library's continuous-batching server."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(SEED + 5)
+    t0 = time.perf_counter()
+    cc, co = gen_serving_choices(rng)
+    nreq = int(SERVE_QPS * args.serve_seconds)
+    queries = gen_requests(rng, cc, co, nreq)
+    arrivals = np.cumsum(rng.exponential(1.0 / SERVE_QPS, size=nreq))
+    clen = np.diff(co)
+    print(f"# serve: {len(co) - 1} choices (mean len {clen.mean():.1f}, {int(co[-1])} bytes), {nreq} requests "
+          f"over {arrivals[-1]:.1f} s; generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    ctx = rfa.Context(0)
+    cs = rfa.Strings(ctx, chars=cc, offsets=co, width=1)
+    t0 = time.perf_counter()
+    srv = rfa.Server(ctx, cs, metric=rfa.RATIO, kind=rfa.SIMILARITY, score_cutoff=SERVE_CUTOFF, k=SERVE_K,
+                     max_batch=args.serve_batch, slots=args.serve_slots, candidate_cap=65536)
+    print(f"# serve: server up (index built) in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    # warm-up: a short burst at the same rate
+    warm = min(nreq, 400)
+    srv.replay(queries[:warm], arrivals[:warm] - arrivals[0])
+    st0 = srv.stats()
+    t0 = time.perf_counter()
+    lat, idx, sc = srv.replay(queries, arrivals)
+    wall = time.perf_counter() - t0
+    st1 = srv.stats()
+    batches = st1["batches"] - st0["batches"]
+    served_qps = nreq / max(wall, 1e-9)
+    lat_ms = lat * 1e3
+    pct = {p: float(np.percentile(lat_ms, p)) for p in (50, 90, 99, 99.9)}
+    print(f"# serve: {nreq} requests in {wall:.2f} s ({served_qps:.0f} QPS), p50 {pct[50]:.3f} ms, "
+          f"p99 {pct[99]:.3f} ms, max {lat_ms.max():.3f} ms, {batches} batches", file=sys.stderr, flush=True)
+    # saturation: every request at t = 0 (the largest rate the server sustains)
+    nb = min(nreq, 8192)
+    t0 = time.perf_counter()
+    blat, bidx, bsc = srv.replay(queries[:nb], np.zeros(nb))
+    bwall = time.perf_counter() - t0
+    st2 = srv.stats()
+    same_burst = bool(np.array_equal(bidx, idx[:nb]) and np.array_equal(bsc.view(np.uint64), sc[:nb].view(np.uint64)))
+    print(f"# serve: saturation {nb} requests in {bwall:.2f} s = {nb / bwall:.0f} QPS "
+          f"(mean batch {nb / max(1, st2['batches'] - st1['batches']):.1f}); burst == paced: {same_burst}",
+          file=sys.stderr, flush=True)
+    check = "paced and burst answers identical" if same_burst else "PACED AND BURST ANSWERS DIFFER"
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        if oracle_py.reference_available():
+            ref = oracle_py.Reference()
+            cores = min(len(os.sched_getaffinity(0)), 16)
+            ns = cores * 2
+            while True:
+                qc_s, qo_s = pack_u8(queries[:ns])
+                t0 = time.perf_counter()
+                ri, rs, secs = ref.extract_ratio_mt(qc_s, qo_s, cc, co, SERVE_CUTOFF, SERVE_K, threads=cores)
+                dt = time.perf_counter() - t0
+                if dt > 5 or ns >= nreq:
+                    break
+                ns = min(nreq, int(ns * max(2.0, 12.0 / max(dt, 1e-3))))
+            bad = int(np.count_nonzero((ri != idx[:ns]) | (rs.view(np.uint64) != sc[:ns].view(np.uint64))))
+            check += "; " + (f"top-{SERVE_K} bit-exact vs reference on the CPU sample ({ns} requests)" if bad == 0
+                             else f"{bad} MISMATCHES vs reference on {ns} requests")
+            cpu = {"value": round(ns / dt, 2), "unit": "requests/s", "cores": cores, "kind": "reference",
+                   "sample": f"first {ns} requests, fuzz::CachedRatio extract() over all {len(co) - 1} choices + "
+                             f"partial sort, one request per thread ({dt:.1f} s)",
+                   "single_request_ms_p50": round(float(np.median(secs)) * 1e3, 1)}
+    line = {
+        "metric": "served p50/p99 latency at 2000 QPS, top-5 fuzz::ratio >= 70 over 2M resident choices",
+        "value": round(pct[99], 3), "unit": "ms (p99)", "n_gpus": 1, "steps": nreq, "warmup": warm,
+        "ms_per_step": round(wall / nreq * 1e3, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"{len(co) - 1} choices of 1-12 Zipf(1.0) words from a 5000-word vocabulary "
+                               f"(4-128 bytes); Poisson {SERVE_QPS:.0f} QPS for {args.serve_seconds} s; "
+                               f"queries = choice + 0-6 edits (8-256 bytes); top-{SERVE_K}, ratio >= {SERVE_CUTOFF}",
+                   "max_batch": args.serve_batch, "slots": args.serve_slots},
+        "latency_ms": {"p50": round(pct[50], 3), "p90": round(pct[90], 3), "p99": round(pct[99], 3),
+                       "p99.9": round(pct[99.9], 3), "max": round(float(lat_ms.max()), 3),
+                       "mean": round(float(lat_ms.mean()), 3)},
+        "served_qps": round(served_qps, 1),
+        "choices_scored_per_s": round(served_qps * (len(co) - 1), 0),
+        "batches": batches, "mean_batch": round(nreq / max(1, batches), 2),
+        "saturation_qps": round(nb / bwall, 1),
+        "dense_fallbacks": st2["dense_fallbacks"],
+        "cpu_baseline": cpu, "check": check,
+    }
+    srv.close()
+    cs.close()
+    ctx.close()
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="cdist", choices=["cdist", "uneven"],
-                    help="cdist: the headline 4096x65536 configuration (default); uneven: 1M independent pairs")
+    ap.add_argument("--workload", default="cdist", choices=["cdist", "uneven", "serve"],
+                    help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
+                         "uneven: 1M independent pairs; serve: the serving leg alone")
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-scorers", action="store_true", help="skip the per-scorer table")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-batch", type=int, default=64)
+    ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -215,6 +382,8 @@ def main():
         sys.exit(2)
     if args.workload == "uneven":
         return workload_uneven(args)
+    if args.workload == "serve":
+        return workload_serve(args)
 
     import rapidfuzz_amd as rfa
     from rapidfuzz_amd import hip
@@ -335,8 +504,15 @@ def main():
         "check": check,
         "scorers": scorers,
     }
-    print(json.dumps(res), flush=True)
     out.free()
+    for s_ in (lev, qs, cs):
+        s_.close()
+    if not args.no_serve:
+        srv = workload_serve(args, emit=False)
+        res["serving"] = {k: srv[k] for k in ("latency_ms", "served_qps", "choices_scored_per_s", "mean_batch",
+                                              "saturation_qps", "dense_fallbacks", "cpu_baseline", "check")}
+        res["serving"]["config"] = srv["config"]["workload"]
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

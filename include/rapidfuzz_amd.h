@@ -142,14 +142,64 @@ rfa_status rfa_cdist(rfa_scorer* scorer, const rfa_strings* choices, int kind, i
 rfa_status rfa_pairs(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics, int kind,
                      uint64_t cutoff_u, double cutoff_d, int schedule, void* out, int out_on_device);
 
-/* Top-k extraction (the README's extract() loop, README.md:153-214): for every
- * query, the k best choices by the scorer's `kind` with cutoff_d, ordered by
- * (score descending, index ascending).  Only double-valued kinds are
- * accepted (normalized kinds, Jaro(-Winkler), ratio).  Slots beyond the
- * number of qualifying choices get index -1 and score 0.
- * out_index[q * k + j], out_score[q * k + j]: host memory. */
+/* Top-k extraction.  Replaces the README's extract() loop (README.md:153-214):
+ * for every query, score = Cached<Metric>(query).<kind>(choice, cutoff_d) for
+ * every choice, results with score >= cutoff_d, ordered by (score descending,
+ * choice index ascending), first k kept.  `kind` must be a double similarity:
+ * RFA_NORMALIZED_SIMILARITY, or RFA_SIMILARITY of Jaro, Jaro-Winkler or
+ * ratio.  Entries beyond the number of qualifying choices get index -1 and
+ * score 0.  out_index[q * k + j], out_score[q * k + j]: host memory. */
 rfa_status rfa_topk(rfa_scorer* scorer, const rfa_strings* choices, int kind, double cutoff_d, uint32_t k,
                     int64_t* out_index, double* out_score);
+
+/* ---- request server (continuous batching) -------------------------------
+ *
+ * No reference counterpart: the reference leaves the extract() loop to the
+ * caller (README.md:153-214).  A server holds a device-resident copy of a
+ * choice set and answers top-k requests (as rfa_topk, one query each).
+ * Requests queue on submit; each of `slots` worker threads, whenever its
+ * previous batch is done, admits the pending requests in arrival order (up
+ * to max_batch; requests for the fused kernel and for the dense path go in
+ * separate batches) and runs them on its own HIP stream, so batches of
+ * different slots overlap on the device.  A request's result does not depend
+ * on the batch it ran in. */
+typedef struct rfa_server rfa_server;
+
+typedef struct {
+    int metric;             /* rfa_metric */
+    int kind;               /* a double similarity kind, as for rfa_topk */
+    double score_cutoff;
+    uint32_t k;
+    uint32_t max_batch;     /* requests admitted into one batch, 1..4096 */
+    uint32_t slots;         /* batches in flight at once, 1..8 */
+    uint32_t candidate_cap; /* results >= cutoff kept per request before ordering; a request
+                               with more is recomputed through the dense path */
+    double prefix_weight;   /* Jaro-Winkler */
+    rfa_weights weights;    /* Levenshtein; all zero means {1,1,1} */
+} rfa_server_config;
+
+typedef struct {
+    uint64_t requests;
+    uint64_t batches;
+    uint64_t max_batch_seen;
+    uint64_t dense_fallbacks;
+} rfa_server_stats;
+
+rfa_status rfa_server_create(rfa_context* ctx, const rfa_strings* choices, const rfa_server_config* cfg,
+                             rfa_server** out);
+/* Finishes the pending requests, then frees the server. */
+rfa_status rfa_server_destroy(rfa_server* srv);
+/* Queues one query (len code units of the choice set's width); returns at once. */
+rfa_status rfa_server_submit(rfa_server* srv, const void* query, uint64_t len, uint64_t* ticket);
+/* Blocks until the request is answered; k entries each (either may be NULL). */
+rfa_status rfa_server_wait(rfa_server* srv, uint64_t ticket, int64_t* out_index, double* out_score);
+/* Load replay: submits query i at arrival_s[i] seconds after the call starts
+ * (from a feeder thread), waits for all, and writes latency_s[i] = answer
+ * time - scheduled arrival.  Results as rfa_server_wait (k per query; the
+ * output pointers may be NULL). */
+rfa_status rfa_server_replay(rfa_server* srv, const void* chars, const uint64_t* offsets, uint64_t count,
+                             const double* arrival_s, double* latency_s, int64_t* out_index, double* out_score);
+rfa_status rfa_server_get_stats(rfa_server* srv, rfa_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,7 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
 
 
 def to_codes(s) -> np.ndarray:
@@ -170,6 +171,10 @@ class Reference:
         L.ref_pairs_u32_mt.restype = ctypes.c_int
         L.ref_pairs_u32_mt.argtypes = [_i32p, _u32p, _u64p, _u32p, _u64p, ctypes.c_size_t, _u64p,
                                        ctypes.c_int]
+        L.ref_extract_ratio_u8_mt.restype = ctypes.c_int
+        L.ref_extract_ratio_u8_mt.argtypes = [_u8p, _u64p, ctypes.c_size_t, _u8p, _u64p, ctypes.c_size_t,
+                                              ctypes.c_double, ctypes.c_size_t, _i64p, _f64p, _f64p,
+                                              ctypes.c_int]
 
     def _err(self):
         return self.lib.ref_last_error().decode()
@@ -224,6 +229,19 @@ class Reference:
         self.lib.ref_pairs_u32_mt(_p(m, _i32p), _p(achars, _u32p), _p(aoff, _u64p), _p(bchars, _u32p),
                                   _p(boff, _u64p), n, _p(out, _u64p), threads)
         return out
+
+
+    def extract_ratio_mt(self, qchars, qoff, cchars, coff, cutoff, k, threads=None):
+        """fuzz::CachedRatio extract() + top-k per query; returns (index, score, seconds per query)."""
+        nq, nc = len(qoff) - 1, len(coff) - 1
+        threads = threads or os.cpu_count() or 1
+        idx = np.zeros(nq * k, dtype=np.int64)
+        sc = np.zeros(nq * k, dtype=np.float64)
+        secs = np.zeros(nq, dtype=np.float64)
+        self.lib.ref_extract_ratio_u8_mt(_p(qchars, _u8p), _p(qoff, _u64p), nq, _p(cchars, _u8p),
+                                         _p(coff, _u64p), nc, cutoff, k, _p(idx, _i64p), _p(sc, _f64p),
+                                         _p(secs, _f64p), threads)
+        return idx.reshape(nq, k), sc.reshape(nq, k), secs
 
 
 def reference_available() -> bool:

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -473,6 +474,43 @@ int ref_pairs_u32_mt(const int32_t* metrics, const uint32_t* a, const uint64_t* 
                 else
                     out[i] = rapidfuzz::indel_distance(a + aoff[i], a + aoff[i + 1], b + boff[i], b + boff[i + 1]);
             }
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nthreads; ++t) ts.emplace_back(worker);
+    for (auto& t : ts) t.join();
+    return 0;
+}
+
+// CPU baseline for serving: the README's extract() loop (README.md:153-214)
+// with fuzz::CachedRatio, then the k best by (score desc, index asc).  One
+// request per thread at a time; seconds[i] is request i's own compute time.
+int ref_extract_ratio_u8_mt(const uint8_t* q, const uint64_t* qoff, size_t nq, const uint8_t* c,
+                            const uint64_t* coff, size_t nc, double cutoff, size_t k, int64_t* out_index,
+                            double* out_score, double* seconds, int nthreads)
+{
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+        std::vector<std::pair<double, size_t>> hits;
+        for (;;) {
+            const size_t i = next.fetch_add(1);
+            if (i >= nq) break;
+            const auto t0 = std::chrono::steady_clock::now();
+            rapidfuzz::fuzz::CachedRatio<uint8_t> scorer(q + qoff[i], q + qoff[i + 1]);
+            hits.clear();
+            for (size_t j = 0; j < nc; ++j) {
+                const double s = scorer.similarity(c + coff[j], c + coff[j + 1], cutoff);
+                if (s >= cutoff) hits.emplace_back(s, j);
+            }
+            const size_t kk = std::min(k, hits.size());
+            std::partial_sort(hits.begin(), hits.begin() + kk, hits.end(), [](const auto& a, const auto& b) {
+                return a.first > b.first || (a.first == b.first && a.second < b.second);
+            });
+            for (size_t r = 0; r < k; ++r) {
+                out_index[i * k + r] = r < kk ? (int64_t)hits[r].second : -1;
+                out_score[i * k + r] = r < kk ? hits[r].first : 0.0;
+            }
+            seconds[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
     };
     std::vector<std::thread> ts;

@@ -21,110 +21,15 @@
 #include <unordered_set>
 #include <vector>
 
-#include "rfa_internal.hpp"
+#include "rfa_host.hpp"
 
-using namespace rfa;
-
-namespace {
+namespace rfa {
 
 thread_local std::string g_err;
 
-rfa_status fail(rfa_status st, const std::string& msg)
-{
-    g_err = msg;
-    return st;
-}
-
-struct HipError : std::runtime_error {
-    hipError_t e;
-    HipError(hipError_t e_, const char* what) : std::runtime_error(std::string(what) + ": " + hipGetErrorString(e_)), e(e_) {}
-};
-
-#define HIP_CHECK(x)                                                                                               \
-    do {                                                                                                           \
-        hipError_t e_ = (x);                                                                                       \
-        if (e_ != hipSuccess) throw HipError(e_, #x);                                                              \
-    } while (0)
-
-struct ApiError : std::runtime_error {
-    rfa_status st;
-    ApiError(rfa_status s, const std::string& m) : std::runtime_error(m), st(s) {}
-};
-
-// Device buffer that grows on demand (kept by the context between calls).
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    void* get(size_t bytes)
-    {
-        if (bytes > cap) {
-            if (p) HIP_CHECK(hipFree(p));
-            p = nullptr;
-            cap = 0;
-            size_t want = std::max<size_t>(bytes, 256);
-            if (hipMalloc(&p, want) != hipSuccess) {
-                (void)hipGetLastError();
-                throw ApiError(RFA_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(want) + " bytes failed");
-            }
-            cap = want;
-        }
-        return p;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-// Host staging of asynchronous uploads: the source vectors stay alive in
-// `keep` until the stream has passed them (see begin_call).
-struct Keep {
-    std::vector<std::shared_ptr<void>> items;
-};
-
-template <typename T>
-T* upload(Keep& keep, DevBuf& b, const std::vector<T>& v, hipStream_t s)
-{
-    T* d = (T*)b.get(std::max<size_t>(v.size(), 1) * sizeof(T));
-    if (!v.empty()) {
-        auto copy = std::make_shared<std::vector<T>>(v);
-        HIP_CHECK(hipMemcpyAsync(d, copy->data(), copy->size() * sizeof(T), hipMemcpyHostToDevice, s));
-        keep.items.push_back(copy);
-    }
-    return d;
-}
-
-uint64_t ceil_div_u64(uint64_t a, uint64_t b) { return a / b + (a % b != 0); }
-
-uint32_t log2_ceil(uint64_t x)
-{
-    uint32_t b = 0;
-    while ((1ull << b) < x) ++b;
-    return b;
-}
-
-} // namespace
-
-struct rfa_context {
-    int device = 0;
-    hipStream_t stream = nullptr;  // all results are ordered on this stream
-    hipStream_t stream2 = nullptr; // second lane for work that overlaps (pairs buckets)
-    hipEvent_t fork = nullptr, join = nullptr;
-    std::mutex mu;
-    int num_cus = 256;
-    size_t lds_per_cu = 160 * 1024;
-    DevBuf queue, pm_scratch, tflag, items_a, items_b, items_idx, items_swap, out_tmp, misc;
-    DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4], plan_keys, plan_vals, plan_temp;
-    size_t lane_arena_bytes_zeroed = 0;
-    uint32_t gen_next = 1;
-    Keep keep; // host sources of in-flight uploads
-};
-
 // Called at the start of every scoring call: releases the host sources of the
 // previous call's uploads once the device has consumed them.
-static void begin_call(rfa_context* ctx)
+void begin_call(rfa_context* ctx)
 {
     if (!ctx->keep.items.empty()) {
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -132,49 +37,6 @@ static void begin_call(rfa_context* ctx)
         ctx->keep.items.clear();
     }
 }
-
-struct rfa_strings {
-    rfa_context* ctx = nullptr;
-    int width = 1;
-    uint64_t count = 0;
-    void* d_chars = nullptr;
-    uint64_t* d_off = nullptr;
-    std::vector<uint8_t> h_chars;
-    std::vector<uint64_t> h_off;
-    uint64_t max_len = 0;
-
-    uint64_t len(uint64_t i) const { return h_off[i + 1] - h_off[i]; }
-    uint32_t unit(uint64_t pos) const
-    {
-        if (width == 1) return h_chars[pos];
-        if (width == 2) return ((const uint16_t*)h_chars.data())[pos];
-        return ((const uint32_t*)h_chars.data())[pos];
-    }
-};
-
-struct TileClass {
-    uint32_t first_tile = 0, ntiles = 0;
-    int word_bits = 64;
-    uint32_t rows_cap = 2;
-};
-
-struct rfa_scorer {
-    rfa_context* ctx = nullptr;
-    int metric = 0;
-    rfa_weights w{1, 1, 1};
-    double prefix_weight = 0.1;
-    rfa_strings* q = nullptr; // owned copy of the queries
-    // tile plan
-    std::vector<uint32_t> tile_order; // query indices, 64 per tile
-    uint32_t* d_tile_order = nullptr;
-    std::vector<TileClass> classes;
-    uint32_t hash_bits = 6;
-    std::vector<uint32_t> block_queries; // scored by the block kernel
-    uint32_t* d_block_queries = nullptr;
-    uint64_t block_max_len = 0;
-};
-
-namespace {
 
 int algo_for(int metric, const rfa_weights& w)
 {
@@ -321,9 +183,39 @@ rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const 
 void free_strings(rfa_strings* s)
 {
     if (!s) return;
-    if (s->d_chars) (void)hipFree(s->d_chars);
-    if (s->d_off) (void)hipFree(s->d_off);
+    if (!s->view) {
+        if (s->d_chars) (void)hipFree(s->d_chars);
+        if (s->d_off) (void)hipFree(s->d_off);
+    }
     delete s;
+}
+
+rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries, const rfa_weights& w,
+                        double prefix_weight)
+{
+    auto* sc = new rfa_scorer();
+    sc->ctx = ctx;
+    sc->metric = metric;
+    sc->w = w;
+    sc->prefix_weight = prefix_weight;
+    try {
+        sc->q = make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
+        plan_scorer(sc);
+    }
+    catch (...) {
+        free_scorer(sc);
+        throw;
+    }
+    return sc;
+}
+
+void free_scorer(rfa_scorer* sc)
+{
+    if (!sc) return;
+    free_strings(sc->q);
+    if (sc->d_tile_order) (void)hipFree(sc->d_tile_order);
+    if (sc->d_block_queries) (void)hipFree(sc->d_block_queries);
+    delete sc;
 }
 
 ScoreSpec make_spec_m(int metric, const rfa_weights& w, double pw, int kind, int semantics, uint64_t cu, double cd,
@@ -343,7 +235,7 @@ ScoreSpec make_spec_m(int metric, const rfa_weights& w, double pw, int kind, int
     s.band_k = ~0ull;
     if (kind == RFA_DISTANCE && is_int_kind(metric, kind) && metric != RFA_LCSSEQ) {
         if (metric == RFA_LEVENSHTEIN)
-            s.band_k = s.ins ? ceil_div_u64(cu, s.ins) : ~0ull;
+            s.band_k = s.ins ? host_ceil_div(cu, s.ins) : ~0ull;
         else
             s.band_k = cu;
     }
@@ -366,7 +258,7 @@ ScoreSpec make_spec(const rfa_scorer* sc, int kind, int semantics, uint64_t cu, 
     s.band_k = ~0ull;
     if (kind == RFA_DISTANCE && is_int_kind(sc->metric, kind) && sc->metric != RFA_LCSSEQ) {
         if (sc->metric == RFA_LEVENSHTEIN)
-            s.band_k = s.ins ? ceil_div_u64(cu, s.ins) : ~0ull;
+            s.band_k = s.ins ? host_ceil_div(cu, s.ins) : ~0ull;
         else
             s.band_k = cu;
     }
@@ -640,28 +532,9 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
     }
 }
 
-template <typename F>
-rfa_status guarded(F&& f)
-{
-    try {
-        f();
-        return RFA_OK;
-    }
-    catch (const ApiError& e) {
-        return fail(e.st, e.what());
-    }
-    catch (const HipError& e) {
-        return fail(RFA_ERR_HIP, e.what());
-    }
-    catch (const std::bad_alloc&) {
-        return fail(RFA_ERR_OUT_OF_MEMORY, "host allocation failed");
-    }
-    catch (const std::exception& e) {
-        return fail(RFA_ERR_INVALID_ARGUMENT, e.what());
-    }
-}
+} // namespace rfa
 
-} // namespace
+using namespace rfa;
 
 extern "C" {
 
@@ -705,7 +578,9 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
                           &ctx->items_swap, &ctx->out_tmp, &ctx->misc, &ctx->lane_arena, &ctx->lane_items,
                           &ctx->lane_swap, &ctx->lane_metric, &ctx->lane_out, &ctx->block_items[0],
                           &ctx->block_items[1], &ctx->block_items[2], &ctx->block_items[3], &ctx->plan_keys,
-                          &ctx->plan_vals, &ctx->plan_temp})
+                          &ctx->plan_vals, &ctx->plan_temp, &ctx->topk_scores, &ctx->topk_run[0],
+                          &ctx->topk_run[1], &ctx->topk_cand_s, &ctx->topk_cand_i, &ctx->topk_counts, &ctx->topk_q,
+                          &ctx->topk_out})
             b->release();
         (void)hipEventDestroy(ctx->fork);
         (void)hipEventDestroy(ctx->join);
@@ -760,23 +635,7 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
                            "Levenshtein weights other than {w,w,w} or Indel-like {w,w,>=2w} are not accelerated");
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_CHECK(hipSetDevice(ctx->device));
-        auto* sc = new rfa_scorer();
-        sc->ctx = ctx;
-        sc->metric = metric;
-        sc->w = w;
-        sc->prefix_weight = prefix_weight;
-        try {
-            sc->q = make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
-            plan_scorer(sc);
-        }
-        catch (...) {
-            free_strings(sc->q);
-            if (sc->d_tile_order) (void)hipFree(sc->d_tile_order);
-            if (sc->d_block_queries) (void)hipFree(sc->d_block_queries);
-            delete sc;
-            throw;
-        }
-        *out = sc;
+        *out = make_scorer(ctx, metric, queries, w, prefix_weight);
     });
 }
 
@@ -787,10 +646,7 @@ rfa_status rfa_scorer_destroy(rfa_scorer* sc)
         std::lock_guard<std::mutex> lk(sc->ctx->mu);
         (void)hipSetDevice(sc->ctx->device);
         (void)hipStreamSynchronize(sc->ctx->stream);
-        free_strings(sc->q);
-        if (sc->d_tile_order) (void)hipFree(sc->d_tile_order);
-        if (sc->d_block_queries) (void)hipFree(sc->d_block_queries);
-        delete sc;
+        free_scorer(sc);
     });
 }
 
@@ -823,8 +679,14 @@ rfa_status rfa_pairs(rfa_context* ctx, const rfa_strings* a, const rfa_strings* 
 rfa_status rfa_topk(rfa_scorer* sc, const rfa_strings* choices, int kind, double cutoff_d, uint32_t k,
                     int64_t* out_index, double* out_score)
 {
-    (void)sc, (void)choices, (void)kind, (void)cutoff_d, (void)k, (void)out_index, (void)out_score;
-    return fail(RFA_ERR_UNSUPPORTED, "rfa_topk: not built yet");
+    if (!sc || !choices) return fail(RFA_ERR_INVALID_ARGUMENT, "scorer or choices is NULL");
+    if (k && (!out_index || !out_score) && sc->q->count) return fail(RFA_ERR_INVALID_ARGUMENT, "outputs are NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(sc->ctx->mu);
+        HIP_CHECK(hipSetDevice(sc->ctx->device));
+        begin_call(sc->ctx);
+        topk_impl(sc, choices, kind, cutoff_d, k, out_index, out_score);
+    });
 }
 
 } // extern "C"

@@ -1,0 +1,216 @@
+// rfa_host.hpp -- host-side state shared by the C ABI translation units:
+// error plumbing, growable device buffers, and the definitions of the opaque
+// handle types declared in rapidfuzz_amd.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rfa_internal.hpp"
+
+namespace rfa {
+
+extern thread_local std::string g_err;
+
+inline rfa_status fail(rfa_status st, const std::string& msg)
+{
+    g_err = msg;
+    return st;
+}
+
+struct HipError : std::runtime_error {
+    hipError_t e;
+    HipError(hipError_t e_, const char* what) : std::runtime_error(std::string(what) + ": " + hipGetErrorString(e_)), e(e_) {}
+};
+
+#define HIP_CHECK(x)                                                                                               \
+    do {                                                                                                           \
+        hipError_t e_ = (x);                                                                                       \
+        if (e_ != hipSuccess) throw ::rfa::HipError(e_, #x);                                                       \
+    } while (0)
+
+struct ApiError : std::runtime_error {
+    rfa_status st;
+    ApiError(rfa_status s, const std::string& m) : std::runtime_error(m), st(s) {}
+};
+
+// Device buffer that grows on demand (kept by its owner between calls).
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes)
+    {
+        if (bytes > cap) {
+            if (p) HIP_CHECK(hipFree(p));
+            p = nullptr;
+            cap = 0;
+            size_t want = std::max<size_t>(bytes, 256);
+            if (hipMalloc(&p, want) != hipSuccess) {
+                (void)hipGetLastError();
+                throw ApiError(RFA_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(want) + " bytes failed");
+            }
+            cap = want;
+        }
+        return p;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Host staging of asynchronous uploads: the source vectors stay alive until
+// the stream has passed them (see begin_call).
+struct Keep {
+    std::vector<std::shared_ptr<void>> items;
+};
+
+template <typename T>
+T* upload(Keep& keep, DevBuf& b, const std::vector<T>& v, hipStream_t s)
+{
+    T* d = (T*)b.get(std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (!v.empty()) {
+        auto copy = std::make_shared<std::vector<T>>(v);
+        HIP_CHECK(hipMemcpyAsync(d, copy->data(), copy->size() * sizeof(T), hipMemcpyHostToDevice, s));
+        keep.items.push_back(copy);
+    }
+    return d;
+}
+
+inline uint64_t host_ceil_div(uint64_t a, uint64_t b) { return a / b + (a % b != 0); }
+
+inline uint32_t log2_ceil(uint64_t x)
+{
+    uint32_t b = 0;
+    while ((1ull << b) < x) ++b;
+    return b;
+}
+
+template <typename F>
+rfa_status guarded(F&& f)
+{
+    try {
+        f();
+        return RFA_OK;
+    }
+    catch (const ApiError& e) {
+        return fail(e.st, e.what());
+    }
+    catch (const HipError& e) {
+        return fail(RFA_ERR_HIP, e.what());
+    }
+    catch (const std::bad_alloc&) {
+        return fail(RFA_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    catch (const std::exception& e) {
+        return fail(RFA_ERR_INVALID_ARGUMENT, e.what());
+    }
+}
+
+// Length-sorted, lane-interleaved copy of a string set for top-k scoring
+// (1-byte units).  Built once per set on first use.
+struct ServeIndex {
+    uint32_t ngroups = 0;
+    std::vector<uint32_t> glen_min, glen_max; // per group, over real slots
+    uint32_t* d_data = nullptr;
+    uint64_t* d_base = nullptr;
+    uint32_t* d_dwords = nullptr;
+    uint32_t* d_glen_min = nullptr;
+    uint32_t* d_slot_len = nullptr;
+    uint32_t* d_slot_idx = nullptr;
+    uint64_t data_bytes = 0;
+    ~ServeIndex();
+};
+
+} // namespace rfa
+
+struct rfa_context {
+    int device = 0;
+    hipStream_t stream = nullptr;  // all results are ordered on this stream
+    hipStream_t stream2 = nullptr; // second lane for work that overlaps (pairs buckets)
+    hipEvent_t fork = nullptr, join = nullptr;
+    std::mutex mu;
+    int num_cus = 256;
+    size_t lds_per_cu = 160 * 1024;
+    rfa::DevBuf queue, pm_scratch, tflag, items_a, items_b, items_idx, items_swap, out_tmp, misc;
+    rfa::DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4], plan_keys, plan_vals,
+        plan_temp;
+    rfa::DevBuf topk_scores, topk_run[2], topk_cand_s, topk_cand_i, topk_counts, topk_q, topk_out;
+    size_t lane_arena_bytes_zeroed = 0;
+    uint32_t gen_next = 1;
+    rfa::Keep keep; // host sources of in-flight uploads
+};
+
+struct rfa_strings {
+    rfa_context* ctx = nullptr;
+    int width = 1;
+    uint64_t count = 0;
+    void* d_chars = nullptr;
+    uint64_t* d_off = nullptr;
+    std::vector<uint8_t> h_chars;
+    std::vector<uint64_t> h_off;
+    uint64_t max_len = 0;
+    bool view = false; // borrows d_chars/d_off of another set
+    std::shared_ptr<rfa::ServeIndex> index; // built on first top-k use
+
+    uint64_t len(uint64_t i) const { return h_off[i + 1] - h_off[i]; }
+    uint32_t unit(uint64_t pos) const
+    {
+        if (width == 1) return h_chars[pos];
+        if (width == 2) return ((const uint16_t*)h_chars.data())[pos];
+        return ((const uint32_t*)h_chars.data())[pos];
+    }
+};
+
+namespace rfa {
+
+struct TileClass {
+    uint32_t first_tile = 0, ntiles = 0;
+    int word_bits = 64;
+    uint32_t rows_cap = 2;
+};
+
+} // namespace rfa
+
+struct rfa_scorer {
+    rfa_context* ctx = nullptr;
+    int metric = 0;
+    rfa_weights w{1, 1, 1};
+    double prefix_weight = 0.1;
+    rfa_strings* q = nullptr; // owned copy of the queries
+    // tile plan
+    std::vector<uint32_t> tile_order; // query indices, 64 per tile
+    uint32_t* d_tile_order = nullptr;
+    std::vector<rfa::TileClass> classes;
+    uint32_t hash_bits = 6;
+    std::vector<uint32_t> block_queries; // scored by the block kernel
+    uint32_t* d_block_queries = nullptr;
+    uint64_t block_max_len = 0;
+};
+
+namespace rfa {
+
+void begin_call(rfa_context* ctx);
+int algo_for(int metric, const rfa_weights& w);
+bool is_int_kind(int metric, int kind);
+rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const uint64_t* off, uint64_t count);
+void free_strings(rfa_strings* s);
+rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries, const rfa_weights& w,
+                        double prefix_weight);
+void free_scorer(rfa_scorer* sc);
+void validate_kind(int metric, int kind, int semantics);
+void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
+                int out_on_device);
+
+// rfa_topk.cpp
+void topk_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, double cutoff, uint32_t k, int64_t* out_index,
+               double* out_score);
+
+} // namespace rfa

@@ -97,6 +97,54 @@ hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t
                        uint64_t* keys_in, uint64_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp,
                        size_t temp_bytes, uint32_t* counters, hipStream_t s);
 
+// Top-k scoring against a resident choice index (rfa_serve.hip).  Choices are
+// sorted by length into groups of 64; group g holds gdwords[g] dwords per lane
+// at gdata[gbase[g] + d * 64 + lane].  Queries [q_begin, q_end) of the batch
+// are split into subsets of serve_queries_per_block(wq) (grid y); query q
+// visits groups [qglo[q], qghi[q]), subset s the union [sub_glo[s], sub_ghi[s]).
+struct ServeArgs {
+    const uint32_t* gdata;
+    const uint64_t* gbase;
+    const uint32_t* gdwords;
+    const uint32_t* glen_min;
+    const uint32_t* slot_len;
+    const uint32_t* slot_idx; // 0xffffffff: padding slot
+    const uint8_t* qchars;
+    const uint64_t* qoff;
+    const uint32_t* qglo;
+    const uint32_t* qghi;
+    const uint32_t* sub_glo;
+    const uint32_t* sub_ghi;
+    uint32_t q_begin, q_end;
+    ScoreSpec spec;
+    double cutoff; // candidates: score >= cutoff
+    uint32_t* counts;
+    double* cand_score;
+    int64_t* cand_index;
+    uint32_t cap; // candidates kept per query (counts may exceed it)
+};
+
+// Per-query selection of the k best (score desc, index asc) entries with
+// score >= cutoff from a candidate list and/or a dense score row.
+struct SelectArgs {
+    const double* lscore; // list source: row q at q * lstride
+    const int64_t* lindex; // entries with index < 0 are skipped
+    const uint32_t* lcount; // entries per row (nullptr: lfixed)
+    uint32_t lfixed, lcap;
+    uint64_t lstride;
+    const double* dscore; // dense source: row q at q * dstride, dn entries, index dbase + t
+    uint64_t dstride, dn;
+    int64_t dbase;
+    double cutoff;
+    uint32_t k;
+    double* out_score; // [nq][k]; missing entries get score 0, index -1
+    int64_t* out_index;
+};
+
+uint32_t serve_queries_per_block(uint32_t wq);
+hipError_t launch_serve(int algo, uint32_t wq, const ServeArgs& a, uint32_t gx, uint32_t nsub, hipStream_t s);
+hipError_t launch_select(const SelectArgs& a, uint32_t nq, hipStream_t s);
+
 size_t lane_arena_words(uint32_t maxw);
 hipError_t launch_lane(int char_width, int maxw, const LaneArgs& a, uint32_t nblocks, hipStream_t s);
 hipError_t launch_lane_short(int char_width, int word_bits, const LaneArgs& a, uint32_t nblocks, hipStream_t s);

@@ -26,7 +26,9 @@ _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED",
 # every symbol include/rapidfuzz_amd.h declares
 EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
             "rfa_context_synchronize", "rfa_context_stream", "rfa_strings_create", "rfa_strings_destroy",
-            "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_pairs", "rfa_topk"]
+            "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_pairs", "rfa_topk",
+            "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
+            "rfa_server_get_stats"]
 
 
 class RfaError(RuntimeError):
@@ -42,6 +44,17 @@ class InvalidArgument(RfaError, ValueError):
 class Weights(ctypes.Structure):
     _fields_ = [("insert_cost", ctypes.c_uint64), ("delete_cost", ctypes.c_uint64),
                 ("replace_cost", ctypes.c_uint64)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("metric", ctypes.c_int), ("kind", ctypes.c_int), ("score_cutoff", ctypes.c_double),
+                ("k", ctypes.c_uint32), ("max_batch", ctypes.c_uint32), ("slots", ctypes.c_uint32),
+                ("candidate_cap", ctypes.c_uint32), ("prefix_weight", ctypes.c_double), ("weights", Weights)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("max_batch_seen", ctypes.c_uint64),
+                ("dense_fallbacks", ctypes.c_uint64)]
 
 
 _vp = ctypes.c_void_p
@@ -74,9 +87,16 @@ def load_library(path: str = LIB_PATH):
     L.rfa_pairs.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                             _vp, ctypes.c_int]
     L.rfa_topk.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp]
+    L.rfa_server_create.argtypes = [_vp, _vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
+    L.rfa_server_destroy.argtypes = [_vp]
+    L.rfa_server_submit.argtypes = [_vp, _vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    L.rfa_server_wait.argtypes = [_vp, ctypes.c_uint64, _vp, _vp]
+    L.rfa_server_replay.argtypes = [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp]
+    L.rfa_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     for name in ("rfa_context_create", "rfa_context_destroy", "rfa_context_synchronize", "rfa_strings_create",
                  "rfa_strings_destroy", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_pairs",
-                 "rfa_topk"):
+                 "rfa_topk", "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait",
+                 "rfa_server_replay", "rfa_server_get_stats"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -237,11 +257,72 @@ class Scorer:
         _check(_lib.rfa_cdist(self.h, choices.h, kind, semantics, cu, cd, out.ctypes.data, 0))
         return out
 
-    def topk(self, choices: Strings, k: int, kind=SIMILARITY, cutoff_d=0.0):
+    def topk(self, choices: Strings, k: int, kind=None, cutoff_d=0.0):
+        """(index, score), each [nq, k]: the extract() results >= cutoff_d ordered by
+        (score desc, index asc); missing entries are index -1, score 0."""
+        if kind is None:
+            kind = SIMILARITY if self.metric in (JARO, JARO_WINKLER, RATIO) else NORMALIZED_SIMILARITY
         idx = np.zeros((self.nq, k), dtype=np.int64)
         sc = np.zeros((self.nq, k), dtype=np.float64)
         _check(_lib.rfa_topk(self.h, choices.h, kind, cutoff_d, k, idx.ctypes.data, sc.ctypes.data))
         return idx, sc
+
+
+class Server:
+    """Continuous-batching top-k request server over a resident choice set."""
+
+    def __init__(self, ctx: Context, choices: Strings, metric=RATIO, kind=None, score_cutoff=0.0, k=5,
+                 max_batch=64, slots=4, candidate_cap=65536, prefix_weight=0.1, weights=(1, 1, 1)):
+        if kind is None:
+            kind = SIMILARITY if metric in (JARO, JARO_WINKLER, RATIO) else NORMALIZED_SIMILARITY
+        self.ctx = ctx
+        self.k = k
+        self.width = choices.width
+        cfg = ServerConfig(metric, kind, score_cutoff, k, max_batch, slots, candidate_cap, prefix_weight,
+                           Weights(*weights))
+        h = _vp()
+        _check(_lib.rfa_server_create(ctx.h, choices.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h and self.ctx.h:
+            _check(_lib.rfa_server_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, query) -> int:
+        q = np.ascontiguousarray(_as_units(query, self.width))
+        t = ctypes.c_uint64()
+        _check(_lib.rfa_server_submit(self.h, q.ctypes.data, q.size, ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        idx = np.zeros(self.k, dtype=np.int64)
+        sc = np.zeros(self.k, dtype=np.float64)
+        _check(_lib.rfa_server_wait(self.h, ticket, idx.ctypes.data, sc.ctypes.data))
+        return idx, sc
+
+    def replay(self, queries, arrival_s):
+        """Submits queries[i] at arrival_s[i] seconds; returns (latency_s, index, score)."""
+        chars, off, _ = pack(queries, self.width)
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        n = len(off) - 1
+        lat = np.zeros(n, dtype=np.float64)
+        idx = np.zeros((n, self.k), dtype=np.int64)
+        sc = np.zeros((n, self.k), dtype=np.float64)
+        _check(_lib.rfa_server_replay(self.h, chars.ctypes.data, off.ctypes.data, n, arr.ctypes.data,
+                                      lat.ctypes.data, idx.ctypes.data, sc.ctypes.data))
+        return lat, idx, sc
+
+    def stats(self):
+        st = ServerStats()
+        _check(_lib.rfa_server_get_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
 
 
 def pairs(ctx: Context, a: Strings, b: Strings, metrics, kind=DISTANCE, cutoff_u=INT64_MAX, cutoff_d=1.0,
