@@ -360,12 +360,33 @@ def workload_serve(args, emit=True):
     return line
 
 
+def workload_copy(args):
+    """Calibration for the HBM counters: a plain device copy of 1 GiB (torch's elementwise copy
+    kernel), so FETCH_SIZE / WRITE_SIZE per dispatch can be compared with 1 GiB read + 1 GiB written."""
+    import torch
+
+    n = 1 << 28  # float32 elements = 1 GiB
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"workload": "copy 1 GiB", "ms": round(dt * 1e3, 3), "GBps": round(2 * n * 4 / dt / 1e9, 1)}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="cdist", choices=["cdist", "uneven", "serve"],
+    ap.add_argument("--workload", default="cdist", choices=["cdist", "uneven", "serve", "copy"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
-                         "uneven: 1M independent pairs; serve: the serving leg alone")
+                         "uneven: 1M independent pairs; serve: the serving leg alone; copy: HBM counter "
+                         "calibration (a plain 1 GiB device copy)")
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -384,6 +405,8 @@ def main():
         return workload_uneven(args)
     if args.workload == "serve":
         return workload_serve(args)
+    if args.workload == "copy":
+        return workload_copy(args)
 
     import rapidfuzz_amd as rfa
     from rapidfuzz_amd import hip

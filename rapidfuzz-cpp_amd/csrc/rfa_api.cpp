@@ -362,7 +362,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
                 uint32_t target_blocks = (uint32_t)ctx->num_cus * 12;
                 uint32_t nbx = std::max<uint32_t>(1, (target_blocks + tc.ntiles - 1) / tc.ntiles);
                 uint32_t cpb = std::max<uint32_t>(64, (ta.ncsel + nbx - 1) / nbx);
-                cpb = ((cpb + 3) / 4) * 4;
+                cpb = ((cpb + 63) / 64) * 64; // whole 8-choice runs for each of the 8 waves
                 ta.choices_per_block = cpb;
                 nbx = (ta.ncsel + cpb - 1) / cpb;
                 HIP_CHECK(launch_tile(algo, width, tc.word_bits, ta, tc.ntiles, nbx, s));

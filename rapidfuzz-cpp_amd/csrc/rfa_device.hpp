@@ -53,9 +53,10 @@ struct HashMap {
     }
 };
 
-// Exclusive prefix sum of one value per thread over a 256-thread block.
-// `tmp` holds at least 4 uint32 of LDS.  Returns the exclusive sum; *total gets the block sum.
-__device__ __forceinline__ uint32_t block_exclusive_scan256(uint32_t v, uint32_t* tmp, uint32_t* total)
+// Exclusive prefix sum of one value per thread over a block of NW waves.
+// `tmp` holds at least NW uint32 of LDS.  Returns the exclusive sum; *total gets the block sum.
+template <uint32_t NW>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* tmp, uint32_t* total)
 {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = v;
@@ -67,7 +68,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan256(uint32_t v, uint32_t
     if (lane == 63) tmp[wave] = x;
     __syncthreads();
     uint32_t base = 0, tot = 0;
-    for (uint32_t w = 0; w < 4; ++w) {
+    for (uint32_t w = 0; w < NW; ++w) {
         uint32_t t = tmp[w];
         if (w < wave) base += t;
         tot += t;

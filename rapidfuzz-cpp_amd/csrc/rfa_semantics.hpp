@@ -333,4 +333,16 @@ __device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64
     }
 }
 
+// The same value as its 64 result bits (uint64 or the double's bit pattern).
+__device__ __forceinline__ uint64_t finalize_bits(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r)
+{
+    union {
+        uint64_t u;
+        double d;
+    } v;
+    v.u = 0;
+    finalize(s, l1, l2, r, &v, 0);
+    return v.u;
+}
+
 } // namespace rfa

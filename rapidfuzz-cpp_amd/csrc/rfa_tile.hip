@@ -43,6 +43,11 @@ struct SymLookup<uint32_t> {
     __device__ __forceinline__ uint32_t operator()(uint32_t c) const { return map.find(c); }
 };
 
+// Threads of a tile workgroup: 8 waves share one tile's PM table, so a CU
+// holds twice the waves a 4-wave workgroup would at the same LDS footprint.
+constexpr uint32_t TILE_WAVES = 8;
+constexpr uint32_t TILE_THREADS = TILE_WAVES * 64;
+
 // Builds the tile's PM rows in LDS.  Returns the lookup functor.
 template <typename CharT, typename W>
 __device__ __forceinline__ SymLookup<CharT> build_tile_pm(const CharT* __restrict__ q, uint64_t qo, uint32_t m,
@@ -54,15 +59,15 @@ __device__ __forceinline__ SymLookup<CharT> build_tile_pm(const CharT* __restric
     if constexpr (sizeof(CharT) == 1) {
         uint32_t* flags = (uint32_t*)extra;         // 256
         uint8_t* rowmap = (uint8_t*)(flags + 256);  // 256
-        uint32_t* tmp = (uint32_t*)(rowmap + 256);  // 4
-        flags[tid] = 0;
+        uint32_t* tmp = (uint32_t*)(rowmap + 256);  // TILE_WAVES
+        if (tid < 256) flags[tid] = 0;
         __syncthreads();
-        for (uint32_t j = wave; j < m; j += 4) flags[q[qo + j]] = 1;
+        for (uint32_t j = wave; j < m; j += TILE_WAVES) flags[q[qo + j]] = 1;
         __syncthreads();
-        uint32_t f = flags[tid];
+        uint32_t f = tid < 256 ? flags[tid] : 0u;
         uint32_t total;
-        uint32_t ex = block_exclusive_scan256(f, tmp, &total);
-        rowmap[tid] = f ? (uint8_t)(ex + 1) : 0;
+        uint32_t ex = block_exclusive_scan<TILE_WAVES>(f, tmp, &total);
+        if (tid < 256) rowmap[tid] = f ? (uint8_t)(ex + 1) : 0;
         R = total + 1;
         look.rowmap = rowmap;
     }
@@ -71,17 +76,17 @@ __device__ __forceinline__ SymLookup<CharT> build_tile_pm(const CharT* __restric
         uint64_t* keys = (uint64_t*)extra;
         uint16_t* rows = (uint16_t*)(keys + H);
         uint32_t* tmp = (uint32_t*)(rows + H + 8);
-        for (uint32_t i = tid; i < H; i += 256) keys[i] = EMPTY_KEY;
+        for (uint32_t i = tid; i < H; i += TILE_THREADS) keys[i] = EMPTY_KEY;
         __syncthreads();
         HashMap map{keys, rows, a.hash_bits};
-        for (uint32_t j = wave; j < m; j += 4) map.insert((uint32_t)q[qo + j]);
+        for (uint32_t j = wave; j < m; j += TILE_WAVES) map.insert((uint32_t)q[qo + j]);
         __syncthreads();
         uint32_t base = 0;
-        for (uint32_t s0 = 0; s0 < H; s0 += 256) {
+        for (uint32_t s0 = 0; s0 < H; s0 += TILE_THREADS) {
             uint32_t i = s0 + tid;
             uint32_t f = (i < H && keys[i] != EMPTY_KEY) ? 1u : 0u;
             uint32_t total;
-            uint32_t ex = block_exclusive_scan256(f, tmp, &total);
+            uint32_t ex = block_exclusive_scan<TILE_WAVES>(f, tmp, &total);
             if (f) rows[i] = (uint16_t)(base + ex + 1);
             base += total;
         }
@@ -89,9 +94,9 @@ __device__ __forceinline__ SymLookup<CharT> build_tile_pm(const CharT* __restric
         look.map = map;
     }
     __syncthreads();
-    for (uint32_t i = tid; i < R * 64; i += 256) pm[i] = 0;
+    for (uint32_t i = tid; i < R * 64; i += TILE_THREADS) pm[i] = 0;
     __syncthreads();
-    for (uint32_t j = wave; j < m; j += 4) {
+    for (uint32_t j = wave; j < m; j += TILE_WAVES) {
         uint32_t r = look((uint32_t)q[qo + j]);
         if constexpr (sizeof(W) == 8)
             atomicOr((unsigned long long*)&pm[r * 64 + lane], 1ull << j);
@@ -103,7 +108,7 @@ __device__ __forceinline__ SymLookup<CharT> build_tile_pm(const CharT* __restric
 }
 
 template <int ALGO, typename CharT, typename W>
-__global__ __launch_bounds__(256) void k_tile(TileArgs a)
+__global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
 {
     extern __shared__ uint64_t smem_u64[];
     W* pm = (W*)smem_u64;
@@ -125,91 +130,125 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
     const uint32_t it0 = blockIdx.x * a.choices_per_block;
     const uint32_t it1 = min(a.ncsel, it0 + a.choices_per_block);
 
-    for (uint32_t it = it0 + wave; it < it1; it += 4) {
-        const uint32_t cidx = uniform(a.corder ? a.corder[it] : it);
-        const uint64_t co0 = a.coff[cidx];
-        const uint32_t n = uniform((uint32_t)(a.coff[cidx + 1] - co0));
-        const uint64_t co = ((uint64_t)uniform((uint32_t)(co0 >> 32)) << 32) | uniform((uint32_t)co0);
-        RawResult res{0, 0, 0};
+    // Each wave takes runs of G consecutive choices.  With the full choice set,
+    // lane l keeps its G results for query row l in registers and stores them
+    // as one contiguous 8*G-byte row segment: the output goes to HBM in whole
+    // lines instead of one 8-byte word per row per choice.  A choice subset
+    // (corder) writes each result as it is produced.
+    constexpr uint32_t G = 8;
+    const bool direct = a.corder != nullptr;
+    for (uint32_t g0 = it0 + uniform(wave) * G; g0 < it1; g0 += TILE_WAVES * G) {
+        const uint32_t gn = min(G, it1 - g0);
+        uint64_t outv[G];
+        for (uint32_t t = 0; t < gn; ++t) {
+            const uint32_t it = g0 + t;
+            const uint32_t cidx = uniform(direct ? a.corder[it] : it);
+            const uint64_t co0 = a.coff[cidx];
+            const uint32_t n = uniform((uint32_t)(a.coff[cidx + 1] - co0));
+            const uint64_t co = ((uint64_t)uniform((uint32_t)(co0 >> 32)) << 32) | uniform((uint32_t)co0);
+            RawResult res{0, 0, 0};
 
-        if constexpr (ALGO == ALGO_LEV || ALGO == ALGO_OSA || ALGO == ALGO_LCS) {
-            // state as 32-bit halves (h* unused for 32-bit words)
-            uint32_t VPl = ~0u, VPh = ~0u, VNl = 0, VNh = 0, D0l = 0, D0h = 0, PMl = 0, PMh = 0;
-            const W* __restrict__ col = pm + lane;
-            for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-                uint32_t row = 0;
-                if (k0 + lane < n) row = look((uint32_t)c[co + k0 + lane]);
-                const uint32_t cnt = min(64u, n - k0);
-                W Xn = col[readlane(row, 0) * 64];
-                for (uint32_t i = 0; i < cnt; ++i) {
-                    const W X = Xn;
-                    Xn = col[readlane(row, min(i + 1, 63u)) * 64]; // prefetch the next row
-                    if constexpr (sizeof(W) == 4) {
-                        if constexpr (ALGO == ALGO_LEV)
-                            lev_step((uint32_t)X, VPl, VNl);
-                        else if constexpr (ALGO == ALGO_OSA)
-                            osa_step((uint32_t)X, VPl, VNl, D0l, PMl);
-                        else
-                            lcs_step((uint32_t)X, VPl);
-                    }
-                    else {
-                        if constexpr (ALGO == ALGO_LEV)
-                            lev_step((uint64_t)X, VPl, VPh, VNl, VNh);
-                        else if constexpr (ALGO == ALGO_OSA)
-                            osa_step((uint64_t)X, VPl, VPh, VNl, VNh, D0l, D0h, PMl, PMh);
-                        else
-                            lcs_step((uint64_t)X, VPl, VPh);
+            if constexpr (ALGO == ALGO_LEV || ALGO == ALGO_OSA || ALGO == ALGO_LCS) {
+                // state as 32-bit halves (h* unused for 32-bit words)
+                uint32_t VPl = ~0u, VPh = ~0u, VNl = 0, VNh = 0, D0l = 0, D0h = 0, PMl = 0, PMh = 0;
+                const W* __restrict__ col = pm + lane;
+                for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+                    uint32_t row = 0;
+                    if (k0 + lane < n) row = look((uint32_t)c[co + k0 + lane]);
+                    const uint32_t cnt = uniform(min(64u, n - k0));
+                    W Xn = col[readlane(row, 0) * 64];
+                    for (uint32_t i = 0; i < cnt; ++i) {
+                        const W X = Xn;
+                        Xn = col[readlane(row, min(i + 1, 63u)) * 64]; // prefetch the next row
+                        if constexpr (sizeof(W) == 4) {
+                            if constexpr (ALGO == ALGO_LEV)
+                                lev_step((uint32_t)X, VPl, VNl);
+                            else if constexpr (ALGO == ALGO_OSA)
+                                osa_step((uint32_t)X, VPl, VNl, D0l, PMl);
+                            else
+                                lcs_step((uint32_t)X, VPl);
+                        }
+                        else {
+                            if constexpr (ALGO == ALGO_LEV)
+                                lev_step((uint64_t)X, VPl, VPh, VNl, VNh);
+                            else if constexpr (ALGO == ALGO_OSA)
+                                osa_step((uint64_t)X, VPl, VPh, VNl, VNh, D0l, D0h, PMl, PMh);
+                            else
+                                lcs_step((uint64_t)X, VPl, VPh);
+                        }
                     }
                 }
-            }
-            const W VP = (sizeof(W) == 8) ? (W)(((uint64_t)VPh << 32) | VPl) : (W)VPl;
-            const W VN = (sizeof(W) == 8) ? (W)(((uint64_t)VNh << 32) | VNl) : (W)VNl;
-            if constexpr (ALGO == ALGO_LCS) {
-                res.v = (uint64_t)popc<W>(~VP); // S is kept in the VP registers
+                const W VP = (sizeof(W) == 8) ? (W)(((uint64_t)VPh << 32) | VPl) : (W)VPl;
+                const W VN = (sizeof(W) == 8) ? (W)(((uint64_t)VNh << 32) | VNl) : (W)VNl;
+                if constexpr (ALGO == ALGO_LCS) {
+                    res.v = (uint64_t)popc<W>(~VP); // S is kept in the VP registers
+                }
+                else {
+                    // D[m][n] = D[0][n] + sum of the final column's vertical deltas
+                    const W M = (m >= BITS) ? ~(W)0 : (((W)1 << m) - 1);
+                    res.v = m ? (uint64_t)n + popc<W>(VP & M) - popc<W>(VN & M) : n;
+                }
             }
             else {
-                // D[m][n] = D[0][n] + sum of the final column's vertical deltas
-                const W M = (m >= BITS) ? ~(W)0 : (((W)1 << m) - 1);
-                res.v = m ? (uint64_t)n + popc<W>(VP & M) - popc<W>(VN & M) : n;
-            }
-        }
-        else {
-            // Jaro word path; the host routes only choices with n <= 64 here.
-            uint32_t row = 0;
-            if (lane < n) row = look((uint32_t)c[co + lane]);
-            uint32_t Bound = max(m, n) / 2;
-            if (Bound > 0) Bound--;
-            W BoundMask = (Bound + 1 >= BITS) ? ~(W)0 : (((W)1 << (Bound + 1)) - 1);
-            W Pf = 0;
-            uint64_t Tf = 0;
-            uint32_t pfx = 0;
-            const uint32_t maxpfx = min(min(m, n), 4u);
-            for (uint32_t j = 0; j < n; ++j) {
-                const uint32_t r = readlane(row, j);
-                const W PMr = pm[r * 64 + lane];
-                if (j < maxpfx && pfx == j && ((PMr >> j) & 1)) pfx = j + 1;
-                const W x = PMr & BoundMask & ~Pf;
-                Pf |= x & ((W)0 - x);
-                Tf |= (uint64_t)(x != 0) << j;
-                BoundMask = (j < Bound) ? (W)((BoundMask << 1) | 1) : (W)(BoundMask << 1);
-            }
-            // count_transpositions_word (Jaro_impl.hpp:258-276), walked in text order
-            uint32_t tr = 0;
-            W P2 = Pf;
-            for (uint32_t j = 0; j < n; ++j) {
-                const uint32_t r = readlane(row, j);
-                const W PMr = pm[r * 64 + lane];
-                if ((Tf >> j) & 1) {
-                    const W pmask = P2 & ((W)0 - P2);
-                    tr += (PMr & pmask) == 0;
-                    P2 ^= pmask;
+                // Jaro word path; the host routes only choices with n <= 64 here.
+                uint32_t row = 0;
+                if (lane < n) row = look((uint32_t)c[co + lane]);
+                uint32_t Bound = max(m, n) / 2;
+                if (Bound > 0) Bound--;
+                W BoundMask = (Bound + 1 >= BITS) ? ~(W)0 : (((W)1 << (Bound + 1)) - 1);
+                W Pf = 0;
+                uint64_t Tf = 0;
+                uint32_t pfx = 0;
+                const uint32_t maxpfx = min(min(m, n), 4u);
+                for (uint32_t j = 0; j < n; ++j) {
+                    const uint32_t r = readlane(row, j);
+                    const W PMr = pm[r * 64 + lane];
+                    if (j < maxpfx && pfx == j && ((PMr >> j) & 1)) pfx = j + 1;
+                    const W x = PMr & BoundMask & ~Pf;
+                    Pf |= x & ((W)0 - x);
+                    Tf |= (uint64_t)(x != 0) << j;
+                    BoundMask = (j < Bound) ? (W)((BoundMask << 1) | 1) : (W)(BoundMask << 1);
                 }
+                // count_transpositions_word (Jaro_impl.hpp:258-276), walked in text order
+                uint32_t tr = 0;
+                W P2 = Pf;
+                for (uint32_t j = 0; j < n; ++j) {
+                    const uint32_t r = readlane(row, j);
+                    const W PMr = pm[r * 64 + lane];
+                    if ((Tf >> j) & 1) {
+                        const W pmask = P2 & ((W)0 - P2);
+                        tr += (PMr & pmask) == 0;
+                        P2 ^= pmask;
+                    }
+                }
+                res.v = (uint64_t)popc<W>(Pf);
+                res.tr = tr;
+                res.prefix = pfx;
             }
-            res.v = (uint64_t)popc<W>(Pf);
-            res.tr = tr;
-            res.prefix = pfx;
+            if (direct) {
+                if (qact) finalize(a.spec, m, n, res, a.out, (uint64_t)qidx * a.out_stride + cidx);
+            }
+            else {
+                outv[t] = finalize_bits(a.spec, m, n, res);
+            }
         }
-        if (qact) finalize(a.spec, m, n, res, a.out, (uint64_t)qidx * a.out_stride + cidx);
+        if (!direct && qact) {
+            uint64_t* dst = (uint64_t*)a.out + (uint64_t)qidx * a.out_stride + g0;
+            if (gn == G && ((uintptr_t)dst & 15) == 0) {
+                uint4* d4 = (uint4*)dst;
+#pragma unroll
+                for (uint32_t t = 0; t < G; t += 2)
+                    d4[t / 2] = make_uint4((uint32_t)outv[t], (uint32_t)(outv[t] >> 32), (uint32_t)outv[t + 1],
+                                           (uint32_t)(outv[t + 1] >> 32));
+            }
+            else if (gn == G) {
+#pragma unroll
+                for (uint32_t t = 0; t < G; ++t) dst[t] = outv[t];
+            }
+            else {
+                for (uint32_t t = 0; t < gn; ++t) dst[t] = outv[t];
+            }
+        }
     }
 }
 
@@ -217,7 +256,7 @@ template <int ALGO, typename CharT, typename W>
 static void launch_one(const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x, size_t lds, hipStream_t s)
 {
     dim3 grid(nblocks_x, ntiles);
-    hipLaunchKernelGGL((k_tile<ALGO, CharT, W>), grid, dim3(256), lds, s, a);
+    hipLaunchKernelGGL((k_tile<ALGO, CharT, W>), grid, dim3(TILE_THREADS), lds, s, a);
 }
 
 template <typename CharT, typename W>
@@ -236,9 +275,9 @@ size_t tile_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t
     size_t pm = (size_t)rows_cap * 64 * (word_bits / 8);
     size_t extra;
     if (char_width == 1)
-        extra = 256 * 4 + 256 + 16;
+        extra = 256 * 4 + 256 + 4 * TILE_WAVES;
     else
-        extra = ((size_t)1 << hash_bits) * 10 + 16 + 16;
+        extra = ((size_t)1 << hash_bits) * 10 + 16 + 4 * TILE_WAVES;
     return pm + extra;
 }
 
