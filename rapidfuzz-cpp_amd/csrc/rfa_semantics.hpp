@@ -246,6 +246,21 @@ __device__ __forceinline__ double multi_fp_similarity(const FpPair& p, double c)
 
 // --- dispatch ----------------------------------------------------------------
 
+// Integer-valued kinds: distance / similarity of Levenshtein, Indel, LCSseq, OSA.
+__device__ __forceinline__ uint64_t finalize_int(const ScoreSpec& s, uint64_t l1, uint64_t l2, uint64_t raw)
+{
+    IntPair p{&s, l1, l2, raw};
+    if (s.semantics == RFA_MULTI) {
+        const uint64_t maximum = (s.metric == RFA_INDEL) ? l1 + l2 : u64max(l1, l2);
+        const uint64_t d = multi_full_distance(p);
+        if (s.kind == RFA_DISTANCE) return d <= s.cutoff_u ? d : s.cutoff_u + 1;
+        if (s.metric == RFA_LCSSEQ) return raw >= s.cutoff_u ? raw : 0;
+        const uint64_t sim = maximum - d;
+        return sim >= s.cutoff_u ? sim : 0;
+    }
+    return s.kind == RFA_DISTANCE ? int_api_distance(p, s.cutoff_u) : int_api_similarity(p, s.cutoff_u);
+}
+
 // Writes the reference API value for one pair into out[idx] (uint64 or double).
 __device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r, void* out,
                                          uint64_t idx)
@@ -298,19 +313,14 @@ __device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64
             od[idx] = int_api_norm_similarity(q, s.cutoff_d / 100) * 100;
         return;
     }
+    if (s.kind == RFA_DISTANCE || s.kind == RFA_SIMILARITY) {
+        ou[idx] = finalize_int(s, l1, l2, r.v);
+        return;
+    }
     if (s.semantics == RFA_MULTI) {
         uint64_t maximum = (s.metric == RFA_INDEL) ? l1 + l2 : u64max(l1, l2);
         uint64_t d = multi_full_distance(p);
         switch (s.kind) {
-        case RFA_DISTANCE: ou[idx] = d <= s.cutoff_u ? d : s.cutoff_u + 1; return;
-        case RFA_SIMILARITY:
-            if (s.metric == RFA_LCSSEQ)
-                ou[idx] = p.raw >= s.cutoff_u ? p.raw : 0;
-            else {
-                uint64_t sim = maximum - d;
-                ou[idx] = sim >= s.cutoff_u ? sim : 0;
-            }
-            return;
         case RFA_NORMALIZED_DISTANCE: {
             double norm = maximum ? (double)d / (double)maximum : 0.0;
             od[idx] = norm <= s.cutoff_d ? norm : 1.0;
@@ -325,12 +335,10 @@ __device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64
         }
         }
     }
-    switch (s.kind) {
-    case RFA_DISTANCE: ou[idx] = int_api_distance(p, s.cutoff_u); return;
-    case RFA_SIMILARITY: ou[idx] = int_api_similarity(p, s.cutoff_u); return;
-    case RFA_NORMALIZED_DISTANCE: od[idx] = int_api_norm_distance(p, s.cutoff_d); return;
-    default: od[idx] = int_api_norm_similarity(p, s.cutoff_d); return;
-    }
+    if (s.kind == RFA_NORMALIZED_DISTANCE)
+        od[idx] = int_api_norm_distance(p, s.cutoff_d);
+    else
+        od[idx] = int_api_norm_similarity(p, s.cutoff_d);
 }
 
 // The same value as its 64 result bits (uint64 or the double's bit pattern).

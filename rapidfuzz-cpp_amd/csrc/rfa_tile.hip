@@ -107,7 +107,9 @@ __device__ __forceinline__ SymLookup<CharT> build_tile_pm(const CharT* __restric
     return look;
 }
 
-template <int ALGO, typename CharT, typename W>
+// INTK: the result kind is integer-valued (the integer epilogue alone is
+// compiled in, which keeps the kernel's register footprint small).
+template <int ALGO, typename CharT, typename W, bool INTK>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
 {
     extern __shared__ uint64_t smem_u64[];
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
     // as one contiguous 8*G-byte row segment: the output goes to HBM in whole
     // lines instead of one 8-byte word per row per choice.  A choice subset
     // (corder) writes each result as it is produced.
-    constexpr uint32_t G = 8;
+    constexpr uint32_t G = INTK ? 16 : 4;
     const bool direct = a.corder != nullptr;
     for (uint32_t g0 = it0 + uniform(wave) * G; g0 < it1; g0 += TILE_WAVES * G) {
         const uint32_t gn = min(G, it1 - g0);
@@ -225,11 +227,16 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
                 res.tr = tr;
                 res.prefix = pfx;
             }
+            uint64_t bits;
+            if constexpr (INTK)
+                bits = finalize_int(a.spec, m, n, res.v);
+            else
+                bits = finalize_bits(a.spec, m, n, res);
             if (direct) {
-                if (qact) finalize(a.spec, m, n, res, a.out, (uint64_t)qidx * a.out_stride + cidx);
+                if (qact) ((uint64_t*)a.out)[(uint64_t)qidx * a.out_stride + cidx] = bits;
             }
             else {
-                outv[t] = finalize_bits(a.spec, m, n, res);
+                outv[t] = bits;
             }
         }
         if (!direct && qact) {
@@ -256,7 +263,14 @@ template <int ALGO, typename CharT, typename W>
 static void launch_one(const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x, size_t lds, hipStream_t s)
 {
     dim3 grid(nblocks_x, ntiles);
-    hipLaunchKernelGGL((k_tile<ALGO, CharT, W>), grid, dim3(TILE_THREADS), lds, s, a);
+    const bool intk = (a.spec.kind == RFA_DISTANCE || a.spec.kind == RFA_SIMILARITY) && a.spec.metric <= RFA_OSA;
+    if constexpr (ALGO != ALGO_JARO) {
+        if (intk) {
+            hipLaunchKernelGGL((k_tile<ALGO, CharT, W, true>), grid, dim3(TILE_THREADS), lds, s, a);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_tile<ALGO, CharT, W, false>), grid, dim3(TILE_THREADS), lds, s, a);
 }
 
 template <typename CharT, typename W>
