@@ -96,6 +96,22 @@ def cpu_baseline(qc, qo, cc, co, ql, cl, target_s=12.0):
                       f"reference library not built"}, None, None
 
 
+def read_profile_facts(path):
+    """`- key: value` lines of a committed rocprofv3 summary (numbers measured on the GPU)."""
+    facts = {}
+    try:
+        for line in open(path):
+            if line.startswith("- ") and ":" in line:
+                k, v = line[2:].split(":", 1)
+                try:
+                    facts[k.strip()] = float(v) if "." in v else int(v)
+                except ValueError:
+                    pass
+    except OSError:
+        pass
+    return facts
+
+
 def pack_u8(strs):
     off = np.zeros(len(strs) + 1, dtype=np.uint64)
     np.cumsum([len(s) for s in strs], out=off[1:])
@@ -502,6 +518,7 @@ def main():
             if sc is not lev:
                 sc.close()
 
+    prof = read_profile_facts(os.path.join(ROOT, "profiles", "tile_levenshtein_cdist_4096x65536.md"))
     res = {
         "metric": "Levenshtein 1-vs-N GCUPS + % INT roofline; served p50/p99 latency at QPS",
         "value": round(gcups, 1),
@@ -519,8 +536,17 @@ def main():
                                f"{LMIN}-{LMAX}, CachedLevenshtein distance, no cutoff (single-word path)",
                    "queries": NQ, "choices": NC, "cells": cells, "seed": SEED},
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 3),
-                     "unit": "TOP/s", "frac": round(achieved_tops / peak_tops, 4), "traffic": None,
-                     "ops_per_launch_pair": ops, "kernel": "k_tile<LEV,u8,64/32-bit words>",
+                     "unit": "TOP/s", "frac": round(achieved_tops / peak_tops, 4),
+                     "traffic": prof.get("traffic_bytes_per_step"),
+                     "traffic_unit": "HBM bytes per step (the 64-bit + 32-bit launch pair), from FETCH_SIZE / "
+                                     "WRITE_SIZE scaled by a 1 GiB copy",
+                     "ops_per_launch_pair": ops,
+                     "ops_model": "44 INT32 ops per 64-cell word-step (2 x the 22 64-bit ops of "
+                                  "Levenshtein_impl.hpp:285-302); 32-bit-word tiles count half a word-step",
+                     "kernel": "k_tile<LEV,u8,64/32-bit words,integer epilogue>",
+                     "valu_issue_utilization": {"w64": prof.get("valu_issue_utilization_64"),
+                                                "w32": prof.get("valu_issue_utilization_32"),
+                                                "source": "profiles/tile_levenshtein_cdist_4096x65536.md"},
                      "hbm_algorithmic_GBps": round(hbm_bytes / (ms_per_step / 1e3) / 1e9, 1),
                      "hbm_frac": round(hbm_bytes / (ms_per_step / 1e3) / 8e12, 4)},
         "cpu_baseline": cpu,
