@@ -1,0 +1,767 @@
+// rapidfuzz_amd.hpp -- C++17 drop-in for the reference's scorer classes,
+// running on one MI355X through the C ABI (include/rapidfuzz_amd.h).
+//
+// The classes keep the reference's names, constructors, method signatures and
+// defaults:
+//   CachedLevenshtein (distance/Levenshtein.hpp:421-492), CachedIndel
+//   (distance/Indel.hpp:148-188), CachedLCSseq (distance/LCSseq.hpp:197-223),
+//   CachedOSA (distance/OSA.hpp:231-275), CachedJaro (distance/Jaro.hpp:195-230),
+//   CachedJaroWinkler (distance/JaroWinkler.hpp:168-209), fuzz::CachedRatio
+//   (fuzz.hpp:98-117), experimental::Multi{Levenshtein,Indel,LCSseq,OSA,Jaro,
+//   JaroWinkler}<MaxLen> and fuzz::experimental::MultiRatio<MaxLen>
+//   (e.g. Levenshtein.hpp:297-419, details/distance.hpp:390-582).
+// A one-pair call is one device round trip; the batch methods added here
+// (distance_many and friends over a resident `Choices` set, extract_topk)
+// are what the GPU is for.
+//
+// Errors: std::invalid_argument where the reference throws it (same messages
+// for the Multi* checks), rapidfuzz_amd::Error for device failures.
+// Threading: const methods may be called from several threads (calls on one
+// device context are serialized inside the library).
+//
+// Code units: each character is converted to an unsigned code unit
+// (char -> uint8_t, char16_t -> uint16_t, char32_t / wchar_t -> uint32_t).
+// When query and choice widths differ both are widened to the larger.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <iterator>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../../include/rapidfuzz_amd.h"
+
+namespace rapidfuzz_amd {
+
+// LevenshteinWeightTable (details/types.hpp:19-23)
+struct LevenshteinWeightTable {
+    size_t insert_cost;
+    size_t delete_cost;
+    size_t replace_cost;
+};
+
+struct Error : std::runtime_error {
+    rfa_status status;
+    Error(rfa_status st, const std::string& msg) : std::runtime_error(msg), status(st) {}
+};
+
+namespace detail {
+
+inline void check(rfa_status st)
+{
+    if (st == RFA_OK) return;
+    const std::string msg = rfa_last_error();
+    if (st == RFA_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+    throw Error(st, msg);
+}
+
+// Process-wide context per device, created on first use.
+inline rfa_context* context(int device = 0)
+{
+    static std::mutex mu;
+    static std::map<int, std::shared_ptr<rfa_context>> ctxs;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ctxs.find(device);
+    if (it != ctxs.end()) return it->second.get();
+    rfa_context* c = nullptr;
+    check(rfa_context_create(device, &c));
+    ctxs[device] = std::shared_ptr<rfa_context>(c, [](rfa_context* p) { rfa_context_destroy(p); });
+    return c;
+}
+
+template <typename T>
+uint32_t to_unit(T c)
+{
+    if constexpr (std::is_integral<T>::value)
+        return static_cast<uint32_t>(static_cast<typename std::make_unsigned<T>::type>(c));
+    else
+        return static_cast<uint32_t>(c);
+}
+
+template <typename T>
+constexpr int width_of()
+{
+    return sizeof(T) == 1 ? 1 : (sizeof(T) == 2 ? 2 : 4);
+}
+
+// Sentences: anything with begin()/end(); character pointers and character
+// arrays (string literals) are C strings that end at the first NUL.
+template <typename S>
+auto to_begin(const S& s)
+{
+    if constexpr (std::is_array<S>::value)
+        return &s[0];
+    else if constexpr (std::is_pointer<S>::value)
+        return s;
+    else
+        return std::begin(s);
+}
+template <typename S>
+auto to_end(const S& s)
+{
+    if constexpr (std::is_array<S>::value) {
+        size_t n = 0;
+        while (n < std::extent<S>::value && s[n]) ++n;
+        return &s[0] + n;
+    }
+    else if constexpr (std::is_pointer<S>::value) {
+        auto e = s;
+        while (*e) ++e;
+        return e;
+    }
+    else
+        return std::end(s);
+}
+
+template <typename S>
+using sentence_char_t = typename std::decay<decltype(*to_begin(std::declval<const S&>()))>::type;
+
+// Packed code units of a list of strings (kept as 32-bit units plus the widest width seen).
+struct Packed {
+    std::vector<uint32_t> units;
+    std::vector<uint64_t> off{0};
+    int width = 1;
+
+    template <typename It>
+    void add(It first, It last)
+    {
+        using T = typename std::decay<decltype(*first)>::type;
+        width = std::max(width, width_of<T>());
+        for (; first != last; ++first) units.push_back(to_unit(*first));
+        off.push_back(units.size());
+    }
+    size_t count() const { return off.size() - 1; }
+};
+
+// A device string set of a given width built from packed units.
+class DeviceStrings {
+public:
+    DeviceStrings(rfa_context* ctx, const Packed& p, int width)
+    {
+        const size_t n = p.units.size();
+        std::vector<uint8_t> bytes(std::max<size_t>(n * width, 1));
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t u = p.units[i];
+            if (width == 1)
+                bytes[i] = static_cast<uint8_t>(u);
+            else if (width == 2) {
+                const uint16_t v = static_cast<uint16_t>(u);
+                std::memcpy(&bytes[i * 2], &v, 2);
+            }
+            else
+                std::memcpy(&bytes[i * 4], &u, 4);
+        }
+        check(rfa_strings_create(ctx, bytes.data(), width, p.off.data(), p.count(), &h_));
+    }
+    ~DeviceStrings() { rfa_strings_destroy(h_); }
+    DeviceStrings(const DeviceStrings&) = delete;
+    DeviceStrings& operator=(const DeviceStrings&) = delete;
+    rfa_strings* get() const { return h_; }
+
+private:
+    rfa_strings* h_ = nullptr;
+};
+
+// rfa_scorer over packed queries, one per code-unit width (built on demand).
+class ScorerSet {
+public:
+    ScorerSet(int metric, Packed q, rfa_weights w, double pw) : metric_(metric), q_(std::move(q)), w_(w), pw_(pw) {}
+    ~ScorerSet()
+    {
+        for (auto& kv : scorers_) rfa_scorer_destroy(kv.second);
+    }
+    ScorerSet(const ScorerSet&) = delete;
+    ScorerSet& operator=(const ScorerSet&) = delete;
+
+    int query_width() const { return q_.width; }
+    size_t count() const { return q_.count(); }
+
+    rfa_scorer* get(int width) const
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = scorers_.find(width);
+        if (it != scorers_.end()) return it->second;
+        rfa_context* ctx = context();
+        DeviceStrings qs(ctx, q_, width);
+        rfa_scorer* sc = nullptr;
+        check(rfa_scorer_create(ctx, metric_, qs.get(), &w_, pw_, &sc));
+        scorers_[width] = sc;
+        return sc;
+    }
+
+private:
+    int metric_;
+    Packed q_;
+    rfa_weights w_;
+    double pw_;
+    mutable std::mutex mu_;
+    mutable std::map<int, rfa_scorer*> scorers_;
+};
+
+template <typename R>
+constexpr bool is_int_result()
+{
+    return std::is_integral<R>::value;
+}
+
+} // namespace detail
+
+// A device-resident set of choices, uploaded once and scored many times.
+class Choices {
+public:
+    template <typename It>
+    Choices(It first, It last)
+    {
+        for (; first != last; ++first) p_.add(detail::to_begin(*first), detail::to_end(*first));
+    }
+    template <typename Container>
+    explicit Choices(const Container& c) : Choices(std::begin(c), std::end(c))
+    {}
+    size_t size() const { return p_.count(); }
+    int width() const { return p_.width; }
+
+    // the set at a code-unit width (uploaded on first use)
+    rfa_strings* at_width(int w) const
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = sets_.find(w);
+        if (it != sets_.end()) return it->second->get();
+        auto s = std::make_shared<detail::DeviceStrings>(detail::context(), p_, w);
+        sets_[w] = s;
+        return s->get();
+    }
+
+private:
+    detail::Packed p_;
+    mutable std::mutex mu_;
+    mutable std::map<int, std::shared_ptr<detail::DeviceStrings>> sets_;
+};
+
+namespace detail {
+
+// Shared implementation of the Cached* scorers: one query, ResType results.
+template <int Metric, typename ResType, int64_t WorstSimilarity, int64_t WorstDistance>
+class CachedBase {
+public:
+    size_t query_size() const { return len_; }
+
+protected:
+    template <typename It>
+    void init(It first, It last, rfa_weights w, double pw)
+    {
+        Packed q;
+        q.add(first, last);
+        len_ = q.count() ? q.off[1] : 0;
+        set_ = std::make_shared<ScorerSet>(Metric, std::move(q), w, pw);
+    }
+
+    // scores of the query against `choices`; out has choices.size() entries
+    template <typename Out>
+    void run(const Choices& choices, int kind, Out* out, uint64_t cu, double cd, int semantics = RFA_CACHED) const
+    {
+        if (!choices.size()) return;
+        const int w = std::max(set_->query_width(), choices.width());
+        check(rfa_cdist(set_->get(w), choices.at_width(w), kind, semantics, cu, cd, out, 0));
+    }
+
+    template <typename It, typename Out>
+    Out one(It first, It last, int kind, uint64_t cu, double cd) const
+    {
+        Packed p;
+        p.add(first, last);
+        const int w = std::max(set_->query_width(), p.width);
+        DeviceStrings s(context(), p, w);
+        Out v{};
+        check(rfa_cdist(set_->get(w), s.get(), kind, RFA_CACHED, cu, cd, &v, 0));
+        return v;
+    }
+
+    static uint64_t u_cut(ResType c) { return static_cast<uint64_t>(c); }
+
+    std::shared_ptr<ScorerSet> set_;
+    size_t len_ = 0;
+
+public:
+    // ---- the reference's methods (details/distance.hpp:211-301, 326-356) ----
+    template <typename InputIt2>
+    ResType distance(InputIt2 first2, InputIt2 last2, ResType score_cutoff = static_cast<ResType>(WorstDistance),
+                     ResType score_hint = static_cast<ResType>(WorstDistance)) const
+    {
+        (void)score_hint;
+        if constexpr (is_int_result<ResType>())
+            return static_cast<ResType>(one<InputIt2, uint64_t>(first2, last2, RFA_DISTANCE, u_cut(score_cutoff), 0.0));
+        else
+            return one<InputIt2, double>(first2, last2, RFA_DISTANCE, 0, score_cutoff);
+    }
+    template <typename Sentence2>
+    ResType distance(const Sentence2& s2, ResType score_cutoff = static_cast<ResType>(WorstDistance),
+                     ResType score_hint = static_cast<ResType>(WorstDistance)) const
+    {
+        return distance(to_begin(s2), to_end(s2), score_cutoff, score_hint);
+    }
+
+    template <typename InputIt2>
+    ResType similarity(InputIt2 first2, InputIt2 last2, ResType score_cutoff = static_cast<ResType>(WorstSimilarity),
+                       ResType score_hint = static_cast<ResType>(WorstSimilarity)) const
+    {
+        (void)score_hint;
+        if constexpr (is_int_result<ResType>())
+            return static_cast<ResType>(
+                one<InputIt2, uint64_t>(first2, last2, RFA_SIMILARITY, u_cut(score_cutoff), 0.0));
+        else
+            return one<InputIt2, double>(first2, last2, RFA_SIMILARITY, 0, score_cutoff);
+    }
+    template <typename Sentence2>
+    ResType similarity(const Sentence2& s2, ResType score_cutoff = static_cast<ResType>(WorstSimilarity),
+                       ResType score_hint = static_cast<ResType>(WorstSimilarity)) const
+    {
+        return similarity(to_begin(s2), to_end(s2), score_cutoff, score_hint);
+    }
+
+    template <typename InputIt2>
+    double normalized_distance(InputIt2 first2, InputIt2 last2, double score_cutoff = 1.0,
+                               double score_hint = 1.0) const
+    {
+        (void)score_hint;
+        return one<InputIt2, double>(first2, last2, RFA_NORMALIZED_DISTANCE, 0, score_cutoff);
+    }
+    template <typename Sentence2>
+    double normalized_distance(const Sentence2& s2, double score_cutoff = 1.0, double score_hint = 1.0) const
+    {
+        return normalized_distance(to_begin(s2), to_end(s2), score_cutoff, score_hint);
+    }
+
+    template <typename InputIt2>
+    double normalized_similarity(InputIt2 first2, InputIt2 last2, double score_cutoff = 0.0,
+                                 double score_hint = 0.0) const
+    {
+        (void)score_hint;
+        return one<InputIt2, double>(first2, last2, RFA_NORMALIZED_SIMILARITY, 0, score_cutoff);
+    }
+    template <typename Sentence2>
+    double normalized_similarity(const Sentence2& s2, double score_cutoff = 0.0, double score_hint = 0.0) const
+    {
+        return normalized_similarity(to_begin(s2), to_end(s2), score_cutoff, score_hint);
+    }
+
+    // ---- batch: the query against every choice (one launch) ----
+    std::vector<ResType> distance_many(const Choices& choices,
+                                       ResType score_cutoff = static_cast<ResType>(WorstDistance)) const
+    {
+        return many(choices, RFA_DISTANCE, score_cutoff);
+    }
+    std::vector<ResType> similarity_many(const Choices& choices,
+                                         ResType score_cutoff = static_cast<ResType>(WorstSimilarity)) const
+    {
+        return many(choices, RFA_SIMILARITY, score_cutoff);
+    }
+    std::vector<double> normalized_distance_many(const Choices& choices, double score_cutoff = 1.0) const
+    {
+        std::vector<double> out(choices.size());
+        run(choices, RFA_NORMALIZED_DISTANCE, out.data(), 0, score_cutoff);
+        return out;
+    }
+    std::vector<double> normalized_similarity_many(const Choices& choices, double score_cutoff = 0.0) const
+    {
+        std::vector<double> out(choices.size());
+        run(choices, RFA_NORMALIZED_SIMILARITY, out.data(), 0, score_cutoff);
+        return out;
+    }
+
+private:
+    std::vector<ResType> many(const Choices& choices, int kind, ResType cutoff) const
+    {
+        std::vector<ResType> out(choices.size());
+        if constexpr (is_int_result<ResType>()) {
+            std::vector<uint64_t> tmp(choices.size());
+            run(choices, kind, tmp.data(), u_cut(cutoff), 0.0);
+            std::transform(tmp.begin(), tmp.end(), out.begin(), [](uint64_t v) { return static_cast<ResType>(v); });
+        }
+        else {
+            run(choices, kind, out.data(), 0, cutoff);
+        }
+        return out;
+    }
+};
+
+constexpr int64_t kIntMax = std::numeric_limits<int64_t>::max();
+
+} // namespace detail
+
+template <typename CharT1>
+struct CachedLevenshtein : detail::CachedBase<RFA_LEVENSHTEIN, size_t, 0, detail::kIntMax> {
+    template <typename Sentence1>
+    explicit CachedLevenshtein(const Sentence1& s1_, LevenshteinWeightTable aWeights = {1, 1, 1})
+        : CachedLevenshtein(detail::to_begin(s1_), detail::to_end(s1_), aWeights)
+    {}
+    template <typename InputIt1>
+    CachedLevenshtein(InputIt1 first1, InputIt1 last1, LevenshteinWeightTable aWeights = {1, 1, 1})
+    {
+        this->init(first1, last1, {aWeights.insert_cost, aWeights.delete_cost, aWeights.replace_cost}, 0.1);
+    }
+};
+
+template <typename CharT1>
+struct CachedIndel : detail::CachedBase<RFA_INDEL, size_t, 0, detail::kIntMax> {
+    template <typename Sentence1>
+    explicit CachedIndel(const Sentence1& s1_) : CachedIndel(detail::to_begin(s1_), detail::to_end(s1_))
+    {}
+    template <typename InputIt1>
+    CachedIndel(InputIt1 first1, InputIt1 last1)
+    {
+        this->init(first1, last1, {1, 1, 1}, 0.1);
+    }
+};
+
+template <typename CharT1>
+struct CachedLCSseq : detail::CachedBase<RFA_LCSSEQ, size_t, 0, detail::kIntMax> {
+    template <typename Sentence1>
+    explicit CachedLCSseq(const Sentence1& s1_) : CachedLCSseq(detail::to_begin(s1_), detail::to_end(s1_))
+    {}
+    template <typename InputIt1>
+    CachedLCSseq(InputIt1 first1, InputIt1 last1)
+    {
+        this->init(first1, last1, {1, 1, 1}, 0.1);
+    }
+};
+
+template <typename CharT1>
+struct CachedOSA : detail::CachedBase<RFA_OSA, size_t, 0, detail::kIntMax> {
+    template <typename Sentence1>
+    explicit CachedOSA(const Sentence1& s1_) : CachedOSA(detail::to_begin(s1_), detail::to_end(s1_))
+    {}
+    template <typename InputIt1>
+    CachedOSA(InputIt1 first1, InputIt1 last1)
+    {
+        this->init(first1, last1, {1, 1, 1}, 0.1);
+    }
+};
+
+template <typename CharT1>
+struct CachedJaro : detail::CachedBase<RFA_JARO, double, 0, 1> {
+    template <typename Sentence1>
+    explicit CachedJaro(const Sentence1& s1_) : CachedJaro(detail::to_begin(s1_), detail::to_end(s1_))
+    {}
+    template <typename InputIt1>
+    CachedJaro(InputIt1 first1, InputIt1 last1)
+    {
+        this->init(first1, last1, {1, 1, 1}, 0.1);
+    }
+};
+
+template <typename CharT1>
+struct CachedJaroWinkler : detail::CachedBase<RFA_JARO_WINKLER, double, 0, 1> {
+    template <typename Sentence1>
+    explicit CachedJaroWinkler(const Sentence1& s1_, double _prefix_weight = 0.1)
+        : CachedJaroWinkler(detail::to_begin(s1_), detail::to_end(s1_), _prefix_weight)
+    {}
+    template <typename InputIt1>
+    CachedJaroWinkler(InputIt1 first1, InputIt1 last1, double _prefix_weight = 0.1)
+    {
+        this->init(first1, last1, {1, 1, 1}, _prefix_weight);
+    }
+};
+
+// Deduction guides (e.g. Levenshtein.hpp:484-492)
+template <typename Sentence1>
+explicit CachedLevenshtein(const Sentence1& s1_, LevenshteinWeightTable aWeights = {1, 1, 1})
+    -> CachedLevenshtein<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+explicit CachedIndel(const Sentence1& s1_) -> CachedIndel<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+explicit CachedLCSseq(const Sentence1& s1_) -> CachedLCSseq<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+explicit CachedOSA(const Sentence1& s1_) -> CachedOSA<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+explicit CachedJaro(const Sentence1& s1_) -> CachedJaro<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+explicit CachedJaroWinkler(const Sentence1& s1_, double _prefix_weight = 0.1)
+    -> CachedJaroWinkler<detail::sentence_char_t<Sentence1>>;
+
+namespace fuzz {
+
+template <typename CharT1>
+struct CachedRatio {
+    template <typename InputIt1>
+    CachedRatio(InputIt1 first1, InputIt1 last1)
+    {
+        detail::Packed q;
+        q.add(first1, last1);
+        set_ = std::make_shared<detail::ScorerSet>(RFA_RATIO, std::move(q), rfa_weights{1, 1, 1}, 0.1);
+    }
+    template <typename Sentence1>
+    CachedRatio(const Sentence1& s1) : CachedRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+
+    template <typename InputIt2>
+    double similarity(InputIt2 first2, InputIt2 last2, double score_cutoff = 0.0, double score_hint = 0.0) const
+    {
+        (void)score_hint;
+        detail::Packed p;
+        p.add(first2, last2);
+        const int w = std::max(set_->query_width(), p.width);
+        detail::DeviceStrings s(detail::context(), p, w);
+        double v = 0;
+        detail::check(rfa_cdist(set_->get(w), s.get(), RFA_SIMILARITY, RFA_CACHED, 0, score_cutoff, &v, 0));
+        return v;
+    }
+    template <typename Sentence2>
+    double similarity(const Sentence2& s2, double score_cutoff = 0.0, double score_hint = 0.0) const
+    {
+        return similarity(detail::to_begin(s2), detail::to_end(s2), score_cutoff, score_hint);
+    }
+
+    // batch: the query against every choice
+    std::vector<double> similarity_many(const Choices& choices, double score_cutoff = 0.0) const
+    {
+        std::vector<double> out(choices.size());
+        if (!choices.size()) return out;
+        const int w = std::max(set_->query_width(), choices.width());
+        detail::check(
+            rfa_cdist(set_->get(w), choices.at_width(w), RFA_SIMILARITY, RFA_CACHED, 0, score_cutoff, out.data(), 0));
+        return out;
+    }
+
+    // the extract() loop + ordering: k best (index, score) with score >= score_cutoff
+    std::vector<std::pair<int64_t, double>> extract_topk(const Choices& choices, uint32_t k,
+                                                         double score_cutoff = 0.0) const
+    {
+        std::vector<int64_t> idx(k);
+        std::vector<double> sc(k);
+        std::vector<std::pair<int64_t, double>> out;
+        if (!k) return out;
+        const int w = std::max(set_->query_width(), choices.width());
+        detail::check(rfa_topk(set_->get(w), choices.at_width(w), RFA_SIMILARITY, score_cutoff, k, idx.data(),
+                               sc.data()));
+        for (uint32_t j = 0; j < k && idx[j] >= 0; ++j) out.emplace_back(idx[j], sc[j]);
+        return out;
+    }
+
+private:
+    std::shared_ptr<detail::ScorerSet> set_;
+};
+
+template <typename Sentence1>
+CachedRatio(const Sentence1& s1) -> CachedRatio<detail::sentence_char_t<Sentence1>>;
+
+} // namespace fuzz
+
+// ---- experimental::Multi* -------------------------------------------------
+
+namespace experimental {
+namespace detail {
+
+// AVX2 lane counts of the reference build (Levenshtein.hpp:303-321): a 256-bit
+// vector holds 256 / (8 * element bytes) strings of up to MaxLen units.
+template <int MaxLen>
+constexpr size_t vec_size()
+{
+    static_assert(MaxLen <= 64, "expected MaxLen <= 64");
+    return MaxLen <= 8 ? 32 : (MaxLen <= 16 ? 16 : (MaxLen <= 32 ? 8 : 4));
+}
+
+template <int Metric, int MaxLen, typename ResType, int64_t WorstSimilarity, int64_t WorstDistance>
+class MultiBase {
+public:
+    explicit MultiBase(size_t count, rfa_weights w = {1, 1, 1}, double pw = 0.1)
+        : input_count_(count), w_(w), pw_(pw)
+    {}
+
+    size_t result_count() const { return (input_count_ + vec_size<MaxLen>() - 1) / vec_size<MaxLen>() * vec_size<MaxLen>(); }
+
+    template <typename Sentence1>
+    void insert(const Sentence1& s1_)
+    {
+        insert(rapidfuzz_amd::detail::to_begin(s1_), rapidfuzz_amd::detail::to_end(s1_));
+    }
+    template <typename InputIt1>
+    void insert(InputIt1 first1, InputIt1 last1)
+    {
+        if (pos_ >= input_count_) throw std::invalid_argument("out of bounds insert");
+        if (std::distance(first1, last1) > MaxLen) throw std::invalid_argument("string longer than MaxLen");
+        q_.add(first1, last1);
+        ++pos_;
+        std::lock_guard<std::mutex> lk(mu_);
+        set_.reset();
+    }
+
+    // scores[i] for i < result_count(); entries past the inserted strings score an empty string
+    template <typename InputIt2>
+    void distance(ResType* scores, size_t score_count, InputIt2 first2, InputIt2 last2,
+                  ResType score_cutoff = static_cast<ResType>(WorstDistance)) const
+    {
+        score(scores, score_count, first2, last2, RFA_DISTANCE, score_cutoff);
+    }
+    template <typename Sentence2>
+    void distance(ResType* scores, size_t score_count, const Sentence2& s2,
+                  ResType score_cutoff = static_cast<ResType>(WorstDistance)) const
+    {
+        distance(scores, score_count, rapidfuzz_amd::detail::to_begin(s2), rapidfuzz_amd::detail::to_end(s2),
+                 score_cutoff);
+    }
+    template <typename InputIt2>
+    void similarity(ResType* scores, size_t score_count, InputIt2 first2, InputIt2 last2,
+                    ResType score_cutoff = static_cast<ResType>(WorstSimilarity)) const
+    {
+        score(scores, score_count, first2, last2, RFA_SIMILARITY, score_cutoff);
+    }
+    template <typename Sentence2>
+    void similarity(ResType* scores, size_t score_count, const Sentence2& s2,
+                    ResType score_cutoff = static_cast<ResType>(WorstSimilarity)) const
+    {
+        similarity(scores, score_count, rapidfuzz_amd::detail::to_begin(s2), rapidfuzz_amd::detail::to_end(s2),
+                   score_cutoff);
+    }
+    template <typename InputIt2>
+    void normalized_distance(double* scores, size_t score_count, InputIt2 first2, InputIt2 last2,
+                             double score_cutoff = 1.0) const
+    {
+        score(scores, score_count, first2, last2, RFA_NORMALIZED_DISTANCE, score_cutoff);
+    }
+    template <typename Sentence2>
+    void normalized_distance(double* scores, size_t score_count, const Sentence2& s2, double score_cutoff = 1.0) const
+    {
+        normalized_distance(scores, score_count, rapidfuzz_amd::detail::to_begin(s2),
+                            rapidfuzz_amd::detail::to_end(s2), score_cutoff);
+    }
+    template <typename InputIt2>
+    void normalized_similarity(double* scores, size_t score_count, InputIt2 first2, InputIt2 last2,
+                               double score_cutoff = 0.0) const
+    {
+        score(scores, score_count, first2, last2, RFA_NORMALIZED_SIMILARITY, score_cutoff);
+    }
+    template <typename Sentence2>
+    void normalized_similarity(double* scores, size_t score_count, const Sentence2& s2,
+                               double score_cutoff = 0.0) const
+    {
+        normalized_similarity(scores, score_count, rapidfuzz_amd::detail::to_begin(s2),
+                              rapidfuzz_amd::detail::to_end(s2), score_cutoff);
+    }
+
+    // batch: every text of `choices`; out[t * result_count() + i]
+    template <typename Out>
+    void score_many(const Choices& choices, int kind, Out* out, double cutoff_d, uint64_t cutoff_u) const
+    {
+        const size_t rc = result_count(), nc = choices.size();
+        if (!nc) return;
+        const int w = std::max(query_width(), choices.width());
+        std::vector<Out> tmp(rc * nc);
+        rapidfuzz_amd::detail::check(
+            rfa_cdist(scorer(w), choices.at_width(w), kind, RFA_MULTI, cutoff_u, cutoff_d, tmp.data(), 0));
+        for (size_t i = 0; i < rc; ++i)
+            for (size_t t = 0; t < nc; ++t) out[t * rc + i] = tmp[i * nc + t];
+    }
+
+protected:
+    int query_width() const { return q_.width; }
+
+    rfa_scorer* scorer(int width) const
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!set_) {
+            rapidfuzz_amd::detail::Packed all = q_;
+            while (all.count() < result_count()) all.off.push_back(all.units.size()); // empty padding lanes
+            set_ = std::make_shared<rapidfuzz_amd::detail::ScorerSet>(Metric, std::move(all), w_, pw_);
+        }
+        return set_->get(width);
+    }
+
+    template <typename InputIt2, typename Out, typename Cut>
+    void score(Out* scores, size_t score_count, InputIt2 first2, InputIt2 last2, int kind, Cut cutoff) const
+    {
+        if (score_count < result_count())
+            throw std::invalid_argument("scores has to have >= result_count() elements");
+        rapidfuzz_amd::detail::Packed p;
+        p.add(first2, last2);
+        const int w = std::max(query_width(), p.width);
+        rapidfuzz_amd::detail::DeviceStrings s(rapidfuzz_amd::detail::context(), p, w);
+        const bool int_out = std::is_integral<Out>::value;
+        std::vector<uint64_t> tmp(result_count());
+        rapidfuzz_amd::detail::check(rfa_cdist(scorer(w), s.get(), kind, RFA_MULTI,
+                                               int_out ? static_cast<uint64_t>(cutoff) : 0,
+                                               int_out ? 0.0 : static_cast<double>(cutoff), tmp.data(), 0));
+        for (size_t i = 0; i < result_count(); ++i) {
+            if constexpr (std::is_integral<Out>::value)
+                scores[i] = static_cast<Out>(tmp[i]);
+            else
+                std::memcpy(&scores[i], &tmp[i], sizeof(double));
+        }
+    }
+
+    size_t input_count_;
+    size_t pos_ = 0;
+    rapidfuzz_amd::detail::Packed q_;
+    rfa_weights w_;
+    double pw_;
+    mutable std::mutex mu_;
+    mutable std::shared_ptr<rapidfuzz_amd::detail::ScorerSet> set_;
+};
+
+constexpr int64_t kIntMax = std::numeric_limits<int64_t>::max();
+
+} // namespace detail
+
+template <int MaxLen>
+struct MultiLevenshtein : detail::MultiBase<RFA_LEVENSHTEIN, MaxLen, size_t, 0, detail::kIntMax> {
+    MultiLevenshtein(size_t count, LevenshteinWeightTable aWeights = {1, 1, 1})
+        : detail::MultiBase<RFA_LEVENSHTEIN, MaxLen, size_t, 0, detail::kIntMax>(
+              count, {aWeights.insert_cost, aWeights.delete_cost, aWeights.replace_cost})
+    {
+        // Levenshtein.hpp:334-340
+        if (aWeights.delete_cost != 1 || aWeights.insert_cost != 1 || aWeights.replace_cost > 2)
+            throw std::invalid_argument("unsupported weights");
+    }
+};
+
+template <int MaxLen>
+struct MultiIndel : detail::MultiBase<RFA_INDEL, MaxLen, size_t, 0, detail::kIntMax> {
+    explicit MultiIndel(size_t count) : detail::MultiBase<RFA_INDEL, MaxLen, size_t, 0, detail::kIntMax>(count) {}
+};
+
+template <int MaxLen>
+struct MultiLCSseq : detail::MultiBase<RFA_LCSSEQ, MaxLen, size_t, 0, detail::kIntMax> {
+    explicit MultiLCSseq(size_t count) : detail::MultiBase<RFA_LCSSEQ, MaxLen, size_t, 0, detail::kIntMax>(count) {}
+};
+
+template <int MaxLen>
+struct MultiOSA : detail::MultiBase<RFA_OSA, MaxLen, size_t, 0, detail::kIntMax> {
+    explicit MultiOSA(size_t count) : detail::MultiBase<RFA_OSA, MaxLen, size_t, 0, detail::kIntMax>(count) {}
+};
+
+template <int MaxLen>
+struct MultiJaro : detail::MultiBase<RFA_JARO, MaxLen, double, 0, 1> {
+    explicit MultiJaro(size_t count) : detail::MultiBase<RFA_JARO, MaxLen, double, 0, 1>(count) {}
+};
+
+template <int MaxLen>
+struct MultiJaroWinkler : detail::MultiBase<RFA_JARO_WINKLER, MaxLen, double, 0, 1> {
+    explicit MultiJaroWinkler(size_t count, double prefix_weight = 0.1)
+        : detail::MultiBase<RFA_JARO_WINKLER, MaxLen, double, 0, 1>(count, {1, 1, 1}, prefix_weight)
+    {}
+};
+
+} // namespace experimental
+
+namespace fuzz {
+namespace experimental {
+
+// fuzz.hpp:43-93: similarity() only, on the 0-100 scale
+template <int MaxLen>
+struct MultiRatio : rapidfuzz_amd::experimental::detail::MultiBase<RFA_RATIO, MaxLen, double, 0, 100> {
+    explicit MultiRatio(size_t count)
+        : rapidfuzz_amd::experimental::detail::MultiBase<RFA_RATIO, MaxLen, double, 0, 100>(count)
+    {}
+};
+
+} // namespace experimental
+} // namespace fuzz
+
+} // namespace rapidfuzz_amd

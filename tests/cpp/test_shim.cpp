@@ -1,0 +1,190 @@
+// test_shim.cpp -- the C++ drop-in classes (rapidfuzz_amd.hpp) on the GPU.
+//
+// Known answers from the reference's own tests (tests-Levenshtein.cpp:144-179,
+// tests-Indel.cpp:204-215, tests-OSA.cpp:62-97, tests-JaroWinkler.cpp:181-182,
+// tests-fuzz.cpp:63-76), batch results equal to one-pair results, Multi*
+// padding and error checks, and extract_topk.  Exit status: 0 pass, 1 fail,
+// 77 no GPU device.
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "rapidfuzz_amd/rapidfuzz_amd.hpp"
+
+namespace rfa = rapidfuzz_amd;
+
+static int g_fail = 0, g_checks = 0;
+
+#define CHECK(cond)                                                                                                \
+    do {                                                                                                           \
+        ++g_checks;                                                                                                \
+        if (!(cond)) {                                                                                             \
+            ++g_fail;                                                                                              \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);                                            \
+        }                                                                                                          \
+    } while (0)
+
+static bool near(double a, double b) { return std::fabs(a - b) < 1e-6; }
+
+template <typename F>
+static bool throws_invalid(F&& f)
+{
+    try {
+        f();
+    }
+    catch (const std::invalid_argument&) {
+        return true;
+    }
+    return false;
+}
+
+static std::string random_string(std::mt19937& rng, size_t lo, size_t hi)
+{
+    std::uniform_int_distribution<size_t> len(lo, hi);
+    std::uniform_int_distribution<int> ch('a', 'e');
+    std::string s(len(rng), 'a');
+    for (auto& c : s) c = static_cast<char>(ch(rng));
+    return s;
+}
+
+int main()
+{
+    try {
+        // ---- known answers ----
+        {
+            rfa::CachedLevenshtein<char> lev(std::string("South Korea"));
+            CHECK(lev.distance(std::string("North Korea")) == 2);
+            CHECK(lev.distance(std::string("North Korea"), 1) == 2);
+            CHECK(lev.distance(std::string("North Korea"), 0) == 1);
+            rfa::CachedLevenshtein<char> lev112(std::string("South Korea"), rfa::LevenshteinWeightTable{1, 1, 2});
+            CHECK(lev112.distance(std::string("North Korea")) == 4);
+            CHECK(lev112.distance(std::string("North Korea"), 2) == 3);
+            rfa::CachedLevenshtein<char> aabc("aabc");
+            CHECK(aabc.distance("cccd") == 4);
+            CHECK(aabc.distance("cccd", 2) == 3);
+        }
+        {
+            rfa::CachedIndel<char> indel(std::string("aaaa"));
+            CHECK(indel.distance(std::string("aaaa")) == 0);
+            CHECK(indel.similarity(std::string("aaaa")) == 8);
+            CHECK(indel.distance(std::string("bbbb")) == 8);
+            CHECK(indel.normalized_similarity(std::string("aaaa")) == 1.0);
+        }
+        {
+            rfa::CachedOSA<char> osa(std::string("CA"));
+            CHECK(osa.distance(std::string("ABC")) == 3);
+            CHECK(osa.distance(std::string("AC")) == 1);
+            rfa::CachedOSA<char> aaaa(std::string("aaaa"));
+            CHECK(aaaa.distance(std::string("")) == 4);
+            CHECK(aaaa.distance(std::string(""), 1) == 2);
+        }
+        {
+            rfa::CachedJaroWinkler<char> jw(std::string("0"));
+            CHECK(near(jw.similarity(std::string("00")), 0.85));
+            rfa::CachedJaroWinkler<char> jw2(std::string("00"));
+            CHECK(near(jw2.similarity(std::string("00")), 1.0));
+        }
+        {
+            rfa::fuzz::CachedRatio<char> ratio(std::string("new york mets"));
+            CHECK(near(ratio.similarity(std::string("new york mets")), 100.0));
+            CHECK(near(ratio.similarity(std::string("the wonderful new york mets")), 65.0));
+        }
+        {
+            // deduction guides and wide code units against narrow ones
+            rfa::CachedLevenshtein lev(std::u32string(U"käse"));
+            CHECK(lev.distance(std::u32string(U"kase")) == 1);
+            CHECK(lev.distance(std::string("k\xe4se")) == 0); // 0xE4 as an unsigned 8-bit unit
+        }
+
+        // ---- batch == one pair at a time ----
+        std::mt19937 rng(7);
+        std::vector<std::string> choices;
+        for (int i = 0; i < 300; ++i) choices.push_back(random_string(rng, 0, 90));
+        const rfa::Choices set(choices);
+        for (int qi = 0; qi < 4; ++qi) {
+            const std::string q = random_string(rng, 0, 80);
+            rfa::CachedLevenshtein<char> lev(q);
+            rfa::CachedIndel<char> indel(q);
+            rfa::CachedOSA<char> osa(q);
+            rfa::CachedJaroWinkler<char> jw(q);
+            rfa::fuzz::CachedRatio<char> ratio(q);
+            const auto d = lev.distance_many(set);
+            const auto dc = lev.distance_many(set, 5);
+            const auto ns = indel.normalized_similarity_many(set, 0.6);
+            const auto od = osa.distance_many(set);
+            const auto js = jw.similarity_many(set);
+            const auto rs = ratio.similarity_many(set, 50.0);
+            for (size_t c = 0; c < choices.size(); c += 7) {
+                CHECK(d[c] == lev.distance(choices[c]));
+                CHECK(dc[c] == lev.distance(choices[c], 5));
+                CHECK(ns[c] == indel.normalized_similarity(choices[c], 0.6));
+                CHECK(od[c] == osa.distance(choices[c]));
+                CHECK(js[c] == jw.similarity(choices[c]));
+                CHECK(rs[c] == ratio.similarity(choices[c], 50.0));
+            }
+            // extract(): the k best by (score desc, index asc) among scores >= cutoff
+            const auto top = ratio.extract_topk(set, 5, 40.0);
+            std::vector<std::pair<int64_t, double>> expect;
+            for (size_t c = 0; c < choices.size(); ++c) {
+                const double s = ratio.similarity_many(set, 40.0)[c];
+                if (s >= 40.0) expect.emplace_back((int64_t)c, s);
+            }
+            std::stable_sort(expect.begin(), expect.end(),
+                             [](const auto& a, const auto& b) { return a.second > b.second; });
+            if (expect.size() > 5) expect.resize(5);
+            CHECK(top == expect);
+        }
+
+        // ---- experimental::Multi* ----
+        {
+            rfa::experimental::MultiLevenshtein<8> multi(3);
+            CHECK(multi.result_count() == 32);
+            multi.insert(std::string("abc"));
+            multi.insert(std::string(""));
+            multi.insert(std::string("abcdefgh"));
+            std::vector<size_t> scores(multi.result_count());
+            multi.distance(scores.data(), scores.size(), std::string("abd"));
+            CHECK(scores[0] == 1 && scores[1] == 3 && scores[2] == 5);
+            CHECK(scores[31] == 3); // padding lanes score the empty string
+            CHECK(throws_invalid([&] { multi.insert(std::string("x")); }));
+            CHECK(throws_invalid([&] { multi.distance(scores.data(), 31, std::string("abd")); }));
+            CHECK(throws_invalid([] { rfa::experimental::MultiLevenshtein<8> bad(2, rfa::LevenshteinWeightTable{1, 1, 3}); }));
+            std::vector<double> ns(multi.result_count());
+            multi.normalized_similarity(ns.data(), ns.size(), std::string("abd"));
+            rfa::CachedLevenshtein<char> c0("abc");
+            CHECK(near(ns[0], 1.0 - 1.0 / 3.0));
+            (void)c0;
+        }
+        {
+            rfa::fuzz::experimental::MultiRatio<16> mr(2);
+            mr.insert(std::string("new york mets"));
+            mr.insert(std::string("atlanta braves"));
+            std::vector<double> s(mr.result_count());
+            mr.similarity(s.data(), s.size(), std::string("the wonderful new york mets"));
+            CHECK(near(s[0], 65.0));
+        }
+        {
+            rfa::experimental::MultiIndel<64> mi(1);
+            mi.insert(std::string(64, 'a'));
+            std::vector<size_t> s(mi.result_count());
+            mi.distance(s.data(), s.size(), std::string(64, 'b'));
+            CHECK(s[0] == 128);
+        }
+    }
+    catch (const rfa::Error& e) {
+        if (e.status == RFA_ERR_NO_DEVICE) {
+            std::printf("no GPU device: %s\n", e.what());
+            return 77;
+        }
+        std::printf("device error: %s\n", e.what());
+        return 1;
+    }
+    catch (const std::exception& e) {
+        std::printf("exception: %s\n", e.what());
+        return 1;
+    }
+    std::printf("%s: %d checks, %d failures\n", g_fail ? "FAIL" : "OK", g_checks, g_fail);
+    return g_fail ? 1 : 0;
+}
