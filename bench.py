@@ -376,6 +376,209 @@ def workload_serve(args, emit=True):
     return line
 
 
+def _cores():
+    try:
+        return min(len(os.sched_getaffinity(0)), 16)
+    except Exception:
+        return min(os.cpu_count() or 1, 16)
+
+
+def workload_latency(args, emit=True):
+    """1-vs-N latency: one query (lowercase a-z, length U{8..32}) against 10,000 candidates, each the
+    query with Poisson(4) random edits (substitute to another letter / insert / delete, uniform);
+    Levenshtein 1/1/1 distance, no cutoff.  One pair per lane on the GPU (the few-query path)."""
+    import rapidfuzz_amd as rfa
+    from rapidfuzz_amd import hip
+
+    rng = np.random.default_rng(SEED + 1)
+    q = rng.integers(97, 123, size=int(rng.integers(8, 33))).astype(np.uint8)
+    cands = []
+    for _ in range(10000):
+        s = list(q)
+        for _ in range(int(rng.poisson(4))):
+            op = int(rng.integers(0, 3))
+            if op == 0 and s:
+                i = int(rng.integers(0, len(s)))
+                s[i] = 97 + (s[i] - 97 + int(rng.integers(1, 26))) % 26
+            elif op == 1:
+                s.insert(int(rng.integers(0, len(s) + 1)), int(rng.integers(97, 123)))
+            elif s:
+                del s[int(rng.integers(0, len(s)))]
+        cands.append(np.array(s, dtype=np.uint8))
+    cc, co = pack_u8(cands)
+    qc, qo = pack_u8([q])
+    cells = float(len(q)) * float(co[-1])
+    ctx = rfa.Context(0)
+    cs = rfa.Strings(ctx, chars=cc, offsets=co, width=1)
+    sc = rfa.Scorer(ctx, rfa.LEVENSHTEIN, rfa.Strings(ctx, chars=qc, offsets=qo, width=1))
+    out = hip.DeviceBuffer(len(cands) * 8)
+    for _ in range(20):
+        sc.cdist(cs, rfa.DISTANCE, out_device_ptr=out.ptr.value)
+    ctx.synchronize()
+    reps = 500
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(ctx.stream)
+    for _ in range(reps):
+        sc.cdist(cs, rfa.DISTANCE, out_device_ptr=out.ptr.value)
+    e1.record(ctx.stream)
+    e1.synchronize()
+    dev_us = e0.elapsed_ms(e1) / reps * 1e3
+    lat = np.zeros(reps)
+    for i in range(reps):
+        t0 = time.perf_counter()
+        got = sc.cdist(cs, rfa.DISTANCE)  # results back in host memory
+        lat[i] = time.perf_counter() - t0
+    p50, p99 = (float(np.percentile(lat, p)) * 1e6 for p in (50, 99))
+    print(f"# latency 1x10000: device {dev_us:.1f} us/call, host round trip p50 {p50:.1f} us p99 {p99:.1f} us, "
+          f"{cells / (p50 / 1e6) / 1e9:.1f} GCUPS at p50", file=sys.stderr, flush=True)
+    cpu, check = None, "not checked"
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        if oracle_py.reference_available():
+            ref = oracle_py.Reference()
+            cut = np.array([(1 << 63) - 1], np.uint64)
+            r = ref.lev_cutoff_mt(qc, qo, cc, co, cut, threads=1)
+            t1 = []
+            for _ in range(20):
+                t0 = time.perf_counter()
+                ref.lev_cutoff_mt(qc, qo, cc, co, cut, threads=1)
+                t1.append(time.perf_counter() - t0)
+            cpu_us = float(np.median(t1)) * 1e6
+            bad = int(np.count_nonzero(r[0] != got[0]))
+            check = "bit-exact vs reference" if bad == 0 else f"{bad} MISMATCHES vs reference"
+            cpu = {"value": round(cpu_us, 1), "unit": "us per 1x10000 call", "cores": 1, "kind": "reference",
+                   "sample": "the full workload, CachedLevenshtein<uint8_t> loop on one core, median of 20",
+                   "gcups": round(cells / (cpu_us / 1e6) / 1e9, 3)}
+    line = {"metric": "1-vs-10,000 Levenshtein distance latency (host round trip, results in host memory)",
+            "value": round(p50, 2), "unit": "us (p50)", "n_gpus": 1, "steps": reps, "warmup": 20,
+            "ms_per_step": round(p50 / 1e3, 4), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u64", "data": "synthetic",
+            "config": {"workload": f"1 query (len {len(q)}) x 10000 candidates (query + Poisson(4) edits), a-z, "
+                                   f"Levenshtein 1/1/1, no cutoff", "cells": cells},
+            "p99_us": round(p99, 2), "device_us_per_call": round(dev_us, 2),
+            "gcups_p50": round(cells / (p50 / 1e6) / 1e9, 2), "cpu_baseline": cpu, "check": check}
+    out.free()
+    ctx.close()
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
+
+
+def _mutate_dna(rng, s):
+    """Per position: substitution 1% (to another base), insertion 0.5% (a random base before it),
+    deletion 0.5%."""
+    u = rng.random(len(s))
+    t = s.copy()
+    sub = u < 0.01
+    t[sub] = (t[sub] + rng.integers(1, 4, size=int(sub.sum()))) % 4
+    ins = (u >= 0.01) & (u < 0.015)
+    dele = (u >= 0.015) & (u < 0.02)
+    keep = ~dele
+    pos = np.flatnonzero(ins)
+    t = np.insert(t, pos, rng.integers(0, 4, size=len(pos)).astype(t.dtype))
+    keep = np.insert(keep, pos, True)
+    return t[keep]
+
+
+def workload_dna(args, emit=True):
+    """Long sequences: 256 queries over {A,C,G,T} of length U{1000..10000} against 16,384 candidates
+    (half derived from a random query with 1% substitutions, 0.5% insertions, 0.5% deletions; half
+    unrelated, length U{1000..10000}); CachedLevenshtein distance with the cutoff k = floor(f * m) of
+    each query, f = 5% and 20%."""
+    import rapidfuzz_amd as rfa
+    from rapidfuzz_amd import hip
+
+    rng = np.random.default_rng(SEED + 3)
+    NQ3, NC3 = 256, 16384
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    qs = [rng.integers(0, 4, size=int(rng.integers(1000, 10001))).astype(np.uint8) for _ in range(NQ3)]
+    cands = []
+    for i in range(NC3):
+        if i % 2 == 0:
+            cands.append(_mutate_dna(rng, qs[int(rng.integers(0, NQ3))]))
+        else:
+            cands.append(rng.integers(0, 4, size=int(rng.integers(1000, 10001))).astype(np.uint8))
+    qs = [acgt[s] for s in qs]
+    cands = [acgt[s] for s in cands]
+    order = rng.permutation(NC3)  # interleave derived and unrelated candidates
+    cands = [cands[i] for i in order]
+    qc, qo = pack_u8(qs)
+    cc, co = pack_u8(cands)
+    ql = np.diff(qo).astype(np.float64)
+    cl = np.diff(co).astype(np.float64)
+    cells = float(ql.sum()) * float(cl.sum())
+    print(f"# dna: {NQ3} x {NC3}, full-matrix cells {cells:.3e}", file=sys.stderr, flush=True)
+    ctx = rfa.Context(0)
+    cs = rfa.Strings(ctx, chars=cc, offsets=co, width=1)
+    # one scorer for all queries; query q is scored with its own cutoff (rfa_cdist_cutoffs)
+    scorer = rfa.Scorer(ctx, rfa.LEVENSHTEIN, rfa.Strings(ctx, chars=qc, offsets=qo, width=1))
+    out = hip.DeviceBuffer(NQ3 * NC3 * 8)
+    res = {}
+    for frac in (0.05, 0.20):
+        cut = np.floor(frac * ql).astype(np.uint64)
+
+        def step():
+            scorer.cdist(cs, rfa.DISTANCE, cutoffs_u=cut, out_device_ptr=out.ptr.value)
+
+        step()
+        ctx.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        steps = max(1, min(args.steps, 3))
+        e0.record(ctx.stream)
+        for _ in range(steps):
+            step()
+        e1.record(ctx.stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        got = np.zeros(NQ3 * NC3, np.uint64)
+        out.to_host(got)
+        got = got.reshape(NQ3, NC3)
+        within = int(np.count_nonzero(got <= cut[:, None]))
+        entry = {"ms": round(ms, 2), "gcups": round(cells / (ms / 1e3) / 1e9, 1), "pairs_within_cutoff": within}
+        print(f"# dna cutoff {frac:.0%}: {ms:.2f} ms, {entry['gcups']} GCUPS (full-matrix cells), "
+              f"{within} pairs within the cutoff", file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle_py
+            if oracle_py.reference_available():
+                ref = oracle_py.Reference()
+                ns = 8
+                while True:
+                    sq, so = pack_u8(qs[:ns])
+                    t0 = time.perf_counter()
+                    r = ref.lev_cutoff_mt(sq, so, cc, co, cut[:ns], threads=_cores())
+                    dt = time.perf_counter() - t0
+                    if dt > 4 or ns >= NQ3:
+                        break
+                    ns = min(NQ3, int(ns * max(2.0, 10.0 / max(dt, 1e-3))))
+                bad = int(np.count_nonzero(r != got[:ns]))
+                scells = float(ql[:ns].sum()) * float(cl.sum())
+                entry["cpu_baseline"] = {"value": round(scells / dt / 1e9, 3), "unit": "GCUPS", "cores": _cores(),
+                                         "kind": "reference",
+                                         "sample": f"first {ns} of {NQ3} queries x all {NC3} candidates, "
+                                                   f"CachedLevenshtein<uint8_t> with the query's cutoff ({dt:.1f} s)"}
+                entry["check"] = ("bit-exact vs reference on the CPU sample" if bad == 0
+                                  else f"{bad} MISMATCHES vs reference")
+                print(f"# dna cutoff {frac:.0%}: CPU {entry['cpu_baseline']['value']} GCUPS; {entry['check']}",
+                      file=sys.stderr, flush=True)
+        res[f"cutoff_{int(frac * 100)}pct"] = entry
+    line = {"metric": "DNA 256 x 16,384 Levenshtein with cutoffs 5% / 20% of the query length, GCUPS (full matrix)",
+            "value": res["cutoff_5pct"]["gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": max(1, min(args.steps, 3)),
+            "warmup": 1, "ms_per_step": res["cutoff_5pct"]["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+            "config": {"workload": f"{NQ3} ACGT queries len 1000-10000 x {NC3} candidates (half derived: 1% sub, "
+                                   f"0.5% ins, 0.5% del; half unrelated), cutoff floor(f*m), f in 5%, 20%",
+                       "cells": cells},
+            **res}
+    scorer.close()
+    out.free()
+    ctx.close()
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
+
+
 def workload_copy(args):
     """Calibration for the HBM counters: a plain device copy of 1 GiB (torch's elementwise copy
     kernel), so FETCH_SIZE / WRITE_SIZE per dispatch can be compared with 1 GiB read + 1 GiB written."""
@@ -399,9 +602,10 @@ def workload_copy(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="cdist", choices=["cdist", "uneven", "serve", "copy"],
+    ap.add_argument("--workload", default="cdist", choices=["cdist", "uneven", "serve", "latency", "dna", "copy"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
-                         "uneven: 1M independent pairs; serve: the serving leg alone; copy: HBM counter "
+                         "uneven: 1M independent pairs; serve: the serving leg alone; latency: 1 x 10,000 "
+                         "Levenshtein; dna: 256 x 16,384 long sequences with band cutoffs; copy: HBM counter "
                          "calibration (a plain 1 GiB device copy)")
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--steps", type=int, default=10)
@@ -423,6 +627,10 @@ def main():
         return workload_serve(args)
     if args.workload == "copy":
         return workload_copy(args)
+    if args.workload == "latency":
+        return workload_latency(args)
+    if args.workload == "dna":
+        return workload_dna(args)
 
     import rapidfuzz_amd as rfa
     from rapidfuzz_amd import hip

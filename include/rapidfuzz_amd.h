@@ -133,6 +133,15 @@ rfa_status rfa_scorer_destroy(rfa_scorer* scorer);
 rfa_status rfa_cdist(rfa_scorer* scorer, const rfa_strings* choices, int kind, int semantics, uint64_t cutoff_u,
                      double cutoff_d, void* out, int out_on_device);
 
+/* As rfa_cdist for the integer kinds, with one cutoff per query:
+ *   out[q * n_choices + c] = Cached<Metric>(query q).<kind>(choice c, cutoffs_u[q])
+ * (or the Multi* value cut at cutoffs_u[q]).  This is the reference's usage loop
+ * when every query has its own cutoff (e.g. a maximum distance proportional to
+ * its length), done as one call so the pairs of all queries run together.
+ * `cutoffs_u` is host memory with one entry per query of the scorer. */
+rfa_status rfa_cdist_cutoffs(rfa_scorer* scorer, const rfa_strings* choices, int kind, int semantics,
+                             const uint64_t* cutoffs_u, void* out, int out_on_device);
+
 /* Independent pairs: out[i] = metrics[i](a[i], b[i]).<kind>(cutoff) with
  * Cached* semantics; metrics[i] is one of Levenshtein (weights {1,1,1}),
  * Indel, LCSseq, OSA.  Replaces per-pair calls of the free functions

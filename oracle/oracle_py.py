@@ -158,6 +158,11 @@ class Reference:
                                        ctypes.c_size_t, _u32p, _u64p, ctypes.c_size_t, ctypes.c_uint64,
                                        ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64,
                                        ctypes.c_double, _u64p, _f64p]
+        L.ref_cdist_multi_w.restype = ctypes.c_int
+        L.ref_cdist_multi_w.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p, _u64p, ctypes.c_size_t,
+                                        _u32p, _u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64,
+                                        ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
+                                        _u64p, _f64p]
         L.ref_cdist_multi.restype = ctypes.c_int
         L.ref_cdist_multi.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p, _u64p, ctypes.c_size_t,
                                       _u32p, _u64p, ctypes.c_size_t, ctypes.c_double, ctypes.c_uint64,
@@ -171,6 +176,9 @@ class Reference:
         L.ref_pairs_u32_mt.restype = ctypes.c_int
         L.ref_pairs_u32_mt.argtypes = [_i32p, _u32p, _u64p, _u32p, _u64p, ctypes.c_size_t, _u64p,
                                        ctypes.c_int]
+        L.ref_lev_cutoff_u8_mt.restype = ctypes.c_int
+        L.ref_lev_cutoff_u8_mt.argtypes = [_u8p, _u64p, ctypes.c_size_t, _u8p, _u64p, ctypes.c_size_t, _u64p,
+                                           _u64p, ctypes.c_int]
         L.ref_extract_ratio_u8_mt.restype = ctypes.c_int
         L.ref_extract_ratio_u8_mt.argtypes = [_u8p, _u64p, ctypes.c_size_t, _u8p, _u64p, ctypes.c_size_t,
                                               ctypes.c_double, ctypes.c_size_t, _i64p, _f64p, _f64p,
@@ -191,9 +199,9 @@ class Reference:
         ou = np.zeros(nq * nc, dtype=np.uint64)
         od = np.zeros(nq * nc, dtype=np.float64)
         if multi_len:
-            r = self.lib.ref_cdist_multi(metric, kind, multi_len, _p(qc, _u32p), _p(qo, _u64p), nq,
-                                         _p(cc, _u32p), _p(co, _u64p), nc, prefix_weight, cutoff_u, cutoff_d,
-                                         _p(ou, _u64p), _p(od, _f64p))
+            r = self.lib.ref_cdist_multi_w(metric, kind, multi_len, _p(qc, _u32p), _p(qo, _u64p), nq,
+                                           _p(cc, _u32p), _p(co, _u64p), nc, weights[0], weights[1], weights[2],
+                                           prefix_weight, cutoff_u, cutoff_d, _p(ou, _u64p), _p(od, _f64p))
         else:
             r = self.lib.ref_cdist_cached(metric, kind, char_width, _p(qc, _u32p), _p(qo, _u64p), nq,
                                           _p(cc, _u32p), _p(co, _u64p), nc, weights[0], weights[1],
@@ -230,6 +238,16 @@ class Reference:
                                   _p(boff, _u64p), n, _p(out, _u64p), threads)
         return out
 
+
+    def lev_cutoff_mt(self, qchars, qoff, cchars, coff, cutoffs, threads=None):
+        """CachedLevenshtein<uint8_t> distance per query with that query's cutoff; [nq, nc] uint64."""
+        nq, nc = len(qoff) - 1, len(coff) - 1
+        threads = threads or os.cpu_count() or 1
+        cut = np.ascontiguousarray(cutoffs, dtype=np.uint64)
+        out = np.zeros(nq * nc, dtype=np.uint64)
+        self.lib.ref_lev_cutoff_u8_mt(_p(qchars, _u8p), _p(qoff, _u64p), nq, _p(cchars, _u8p), _p(coff, _u64p), nc,
+                                      _p(cut, _u64p), _p(out, _u64p), threads)
+        return out.reshape(nq, nc)
 
     def extract_ratio_mt(self, qchars, qoff, cchars, coff, cutoff, k, threads=None):
         """fuzz::CachedRatio extract() + top-k per query; returns (index, score, seconds per query)."""

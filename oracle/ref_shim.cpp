@@ -159,10 +159,10 @@ struct AnyMulti {
     rapidfuzz::experimental::MultiJaroWinkler<MaxLen>* jw = nullptr;
     rapidfuzz::fuzz::experimental::MultiRatio<MaxLen>* ratio = nullptr;
 
-    AnyMulti(int m, size_t n_, double pw) : metric(m), n(n_)
+    AnyMulti(int m, size_t n_, double pw, rapidfuzz::LevenshteinWeightTable w = {1, 1, 1}) : metric(m), n(n_)
     {
         switch (metric) {
-        case LEV: lev = new rapidfuzz::experimental::MultiLevenshtein<MaxLen>(n); break;
+        case LEV: lev = new rapidfuzz::experimental::MultiLevenshtein<MaxLen>(n, w); break;
         case INDEL: indel = new rapidfuzz::experimental::MultiIndel<MaxLen>(n); break;
         case LCS: lcs = new rapidfuzz::experimental::MultiLCSseq<MaxLen>(n); break;
         case OSA: osa = new rapidfuzz::experimental::MultiOSA<MaxLen>(n); break;
@@ -253,9 +253,9 @@ struct AnyMulti {
 template <int MaxLen, typename CharT>
 int multi_cdist_impl(int metric, int kind, const uint32_t* q, const uint64_t* qoff, size_t nq,
                      const uint32_t* c, const uint64_t* coff, size_t nc, double pw, uint64_t cu,
-                     double cd, uint64_t* ou, double* od)
+                     double cd, uint64_t* ou, double* od, rapidfuzz::LevenshteinWeightTable w = {1, 1, 1})
 {
-    AnyMulti<MaxLen> m(metric, nq, pw);
+    AnyMulti<MaxLen> m(metric, nq, pw, w);
     for (size_t i = 0; i < nq; ++i) m.insert(take<CharT>(q + qoff[i], qoff[i + 1] - qoff[i]));
     size_t rc = m.result_count();
     std::vector<size_t> su(rc);
@@ -382,20 +382,23 @@ int ref_cdist_cached(int metric, int kind, int char_width, const uint32_t* q, co
     }
 }
 
-// experimental::Multi<Metric><max_len> cdist: all queries inserted into one scorer.
-int ref_cdist_multi(int metric, int kind, int max_len, const uint32_t* q, const uint64_t* qoff, size_t nq,
-                    const uint32_t* c, const uint64_t* coff, size_t nc, double pw, uint64_t cu, double cd,
-                    uint64_t* ou, double* od)
+// experimental::Multi<Metric><max_len> cdist: all queries inserted into one
+// scorer; ins/del/rep are MultiLevenshtein's weights (other metrics ignore them).
+int ref_cdist_multi_w(int metric, int kind, int max_len, const uint32_t* q, const uint64_t* qoff, size_t nq,
+                      const uint32_t* c, const uint64_t* coff, size_t nc, uint64_t ins, uint64_t del, uint64_t rep,
+                      double pw, uint64_t cu, double cd, uint64_t* ou, double* od)
 {
+    const rapidfuzz::LevenshteinWeightTable w{static_cast<size_t>(ins), static_cast<size_t>(del),
+                                              static_cast<size_t>(rep)};
     try {
         switch (max_len) {
-        case 8: return multi_cdist_impl<8, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od);
+        case 8: return multi_cdist_impl<8, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od, w);
         case 16:
-            return multi_cdist_impl<16, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od);
+            return multi_cdist_impl<16, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od, w);
         case 32:
-            return multi_cdist_impl<32, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od);
+            return multi_cdist_impl<32, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od, w);
         case 64:
-            return multi_cdist_impl<64, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od);
+            return multi_cdist_impl<64, uint32_t>(metric, kind, q, qoff, nq, c, coff, nc, pw, cu, cd, ou, od, w);
         }
         g_err = "max_len must be 8, 16, 32 or 64";
         return -1;
@@ -404,6 +407,13 @@ int ref_cdist_multi(int metric, int kind, int max_len, const uint32_t* q, const 
         g_err = e.what();
         return -2;
     }
+}
+
+int ref_cdist_multi(int metric, int kind, int max_len, const uint32_t* q, const uint64_t* qoff, size_t nq,
+                    const uint32_t* c, const uint64_t* coff, size_t nc, double pw, uint64_t cu, double cd,
+                    uint64_t* ou, double* od)
+{
+    return ref_cdist_multi_w(metric, kind, max_len, q, qoff, nq, c, coff, nc, 1, 1, 1, pw, cu, cd, ou, od);
 }
 
 // CPU baseline: Cached* per query, queries spread over nthreads std::threads.
@@ -474,6 +484,30 @@ int ref_pairs_u32_mt(const int32_t* metrics, const uint32_t* a, const uint64_t* 
                 else
                     out[i] = rapidfuzz::indel_distance(a + aoff[i], a + aoff[i + 1], b + boff[i], b + boff[i + 1]);
             }
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nthreads; ++t) ts.emplace_back(worker);
+    for (auto& t : ts) t.join();
+    return 0;
+}
+
+// CPU baseline for 1-vs-N and banded scoring: CachedLevenshtein<uint8_t> per
+// query with that query's cutoff (Levenshtein.hpp:421-482), threads over
+// (query, chunk of 256 choices) tasks.  out[i * nc + j].
+int ref_lev_cutoff_u8_mt(const uint8_t* q, const uint64_t* qoff, size_t nq, const uint8_t* c, const uint64_t* coff,
+                         size_t nc, const uint64_t* cutoffs, uint64_t* out, int nthreads)
+{
+    const size_t chunk = 256, per_q = (nc + chunk - 1) / chunk;
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+        for (;;) {
+            const size_t t = next.fetch_add(1);
+            if (t >= nq * per_q) break;
+            const size_t i = t / per_q, j0 = (t % per_q) * chunk, j1 = std::min(nc, j0 + chunk);
+            rapidfuzz::CachedLevenshtein<uint8_t> scorer(q + qoff[i], q + qoff[i + 1]);
+            for (size_t j = j0; j < j1; ++j)
+                out[i * nc + j] = scorer.distance(c + coff[j], c + coff[j + 1], static_cast<size_t>(cutoffs[i]));
         }
     };
     std::vector<std::thread> ts;

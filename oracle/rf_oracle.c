@@ -485,13 +485,15 @@ static double multi_fp_similarity(const Pair* p, double c)
 }
 
 int orc_multi_score(int metric, int kind, const uint32_t* s1, size_t l1, const uint32_t* s2,
-                    size_t l2, double prefix_weight, uint64_t cutoff_u, double cutoff_d,
-                    uint64_t* out_u, double* out_d)
+                    size_t l2, uint64_t ins, uint64_t del, uint64_t rep, double prefix_weight,
+                    uint64_t cutoff_u, double cutoff_d, uint64_t* out_u, double* out_d)
 {
     Pair p = {metric, s1, s2, l1, l2, 1, 1, 1, prefix_weight};
+    if (metric == ORC_LEVENSHTEIN && (ins != 1 || del != 1 || rep > 2)) return -1;
     if (metric == ORC_LEVENSHTEIN || metric == ORC_OSA || metric == ORC_INDEL ||
         metric == ORC_LCSSEQ) {
         uint64_t maximum = (metric == ORC_INDEL) ? (uint64_t)l1 + l2 : u64max(l1, l2);
+        if (metric == ORC_LEVENSHTEIN) maximum = lev_maximum(l1, l2, ins, del, rep);
         uint64_t d = multi_raw_distance(&p);
         if (metric == ORC_LCSSEQ && kind == ORC_SIMILARITY) {
             uint64_t lcs = maximum - d;
@@ -579,8 +581,8 @@ int orc_cdist(int metric, int kind, int multi, const uint32_t* qchars, const uin
             uint64_t u = 0;
             double d = 0;
             if (multi)
-                r = orc_multi_score(metric, kind, s1, l1, s2, l2, prefix_weight, cutoff_u, cutoff_d, &u,
-                                    &d);
+                r = orc_multi_score(metric, kind, s1, l1, s2, l2, ins, del, rep, prefix_weight, cutoff_u,
+                                    cutoff_d, &u, &d);
             else
                 r = orc_cached_score(metric, kind, s1, l1, s2, l2, ins, del, rep, prefix_weight,
                                      cutoff_u, cutoff_d, &u, &d);

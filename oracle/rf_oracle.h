@@ -57,10 +57,13 @@ int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const 
                      uint64_t cutoff_u, double cutoff_d, uint64_t* out_u, double* out_d);
 
 /* ---- API layer: experimental::Multi<Metric><MaxLen> for one inserted s1 ----
- * (the Multi* classes score unbounded first and apply the cutoff after). */
+ * (the Multi* classes score unbounded first and apply the cutoff after).
+ * MultiLevenshtein accepts weights {1, 1, <= 2} only (Levenshtein.hpp:334-340,
+ * else -1); its distance is the uniform one whatever the weights
+ * (Levenshtein.hpp:383-399) while maximum() uses them (:401-404). */
 int orc_multi_score(int metric, int kind, const uint32_t* s1, size_t l1, const uint32_t* s2,
-                    size_t l2, double prefix_weight, uint64_t cutoff_u, double cutoff_d,
-                    uint64_t* out_u, double* out_d);
+                    size_t l2, uint64_t ins, uint64_t del, uint64_t rep, double prefix_weight,
+                    uint64_t cutoff_u, double cutoff_d, uint64_t* out_u, double* out_d);
 
 /* ---- batch helpers (single thread), out[q * nc + c] ----
  * chars: packed code units; offsets: count+1 entries. */

@@ -57,6 +57,11 @@ int algo_for(int metric, const rfa_weights& w)
     return -1;
 }
 
+bool multi_levenshtein_weights(const rfa_weights& w)
+{
+    return w.insert_cost == 1 && w.delete_cost == 1 && w.replace_cost <= 2;
+}
+
 bool is_int_kind(int metric, int kind)
 {
     return (metric == RFA_LEVENSHTEIN || metric == RFA_INDEL || metric == RFA_LCSSEQ || metric == RFA_OSA) &&
@@ -130,6 +135,19 @@ void plan_scorer(rfa_scorer* sc)
     }
     sc->hash_bits = std::max<uint32_t>(6, log2_ceil(2ull * max_rows_all));
     for (uint32_t qi : sc->block_queries) sc->block_max_len = std::max(sc->block_max_len, q->len(qi));
+    sc->block_min_len = sc->block_max_len;
+    for (uint32_t qi : sc->block_queries) sc->block_min_len = std::min(sc->block_min_len, q->len(qi));
+    if (q->width == 1) { // 8-bit block patterns are sized by their distinct symbols
+        uint32_t rows = 0;
+        for (uint32_t qi : sc->block_queries) {
+            bool seen[256] = {};
+            uint32_t d = 0;
+            for (uint64_t pos = q->h_off[qi]; pos < q->h_off[qi + 1]; ++pos)
+                if (!seen[q->h_chars[pos]]) seen[q->h_chars[pos]] = true, ++d;
+            rows = std::max(rows, d + 1);
+        }
+        sc->block_u8_rows = rows;
+    }
     // longest first: the block queue is then longest-processing-time-first
     std::stable_sort(sc->block_queries.begin(), sc->block_queries.end(),
                      [&](uint32_t a, uint32_t b) { return q->len(a) > q->len(b); });
@@ -275,7 +293,7 @@ void validate_kind(int metric, int kind, int semantics)
 
 // Block-kernel launch over a cross product (queries x choices) or explicit items.
 void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_pattern, uint64_t max_text,
-               bool reuse, uint32_t* queue = nullptr)
+               bool reuse, uint32_t* queue = nullptr, uint64_t min_pattern = 0, uint32_t u8_rows = 0)
 {
     if (a.nitems == 0) return;
     hipStream_t s = ctx->stream;
@@ -283,15 +301,22 @@ void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_
     const uint32_t slots = std::max<uint32_t>(1, (W + 63) / 64);
     if (slots > 32) throw ApiError(RFA_ERR_UNSUPPORTED, "patterns longer than 131072 code units are not supported");
     // patterns whose map + PM rows fit in LDS_BUDGET run from LDS; larger ones from a
-    // global slice of the workgroup (sized for the largest pattern)
+    // global slice of the workgroup (sized for the largest pattern).  When not even the
+    // shortest pattern of the launch fits, no LDS is reserved, so the LDS does not cap
+    // the number of waves per CU.
     constexpr uint64_t LDS_BUDGET = 40 * 1024;
-    const uint64_t need_max = block_arena_bytes(width, (uint32_t)max_pattern);
-    a.lds_bytes = (uint32_t)std::min<uint64_t>(need_max, LDS_BUDGET);
+    const uint64_t need_max = block_arena_bytes(width, (uint32_t)max_pattern, u8_rows);
+    const uint64_t need_min = block_arena_bytes(width, (uint32_t)min_pattern, u8_rows ? 2 : 0);
+    a.lds_bytes = need_min > LDS_BUDGET ? 0u : (uint32_t)std::min<uint64_t>(need_max, LDS_BUDGET);
     a.reuse_pattern = reuse ? 1u : 0u;
     const uint32_t waves_per_cu =
         (uint32_t)std::min<uint64_t>(16, std::max<uint64_t>(1, ctx->lds_per_cu / (a.lds_bytes + 512)));
-    const uint32_t nblocks = (uint32_t)std::min<uint64_t>(a.nitems, (uint64_t)ctx->num_cus * waves_per_cu);
+    uint32_t nblocks = (uint32_t)std::min<uint64_t>(a.nitems, (uint64_t)ctx->num_cus * waves_per_cu);
     if (need_max > LDS_BUDGET) {
+        // global arenas: at most 2 GiB in total, at least one workgroup per CU
+        constexpr uint64_t ARENA_TOTAL = 2ull << 30;
+        const uint64_t cap = std::max<uint64_t>((uint64_t)ctx->num_cus, ARENA_TOTAL / need_max);
+        nblocks = (uint32_t)std::min<uint64_t>(nblocks, cap);
         a.pm_scratch_words = (need_max + 7) / 8;
         a.pm_scratch = (uint64_t*)ctx->pm_scratch.get(a.pm_scratch_words * 8 * nblocks);
     }
@@ -310,13 +335,30 @@ void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_
 }
 
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
-                int out_on_device)
+                int out_on_device, const uint64_t* qcut_host)
 {
     rfa_context* ctx = sc->ctx;
     validate_kind(sc->metric, kind, semantics);
+    const uint64_t* d_qcut = nullptr;
+    if (qcut_host) {
+        if (!is_int_kind(sc->metric, kind))
+            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "per-query cutoffs are for the integer kinds");
+        std::vector<uint64_t> v(qcut_host, qcut_host + sc->q->count);
+        cu = v.empty() ? 0 : *std::max_element(v.begin(), v.end()); // launch-level bound; items use their own
+        d_qcut = upload(ctx->keep, ctx->qcut, v, ctx->stream);
+    }
     if (ch->width != sc->q->width)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
-    const int algo = algo_for(sc->metric, sc->w);
+    int algo = algo_for(sc->metric, sc->w);
+    if (semantics == RFA_MULTI && sc->metric == RFA_LEVENSHTEIN) {
+        // MultiLevenshtein takes weights {1, 1, <= 2} (Levenshtein.hpp:334-340) and always runs
+        // the uniform recurrence (Levenshtein.hpp:383-399); maximum() uses the weights.
+        if (!multi_levenshtein_weights(sc->w)) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unsupported weights");
+        algo = ALGO_LEV;
+    }
+    if (algo < 0)
+        throw ApiError(RFA_ERR_UNSUPPORTED,
+                       "Levenshtein weights other than {w,w,w} or Indel-like {w,w,>=2w} are not accelerated");
     hipStream_t s = ctx->stream;
     const uint64_t nq = sc->q->count, nc = ch->count;
     const size_t bytes = (size_t)nq * nc * 8;
@@ -325,6 +367,14 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
     void* dout = out_on_device ? out : ctx->out_tmp.get(bytes);
     const ScoreSpec spec = make_spec(sc, kind, semantics, cu, cd, algo);
     const int width = sc->q->width;
+    // a few queries against a set: one pair per lane (choices across lanes)
+    if (cdist_few_queries(sc, ch, spec, algo, dout, d_qcut)) {
+        if (!out_on_device) {
+            HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
+        return;
+    }
 
     // choices routed to the block kernel for Jaro (texts longer than one word)
     std::vector<uint32_t> jaro_long;
@@ -354,6 +404,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ta.hash_bits = sc->hash_bits;
         ta.spec = spec;
         ta.out = dout;
+        ta.qcut = d_qcut;
         if (ta.ncsel) {
             for (const TileClass& tc : sc->classes) {
                 ta.tile_base = tc.first_tile;
@@ -387,7 +438,9 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ba.nitems = (uint32_t)ni;
         ba.spec = spec;
         ba.out = dout;
-        run_block(ctx, algo, width, ba, sc->block_max_len, ch->max_len, true);
+        ba.qcut = d_qcut;
+        run_block(ctx, algo, width, ba, sc->block_max_len, ch->max_len, true, nullptr, sc->block_min_len,
+                  sc->block_u8_rows);
     }
     // ---- Jaro: short queries x long choices ----
     if (!jaro_long.empty() && !sc->tile_order.empty()) {
@@ -580,7 +633,7 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
                           &ctx->block_items[1], &ctx->block_items[2], &ctx->block_items[3], &ctx->plan_keys,
                           &ctx->plan_vals, &ctx->plan_temp, &ctx->topk_scores, &ctx->topk_run[0],
                           &ctx->topk_run[1], &ctx->topk_cand_s, &ctx->topk_cand_i, &ctx->topk_counts, &ctx->topk_q,
-                          &ctx->topk_out})
+                          &ctx->topk_out, &ctx->qcut})
             b->release();
         (void)hipEventDestroy(ctx->fork);
         (void)hipEventDestroy(ctx->join);
@@ -630,7 +683,8 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
         if (metric < RFA_LEVENSHTEIN || metric > RFA_RATIO) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
         if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};
-        if (algo_for(metric, w) < 0)
+        // weights MultiLevenshtein accepts are kept even when Cached* scoring cannot use them
+        if (algo_for(metric, w) < 0 && !(metric == RFA_LEVENSHTEIN && multi_levenshtein_weights(w)))
             throw ApiError(RFA_ERR_UNSUPPORTED,
                            "Levenshtein weights other than {w,w,w} or Indel-like {w,w,>=2w} are not accelerated");
         std::lock_guard<std::mutex> lk(ctx->mu);
@@ -660,6 +714,20 @@ rfa_status rfa_cdist(rfa_scorer* sc, const rfa_strings* choices, int kind, int s
         HIP_CHECK(hipSetDevice(sc->ctx->device));
         begin_call(sc->ctx);
         cdist_impl(sc, choices, kind, semantics, cutoff_u, cutoff_d, out, out_on_device);
+    });
+}
+
+rfa_status rfa_cdist_cutoffs(rfa_scorer* sc, const rfa_strings* choices, int kind, int semantics,
+                             const uint64_t* cutoffs_u, void* out, int out_on_device)
+{
+    if (!sc || !choices) return fail(RFA_ERR_INVALID_ARGUMENT, "scorer or choices is NULL");
+    if (!cutoffs_u && sc->q->count) return fail(RFA_ERR_INVALID_ARGUMENT, "cutoffs_u is NULL");
+    if (!out && sc->q->count && choices->count) return fail(RFA_ERR_INVALID_ARGUMENT, "out is NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(sc->ctx->mu);
+        HIP_CHECK(hipSetDevice(sc->ctx->device));
+        begin_call(sc->ctx);
+        cdist_impl(sc, choices, kind, semantics, 0, 0.0, out, out_on_device, cutoffs_u);
     });
 }
 

@@ -260,6 +260,149 @@ __device__ uint64_t run_lev(const Pattern<CharT, LDS>& P, uint32_t m, const Char
     return readlane(score, last_lane);
 }
 
+// Levenshtein with a cutoff k (|m - n| <= k < max(m, n)): exact when the
+// distance is <= k, otherwise some value > k.  Ukkonen's band in the spirit of
+// Levenshtein_impl.hpp:641-850, formulated for words across lanes:
+//   * a cell (i, j) can lie on a path of cost <= k only if
+//     |i - j| + |(m - n) - (i - j)| <= k, i.e. i - j in [dlo, dhi]; at text
+//     column j only the words overlapping rows [j + dlo, j + dhi] are updated
+//     (whole slots outside them are skipped);
+//   * words above the band are frozen and the first active word takes the
+//     horizontal carry-in (HP = 1, HN = 0); a word entering the band at the
+//     bottom starts from the word above (VP = all ones).  Both only
+//     over-estimate cells, so cells on a path of cost <= k stay exact;
+//   * each word keeps the value of its bottom cell; a word whose cells are all
+//     > k (bottom - rows + 1 > k) is dropped from the top of the band, and an
+//     empty band ends the pair early with k + 1.
+template <int MAXS, typename CharT, bool LDS>
+__device__ uint64_t run_lev_band(const Pattern<CharT, LDS>& P, uint32_t m, const CharT* __restrict__ t, uint32_t n,
+                                 uint32_t k)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t W = P.W, lastw = W - 1, lastbit = (m - 1) & 63;
+    const int32_t dm = (int32_t)m - (int32_t)n;
+    const int32_t dlo = -(int32_t)((uint32_t)((int32_t)k - dm) >> 1); // ceil((dm - k) / 2)
+    const int32_t dhi = (int32_t)((uint32_t)((int32_t)k + dm) >> 1);  // floor((dm + k) / 2)
+    uint64_t VP[MAXS], VN[MAXS];
+    uint32_t sc[MAXS]; // value of the word's bottom cell in the current column
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s) {
+        const uint32_t w = s * 64 + lane;
+        VP[s] = ~0ull;
+        VN[s] = 0;
+        sc[s] = min((w + 1) * 64, m); // column 0: D[i][0] = i
+    }
+    auto word_lo = [&](uint32_t j) { return (uint32_t)(max(1, (int32_t)j + dlo) - 1) >> 6; };
+    auto word_hi = [&](uint32_t j) { return (uint32_t)(min((int32_t)m, (int32_t)j + dhi) - 1) >> 6; };
+    uint32_t fb = word_lo(1), lb = word_hi(1);
+    // PM words of the next text unit are loaded one column ahead (slots of the
+    // current band and the one below it, where the band can grow)
+    auto load_row = [&](uint32_t r, uint64_t* Xd) {
+        const uint64_t* pmrow = P.pm + (size_t)r * P.Wp;
+        const uint32_t s0 = fb >> 6, s1 = (lb >> 6) + 1;
+#pragma unroll
+        for (int s = 0; s < MAXS; ++s) {
+            const uint32_t w = s * 64 + lane;
+            Xd[s] = ((uint32_t)s >= s0 && (uint32_t)s <= s1 && w < P.W) ? pmrow[w] : 0;
+        }
+    };
+    uint64_t Xn[MAXS];
+    for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+        uint32_t row = 0;
+        if (k0 + lane < n) row = P.map.find((uint32_t)t[k0 + lane]);
+        const uint32_t cnt = min(64u, n - k0);
+        load_row(readlane(row, 0), Xn);
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t j = k0 + i + 1; // the column this text unit produces
+            uint64_t Xc[MAXS];
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) Xc[s] = Xn[s];
+            if (i + 1 < cnt) load_row(readlane(row, i + 1), Xn);
+            fb = max(fb, word_lo(j));
+            const uint32_t nl = word_hi(j);
+            for (; lb < nl; ++lb) {
+                // word lb + 1 enters the band: previous column from the word above
+                const uint32_t wa = lb, wn = lb + 1;
+                uint32_t v = 0;
+#pragma unroll
+                for (int s = 0; s < MAXS; ++s)
+                    if ((uint32_t)s == (wa >> 6)) v = sc[s];
+                const uint32_t above = readlane(v, wa & 63);
+                const uint32_t rows = wn == lastw ? m - wn * 64 : 64u;
+#pragma unroll
+                for (int s = 0; s < MAXS; ++s) {
+                    if ((uint32_t)s == (wn >> 6) && lane == (wn & 63)) {
+                        VP[s] = ~0ull;
+                        VN[s] = 0;
+                        sc[s] = above + rows;
+                    }
+                }
+            }
+            if (fb > lb) return (uint64_t)k + 1;
+            const uint32_t sf = fb >> 6, sl = lb >> 6;
+            uint64_t hn_c0 = 0, hp_c0 = 1;
+            uint64_t alive[MAXS];
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) {
+                alive[s] = 0;
+                if ((uint32_t)s < sf || (uint32_t)s > sl) continue;
+                const uint32_t w = s * 64 + lane;
+                const bool act = w >= fb && w <= lb;
+                const uint64_t X = act ? Xc[s] : 0;
+                const uint64_t vp = VP[s], vn = VN[s];
+                const uint64_t X1 = X | 1;
+                const uint64_t d0a = (((X & vp) + vp) ^ vp) | X | vn;
+                const uint64_t d0b = (((X1 & vp) + vp) ^ vp) | X1 | vn;
+                const bool ga = act && ((d0a & vp) >> 63);
+                const bool gb = act && ((d0b & vp) >> 63);
+                const uint64_t G = ballot(ga), Pp = ballot(gb && !ga);
+                uint64_t sum;
+                const uint64_t cout = add3_carry(G | Pp, G, hn_c0, &sum);
+                const uint64_t cin = sum ^ Pp;
+                const uint32_t hin = lane_bit(cin, lane);
+                const uint64_t D0 = hin ? d0b : d0a;
+                uint64_t HP = vn | ~(D0 | vp);
+                uint64_t HN = D0 & vp;
+                const uint64_t HPm = ballot(act && (HP >> 63));
+                const uint64_t first = (uint32_t)s == sf ? (1ull << (fb & 63)) : 0ull;
+                const uint32_t hpin = lane_bit((HPm << 1) | hp_c0 | first, lane);
+                const uint32_t hb = w == lastw ? lastbit : 63u;
+                const int32_t dv = (int32_t)((HP >> hb) & 1) - (int32_t)((HN >> hb) & 1);
+                HP = (HP << 1) | hpin;
+                HN = (HN << 1) | hin;
+                if (act) {
+                    VP[s] = HN | ~(D0 | HP);
+                    VN[s] = HP & D0;
+                    sc[s] = (uint32_t)((int32_t)sc[s] + dv);
+                }
+                const uint32_t rows = w == lastw ? m - w * 64 : 64u;
+                alive[s] = ballot(act && sc[s] + 1 <= k + rows); // some cell of the word is <= k
+                hn_c0 = cout;
+                hp_c0 = HPm >> 63;
+            }
+            // drop dead words from the top of the band
+            bool found = false;
+#pragma unroll
+            for (int s = 0; s < MAXS; ++s) {
+                if (!found && (uint32_t)s >= sf && (uint32_t)s <= sl) {
+                    const uint64_t a = alive[s] & ((uint32_t)s == sf ? (~0ull << (fb & 63)) : ~0ull);
+                    if (a) {
+                        fb = (uint32_t)s * 64 + (uint32_t)__builtin_ctzll(a);
+                        found = true;
+                    }
+                }
+            }
+            if (!found) return (uint64_t)k + 1;
+        }
+    }
+    if (lastw < fb || lastw > lb) return (uint64_t)k + 1;
+    uint32_t v = 0;
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+        if ((uint32_t)s == (lastw >> 6)) v = sc[s];
+    return readlane(v, lastw & 63);
+}
+
 template <int MAXS, typename CharT, bool LDS>
 __device__ uint64_t run_lcs(const Pattern<CharT, LDS>& P, const CharT* __restrict__ t, uint32_t n)
 {
@@ -579,27 +722,55 @@ __host__ __device__ inline uint64_t pattern_arena_bytes(uint32_t m)
     return map + rows * W * 8;
 }
 
+// Arena bytes of an 8-bit pattern from its distinct symbols (rows = distinct + 1):
+// the row map, then one PM row per symbol.  The whole wave calls it.
+__device__ __forceinline__ uint64_t u8_arena_need(const uint8_t* __restrict__ p, uint32_t m, uint32_t* present)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    if (lane < 8) present[lane] = 0;
+    __syncthreads();
+    for (uint32_t i = lane; i < m; i += 64) {
+        const uint32_t c = p[i];
+        atomicOr(&present[c >> 5], 1u << (c & 31));
+    }
+    __syncthreads();
+    uint32_t cnt = lane < 8 ? (uint32_t)__popc(present[lane]) : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    return 1024 + (uint64_t)(cnt + 1) * ((m + 63) / 64) * 8;
+}
+
 // One item with the arena in LDS (LDS = true) or in the workgroup's global slice.
-template <int ALGO, int MAXS, typename CharT, bool LDS>
+template <int ALGO, int MAXS, typename CharT, bool LDS, bool BAND>
 __device__ __forceinline__ RawResult block_item(int algo, const CharT* pp, uint32_t m, const CharT* tt, uint32_t n,
                                                 uint8_t* arena, uint64_t* tflag, uint64_t key, bool reuse,
-                                                Pattern<CharT, LDS>& P, uint64_t& prev_key)
+                                                Pattern<CharT, LDS>& P, uint64_t& prev_key, uint64_t band)
 {
     RawResult res{0, 0, 0};
     if (key != prev_key || !reuse) {
         P = build_pattern<CharT, LDS>(pp, m, arena, pattern_hash_bits<CharT>(m));
         prev_key = key;
     }
+    // Levenshtein needed exactly only up to `band`: the banded recurrence
+    auto lev = [&]() -> uint64_t {
+        if constexpr (BAND) {
+            if (band < (uint64_t)max(m, n)) {
+                if ((uint64_t)(m > n ? m - n : n - m) > band) return band + 1;
+                return run_lev_band<MAXS>(P, m, tt, n, (uint32_t)band);
+            }
+        }
+        return run_lev<MAXS>(P, m, tt, n);
+    };
     if constexpr (ALGO == ALGO_MIXED) {
         if (algo == ALGO_LEV)
-            res.v = run_lev<MAXS>(P, m, tt, n);
+            res.v = lev();
         else if (algo == ALGO_OSA)
             res.v = run_osa<MAXS>(P, m, tt, n);
         else
             res.v = run_lcs<MAXS>(P, tt, n);
     }
     else if constexpr (ALGO == ALGO_LEV)
-        res.v = run_lev<MAXS>(P, m, tt, n);
+        res.v = lev();
     else if constexpr (ALGO == ALGO_LCS)
         res.v = run_lcs<MAXS>(P, tt, n);
     else if constexpr (ALGO == ALGO_OSA)
@@ -609,7 +780,10 @@ __device__ __forceinline__ RawResult block_item(int algo, const CharT* pp, uint3
     return res;
 }
 
-template <int ALGO, int MAXS, typename CharT>
+// BAND: Levenshtein items with a cutoff below max(m, n) run the banded
+// recurrence (a separate instantiation, so unbanded launches keep their
+// register footprint).
+template <int ALGO, int MAXS, typename CharT, bool BAND>
 __global__ __launch_bounds__(64) void k_block(BlockArgs a)
 {
     extern __shared__ uint64_t smem_u64[];
@@ -620,7 +794,8 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a)
     const CharT* __restrict__ B = (const CharT*)a.bchars;
     const uint32_t lane = threadIdx.x & 63;
     __shared__ uint32_t s_item;
-    uint64_t keyL = ~0ull, keyG = ~0ull;
+    __shared__ uint32_t present[8];
+    uint64_t keyL = ~0ull, keyG = ~0ull, need_key = ~0ull, need_u8 = 0;
     Pattern<CharT, true> PL{};
     Pattern<CharT, false> PG{};
     for (;;) {
@@ -657,6 +832,10 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a)
             algo = spec.metric == RFA_LEVENSHTEIN ? ALGO_LEV : (spec.metric == RFA_OSA ? ALGO_OSA : ALGO_LCS);
             if (spec.kind == RFA_DISTANCE && spec.metric != RFA_LCSSEQ) spec.band_k = spec.cutoff_u;
         }
+        if (a.qcut) {
+            spec.cutoff_u = a.qcut[ia];
+            spec.band_k = band_from_cutoff(spec);
+        }
         const uint64_t oa = a.aoff[ia], ob = a.boff[ib];
         const uint32_t la = (uint32_t)(a.aoff[ia + 1] - oa), lb = (uint32_t)(a.boff[ib + 1] - ob);
         const CharT* pp = swap ? B + ob : A + oa;
@@ -669,12 +848,24 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a)
             }
             else {
                 const uint64_t key = ((uint64_t)swap << 63) | (swap ? ib : ia);
-                if (pattern_arena_bytes<CharT>(m) <= a.lds_bytes)
-                    res = block_item<ALGO, MAXS, CharT, true>(algo, pp, m, tt, n, lds_arena, tflag, key,
-                                                              a.reuse_pattern, PL, keyL);
+                uint64_t need;
+                if constexpr (sizeof(CharT) == 1) {
+                    // 8-bit patterns are sized by their distinct symbols
+                    if (key != need_key) {
+                        need_u8 = u8_arena_need((const uint8_t*)pp, m, present);
+                        need_key = key;
+                    }
+                    need = need_u8;
+                }
+                else {
+                    need = pattern_arena_bytes<CharT>(m);
+                }
+                if (need <= a.lds_bytes)
+                    res = block_item<ALGO, MAXS, CharT, true, BAND>(algo, pp, m, tt, n, lds_arena, tflag, key,
+                                                              a.reuse_pattern, PL, keyL, spec.band_k);
                 else
-                    res = block_item<ALGO, MAXS, CharT, false>(algo, pp, m, tt, n, glb_arena, tflag, key,
-                                                               a.reuse_pattern, PG, keyG);
+                    res = block_item<ALGO, MAXS, CharT, false, BAND>(algo, pp, m, tt, n, glb_arena, tflag, key,
+                                                               a.reuse_pattern, PG, keyG, spec.band_k);
             }
         }
         if (lane == 0) finalize(spec, la, lb, res, a.out, oidx);
@@ -684,7 +875,14 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a)
 template <int ALGO, int MAXS, typename CharT>
 static void launch3(const BlockArgs& a, uint32_t nblocks, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT>), dim3(nblocks), dim3(64), a.lds_bytes, s, a);
+    // banded Levenshtein: Levenshtein / mixed launches with a distance cutoff, patterns <= 32768 units
+    if constexpr ((ALGO == ALGO_LEV || ALGO == ALGO_MIXED) && MAXS <= 8) {
+        if (a.spec.kind == RFA_DISTANCE && a.spec.cutoff_u < (1ull << 31)) {
+            hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT, true>), dim3(nblocks), dim3(64), a.lds_bytes, s, a);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT, false>), dim3(nblocks), dim3(64), a.lds_bytes, s, a);
 }
 
 template <int ALGO, typename CharT>
@@ -717,8 +915,10 @@ void launch_block_w(int algo, const BlockArgs& a, uint32_t nblocks, uint32_t slo
     }
 }
 
-inline uint64_t block_arena_need(int char_width, uint32_t m)
+// rows: distinct symbols + 1 of the largest 8-bit pattern when known (0: the alphabet bound)
+inline uint64_t block_arena_need(int char_width, uint32_t m, uint32_t rows = 0)
 {
+    if (char_width == 1 && rows) return 1024 + (uint64_t)rows * ((m + 63) / 64) * 8;
     if (char_width == 1) return pattern_arena_bytes<uint8_t>(m);
     if (char_width == 2) return pattern_arena_bytes<uint16_t>(m);
     return pattern_arena_bytes<uint32_t>(m);

@@ -5,7 +5,10 @@ namespace rfa {
 
 template void launch_block_w<uint8_t>(int, const BlockArgs&, uint32_t, uint32_t, hipStream_t);
 
-uint64_t block_arena_bytes(int char_width, uint32_t max_m) { return block_arena_need(char_width, max_m); }
+uint64_t block_arena_bytes(int char_width, uint32_t max_m, uint32_t rows)
+{
+    return block_arena_need(char_width, max_m, rows);
+}
 
 extern template void launch_block_w<uint16_t>(int, const BlockArgs&, uint32_t, uint32_t, hipStream_t);
 extern template void launch_block_w<uint32_t>(int, const BlockArgs&, uint32_t, uint32_t, hipStream_t);

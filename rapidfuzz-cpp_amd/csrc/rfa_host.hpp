@@ -142,7 +142,7 @@ struct rfa_context {
     rfa::DevBuf queue, pm_scratch, tflag, items_a, items_b, items_idx, items_swap, out_tmp, misc;
     rfa::DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4], plan_keys, plan_vals,
         plan_temp;
-    rfa::DevBuf topk_scores, topk_run[2], topk_cand_s, topk_cand_i, topk_counts, topk_q, topk_out;
+    rfa::DevBuf topk_scores, topk_run[2], topk_cand_s, topk_cand_i, topk_counts, topk_q, topk_out, qcut;
     size_t lane_arena_bytes_zeroed = 0;
     uint32_t gen_next = 1;
     rfa::Keep keep; // host sources of in-flight uploads
@@ -192,13 +192,15 @@ struct rfa_scorer {
     uint32_t hash_bits = 6;
     std::vector<uint32_t> block_queries; // scored by the block kernel
     uint32_t* d_block_queries = nullptr;
-    uint64_t block_max_len = 0;
+    uint64_t block_max_len = 0, block_min_len = 0;
+    uint32_t block_u8_rows = 0; // distinct symbols + 1 of the 8-bit block queries (0: unknown)
 };
 
 namespace rfa {
 
 void begin_call(rfa_context* ctx);
 int algo_for(int metric, const rfa_weights& w);
+bool multi_levenshtein_weights(const rfa_weights& w);
 bool is_int_kind(int metric, int kind);
 rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const uint64_t* off, uint64_t count);
 void free_strings(rfa_strings* s);
@@ -206,10 +208,15 @@ rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries
                         double prefix_weight);
 void free_scorer(rfa_scorer* sc);
 void validate_kind(int metric, int kind, int semantics);
+// qcut_host: optional per-query integer cutoffs (host memory, one per query)
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
-                int out_on_device);
+                int out_on_device, const uint64_t* qcut_host = nullptr);
 
 // rfa_topk.cpp
+// cdist of at most 16 one-byte queries (<= 256 units, LCS or uniform Levenshtein
+// recurrence) against a set, one pair per lane; false when it does not apply.
+bool cdist_few_queries(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, int algo, void* dout,
+                       const uint64_t* d_qcut = nullptr);
 void topk_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, double cutoff, uint32_t k, int64_t* out_index,
                double* out_score);
 

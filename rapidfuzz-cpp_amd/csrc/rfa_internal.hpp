@@ -24,6 +24,7 @@ struct TileArgs {
     uint32_t hash_bits;  // log2 slots of the LDS symbol hash (16/32-bit units)
     ScoreSpec spec;
     void* out;
+    const uint64_t* qcut; // per-query integer cutoff (nullptr: spec.cutoff_u)
 };
 
 size_t tile_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t hash_bits);
@@ -60,10 +61,12 @@ struct BlockArgs {
     const uint32_t* item_packed;
     const uint32_t* nitems_dev;
     const int32_t* pair_metric;
+    const uint64_t* qcut; // cross mode: integer cutoff per string of set A (nullptr: spec.cutoff_u)
 };
 
-// arena bytes of the largest pattern (symbol map + PM rows)
-uint64_t block_arena_bytes(int char_width, uint32_t max_m);
+// arena bytes of the largest pattern (symbol map + PM rows); rows: distinct symbols + 1
+// of the largest 8-bit pattern when known, 0 for the alphabet bound
+uint64_t block_arena_bytes(int char_width, uint32_t max_m, uint32_t rows = 0);
 hipError_t launch_block(int algo, int char_width, const BlockArgs& a, uint32_t nblocks, uint32_t slots, hipStream_t s);
 
 // Lane kernel (rfa_lane.hip): one pair per lane, per-lane 128-slot maps per 64-unit block.
@@ -122,6 +125,12 @@ struct ServeArgs {
     double* cand_score;
     int64_t* cand_index;
     uint32_t cap; // candidates kept per query (counts may exceed it)
+    // dense mode (cdist of a few queries): every result is written to
+    // dense_out[qrow[q] * dense_stride + choice] and no candidates are kept
+    uint64_t* dense_out;
+    const uint32_t* qrow;
+    uint64_t dense_stride;
+    const uint64_t* qcut; // dense mode: integer cutoff per query row (nullptr: spec.cutoff_u)
 };
 
 // Per-query selection of the k best (score desc, index asc) entries with

@@ -143,7 +143,7 @@ __device__ __forceinline__ uint64_t multi_full_distance(const IntPair& p)
     case RFA_INDEL:
     case RFA_RATIO: return indel_from_lcs(p);
     case RFA_LCSSEQ: return u64max(p.l1, p.l2) - p.raw;
-    default: return p.raw; // Levenshtein / OSA (empty s1 already gives s2.size())
+    default: return p.raw; // Levenshtein (always the uniform recurrence here) / OSA
     }
 }
 
@@ -244,6 +244,17 @@ __device__ __forceinline__ double multi_fp_similarity(const FpPair& p, double c)
     return s;
 }
 
+// The raw distance a fixed integer cutoff needs exactly (larger values only
+// need to stay larger): distance kinds of Levenshtein (in units of its common
+// weight), Indel and OSA; everything else needs the exact value.
+__device__ __forceinline__ uint64_t band_from_cutoff(const ScoreSpec& s)
+{
+    if (s.kind != RFA_DISTANCE) return ~0ull;
+    if (s.metric == RFA_LEVENSHTEIN) return s.ins ? ceil_div_u64(s.cutoff_u, s.ins) : ~0ull;
+    if (s.metric == RFA_INDEL || s.metric == RFA_OSA) return s.cutoff_u;
+    return ~0ull;
+}
+
 // --- dispatch ----------------------------------------------------------------
 
 // Integer-valued kinds: distance / similarity of Levenshtein, Indel, LCSseq, OSA.
@@ -251,7 +262,8 @@ __device__ __forceinline__ uint64_t finalize_int(const ScoreSpec& s, uint64_t l1
 {
     IntPair p{&s, l1, l2, raw};
     if (s.semantics == RFA_MULTI) {
-        const uint64_t maximum = (s.metric == RFA_INDEL) ? l1 + l2 : u64max(l1, l2);
+        // MultiLevenshtein's maximum() uses its weights (Levenshtein.hpp:401-404)
+        const uint64_t maximum = int_maximum(p);
         const uint64_t d = multi_full_distance(p);
         if (s.kind == RFA_DISTANCE) return d <= s.cutoff_u ? d : s.cutoff_u + 1;
         if (s.metric == RFA_LCSSEQ) return raw >= s.cutoff_u ? raw : 0;
@@ -318,7 +330,7 @@ __device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64
         return;
     }
     if (s.semantics == RFA_MULTI) {
-        uint64_t maximum = (s.metric == RFA_INDEL) ? l1 + l2 : u64max(l1, l2);
+        uint64_t maximum = int_maximum(p);
         uint64_t d = multi_full_distance(p);
         switch (s.kind) {
         case RFA_NORMALIZED_DISTANCE: {

@@ -17,7 +17,8 @@
 //     as in the reference's own length filters LCSseq_impl.hpp:387-389 and
 //     Levenshtein_impl.hpp:869-870);
 //   * results >= cutoff are appended to the query's candidate list with one
-//     atomic per wave.
+//     atomic per wave; in dense mode (cdist of a few queries against a set,
+//     one pair per lane) every result is written to its row instead.
 //
 // k_select orders candidates: one workgroup per query, thread-local sorted
 // lists of KT entries, then block-wide argmax rounds.  A pass ends early when
@@ -220,6 +221,13 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a)
         for (int q = 0; q < QPB; ++q) {
             if (!(act & (1u << q))) continue;
             RawResult r{st[q].raw(qm[q], n), 0, 0};
+            if (a.dense_out) {
+                const uint32_t qr = a.qrow[q0 + q];
+                ScoreSpec sp = a.spec;
+                if (a.qcut) sp.cutoff_u = a.qcut[qr];
+                if (cidx != 0xffffffffu) a.dense_out[(uint64_t)qr * a.dense_stride + cidx] = finalize_bits(sp, qm[q], n, r);
+                continue;
+            }
             double score;
             finalize(a.spec, qm[q], n, r, &score, 0);
             const bool pass = cidx != 0xffffffffu && score >= a.cutoff;

@@ -127,6 +127,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
     const uint32_t m = qact ? (uint32_t)(a.qoff[qidx + 1] - qo) : 0;
 
     SymLookup<CharT> look = build_tile_pm<CharT, W>(q, qo, m, pm, extra, a);
+    ScoreSpec spec = a.spec;
+    if (a.qcut && qact) spec.cutoff_u = a.qcut[qidx];
 
     constexpr uint32_t BITS = sizeof(W) * 8;
     const uint32_t it0 = blockIdx.x * a.choices_per_block;
@@ -229,9 +231,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
             }
             uint64_t bits;
             if constexpr (INTK)
-                bits = finalize_int(a.spec, m, n, res.v);
+                bits = finalize_int(spec, m, n, res.v);
             else
-                bits = finalize_bits(a.spec, m, n, res);
+                bits = finalize_bits(spec, m, n, res);
             if (direct) {
                 if (qact) ((uint64_t*)a.out)[(uint64_t)qidx * a.out_stride + cidx] = bits;
             }

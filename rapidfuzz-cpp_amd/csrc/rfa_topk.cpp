@@ -321,6 +321,61 @@ static void enqueue_fused(const ServeIndex& ix, const FusedPlan& p, const ScoreS
     }
 }
 
+// ---- cdist of a few queries: one pair per lane --------------------------------
+
+bool cdist_few_queries(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, int algo, void* dout,
+                       const uint64_t* d_qcut)
+{
+    rfa_context* ctx = sc->ctx;
+    const uint64_t nq = sc->q->count, nc = ch->count;
+    if (nq == 0 || nq > 16 || nc == 0 || nc >= 0xffffffffull) return false;
+    if (ch->width != 1 || sc->q->width != 1 || sc->q->max_len > 256) return false;
+    if (ch->view) return false; // chunk views of another set carry no host copy to index
+    if (algo != ALGO_LCS && algo != ALGO_LEV) return false;
+    auto ix = get_index(ch);
+    std::vector<uint32_t> order(nq);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sc->q->len(a) < sc->q->len(b); });
+    std::vector<std::pair<const uint8_t*, uint32_t>> qs(nq);
+    for (uint64_t i = 0; i < nq; ++i)
+        qs[i] = {sc->q->h_chars.data() + sc->q->h_off[order[i]], (uint32_t)sc->q->len(order[i])};
+    FusedKind f;
+    f.ok = true;
+    f.algo = algo;
+    f.bound = Bound::NONE; // every result is wanted: all groups
+    FusedPlan p;
+    plan_fused(*ix, f, qs, ch->max_len, ctx->num_cus, p);
+    const size_t off_qrow = align8(p.blob.size());
+    p.blob.resize(off_qrow + nq * 4);
+    std::memcpy(p.blob.data() + off_qrow, order.data(), nq * 4);
+    hipStream_t s = ctx->stream;
+    const uint8_t* d_blob = upload(ctx->keep, ctx->topk_q, p.blob, s);
+    ServeArgs a{};
+    a.gdata = ix->d_data;
+    a.gbase = ix->d_base;
+    a.gdwords = ix->d_dwords;
+    a.glen_min = ix->d_glen_min;
+    a.slot_len = ix->d_slot_len;
+    a.slot_idx = ix->d_slot_idx;
+    a.qchars = d_blob;
+    a.qoff = (const uint64_t*)(d_blob + p.off_qoff);
+    a.qglo = (const uint32_t*)(d_blob + p.off_qglo);
+    a.qghi = (const uint32_t*)(d_blob + p.off_qghi);
+    a.spec = spec;
+    a.dense_out = (uint64_t*)dout;
+    a.qrow = (const uint32_t*)(d_blob + off_qrow);
+    a.dense_stride = nc;
+    a.qcut = d_qcut;
+    for (const auto& L : p.launches) {
+        a.q_begin = L.q_begin;
+        a.q_end = L.q_end;
+        a.sub_glo = (const uint32_t*)(d_blob + p.off_sglo) + L.sub_first;
+        a.sub_ghi = (const uint32_t*)(d_blob + p.off_sghi) + L.sub_first;
+        HIP_CHECK(launch_serve(algo, L.wq, a, L.gx, L.nsub, s));
+    }
+    return true;
+}
+
 // ---- dense path -------------------------------------------------------------
 
 // Top-k of every query of `sc` against `ch` through rfa_cdist, chunked over

@@ -26,7 +26,8 @@ _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED",
 # every symbol include/rapidfuzz_amd.h declares
 EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
             "rfa_context_synchronize", "rfa_context_stream", "rfa_strings_create", "rfa_strings_destroy",
-            "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_pairs", "rfa_topk",
+            "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_cdist_cutoffs",
+            "rfa_pairs", "rfa_topk",
             "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
             "rfa_server_get_stats"]
 
@@ -84,6 +85,7 @@ def load_library(path: str = LIB_PATH):
     L.rfa_scorer_destroy.argtypes = [_vp]
     L.rfa_cdist.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _vp,
                             ctypes.c_int]
+    L.rfa_cdist_cutoffs.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int]
     L.rfa_pairs.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                             _vp, ctypes.c_int]
     L.rfa_topk.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp]
@@ -94,7 +96,8 @@ def load_library(path: str = LIB_PATH):
     L.rfa_server_replay.argtypes = [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp]
     L.rfa_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     for name in ("rfa_context_create", "rfa_context_destroy", "rfa_context_synchronize", "rfa_strings_create",
-                 "rfa_strings_destroy", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_pairs",
+                 "rfa_strings_destroy", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_cdist_cutoffs",
+                 "rfa_pairs",
                  "rfa_topk", "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait",
                  "rfa_server_replay", "rfa_server_get_stats"):
         getattr(L, name).restype = ctypes.c_int
@@ -245,7 +248,16 @@ class Scorer:
             pass
 
     def cdist(self, choices: Strings, kind=DISTANCE, cutoff_u=None, cutoff_d=None, semantics=CACHED,
-              out_device_ptr=None):
+              out_device_ptr=None, cutoffs_u=None):
+        """All queries x all choices.  cutoffs_u: one integer cutoff per query (integer kinds)."""
+        if cutoffs_u is not None:
+            cut = np.ascontiguousarray(cutoffs_u, dtype=np.uint64)
+            if out_device_ptr is not None:
+                _check(_lib.rfa_cdist_cutoffs(self.h, choices.h, kind, semantics, cut.ctypes.data, out_device_ptr, 1))
+                return None
+            out = np.zeros((self.nq, choices.count), dtype=np.uint64)
+            _check(_lib.rfa_cdist_cutoffs(self.h, choices.h, kind, semantics, cut.ctypes.data, out.ctypes.data, 0))
+            return out
         du, dd = default_cutoffs(self.metric, kind)
         cu = du if cutoff_u is None else cutoff_u
         cd = dd if cutoff_d is None else cutoff_d

@@ -26,12 +26,13 @@ def _run():
 
 def test_without_device_fails_cleanly():
     """On a machine without a GPU the classes throw Error(RFA_ERR_NO_DEVICE), nothing crashes."""
+    import rapidfuzz_amd as rfa
+
     try:
-        import torch
-        if torch.cuda.is_available():
-            pytest.skip("a GPU is present")
-    except ImportError:
-        pass
+        rfa.Context(0).close()
+        pytest.skip("a GPU is present")
+    except rfa.RfaError as e:
+        assert e.status == 5, e  # RFA_ERR_NO_DEVICE
     r = _run()
     assert r.returncode == 77, r.stdout + r.stderr
 

@@ -121,3 +121,119 @@ def test_ratio_rejects_distance(ctx):
     sc = rfa.Scorer(ctx, rfa.RATIO, qs)
     with pytest.raises(rfa.InvalidArgument):
         sc.cdist(cs, rfa.DISTANCE)
+
+
+@pytest.mark.parametrize("nq", [1, 5, 16])
+@pytest.mark.parametrize("metric,weights", [(0, (1, 1, 1)), (0, (2, 2, 2)), (0, (1, 1, 2)), (1, (1, 1, 1)),
+                                            (2, (1, 1, 1)), (6, (1, 1, 1))])
+def test_few_queries_one_pair_per_lane(ctx, oracle, nq, metric, weights):
+    """At most 16 one-byte queries of <= 256 units: one pair per lane over the length-sorted set."""
+    rng = np.random.default_rng(40 + nq + 3 * metric + weights[2])
+    queries = _gen.random_strings(rng, nq - 1, 0, 256, alpha_n=6) + [np.zeros(0, np.uint32)]
+    choices = _gen.random_strings(rng, 150, 0, 300, alpha_n=6)
+    choices += [_gen.mutate(rng, q, int(rng.integers(0, 6)), alpha_n=6) for q in queries]
+    # MultiLevenshtein takes weights {1, 1, <= 2} only
+    multi_ok = metric != 0 or (weights[0] == 1 and weights[1] == 1 and weights[2] <= 2)
+    for semantics in ((0, 1) if multi_ok else (0,)):
+        check(ctx, oracle, metric, queries, choices, width=1, weights=weights, semantics=semantics)
+
+
+@pytest.mark.parametrize("weights", [(1, 1, 0), (1, 1, 1), (1, 1, 2)])
+def test_multi_levenshtein_weights(ctx, oracle, weights):
+    """MultiLevenshtein: the uniform distance whatever the weights, maximum() from the weights;
+    both the tile path (> 16 queries) and the one-pair-per-lane path (<= 16 queries)."""
+    rng = np.random.default_rng(11 + weights[2])
+    for nq in (40, 7):
+        queries = _gen.random_strings(rng, nq, 0, 64, alpha_n=4)
+        choices = _gen.random_strings(rng, 40, 0, 90, alpha_n=4)
+        check(ctx, oracle, 0, queries, choices, width=1, weights=weights, semantics=1)
+
+
+def test_multi_levenshtein_rejects_weights(ctx):
+    import rapidfuzz_amd as rfa
+
+    sc = rfa.Scorer(ctx, 0, ctx.strings([b"abc"]), weights=(2, 2, 2))
+    with pytest.raises(rfa.InvalidArgument):
+        sc.cdist(ctx.strings([b"abd"]), rfa.DISTANCE, semantics=rfa.MULTI)
+    sc0 = rfa.Scorer(ctx, 0, ctx.strings([b"abc"]), weights=(1, 1, 0))  # Multi-only weights
+    with pytest.raises(rfa.RfaError):
+        sc0.cdist(ctx.strings([b"abd"]), rfa.DISTANCE)  # Cached scoring with them is not accelerated
+
+
+def _dna_case(seed, nq=6, nc=40, lo=300, hi=1500):
+    rng = np.random.default_rng(seed)
+    queries = [rng.integers(65, 69, size=int(rng.integers(lo, hi))).astype(np.uint32) for _ in range(nq)]
+    choices = []
+    for i in range(nc):
+        q = queries[i % nq]
+        kind = i % 4
+        if kind == 0:
+            choices.append(_gen.mutate(rng, q, int(rng.integers(0, len(q) // 20 + 1)), alpha_lo=65, alpha_n=4))
+        elif kind == 1:
+            choices.append(_gen.mutate(rng, q, int(rng.integers(len(q) // 20, len(q) // 5 + 2)), alpha_lo=65,
+                                       alpha_n=4))
+        elif kind == 2:
+            choices.append(rng.integers(65, 69, size=len(q)).astype(np.uint32))
+        else:
+            choices.append(rng.integers(65, 69, size=max(1, len(q) + int(rng.integers(-40, 40)))).astype(np.uint32))
+    return queries, choices
+
+
+def test_banded_levenshtein_cutoffs(ctx, oracle):
+    """Levenshtein distance with cutoffs on long patterns: the banded block recurrence must equal the
+    oracle at every cutoff, in particular where the cutoff is at or next to the distance."""
+    import rapidfuzz_amd as rfa
+
+    queries, choices = _dna_case(3)
+    full = oracle.cdist(0, 0, queries, choices)
+    cutoffs = sorted({0, 1, 5, 20, 60, 150, 400, 900} | {int(d) + e for d in full.ravel()[::7] for e in (-1, 0, 1)
+                                                         if int(d) + e >= 0})
+    qs, cs = ctx.strings(queries, width=1), ctx.strings(choices, width=1)
+    sc = rfa.Scorer(ctx, 0, qs)
+    for cu in cutoffs:
+        for semantics in (0, 1):
+            got = sc.cdist(cs, 0, cutoff_u=cu, semantics=semantics)
+            exp = np.where(full <= cu, full, cu + 1).astype(np.uint64)
+            assert np.array_equal(got, exp), f"cutoff {cu} semantics {semantics}: {np.count_nonzero(got != exp)} differ"
+
+
+def test_banded_levenshtein_pairs(ctx, oracle):
+    """The same through rfa_pairs (the mixed-metric block kernel), both schedules."""
+    import rapidfuzz_amd as rfa
+
+    queries, choices = _dna_case(4, nq=10, nc=60, lo=100, hi=2500)
+    a = [queries[i % len(queries)] for i in range(len(choices))]
+    b = choices
+    metrics = np.zeros(len(a), np.int32)
+    A, B = ctx.strings(a, width=1), ctx.strings(b, width=1)
+    for cu in (0, 3, 30, 120, 500):
+        exp = oracle.pairs(metrics, a, b, kind=0, cutoff_u=cu)
+        for schedule in (0, 1):
+            got = rfa.pairs(ctx, A, B, metrics, kind=0, cutoff_u=cu, schedule=schedule)
+            assert np.array_equal(got, exp), f"cutoff {cu} schedule {schedule}"
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", ["tile", "few", "block"])
+def test_per_query_cutoffs(ctx, oracle, metric, shape):
+    """rfa_cdist_cutoffs: query q scored with its own cutoff, as a loop of Cached<Metric>(q) calls."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(70 + metric + 5 * len(shape))
+    if shape == "tile":
+        queries = _gen.random_strings(rng, 40, 0, 64, alpha_n=4)
+    elif shape == "few":
+        queries = _gen.random_strings(rng, 6, 0, 200, alpha_n=4)
+    else:
+        queries = _gen.random_strings(rng, 12, 100, 700, alpha_n=4)
+    choices = _gen.random_strings(rng, 30, 0, 300, alpha_n=4)
+    choices += [_gen.mutate(rng, q, int(rng.integers(0, 10))) for q in queries[:10]]
+    sc = rfa.Scorer(ctx, metric, ctx.strings(queries, width=1))
+    cs = ctx.strings(choices, width=1)
+    for kind in (0, 1):
+        for semantics in (0, 1):
+            cuts = rng.integers(0, 60, size=len(queries)).astype(np.uint64)
+            got = sc.cdist(cs, kind, semantics=semantics, cutoffs_u=cuts)
+            exp = np.stack([oracle.cdist(metric, kind, [queries[i]], choices, cutoff_u=int(cuts[i]),
+                                         multi=bool(semantics))[0] for i in range(len(queries))])
+            assert np.array_equal(got, exp), f"kind {kind} semantics {semantics}: {np.count_nonzero(got != exp)} differ"

@@ -97,3 +97,21 @@ def test_jaro_winkler_prefix_filter(oracle, reference):
                 exp = reference.cdist(5, kind, q, c, prefix_weight=pw, cutoff_d=cd)
                 got = oracle.cdist(5, kind, q, c, prefix_weight=pw, cutoff_d=cd)
                 assert _same(got, exp), f"JW pw {pw} cutoff {cd} kind {kind}"
+
+
+@pytest.mark.parametrize("weights", [(1, 1, 0), (1, 1, 1), (1, 1, 2)])
+def test_multi_levenshtein_weights(oracle, reference, weights):
+    """MultiLevenshtein: uniform distance whatever the weights, maximum() from the weights."""
+    rng = np.random.default_rng(sum(weights))
+    q = _gen.random_strings(rng, 16, 0, 32, alpha_n=4)
+    c = _gen.random_strings(rng, 30, 0, 60, alpha_n=4)
+    for kind in (0, 1, 2, 3):
+        for cu, cd in _cutoffs(0, kind):
+            exp = reference.cdist(0, kind, q, c, weights=weights, cutoff_u=cu, cutoff_d=cd, multi_len=32)
+            got = oracle.cdist(0, kind, q, c, weights=weights, cutoff_u=cu, cutoff_d=cd, multi=True)
+            assert _same(got, exp), f"MultiLevenshtein w {weights} kind {kind} cutoff {cu}/{cd}"
+
+
+def test_multi_levenshtein_rejects_weights(oracle):
+    with pytest.raises(ValueError):
+        oracle.cdist(0, 0, [[97]], [[98]], weights=(1, 1, 3), multi=True)
