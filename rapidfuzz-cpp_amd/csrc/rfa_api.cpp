@@ -186,7 +186,9 @@ rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const 
     }
     s->h_chars.resize(nunits * width);
     if (nunits) std::memcpy(s->h_chars.data(), chars, nunits * width);
-    size_t cb = std::max<size_t>(nunits * width, 64);
+    // 64 bytes of tail padding: kernels load code units in short vectors and may read
+    // a few units past the last string
+    size_t cb = nunits * width + 64;
     if (hipMalloc(&s->d_chars, cb) != hipSuccess || hipMalloc((void**)&s->d_off, s->h_off.size() * 8) != hipSuccess) {
         (void)hipGetLastError();
         if (s->d_chars) (void)hipFree(s->d_chars);
@@ -498,10 +500,19 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
     const uint64_t n = a->count;
     if (n == 0) return;
     if (!metrics) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "metrics is NULL");
-    for (uint64_t i = 0; i < n; ++i)
-        if (metrics[i] < RFA_LEVENSHTEIN || metrics[i] > RFA_OSA)
-            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "pair metrics must be Levenshtein, Indel, LCSseq or OSA");
     if (n > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 pairs");
+    // one pass: validate, note OSA, stage into pinned memory
+    int slot_idx = 0;
+    int32_t* staged = (int32_t*)ctx->staging.acquire(n * 4, &slot_idx);
+    bool has_osa = false;
+    bool bad_metric = false;
+    for (uint64_t i = 0; i < n; ++i) {
+        const int32_t m = metrics[i];
+        bad_metric |= (m < RFA_LEVENSHTEIN) | (m > RFA_OSA);
+        has_osa |= m == RFA_OSA;
+        staged[i] = m;
+    }
+    if (bad_metric) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "pair metrics must be Levenshtein, Indel, LCSseq or OSA");
     hipStream_t s = ctx->stream, s2 = ctx->stream2;
     const size_t bytes = (size_t)n * 8;
     void* dout = out_on_device ? out : ctx->out_tmp.get(bytes);
@@ -524,8 +535,9 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         uint64_t maxm = 0;
         for (uint64_t i = 0; i < n; ++i) maxm = std::max(maxm, a->len(i));
         if (maxm > 4096) throw ApiError(RFA_ERR_UNSUPPORTED, "the plain schedule handles patterns up to 4096 units");
-        std::vector<int32_t> met(metrics, metrics + n);
-        la.item_metric = upload(ctx->keep, ctx->lane_metric, met, s2);
+        int32_t* dmet = (int32_t*)ctx->lane_metric.get(n * 4);
+        ctx->staging.copy(slot_idx, dmet, n * 4, s2);
+        la.item_metric = dmet;
         la.nitems = n;
         const uint32_t nlanes = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 4 * 64,
                                                              ((n + 255) / 256) * 256);
@@ -534,24 +546,25 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         HIP_CHECK(launch_lane(width, 64, la, nlanes / 256, s2));
     }
     else {
-        // bucketed, planned on the device: pattern = the shorter string; <= 64 units -> lane
-        // kernel, longer -> block kernel (longest-processing-time-first queue)
-        std::vector<int32_t> met(metrics, metrics + n);
-        const int32_t* dmet = upload(ctx->keep, ctx->lane_metric, met, s);
+        // bucketed, planned on the device: pattern = the shorter string; Levenshtein / Indel /
+        // LCSseq patterns of <= 4096 units -> segmented kernel, OSA patterns of <= 64 units ->
+        // lane kernel, the rest -> block kernel (longest-processing-time-first queue)
+        int32_t* dmet = (int32_t*)ctx->lane_metric.get(n * 4);
+        ctx->staging.copy(slot_idx, dmet, n * 4, s);
         uint64_t* keys = (uint64_t*)ctx->plan_keys.get(n * 16);
         uint32_t* vals = (uint32_t*)ctx->plan_vals.get(n * 8);
         const size_t temp_bytes = plan_temp_bytes(n);
         void* temp = ctx->plan_temp.get(temp_bytes);
-        uint32_t* counters = (uint32_t*)ctx->queue.get(64); // [0] queue, [1] end of long, [2] end of short-64
+        uint32_t* counters = (uint32_t*)ctx->queue.get(64); // range ends and queues, written by the plan
         HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, keys, keys + n, vals, vals + n, temp, temp_bytes, counters,
                               s));
         const uint32_t* packed = vals + n;
         const uint64_t maxm = std::min(a->max_len, b->max_len);
         const uint64_t maxn = std::max(a->max_len, b->max_len);
-        // the short-pattern kernels (second stream) overlap the long-pattern queue (first stream)
+        // the segmented and lane kernels (second stream) overlap the block queue (first stream)
         HIP_CHECK(hipEventRecord(ctx->fork, s));
         HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
-        if (maxm > 64) {
+        if (maxm > 4096 || (has_osa && maxm > 64)) {
             BlockArgs ba{};
             ba.achars = a->d_chars;
             ba.aoff = a->d_off;
@@ -565,17 +578,33 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
             ba.out = dout;
             run_block(ctx, ALGO_MIXED, width, ba, maxm, maxn, false, counters);
         }
-        la.item_packed = packed;
-        la.pair_metric = dmet;
-        la.nitems = n;
-        // short-64 range [end_long, end_s64) then short-32 range [end_s64, n)
-        const uint32_t nblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 6, (n + 63) / 64);
-        la.begin_dev = counters + 1;
-        la.end_dev = counters + 2;
-        HIP_CHECK(launch_lane_short(width, 64, la, nblk, s2));
-        la.begin_dev = counters + 2;
-        la.end_dev = nullptr;
-        HIP_CHECK(launch_lane_short(width, 32, la, nblk, s2));
+        SegArgs sa{};
+        sa.achars = a->d_chars;
+        sa.aoff = a->d_off;
+        sa.bchars = b->d_chars;
+        sa.boff = b->d_off;
+        sa.item_packed = packed;
+        sa.pair_metric = dmet;
+        sa.counters = counters;
+        sa.queue = counters + 11;
+        sa.spec = base;
+        sa.out = dout;
+        const uint32_t nb = seg_symbol_bits(width, std::max(a->max_unit(), b->max_unit()));
+        const uint32_t sblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8, (n + 255) / 256 * 4);
+        HIP_CHECK(launch_seg(width, nb, sa, std::max(sblk, 1u), s2));
+        if (has_osa) {
+            la.item_packed = packed;
+            la.pair_metric = dmet;
+            la.nitems = n;
+            // OSA 33-64 unit range [counters[1], counters[2]), then 1-32 units [counters[2], counters[3])
+            const uint32_t nblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 6, (n + 63) / 64);
+            la.begin_dev = counters + 1;
+            la.end_dev = counters + 2;
+            HIP_CHECK(launch_lane_short(width, 64, la, nblk, s2));
+            la.begin_dev = counters + 2;
+            la.end_dev = counters + 3;
+            HIP_CHECK(launch_lane_short(width, 32, la, nblk, s2));
+        }
     }
     HIP_CHECK(hipEventRecord(ctx->join, s2));
     HIP_CHECK(hipStreamWaitEvent(s, ctx->join, 0));
@@ -635,6 +664,7 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
                           &ctx->topk_run[1], &ctx->topk_cand_s, &ctx->topk_cand_i, &ctx->topk_counts, &ctx->topk_q,
                           &ctx->topk_out, &ctx->qcut})
             b->release();
+        ctx->staging.release();
         (void)hipEventDestroy(ctx->fork);
         (void)hipEventDestroy(ctx->join);
         (void)hipStreamDestroy(ctx->stream2);

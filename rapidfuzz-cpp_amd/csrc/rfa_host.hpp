@@ -84,6 +84,61 @@ T* upload(Keep& keep, DevBuf& b, const std::vector<T>& v, hipStream_t s)
     return d;
 }
 
+// Pinned host staging for per-call uploads.  Two slots are used in turn, and a
+// slot is rewritten only after the copy that last read it has completed, so a
+// call never waits for the device work of the call before it.
+struct Staging {
+    struct Slot {
+        void* host = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    Slot slot[2];
+    int next = 0;
+
+    // a pinned buffer of >= bytes that no pending copy reads; *idx names it for copy()
+    void* acquire(size_t bytes, int* idx)
+    {
+        Slot& s = slot[next];
+        *idx = next;
+        next ^= 1;
+        if (s.pending) {
+            HIP_CHECK(hipEventSynchronize(s.done));
+            s.pending = false;
+        }
+        if (!s.done) HIP_CHECK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        if (s.cap < bytes) {
+            if (s.host) HIP_CHECK(hipHostFree(s.host));
+            s.host = nullptr;
+            s.cap = 0;
+            const size_t want = std::max<size_t>(bytes, 4096);
+            if (hipHostMalloc(&s.host, want, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                s.host = nullptr;
+                throw ApiError(RFA_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+            }
+            s.cap = want;
+        }
+        return s.host;
+    }
+    void copy(int idx, void* dst, size_t bytes, hipStream_t st)
+    {
+        HIP_CHECK(hipMemcpyAsync(dst, slot[idx].host, bytes, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipEventRecord(slot[idx].done, st));
+        slot[idx].pending = true;
+    }
+    // after the streams are idle
+    void release()
+    {
+        for (Slot& s : slot) {
+            if (s.done) (void)hipEventDestroy(s.done);
+            if (s.host) (void)hipHostFree(s.host);
+            s = Slot{};
+        }
+    }
+};
+
 inline uint64_t host_ceil_div(uint64_t a, uint64_t b) { return a / b + (a % b != 0); }
 
 inline uint32_t log2_ceil(uint64_t x)
@@ -146,6 +201,7 @@ struct rfa_context {
     size_t lane_arena_bytes_zeroed = 0;
     uint32_t gen_next = 1;
     rfa::Keep keep; // host sources of in-flight uploads
+    rfa::Staging staging; // pinned per-call uploads of the pairs path
 };
 
 struct rfa_strings {
@@ -160,7 +216,30 @@ struct rfa_strings {
     bool view = false; // borrows d_chars/d_off of another set
     std::shared_ptr<rfa::ServeIndex> index; // built on first top-k use
 
+    mutable int64_t max_unit_cached = -1;
+
     uint64_t len(uint64_t i) const { return h_off[i + 1] - h_off[i]; }
+    // largest code unit of the set (views, which hold no host copy, report the width's maximum)
+    uint32_t max_unit() const
+    {
+        if (max_unit_cached >= 0) return (uint32_t)max_unit_cached;
+        uint32_t mx = 0;
+        const uint64_t nunits = h_off.empty() ? 0 : h_off.back() - h_off.front();
+        if (h_chars.size() < nunits * width)
+            mx = width == 1 ? 0xffu : (width == 2 ? 0xffffu : 0xffffffffu);
+        else if (width == 1)
+            for (uint8_t c : h_chars) mx = c > mx ? c : mx;
+        else if (width == 2) {
+            const uint16_t* u = (const uint16_t*)h_chars.data();
+            for (size_t i = 0, e = h_chars.size() / 2; i < e; ++i) mx = u[i] > mx ? u[i] : mx;
+        }
+        else {
+            const uint32_t* u = (const uint32_t*)h_chars.data();
+            for (size_t i = 0, e = h_chars.size() / 4; i < e; ++i) mx = u[i] > mx ? u[i] : mx;
+        }
+        max_unit_cached = mx;
+        return mx;
+    }
     uint32_t unit(uint64_t pos) const
     {
         if (width == 1) return h_chars[pos];
@@ -193,6 +272,7 @@ struct rfa_scorer {
     std::vector<uint32_t> block_queries; // scored by the block kernel
     uint32_t* d_block_queries = nullptr;
     uint64_t block_max_len = 0, block_min_len = 0;
+    std::shared_ptr<void> few_plan; // cached plan of the one-pair-per-lane path (last choice set)
     uint32_t block_u8_rows = 0; // distinct symbols + 1 of the 8-bit block queries (0: unknown)
 };
 

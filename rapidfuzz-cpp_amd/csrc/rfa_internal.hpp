@@ -94,8 +94,29 @@ struct LaneArgs {
     const int32_t* pair_metric;
 };
 
+// Segmented kernel (rfa_seg.hip): pairs with patterns of <= 4096 units under the
+// Levenshtein or LCS recurrence, SEG = 64 >> k lanes per pair for class k = 0..6.
+struct SegArgs {
+    const void* achars;
+    const uint64_t* aoff;
+    const void* bchars;
+    const uint64_t* boff;
+    const uint32_t* item_packed; // planned items, pair | swap << 31
+    const int32_t* pair_metric;
+    const uint32_t* counters;    // class k: items [counters[3 + k], counters[4 + k])
+    uint32_t* queue;             // group counter, zero at launch
+    ScoreSpec spec;
+    void* out;                   // out[pair]
+};
+
+// symbol bits the segmented kernel compares for a launch whose largest code unit is max_symbol
+uint32_t seg_symbol_bits(int char_width, uint64_t max_symbol);
+hipError_t launch_seg(int char_width, uint32_t nb, const SegArgs& a, uint32_t nblocks, hipStream_t s);
+
 size_t plan_temp_bytes(uint64_t n);
-// counters[0]: block queue, [1]: end of the long range, [2]: end of the short-64 range
+// counters[0]: block queue; [1]: end of the block range; [2], [3]: ends of the OSA
+// 33-64 and 1-32 unit ranges; [4 + k]: end of segmented class k (k = 0..6);
+// [11]: segmented kernel queue
 hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t* metric, uint64_t n,
                        uint64_t* keys_in, uint64_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp,
                        size_t temp_bytes, uint32_t* counters, hipStream_t s);

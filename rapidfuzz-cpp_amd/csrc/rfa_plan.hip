@@ -1,14 +1,19 @@
 // rfa_plan.hip -- device-side planning of the bucketed pairs schedule.
 //
-//   classify : per pair, the pattern (the shorter string), its word count and
-//              a sort key: long patterns (> 64 units) get 2^40 + words * text
-//              length (their cost), short ones get the text length.
-//   sort     : one descending radix sort of (key, pair|swap<<31) -- long pairs
-//              come first, most expensive first (longest-processing-time
-//              order for the block kernel's queue); short pairs follow, grouped
-//              by text length so the lanes of a wave finish together.
-//   split    : one thread finds the long/short boundary and resets the block
-//              kernel's queue counter.
+//   classify : per pair, the kernel, the pattern (the shorter string) and a
+//              sort key (descending order):
+//                segmented : Levenshtein / Indel / LCSseq pairs whose pattern
+//                            (the shorter string) has <= 4096 units; class
+//                            c = log2(lanes per pair) in bits 36-38,
+//                            Levenshtein flag in bit 35, text length below
+//                OSA short : 2^39 (+ 2^38 for 33-64 units) + text length
+//                block     : the rest; key 2^40 + words * text length (cost)
+//   sort     : one descending radix sort of (key, pair|swap<<31) -- block pairs
+//              come first, most expensive first (longest-processing-time order
+//              for the block kernel's queue); the segmented classes follow from
+//              widest to narrowest, grouped by metric and text length so the
+//              lanes of a wave finish together.
+//   split    : one thread finds the range boundaries and resets both queues.
 // The host never reads the plan back, so a call costs O(1) host work.
 #include <hipcub/hipcub.hpp>
 
@@ -16,39 +21,52 @@
 
 namespace rfa {
 
-constexpr uint64_t LONG_FLAG = 1ull << 40;
-constexpr uint64_t S64_FLAG = 1ull << 39;
-constexpr uint64_t LOW37 = (1ull << 37) - 1;
+constexpr uint64_t BLOCK_FLAG = 1ull << 40;
+constexpr uint64_t OSA_FLAG = 1ull << 39;
+constexpr uint64_t S64_FLAG = 1ull << 38;
+constexpr uint64_t LEV_FLAG = 1ull << 35;
+constexpr uint64_t LOW35 = (1ull << 35) - 1;
+constexpr uint32_t SEG_MAX_UNITS = 4096;
 
-// key bits: [40] long pattern (> 64 units) with the cost (words x text length) below;
-// otherwise [39] pattern of 33-64 units, [37:38] metric, [0:36] text length.
+__device__ __forceinline__ uint64_t pow2_at_least(uint64_t x)
+{
+    uint64_t c = 0;
+    while ((1ull << c) < x) ++c;
+    return c;
+}
+
 __global__ void k_classify(const uint64_t* __restrict__ aoff, const uint64_t* __restrict__ boff,
                            const int32_t* __restrict__ metric, uint64_t n, uint64_t* __restrict__ keys,
                            uint32_t* __restrict__ vals)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t la = aoff[i + 1] - aoff[i], lb = boff[i + 1] - boff[i];
-        const bool swap = lb < la;
-        const uint64_t m = swap ? lb : la, t = swap ? la : lb;
+        const bool pat_b = lb < la; // the pattern is the shorter string
+        const uint64_t s = pat_b ? lb : la, l = pat_b ? la : lb;
+        const int32_t met = metric[i];
+        const uint64_t tl = l < LOW35 ? l : LOW35;
         uint64_t key;
-        if (m > 64) {
-            const uint64_t cost = ((m + 63) / 64) * t;
-            key = LONG_FLAG | (cost < (LONG_FLAG - 1) ? cost : (LONG_FLAG - 1));
+        if (met != RFA_OSA && s <= SEG_MAX_UNITS) {
+            key = (pow2_at_least((s + 63) / 64) << 36) | (met == RFA_LEVENSHTEIN ? LEV_FLAG : 0) | tl;
+        }
+        else if (met == RFA_OSA && s <= 64) {
+            key = OSA_FLAG | (s > 32 ? S64_FLAG : 0) | tl;
         }
         else {
-            key = (m > 32 ? S64_FLAG : 0) | ((uint64_t)(metric[i] & 3) << 37) | (t < LOW37 ? t : LOW37);
+            const uint64_t cost = ((s + 63) / 64) * l;
+            key = BLOCK_FLAG | (cost < (BLOCK_FLAG - 1) ? cost : (BLOCK_FLAG - 1));
         }
         keys[i] = key;
-        vals[i] = (uint32_t)i | (swap ? 0x80000000u : 0u);
+        vals[i] = (uint32_t)i | (pat_b ? 0x80000000u : 0u);
     }
 }
 
-// First index in [lo, hi) whose key lacks `flag` (keys sorted descending, flagged first).
-__device__ uint64_t first_without(const uint64_t* keys, uint64_t lo, uint64_t hi, uint64_t flag)
+// First index in [lo, hi) whose key is below thr (keys sorted descending).
+__device__ uint64_t first_below(const uint64_t* keys, uint64_t lo, uint64_t hi, uint64_t thr)
 {
     while (lo < hi) {
         uint64_t mid = (lo + hi) / 2;
-        if (keys[mid] & flag)
+        if (keys[mid] >= thr)
             lo = mid + 1;
         else
             hi = mid;
@@ -59,17 +77,24 @@ __device__ uint64_t first_without(const uint64_t* keys, uint64_t lo, uint64_t hi
 __global__ void k_split(const uint64_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ counters)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint64_t end_long = first_without(keys, 0, n, LONG_FLAG);
-    const uint64_t end_s64 = first_without(keys, end_long, n, S64_FLAG);
+    uint64_t e = first_below(keys, 0, n, BLOCK_FLAG);
+    counters[1] = (uint32_t)e;
+    e = first_below(keys, e, n, OSA_FLAG | S64_FLAG);
+    counters[2] = (uint32_t)e;
+    e = first_below(keys, e, n, OSA_FLAG);
+    counters[3] = (uint32_t)e;
+    for (uint32_t k = 0; k < 7; ++k) {
+        e = first_below(keys, e, n, (uint64_t)(6 - k) << 36);
+        counters[4 + k] = (uint32_t)e;
+    }
     counters[0] = 0;
-    counters[1] = (uint32_t)end_long;
-    counters[2] = (uint32_t)end_s64;
+    counters[11] = 0;
 }
 
 size_t plan_temp_bytes(uint64_t n)
 {
     size_t bytes = 0;
-    hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+    (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
                                                  (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 41);
     return bytes;
 }

@@ -1,0 +1,312 @@
+// rfa_seg.hip -- segmented pairs kernel: independent pairs whose pattern (the
+// shorter string) has at most 4096 units and whose metric runs the Levenshtein
+// or the LCS recurrence (Levenshtein, Indel, LCSseq).
+//
+// Layout: a pair takes a segment of SEG lanes (SEG = the power of two >= its
+// word count, 1..64); lane w of the segment holds pattern word w, so a wave
+// runs 64 / SEG pairs at once.  The planner sorts pairs by (SEG class, metric,
+// text length), so a wave's pairs share their recurrence and trip count.
+//
+// Match masks without a symbol map: each lane keeps the bit-planes of its 64
+// pattern units, plane_b bit i = bit b of unit i, and for text symbol c
+//   PM(c) = AND_b (plane_b XNOR bit_b(c))   over the NB low symbol bits
+// (NB >= bit length of the largest symbol of the launch; a text symbol with a
+// bit set at or above NB matches nothing).  That is NB three-input bit ops per
+// 32-bit half word and no LDS, so occupancy is set by registers alone.
+//
+// Carries between the words of a pair use the carry-lookahead of the block
+// kernel (generate / propagate ballots resolved by one 64-bit add), with the
+// last lane of every segment masked out of the add so no carry crosses into
+// the next pair, and the HP carry-in forced to 1 at segment starts
+// (Levenshtein_impl.hpp:700-736, LCSseq_impl.hpp:286-314).
+//
+// Work: group g of class k holds items [begin_k + g * PPW, +PPW) of the sorted
+// list; waves take groups from one queue, widest class first (the longest
+// pairs start first).
+#include "rfa_block_impl.hpp"
+#include "rfa_device.hpp"
+#include "rfa_internal.hpp"
+#include "rfa_semantics.hpp"
+#include "rfa_steps.hpp"
+
+namespace rfa {
+
+// bit i set for every lane i that starts a segment of SEG lanes
+template <uint32_t SEG>
+__device__ __forceinline__ constexpr uint64_t seg_first_mask()
+{
+    uint64_t m = 0;
+    for (uint32_t i = 0; i < 64; i += SEG) m |= 1ull << i;
+    return m;
+}
+
+constexpr uint32_t T_AND_XNOR = (TA & ~(TB ^ TC)) & 0xFF; // a & ~(b ^ c)
+
+// K consecutive code units from p (any alignment: the loads are dword loads of
+// the covering bytes).  Reads up to K - 1 units past the string's end, which the
+// 64-byte tail padding of every string set keeps inside the allocation.
+template <typename CharT, uint32_t K>
+__device__ __forceinline__ void load_units(const CharT* p, uint32_t* out)
+{
+    CharT tmp[K];
+    __builtin_memcpy(tmp, p, sizeof(tmp));
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) out[k] = (uint32_t)tmp[k];
+}
+
+// Recurrence over the text for one word per lane; returns the lane's share of
+// the pair's raw value (Levenshtein: sum of vertical deltas of the last
+// column; LCS: matched rows).
+template <uint32_t SEG, typename CharT, uint32_t NB>
+__device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const CharT* __restrict__ t, uint32_t cnt,
+                                           uint32_t n, bool wact, bool lev, uint32_t nmax, uint32_t lane)
+{
+    constexpr uint64_t FIRSTM = seg_first_mask<SEG>();
+    constexpr uint64_t LASTM = FIRSTM << (SEG - 1);
+    // bit-planes of this word's units (lo: units 0-31, hi: units 32-63), built from
+    // loads of CH units; units past cnt only set bits that vml / vmh mask off
+    constexpr uint32_t CH = 8;
+    uint32_t pl[NB], ph[NB];
+#pragma unroll
+    for (uint32_t b = 0; b < NB; ++b) pl[b] = ph[b] = 0;
+    for (uint32_t i0 = 0; i0 < min(cnt, 32u); i0 += CH) {
+        uint32_t u[CH];
+        load_units<CharT, CH>(p + i0, u);
+#pragma unroll
+        for (uint32_t k = 0; k < CH; ++k)
+#pragma unroll
+            for (uint32_t b = 0; b < NB; ++b) pl[b] |= ((u[k] >> b) & 1u) << (i0 + k);
+    }
+    for (uint32_t i0 = 32; i0 < cnt; i0 += CH) {
+        uint32_t u[CH];
+        load_units<CharT, CH>(p + i0, u);
+#pragma unroll
+        for (uint32_t k = 0; k < CH; ++k)
+#pragma unroll
+            for (uint32_t b = 0; b < NB; ++b) ph[b] |= ((u[k] >> b) & 1u) << (i0 - 32 + k);
+    }
+    const uint32_t vml = cnt >= 32 ? ~0u : ((1u << cnt) - 1u);
+    const uint32_t vmh = cnt >= 64 ? ~0u : (cnt > 32 ? ((1u << (cnt - 32)) - 1u) : 0u);
+
+    const bool any_lev = ballot(wact && lev) != 0, any_lcs = ballot(wact && !lev) != 0;
+    uint64_t VP = ~0ull, VN = 0; // LCS keeps S in VP
+
+    // one text position: column j, text unit c
+    auto step = [&](uint32_t j, uint32_t c) {
+        const bool act = wact && j < n;
+        uint32_t xl = vml, xh = vmh;
+        if constexpr (NB < 32) {
+            if ((c >> NB) != 0) xl = xh = 0;
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < NB; ++b) {
+            const uint32_t sb = (uint32_t)(-(int32_t)((c >> b) & 1u)); // all ones iff bit b of c
+            xl = bop3<T_AND_XNOR>(xl, pl[b], sb);
+            xh = bop3<T_AND_XNOR>(xh, ph[b], sb);
+        }
+        const uint64_t X = ((uint64_t)xh << 32) | xl;
+        if (any_lev) {
+            const bool al = act && lev;
+            const uint64_t vp = VP, vn = VN;
+            uint64_t D0, HP, HN;
+            if constexpr (SEG == 1) {
+                D0 = (((X & vp) + vp) ^ vp) | X | vn;
+                HP = vn | ~(D0 | vp);
+                HN = D0 & vp;
+                HP = (HP << 1) | 1u;
+                HN = HN << 1;
+            }
+            else {
+                const uint64_t X1 = X | 1;
+                const uint64_t d0a = (((X & vp) + vp) ^ vp) | X | vn;
+                const uint64_t d0b = (((X1 & vp) + vp) ^ vp) | X1 | vn;
+                // outgoing HN carry for incoming carry 0 / 1 (monotone in it)
+                const bool ga = al && ((d0a & vp) >> 63);
+                const bool gb = al && ((d0b & vp) >> 63);
+                const uint64_t G = ballot(ga) & ~LASTM, Pp = ballot(gb && !ga) & ~LASTM;
+                const uint32_t hin = lane_bit(((G | Pp) + G) ^ Pp, lane);
+                D0 = hin ? d0b : d0a;
+                HP = vn | ~(D0 | vp);
+                HN = D0 & vp;
+                const uint64_t HPm = ballot(al && (HP >> 63));
+                const uint32_t hpin = lane_bit((HPm << 1) | FIRSTM, lane);
+                HP = (HP << 1) | hpin;
+                HN = (HN << 1) | hin;
+            }
+            if (al) {
+                VP = HN | ~(D0 | HP);
+                VN = HP & D0;
+            }
+        }
+        if (any_lcs) {
+            const bool ac = act && !lev;
+            const uint64_t Sv = VP;
+            const uint64_t u = Sv & X;
+            uint64_t x = Sv + u;
+            if constexpr (SEG > 1) {
+                const bool gg = ac && (x < Sv);
+                const bool pp = ac && !gg && (x == ~0ull);
+                const uint64_t G = ballot(gg) & ~LASTM, Pp = ballot(pp) & ~LASTM;
+                x += lane_bit(((G | Pp) + G) ^ Pp, lane);
+            }
+            if (ac) VP = x | (Sv - u);
+        }
+    };
+    auto chunk = [&](const uint32_t* u, uint32_t j0) {
+#pragma unroll
+        for (uint32_t k = 0; k < CH; ++k) {
+            if (j0 + k >= nmax) break;
+            step(j0 + k, u[k]);
+        }
+    };
+    // text units CH at a time in two register buffers: one chunk is loaded while
+    // the other runs (fixed roles, so no register copies between chunks).  Every
+    // lane issues every load -- lanes with nothing left re-read the start of their
+    // text -- so the loads stay unconditional and their waits exact.
+    const bool tl = wact && n > 0;
+    uint32_t ua[CH], ub[CH];
+    load_units<CharT, CH>(t, ua);
+    for (uint32_t j0 = 0; j0 < nmax; j0 += 2 * CH) {
+        load_units<CharT, CH>(tl && j0 + CH < n ? t + j0 + CH : t, ub);
+        chunk(ua, j0);
+        if (j0 + CH >= nmax) break;
+        load_units<CharT, CH>(tl && j0 + 2 * CH < n ? t + j0 + 2 * CH : t, ua);
+        chunk(ub, j0 + CH);
+    }
+    if (!wact) return 0;
+    const uint64_t M = ((uint64_t)vmh << 32) | vml;
+    if (lev) return __popcll(VP & M) - __popcll(VN & M);
+    return __popcll(~VP & M);
+}
+
+// One group of PPW pairs of class SEG, items [first, min(first + PPW, end)).
+// INTK: integer result kinds (distance, similarity) with the integer epilogue.
+template <uint32_t SEG, typename CharT, uint32_t NB, bool INTK>
+__device__ __forceinline__ void seg_group(const SegArgs& a, uint32_t first, uint32_t end, uint32_t lane)
+{
+    const CharT* __restrict__ A = (const CharT*)a.achars;
+    const CharT* __restrict__ B = (const CharT*)a.bchars;
+    const uint32_t slot = lane / SEG, w = lane % SEG;
+    const uint32_t item = first + slot;
+    const bool valid = item < end;
+    uint32_t pair = 0, la = 0, lb = 0, m = 0, n = 0;
+    const CharT* p = A;
+    const CharT* t = B;
+    ScoreSpec spec = a.spec;
+    if (valid) {
+        const uint32_t pk = a.item_packed[item];
+        pair = pk & 0x7fffffffu;
+        const bool swap = (pk >> 31) != 0;
+        const uint64_t oa = a.aoff[pair], ob = a.boff[pair];
+        la = (uint32_t)(a.aoff[pair + 1] - oa);
+        lb = (uint32_t)(a.boff[pair + 1] - ob);
+        p = swap ? B + ob : A + oa;
+        t = swap ? A + oa : B + ob;
+        m = swap ? lb : la;
+        n = swap ? la : lb;
+        spec.metric = a.pair_metric[pair];
+        if (spec.kind == RFA_DISTANCE && spec.metric != RFA_LCSSEQ) spec.band_k = spec.cutoff_u;
+    }
+    const bool lev = spec.metric == RFA_LEVENSHTEIN;
+    RawResult res{0, 0, 0};
+    bool done = !valid;
+    if (valid && length_filter(spec, lev ? ALGO_LEV : ALGO_LCS, la, lb, &res)) done = true;
+    const uint32_t W = (m + 63) / 64;
+    const bool wact = !done && w < W;
+    const uint32_t cnt = wact ? min(64u, m - w * 64) : 0u; // pattern units of this word
+    uint32_t nmax = wact ? n : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor(nmax, off, 64));
+    nmax = uniform(nmax);
+    int32_t v = seg_run<SEG, CharT, NB>(p + (wact ? w * 64 : 0u), t, cnt, n, wact, lev, nmax, lane);
+#pragma unroll
+    for (uint32_t off = SEG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (valid && w == 0) {
+        if (!done) res.v = lev ? (uint64_t)((int64_t)n + v) : (uint64_t)v;
+        if constexpr (INTK)
+            ((uint64_t*)a.out)[pair] = finalize_int(spec, la, lb, res.v);
+        else
+            finalize(spec, la, lb, res, a.out, pair);
+    }
+}
+
+// Classes k = 0..6 hold SEG = 64 >> k; counters[3 + k] .. counters[4 + k] are their item ranges.
+template <typename CharT, uint32_t NB, bool INTK>
+__global__ __launch_bounds__(256) void k_seg(SegArgs a)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t gend[7], ibeg[7];
+    uint32_t acc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 7; ++k) {
+        const uint32_t b = uniform(a.counters[3 + k]), e = uniform(a.counters[4 + k]);
+        const uint32_t ppw = 1u << k; // pairs per wave: 64 / SEG
+        ibeg[k] = b;
+        acc += e > b ? (e - b + ppw - 1) / ppw : 0u;
+        gend[k] = acc;
+    }
+    for (;;) {
+        uint32_t g = 0;
+        if (lane == 0) g = atomicAdd(a.queue, 1u);
+        g = uniform((uint32_t)__shfl((int)g, 0, 64));
+        if (g >= acc) break;
+        uint32_t k = 0;
+        while (k < 6 && g >= gend[k]) ++k;
+        const uint32_t g0 = k ? gend[k - 1] : 0u;
+        const uint32_t first = ibeg[k] + ((g - g0) << k);
+        const uint32_t end = uniform(a.counters[4 + k]);
+        switch (k) {
+        case 0: seg_group<64, CharT, NB, INTK>(a, first, end, lane); break;
+        case 1: seg_group<32, CharT, NB, INTK>(a, first, end, lane); break;
+        case 2: seg_group<16, CharT, NB, INTK>(a, first, end, lane); break;
+        case 3: seg_group<8, CharT, NB, INTK>(a, first, end, lane); break;
+        case 4: seg_group<4, CharT, NB, INTK>(a, first, end, lane); break;
+        case 5: seg_group<2, CharT, NB, INTK>(a, first, end, lane); break;
+        default: seg_group<1, CharT, NB, INTK>(a, first, end, lane); break;
+        }
+    }
+}
+
+uint32_t seg_symbol_bits(int char_width, uint64_t max_symbol)
+{
+    uint32_t bits = 1;
+    while (bits < 32 && (max_symbol >> bits) != 0) ++bits;
+    if (char_width == 1 || bits <= 8) return 8;
+    if (char_width == 2 || bits <= 16) return 16;
+    return bits <= 24 ? 24 : 32;
+}
+
+template <bool INTK>
+static void launch_seg_k(int char_width, uint32_t nb, const SegArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    const dim3 grid(nblocks), block(256);
+    if (char_width == 1)
+        hipLaunchKernelGGL((k_seg<uint8_t, 8, INTK>), grid, block, 0, s, a);
+    else if (char_width == 2) {
+        if (nb <= 8)
+            hipLaunchKernelGGL((k_seg<uint16_t, 8, INTK>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_seg<uint16_t, 16, INTK>), grid, block, 0, s, a);
+    }
+    else {
+        if (nb <= 8)
+            hipLaunchKernelGGL((k_seg<uint32_t, 8, INTK>), grid, block, 0, s, a);
+        else if (nb <= 16)
+            hipLaunchKernelGGL((k_seg<uint32_t, 16, INTK>), grid, block, 0, s, a);
+        else if (nb <= 24)
+            hipLaunchKernelGGL((k_seg<uint32_t, 24, INTK>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_seg<uint32_t, 32, INTK>), grid, block, 0, s, a);
+    }
+}
+
+hipError_t launch_seg(int char_width, uint32_t nb, const SegArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    if (a.spec.kind == RFA_DISTANCE || a.spec.kind == RFA_SIMILARITY)
+        launch_seg_k<true>(char_width, nb, a, nblocks, s);
+    else
+        launch_seg_k<false>(char_width, nb, a, nblocks, s);
+    return hipGetLastError();
+}
+
+} // namespace rfa

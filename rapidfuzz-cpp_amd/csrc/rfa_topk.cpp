@@ -323,6 +323,19 @@ static void enqueue_fused(const ServeIndex& ix, const FusedPlan& p, const ScoreS
 
 // ---- cdist of a few queries: one pair per lane --------------------------------
 
+// The plan of a scorer's queries against one choice set, uploaded once (the
+// queries and the set are immutable, so repeated calls reuse it).
+struct FewQueryPlan {
+    std::shared_ptr<ServeIndex> ix; // identifies the set (and keeps its index alive)
+    FusedPlan p;
+    size_t off_qrow = 0;
+    uint8_t* d_blob = nullptr;
+    ~FewQueryPlan()
+    {
+        if (d_blob) (void)hipFree(d_blob);
+    }
+};
+
 bool cdist_few_queries(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, int algo, void* dout,
                        const uint64_t* d_qcut)
 {
@@ -333,23 +346,33 @@ bool cdist_few_queries(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& s
     if (ch->view) return false; // chunk views of another set carry no host copy to index
     if (algo != ALGO_LCS && algo != ALGO_LEV) return false;
     auto ix = get_index(ch);
-    std::vector<uint32_t> order(nq);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sc->q->len(a) < sc->q->len(b); });
-    std::vector<std::pair<const uint8_t*, uint32_t>> qs(nq);
-    for (uint64_t i = 0; i < nq; ++i)
-        qs[i] = {sc->q->h_chars.data() + sc->q->h_off[order[i]], (uint32_t)sc->q->len(order[i])};
-    FusedKind f;
-    f.ok = true;
-    f.algo = algo;
-    f.bound = Bound::NONE; // every result is wanted: all groups
-    FusedPlan p;
-    plan_fused(*ix, f, qs, ch->max_len, ctx->num_cus, p);
-    const size_t off_qrow = align8(p.blob.size());
-    p.blob.resize(off_qrow + nq * 4);
-    std::memcpy(p.blob.data() + off_qrow, order.data(), nq * 4);
+    auto* cached = static_cast<FewQueryPlan*>(sc->few_plan.get());
+    if (!cached || cached->ix != ix) {
+        auto plan = std::make_shared<FewQueryPlan>();
+        plan->ix = ix;
+        std::vector<uint32_t> order(nq);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return sc->q->len(a) < sc->q->len(b); });
+        std::vector<std::pair<const uint8_t*, uint32_t>> qs(nq);
+        for (uint64_t i = 0; i < nq; ++i)
+            qs[i] = {sc->q->h_chars.data() + sc->q->h_off[order[i]], (uint32_t)sc->q->len(order[i])};
+        FusedKind f;
+        f.ok = true;
+        f.algo = algo;
+        f.bound = Bound::NONE; // every result is wanted: all groups
+        plan_fused(*ix, f, qs, ch->max_len, ctx->num_cus, plan->p);
+        plan->off_qrow = align8(plan->p.blob.size());
+        plan->p.blob.resize(plan->off_qrow + nq * 4);
+        std::memcpy(plan->p.blob.data() + plan->off_qrow, order.data(), nq * 4);
+        plan->d_blob = (uint8_t*)to_device(plan->p.blob);
+        sc->few_plan = plan;
+        cached = plan.get();
+    }
+    const FusedPlan& p = cached->p;
+    const size_t off_qrow = cached->off_qrow;
     hipStream_t s = ctx->stream;
-    const uint8_t* d_blob = upload(ctx->keep, ctx->topk_q, p.blob, s);
+    const uint8_t* d_blob = cached->d_blob;
     ServeArgs a{};
     a.gdata = ix->d_data;
     a.gbase = ix->d_base;

@@ -82,3 +82,74 @@ def test_pairs_rejects_count_mismatch(ctx):
 
     with pytest.raises(rfa.InvalidArgument):
         rfa.pairs(ctx, ctx.strings([b"ab", b"c"]), ctx.strings([b"ab"]), [0, 0])
+
+
+def _class_pairs(rng, width, sym_hi):
+    """Patterns across every lanes-per-pair class (1..64 words) and the block boundary
+    (4096 / 4097 units), texts up to twice as long, symbols below sym_hi."""
+    lens = [0, 1, 5, 31, 32, 33, 63, 64, 65, 127, 128, 129, 200, 255, 256, 257, 511, 512, 513, 1000, 1023, 1024,
+            1025, 2047, 2048, 2049, 3000, 4095, 4096, 4097]
+    a, b = [], []
+    lo = max(0, sym_hi - 6)  # a small alphabet near the top of the range, so every symbol bit matters
+    for m in lens:
+        for _ in range(3):
+            s = rng.integers(lo, sym_hi, size=m).astype(np.uint64)
+            if rng.random() < 0.5:
+                t = _gen.mutate(rng, s - lo, int(rng.integers(0, 12)), alpha_lo=0, alpha_n=sym_hi - lo).astype(np.uint64) + lo
+            else:
+                t = rng.integers(lo, sym_hi, size=int(rng.integers(m, 2 * m + 3))).astype(np.uint64)
+            # a few symbols of the wider range on both sides
+            if m and rng.random() < 0.5:
+                s[rng.integers(0, m)] = rng.integers(0, sym_hi)
+            if rng.random() < 0.5:
+                a.append(s)
+                b.append(t)
+            else:
+                a.append(t)
+                b.append(s)
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[width]
+    return [x.astype(dt).astype(np.uint32) for x in a], [x.astype(dt).astype(np.uint32) for x in b]
+
+
+@pytest.mark.parametrize("width,sym_hi", [(1, 256), (1, 100), (2, 200), (2, 65536), (4, 250), (4, 65536),
+                                          (4, 1 << 20), (4, 1 << 31)])
+@pytest.mark.parametrize("kind,cu,cd", [(0, None, None), (0, 40, None), (1, 30, None), (2, None, 0.4),
+                                        (3, None, 0.6)])
+def test_pairs_segment_classes(ctx, oracle, width, sym_hi, kind, cu, cd):
+    """The segmented kernel's classes (pattern words 1..64), symbol bit counts 8/16/24/32,
+    metrics mixed inside a class, OSA and > 4096-unit patterns routed elsewhere."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(sym_hi % 1000 + width + 17 * kind)
+    a, b = _class_pairs(rng, width, sym_hi)
+    metrics = rng.integers(0, 3, size=len(a)).astype(np.int32)
+    metrics[rng.random(len(a)) < 0.1] = 3
+    A = ctx.strings(a, width=width)
+    B = ctx.strings(b, width=width)
+    if cu is None:
+        cu = rfa.INT64_MAX if kind == 0 else 0
+    if cd is None:
+        cd = 1.0 if kind == 2 else 0.0
+    got = rfa.pairs(ctx, A, B, metrics, kind=kind, cutoff_u=cu, cutoff_d=cd)
+    exp = oracle.pairs(metrics, a, b, kind=kind, cutoff_u=cu, cutoff_d=cd)
+    if got.dtype == np.float64:
+        bad = np.flatnonzero(got.view(np.uint64) != exp.view(np.uint64))
+    else:
+        bad = np.flatnonzero(got != exp)
+    assert bad.size == 0, (f"{bad.size} mismatches; first {bad[0]}: metric {metrics[bad[0]]} "
+                           f"lens {len(a[bad[0]])}/{len(b[bad[0]])} got {got[bad[0]]} expected {exp[bad[0]]}")
+
+
+def test_pairs_many_short_mixed(ctx, oracle):
+    """Thousands of short pairs: full 64-pair waves, metric boundaries inside waves, ragged tails."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(99)
+    a, b = _pairs(rng, 5000, 300, 6, 4)
+    metrics = rng.integers(0, 3, size=len(a)).astype(np.int32)
+    A, B = ctx.strings(a, width=4), ctx.strings(b, width=4)
+    got = rfa.pairs(ctx, A, B, metrics)
+    exp = oracle.pairs(metrics, a, b)
+    assert np.array_equal(got, exp)
+    plain = rfa.pairs(ctx, A, B, metrics, schedule=rfa.SCHEDULE_PLAIN)
+    assert np.array_equal(plain, exp)
