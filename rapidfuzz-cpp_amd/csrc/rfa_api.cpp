@@ -551,11 +551,11 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         // lane kernel, the rest -> block kernel (longest-processing-time-first queue)
         int32_t* dmet = (int32_t*)ctx->lane_metric.get(n * 4);
         ctx->staging.copy(slot_idx, dmet, n * 4, s);
-        uint64_t* keys = (uint64_t*)ctx->plan_keys.get(n * 16);
+        uint32_t* keys = (uint32_t*)ctx->plan_keys.get(n * 8);
         uint32_t* vals = (uint32_t*)ctx->plan_vals.get(n * 8);
         const size_t temp_bytes = plan_temp_bytes(n);
         void* temp = ctx->plan_temp.get(temp_bytes);
-        uint32_t* counters = (uint32_t*)ctx->queue.get(64); // range ends and queues, written by the plan
+        uint32_t* counters = (uint32_t*)ctx->queue.get(128); // 18 range ends and the block queue, written by the plan
         HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, keys, keys + n, vals, vals + n, temp, temp_bytes, counters,
                               s));
         const uint32_t* packed = vals + n;
@@ -586,12 +586,12 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         sa.item_packed = packed;
         sa.pair_metric = dmet;
         sa.counters = counters;
-        sa.queue = counters + 11;
         sa.spec = base;
         sa.out = dout;
         const uint32_t nb = seg_symbol_bits(width, std::max(a->max_unit(), b->max_unit()));
-        const uint32_t sblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8, (n + 255) / 256 * 4);
-        HIP_CHECK(launch_seg(width, nb, sa, std::max(sblk, 1u), s2));
+        // no more waves than pairs (each wave holds up to 64 pairs, at least one)
+        const uint32_t max_blocks = (uint32_t)std::min<uint64_t>((n + 3) / 4, 1u << 20);
+        HIP_CHECK(launch_seg(width, nb, sa, (uint32_t)ctx->num_cus, max_blocks, s2));
         if (has_osa) {
             la.item_packed = packed;
             la.pair_metric = dmet;

@@ -95,7 +95,11 @@ struct LaneArgs {
 };
 
 // Segmented kernel (rfa_seg.hip): pairs with patterns of <= 4096 units under the
-// Levenshtein or LCS recurrence, SEG = 64 >> k lanes per pair for class k = 0..6.
+// Levenshtein or LCS recurrence.  Ranges r = 0..13 of the plan: r < 7 holds the
+// pairs with texts of >= SEG_CHAIN_STEPS units, r >= 7 the rest; range r runs
+// SEG = 64 >> (r % 7) lanes per pair.
+constexpr uint32_t SEG_CLASSES = 7;
+constexpr uint32_t SEG_CHAIN_STEPS = 512;
 struct SegArgs {
     const void* achars;
     const uint64_t* aoff;
@@ -103,22 +107,23 @@ struct SegArgs {
     const uint64_t* boff;
     const uint32_t* item_packed; // planned items, pair | swap << 31
     const int32_t* pair_metric;
-    const uint32_t* counters;    // class k: items [counters[3 + k], counters[4 + k])
-    uint32_t* queue;             // group counter, zero at launch
+    const uint32_t* counters;    // range r: items [counters[3 + r], counters[4 + r])
     ScoreSpec spec;
     void* out;                   // out[pair]
 };
 
 // symbol bits the segmented kernel compares for a launch whose largest code unit is max_symbol
 uint32_t seg_symbol_bits(int char_width, uint64_t max_symbol);
-hipError_t launch_seg(int char_width, uint32_t nb, const SegArgs& a, uint32_t nblocks, hipStream_t s);
+// grid: the resident workgroups of the chosen instantiation on num_cus CUs, at most max_blocks
+hipError_t launch_seg(int char_width, uint32_t nb, const SegArgs& a, uint32_t num_cus, uint32_t max_blocks,
+                      hipStream_t s);
 
 size_t plan_temp_bytes(uint64_t n);
 // counters[0]: block queue; [1]: end of the block range; [2], [3]: ends of the OSA
-// 33-64 and 1-32 unit ranges; [4 + k]: end of segmented class k (k = 0..6);
+// 33-64 and 1-32 unit ranges; [4 + r]: end of segmented range r (r = 0..13);
 // [11]: segmented kernel queue
 hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t* metric, uint64_t n,
-                       uint64_t* keys_in, uint64_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp,
+                       uint32_t* keys_in, uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp,
                        size_t temp_bytes, uint32_t* counters, hipStream_t s);
 
 // Top-k scoring against a resident choice index (rfa_serve.hip).  Choices are

@@ -1,6 +1,9 @@
-// rfa_seg.hip -- segmented pairs kernel: independent pairs whose pattern (the
-// shorter string) has at most 4096 units and whose metric runs the Levenshtein
-// or the LCS recurrence (Levenshtein, Indel, LCSseq).
+// rfa_seg.hip -- segmented pairs kernel: independent pairs whose pattern has at
+// most 4096 units and whose metric runs the Levenshtein or the LCS recurrence
+// (Levenshtein, Indel, LCSseq).  The planner picks the pattern: the shorter
+// string, or the longer one when that has 512-4096 units, which turns one long
+// sequential chain (1 lane x 3000 text steps) into a wide short one (47 lanes x
+// 60 steps) at no more lane-steps.
 //
 // Layout: a pair takes a segment of SEG lanes (SEG = the power of two >= its
 // word count, 1..64); lane w of the segment holds pattern word w, so a wave
@@ -20,9 +23,14 @@
 // the next pair, and the HP carry-in forced to 1 at segment starts
 // (Levenshtein_impl.hpp:700-736, LCSseq_impl.hpp:286-314).
 //
-// Work: group g of class k holds items [begin_k + g * PPW, +PPW) of the sorted
-// list; waves take groups from one queue, widest class first (the longest
-// pairs start first).
+// Work: group g of range r holds items [begin_r + g * PPW, +PPW) of the sorted
+// list: long-text pairs first, then widest class first and longest texts first
+// within a class.  The grid
+// is exactly the resident waves, and wave w takes groups w, w + W, w + 2W, ...
+// (round-robin over a longest-first list).  No shared queue: a device-wide
+// atomic per group costs more than a short group's whole scan.
+#include <algorithm>
+
 #include "rfa_block_impl.hpp"
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
@@ -230,31 +238,36 @@ __device__ __forceinline__ void seg_group(const SegArgs& a, uint32_t first, uint
     }
 }
 
-// Classes k = 0..6 hold SEG = 64 >> k; counters[3 + k] .. counters[4 + k] are their item ranges.
+// Ranges r = 0..13: counters[3 + r] .. counters[4 + r], SEG = 64 >> (r % 7); ranges
+// r < 7 hold the long-text pairs, whose waves run at raised issue priority so the
+// longest sequential chains are not slowed by the many short groups around them.
 template <typename CharT, uint32_t NB, bool INTK>
 __global__ __launch_bounds__(256) void k_seg(SegArgs a)
 {
+    constexpr uint32_t NR = 2 * SEG_CLASSES;
     const uint32_t lane = threadIdx.x & 63;
-    uint32_t gend[7], ibeg[7];
+    uint32_t gend[NR], ibeg[NR];
     uint32_t acc = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < 7; ++k) {
-        const uint32_t b = uniform(a.counters[3 + k]), e = uniform(a.counters[4 + k]);
-        const uint32_t ppw = 1u << k; // pairs per wave: 64 / SEG
-        ibeg[k] = b;
+    for (uint32_t r = 0; r < NR; ++r) {
+        const uint32_t b = uniform(a.counters[3 + r]), e = uniform(a.counters[4 + r]);
+        const uint32_t ppw = 1u << (r % SEG_CLASSES); // pairs per wave: 64 / SEG
+        ibeg[r] = b;
         acc += e > b ? (e - b + ppw - 1) / ppw : 0u;
-        gend[k] = acc;
+        gend[r] = acc;
     }
-    for (;;) {
-        uint32_t g = 0;
-        if (lane == 0) g = atomicAdd(a.queue, 1u);
-        g = uniform((uint32_t)__shfl((int)g, 0, 64));
-        if (g >= acc) break;
-        uint32_t k = 0;
-        while (k < 6 && g >= gend[k]) ++k;
-        const uint32_t g0 = k ? gend[k - 1] : 0u;
-        const uint32_t first = ibeg[k] + ((g - g0) << k);
-        const uint32_t end = uniform(a.counters[4 + k]);
+    const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t g = blockIdx.x * (blockDim.x / 64) + uniform(threadIdx.x / 64); g < acc; g += nwaves) {
+        uint32_t r = 0;
+        while (r < NR - 1 && g >= gend[r]) ++r;
+        const uint32_t g0 = r ? gend[r - 1] : 0u;
+        const uint32_t k = r % SEG_CLASSES;
+        const uint32_t first = ibeg[r] + ((g - g0) << k);
+        const uint32_t end = uniform(a.counters[4 + r]);
+        if (r < SEG_CLASSES)
+            __builtin_amdgcn_s_setprio(2);
+        else
+            __builtin_amdgcn_s_setprio(0);
         switch (k) {
         case 0: seg_group<64, CharT, NB, INTK>(a, first, end, lane); break;
         case 1: seg_group<32, CharT, NB, INTK>(a, first, end, lane); break;
@@ -276,36 +289,60 @@ uint32_t seg_symbol_bits(int char_width, uint64_t max_symbol)
     return bits <= 24 ? 24 : 32;
 }
 
-template <bool INTK>
-static void launch_seg_k(int char_width, uint32_t nb, const SegArgs& a, uint32_t nblocks, hipStream_t s)
+// resident workgroups per CU of one instantiation (queried once)
+template <typename CharT, uint32_t NB, bool INTK>
+static uint32_t seg_blocks_per_cu()
 {
-    const dim3 grid(nblocks), block(256);
+    static const uint32_t v = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_seg<CharT, NB, INTK>, 256, 0) != hipSuccess || b < 1) {
+            (void)hipGetLastError();
+            b = 1;
+        }
+        return (uint32_t)b;
+    }();
+    return v;
+}
+
+template <typename CharT, uint32_t NB, bool INTK>
+static void launch_one_seg(const SegArgs& a, uint32_t num_cus, uint32_t max_blocks, hipStream_t s)
+{
+    uint32_t blocks = num_cus * seg_blocks_per_cu<CharT, NB, INTK>();
+    blocks = std::max(1u, std::min(blocks, max_blocks));
+    hipLaunchKernelGGL((k_seg<CharT, NB, INTK>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+template <bool INTK>
+static void launch_seg_k(int char_width, uint32_t nb, const SegArgs& a, uint32_t num_cus, uint32_t max_blocks,
+                         hipStream_t s)
+{
     if (char_width == 1)
-        hipLaunchKernelGGL((k_seg<uint8_t, 8, INTK>), grid, block, 0, s, a);
+        launch_one_seg<uint8_t, 8, INTK>(a, num_cus, max_blocks, s);
     else if (char_width == 2) {
         if (nb <= 8)
-            hipLaunchKernelGGL((k_seg<uint16_t, 8, INTK>), grid, block, 0, s, a);
+            launch_one_seg<uint16_t, 8, INTK>(a, num_cus, max_blocks, s);
         else
-            hipLaunchKernelGGL((k_seg<uint16_t, 16, INTK>), grid, block, 0, s, a);
+            launch_one_seg<uint16_t, 16, INTK>(a, num_cus, max_blocks, s);
     }
     else {
         if (nb <= 8)
-            hipLaunchKernelGGL((k_seg<uint32_t, 8, INTK>), grid, block, 0, s, a);
+            launch_one_seg<uint32_t, 8, INTK>(a, num_cus, max_blocks, s);
         else if (nb <= 16)
-            hipLaunchKernelGGL((k_seg<uint32_t, 16, INTK>), grid, block, 0, s, a);
+            launch_one_seg<uint32_t, 16, INTK>(a, num_cus, max_blocks, s);
         else if (nb <= 24)
-            hipLaunchKernelGGL((k_seg<uint32_t, 24, INTK>), grid, block, 0, s, a);
+            launch_one_seg<uint32_t, 24, INTK>(a, num_cus, max_blocks, s);
         else
-            hipLaunchKernelGGL((k_seg<uint32_t, 32, INTK>), grid, block, 0, s, a);
+            launch_one_seg<uint32_t, 32, INTK>(a, num_cus, max_blocks, s);
     }
 }
 
-hipError_t launch_seg(int char_width, uint32_t nb, const SegArgs& a, uint32_t nblocks, hipStream_t s)
+hipError_t launch_seg(int char_width, uint32_t nb, const SegArgs& a, uint32_t num_cus, uint32_t max_blocks,
+                      hipStream_t s)
 {
     if (a.spec.kind == RFA_DISTANCE || a.spec.kind == RFA_SIMILARITY)
-        launch_seg_k<true>(char_width, nb, a, nblocks, s);
+        launch_seg_k<true>(char_width, nb, a, num_cus, max_blocks, s);
     else
-        launch_seg_k<false>(char_width, nb, a, nblocks, s);
+        launch_seg_k<false>(char_width, nb, a, num_cus, max_blocks, s);
     return hipGetLastError();
 }
 
