@@ -178,6 +178,27 @@ def workload_uneven(args):
     same = bool(np.array_equal(res["bucketed"]["out"], res["plain"]["out"]))
     cpu = None
     check = "schedules agree" if same else "SCHEDULES DISAGREE"
+    # algorithmic INT32 work of the bucketed schedule: the segmented kernel takes the shorter
+    # string as the pattern, or the longer one when it has 512-4096 units
+    s_len, l_len = np.minimum(la, lb).astype(np.float64), np.maximum(la, lb).astype(np.float64)
+    flip = (l_len >= 512) & (l_len <= 4096)
+    words = np.ceil(np.where(flip, l_len, s_len) / 64)
+    steps = np.where(flip, s_len, l_len)
+    ops_per = np.where(metrics == 0, np.where(words == 1, OPS[0], 42.0), 10.0)
+    ops = float(np.sum(words * steps * ops_per))
+    cus = hip.device_attribute(hip.ATTR_CU_COUNT)
+    clk_khz = hip.device_attribute(hip.ATTR_CLOCK_RATE_KHZ)
+    peak_tops = cus * 64 * clk_khz * 1e3 / 1e12
+    achieved_tops = ops / (res["bucketed"]["ms"] / 1e3) / 1e12
+    prof = read_profile_facts(os.path.join(ROOT, "profiles", "pairs_uneven_1M_u32.md"))
+    roofline = {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 3), "unit": "TOP/s",
+                "frac": round(achieved_tops / peak_tops, 4), "traffic": None,
+                "ops_per_step": ops,
+                "ops_model": "word-steps x INT32 ops per word-step: Levenshtein 44 (one word) / 42 (block "
+                             "word), LCS 10; timed over the whole call (plan + segmented kernel)",
+                "kernel": "k_seg<u32, 16 symbol bits, integer epilogue>",
+                "valu_issue_utilization": prof.get("valu_issue_utilization"),
+                "valu_source": "profiles/pairs_uneven_1M_u32.md"}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py
@@ -215,7 +236,7 @@ def workload_uneven(args):
         "plain": {k: v for k, v in res["plain"].items() if k != "out"},
         "bucketed": {k: v for k, v in res["bucketed"].items() if k != "out"},
         "speedup_vs_plain": round(res["plain"]["ms"] / res["bucketed"]["ms"], 2),
-        "cpu_baseline": cpu, "check": check,
+        "roofline": roofline, "cpu_baseline": cpu, "check": check,
     }
     print(json.dumps(line), flush=True)
 

@@ -480,7 +480,7 @@ uint64_t* lane_arena(rfa_context* ctx, uint32_t maxw, uint32_t nlanes, uint64_t 
     // and restart the counter (with a fresh zeroing) before it would wrap.
     const uint64_t per_lane = (nitems + nlanes - 1) / nlanes + 1;
     if (ctx->lane_arena_bytes_zeroed < bytes || (uint64_t)ctx->gen_next + per_lane >= 0xffffff00ull) {
-        HIP_CHECK(hipMemsetAsync(arena, 0, ctx->lane_arena.cap, ctx->stream2));
+        HIP_CHECK(hipMemsetAsync(arena, 0, ctx->lane_arena.cap, ctx->stream));
         ctx->lane_arena_bytes_zeroed = ctx->lane_arena.cap;
         ctx->gen_next = 1;
     }
@@ -529,21 +529,19 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
     la.out = dout;
 
     if (schedule == RFA_SCHEDULE_PLAIN) {
-        HIP_CHECK(hipEventRecord(ctx->fork, s));
-        HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
         // one pair per lane, input order, all words in the lane
         uint64_t maxm = 0;
         for (uint64_t i = 0; i < n; ++i) maxm = std::max(maxm, a->len(i));
         if (maxm > 4096) throw ApiError(RFA_ERR_UNSUPPORTED, "the plain schedule handles patterns up to 4096 units");
         int32_t* dmet = (int32_t*)ctx->lane_metric.get(n * 4);
-        ctx->staging.copy(slot_idx, dmet, n * 4, s2);
+        ctx->staging.copy(slot_idx, dmet, n * 4, s);
         la.item_metric = dmet;
         la.nitems = n;
         const uint32_t nlanes = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 4 * 64,
                                                              ((n + 255) / 256) * 256);
         la.arena_words = lane_arena_words(64);
         la.arena = lane_arena(ctx, 64, nlanes, n, &la.gen_base);
-        HIP_CHECK(launch_lane(width, 64, la, nlanes / 256, s2));
+        HIP_CHECK(launch_lane(width, 64, la, nlanes / 256, s));
     }
     else {
         // bucketed, planned on the device: pattern = the shorter string; Levenshtein / Indel /
@@ -551,20 +549,23 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         // lane kernel, the rest -> block kernel (longest-processing-time-first queue)
         int32_t* dmet = (int32_t*)ctx->lane_metric.get(n * 4);
         ctx->staging.copy(slot_idx, dmet, n * 4, s);
-        uint32_t* keys = (uint32_t*)ctx->plan_keys.get(n * 8);
+        uint32_t* bins = (uint32_t*)ctx->plan_keys.get(n * 4);
         uint32_t* vals = (uint32_t*)ctx->plan_vals.get(n * 8);
         const size_t temp_bytes = plan_temp_bytes(n);
         void* temp = ctx->plan_temp.get(temp_bytes);
         uint32_t* counters = (uint32_t*)ctx->queue.get(128); // 18 range ends and the block queue, written by the plan
-        HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, keys, keys + n, vals, vals + n, temp, temp_bytes, counters,
-                              s));
+        HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, bins, vals, vals + n, temp, temp_bytes, counters, s));
         const uint32_t* packed = vals + n;
         const uint64_t maxm = std::min(a->max_len, b->max_len);
         const uint64_t maxn = std::max(a->max_len, b->max_len);
-        // the segmented and lane kernels (second stream) overlap the block queue (first stream)
-        HIP_CHECK(hipEventRecord(ctx->fork, s));
-        HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
-        if (maxm > 4096 || (has_osa && maxm > 64)) {
+        // with block work, the segmented and lane kernels (second stream) overlap the block
+        // queue (first stream); without, everything stays on the first stream
+        const bool need_block = maxm > 4096 || (has_osa && maxm > 64);
+        hipStream_t sk = s;
+        if (need_block) {
+            HIP_CHECK(hipEventRecord(ctx->fork, s));
+            HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
+            sk = s2;
             BlockArgs ba{};
             ba.achars = a->d_chars;
             ba.aoff = a->d_off;
@@ -591,7 +592,7 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         const uint32_t nb = seg_symbol_bits(width, std::max(a->max_unit(), b->max_unit()));
         // no more waves than pairs (each wave holds up to 64 pairs, at least one)
         const uint32_t max_blocks = (uint32_t)std::min<uint64_t>((n + 3) / 4, 1u << 20);
-        HIP_CHECK(launch_seg(width, nb, sa, (uint32_t)ctx->num_cus, max_blocks, s2));
+        HIP_CHECK(launch_seg(width, nb, sa, (uint32_t)ctx->num_cus, max_blocks, sk));
         if (has_osa) {
             la.item_packed = packed;
             la.pair_metric = dmet;
@@ -600,14 +601,16 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
             const uint32_t nblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 6, (n + 63) / 64);
             la.begin_dev = counters + 1;
             la.end_dev = counters + 2;
-            HIP_CHECK(launch_lane_short(width, 64, la, nblk, s2));
+            HIP_CHECK(launch_lane_short(width, 64, la, nblk, sk));
             la.begin_dev = counters + 2;
             la.end_dev = counters + 3;
-            HIP_CHECK(launch_lane_short(width, 32, la, nblk, s2));
+            HIP_CHECK(launch_lane_short(width, 32, la, nblk, sk));
+        }
+        if (need_block) {
+            HIP_CHECK(hipEventRecord(ctx->join, s2));
+            HIP_CHECK(hipStreamWaitEvent(s, ctx->join, 0));
         }
     }
-    HIP_CHECK(hipEventRecord(ctx->join, s2));
-    HIP_CHECK(hipStreamWaitEvent(s, ctx->join, 0));
     if (!out_on_device) {
         HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));

@@ -119,12 +119,12 @@ hipError_t launch_seg(int char_width, uint32_t nb, const SegArgs& a, uint32_t nu
                       hipStream_t s);
 
 size_t plan_temp_bytes(uint64_t n);
+// Counting sort of the pairs into kernel ranges (items vals_out[k] = pair | pattern-is-B << 31).
 // counters[0]: block queue; [1]: end of the block range; [2], [3]: ends of the OSA
-// 33-64 and 1-32 unit ranges; [4 + r]: end of segmented range r (r = 0..13);
-// [11]: segmented kernel queue
-hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t* metric, uint64_t n,
-                       uint32_t* keys_in, uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp,
-                       size_t temp_bytes, uint32_t* counters, hipStream_t s);
+// 33-64 and 1-32 unit ranges; [4 + r]: end of segmented range r (r = 0..13).
+hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t* metric, uint64_t n, uint32_t* bins,
+                       uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes, uint32_t* counters,
+                       hipStream_t s);
 
 // Top-k scoring against a resident choice index (rfa_serve.hip).  Choices are
 // sorted by length into groups of 64; group g holds gdwords[g] dwords per lane

@@ -1,23 +1,30 @@
-// rfa_plan.hip -- device-side planning of the bucketed pairs schedule.
+// rfa_plan.hip -- device-side planning of the bucketed pairs schedule: a
+// counting sort of the pairs into bins whose ascending order is the order the
+// kernels take them in.
 //
-//   classify : per pair, the kernel, the pattern side and a 32-bit sort key
-//              (descending order):
-//                segmented : Levenshtein / Indel / LCSseq pairs whose shorter
-//                            string has <= 4096 units; the pattern is the
-//                            shorter string, or the longer one when it has
-//                            512-4096 units; bit 28 for texts of >= 512 units
-//                            (long sequential chains, scheduled first), class
-//                            c = log2(lanes per pair) in bits 25-27,
-//                            Levenshtein flag in bit 24, text length below
-//                OSA short : 2^30 (+ 2^29 for 33-64 units) + text length
-//                block     : the rest; key 2^31 + words * text length (cost,
-//                            saturated)
-//   sort     : one descending radix sort of (key, pair|swap<<31) -- block pairs
-//              come first, most expensive first (longest-processing-time order
-//              for the block kernel's queue); the segmented classes follow from
-//              widest to narrowest, grouped by metric and text length so the
-//              lanes of a wave finish together.
-//   split    : one thread finds the range boundaries and resets the block queue.
+//   bins (per pair, from its two lengths and its metric):
+//     block     : patterns (the shorter string) of > 4096 units, and OSA
+//                 patterns of > 64 units; 256 bins of words x text length on a
+//                 log scale (4 bins per power of two), most expensive first
+//                 (longest-processing-time order for the block kernel's queue)
+//     OSA short : OSA patterns of 33-64 units, then of 1-32 units, each by text
+//                 length, longest first
+//     segmented : Levenshtein / Indel / LCSseq pairs whose shorter string has
+//                 <= 4096 units.  The pattern is the shorter string, or the
+//                 longer one when that has 512-4096 units (a 60-unit string
+//                 against a 3000-unit one then runs 64 lanes x 60 steps instead
+//                 of 1 lane x 3000 steps).  14 ranges: pairs with texts of
+//                 >= 512 units first (long sequential chains), then the rest;
+//                 within each, lanes-per-pair classes 64, 32, ..., 1; within a
+//                 class Levenshtein before the LCS metrics, each by text length
+//                 (exact below 256 units, 32 bins per power of two above),
+//                 longest first, so the lanes of a wave finish together.
+//   passes   : k_bin_count -- bin of every pair, per-workgroup histograms in
+//                             LDS, written bin-major;
+//              exclusive scan of the histograms (one hipcub scan) -- the
+//                             output position of each (bin, workgroup);
+//              k_bin_scatter -- items placed with LDS cursors;
+//              k_bounds      -- range boundaries for the kernels.
 // The host never reads the plan back, so a call costs O(1) host work.
 #include <hipcub/hipcub.hpp>
 
@@ -25,103 +32,147 @@
 
 namespace rfa {
 
-constexpr uint32_t BLOCK_FLAG = 1u << 31;
-constexpr uint32_t OSA_FLAG = 1u << 30;
-constexpr uint32_t S64_FLAG = 1u << 29;
-constexpr uint32_t CHAIN_FLAG = 1u << 28; // segmented pairs with long texts
-constexpr uint32_t LEV_FLAG = 1u << 24;
 constexpr uint32_t SEG_MAX_UNITS = 4096;
+constexpr uint32_t TBINS = 1024; // text-length bins
+constexpr uint32_t BLOCK_BINS = 256;
+constexpr uint32_t OSA_BASE = BLOCK_BINS;
+constexpr uint32_t SEG_BASE = OSA_BASE + 2 * TBINS;
+constexpr uint32_t NBINS = SEG_BASE + 2 * SEG_CLASSES * 2 * TBINS; // 30,976
+constexpr uint32_t PLAN_THREADS = 1024;
+constexpr uint32_t PLAN_MAX_GROUPS = 128;
 
-__device__ __forceinline__ uint32_t pow2_at_least(uint64_t x)
+// text length -> [0, TBINS): exact below 256, then 32 bins per power of two
+__device__ __forceinline__ uint32_t text_bin(uint64_t t)
+{
+    const uint32_t v = (uint32_t)(t < 0xffffffffull ? t : 0xffffffffull);
+    if (v < 256) return v;
+    const uint32_t lg = 31 - __builtin_clz(v);
+    return 256 + (lg - 8) * 32 + ((v >> (lg - 5)) & 31);
+}
+
+// cost -> [0, 256): 4 bins per power of two
+__device__ __forceinline__ uint32_t cost_code(uint64_t c)
+{
+    if (c < 4) return (uint32_t)c;
+    const uint32_t lg = 63 - __builtin_clzll(c);
+    return lg * 4 + (uint32_t)((c >> (lg - 2)) & 3);
+}
+
+__device__ __forceinline__ uint32_t words_class(uint64_t m)
 {
     uint32_t c = 0;
-    while ((1ull << c) < x) ++c;
-    return c;
+    while ((1ull << c) < (m + 63) / 64) ++c;
+    return c; // log2 of lanes per pair, 0..6 for m <= 4096
 }
 
-__device__ __forceinline__ uint32_t sat(uint64_t v, uint32_t bits) { return (uint32_t)(v < (1ull << bits) ? v : (1ull << bits) - 1); }
-
-__global__ void k_classify(const uint64_t* __restrict__ aoff, const uint64_t* __restrict__ boff,
-                           const int32_t* __restrict__ metric, uint64_t n, uint32_t* __restrict__ keys,
-                           uint32_t* __restrict__ vals)
+// bin and packed item (pair | pattern-is-B << 31) of pair i
+__device__ __forceinline__ uint32_t pair_bin(const uint64_t* __restrict__ aoff, const uint64_t* __restrict__ boff,
+                                             const int32_t* __restrict__ metric, uint64_t i, uint32_t* packed)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t la = aoff[i + 1] - aoff[i], lb = boff[i + 1] - boff[i];
-        bool pat_b = lb < la; // the pattern is the shorter string ...
-        const uint64_t s = pat_b ? lb : la, l = pat_b ? la : lb;
-        const int32_t met = metric[i];
-        const uint32_t lev = met == RFA_LEVENSHTEIN ? LEV_FLAG : 0u;
-        uint32_t key;
-        if (met != RFA_OSA && l <= SEG_MAX_UNITS && l >= SEG_CHAIN_STEPS) {
-            // ... except in the segmented kernel when the longer string has >= 512 units:
-            // then the longer string is the pattern, spread over up to 64 lanes, and the
-            // sequential chain is the shorter string (a 60-unit string against a
-            // 3000-unit one: 64 lanes x 60 steps instead of 1 lane x 3000 steps)
+    const uint64_t la = aoff[i + 1] - aoff[i], lb = boff[i + 1] - boff[i];
+    bool pat_b = lb < la; // the pattern is the shorter string ...
+    const uint64_t s = pat_b ? lb : la, l = pat_b ? la : lb;
+    const int32_t met = metric[i];
+    uint32_t bin;
+    if (met != RFA_OSA && s <= SEG_MAX_UNITS) {
+        uint64_t m = s, t = l;
+        if (l <= SEG_MAX_UNITS && l >= SEG_CHAIN_STEPS) {
+            // ... except in the segmented kernel when the longer string has 512-4096 units
             pat_b = !pat_b;
-            key = (s >= SEG_CHAIN_STEPS ? CHAIN_FLAG : 0u) | (pow2_at_least((l + 63) / 64) << 25) | lev | (uint32_t)s;
+            m = l;
+            t = s;
         }
-        else if (met != RFA_OSA && s <= SEG_MAX_UNITS) {
-            key = (l >= SEG_CHAIN_STEPS ? CHAIN_FLAG : 0u) | (pow2_at_least((s + 63) / 64) << 25) | lev | sat(l, 24);
-        }
-        else if (met == RFA_OSA && s <= 64) {
-            key = OSA_FLAG | (s > 32 ? S64_FLAG : 0u) | sat(l, 29);
-        }
-        else {
-            key = BLOCK_FLAG | sat(((s + 63) / 64) * l, 31);
-        }
-        keys[i] = key;
-        vals[i] = (uint32_t)i | (pat_b ? 0x80000000u : 0u);
+        const uint32_t r = (t >= SEG_CHAIN_STEPS ? 0u : SEG_CLASSES) + (SEG_CLASSES - 1 - words_class(m));
+        bin = SEG_BASE + (2 * r + (met == RFA_LEVENSHTEIN ? 0u : 1u)) * TBINS + (TBINS - 1 - text_bin(t));
     }
+    else if (met == RFA_OSA && s <= 64) {
+        bin = OSA_BASE + (s > 32 ? 0u : TBINS) + (TBINS - 1 - text_bin(l));
+    }
+    else {
+        const uint32_t c = cost_code(((s + 63) / 64) * l);
+        bin = BLOCK_BINS - 1 - (c < BLOCK_BINS ? c : BLOCK_BINS - 1);
+    }
+    *packed = (uint32_t)i | (pat_b ? 0x80000000u : 0u);
+    return bin;
 }
 
-// First index in [lo, hi) whose key is below thr (keys sorted descending).
-__device__ uint64_t first_below(const uint32_t* keys, uint64_t lo, uint64_t hi, uint32_t thr)
+// Workgroup g bins pairs [g * chunk, min(n, (g + 1) * chunk)); hist[bin * groups + g].
+__global__ __launch_bounds__(PLAN_THREADS) void k_bin_count(const uint64_t* __restrict__ aoff,
+                                                            const uint64_t* __restrict__ boff,
+                                                            const int32_t* __restrict__ metric, uint64_t n,
+                                                            uint64_t chunk, uint32_t* __restrict__ bins,
+                                                            uint32_t* __restrict__ vals, uint32_t* __restrict__ hist)
 {
-    while (lo < hi) {
-        uint64_t mid = (lo + hi) / 2;
-        if (keys[mid] >= thr)
-            lo = mid + 1;
-        else
-            hi = mid;
+    __shared__ uint32_t h[NBINS];
+    for (uint32_t b = threadIdx.x; b < NBINS; b += PLAN_THREADS) h[b] = 0;
+    __syncthreads();
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = std::min<uint64_t>(n, lo + chunk);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += PLAN_THREADS) {
+        uint32_t pk;
+        const uint32_t b = pair_bin(aoff, boff, metric, i, &pk);
+        bins[i] = b;
+        vals[i] = pk;
+        atomicAdd(&h[b], 1u);
     }
-    return lo;
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < NBINS; b += PLAN_THREADS) hist[(uint64_t)b * gridDim.x + blockIdx.x] = h[b];
 }
 
-__global__ void k_split(const uint32_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ counters)
+__global__ __launch_bounds__(PLAN_THREADS) void k_bin_scatter(uint64_t n, uint64_t chunk,
+                                                              const uint32_t* __restrict__ bins,
+                                                              const uint32_t* __restrict__ vals,
+                                                              const uint32_t* __restrict__ pos,
+                                                              uint32_t* __restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint64_t e = first_below(keys, 0, n, BLOCK_FLAG);
-    counters[1] = (uint32_t)e;
-    e = first_below(keys, e, n, OSA_FLAG | S64_FLAG);
-    counters[2] = (uint32_t)e;
-    e = first_below(keys, e, n, OSA_FLAG);
-    counters[3] = (uint32_t)e;
-    for (uint32_t r = 0; r < 2 * SEG_CLASSES; ++r) {
-        const uint32_t c = SEG_CLASSES - 1 - r % SEG_CLASSES;
-        e = first_below(keys, e, n, (r < SEG_CLASSES ? CHAIN_FLAG : 0u) | (c << 25));
-        counters[4 + r] = (uint32_t)e;
-    }
-    counters[0] = 0;
+    __shared__ uint32_t cur[NBINS];
+    for (uint32_t b = threadIdx.x; b < NBINS; b += PLAN_THREADS) cur[b] = pos[(uint64_t)b * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = std::min<uint64_t>(n, lo + chunk);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += PLAN_THREADS) out[atomicAdd(&cur[bins[i]], 1u)] = vals[i];
+}
+
+// counters[1..3]: ends of the block and OSA ranges; counters[4 + r]: end of segmented range r
+__global__ void k_bounds(const uint32_t* __restrict__ pos, uint32_t groups, uint64_t n, uint32_t* __restrict__ counters)
+{
+    const uint32_t t = threadIdx.x;
+    auto start_of = [&](uint32_t bin) { return bin < NBINS ? pos[(uint64_t)bin * groups] : (uint32_t)n; };
+    if (t == 0) counters[0] = 0;
+    if (t == 1) counters[1] = start_of(OSA_BASE);
+    if (t == 2) counters[2] = start_of(OSA_BASE + TBINS);
+    if (t == 3) counters[3] = start_of(SEG_BASE);
+    if (t < 2 * SEG_CLASSES) counters[4 + t] = start_of(SEG_BASE + (t + 1) * 2 * TBINS);
+}
+
+static uint32_t plan_groups(uint64_t n)
+{
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(PLAN_MAX_GROUPS, (n + 8191) / 8192));
 }
 
 size_t plan_temp_bytes(uint64_t n)
 {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32);
-    return bytes;
+    const size_t cells = (size_t)NBINS * plan_groups(n);
+    size_t scan = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)cells);
+    return 2 * cells * sizeof(uint32_t) + scan + 256;
 }
 
-hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t* metric, uint64_t n,
-                       uint32_t* keys_in, uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp,
-                       size_t temp_bytes, uint32_t* counters, hipStream_t s)
+hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t* metric, uint64_t n, uint32_t* bins,
+                       uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes, uint32_t* counters,
+                       hipStream_t s)
 {
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, s, aoff, boff, metric, n, keys_in, vals_in);
-    hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out,
-                                                                (int)n, 0, 32, s);
+    const uint32_t groups = plan_groups(n);
+    const uint64_t chunk = (n + groups - 1) / groups;
+    const size_t cells = (size_t)NBINS * groups;
+    uint32_t* hist = (uint32_t*)temp;
+    uint32_t* pos = hist + cells;
+    void* scan_temp = (void*)(((uintptr_t)(pos + cells) + 255) & ~(uintptr_t)255);
+    size_t scan_bytes = temp_bytes - ((uint8_t*)scan_temp - (uint8_t*)temp);
+    hipLaunchKernelGGL(k_bin_count, dim3(groups), dim3(PLAN_THREADS), 0, s, aoff, boff, metric, n, chunk, bins,
+                       vals_in, hist);
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(scan_temp, scan_bytes, hist, pos, (int)cells, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_split, dim3(1), dim3(64), 0, s, keys_out, n, counters);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(groups), dim3(PLAN_THREADS), 0, s, n, chunk, bins, vals_in, pos, vals_out);
+    hipLaunchKernelGGL(k_bounds, dim3(1), dim3(64), 0, s, pos, groups, n, counters);
     return hipGetLastError();
 }
 
