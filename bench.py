@@ -129,7 +129,7 @@ def zipf_units(rng, n, s=1.1, alphabet=65536):
     return perm[np.minimum(idx, alphabet - 1)]
 
 
-def workload_uneven(args):
+def workload_uneven(args, emit=True):
     """1,000,000 independent pairs, log-normal lengths (median 40, sigma 1) clipped to 4-4096,
     uint32 Zipf(1.1) code points over 65,536 symbols, Levenshtein / Indel 50/50, no cutoff."""
     import rapidfuzz_amd as rfa
@@ -238,7 +238,9 @@ def workload_uneven(args):
         "speedup_vs_plain": round(res["plain"]["ms"] / res["bucketed"]["ms"], 2),
         "roofline": roofline, "cpu_baseline": cpu, "check": check,
     }
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 SERVE_CHOICES = 2_000_000
@@ -640,6 +642,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-scorers", action="store_true", help="skip the per-scorer table")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the latency, DNA and uneven-batch summaries of the default run")
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -796,6 +800,21 @@ def main():
         res["serving"] = {k: srv[k] for k in ("latency_ms", "served_qps", "choices_scored_per_s", "mean_batch",
                                               "saturation_qps", "dense_fallbacks", "cpu_baseline", "check")}
         res["serving"]["config"] = srv["config"]["workload"]
+    if not args.no_configs:
+        # the other configurations, each summarized from its own workload line
+        keep = ("metric", "value", "unit", "ms_per_step", "higher_is_better", "roofline", "cpu_baseline", "check")
+        res["configs"] = {}
+        lat = workload_latency(args, emit=False)
+        res["configs"]["latency_1x10000"] = {**{k: lat.get(k) for k in keep},
+                                             "p99_us": lat["p99_us"], "device_us_per_call": lat["device_us_per_call"],
+                                             "config": lat["config"]["workload"]}
+        dna = workload_dna(args, emit=False)
+        res["configs"]["dna_256x16384"] = {**{k: dna.get(k) for k in keep}, "cutoff_5pct": dna["cutoff_5pct"],
+                                           "cutoff_20pct": dna["cutoff_20pct"], "config": dna["config"]["workload"]}
+        une = workload_uneven(args, emit=False)
+        res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
+                                             "speedup_vs_plain": une["speedup_vs_plain"],
+                                             "config": une["config"]["workload"]}
     print(json.dumps(res), flush=True)
 
 

@@ -333,6 +333,10 @@ void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_
         a.queue = (uint32_t*)ctx->queue.get(64);
         HIP_CHECK(hipMemsetAsync(a.queue, 0, 4, s));
     }
+    // device-planned items are few and sorted most expensive first: one per grab;
+    // cross products (every query x every choice, most settled by the length filter)
+    // are taken in batches of consecutive items, about 8 grabs per workgroup
+    a.batch = a.nitems_dev ? 1u : (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, a.nitems / (8ull * nblocks)));
     HIP_CHECK(launch_block(algo, width, a, nblocks, slots, s));
 }
 

@@ -798,11 +798,18 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a)
     uint64_t keyL = ~0ull, keyG = ~0ull, need_key = ~0ull, need_u8 = 0;
     Pattern<CharT, true> PL{};
     Pattern<CharT, false> PG{};
+    // items are taken from the queue a.batch at a time: a device-wide atomic per
+    // item would cost more than the many items the length filter settles at once
+    uint32_t next = 0, stop = 0;
     for (;;) {
-        if (lane == 0) s_item = atomicAdd(a.queue, 1u);
-        __syncthreads();
-        const uint32_t item = uniform(s_item);
-        __syncthreads();
+        if (next >= stop) {
+            if (lane == 0) s_item = atomicAdd(a.queue, a.batch);
+            __syncthreads();
+            next = uniform(s_item);
+            __syncthreads();
+            stop = next + a.batch;
+        }
+        const uint32_t item = next++;
         const uint32_t nitems = a.nitems_dev ? uniform(*a.nitems_dev) : a.nitems;
         if (item >= nitems) break;
         uint32_t ia, ib;

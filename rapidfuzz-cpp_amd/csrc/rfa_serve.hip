@@ -136,7 +136,9 @@ struct QState {
     }
 };
 
-template <int ALGO, int WQ, int QPB>
+// DENSE: the cdist-of-a-few-queries mode (every result written); a separate
+// instantiation, so the top-k kernels keep their register footprint.
+template <int ALGO, int WQ, int QPB, bool DENSE>
 __global__ __launch_bounds__(256) void k_serve(ServeArgs a)
 {
     extern __shared__ uint64_t pm[]; // [QPB][256][WQ]
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a)
         for (int q = 0; q < QPB; ++q) {
             if (!(act & (1u << q))) continue;
             RawResult r{st[q].raw(qm[q], n), 0, 0};
-            if (a.dense_out) {
+            if constexpr (DENSE) {
                 const uint32_t qr = a.qrow[q0 + q];
                 ScoreSpec sp = a.spec;
                 if (a.qcut) sp.cutoff_u = a.qcut[qr];
@@ -418,26 +420,32 @@ __global__ __launch_bounds__(256) void k_select(SelectArgs a)
 
 // ---- launchers ------------------------------------------------------------------
 
-template <int ALGO, int WQ, int QPB>
+template <int ALGO, int WQ, int QPB, bool DENSE>
 static hipError_t launch_serve_t(const ServeArgs& a, uint32_t gx, uint32_t nsub, hipStream_t s)
 {
     const size_t lds = (size_t)QPB * 256 * WQ * 8;
-    hipLaunchKernelGGL((k_serve<ALGO, WQ, QPB>), dim3(gx, nsub), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((k_serve<ALGO, WQ, QPB, DENSE>), dim3(gx, nsub), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
 uint32_t serve_queries_per_block(uint32_t wq) { return wq <= 2 ? 8 : 4; }
 
-hipError_t launch_serve(int algo, uint32_t wq, const ServeArgs& a, uint32_t gx, uint32_t nsub, hipStream_t s)
+template <bool DENSE>
+static hipError_t launch_serve_d(int algo, uint32_t wq, const ServeArgs& a, uint32_t gx, uint32_t nsub, hipStream_t s)
 {
     if (algo == ALGO_LCS) {
-        if (wq <= 1) return launch_serve_t<ALGO_LCS, 1, 8>(a, gx, nsub, s);
-        if (wq <= 2) return launch_serve_t<ALGO_LCS, 2, 8>(a, gx, nsub, s);
-        return launch_serve_t<ALGO_LCS, 4, 4>(a, gx, nsub, s);
+        if (wq <= 1) return launch_serve_t<ALGO_LCS, 1, 8, DENSE>(a, gx, nsub, s);
+        if (wq <= 2) return launch_serve_t<ALGO_LCS, 2, 8, DENSE>(a, gx, nsub, s);
+        return launch_serve_t<ALGO_LCS, 4, 4, DENSE>(a, gx, nsub, s);
     }
-    if (wq <= 1) return launch_serve_t<ALGO_LEV, 1, 8>(a, gx, nsub, s);
-    if (wq <= 2) return launch_serve_t<ALGO_LEV, 2, 8>(a, gx, nsub, s);
-    return launch_serve_t<ALGO_LEV, 4, 4>(a, gx, nsub, s);
+    if (wq <= 1) return launch_serve_t<ALGO_LEV, 1, 8, DENSE>(a, gx, nsub, s);
+    if (wq <= 2) return launch_serve_t<ALGO_LEV, 2, 8, DENSE>(a, gx, nsub, s);
+    return launch_serve_t<ALGO_LEV, 4, 4, DENSE>(a, gx, nsub, s);
+}
+
+hipError_t launch_serve(int algo, uint32_t wq, const ServeArgs& a, uint32_t gx, uint32_t nsub, hipStream_t s)
+{
+    return a.dense_out ? launch_serve_d<true>(algo, wq, a, gx, nsub, s) : launch_serve_d<false>(algo, wq, a, gx, nsub, s);
 }
 
 hipError_t launch_select(const SelectArgs& a, uint32_t nq, hipStream_t s)
