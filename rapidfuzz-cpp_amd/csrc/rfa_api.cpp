@@ -311,6 +311,8 @@ void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_
     const uint64_t need_min = block_arena_bytes(width, (uint32_t)min_pattern, u8_rows ? 2 : 0);
     a.lds_bytes = need_min > LDS_BUDGET ? 0u : (uint32_t)std::min<uint64_t>(need_max, LDS_BUDGET);
     a.reuse_pattern = reuse ? 1u : 0u;
+    // (banded launches also hold an 8 KiB q-gram set; their register footprint, not the LDS,
+    // bounds them to 3 waves per SIMD, and surplus persistent workgroups only start late)
     const uint32_t waves_per_cu =
         (uint32_t)std::min<uint64_t>(16, std::max<uint64_t>(1, ctx->lds_per_cu / (a.lds_bytes + 512)));
     uint32_t nblocks = (uint32_t)std::min<uint64_t>(a.nitems, (uint64_t)ctx->num_cus * waves_per_cu);
@@ -445,6 +447,15 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ba.spec = spec;
         ba.out = dout;
         ba.qcut = d_qcut;
+        // the q-gram filter can reject pairs only where 7k + 7 < m
+        if (kind == RFA_DISTANCE)
+            for (uint32_t qi : sc->block_queries) {
+                const uint64_t kq = qcut_host ? qcut_host[qi] : cu;
+                if (kq < (1ull << 40) && 7 * kq + 7 < sc->q->len(qi)) {
+                    ba.qgram = 1;
+                    break;
+                }
+            }
         run_block(ctx, algo, width, ba, sc->block_max_len, ch->max_len, true, nullptr, sc->block_min_len,
                   sc->block_u8_rows);
     }
@@ -581,6 +592,7 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
             ba.nitems = (uint32_t)n; // upper bound, for the grid size
             ba.spec = base;
             ba.out = dout;
+            ba.qgram = kind == RFA_DISTANCE && cu < (1ull << 40) && 7 * cu + 7 < maxn ? 1u : 0u;
             run_block(ctx, ALGO_MIXED, width, ba, maxm, maxn, false, counters);
         }
         SegArgs sa{};

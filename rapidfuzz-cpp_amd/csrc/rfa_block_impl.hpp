@@ -740,11 +740,62 @@ __device__ __forceinline__ uint64_t u8_arena_need(const uint8_t* __restrict__ p,
     return 1024 + (uint64_t)(cnt + 1) * ((m + 63) / 64) * 8;
 }
 
+// Q-gram count filter for the banded recurrence (Ukkonen, "Approximate string
+// matching with q-grams and maximal matches", 1992): if ed(p, t) <= k, every
+// edit touches at most Q of t's Q-grams, so at least (n - Q + 1) - k*Q of them
+// occur verbatim in p.  The pattern's Q-grams are hashed into a 2^16-bit set;
+// a hash collision can only add hits, so a pair with fewer hits than the bound
+// has distance > k for certain, and its answer is k + 1 without the recurrence.
+constexpr uint32_t QGRAM_Q = 8;
+constexpr uint32_t QGRAM_WORDS = 2048; // 65,536 bits
+
+template <typename CharT>
+__device__ __forceinline__ uint32_t qgram_hash(const CharT* s)
+{
+    uint32_t u[QGRAM_Q];
+    load_units<CharT, QGRAM_Q>(s, u);
+    uint64_t x = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < QGRAM_Q; ++i) x = sizeof(CharT) == 1 ? (x | ((uint64_t)u[i] << (8 * i))) : (x * 0x100000001B3ull) ^ u[i];
+    return (uint32_t)((x * 0x9E3779B97F4A7C15ull) >> 48);
+}
+
+// True when the filter proves ed(p, t) > k.  The whole wave calls it; `bits`
+// caches the set of the pattern named by `key` (bkey).
+template <typename CharT>
+__device__ bool qgram_rules_out(const CharT* __restrict__ p, uint32_t m, const CharT* __restrict__ t, uint32_t n,
+                                uint64_t k, uint32_t* bits, uint64_t key, uint64_t& bkey)
+{
+    if (m < QGRAM_Q || n < QGRAM_Q) return false;
+    const int64_t need = (int64_t)(n - QGRAM_Q + 1) - (int64_t)k * QGRAM_Q;
+    if (need <= 0) return false;
+    const uint32_t lane = threadIdx.x & 63;
+    if (key != bkey) {
+        for (uint32_t i = lane; i < QGRAM_WORDS; i += 64) bits[i] = 0;
+        __syncthreads();
+        for (uint32_t i = lane; i + QGRAM_Q <= m; i += 64) {
+            const uint32_t h = qgram_hash(p + i);
+            atomicOr(&bits[h >> 5], 1u << (h & 31));
+        }
+        __syncthreads();
+        bkey = key;
+    }
+    uint32_t hits = 0;
+    for (uint32_t i = lane; i + QGRAM_Q <= n; i += 64) {
+        const uint32_t h = qgram_hash(t + i);
+        hits += (bits[h >> 5] >> (h & 31)) & 1u;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off, 64);
+    return (int64_t)uniform(hits) < need;
+}
+
 // One item with the arena in LDS (LDS = true) or in the workgroup's global slice.
-template <int ALGO, int MAXS, typename CharT, bool LDS, bool BAND>
+template <int ALGO, int MAXS, typename CharT, bool LDS, bool BAND, bool QG>
 __device__ __forceinline__ RawResult block_item(int algo, const CharT* pp, uint32_t m, const CharT* tt, uint32_t n,
                                                 uint8_t* arena, uint64_t* tflag, uint64_t key, bool reuse,
-                                                Pattern<CharT, LDS>& P, uint64_t& prev_key, uint64_t band)
+                                                Pattern<CharT, LDS>& P, uint64_t& prev_key, uint64_t band,
+                                                uint32_t* qbits, uint64_t& qkey)
 {
     RawResult res{0, 0, 0};
     if (key != prev_key || !reuse) {
@@ -756,6 +807,9 @@ __device__ __forceinline__ RawResult block_item(int algo, const CharT* pp, uint3
         if constexpr (BAND) {
             if (band < (uint64_t)max(m, n)) {
                 if ((uint64_t)(m > n ? m - n : n - m) > band) return band + 1;
+                if constexpr (QG) {
+                    if (qgram_rules_out(pp, m, tt, n, band, qbits, key, qkey)) return band + 1;
+                }
                 return run_lev_band<MAXS>(P, m, tt, n, (uint32_t)band);
             }
         }
@@ -782,8 +836,10 @@ __device__ __forceinline__ RawResult block_item(int algo, const CharT* pp, uint3
 
 // BAND: Levenshtein items with a cutoff below max(m, n) run the banded
 // recurrence (a separate instantiation, so unbanded launches keep their
-// register footprint).
-template <int ALGO, int MAXS, typename CharT, bool BAND>
+// register footprint).  QG: banded launches whose cutoffs are small enough for
+// the q-gram filter to reject pairs (7k + 7 < m for some pattern) also hold its
+// 8 KiB set in LDS.
+template <int ALGO, int MAXS, typename CharT, bool BAND, bool QG>
 __global__ __launch_bounds__(64) void k_block(BlockArgs a)
 {
     extern __shared__ uint64_t smem_u64[];
@@ -795,7 +851,8 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a)
     const uint32_t lane = threadIdx.x & 63;
     __shared__ uint32_t s_item;
     __shared__ uint32_t present[8];
-    uint64_t keyL = ~0ull, keyG = ~0ull, need_key = ~0ull, need_u8 = 0;
+    __shared__ uint32_t qbits[QG ? QGRAM_WORDS : 1]; // q-gram set of the current pattern
+    uint64_t keyL = ~0ull, keyG = ~0ull, need_key = ~0ull, need_u8 = 0, qkey = ~0ull;
     Pattern<CharT, true> PL{};
     Pattern<CharT, false> PG{};
     // items are taken from the queue a.batch at a time: a device-wide atomic per
@@ -868,11 +925,11 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a)
                     need = pattern_arena_bytes<CharT>(m);
                 }
                 if (need <= a.lds_bytes)
-                    res = block_item<ALGO, MAXS, CharT, true, BAND>(algo, pp, m, tt, n, lds_arena, tflag, key,
-                                                              a.reuse_pattern, PL, keyL, spec.band_k);
+                    res = block_item<ALGO, MAXS, CharT, true, BAND, QG>(algo, pp, m, tt, n, lds_arena, tflag, key,
+                                                              a.reuse_pattern, PL, keyL, spec.band_k, qbits, qkey);
                 else
-                    res = block_item<ALGO, MAXS, CharT, false, BAND>(algo, pp, m, tt, n, glb_arena, tflag, key,
-                                                               a.reuse_pattern, PG, keyG, spec.band_k);
+                    res = block_item<ALGO, MAXS, CharT, false, BAND, QG>(algo, pp, m, tt, n, glb_arena, tflag, key,
+                                                               a.reuse_pattern, PG, keyG, spec.band_k, qbits, qkey);
             }
         }
         if (lane == 0) finalize(spec, la, lb, res, a.out, oidx);
@@ -885,11 +942,15 @@ static void launch3(const BlockArgs& a, uint32_t nblocks, hipStream_t s)
     // banded Levenshtein: Levenshtein / mixed launches with a distance cutoff, patterns <= 32768 units
     if constexpr ((ALGO == ALGO_LEV || ALGO == ALGO_MIXED) && MAXS <= 8) {
         if (a.spec.kind == RFA_DISTANCE && a.spec.cutoff_u < (1ull << 31)) {
-            hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT, true>), dim3(nblocks), dim3(64), a.lds_bytes, s, a);
+            if (a.qgram)
+                hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT, true, true>), dim3(nblocks), dim3(64), a.lds_bytes, s, a);
+            else
+                hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT, true, false>), dim3(nblocks), dim3(64), a.lds_bytes, s,
+                                   a);
             return;
         }
     }
-    hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT, false>), dim3(nblocks), dim3(64), a.lds_bytes, s, a);
+    hipLaunchKernelGGL((k_block<ALGO, MAXS, CharT, false, false>), dim3(nblocks), dim3(64), a.lds_bytes, s, a);
 }
 
 template <int ALGO, typename CharT>

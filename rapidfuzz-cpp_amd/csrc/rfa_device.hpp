@@ -53,6 +53,18 @@ struct HashMap {
     }
 };
 
+// K consecutive code units from p (any alignment: the loads are dword loads of
+// the covering bytes).  Callers that read past a string's end rely on the
+// 64-byte tail padding of every string set.
+template <typename CharT, uint32_t K>
+__device__ __forceinline__ void load_units(const CharT* p, uint32_t* out)
+{
+    CharT tmp[K];
+    __builtin_memcpy(tmp, p, sizeof(tmp));
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) out[k] = (uint32_t)tmp[k];
+}
+
 // Exclusive prefix sum of one value per thread over a block of NW waves.
 // `tmp` holds at least NW uint32 of LDS.  Returns the exclusive sum; *total gets the block sum.
 template <uint32_t NW>

@@ -49,6 +49,7 @@ struct BlockArgs {
     uint32_t reuse_pattern;    // consecutive items with the same pattern reuse its PM
     uint32_t* queue;           // work counter, zero at launch
     uint32_t batch;            // items taken per queue grab (>= 1)
+    uint32_t qgram;            // banded launches: run the q-gram count filter (some 7k + 7 < m)
     uint64_t* pm_scratch;      // global arenas (when the PM does not fit in LDS)
     uint64_t pm_scratch_words; // uint64 words per workgroup arena
     uint64_t* tflag_scratch;   // Jaro text flags, tflag_words per workgroup

@@ -50,18 +50,6 @@ __device__ __forceinline__ constexpr uint64_t seg_first_mask()
 
 constexpr uint32_t T_AND_XNOR = (TA & ~(TB ^ TC)) & 0xFF; // a & ~(b ^ c)
 
-// K consecutive code units from p (any alignment: the loads are dword loads of
-// the covering bytes).  Reads up to K - 1 units past the string's end, which the
-// 64-byte tail padding of every string set keeps inside the allocation.
-template <typename CharT, uint32_t K>
-__device__ __forceinline__ void load_units(const CharT* p, uint32_t* out)
-{
-    CharT tmp[K];
-    __builtin_memcpy(tmp, p, sizeof(tmp));
-#pragma unroll
-    for (uint32_t k = 0; k < K; ++k) out[k] = (uint32_t)tmp[k];
-}
-
 // Recurrence over the text for one word per lane; returns the lane's share of
 // the pair's raw value (Levenshtein: sum of vertical deltas of the last
 // column; LCS: matched rows).

@@ -197,6 +197,37 @@ def test_banded_levenshtein_cutoffs(ctx, oracle):
             assert np.array_equal(got, exp), f"cutoff {cu} semantics {semantics}: {np.count_nonzero(got != exp)} differ"
 
 
+@pytest.mark.parametrize("width", [1, 2, 4])
+def test_banded_qgram_filter_bound(ctx, oracle, width):
+    """The q-gram count filter in front of the banded recurrence must never reject a pair whose
+    distance is within the cutoff.  Substitutions every 8th unit destroy the most 8-grams per edit,
+    so distances d with cutoffs d - 1, d, d + 1 sit exactly on the filter's bound; unrelated strings
+    and random mutations cover the rejecting side.  16/32-bit units take the hashed q-gram path."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(90 + width)
+    base = {1: 65, 2: 40000, 4: 3000000}[width]
+    queries, choices = [], []
+    for L in (120, 333, 800, 1500):
+        q = (rng.integers(0, 4, size=L) + base).astype(np.uint32)
+        queries.append(q)
+        for step in (8, 9, 16, 40):
+            t = q.copy()
+            t[::step] = base + (t[::step] - base + 1) % 4
+            choices.append(t)
+        choices.append(_gen.mutate(rng, q - base, int(rng.integers(1, L // 30 + 2)), alpha_lo=0, alpha_n=4)
+                       .astype(np.uint32) + base)
+        choices.append((rng.integers(0, 4, size=L + int(rng.integers(-5, 6))) + base).astype(np.uint32))
+    full = oracle.cdist(0, 0, queries, choices)
+    cutoffs = sorted({0, 2, 7, 15, 40} | {int(d) + e for d in full.ravel() for e in (-1, 0, 1) if 0 <= int(d) + e < 400})
+    sc = rfa.Scorer(ctx, 0, ctx.strings(queries, width=width))
+    cs = ctx.strings(choices, width=width)
+    for cu in cutoffs:
+        got = sc.cdist(cs, 0, cutoff_u=cu)
+        exp = np.where(full <= cu, full, cu + 1).astype(np.uint64)
+        assert np.array_equal(got, exp), f"cutoff {cu}: {np.count_nonzero(got != exp)} differ"
+
+
 def test_banded_levenshtein_pairs(ctx, oracle):
     """The same through rfa_pairs (the mixed-metric block kernel), both schedules."""
     import rapidfuzz_amd as rfa
