@@ -518,14 +518,14 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
     if (n > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 pairs");
     // one pass: validate, note OSA, stage into pinned memory
     int slot_idx = 0;
-    int32_t* staged = (int32_t*)ctx->staging.acquire(n * 4, &slot_idx);
+    uint8_t* staged = (uint8_t*)ctx->staging.acquire(n, &slot_idx); // one byte per pair on the device
     bool has_osa = false;
     bool bad_metric = false;
     for (uint64_t i = 0; i < n; ++i) {
         const int32_t m = metrics[i];
         bad_metric |= (m < RFA_LEVENSHTEIN) | (m > RFA_OSA);
         has_osa |= m == RFA_OSA;
-        staged[i] = m;
+        staged[i] = (uint8_t)m;
     }
     if (bad_metric) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "pair metrics must be Levenshtein, Indel, LCSseq or OSA");
     hipStream_t s = ctx->stream, s2 = ctx->stream2;
@@ -548,8 +548,8 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         uint64_t maxm = 0;
         for (uint64_t i = 0; i < n; ++i) maxm = std::max(maxm, a->len(i));
         if (maxm > 4096) throw ApiError(RFA_ERR_UNSUPPORTED, "the plain schedule handles patterns up to 4096 units");
-        int32_t* dmet = (int32_t*)ctx->lane_metric.get(n * 4);
-        ctx->staging.copy(slot_idx, dmet, n * 4, s);
+        uint8_t* dmet = (uint8_t*)ctx->lane_metric.get(n);
+        ctx->staging.copy(slot_idx, dmet, n, s);
         la.item_metric = dmet;
         la.nitems = n;
         const uint32_t nlanes = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 4 * 64,
@@ -562,8 +562,8 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         // bucketed, planned on the device: pattern = the shorter string; Levenshtein / Indel /
         // LCSseq patterns of <= 4096 units -> segmented kernel, OSA patterns of <= 64 units ->
         // lane kernel, the rest -> block kernel (longest-processing-time-first queue)
-        int32_t* dmet = (int32_t*)ctx->lane_metric.get(n * 4);
-        ctx->staging.copy(slot_idx, dmet, n * 4, s);
+        uint8_t* dmet = (uint8_t*)ctx->lane_metric.get(n);
+        ctx->staging.copy(slot_idx, dmet, n, s);
         uint32_t* bins = (uint32_t*)ctx->plan_keys.get(n * 4);
         uint32_t* vals = (uint32_t*)ctx->plan_vals.get(n * 8);
         const size_t temp_bytes = plan_temp_bytes(n);

@@ -62,7 +62,7 @@ struct BlockArgs {
     // the metric of a pair is pair_metric[pair] and its result goes to out[pair]
     const uint32_t* item_packed;
     const uint32_t* nitems_dev;
-    const int32_t* pair_metric;
+    const uint8_t* pair_metric; // metric per pair (RFA_LEVENSHTEIN .. RFA_OSA)
     const uint64_t* qcut; // cross mode: integer cutoff per string of set A (nullptr: spec.cutoff_u)
 };
 
@@ -80,7 +80,7 @@ struct LaneArgs {
     const uint32_t* item_a;      // nullptr: item index
     const uint32_t* item_b;      // nullptr: item index
     const uint8_t* item_swap;    // nullptr: pattern is always A's string
-    const int32_t* item_metric;  // nullptr: spec.metric for every item
+    const uint8_t* item_metric;  // nullptr: spec.metric for every item
     const uint64_t* out_index;   // nullptr: out[item]
     uint64_t nitems;
     uint64_t* arena;             // per-lane maps
@@ -93,7 +93,7 @@ struct LaneArgs {
     const uint32_t* item_packed;
     const uint32_t* begin_dev;
     const uint32_t* end_dev; // nullptr: nitems
-    const int32_t* pair_metric;
+    const uint8_t* pair_metric; // metric per pair (RFA_LEVENSHTEIN .. RFA_OSA)
 };
 
 // Segmented kernel (rfa_seg.hip): pairs with patterns of <= 4096 units under the
@@ -108,7 +108,7 @@ struct SegArgs {
     const void* bchars;
     const uint64_t* boff;
     const uint32_t* item_packed; // planned items, pair | swap << 31
-    const int32_t* pair_metric;
+    const uint8_t* pair_metric; // metric per pair (RFA_LEVENSHTEIN .. RFA_OSA)
     const uint32_t* counters;    // range r: items [counters[3 + r], counters[4 + r])
     ScoreSpec spec;
     void* out;                   // out[pair]
@@ -124,7 +124,7 @@ size_t plan_temp_bytes(uint64_t n);
 // Counting sort of the pairs into kernel ranges (items vals_out[k] = pair | pattern-is-B << 31).
 // counters[0]: block queue; [1]: end of the block range; [2], [3]: ends of the OSA
 // 33-64 and 1-32 unit ranges; [4 + r]: end of segmented range r (r = 0..13).
-hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t* metric, uint64_t n, uint32_t* bins,
+hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const uint8_t* metric, uint64_t n, uint32_t* bins,
                        uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes, uint32_t* counters,
                        hipStream_t s);
 

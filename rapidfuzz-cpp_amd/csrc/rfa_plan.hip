@@ -67,12 +67,12 @@ __device__ __forceinline__ uint32_t words_class(uint64_t m)
 
 // bin and packed item (pair | pattern-is-B << 31) of pair i
 __device__ __forceinline__ uint32_t pair_bin(const uint64_t* __restrict__ aoff, const uint64_t* __restrict__ boff,
-                                             const int32_t* __restrict__ metric, uint64_t i, uint32_t* packed)
+                                             const uint8_t* __restrict__ metric, uint64_t i, uint32_t* packed)
 {
     const uint64_t la = aoff[i + 1] - aoff[i], lb = boff[i + 1] - boff[i];
     bool pat_b = lb < la; // the pattern is the shorter string ...
     const uint64_t s = pat_b ? lb : la, l = pat_b ? la : lb;
-    const int32_t met = metric[i];
+    const int32_t met = (int32_t)metric[i];
     uint32_t bin;
     if (met != RFA_OSA && s <= SEG_MAX_UNITS) {
         uint64_t m = s, t = l;
@@ -99,7 +99,7 @@ __device__ __forceinline__ uint32_t pair_bin(const uint64_t* __restrict__ aoff, 
 // Workgroup g bins pairs [g * chunk, min(n, (g + 1) * chunk)); hist[bin * groups + g].
 __global__ __launch_bounds__(PLAN_THREADS) void k_bin_count(const uint64_t* __restrict__ aoff,
                                                             const uint64_t* __restrict__ boff,
-                                                            const int32_t* __restrict__ metric, uint64_t n,
+                                                            const uint8_t* __restrict__ metric, uint64_t n,
                                                             uint64_t chunk, uint32_t* __restrict__ bins,
                                                             uint32_t* __restrict__ vals, uint32_t* __restrict__ hist)
 {
@@ -156,7 +156,7 @@ size_t plan_temp_bytes(uint64_t n)
     return 2 * cells * sizeof(uint32_t) + scan + 256;
 }
 
-hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const int32_t* metric, uint64_t n, uint32_t* bins,
+hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const uint8_t* metric, uint64_t n, uint32_t* bins,
                        uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes, uint32_t* counters,
                        hipStream_t s)
 {
