@@ -92,6 +92,12 @@ rfa_status rfa_context_destroy(rfa_context* ctx);
 rfa_status rfa_context_synchronize(rfa_context* ctx);
 /* The context's hipStream_t, for callers that time or order work. */
 void* rfa_context_stream(rfa_context* ctx);
+/* Context options (no reference counterpart).
+ * RFA_OPTION_PAIRS_GRAPH (default 0): 1 = an rfa_pairs call (bucketed schedule)
+ * identical to the previous one (same string sets, kind, cutoffs, output and
+ * buffer sizes) replays its launch sequence from a captured HIP graph. */
+#define RFA_OPTION_PAIRS_GRAPH 1
+rfa_status rfa_context_set_option(rfa_context* ctx, int option, int64_t value);
 /* Message of the last failed call on this thread ("" if none). */
 const char* rfa_last_error(void);
 int rfa_abi_version(void);

@@ -563,68 +563,124 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         // LCSseq patterns of <= 4096 units -> segmented kernel, OSA patterns of <= 64 units ->
         // lane kernel, the rest -> block kernel (longest-processing-time-first queue)
         uint8_t* dmet = (uint8_t*)ctx->lane_metric.get(n);
-        ctx->staging.copy(slot_idx, dmet, n, s);
         uint32_t* bins = (uint32_t*)ctx->plan_keys.get(n * 4);
         uint32_t* vals = (uint32_t*)ctx->plan_vals.get(n * 8);
         const size_t temp_bytes = plan_temp_bytes(n);
         void* temp = ctx->plan_temp.get(temp_bytes);
         uint32_t* counters = (uint32_t*)ctx->queue.get(128); // 18 range ends and the block queue, written by the plan
-        HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, bins, vals, vals + n, temp, temp_bytes, counters, s));
         const uint32_t* packed = vals + n;
         const uint64_t maxm = std::min(a->max_len, b->max_len);
         const uint64_t maxn = std::max(a->max_len, b->max_len);
-        // with block work, the segmented and lane kernels (second stream) overlap the block
-        // queue (first stream); without, everything stays on the first stream
         const bool need_block = maxm > 4096 || (has_osa && maxm > 64);
-        hipStream_t sk = s;
-        if (need_block) {
-            HIP_CHECK(hipEventRecord(ctx->fork, s));
-            HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
-            sk = s2;
-            BlockArgs ba{};
-            ba.achars = a->d_chars;
-            ba.aoff = a->d_off;
-            ba.bchars = b->d_chars;
-            ba.boff = b->d_off;
-            ba.item_packed = packed;
-            ba.nitems_dev = counters + 1;
-            ba.pair_metric = dmet;
-            ba.nitems = (uint32_t)n; // upper bound, for the grid size
-            ba.spec = base;
-            ba.out = dout;
-            ba.qgram = kind == RFA_DISTANCE && cu < (1ull << 40) && 7 * cu + 7 < maxn ? 1u : 0u;
-            run_block(ctx, ALGO_MIXED, width, ba, maxm, maxn, false, counters);
-        }
-        SegArgs sa{};
-        sa.achars = a->d_chars;
-        sa.aoff = a->d_off;
-        sa.bchars = b->d_chars;
-        sa.boff = b->d_off;
-        sa.item_packed = packed;
-        sa.pair_metric = dmet;
-        sa.counters = counters;
-        sa.spec = base;
-        sa.out = dout;
         const uint32_t nb = seg_symbol_bits(width, std::max(a->max_unit(), b->max_unit()));
-        // no more waves than pairs (each wave holds up to 64 pairs, at least one)
-        const uint32_t max_blocks = (uint32_t)std::min<uint64_t>((n + 3) / 4, 1u << 20);
-        HIP_CHECK(launch_seg(width, nb, sa, (uint32_t)ctx->num_cus, max_blocks, sk));
-        if (has_osa) {
-            la.item_packed = packed;
-            la.pair_metric = dmet;
-            la.nitems = n;
-            // OSA 33-64 unit range [counters[1], counters[2]), then 1-32 units [counters[2], counters[3])
-            const uint32_t nblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 6, (n + 63) / 64);
-            la.begin_dev = counters + 1;
-            la.end_dev = counters + 2;
-            HIP_CHECK(launch_lane_short(width, 64, la, nblk, sk));
-            la.begin_dev = counters + 2;
-            la.end_dev = counters + 3;
-            HIP_CHECK(launch_lane_short(width, 32, la, nblk, sk));
+        // the launch sequence after the metric upload; everything it reads on the host is in
+        // the graph key below
+        auto enqueue = [&]() {
+            HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, bins, vals, vals + n, temp, temp_bytes, counters, s));
+            // with block work, the segmented and lane kernels (second stream) overlap the block
+            // queue (first stream); without, everything stays on the first stream
+            hipStream_t sk = s;
+            if (need_block) {
+                HIP_CHECK(hipEventRecord(ctx->fork, s));
+                HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
+                sk = s2;
+                BlockArgs ba{};
+                ba.achars = a->d_chars;
+                ba.aoff = a->d_off;
+                ba.bchars = b->d_chars;
+                ba.boff = b->d_off;
+                ba.item_packed = packed;
+                ba.nitems_dev = counters + 1;
+                ba.pair_metric = dmet;
+                ba.nitems = (uint32_t)n; // upper bound, for the grid size
+                ba.spec = base;
+                ba.out = dout;
+                ba.qgram = kind == RFA_DISTANCE && cu < (1ull << 40) && 7 * cu + 7 < maxn ? 1u : 0u;
+                run_block(ctx, ALGO_MIXED, width, ba, maxm, maxn, false, counters);
+            }
+            SegArgs sa{};
+            sa.achars = a->d_chars;
+            sa.aoff = a->d_off;
+            sa.bchars = b->d_chars;
+            sa.boff = b->d_off;
+            sa.item_packed = packed;
+            sa.pair_metric = dmet;
+            sa.counters = counters;
+            sa.spec = base;
+            sa.out = dout;
+            // no more waves than pairs (each wave holds up to 64 pairs, at least one)
+            const uint32_t max_blocks = (uint32_t)std::min<uint64_t>((n + 3) / 4, 1u << 20);
+            HIP_CHECK(launch_seg(width, nb, sa, (uint32_t)ctx->num_cus, max_blocks, sk));
+            if (has_osa) {
+                la.item_packed = packed;
+                la.pair_metric = dmet;
+                la.nitems = n;
+                // OSA 33-64 unit range [counters[1], counters[2]), then 1-32 units [counters[2], counters[3])
+                const uint32_t nblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 6, (n + 63) / 64);
+                la.begin_dev = counters + 1;
+                la.end_dev = counters + 2;
+                HIP_CHECK(launch_lane_short(width, 64, la, nblk, sk));
+                la.begin_dev = counters + 2;
+                la.end_dev = counters + 3;
+                HIP_CHECK(launch_lane_short(width, 32, la, nblk, sk));
+            }
+            if (need_block) {
+                HIP_CHECK(hipEventRecord(ctx->join, s2));
+                HIP_CHECK(hipStreamWaitEvent(s, ctx->join, 0));
+            }
+        };
+        // every host value the launch sequence bakes in: same key, same launches
+        auto key_now = [&]() {
+            uint64_t cdb;
+            std::memcpy(&cdb, &cd, 8);
+            return std::vector<uint64_t>{(uint64_t)(uintptr_t)a->d_chars, (uint64_t)(uintptr_t)a->d_off,
+                                         (uint64_t)(uintptr_t)b->d_chars, (uint64_t)(uintptr_t)b->d_off, n,
+                                         (uint64_t)kind, cu, cdb, (uint64_t)(uintptr_t)dout, (uint64_t)width,
+                                         (uint64_t)has_osa, maxm, maxn, nb, (uint64_t)ctx->num_cus,
+                                         (uint64_t)(uintptr_t)ctx->lane_metric.p, ctx->lane_metric.cap,
+                                         (uint64_t)(uintptr_t)ctx->plan_keys.p, ctx->plan_keys.cap,
+                                         (uint64_t)(uintptr_t)ctx->plan_vals.p, ctx->plan_vals.cap,
+                                         (uint64_t)(uintptr_t)ctx->plan_temp.p, ctx->plan_temp.cap,
+                                         (uint64_t)(uintptr_t)ctx->queue.p, ctx->queue.cap,
+                                         (uint64_t)(uintptr_t)ctx->pm_scratch.p, ctx->pm_scratch.cap};
+        };
+        // the metric upload stays outside the graph: its pinned source alternates per call
+        ctx->staging.copy(slot_idx, dmet, n, s);
+        PairsGraph& g = ctx->pairs_graph;
+        const std::vector<uint64_t> key = ctx->pairs_graph_on ? key_now() : std::vector<uint64_t>{};
+        if (!ctx->pairs_graph_on) {
+            enqueue();
         }
-        if (need_block) {
-            HIP_CHECK(hipEventRecord(ctx->join, s2));
-            HIP_CHECK(hipStreamWaitEvent(s, ctx->join, 0));
+        else if (g.exec && key == g.key) {
+            HIP_CHECK(hipGraphLaunch(g.exec, s));
+        }
+        else if (key == g.last_eager) {
+            // the same call as the last one, whose buffers are all sized: capture it once and
+            // replay the graph from now on
+            g.release();
+            HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipGraph_t graph = nullptr;
+            try {
+                enqueue();
+            }
+            catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            HIP_CHECK(hipStreamEndCapture(s, &graph));
+            const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                g.exec = nullptr;
+                throw HipError(ie, "hipGraphInstantiate");
+            }
+            g.key = key;
+            HIP_CHECK(hipGraphLaunch(g.exec, s));
+        }
+        else {
+            enqueue();
+            g.last_eager = key_now();
         }
     }
     if (!out_on_device) {
@@ -684,6 +740,7 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
                           &ctx->topk_out, &ctx->qcut})
             b->release();
         ctx->staging.release();
+        ctx->pairs_graph.release();
         (void)hipEventDestroy(ctx->fork);
         (void)hipEventDestroy(ctx->join);
         (void)hipStreamDestroy(ctx->stream2);
@@ -699,6 +756,22 @@ rfa_status rfa_context_synchronize(rfa_context* ctx)
 }
 
 void* rfa_context_stream(rfa_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+rfa_status rfa_context_set_option(rfa_context* ctx, int option, int64_t value)
+{
+    if (!ctx) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (option != RFA_OPTION_PAIRS_GRAPH) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown option");
+        if (value != 0 && value != 1) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "RFA_OPTION_PAIRS_GRAPH is 0 or 1");
+        ctx->pairs_graph_on = value != 0;
+        if (!ctx->pairs_graph_on) {
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            ctx->pairs_graph.release();
+            ctx->pairs_graph.last_eager.clear();
+        }
+    });
+}
 
 rfa_status rfa_strings_create(rfa_context* ctx, const void* chars, int char_width, const uint64_t* offsets,
                               uint64_t count, rfa_strings** out)

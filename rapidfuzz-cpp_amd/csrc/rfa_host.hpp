@@ -139,6 +139,21 @@ struct Staging {
     }
 };
 
+// The pairs schedule's launch sequence as a captured HIP graph.  A call whose
+// key (every host value the sequence bakes in: string and buffer addresses and
+// sizes, kind, cutoffs, output) equals the previous eager call's is captured
+// once; later calls with that key replay the graph.
+struct PairsGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> key, last_eager;
+    void release()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+};
+
 inline uint64_t host_ceil_div(uint64_t a, uint64_t b) { return a / b + (a % b != 0); }
 
 inline uint32_t log2_ceil(uint64_t x)
@@ -202,6 +217,8 @@ struct rfa_context {
     uint32_t gen_next = 1;
     rfa::Keep keep; // host sources of in-flight uploads
     rfa::Staging staging; // pinned per-call uploads of the pairs path
+    rfa::PairsGraph pairs_graph;
+    bool pairs_graph_on = false; // RFA_OPTION_PAIRS_GRAPH
 };
 
 struct rfa_strings {

@@ -18,6 +18,7 @@ LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO = range(7)
 DISTANCE, SIMILARITY, NORMALIZED_DISTANCE, NORMALIZED_SIMILARITY = range(4)
 CACHED, MULTI = 0, 1
 SCHEDULE_BUCKETED, SCHEDULE_PLAIN = 0, 1
+OPTION_PAIRS_GRAPH = 1
 INT64_MAX = (1 << 63) - 1
 
 _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED", 3: "RFA_ERR_HIP",
@@ -25,7 +26,7 @@ _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED",
 
 # every symbol include/rapidfuzz_amd.h declares
 EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
-            "rfa_context_synchronize", "rfa_context_stream", "rfa_strings_create", "rfa_strings_destroy",
+            "rfa_context_synchronize", "rfa_context_stream", "rfa_context_set_option", "rfa_strings_create", "rfa_strings_destroy",
             "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_cdist_cutoffs",
             "rfa_pairs", "rfa_topk",
             "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
@@ -76,6 +77,7 @@ def load_library(path: str = LIB_PATH):
     L.rfa_context_synchronize.argtypes = [_vp]
     L.rfa_context_stream.argtypes = [_vp]
     L.rfa_context_stream.restype = _vp
+    L.rfa_context_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int64]
     L.rfa_strings_create.argtypes = [_vp, _vp, ctypes.c_int, _vp, ctypes.c_uint64, ctypes.POINTER(_vp)]
     L.rfa_strings_destroy.argtypes = [_vp]
     L.rfa_strings_count.argtypes = [_vp]
@@ -174,6 +176,10 @@ class Context:
 
     def synchronize(self):
         _check(_lib.rfa_context_synchronize(self.h))
+
+    def set_option(self, option: int, value: int):
+        """rfa_context_set_option, e.g. OPTION_PAIRS_GRAPH: replay repeated pairs calls from a HIP graph."""
+        _check(_lib.rfa_context_set_option(self.h, option, value))
 
     def close(self):
         """Destroys the context; strings and scorers made from it become unusable

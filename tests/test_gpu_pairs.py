@@ -153,3 +153,55 @@ def test_pairs_many_short_mixed(ctx, oracle):
     assert np.array_equal(got, exp)
     plain = rfa.pairs(ctx, A, B, metrics, schedule=rfa.SCHEDULE_PLAIN)
     assert np.array_equal(plain, exp)
+
+
+@pytest.mark.parametrize("with_block", [False, True])
+def test_pairs_graph_replays(ctx, oracle, with_block):
+    """Repeated identical rfa_pairs calls are captured once into a HIP graph and replayed.  Every
+    replay must equal the oracle, with eager work (other pairs calls, a cdist, synchronizations)
+    between the replays, new metric values between replays, and a larger call in between that
+    regrows the plan buffers (which forces a new capture)."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(123 + with_block)
+    a, b = _pairs(rng, 700, 600, 5, 4)
+    if with_block:  # block kernel work (> 4096-unit patterns, long OSA) overlapping on the second stream
+        a += [rng.integers(97, 102, size=4200).astype(np.uint32), rng.integers(97, 102, size=300).astype(np.uint32)]
+        b += [rng.integers(97, 102, size=4300).astype(np.uint32), rng.integers(97, 102, size=280).astype(np.uint32)]
+    n = len(a)
+    A, B = ctx.strings(a, width=4), ctx.strings(b, width=4)
+    other_a, other_b = _pairs(rng, 300, 200, 5, 4)
+    OA, OB = ctx.strings(other_a, width=4), ctx.strings(other_b, width=4)
+    big_a, big_b = _pairs(rng, 3000, 300, 5, 4)
+    BA, BB = ctx.strings(big_a, width=4), ctx.strings(big_b, width=4)
+    sc = rfa.Scorer(ctx, 0, ctx.strings(a[:20], width=4))
+    ctx.set_option(rfa.OPTION_PAIRS_GRAPH, 1)
+    try:
+        _graph_replays(ctx, oracle, rng, A, B, a, b, OA, OB, other_a, other_b, BA, BB, big_a, big_b, sc, with_block)
+    finally:
+        ctx.set_option(rfa.OPTION_PAIRS_GRAPH, 0)
+
+
+def _graph_replays(ctx, oracle, rng, A, B, a, b, OA, OB, other_a, other_b, BA, BB, big_a, big_b, sc, with_block):
+    import rapidfuzz_amd as rfa
+
+    n = len(a)
+    for it in range(8):
+        metrics = rng.integers(0, 3, size=n).astype(np.int32)
+        if with_block:
+            metrics[-1] = 3  # the long OSA pair
+        if it < 3:
+            metrics = np.resize(np.array([0, 1, 2], np.int32), n)
+            if with_block:
+                metrics[-1] = 3
+        got = rfa.pairs(ctx, A, B, metrics, kind=0, cutoff_u=rfa.INT64_MAX)
+        exp = oracle.pairs(metrics, a, b)
+        assert np.array_equal(got, exp), f"replay {it}: {np.count_nonzero(got != exp)} differ"
+        # eager work between the replays
+        om = rng.integers(0, 3, size=len(other_a)).astype(np.int32)
+        assert np.array_equal(rfa.pairs(ctx, OA, OB, om), oracle.pairs(om, other_a, other_b))
+        sc.cdist(OB, 0)
+        ctx.synchronize()
+        if it == 5:
+            bm = rng.integers(0, 3, size=len(big_a)).astype(np.int32)
+            assert np.array_equal(rfa.pairs(ctx, BA, BB, bm), oracle.pairs(bm, big_a, big_b))
