@@ -154,6 +154,11 @@ def workload_uneven(args, emit=True):
     B = rfa.Strings(ctx, chars=bc, offsets=bo, width=4)
     out = hip.DeviceBuffer(N * 8)
     stream = ctx.stream
+    # the schedule's autotuner (untimed): picks the long-pair threshold on this batch
+    t0 = time.perf_counter()
+    chain = rfa.pairs_autotune(ctx, A, B, metrics)
+    print(f"# uneven: autotuned long-pair threshold {chain} ({time.perf_counter() - t0:.2f} s)", file=sys.stderr,
+          flush=True)
     res = {}
     for name, sched in (("bucketed", rfa.SCHEDULE_BUCKETED), ("plain", rfa.SCHEDULE_PLAIN)):
         for _ in range(args.warmup):
@@ -179,9 +184,9 @@ def workload_uneven(args, emit=True):
     cpu = None
     check = "schedules agree" if same else "SCHEDULES DISAGREE"
     # algorithmic INT32 work of the bucketed schedule: the segmented kernel takes the shorter
-    # string as the pattern, or the longer one when it has 512-4096 units
+    # string as the pattern, or the longer one when it has chain-4096 units
     s_len, l_len = np.minimum(la, lb).astype(np.float64), np.maximum(la, lb).astype(np.float64)
-    flip = (l_len >= 512) & (l_len <= 4096)
+    flip = (l_len >= chain) & (l_len <= 4096)
     words = np.ceil(np.where(flip, l_len, s_len) / 64)
     steps = np.where(flip, s_len, l_len)
     ops_per = np.where(metrics == 0, np.where(words == 1, OPS[0], 42.0), 10.0)
@@ -236,6 +241,7 @@ def workload_uneven(args, emit=True):
         "plain": {k: v for k, v in res["plain"].items() if k != "out"},
         "bucketed": {k: v for k, v in res["bucketed"].items() if k != "out"},
         "speedup_vs_plain": round(res["plain"]["ms"] / res["bucketed"]["ms"], 2),
+        "autotuned_chain_threshold": chain,
         "roofline": roofline, "cpu_baseline": cpu, "check": check,
     }
     if emit:
@@ -814,6 +820,7 @@ def main():
         une = workload_uneven(args, emit=False)
         res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
                                              "speedup_vs_plain": une["speedup_vs_plain"],
+                                             "autotuned_chain_threshold": une["autotuned_chain_threshold"],
                                              "config": une["config"]["workload"]}
     print(json.dumps(res), flush=True)
 

@@ -97,6 +97,12 @@ void* rfa_context_stream(rfa_context* ctx);
  * identical to the previous one (same string sets, kind, cutoffs, output and
  * buffer sizes) replays its launch sequence from a captured HIP graph. */
 #define RFA_OPTION_PAIRS_GRAPH 1
+/* RFA_OPTION_PAIRS_CHAIN (default 512, in [1, 2^31]): in the bucketed pairs
+ * schedule, a pair whose longer string has this many units or more (up to
+ * 4096) uses the longer string as the bit-parallel pattern (lanes per pair =
+ * its words), and pairs whose text has this many units or more are scheduled
+ * first.  rfa_pairs_autotune sets it by measurement. */
+#define RFA_OPTION_PAIRS_CHAIN 2
 rfa_status rfa_context_set_option(rfa_context* ctx, int option, int64_t value);
 /* Message of the last failed call on this thread ("" if none). */
 const char* rfa_last_error(void);
@@ -156,6 +162,13 @@ rfa_status rfa_cdist_cutoffs(rfa_scorer* scorer, const rfa_strings* choices, int
  * number of strings and the same char width. */
 rfa_status rfa_pairs(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics, int kind,
                      uint64_t cutoff_u, double cutoff_d, int schedule, void* out, int out_on_device);
+/* Autotuner of the bucketed pairs schedule (no reference counterpart): times
+ * the schedule on this batch for each candidate RFA_OPTION_PAIRS_CHAIN value
+ * (128 .. 2048, and 4097 = the shorter string is always the pattern), keeps the
+ * fastest in the context and returns it in *chosen.  Results do not depend on
+ * the choice; it runs the batch 24 times. */
+rfa_status rfa_pairs_autotune(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics,
+                              int kind, uint64_t cutoff_u, double cutoff_d, uint32_t* chosen);
 
 /* Top-k extraction.  Replaces the README's extract() loop (README.md:153-214):
  * for every query, score = Cached<Metric>(query).<kind>(choice, cutoff_d) for

@@ -576,7 +576,8 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         // the launch sequence after the metric upload; everything it reads on the host is in
         // the graph key below
         auto enqueue = [&]() {
-            HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, bins, vals, vals + n, temp, temp_bytes, counters, s));
+            HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, ctx->seg_chain_steps, bins, vals, vals + n, temp,
+                                  temp_bytes, counters, s));
             // with block work, the segmented and lane kernels (second stream) overlap the block
             // queue (first stream); without, everything stays on the first stream
             hipStream_t sk = s;
@@ -637,6 +638,7 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
                                          (uint64_t)(uintptr_t)b->d_chars, (uint64_t)(uintptr_t)b->d_off, n,
                                          (uint64_t)kind, cu, cdb, (uint64_t)(uintptr_t)dout, (uint64_t)width,
                                          (uint64_t)has_osa, maxm, maxn, nb, (uint64_t)ctx->num_cus,
+                                         (uint64_t)ctx->seg_chain_steps,
                                          (uint64_t)(uintptr_t)ctx->lane_metric.p, ctx->lane_metric.cap,
                                          (uint64_t)(uintptr_t)ctx->plan_keys.p, ctx->plan_keys.cap,
                                          (uint64_t)(uintptr_t)ctx->plan_vals.p, ctx->plan_vals.cap,
@@ -762,6 +764,12 @@ rfa_status rfa_context_set_option(rfa_context* ctx, int option, int64_t value)
     if (!ctx) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     return guarded([&] {
         std::lock_guard<std::mutex> lk(ctx->mu);
+        if (option == RFA_OPTION_PAIRS_CHAIN) {
+            if (value < 1 || value > (int64_t)1 << 31)
+                throw ApiError(RFA_ERR_INVALID_ARGUMENT, "RFA_OPTION_PAIRS_CHAIN must be in [1, 2^31]");
+            ctx->seg_chain_steps = (uint32_t)value;
+            return;
+        }
         if (option != RFA_OPTION_PAIRS_GRAPH) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown option");
         if (value != 0 && value != 1) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "RFA_OPTION_PAIRS_GRAPH is 0 or 1");
         ctx->pairs_graph_on = value != 0;
@@ -863,6 +871,55 @@ rfa_status rfa_pairs(rfa_context* ctx, const rfa_strings* a, const rfa_strings* 
         HIP_CHECK(hipSetDevice(ctx->device));
         begin_call(ctx);
         pairs_impl(ctx, a, b, metrics, kind, cutoff_u, cutoff_d, schedule, out, out_on_device);
+    });
+}
+
+
+rfa_status rfa_pairs_autotune(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics,
+                              int kind, uint64_t cutoff_u, double cutoff_d, uint32_t* chosen)
+{
+    if (!ctx || !a || !b) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, a or b is NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_CHECK(hipSetDevice(ctx->device));
+        begin_call(ctx);
+        // candidates for the length from which the longer string is the pattern and from
+        // which pairs are scheduled first; 4097 keeps the shorter string everywhere
+        static const uint32_t cand[] = {128, 256, 512, 1024, 2048, 4097};
+        const uint32_t keep = ctx->seg_chain_steps;
+        void* dout = ctx->out_tmp.get((size_t)a->count * 8 + 8);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        uint32_t best = keep;
+        float best_ms = -1.0f;
+        try {
+            for (uint32_t c : cand) {
+                ctx->seg_chain_steps = c;
+                pairs_impl(ctx, a, b, metrics, kind, cutoff_u, cutoff_d, RFA_SCHEDULE_BUCKETED, dout, 1); // warm-up
+                HIP_CHECK(hipEventRecord(e0, ctx->stream));
+                for (int r = 0; r < 3; ++r)
+                    pairs_impl(ctx, a, b, metrics, kind, cutoff_u, cutoff_d, RFA_SCHEDULE_BUCKETED, dout, 1);
+                HIP_CHECK(hipEventRecord(e1, ctx->stream));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (best_ms < 0.0f || ms < best_ms) {
+                    best_ms = ms;
+                    best = c;
+                }
+            }
+        }
+        catch (...) {
+            ctx->seg_chain_steps = keep;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->seg_chain_steps = best;
+        if (chosen) *chosen = best;
     });
 }
 

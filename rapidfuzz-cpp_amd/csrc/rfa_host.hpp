@@ -219,6 +219,7 @@ struct rfa_context {
     rfa::Staging staging; // pinned per-call uploads of the pairs path
     rfa::PairsGraph pairs_graph;
     bool pairs_graph_on = false; // RFA_OPTION_PAIRS_GRAPH
+    uint32_t seg_chain_steps = rfa::SEG_CHAIN_STEPS; // RFA_OPTION_PAIRS_CHAIN, set by rfa_pairs_autotune
 };
 
 struct rfa_strings {

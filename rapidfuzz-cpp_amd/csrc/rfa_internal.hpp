@@ -98,8 +98,10 @@ struct LaneArgs {
 
 // Segmented kernel (rfa_seg.hip): pairs with patterns of <= 4096 units under the
 // Levenshtein or LCS recurrence.  Ranges r = 0..13 of the plan: r < 7 holds the
-// pairs with texts of >= SEG_CHAIN_STEPS units, r >= 7 the rest; range r runs
-// SEG = 64 >> (r % 7) lanes per pair.
+// pairs with texts of >= chain_steps units (the long sequential chains), r >= 7
+// the rest; range r runs SEG = 64 >> (r % 7) lanes per pair.  chain_steps is
+// also the length from which the longer string becomes the pattern (tuned per
+// context by rfa_pairs_autotune; SEG_CHAIN_STEPS by default).
 constexpr uint32_t SEG_CLASSES = 7;
 constexpr uint32_t SEG_CHAIN_STEPS = 512;
 struct SegArgs {
@@ -124,9 +126,9 @@ size_t plan_temp_bytes(uint64_t n);
 // Counting sort of the pairs into kernel ranges (items vals_out[k] = pair | pattern-is-B << 31).
 // counters[0]: block queue; [1]: end of the block range; [2], [3]: ends of the OSA
 // 33-64 and 1-32 unit ranges; [4 + r]: end of segmented range r (r = 0..13).
-hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const uint8_t* metric, uint64_t n, uint32_t* bins,
-                       uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes, uint32_t* counters,
-                       hipStream_t s);
+hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const uint8_t* metric, uint64_t n,
+                       uint32_t chain_steps, uint32_t* bins, uint32_t* vals_in, uint32_t* vals_out, void* temp,
+                       size_t temp_bytes, uint32_t* counters, hipStream_t s);
 
 // Top-k scoring against a resident choice index (rfa_serve.hip).  Choices are
 // sorted by length into groups of 64; group g holds gdwords[g] dwords per lane

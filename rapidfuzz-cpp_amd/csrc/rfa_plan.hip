@@ -11,10 +11,11 @@
 //                 length, longest first
 //     segmented : Levenshtein / Indel / LCSseq pairs whose shorter string has
 //                 <= 4096 units.  The pattern is the shorter string, or the
-//                 longer one when that has 512-4096 units (a 60-unit string
-//                 against a 3000-unit one then runs 64 lanes x 60 steps instead
-//                 of 1 lane x 3000 steps).  14 ranges: pairs with texts of
-//                 >= 512 units first (long sequential chains), then the rest;
+//                 longer one when that has C-4096 units (C = chain_steps, 512
+//                 by default: a 60-unit string against a 3000-unit one then
+//                 runs 64 lanes x 60 steps instead of 1 lane x 3000 steps).
+//                 14 ranges: pairs with texts of >= C units first (long
+//                 sequential chains), then the rest;
 //                 within each, lanes-per-pair classes 64, 32, ..., 1; within a
 //                 class Levenshtein before the LCS metrics, each by text length
 //                 (exact below 256 units, 32 bins per power of two above),
@@ -67,7 +68,8 @@ __device__ __forceinline__ uint32_t words_class(uint64_t m)
 
 // bin and packed item (pair | pattern-is-B << 31) of pair i
 __device__ __forceinline__ uint32_t pair_bin(const uint64_t* __restrict__ aoff, const uint64_t* __restrict__ boff,
-                                             const uint8_t* __restrict__ metric, uint64_t i, uint32_t* packed)
+                                             const uint8_t* __restrict__ metric, uint64_t i, uint32_t chain,
+                                             uint32_t* packed)
 {
     const uint64_t la = aoff[i + 1] - aoff[i], lb = boff[i + 1] - boff[i];
     bool pat_b = lb < la; // the pattern is the shorter string ...
@@ -76,13 +78,13 @@ __device__ __forceinline__ uint32_t pair_bin(const uint64_t* __restrict__ aoff, 
     uint32_t bin;
     if (met != RFA_OSA && s <= SEG_MAX_UNITS) {
         uint64_t m = s, t = l;
-        if (l <= SEG_MAX_UNITS && l >= SEG_CHAIN_STEPS) {
-            // ... except in the segmented kernel when the longer string has 512-4096 units
+        if (l <= SEG_MAX_UNITS && l >= chain) {
+            // ... except in the segmented kernel when the longer string has chain-4096 units
             pat_b = !pat_b;
             m = l;
             t = s;
         }
-        const uint32_t r = (t >= SEG_CHAIN_STEPS ? 0u : SEG_CLASSES) + (SEG_CLASSES - 1 - words_class(m));
+        const uint32_t r = (t >= chain ? 0u : SEG_CLASSES) + (SEG_CLASSES - 1 - words_class(m));
         bin = SEG_BASE + (2 * r + (met == RFA_LEVENSHTEIN ? 0u : 1u)) * TBINS + (TBINS - 1 - text_bin(t));
     }
     else if (met == RFA_OSA && s <= 64) {
@@ -100,7 +102,7 @@ __device__ __forceinline__ uint32_t pair_bin(const uint64_t* __restrict__ aoff, 
 __global__ __launch_bounds__(PLAN_THREADS) void k_bin_count(const uint64_t* __restrict__ aoff,
                                                             const uint64_t* __restrict__ boff,
                                                             const uint8_t* __restrict__ metric, uint64_t n,
-                                                            uint64_t chunk, uint32_t* __restrict__ bins,
+                                                            uint32_t chain, uint64_t chunk, uint32_t* __restrict__ bins,
                                                             uint32_t* __restrict__ vals, uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t h[NBINS];
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_bin_count(const uint64_t* __re
     const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = std::min<uint64_t>(n, lo + chunk);
     for (uint64_t i = lo + threadIdx.x; i < hi; i += PLAN_THREADS) {
         uint32_t pk;
-        const uint32_t b = pair_bin(aoff, boff, metric, i, &pk);
+        const uint32_t b = pair_bin(aoff, boff, metric, i, chain, &pk);
         bins[i] = b;
         vals[i] = pk;
         atomicAdd(&h[b], 1u);
@@ -156,9 +158,9 @@ size_t plan_temp_bytes(uint64_t n)
     return 2 * cells * sizeof(uint32_t) + scan + 256;
 }
 
-hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const uint8_t* metric, uint64_t n, uint32_t* bins,
-                       uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes, uint32_t* counters,
-                       hipStream_t s)
+hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const uint8_t* metric, uint64_t n,
+                       uint32_t chain_steps, uint32_t* bins, uint32_t* vals_in, uint32_t* vals_out, void* temp,
+                       size_t temp_bytes, uint32_t* counters, hipStream_t s)
 {
     const uint32_t groups = plan_groups(n);
     const uint64_t chunk = (n + groups - 1) / groups;
@@ -167,8 +169,8 @@ hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const uint8_t
     uint32_t* pos = hist + cells;
     void* scan_temp = (void*)(((uintptr_t)(pos + cells) + 255) & ~(uintptr_t)255);
     size_t scan_bytes = temp_bytes - ((uint8_t*)scan_temp - (uint8_t*)temp);
-    hipLaunchKernelGGL(k_bin_count, dim3(groups), dim3(PLAN_THREADS), 0, s, aoff, boff, metric, n, chunk, bins,
-                       vals_in, hist);
+    hipLaunchKernelGGL(k_bin_count, dim3(groups), dim3(PLAN_THREADS), 0, s, aoff, boff, metric, n, chain_steps, chunk,
+                       bins, vals_in, hist);
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(scan_temp, scan_bytes, hist, pos, (int)cells, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin_scatter, dim3(groups), dim3(PLAN_THREADS), 0, s, n, chunk, bins, vals_in, pos, vals_out);

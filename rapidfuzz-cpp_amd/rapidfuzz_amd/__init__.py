@@ -18,7 +18,7 @@ LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO = range(7)
 DISTANCE, SIMILARITY, NORMALIZED_DISTANCE, NORMALIZED_SIMILARITY = range(4)
 CACHED, MULTI = 0, 1
 SCHEDULE_BUCKETED, SCHEDULE_PLAIN = 0, 1
-OPTION_PAIRS_GRAPH = 1
+OPTION_PAIRS_GRAPH, OPTION_PAIRS_CHAIN = 1, 2
 INT64_MAX = (1 << 63) - 1
 
 _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED", 3: "RFA_ERR_HIP",
@@ -28,7 +28,7 @@ _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED",
 EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
             "rfa_context_synchronize", "rfa_context_stream", "rfa_context_set_option", "rfa_strings_create", "rfa_strings_destroy",
             "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_cdist_cutoffs",
-            "rfa_pairs", "rfa_topk",
+            "rfa_pairs", "rfa_pairs_autotune", "rfa_topk",
             "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
             "rfa_server_get_stats"]
 
@@ -90,6 +90,7 @@ def load_library(path: str = LIB_PATH):
     L.rfa_cdist_cutoffs.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int]
     L.rfa_pairs.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                             _vp, ctypes.c_int]
+    L.rfa_pairs_autotune.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _vp]
     L.rfa_topk.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp]
     L.rfa_server_create.argtypes = [_vp, _vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
     L.rfa_server_destroy.argtypes = [_vp]
@@ -99,7 +100,7 @@ def load_library(path: str = LIB_PATH):
     L.rfa_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     for name in ("rfa_context_create", "rfa_context_destroy", "rfa_context_synchronize", "rfa_strings_create",
                  "rfa_strings_destroy", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_cdist_cutoffs",
-                 "rfa_pairs",
+                 "rfa_pairs", "rfa_pairs_autotune", "rfa_context_set_option",
                  "rfa_topk", "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait",
                  "rfa_server_replay", "rfa_server_get_stats"):
         getattr(L, name).restype = ctypes.c_int
@@ -353,3 +354,12 @@ def pairs(ctx: Context, a: Strings, b: Strings, metrics, kind=DISTANCE, cutoff_u
     out = np.zeros(a.count, dtype=dt)
     _check(_lib.rfa_pairs(ctx.h, a.h, b.h, m.ctypes.data, kind, cutoff_u, cutoff_d, schedule, out.ctypes.data, 0))
     return out
+
+
+def pairs_autotune(ctx: Context, a: Strings, b: Strings, metrics, kind=DISTANCE, cutoff_u=INT64_MAX, cutoff_d=1.0):
+    """rfa_pairs_autotune: times the bucketed schedule for each candidate long-pair threshold on this
+    batch, keeps the fastest in the context and returns it."""
+    m = np.ascontiguousarray(np.asarray(metrics, dtype=np.int32))
+    chosen = ctypes.c_uint32(0)
+    _check(_lib.rfa_pairs_autotune(ctx.h, a.h, b.h, m.ctypes.data, kind, cutoff_u, cutoff_d, ctypes.byref(chosen)))
+    return int(chosen.value)

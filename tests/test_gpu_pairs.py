@@ -205,3 +205,29 @@ def _graph_replays(ctx, oracle, rng, A, B, a, b, OA, OB, other_a, other_b, BA, B
         if it == 5:
             bm = rng.integers(0, 3, size=len(big_a)).astype(np.int32)
             assert np.array_equal(rfa.pairs(ctx, BA, BB, bm), oracle.pairs(bm, big_a, big_b))
+
+
+def test_pairs_chain_threshold_and_autotune(ctx, oracle):
+    """Every long-pair threshold (which string is the pattern, which pairs run first) gives the
+    same results; the autotuner picks one of its candidates and keeps it in the context."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(321)
+    a, b = _pairs(rng, 600, 3000, 5, 4)
+    a += [rng.integers(97, 102, size=3500).astype(np.uint32), rng.integers(97, 102, size=20).astype(np.uint32)]
+    b += [rng.integers(97, 102, size=40).astype(np.uint32), rng.integers(97, 102, size=4000).astype(np.uint32)]
+    metrics = rng.integers(0, 3, size=len(a)).astype(np.int32)
+    A, B = ctx.strings(a, width=4), ctx.strings(b, width=4)
+    exp = oracle.pairs(metrics, a, b)
+    try:
+        for chain in (1, 64, 128, 512, 2048, 4097, 1 << 31):
+            ctx.set_option(rfa.OPTION_PAIRS_CHAIN, chain)
+            got = rfa.pairs(ctx, A, B, metrics)
+            assert np.array_equal(got, exp), f"chain {chain}: {np.count_nonzero(got != exp)} differ"
+        chosen = rfa.pairs_autotune(ctx, A, B, metrics)
+        assert chosen in (128, 256, 512, 1024, 2048, 4097)
+        assert np.array_equal(rfa.pairs(ctx, A, B, metrics), exp)
+    finally:
+        ctx.set_option(rfa.OPTION_PAIRS_CHAIN, 512)
+    with pytest.raises(rfa.InvalidArgument):
+        ctx.set_option(rfa.OPTION_PAIRS_CHAIN, 0)
