@@ -442,7 +442,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ba.cross_nb = (uint32_t)nc;
         ba.out_stride = nc;
         uint64_t ni = (uint64_t)sc->block_queries.size() * nc;
-        if (ni > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "too many long-query pairs in one call");
+        if (ni > BLOCK_MAX_ITEMS) throw ApiError(RFA_ERR_UNSUPPORTED, "too many long-query pairs in one call");
         ba.nitems = (uint32_t)ni;
         ba.spec = spec;
         ba.out = dout;
@@ -471,7 +471,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ba.cross_nb = (uint32_t)jaro_long.size();
         ba.out_stride = nc;
         uint64_t ni = (uint64_t)sc->tile_order.size() * jaro_long.size();
-        if (ni > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "too many long-choice pairs in one call");
+        if (ni > BLOCK_MAX_ITEMS) throw ApiError(RFA_ERR_UNSUPPORTED, "too many long-choice pairs in one call");
         ba.nitems = (uint32_t)ni;
         ba.spec = spec;
         ba.out = dout;
@@ -515,7 +515,7 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
     const uint64_t n = a->count;
     if (n == 0) return;
     if (!metrics) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "metrics is NULL");
-    if (n > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 pairs");
+    if (n > BLOCK_MAX_ITEMS) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 - 2^26 pairs");
     // one pass: validate, note OSA, stage into pinned memory
     int slot_idx = 0;
     uint8_t* staged = (uint8_t*)ctx->staging.acquire(n, &slot_idx); // one byte per pair on the device

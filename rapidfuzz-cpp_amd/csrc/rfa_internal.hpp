@@ -66,6 +66,10 @@ struct BlockArgs {
     const uint64_t* qcut; // cross mode: integer cutoff per string of set A (nullptr: spec.cutoff_u)
 };
 
+// Largest item count of one block-kernel launch: the 32-bit queue head runs past the
+// last item by at most one batch (<= 1024 items) per workgroup and must not wrap.
+constexpr uint64_t BLOCK_MAX_ITEMS = 0xffffffffull - (1ull << 26);
+
 // arena bytes of the largest pattern (symbol map + PM rows); rows: distinct symbols + 1
 // of the largest 8-bit pattern when known, 0 for the alphabet bound
 uint64_t block_arena_bytes(int char_width, uint32_t max_m, uint32_t rows = 0);
