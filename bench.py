@@ -500,6 +500,87 @@ def workload_latency(args, emit=True):
     return line
 
 
+def workload_positional(args, emit=True):
+    """Hamming (padded), Prefix and Postfix cdist on the headline shape: 4096 x 65536 uint8 printable
+    ASCII strings of length U{1..64}, CachedHamming distance / CachedPrefix / CachedPostfix similarity,
+    no cutoff.  The kernels write 2.15 GB of results per step, so the bound is HBM."""
+    import rapidfuzz_amd as rfa
+    from rapidfuzz_amd import hip
+
+    rng = np.random.default_rng(SEED)
+    qc, qo, ql = gen(rng, NQ)
+    cc, co, cl = gen(rng, NC)
+    ctx = rfa.Context(0)
+    qs = rfa.Strings(ctx, chars=qc, offsets=qo, width=1)
+    cs = rfa.Strings(ctx, chars=cc, offsets=co, width=1)
+    out = hip.DeviceBuffer(NQ * NC * 8)
+    pairs = float(NQ) * float(NC)
+    res = {}
+    for name, metric, kind in (("hamming", rfa.HAMMING, rfa.DISTANCE), ("prefix", rfa.PREFIX, rfa.SIMILARITY),
+                               ("postfix", rfa.POSTFIX, rfa.SIMILARITY)):
+        sc = rfa.Scorer(ctx, metric, qs)
+        for _ in range(max(1, args.warmup)):
+            sc.cdist(cs, kind, out_device_ptr=out.ptr.value)
+        ctx.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        steps = max(1, args.steps)
+        e0.record(ctx.stream)
+        for _ in range(steps):
+            sc.cdist(cs, kind, out_device_ptr=out.ptr.value)
+        e1.record(ctx.stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        gbps = NQ * NC * 8 / (ms / 1e3) / 1e9
+        entry = {"ms": round(ms, 3), "gpairs_per_s": round(pairs / (ms / 1e3) / 1e9, 2),
+                 "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": 8000.0, "unit": "GB/s",
+                              "frac": round(gbps / 8000.0, 4), "traffic": None,
+                              "bytes_model": "the 8-byte results (2.15 GB); strings are L2-resident"}}
+        print(f"# positional {name}: {ms:.3f} ms/step, {entry['gpairs_per_s']} Gpairs/s, {gbps:.0f} GB/s of results",
+              file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle_py
+            if oracle_py.reference_available():
+                ref = oracle_py.Reference()
+                cores = _cores()
+                nq = 16
+                while True:
+                    idx = np.linspace(0, NQ - 1, nq).astype(np.int64)
+                    sq, so = pack_u8([qc[int(qo[i]):int(qo[i + 1])] for i in idx])
+                    t0 = time.perf_counter()
+                    r = ref.cdist_u8_mt(metric, kind, sq, so, cc, co, threads=cores)
+                    dt = time.perf_counter() - t0
+                    if dt > 2 or nq >= NQ:
+                        break
+                    nq = min(NQ, nq * 4)
+                rows = np.zeros(NC, np.uint64)
+                bad = 0
+                for k_, i in enumerate(idx):
+                    out.to_host(rows, offset=int(i) * NC * 8)
+                    bad += int(np.count_nonzero(rows != r[k_]))
+                entry["cpu_baseline"] = {"value": round(nq * NC / dt / 1e9, 4), "unit": "Gpairs/s", "cores": cores,
+                                         "kind": "reference",
+                                         "sample": f"{nq} of {NQ} queries x all {NC} choices, Cached{name.title()} "
+                                                   f"({dt:.1f} s)"}
+                entry["check"] = "bit-exact vs reference on the CPU sample" if bad == 0 else f"{bad} MISMATCHES"
+        sc.close()
+        res[name] = entry
+    line = {"metric": "Hamming / Prefix / Postfix cdist 4096 x 65536, Gpairs/s", "value": res["hamming"]["gpairs_per_s"],
+            "unit": "Gpairs/s", "n_gpus": 1, "steps": max(1, args.steps), "warmup": max(1, args.warmup),
+            "ms_per_step": res["hamming"]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u8", "data": "synthetic",
+            "config": {"workload": f"cdist {NQ} x {NC} uint8 printable, len {LMIN}-{LMAX}; Hamming distance, "
+                                   f"Prefix / Postfix similarity, no cutoff"},
+            **res}
+    out.free()
+    for x in (qs, cs):
+        x.close()
+    ctx.close()
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
+
+
 def _mutate_dna(rng, s):
     """Per position: substitution 1% (to another base), insertion 0.5% (a random base before it),
     deletion 0.5%."""
@@ -637,7 +718,8 @@ def workload_copy(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="cdist", choices=["cdist", "uneven", "serve", "latency", "dna", "copy"],
+    ap.add_argument("--workload", default="cdist",
+                    choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "copy"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
                          "uneven: 1M independent pairs; serve: the serving leg alone; latency: 1 x 10,000 "
                          "Levenshtein; dna: 256 x 16,384 long sequences with band cutoffs; copy: HBM counter "
@@ -668,6 +750,8 @@ def main():
         return workload_latency(args)
     if args.workload == "dna":
         return workload_dna(args)
+    if args.workload == "positional":
+        return workload_positional(args)
 
     import rapidfuzz_amd as rfa
     from rapidfuzz_amd import hip
@@ -817,6 +901,9 @@ def main():
         dna = workload_dna(args, emit=False)
         res["configs"]["dna_256x16384"] = {**{k: dna.get(k) for k in keep}, "cutoff_5pct": dna["cutoff_5pct"],
                                            "cutoff_20pct": dna["cutoff_20pct"], "config": dna["config"]["workload"]}
+        pos = workload_positional(args, emit=False)
+        res["configs"]["positional_4096x65536"] = {k: pos[k] for k in ("metric", "value", "unit", "ms_per_step",
+                                                                        "hamming", "prefix", "postfix")}
         une = workload_uneven(args, emit=False)
         res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
                                              "speedup_vs_plain": une["speedup_vs_plain"],

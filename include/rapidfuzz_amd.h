@@ -49,7 +49,10 @@ typedef enum {
     RFA_OSA = 3,          /* CachedOSA / experimental::MultiOSA */
     RFA_JARO = 4,         /* CachedJaro / experimental::MultiJaro */
     RFA_JARO_WINKLER = 5, /* CachedJaroWinkler / experimental::MultiJaroWinkler */
-    RFA_RATIO = 6         /* fuzz::CachedRatio / fuzz::experimental::MultiRatio */
+    RFA_RATIO = 6,        /* fuzz::CachedRatio / fuzz::experimental::MultiRatio */
+    RFA_HAMMING = 7,      /* CachedHamming (Hamming.hpp:131-161) with pad = true; Cached semantics only */
+    RFA_PREFIX = 8,       /* CachedPrefix (Prefix.hpp:65-94); Cached semantics only */
+    RFA_POSTFIX = 9       /* CachedPostfix (Postfix.hpp:65-94); Cached semantics only */
 } rfa_metric;
 
 /* Scorer method (details/distance.hpp:211-301, 326-356). */

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "librf_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "librfref.so")
 
-LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO = range(7)
+LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX = range(10)
 DISTANCE, SIMILARITY, NORM_DISTANCE, NORM_SIMILARITY = range(4)
 INT64_MAX = (1 << 63) - 1
 
@@ -56,7 +56,7 @@ def _p(a, t):
 
 
 def is_int_result(metric: int, kind: int) -> bool:
-    return metric in (LEVENSHTEIN, INDEL, LCSSEQ, OSA) and kind in (DISTANCE, SIMILARITY)
+    return metric in (LEVENSHTEIN, INDEL, LCSSEQ, OSA, HAMMING, PREFIX, POSTFIX) and kind in (DISTANCE, SIMILARITY)
 
 
 class Oracle:

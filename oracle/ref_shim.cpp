@@ -25,7 +25,7 @@
 
 namespace {
 
-enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6 };
+enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9 };
 enum { DIST = 0, SIM = 1, NDIST = 2, NSIM = 3 };
 
 thread_local std::string g_err;
@@ -76,6 +76,9 @@ struct AnyCached {
     rapidfuzz::CachedJaro<CharT>* jaro = nullptr;
     rapidfuzz::CachedJaroWinkler<CharT>* jw = nullptr;
     rapidfuzz::fuzz::CachedRatio<CharT>* ratio = nullptr;
+    rapidfuzz::CachedHamming<CharT>* ham = nullptr;
+    rapidfuzz::CachedPrefix<CharT>* pre = nullptr;
+    rapidfuzz::CachedPostfix<CharT>* post = nullptr;
 
     AnyCached(int m, std::vector<CharT> q, rapidfuzz::LevenshteinWeightTable w_, double pw_)
         : metric(m), s1(std::move(q)), w(w_), pw(pw_)
@@ -88,11 +91,17 @@ struct AnyCached {
         case JARO: jaro = new rapidfuzz::CachedJaro<CharT>(s1); break;
         case JW: jw = new rapidfuzz::CachedJaroWinkler<CharT>(s1, pw); break;
         case RATIO: ratio = new rapidfuzz::fuzz::CachedRatio<CharT>(s1); break;
+        case HAMMING: ham = new rapidfuzz::CachedHamming<CharT>(s1, true); break;
+        case PREFIX: pre = new rapidfuzz::CachedPrefix<CharT>(s1); break;
+        case POSTFIX: post = new rapidfuzz::CachedPostfix<CharT>(s1); break;
         default: throw std::invalid_argument("unknown metric");
         }
     }
     ~AnyCached()
     {
+        delete ham;
+        delete pre;
+        delete post;
         delete lev;
         delete indel;
         delete lcs;
@@ -109,6 +118,9 @@ struct AnyCached {
         case INDEL: return run_int_metric(*indel, kind, b, e, cu, cd, ou, od);
         case LCS: return run_int_metric(*lcs, kind, b, e, cu, cd, ou, od);
         case OSA: return run_int_metric(*osa, kind, b, e, cu, cd, ou, od);
+        case HAMMING: return run_int_metric(*ham, kind, b, e, cu, cd, ou, od);
+        case PREFIX: return run_int_metric(*pre, kind, b, e, cu, cd, ou, od);
+        case POSTFIX: return run_int_metric(*post, kind, b, e, cu, cd, ou, od);
         case JARO: return run_fp_metric(*jaro, kind, b, e, cd, od);
         case JW: return run_fp_metric(*jw, kind, b, e, cd, od);
         case RATIO:

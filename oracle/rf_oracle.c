@@ -234,6 +234,49 @@ static uint64_t lev_maximum(uint64_t l1, uint64_t l2, uint64_t ins, uint64_t del
     return m;
 }
 
+/* Hamming with padding (Hamming_impl.hpp:23-35; the naive restatement in
+ * rapidfuzz_reference/Hamming.hpp:10-23 without its equal-length requirement):
+ * max(l1, l2) minus the equal positions of the common length. */
+uint64_t orc_hamming_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2)
+{
+    size_t n = l1 < l2 ? l1 : l2;
+    uint64_t dist = l1 > l2 ? l1 : l2;
+    for (size_t i = 0; i < n; ++i) dist -= (s1[i] == s2[i]);
+    return dist;
+}
+
+/* Prefix::_similarity (Prefix_impl.hpp:23-27): length of the common prefix. */
+uint64_t orc_prefix_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2)
+{
+    size_t i = 0;
+    while (i < l1 && i < l2 && s1[i] == s2[i]) ++i;
+    return i;
+}
+
+/* Postfix::_similarity (Postfix_impl.hpp:23-27): length of the common suffix. */
+uint64_t orc_postfix_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2)
+{
+    size_t i = 0;
+    while (i < l1 && i < l2 && s1[l1 - 1 - i] == s2[l2 - 1 - i]) ++i;
+    return i;
+}
+
+/* CachedPrefix / CachedPostfix::_similarity: the length when it reaches the
+ * cutoff, else 0 (Prefix.hpp:84-88, Postfix.hpp:84-88). */
+static uint64_t affix_cached_similarity(const Pair* p, uint64_t c)
+{
+    uint64_t sim = p->metric == ORC_PREFIX ? orc_prefix_raw(p->s1, p->l1, p->s2, p->l2)
+                                           : orc_postfix_raw(p->s1, p->l1, p->s2, p->l2);
+    return sim >= c ? sim : 0;
+}
+
+/* CachedHamming::_distance (Hamming.hpp:153-157, pad = true). */
+static uint64_t hamming_cached_distance(const Pair* p, uint64_t c)
+{
+    uint64_t d = orc_hamming_raw(p->s1, p->l1, p->s2, p->l2);
+    return d <= c ? d : c + 1;
+}
+
 /* CachedLCSseq::_similarity -> lcs_seq_similarity (LCSseq_impl.hpp:375-403):
  * the exact LCS when it reaches the cutoff, else 0. */
 static uint64_t lcs_cached_similarity(const Pair* p, uint64_t c)
@@ -307,9 +350,20 @@ static uint64_t int_maximum(const Pair* p)
     case ORC_LEVENSHTEIN: return lev_maximum(p->l1, p->l2, p->ins, p->del, p->rep);
     case ORC_INDEL: return (uint64_t)p->l1 + (uint64_t)p->l2;
     case ORC_LCSSEQ:
-    case ORC_OSA: return u64max(p->l1, p->l2);
+    case ORC_OSA:
+    case ORC_HAMMING:
+    case ORC_PREFIX:
+    case ORC_POSTFIX: return u64max(p->l1, p->l2);
     }
     return 0;
+}
+
+/* similarity-type integer metrics (CachedSimilarityBase) */
+static int is_sim_metric(int metric) { return metric == ORC_LCSSEQ || metric == ORC_PREFIX || metric == ORC_POSTFIX; }
+
+static uint64_t sim_cached_similarity(const Pair* p, uint64_t c)
+{
+    return p->metric == ORC_LCSSEQ ? lcs_cached_similarity(p, c) : affix_cached_similarity(p, c);
 }
 
 /* derived._distance for distance-type integer metrics */
@@ -319,29 +373,30 @@ static uint64_t int_distance_core(const Pair* p, uint64_t c)
     case ORC_LEVENSHTEIN: return lev_cached_distance(p, c);
     case ORC_INDEL: return indel_cached_distance(p, c);
     case ORC_OSA: return osa_cached_distance(p, c);
+    case ORC_HAMMING: return hamming_cached_distance(p, c);
     }
     return 0;
 }
 
-/* CachedSimilarityBase::_distance for LCSseq (details/distance.hpp:359-369). */
+/* CachedSimilarityBase::_distance for LCSseq, Prefix, Postfix (details/distance.hpp:359-369). */
 static uint64_t lcs_cached_distance(const Pair* p, uint64_t c)
 {
     uint64_t maximum = int_maximum(p);
     uint64_t cs = (maximum > c) ? maximum - c : 0;
-    uint64_t sim = lcs_cached_similarity(p, cs);
+    uint64_t sim = sim_cached_similarity(p, cs);
     uint64_t dist = maximum - sim;
     return (dist <= c) ? dist : c + 1;
 }
 
 static uint64_t int_api_distance(const Pair* p, uint64_t c)
 {
-    if (p->metric == ORC_LCSSEQ) return lcs_cached_distance(p, c);
+    if (is_sim_metric(p->metric)) return lcs_cached_distance(p, c);
     return int_distance_core(p, c);
 }
 
 static uint64_t int_api_similarity(const Pair* p, uint64_t c)
 {
-    if (p->metric == ORC_LCSSEQ) return lcs_cached_similarity(p, c);
+    if (is_sim_metric(p->metric)) return sim_cached_similarity(p, c);
     /* CachedDistanceBase::_similarity (details/distance.hpp:306-317) */
     uint64_t maximum = int_maximum(p);
     if (c > maximum) return 0;
@@ -416,6 +471,9 @@ int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const 
     case ORC_INDEL:
     case ORC_LCSSEQ:
     case ORC_OSA:
+    case ORC_HAMMING:
+    case ORC_PREFIX:
+    case ORC_POSTFIX:
         if (metric != ORC_LEVENSHTEIN) p.ins = p.del = 1, p.rep = (metric == ORC_INDEL) ? 2 : 1;
         switch (kind) {
         case ORC_DISTANCE: *out_u = int_api_distance(&p, cutoff_u); return 0;

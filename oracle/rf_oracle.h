@@ -33,7 +33,10 @@ enum {
     ORC_OSA = 3,
     ORC_JARO = 4,
     ORC_JARO_WINKLER = 5,
-    ORC_RATIO = 6
+    ORC_RATIO = 6,
+    ORC_HAMMING = 7, /* CachedHamming with pad = true */
+    ORC_PREFIX = 8,
+    ORC_POSTFIX = 9
 };
 
 enum { ORC_DISTANCE = 0, ORC_SIMILARITY = 1, ORC_NORM_DISTANCE = 2, ORC_NORM_SIMILARITY = 3 };
@@ -43,6 +46,10 @@ uint64_t orc_levenshtein_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, 
                              uint64_t ins, uint64_t del, uint64_t rep);
 uint64_t orc_lcs_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 uint64_t orc_osa_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
+/* positional metrics: Hamming distance with padding, common prefix / suffix length */
+uint64_t orc_hamming_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
+uint64_t orc_prefix_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
+uint64_t orc_postfix_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 void orc_jaro_raw(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, uint64_t* common,
                   uint64_t* transpositions);
 double orc_jaro_similarity(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, double cutoff);

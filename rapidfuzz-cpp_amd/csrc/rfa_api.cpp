@@ -51,6 +51,9 @@ int algo_for(int metric, const rfa_weights& w)
     case RFA_LCSSEQ:
     case RFA_RATIO: return ALGO_LCS;
     case RFA_OSA: return ALGO_OSA;
+    case RFA_HAMMING:
+    case RFA_PREFIX:
+    case RFA_POSTFIX: return ALGO_POSIT;
     case RFA_JARO:
     case RFA_JARO_WINKLER: return ALGO_JARO;
     }
@@ -64,7 +67,8 @@ bool multi_levenshtein_weights(const rfa_weights& w)
 
 bool is_int_kind(int metric, int kind)
 {
-    return (metric == RFA_LEVENSHTEIN || metric == RFA_INDEL || metric == RFA_LCSSEQ || metric == RFA_OSA) &&
+    return (metric == RFA_LEVENSHTEIN || metric == RFA_INDEL || metric == RFA_LCSSEQ || metric == RFA_OSA ||
+            metric == RFA_HAMMING || metric == RFA_PREFIX || metric == RFA_POSTFIX) &&
            (kind == RFA_DISTANCE || kind == RFA_SIMILARITY);
 }
 
@@ -291,6 +295,8 @@ void validate_kind(int metric, int kind, int semantics)
     if (semantics != RFA_CACHED && semantics != RFA_MULTI) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown semantics");
     if (metric == RFA_RATIO && kind != RFA_SIMILARITY)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "ratio only provides similarity");
+    if (semantics == RFA_MULTI && (metric == RFA_HAMMING || metric == RFA_PREFIX || metric == RFA_POSTFIX))
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "Hamming, Prefix and Postfix have no Multi* scorer");
 }
 
 // Block-kernel launch over a cross product (queries x choices) or explicit items.
@@ -375,6 +381,26 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
     void* dout = out_on_device ? out : ctx->out_tmp.get(bytes);
     const ScoreSpec spec = make_spec(sc, kind, semantics, cu, cd, algo);
     const int width = sc->q->width;
+    if (algo == ALGO_POSIT) {
+        // positional metrics: one kernel for every query length
+        PositArgs pa{};
+        pa.qchars = sc->q->d_chars;
+        pa.qoff = sc->q->d_off;
+        pa.nq = (uint32_t)nq;
+        pa.cchars = ch->d_chars;
+        pa.coff = ch->d_off;
+        pa.nc = (uint32_t)nc;
+        pa.spec = spec;
+        pa.out = dout;
+        pa.out_stride = nc;
+        pa.qcut = d_qcut;
+        HIP_CHECK(launch_posit(width, pa, (uint32_t)ctx->num_cus, s));
+        if (!out_on_device) {
+            HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
+        return;
+    }
     // a few queries against a set: one pair per lane (choices across lanes)
     if (cdist_few_queries(sc, ch, spec, algo, dout, d_qcut)) {
         if (!out_on_device) {
@@ -810,7 +836,7 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
 {
     if (!ctx || !queries || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, queries or out is NULL");
     return guarded([&] {
-        if (metric < RFA_LEVENSHTEIN || metric > RFA_RATIO) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
+        if (metric < RFA_LEVENSHTEIN || metric > RFA_POSTFIX) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
         if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};
         // weights MultiLevenshtein accepts are kept even when Cached* scoring cannot use them

@@ -126,6 +126,24 @@ uint32_t seg_symbol_bits(int char_width, uint64_t max_symbol);
 hipError_t launch_seg(int char_width, uint32_t nb, const SegArgs& a, uint32_t num_cus, uint32_t max_blocks,
                       hipStream_t s);
 
+// Positional metrics (rfa_posit.hip): Hamming / Prefix / Postfix cdist, query tiles
+// of 64 against runs of 8 consecutive choices; out[q * out_stride + c].
+struct PositArgs {
+    const void* qchars;
+    const uint64_t* qoff;
+    uint32_t nq;
+    const void* cchars;
+    const uint64_t* coff;
+    uint32_t nc;
+    ScoreSpec spec;
+    void* out;
+    uint64_t out_stride;
+    const uint64_t* qcut; // integer cutoff per query (nullptr: spec.cutoff_u)
+    uint32_t q_base;      // first query of the launch (set by launch_posit)
+};
+
+hipError_t launch_posit(int char_width, const PositArgs& a, uint32_t num_cus, hipStream_t s);
+
 size_t plan_temp_bytes(uint64_t n);
 // Counting sort of the pairs into kernel ranges (items vals_out[k] = pair | pattern-is-B << 31).
 // counters[0]: block queue; [1]: end of the block range; [2], [3]: ends of the OSA

@@ -62,6 +62,9 @@ __device__ __forceinline__ uint64_t int_maximum(const IntPair& p)
     }
 }
 
+// similarity-type integer metrics (CachedSimilarityBase): the raw value is the similarity
+__device__ __forceinline__ bool is_sim_int(int32_t m) { return m == RFA_LCSSEQ || m == RFA_PREFIX || m == RFA_POSTFIX; }
+
 // exact Indel distance from the LCS length
 __device__ __forceinline__ uint64_t indel_from_lcs(const IntPair& p) { return p.l1 + p.l2 - 2 * p.raw; }
 
@@ -92,7 +95,7 @@ __device__ __forceinline__ uint64_t int_distance_core(const IntPair& p, uint64_t
     return exact <= c ? exact : c + 1;
 }
 
-// LCSseq is a similarity metric: CachedSimilarityBase::_distance
+// LCSseq, Prefix and Postfix are similarity metrics: CachedSimilarityBase::_distance
 __device__ __forceinline__ uint64_t lcs_distance(const IntPair& p, uint64_t c)
 {
     uint64_t maximum = int_maximum(p);
@@ -104,13 +107,13 @@ __device__ __forceinline__ uint64_t lcs_distance(const IntPair& p, uint64_t c)
 
 __device__ __forceinline__ uint64_t int_api_distance(const IntPair& p, uint64_t c)
 {
-    if (p.s->metric == RFA_LCSSEQ) return lcs_distance(p, c);
+    if (is_sim_int(p.s->metric)) return lcs_distance(p, c);
     return int_distance_core(p, c);
 }
 
 __device__ __forceinline__ uint64_t int_api_similarity(const IntPair& p, uint64_t c)
 {
-    if (p.s->metric == RFA_LCSSEQ) return p.raw >= c ? p.raw : 0;
+    if (is_sim_int(p.s->metric)) return p.raw >= c ? p.raw : 0;
     uint64_t maximum = int_maximum(p);
     if (c > maximum) return 0;
     uint64_t dist = int_distance_core(p, maximum - c);

@@ -857,7 +857,7 @@ rfa_status rfa_server_create(rfa_context* ctx, const rfa_strings* choices, const
 {
     if (!ctx || !choices || !cfg || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, choices, cfg or out is NULL");
     return guarded([&] {
-        if (cfg->metric < RFA_LEVENSHTEIN || cfg->metric > RFA_RATIO)
+        if (cfg->metric < RFA_LEVENSHTEIN || cfg->metric > RFA_POSTFIX)
             throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         validate_topk(cfg->metric, cfg->kind);
         if (cfg->k == 0) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "k must be positive");

@@ -6,7 +6,9 @@
 //   CachedLevenshtein (distance/Levenshtein.hpp:421-492), CachedIndel
 //   (distance/Indel.hpp:148-188), CachedLCSseq (distance/LCSseq.hpp:197-223),
 //   CachedOSA (distance/OSA.hpp:231-275), CachedJaro (distance/Jaro.hpp:195-230),
-//   CachedJaroWinkler (distance/JaroWinkler.hpp:168-209), fuzz::CachedRatio
+//   CachedJaroWinkler (distance/JaroWinkler.hpp:168-209), CachedHamming
+//   (distance/Hamming.hpp:131-161), CachedPrefix (distance/Prefix.hpp:65-94),
+//   CachedPostfix (distance/Postfix.hpp:65-94), fuzz::CachedRatio
 //   (fuzz.hpp:98-117), experimental::Multi{Levenshtein,Indel,LCSseq,OSA,Jaro,
 //   JaroWinkler}<MaxLen> and fuzz::experimental::MultiRatio<MaxLen>
 //   (e.g. Levenshtein.hpp:297-419, details/distance.hpp:390-582).
@@ -227,6 +229,7 @@ public:
     {}
     size_t size() const { return p_.count(); }
     int width() const { return p_.width; }
+    size_t length(size_t i) const { return p_.off[i + 1] - p_.off[i]; }
 
     // the set at a code-unit width (uploaded on first use)
     rfa_strings* at_width(int w) const
@@ -263,11 +266,19 @@ protected:
         set_ = std::make_shared<ScorerSet>(Metric, std::move(q), w, pw);
     }
 
+    // Hamming without padding: unequal lengths are the reference's error (Hamming_impl.hpp:27)
+    void check_length(size_t len2) const
+    {
+        if (equal_len_ && len2 != len_) throw std::invalid_argument("Sequences are not the same length.");
+    }
+
     // scores of the query against `choices`; out has choices.size() entries
     template <typename Out>
     void run(const Choices& choices, int kind, Out* out, uint64_t cu, double cd, int semantics = RFA_CACHED) const
     {
         if (!choices.size()) return;
+        if (equal_len_)
+            for (size_t i = 0; i < choices.size(); ++i) check_length(choices.length(i));
         const int w = std::max(set_->query_width(), choices.width());
         check(rfa_cdist(set_->get(w), choices.at_width(w), kind, semantics, cu, cd, out, 0));
     }
@@ -277,6 +288,7 @@ protected:
     {
         Packed p;
         p.add(first, last);
+        check_length(p.off[1]);
         const int w = std::max(set_->query_width(), p.width);
         DeviceStrings s(context(), p, w);
         Out v{};
@@ -288,6 +300,7 @@ protected:
 
     std::shared_ptr<ScorerSet> set_;
     size_t len_ = 0;
+    bool equal_len_ = false;
 
 public:
     // ---- the reference's methods (details/distance.hpp:211-301, 326-356) ----
@@ -470,6 +483,47 @@ struct CachedJaroWinkler : detail::CachedBase<RFA_JARO_WINKLER, double, 0, 1> {
     }
 };
 
+// Hamming.hpp:131-161 (pad = false: unequal lengths throw std::invalid_argument)
+template <typename CharT1>
+struct CachedHamming : detail::CachedBase<RFA_HAMMING, size_t, 0, detail::kIntMax> {
+    template <typename Sentence1>
+    explicit CachedHamming(const Sentence1& s1_, bool pad_ = true)
+        : CachedHamming(detail::to_begin(s1_), detail::to_end(s1_), pad_)
+    {}
+    template <typename InputIt1>
+    CachedHamming(InputIt1 first1, InputIt1 last1, bool pad_ = true)
+    {
+        this->init(first1, last1, {1, 1, 1}, 0.1);
+        this->equal_len_ = !pad_;
+    }
+};
+
+// Prefix.hpp:65-94
+template <typename CharT1>
+struct CachedPrefix : detail::CachedBase<RFA_PREFIX, size_t, 0, detail::kIntMax> {
+    template <typename Sentence1>
+    explicit CachedPrefix(const Sentence1& s1_) : CachedPrefix(detail::to_begin(s1_), detail::to_end(s1_))
+    {}
+    template <typename InputIt1>
+    CachedPrefix(InputIt1 first1, InputIt1 last1)
+    {
+        this->init(first1, last1, {1, 1, 1}, 0.1);
+    }
+};
+
+// Postfix.hpp:65-94
+template <typename CharT1>
+struct CachedPostfix : detail::CachedBase<RFA_POSTFIX, size_t, 0, detail::kIntMax> {
+    template <typename Sentence1>
+    explicit CachedPostfix(const Sentence1& s1_) : CachedPostfix(detail::to_begin(s1_), detail::to_end(s1_))
+    {}
+    template <typename InputIt1>
+    CachedPostfix(InputIt1 first1, InputIt1 last1)
+    {
+        this->init(first1, last1, {1, 1, 1}, 0.1);
+    }
+};
+
 // Deduction guides (e.g. Levenshtein.hpp:484-492)
 template <typename Sentence1>
 explicit CachedLevenshtein(const Sentence1& s1_, LevenshteinWeightTable aWeights = {1, 1, 1})
@@ -485,6 +539,12 @@ explicit CachedJaro(const Sentence1& s1_) -> CachedJaro<detail::sentence_char_t<
 template <typename Sentence1>
 explicit CachedJaroWinkler(const Sentence1& s1_, double _prefix_weight = 0.1)
     -> CachedJaroWinkler<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+explicit CachedHamming(const Sentence1& s1_, bool pad_ = true) -> CachedHamming<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+explicit CachedPrefix(const Sentence1& s1_) -> CachedPrefix<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+explicit CachedPostfix(const Sentence1& s1_) -> CachedPostfix<detail::sentence_char_t<Sentence1>>;
 
 namespace fuzz {
 

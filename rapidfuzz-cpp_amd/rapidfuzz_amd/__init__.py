@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "librapidfuzz_amd.so")
 
-LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO = range(7)
+LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX = range(10)
 DISTANCE, SIMILARITY, NORMALIZED_DISTANCE, NORMALIZED_SIMILARITY = range(4)
 CACHED, MULTI = 0, 1
 SCHEDULE_BUCKETED, SCHEDULE_PLAIN = 0, 1
@@ -117,7 +117,7 @@ def _check(status):
 
 
 def is_int_result(metric, kind):
-    return metric in (LEVENSHTEIN, INDEL, LCSSEQ, OSA) and kind in (DISTANCE, SIMILARITY)
+    return metric in (LEVENSHTEIN, INDEL, LCSSEQ, OSA, HAMMING, PREFIX, POSTFIX) and kind in (DISTANCE, SIMILARITY)
 
 
 def default_cutoffs(metric, kind):

@@ -2,7 +2,7 @@
 //
 // Known answers from the reference's own tests (tests-Levenshtein.cpp:144-179,
 // tests-Indel.cpp:204-215, tests-OSA.cpp:62-97, tests-JaroWinkler.cpp:181-182,
-// tests-fuzz.cpp:63-76), batch results equal to one-pair results, Multi*
+// tests-fuzz.cpp:63-76, tests-Hamming.cpp:99-108), batch results equal to one-pair results, Multi*
 // padding and error checks, and extract_topk.  Exit status: 0 pass, 1 fail,
 // 77 no GPU device.
 #include <cmath>
@@ -164,6 +164,45 @@ int main()
             std::vector<double> s(mr.result_count());
             mr.similarity(s.data(), s.size(), std::string("the wonderful new york mets"));
             CHECK(near(s[0], 65.0));
+        }
+        {
+            // tests-Hamming.cpp:99-108
+            rfa::CachedHamming<char> h(std::string("aaaa"));
+            CHECK(h.distance(std::string("aaaa")) == 0);
+            CHECK(h.distance(std::string("abaa")) == 1);
+            CHECK(h.distance(std::string("aaba")) == 1);
+            CHECK(rfa::CachedHamming<char>(std::string("abaa")).distance(std::string("aaba")) == 2);
+            CHECK(h.distance(std::string("aaaaa")) == 1);
+            CHECK(rfa::CachedHamming<char>(std::string("aaaaa")).distance(std::string("aaaa")) == 1);
+            CHECK(h.similarity(std::string("abaa")) == 3);
+            CHECK(near(h.normalized_distance(std::string("abab")), 0.5));
+            rfa::CachedHamming<char> strict(std::string("aaaa"), false);
+            CHECK(strict.distance(std::string("abaa")) == 1);
+            CHECK(throws_invalid([&] { (void)strict.distance(std::string("aaaaa")); }));
+            rfa::CachedPrefix<char> pre(std::string("prefix_sample"));
+            CHECK(pre.similarity(std::string("prefix_other")) == 7);
+            CHECK(pre.similarity(std::string("prefix_other"), 8) == 0);
+            CHECK(pre.distance(std::string("prefix_other")) == 6);
+            rfa::CachedPostfix<char> post(std::string("sample_postfix"));
+            CHECK(post.similarity(std::string("other_postfix")) == 8);
+            CHECK(near(post.normalized_similarity(std::string("other_postfix")), 8.0 / 14.0));
+            // batch equals one-pair calls
+            std::mt19937 rng(5);
+            std::vector<std::string> cs;
+            for (int i = 0; i < 300; ++i) cs.push_back(random_string(rng, 0, 40));
+            rfa::Choices ch(cs);
+            const std::string q = random_string(rng, 10, 30);
+            rfa::CachedHamming<char> hq(q);
+            rfa::CachedPrefix<char> pq(q);
+            rfa::CachedPostfix<char> sq(q);
+            auto hd = hq.distance_many(ch);
+            auto pd = pq.similarity_many(ch);
+            auto sd = sq.similarity_many(ch);
+            for (size_t i = 0; i < cs.size(); ++i) {
+                CHECK(hd[i] == hq.distance(cs[i]));
+                CHECK(pd[i] == pq.similarity(cs[i]));
+                CHECK(sd[i] == sq.similarity(cs[i]));
+            }
         }
         {
             rfa::experimental::MultiIndel<64> mi(1);

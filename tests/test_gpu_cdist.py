@@ -16,7 +16,7 @@ def _kinds(metric):
 
 def _cutoffs(metric, kind):
     """(cutoff_u, cutoff_d) pairs exercising the reference's cutoff branches."""
-    int_kind = metric in (0, 1, 2, 3) and kind in (0, 1)
+    int_kind = metric in (0, 1, 2, 3, 7, 8, 9) and kind in (0, 1)
     if int_kind:
         return [(None, None), (0, None), (1, None), (3, None), (7, None), (20, None)]
     if metric == 6:
@@ -268,3 +268,57 @@ def test_per_query_cutoffs(ctx, oracle, metric, shape):
             exp = np.stack([oracle.cdist(metric, kind, [queries[i]], choices, cutoff_u=int(cuts[i]),
                                          multi=bool(semantics))[0] for i in range(len(queries))])
             assert np.array_equal(got, exp), f"kind {kind} semantics {semantics}: {np.count_nonzero(got != exp)} differ"
+
+
+@pytest.mark.parametrize("metric", [7, 8, 9])
+@pytest.mark.parametrize("width", [1, 2, 4])
+def test_positional_metrics(ctx, oracle, metric, width):
+    """Hamming (padded), Prefix and Postfix: every kind and cutoff, queries and choices of 0-150
+    units sharing prefixes and suffixes, 8/16/32-bit units, per-query cutoffs."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(400 + metric * 3 + width)
+    base = {1: 97, 2: 50000, 4: 2000000}[width]
+    stems = [rng.integers(0, 3, size=int(rng.integers(0, 150))).astype(np.uint32) + base for _ in range(8)]
+
+    def variant(s):
+        t = s.copy()
+        if len(t) and rng.random() < 0.7:
+            t[rng.integers(0, len(t), size=int(rng.integers(1, 4)))] = base + 3
+        if rng.random() < 0.5:
+            t = np.concatenate([t, rng.integers(0, 3, size=int(rng.integers(0, 9))).astype(np.uint32) + base])
+        if rng.random() < 0.3:
+            t = np.concatenate([rng.integers(0, 3, size=int(rng.integers(1, 5))).astype(np.uint32) + base, t])
+        return t
+
+    queries = [variant(stems[i % 8]) for i in range(70)] + [np.zeros(0, np.uint32)]
+    choices = [variant(stems[i % 8]) for i in range(150)] + [np.zeros(0, np.uint32), stems[0][:1]]
+    sc = rfa.Scorer(ctx, metric, ctx.strings(queries, width=width))
+    cs = ctx.strings(choices, width=width)
+    for kind in (0, 1, 2, 3):
+        cuts = [(None, None)]
+        if kind in (0, 1):
+            cuts += [(c, None) for c in (0, 1, 3, 10, 60)]
+        else:
+            cuts += [(None, c) for c in (0.0, 0.2, 0.5, 0.8, 1.0)]
+        for cu, cd in cuts:
+            kw = {}
+            if cu is not None:
+                kw["cutoff_u"] = cu
+            if cd is not None:
+                kw["cutoff_d"] = cd
+            got = sc.cdist(cs, kind, **kw)
+            exp = oracle.cdist(metric, kind, queries, choices, **kw)
+            if got.dtype == np.float64:
+                same = np.array_equal(got.view(np.uint64), exp.view(np.uint64))
+            else:
+                same = np.array_equal(got, exp)
+            assert same, f"kind {kind} cutoff {cu}/{cd}: {np.count_nonzero(got != exp)} differ"
+    cuts = rng.integers(0, 40, size=len(queries)).astype(np.uint64)
+    for kind in (0, 1):
+        got = sc.cdist(cs, kind, cutoffs_u=cuts)
+        exp = np.stack([oracle.cdist(metric, kind, [queries[i]], choices, cutoff_u=int(cuts[i]))[0]
+                        for i in range(len(queries))])
+        assert np.array_equal(got, exp), f"per-query cutoffs kind {kind}"
+    with pytest.raises(rfa.InvalidArgument):
+        sc.cdist(cs, 0, semantics=1)

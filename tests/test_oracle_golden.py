@@ -10,7 +10,7 @@ DATA = G.load()
 def _cutoffs(c):
     kw = {}
     if "cutoff" in c:
-        if isinstance(c["expected"], float) or c["kind"] >= 2 or c["metric"] >= 4:
+        if isinstance(c["expected"], float) or c["kind"] >= 2 or c["metric"] in (4, 5, 6):
             kw["cutoff_d"] = c["cutoff"]
         else:
             kw["cutoff_u"] = c["cutoff"]

@@ -12,7 +12,8 @@ import pytest
 
 import _gen
 
-METRICS = [0, 1, 2, 3, 4, 5, 6]
+METRICS = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9]
+MULTI_METRICS = [0, 1, 2, 3, 4, 5, 6]  # the reference has no Multi* for Hamming / Prefix / Postfix
 
 
 def _kinds(metric):
@@ -20,7 +21,7 @@ def _kinds(metric):
 
 
 def _cutoffs(metric, kind):
-    if metric in (0, 1, 2, 3) and kind in (0, 1):
+    if metric in (0, 1, 2, 3, 7, 8, 9) and kind in (0, 1):
         return [(None, None), (0, None), (1, None), (2, None), (5, None), (17, None)]
     if metric == 6:
         return [(None, 0.0), (None, 50.0), (None, 70.0), (None, 80.0), (None, 100.0)]
@@ -68,7 +69,7 @@ def test_cached(oracle, reference, metric):
                                          f"{len(bad)} differ; q{i} c{j}: oracle {got[i, j]!r} reference {exp[i, j]!r}")
 
 
-@pytest.mark.parametrize("metric", METRICS)
+@pytest.mark.parametrize("metric", MULTI_METRICS)
 @pytest.mark.parametrize("max_len", [8, 16, 32, 64])
 def test_multi(oracle, reference, metric, max_len):
     rng = np.random.default_rng(7 * max_len + metric)
@@ -115,3 +116,18 @@ def test_multi_levenshtein_weights(oracle, reference, weights):
 def test_multi_levenshtein_rejects_weights(oracle):
     with pytest.raises(ValueError):
         oracle.cdist(0, 0, [[97]], [[98]], weights=(1, 1, 3), multi=True)
+
+
+def test_positional_metrics_brute_force(oracle):
+    """Hamming (padded), common prefix and common suffix against their definitions."""
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        a = rng.integers(97, 100, size=int(rng.integers(0, 12))).astype(np.uint32)
+        b = rng.integers(97, 100, size=int(rng.integers(0, 12))).astype(np.uint32)
+        n = min(len(a), len(b))
+        ham = max(len(a), len(b)) - int(np.count_nonzero(a[:n] == b[:n]))
+        pre = next((i for i in range(n) if a[i] != b[i]), n)
+        post = next((i for i in range(n) if a[len(a) - 1 - i] != b[len(b) - 1 - i]), n)
+        assert oracle.score(7, 0, a, b) == ham
+        assert oracle.score(8, 1, a, b) == pre
+        assert oracle.score(9, 1, a, b) == post
