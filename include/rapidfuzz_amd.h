@@ -52,7 +52,9 @@ typedef enum {
     RFA_RATIO = 6,        /* fuzz::CachedRatio / fuzz::experimental::MultiRatio */
     RFA_HAMMING = 7,      /* CachedHamming (Hamming.hpp:131-161) with pad = true; Cached semantics only */
     RFA_PREFIX = 8,       /* CachedPrefix (Prefix.hpp:65-94); Cached semantics only */
-    RFA_POSTFIX = 9       /* CachedPostfix (Postfix.hpp:65-94); Cached semantics only */
+    RFA_POSTFIX = 9,      /* CachedPostfix (Postfix.hpp:65-94); Cached semantics only */
+    RFA_PARTIAL_RATIO = 10 /* fuzz::CachedPartialRatio (fuzz.hpp:169-200, fuzz_impl.hpp:57-279): best ratio of
+                              the shorter string against the longer one's windows; similarity, Cached only */
 } rfa_metric;
 
 /* Scorer method (details/distance.hpp:211-301, 326-356). */

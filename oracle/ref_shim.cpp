@@ -25,7 +25,7 @@
 
 namespace {
 
-enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9 };
+enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10 };
 enum { DIST = 0, SIM = 1, NDIST = 2, NSIM = 3 };
 
 thread_local std::string g_err;
@@ -79,6 +79,7 @@ struct AnyCached {
     rapidfuzz::CachedHamming<CharT>* ham = nullptr;
     rapidfuzz::CachedPrefix<CharT>* pre = nullptr;
     rapidfuzz::CachedPostfix<CharT>* post = nullptr;
+    rapidfuzz::fuzz::CachedPartialRatio<CharT>* partial = nullptr;
 
     AnyCached(int m, std::vector<CharT> q, rapidfuzz::LevenshteinWeightTable w_, double pw_)
         : metric(m), s1(std::move(q)), w(w_), pw(pw_)
@@ -94,6 +95,7 @@ struct AnyCached {
         case HAMMING: ham = new rapidfuzz::CachedHamming<CharT>(s1, true); break;
         case PREFIX: pre = new rapidfuzz::CachedPrefix<CharT>(s1); break;
         case POSTFIX: post = new rapidfuzz::CachedPostfix<CharT>(s1); break;
+        case PARTIAL: partial = new rapidfuzz::fuzz::CachedPartialRatio<CharT>(s1); break;
         default: throw std::invalid_argument("unknown metric");
         }
     }
@@ -102,6 +104,7 @@ struct AnyCached {
         delete ham;
         delete pre;
         delete post;
+        delete partial;
         delete lev;
         delete indel;
         delete lcs;
@@ -126,6 +129,10 @@ struct AnyCached {
         case RATIO:
             if (kind != SIM) return -1;
             *od = ratio->similarity(b, e, cd);
+            return 1;
+        case PARTIAL:
+            if (kind != SIM) return -1;
+            *od = partial->similarity(b, e, cd);
             return 1;
         }
         return -1;

@@ -461,6 +461,141 @@ static double fp_norm_similarity(const Pair* p, double c)
     return (ns >= c) ? ns : 0.0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* fuzz::partial_ratio (fuzz_impl.hpp:57-279)                                */
+/* ------------------------------------------------------------------------ */
+
+/* CachedRatio(s1).similarity(s2, cutoff) (fuzz_impl.hpp:42-47) */
+static double ratio_similarity(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, double cutoff)
+{
+    Pair p = {ORC_INDEL, s1, s2, l1, l2, 1, 1, 2, 0.0};
+    return int_api_norm_similarity(&p, cutoff / 100) * 100;
+}
+
+/* CachedIndel(s1).distance(s2) with the default (unbounded) cutoff */
+static size_t indel_distance_full(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2)
+{
+    return l1 + l2 - 2 * (size_t)orc_lcs_raw(s1, l1, s2, l2);
+}
+
+/* CharSet::find of s1's units */
+static int unit_in(const uint32_t* s, size_t l, uint32_t c)
+{
+    for (size_t i = 0; i < l; ++i)
+        if (s[i] == c) return 1;
+    return 0;
+}
+
+typedef struct {
+    size_t first, second;
+} Window;
+
+/* fuzz_detail::partial_ratio_impl (fuzz_impl.hpp:68-179): s1 is the needle, len1 <= len2.
+ * The full windows s2[p, p + len1), p < len2 - len1, are searched by bisection of
+ * [0, len2 - len1 - 1] with the reference's bound on the distances between two
+ * evaluated windows; then the prefix and suffix windows. */
+static double partial_ratio_impl(const uint32_t* s1, size_t len1, const uint32_t* s2, size_t len2,
+                                 double score_cutoff)
+{
+    double res = 0;
+    if (len2 > len1) {
+        const size_t maximum = len1 * 2;
+        const double norm_cutoff_sim = normsim_to_normdist(score_cutoff / 100);
+        size_t cutoff_dist = (size_t)ceil((double)maximum * norm_cutoff_sim);
+        size_t best_dist = SIZE_MAX;
+        const size_t nscores = len2 - len1;
+        size_t* scores = (size_t*)malloc(nscores * sizeof(size_t));
+        Window* windows = (Window*)malloc(2 * nscores * sizeof(Window) + sizeof(Window));
+        Window* next = (Window*)malloc(2 * nscores * sizeof(Window) + sizeof(Window));
+        size_t nwin = 1, nnext = 0;
+        for (size_t i = 0; i < nscores; ++i) scores[i] = SIZE_MAX;
+        windows[0].first = 0;
+        windows[0].second = len2 - len1 - 1;
+        int perfect = 0;
+        while (nwin && !perfect) {
+            for (size_t k = 0; k < nwin && !perfect; ++k) {
+                const size_t f = windows[k].first, sc = windows[k].second;
+                if (scores[f] == SIZE_MAX) {
+                    scores[f] = indel_distance_full(s1, len1, s2 + f, len1);
+                    if (scores[f] < cutoff_dist) {
+                        cutoff_dist = best_dist = scores[f];
+                        if (best_dist == 0) perfect = 1;
+                    }
+                }
+                if (perfect) break;
+                if (scores[sc] == SIZE_MAX) {
+                    scores[sc] = indel_distance_full(s1, len1, s2 + sc, len1);
+                    if (scores[sc] < cutoff_dist) {
+                        cutoff_dist = best_dist = scores[sc];
+                        if (best_dist == 0) perfect = 1;
+                    }
+                }
+                if (perfect) break;
+                const size_t cell_diff = sc - f;
+                if (cell_diff == 1) continue;
+                const size_t known_edits = scores[f] > scores[sc] ? scores[f] - scores[sc] : scores[sc] - scores[f];
+                const size_t max_score_improvement = (cell_diff - known_edits / 2) / 2 * 2;
+                const ptrdiff_t min_score =
+                    (ptrdiff_t)(scores[f] < scores[sc] ? scores[f] : scores[sc]) - (ptrdiff_t)max_score_improvement;
+                if (min_score < (ptrdiff_t)cutoff_dist) {
+                    const size_t center = cell_diff / 2;
+                    next[nnext].first = f;
+                    next[nnext].second = f + center;
+                    ++nnext;
+                    next[nnext].first = f + center;
+                    next[nnext].second = sc;
+                    ++nnext;
+                }
+            }
+            Window* t = windows;
+            windows = next;
+            next = t;
+            nwin = nnext;
+            nnext = 0;
+        }
+        free(scores);
+        free(windows);
+        free(next);
+        if (perfect) return 100;
+        double score = 1.0 - ((double)best_dist / (double)maximum);
+        score *= 100;
+        if (score >= score_cutoff) score_cutoff = res = score;
+    }
+    for (size_t i = 1; i < len1; ++i) {
+        if (!unit_in(s1, len1, s2[i - 1])) continue;
+        const double ls = ratio_similarity(s1, len1, s2, i, score_cutoff);
+        if (ls > res) {
+            score_cutoff = res = ls;
+            if (res == 100.0) return res;
+        }
+    }
+    for (size_t i = len2 - len1; i < len2; ++i) {
+        if (!unit_in(s1, len1, s2[i])) continue;
+        const double ls = ratio_similarity(s1, len1, s2 + i, len2 - i, score_cutoff);
+        if (ls > res) {
+            score_cutoff = res = ls;
+            if (res == 100.0) return res;
+        }
+    }
+    return res;
+}
+
+/* partial_ratio_alignment(...).score (fuzz_impl.hpp:193-228); CachedPartialRatio::similarity
+ * (fuzz_impl.hpp:255-279) returns the same value. */
+double orc_partial_ratio(const uint32_t* s1, size_t len1, const uint32_t* s2, size_t len2, double score_cutoff)
+{
+    if (len1 > len2) return orc_partial_ratio(s2, len2, s1, len1, score_cutoff);
+    if (score_cutoff > 100) return 0;
+    if (!len1 || !len2) return (len1 == len2) ? 100.0 : 0.0;
+    const double score = partial_ratio_impl(s1, len1, s2, len2, score_cutoff);
+    if (score != 100 && len1 == len2) {
+        score_cutoff = (score_cutoff < score) ? score : score_cutoff; /* std::max */
+        const double score2 = partial_ratio_impl(s2, len2, s1, len1, score_cutoff);
+        if (score2 > score) return score2;
+    }
+    return score;
+}
+
 int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const uint32_t* s2,
                      size_t l2, uint64_t ins, uint64_t del, uint64_t rep, double prefix_weight,
                      uint64_t cutoff_u, double cutoff_d, uint64_t* out_u, double* out_d)
@@ -498,6 +633,10 @@ int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const 
         p.ins = p.del = 1;
         p.rep = 2;
         *out_d = int_api_norm_similarity(&p, cutoff_d / 100) * 100;
+        return 1;
+    case ORC_PARTIAL_RATIO:
+        if (kind != ORC_SIMILARITY) return -1;
+        *out_d = orc_partial_ratio(s1, l1, s2, l2, cutoff_d);
         return 1;
     }
     return -1;

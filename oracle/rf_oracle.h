@@ -36,7 +36,8 @@ enum {
     ORC_RATIO = 6,
     ORC_HAMMING = 7, /* CachedHamming with pad = true */
     ORC_PREFIX = 8,
-    ORC_POSTFIX = 9
+    ORC_POSTFIX = 9,
+    ORC_PARTIAL_RATIO = 10 /* fuzz::CachedPartialRatio: similarity only */
 };
 
 enum { ORC_DISTANCE = 0, ORC_SIMILARITY = 1, ORC_NORM_DISTANCE = 2, ORC_NORM_SIMILARITY = 3 };
@@ -55,6 +56,9 @@ void orc_jaro_raw(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, ui
 double orc_jaro_similarity(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, double cutoff);
 double orc_jaro_winkler_similarity(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl,
                                    double prefix_weight, double cutoff);
+
+/* fuzz::partial_ratio(s1, s2, cutoff): the reference's window search, step by step */
+double orc_partial_ratio(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, double cutoff);
 
 /* ---- API layer: Cached<Metric>(s1).<kind>(s2, cutoff) ----
  * Returns 0 and writes *out_u for integer results, 1 and writes *out_d for

@@ -54,6 +54,7 @@ int algo_for(int metric, const rfa_weights& w)
     case RFA_HAMMING:
     case RFA_PREFIX:
     case RFA_POSTFIX: return ALGO_POSIT;
+    case RFA_PARTIAL_RATIO: return ALGO_PARTIAL;
     case RFA_JARO:
     case RFA_JARO_WINKLER: return ALGO_JARO;
     }
@@ -293,8 +294,10 @@ void validate_kind(int metric, int kind, int semantics)
 {
     if (kind < RFA_DISTANCE || kind > RFA_NORMALIZED_SIMILARITY) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown kind");
     if (semantics != RFA_CACHED && semantics != RFA_MULTI) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown semantics");
-    if (metric == RFA_RATIO && kind != RFA_SIMILARITY)
+    if ((metric == RFA_RATIO || metric == RFA_PARTIAL_RATIO) && kind != RFA_SIMILARITY)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "ratio only provides similarity");
+    if (semantics == RFA_MULTI && metric == RFA_PARTIAL_RATIO)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "partial_ratio has no Multi* scorer");
     if (semantics == RFA_MULTI && (metric == RFA_HAMMING || metric == RFA_PREFIX || metric == RFA_POSTFIX))
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "Hamming, Prefix and Postfix have no Multi* scorer");
 }
@@ -348,6 +351,88 @@ void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_
     HIP_CHECK(launch_block(algo, width, a, nblocks, slots, s));
 }
 
+// partial_ratio: tile-planned queries (<= 64 units) through the tile kernel, the
+// rest one wave per pair.  The long path holds the shorter string of each pair
+// in registers: up to PARTIAL_MAX_WORDS * 64 units.
+void cdist_partial(rfa_scorer* sc, const rfa_strings* ch, double cd, double* dout)
+{
+    rfa_context* ctx = sc->ctx;
+    hipStream_t s = ctx->stream;
+    const uint64_t nc = ch->count;
+    const int width = sc->q->width;
+    ScoreSpec spec{};
+    spec.metric = RFA_PARTIAL_RATIO;
+    spec.kind = RFA_SIMILARITY;
+    spec.semantics = RFA_CACHED;
+    spec.algo = ALGO_PARTIAL;
+    spec.cutoff_d = cd;
+    spec.band_k = ~0ull;
+    if (!sc->block_queries.empty()) {
+        const uint64_t needle = std::min<uint64_t>(sc->block_max_len, ch->max_len);
+        if ((needle + 63) / 64 > PARTIAL_MAX_WORDS)
+            throw ApiError(RFA_ERR_UNSUPPORTED, "partial_ratio of two strings longer than 1024 code units");
+    }
+    if (!sc->classes.empty()) {
+        TileArgs ta{};
+        ta.qchars = sc->q->d_chars;
+        ta.qoff = sc->q->d_off;
+        ta.qorder = sc->d_tile_order;
+        ta.nq = (uint32_t)sc->tile_order.size();
+        ta.cchars = ch->d_chars;
+        ta.coff = ch->d_off;
+        ta.corder = nullptr;
+        ta.ncsel = (uint32_t)nc;
+        ta.out_stride = nc;
+        ta.hash_bits = sc->hash_bits;
+        ta.spec = spec;
+        ta.out = dout;
+        for (const TileClass& tc : sc->classes) {
+            ta.tile_base = tc.first_tile;
+            ta.rows_cap = tc.rows_cap;
+            uint32_t target_blocks = (uint32_t)ctx->num_cus * 12;
+            uint32_t nbx = std::max<uint32_t>(1, (target_blocks + tc.ntiles - 1) / tc.ntiles);
+            uint32_t cpb = std::max<uint32_t>(32, (ta.ncsel + nbx - 1) / nbx);
+            cpb = ((cpb + 31) / 32) * 32; // whole 4-choice runs for each of the 8 waves
+            ta.choices_per_block = cpb;
+            nbx = (ta.ncsel + cpb - 1) / cpb;
+            HIP_CHECK(launch_ptile(width, tc.word_bits, ta, tc.ntiles, nbx, s));
+        }
+    }
+    if (!sc->block_queries.empty()) {
+        PartialLongArgs pa{};
+        pa.qchars = sc->q->d_chars;
+        pa.qoff = sc->q->d_off;
+        pa.qlist = sc->d_block_queries;
+        pa.nql = (uint32_t)sc->block_queries.size();
+        pa.cchars = ch->d_chars;
+        pa.coff = ch->d_off;
+        pa.nc = (uint32_t)nc;
+        pa.out_stride = nc;
+        pa.out = dout;
+        pa.cutoff = cd;
+        const uint32_t needle_words =
+            (uint32_t)std::max<uint64_t>(1, (std::min<uint64_t>(sc->block_max_len, ch->max_len) + 63) / 64);
+        constexpr uint64_t LDS_BUDGET = 40 * 1024;
+        const uint64_t need = block_arena_bytes(width, (uint32_t)sc->block_max_len, sc->block_u8_rows);
+        const uint64_t items = (uint64_t)pa.nql * nc;
+        uint32_t nblocks;
+        if (need <= LDS_BUDGET) {
+            pa.lds_bytes = (uint32_t)need;
+            const uint32_t waves_per_cu =
+                (uint32_t)std::min<uint64_t>(16, std::max<uint64_t>(1, ctx->lds_per_cu / (need + 512)));
+            nblocks = (uint32_t)std::min<uint64_t>(items, (uint64_t)ctx->num_cus * waves_per_cu);
+        }
+        else {
+            constexpr uint64_t ARENA_TOTAL = 2ull << 30;
+            const uint64_t cap = std::max<uint64_t>((uint64_t)ctx->num_cus, ARENA_TOTAL / need);
+            nblocks = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(items, (uint64_t)ctx->num_cus * 8), cap);
+            pa.pm_scratch_words = (need + 7) / 8;
+            pa.pm_scratch = (uint64_t*)ctx->pm_scratch.get(pa.pm_scratch_words * 8 * nblocks);
+        }
+        if (items) HIP_CHECK(launch_plong(width, needle_words, pa, std::max<uint32_t>(1, nblocks), s));
+    }
+}
+
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
                 int out_on_device, const uint64_t* qcut_host)
 {
@@ -381,6 +466,14 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
     void* dout = out_on_device ? out : ctx->out_tmp.get(bytes);
     const ScoreSpec spec = make_spec(sc, kind, semantics, cu, cd, algo);
     const int width = sc->q->width;
+    if (algo == ALGO_PARTIAL) {
+        cdist_partial(sc, ch, cd, (double*)dout);
+        if (!out_on_device) {
+            HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
+        return;
+    }
     if (algo == ALGO_POSIT) {
         // positional metrics: one kernel for every query length
         PositArgs pa{};
@@ -836,7 +929,8 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
 {
     if (!ctx || !queries || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, queries or out is NULL");
     return guarded([&] {
-        if (metric < RFA_LEVENSHTEIN || metric > RFA_POSTFIX) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
+        if (metric < RFA_LEVENSHTEIN || metric > RFA_PARTIAL_RATIO)
+            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
         if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};
         // weights MultiLevenshtein accepts are kept even when Cached* scoring cannot use them

@@ -144,6 +144,33 @@ struct PositArgs {
 
 hipError_t launch_posit(int char_width, const PositArgs& a, uint32_t num_cus, hipStream_t s);
 
+// partial_ratio (rfa_partial.hip).  Query tiles of at most 64 units run the tile
+// kernel (TileArgs, the tile plan of the scorer; double results); longer queries
+// run one wave per (query, choice) pair with the windows across the lanes.
+hipError_t launch_ptile(int char_width, int word_bits, const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x,
+                        hipStream_t s);
+
+struct PartialLongArgs {
+    const void* qchars;
+    const uint64_t* qoff;
+    const uint32_t* qlist; // the long queries
+    uint32_t nql;
+    const void* cchars;
+    const uint64_t* coff;
+    uint32_t nc;
+    uint64_t out_stride;
+    double* out;             // out[q * out_stride + c]
+    double cutoff;
+    uint64_t* pm_scratch;    // global pattern arenas (when the pattern does not fit in LDS)
+    uint64_t pm_scratch_words;
+    uint32_t lds_bytes;      // 0: patterns live in pm_scratch
+};
+
+// Widest shorter string (in 64-unit words) the long path takes.
+constexpr uint32_t PARTIAL_MAX_WORDS = 16;
+hipError_t launch_plong(int char_width, uint32_t needle_words, const PartialLongArgs& a, uint32_t nblocks,
+                        hipStream_t s);
+
 size_t plan_temp_bytes(uint64_t n);
 // Counting sort of the pairs into kernel ranges (items vals_out[k] = pair | pattern-is-B << 31).
 // counters[0]: block queue; [1]: end of the block range; [2], [3]: ends of the OSA

@@ -1,0 +1,531 @@
+// rfa_partial.hip -- fuzz::partial_ratio for cdist (fuzz_impl.hpp:57-279,
+// CachedPartialRatio::similarity fuzz_impl.hpp:255-279).
+//
+// The reference scores the shorter string s1 (the needle, length l1) against
+// windows of the longer one s2 (length l2 >= l1):
+//   full windows   s2[p, p + l1) for p < l2 - l1, searched by bisection with a
+//                  distance bound; their best Indel distance d counts only if
+//                  d < ceil(2 l1 * NormSim_to_NormDist(cutoff / 100)), and then
+//                  scores (1 - d / 2 l1) * 100 when that is >= cutoff;
+//   prefix windows s2[0, i) for 0 < i < l1, skipped unless s2[i - 1] is in s1;
+//   suffix windows s2[i, l2) for l2 - l1 <= i < l2, skipped unless s2[i] is in
+//                  s1; both score CachedRatio(s1).similarity(window, cutoff);
+//   and, when l1 == l2 and the score is below 100, the same with the roles
+//   swapped.  The result is the largest score found (each score passes the
+//   cutoff on its own: a window scoring above the running best also passes the
+//   raised cutoff, and ratios of distinct fractions never round to the same
+//   double), so every window can be scored independently.  The bisection's
+//   pruning bound holds exactly (adjacent windows differ by at most 2 in Indel
+//   distance, and distances of equal-length windows are even), so the minimum
+//   over all full windows equals the one it finds whenever that passes.
+//
+// Doubles are computed only for windows whose ratio 2 lcs / (l1 + L) is at
+// least that of the best window so far (a smaller fraction rounds to a
+// smaller or equal double).
+//
+// Tile kernel (queries of at most 64 units, one per lane, PM[row][lane] in LDS,
+// the choice's code units uniform across the wave):
+//   query = needle (m <= n): for every window start p of the choice, the query's
+//     PM against c[p + j], j < min(m, n - p); the window's LCS is popcount(~S).
+//     Window 0's running counts give the prefix windows.
+//   choice = needle (m >= n): the pattern of the window q[p, p + n) is the
+//     query's PM shifted right by p and masked to n bits, the text is the
+//     choice; the final state of window 0 gives every prefix window
+//     (popcount of its low i bits).
+// Long kernel (longer queries, one wave per pair): the same two cases with the
+// windows across the lanes and the needle's words in registers.
+#include <cmath>
+
+#include "rfa_block_impl.hpp"
+#include "rfa_device.hpp"
+#include "rfa_internal.hpp"
+#include "rfa_semantics.hpp"
+#include "rfa_steps.hpp"
+#include "rfa_tile_pm.hpp"
+
+namespace rfa {
+
+// best window so far: its score (0: none) and its ratio num / den = 2 lcs / (l1 + L)
+struct PBest {
+    double v;
+    uint32_t num, den;
+};
+
+// CachedRatio(s1).similarity(window, cutoff) from the window's LCS (fuzz_impl.hpp:43-47)
+__device__ __forceinline__ double window_ratio(const ScoreSpec& indel, uint32_t l1, uint32_t L, uint32_t lcs,
+                                               double cut)
+{
+    IntPair p{&indel, l1, L, lcs};
+    return int_api_norm_similarity(p, cut / 100) * 100;
+}
+
+// a prefix or suffix window (fuzz_impl.hpp:151-175)
+__device__ __forceinline__ void offer(PBest& b, const ScoreSpec& indel, uint32_t l1, uint32_t L, uint32_t lcs,
+                                      double cut)
+{
+    if (lcs == 0) return;
+    if ((uint64_t)(2 * lcs) * b.den < (uint64_t)b.num * (l1 + L)) return;
+    const double v = window_ratio(indel, l1, L, lcs, cut);
+    if (v > b.v) {
+        b.v = v;
+        b.num = 2 * lcs;
+        b.den = l1 + L;
+    }
+}
+
+// the full windows' best distance d (fuzz_impl.hpp:81-148)
+__device__ __forceinline__ void offer_full(PBest& b, uint32_t l1, uint32_t d, double cut)
+{
+    if (d == 0xffffffffu) return;
+    const uint64_t maximum = 2ull * l1;
+    const double ncs = dmin(1.0, 1.0 - cut / 100 + 0.00001); // NormSim_to_NormDist
+    const uint64_t cutoff_dist = (uint64_t)ceil((double)maximum * ncs);
+    if ((uint64_t)d >= cutoff_dist) return;
+    double score = 1.0 - (double)d / (double)maximum;
+    score *= 100;
+    if (score >= cut && score > b.v) {
+        b.v = score;
+        b.num = (uint32_t)(maximum - d);
+        b.den = (uint32_t)maximum;
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+    return uniform(v);
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, 64));
+    return uniform(v);
+}
+
+__device__ __forceinline__ double wave_max_f64(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// one-word LCS state (LCSseq_impl.hpp:219-252); 64-bit words as two halves
+template <typename W>
+struct Lcs1;
+
+template <>
+struct Lcs1<uint32_t> {
+    uint32_t S;
+    __device__ __forceinline__ void reset() { S = ~0u; }
+    __device__ __forceinline__ void step(uint32_t X) { lcs_step(X, S); }
+    __device__ __forceinline__ uint32_t count() const { return (uint32_t)__popc(~S); }
+    __device__ __forceinline__ uint32_t zero_bit(uint32_t k) const { return (~S >> k) & 1u; }
+};
+
+template <>
+struct Lcs1<uint64_t> {
+    uint32_t Sl, Sh;
+    __device__ __forceinline__ void reset() { Sl = Sh = ~0u; }
+    __device__ __forceinline__ void step(uint64_t X) { lcs_step(X, Sl, Sh); }
+    __device__ __forceinline__ uint32_t count() const { return (uint32_t)(__popc(~Sl) + __popc(~Sh)); }
+    __device__ __forceinline__ uint32_t zero_bit(uint32_t k) const
+    {
+        return k < 32 ? (~Sl >> k) & 1u : (~Sh >> (k - 32)) & 1u;
+    }
+};
+
+// ---- tile kernel --------------------------------------------------------------
+
+template <typename CharT, typename W>
+__global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
+{
+    extern __shared__ uint64_t smem_u64[];
+    W* pm = (W*)smem_u64;
+    uint8_t* extra = (uint8_t*)(pm + (size_t)a.rows_cap * 64);
+    const CharT* __restrict__ q = (const CharT*)a.qchars;
+    const CharT* __restrict__ c = (const CharT*)a.cchars;
+    constexpr uint32_t BITS = sizeof(W) * 8;
+
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t tile = blockIdx.y + a.tile_base;
+    const uint32_t qpos = tile * 64 + lane;
+    const bool qact = qpos < a.nq;
+    const uint32_t qidx = qact ? a.qorder[qpos] : 0;
+    const uint64_t qo = qact ? a.qoff[qidx] : 0;
+    const uint32_t m = qact ? (uint32_t)(a.qoff[qidx + 1] - qo) : 0;
+
+    SymLookup<CharT> look = build_tile_pm<CharT, W>(q, qo, m, pm, extra, a);
+    ScoreSpec indel = a.spec;
+    indel.metric = RFA_INDEL;
+    const double cut = a.spec.cutoff_d;
+    const W* __restrict__ col = pm + lane;
+
+    const uint32_t it0 = blockIdx.x * a.choices_per_block;
+    const uint32_t it1 = min(a.ncsel, it0 + a.choices_per_block);
+    constexpr uint32_t G = 4;
+    for (uint32_t g0 = it0 + uniform(wave) * G; g0 < it1; g0 += TILE_WAVES * G) {
+        const uint32_t gn = min(G, it1 - g0);
+        uint64_t outv[G];
+        for (uint32_t t = 0; t < gn; ++t) {
+            const uint32_t cidx = g0 + t;
+            const uint64_t co0 = a.coff[cidx];
+            const uint32_t n = uniform((uint32_t)(a.coff[cidx + 1] - co0));
+            const uint64_t co = ((uint64_t)uniform((uint32_t)(co0 >> 32)) << 32) | uniform((uint32_t)co0);
+            PBest b{0.0, 0u, 1u};
+
+            // ---- the query is the needle: windows of the choice ----
+            const bool on1 = qact && m > 0 && m <= n && cut <= 100;
+            const uint32_t mm = wave_max_u32(on1 ? m : 0u);
+            if (mm) {
+                uint32_t bestd = 0xffffffffu;
+                for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+                    uint32_t rowA = 0, rowB = 0;
+                    if (k0 + lane < n) rowA = look((uint32_t)c[co + k0 + lane]);
+                    if (k0 + 64 + lane < n) rowB = look((uint32_t)c[co + k0 + 64 + lane]);
+                    const uint32_t pend = min(n, k0 + 64);
+                    for (uint32_t p = k0; p < pend; ++p) {
+                        const uint32_t steps = min(mm, n - p);
+                        Lcs1<W> S;
+                        S.reset();
+                        bool first_in = false;
+                        for (uint32_t j = 0; j < steps; ++j) {
+                            const uint32_t idx = p - k0 + j; // < 127
+                            const uint32_t r = idx < 64 ? readlane(rowA, idx) : readlane(rowB, idx - 64);
+                            W X = col[r * 64];
+                            if (j == 0) first_in = X != 0;
+                            X = j < m ? X : (W)0;
+                            S.step(X);
+                            if (p == 0) {
+                                // prefix window c[0, j + 1): its last unit must occur in the query
+                                if (on1 && j + 1 < m && X != 0) offer(b, indel, m, j + 1, S.count(), cut);
+                            }
+                        }
+                        if (on1) {
+                            const uint32_t lcs = S.count();
+                            if (p + m < n)
+                                bestd = min(bestd, 2 * (m - lcs));
+                            else if (first_in)
+                                offer(b, indel, m, n - p, lcs, cut); // suffix window c[p, n)
+                        }
+                    }
+                }
+                if (on1) offer_full(b, m, bestd, cut);
+            }
+
+            // ---- the choice is the needle (and the swapped pass of equal lengths) ----
+            const bool on2 = qact && n > 0 && m >= n && cut <= 100;
+            const uint32_t mM = wave_max_u32(on2 ? m : 0u);
+            if (mM) {
+                // n <= m <= 64: the whole choice in one register
+                uint32_t rowC = 0;
+                if (lane < n) rowC = look((uint32_t)c[co + lane]);
+                const W maskn = n >= BITS ? ~(W)0 : (((W)1 << n) - 1);
+                W U = 0; // bit k: q[k] occurs in the choice
+                for (uint32_t j = 0; j < n; ++j) U |= col[readlane(rowC, j) * 64];
+                uint32_t bestd = 0xffffffffu;
+                for (uint32_t p = 0; p < mM; ++p) {
+                    Lcs1<W> S;
+                    S.reset();
+                    for (uint32_t j = 0; j < n; ++j) {
+                        const W Xq = col[readlane(rowC, j) * 64];
+                        S.step((W)(Xq >> p) & maskn);
+                    }
+                    if (on2 && p < m) {
+                        const uint32_t lcs = S.count();
+                        if (p + n < m)
+                            bestd = min(bestd, 2 * (n - lcs));
+                        else if ((U >> p) & 1)
+                            offer(b, indel, n, m - p, lcs, cut); // suffix window q[p, m)
+                        if (p == 0) {
+                            // prefix windows q[0, i): LCS = zero bits of S below i
+                            uint32_t cum = 0;
+                            for (uint32_t i = 1; i < n; ++i) {
+                                cum += S.zero_bit(i - 1);
+                                if ((U >> (i - 1)) & 1) offer(b, indel, n, i, cum, cut);
+                            }
+                        }
+                    }
+                }
+                if (on2) offer_full(b, n, bestd, cut);
+            }
+
+            double v;
+            if (cut > 100)
+                v = 0.0;
+            else if (m == 0 || n == 0)
+                v = (m == n) ? 100.0 : 0.0;
+            else
+                v = b.v;
+            union {
+                double d;
+                uint64_t u;
+            } cv;
+            cv.d = v;
+            outv[t] = cv.u;
+        }
+        if (qact) {
+            uint64_t* dst = (uint64_t*)a.out + (uint64_t)qidx * a.out_stride + g0;
+            if (gn == G && ((uintptr_t)dst & 15) == 0) {
+                uint4* d4 = (uint4*)dst;
+#pragma unroll
+                for (uint32_t t = 0; t < G; t += 2)
+                    d4[t / 2] = make_uint4((uint32_t)outv[t], (uint32_t)(outv[t] >> 32), (uint32_t)outv[t + 1],
+                                           (uint32_t)(outv[t + 1] >> 32));
+            }
+            else {
+                for (uint32_t t = 0; t < gn; ++t) dst[t] = outv[t];
+            }
+        }
+    }
+}
+
+// ---- long kernel: one wave per pair, windows across the lanes ------------------
+
+// S += S & X with the carry chain across the needle's words; S | (S & ~X)
+template <int WMAX>
+__device__ __forceinline__ void lcs_words(uint64_t (&S)[WMAX], const uint64_t (&X)[WMAX], uint32_t nw)
+{
+    uint64_t carry = 0;
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) {
+        if ((uint32_t)w < nw) {
+            const uint64_t s = S[w], u = s & X[w];
+            const uint64_t t = s + u;
+            const uint64_t c1 = t < s;
+            const uint64_t t2 = t + carry;
+            const uint64_t c2 = t2 < t;
+            S[w] = t2 | (s & ~X[w]);
+            carry = c1 | c2;
+        }
+    }
+}
+
+template <int WMAX>
+__device__ __forceinline__ uint32_t zeros(const uint64_t (&S)[WMAX], uint32_t nw)
+{
+    uint32_t z = 0;
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w)
+        if ((uint32_t)w < nw) z += (uint32_t)__popcll(~S[w]);
+    return z;
+}
+
+// the query (m units) is the needle, m <= n: window p = lane + 64k of the choice
+template <int WMAX, typename CharT, bool LDS>
+__device__ void plong_query_needle(const Pattern<CharT, LDS>& P, uint32_t m, const CharT* __restrict__ T, uint32_t n,
+                                   const ScoreSpec& indel, double cut, PBest& b)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nw = (m + 63) / 64;
+    uint32_t bestd = 0xffffffffu;
+    for (uint32_t p0 = 0; p0 < n; p0 += 64) {
+        const uint32_t p = p0 + lane;
+        const bool act = p < n;
+        const uint32_t L = act ? min(m, n - p) : 0u;
+        const uint32_t steps = min(m, n - p0);
+        uint64_t S[WMAX], X[WMAX];
+#pragma unroll
+        for (int w = 0; w < WMAX; ++w) S[w] = ~0ull;
+        bool first_in = false;
+        for (uint32_t j = 0; j < steps; ++j) {
+            const uint32_t r = j < L ? P.map.find((uint32_t)T[p + j]) : 0u;
+            if (j == 0) first_in = r != 0;
+            const uint64_t* row = P.pm + (size_t)r * P.Wp;
+#pragma unroll
+            for (int w = 0; w < WMAX; ++w) X[w] = (uint32_t)w < nw ? row[w] : 0ull;
+            lcs_words<WMAX>(S, X, nw);
+            // prefix window c[0, j + 1) (lane 0 holds window 0)
+            if (p == 0 && j + 1 < m && r != 0) offer(b, indel, m, j + 1, zeros<WMAX>(S, nw), cut);
+        }
+        if (act) {
+            const uint32_t lcs = zeros<WMAX>(S, nw);
+            if (p + m < n)
+                bestd = min(bestd, 2 * (m - lcs));
+            else if (first_in)
+                offer(b, indel, m, n - p, lcs, cut);
+        }
+    }
+    offer_full(b, m, wave_min_u32(bestd), cut);
+}
+
+// the choice (n units) is the needle, n <= m: window q[p, p + n) with p = lane + 64k,
+// its pattern the query's PM from bit p, masked to n bits; the text is the choice
+template <int WMAX, typename CharT, bool LDS>
+__device__ void plong_choice_needle(const Pattern<CharT, LDS>& P, uint32_t m, const CharT* __restrict__ T, uint32_t n,
+                                    const ScoreSpec& indel, double cut, PBest& b)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nw = (n + 63) / 64, qw = P.W;
+    const uint64_t last = (n & 63) ? ((1ull << (n & 63)) - 1) : ~0ull;
+    uint32_t bestd = 0xffffffffu;
+    for (uint32_t p0 = 0; p0 < m; p0 += 64) {
+        const uint32_t p = p0 + lane;
+        const bool act = p < m;
+        const uint32_t base = p >> 6, sh = p & 63;
+        uint64_t S[WMAX], U[WMAX], X[WMAX];
+#pragma unroll
+        for (int w = 0; w < WMAX; ++w) {
+            S[w] = ~0ull;
+            U[w] = 0;
+        }
+        for (uint32_t j = 0; j < n; ++j) {
+            const uint32_t r = P.map.find((uint32_t)T[j]);
+            const uint64_t* row = P.pm + (size_t)r * P.Wp;
+#pragma unroll
+            for (int w = 0; w < WMAX; ++w) {
+                uint64_t x = 0;
+                if ((uint32_t)w < nw) {
+                    const uint32_t k = base + w;
+                    const uint64_t lo = (act && k < qw) ? row[k] : 0ull;
+                    const uint64_t hi = (act && k + 1 < qw) ? row[k + 1] : 0ull;
+                    x = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+                    if ((uint32_t)w == nw - 1) x &= last;
+                    U[w] |= x;
+                }
+                X[w] = x;
+            }
+            lcs_words<WMAX>(S, X, nw);
+        }
+        if (act) {
+            const uint32_t lcs = zeros<WMAX>(S, nw);
+            if (p + n < m)
+                bestd = min(bestd, 2 * (n - lcs));
+            else if (U[0] & 1)
+                offer(b, indel, n, m - p, lcs, cut); // suffix window q[p, m), q[p] in the choice
+            if (p == 0) {
+                // prefix windows q[0, i), 0 < i < n
+                uint32_t cum = 0;
+#pragma unroll
+                for (int w = 0; w < WMAX; ++w) {
+                    if ((uint32_t)w < nw) {
+                        const uint64_t z = ~S[w], u = U[w];
+                        for (uint32_t k = 0; k < 64; ++k) {
+                            const uint32_t i = (uint32_t)w * 64 + k + 1;
+                            if (i >= n) break;
+                            cum += (uint32_t)((z >> k) & 1);
+                            if ((u >> k) & 1) offer(b, indel, n, i, cum, cut);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    offer_full(b, n, wave_min_u32(bestd), cut);
+}
+
+template <typename CharT, int WMAX, bool LDS>
+__global__ __launch_bounds__(64) void k_plong(PartialLongArgs a)
+{
+    extern __shared__ uint64_t smem_u64[];
+    uint8_t* arena = LDS ? (uint8_t*)smem_u64 : (uint8_t*)(a.pm_scratch + (size_t)blockIdx.x * a.pm_scratch_words);
+    const CharT* __restrict__ Q = (const CharT*)a.qchars;
+    const CharT* __restrict__ C = (const CharT*)a.cchars;
+    const uint32_t lane = threadIdx.x & 63;
+    ScoreSpec indel{};
+    indel.metric = RFA_INDEL;
+    indel.kind = RFA_NORMALIZED_SIMILARITY;
+    const double cut = a.cutoff;
+    // a contiguous share of the (query, choice) items: the pattern changes rarely
+    const uint64_t total = (uint64_t)a.nql * a.nc;
+    const uint64_t lo = total * blockIdx.x / gridDim.x, hi = total * (blockIdx.x + 1) / gridDim.x;
+    uint32_t prev = 0xffffffffu;
+    Pattern<CharT, LDS> P{};
+    for (uint64_t it = lo; it < hi; ++it) {
+        const uint32_t qk = (uint32_t)(it / a.nc), ci = (uint32_t)(it % a.nc);
+        const uint32_t qi = a.qlist[qk];
+        const uint64_t qo = a.qoff[qi];
+        const uint32_t m = (uint32_t)(a.qoff[qi + 1] - qo);
+        if (qi != prev) {
+            P = build_pattern<CharT, LDS>(Q + qo, m, arena, pattern_hash_bits<CharT>(m));
+            prev = qi;
+        }
+        const uint64_t co = a.coff[ci];
+        const uint32_t n = (uint32_t)(a.coff[ci + 1] - co);
+        double v;
+        if (cut > 100) {
+            v = 0.0;
+        }
+        else if (m == 0 || n == 0) {
+            v = (m == n) ? 100.0 : 0.0;
+        }
+        else {
+            PBest b{0.0, 0u, 1u};
+            if (m <= n) plong_query_needle<WMAX>(P, m, C + co, n, indel, cut, b);
+            if (m >= n) plong_choice_needle<WMAX>(P, m, C + co, n, indel, cut, b);
+            v = wave_max_f64(b.v);
+        }
+        if (lane == 0) a.out[(uint64_t)qi * a.out_stride + ci] = v;
+    }
+}
+
+// ---- launches -------------------------------------------------------------------
+
+template <typename CharT, typename W>
+static void launch_ptile_w(const TileArgs& a, uint32_t ntiles, uint32_t nbx, size_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_ptile<CharT, W>), dim3(nbx, ntiles), dim3(TILE_THREADS), lds, s, a);
+}
+
+hipError_t launch_ptile(int char_width, int word_bits, const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x,
+                        hipStream_t s)
+{
+    const size_t lds = tile_lds_bytes(char_width, word_bits, a.rows_cap, a.hash_bits);
+    if (word_bits == 32) {
+        if (char_width == 1)
+            launch_ptile_w<uint8_t, uint32_t>(a, ntiles, nblocks_x, lds, s);
+        else if (char_width == 2)
+            launch_ptile_w<uint16_t, uint32_t>(a, ntiles, nblocks_x, lds, s);
+        else
+            launch_ptile_w<uint32_t, uint32_t>(a, ntiles, nblocks_x, lds, s);
+    }
+    else {
+        if (char_width == 1)
+            launch_ptile_w<uint8_t, uint64_t>(a, ntiles, nblocks_x, lds, s);
+        else if (char_width == 2)
+            launch_ptile_w<uint16_t, uint64_t>(a, ntiles, nblocks_x, lds, s);
+        else
+            launch_ptile_w<uint32_t, uint64_t>(a, ntiles, nblocks_x, lds, s);
+    }
+    return hipGetLastError();
+}
+
+template <typename CharT, int WMAX>
+static void launch_plong_w(const PartialLongArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    if (a.lds_bytes)
+        hipLaunchKernelGGL((k_plong<CharT, WMAX, true>), dim3(nblocks), dim3(64), a.lds_bytes, s, a);
+    else
+        hipLaunchKernelGGL((k_plong<CharT, WMAX, false>), dim3(nblocks), dim3(64), 0, s, a);
+}
+
+template <typename CharT>
+static void launch_plong_c(uint32_t words, const PartialLongArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    if (words <= 1)
+        launch_plong_w<CharT, 1>(a, nblocks, s);
+    else if (words <= 2)
+        launch_plong_w<CharT, 2>(a, nblocks, s);
+    else if (words <= 4)
+        launch_plong_w<CharT, 4>(a, nblocks, s);
+    else if (words <= 8)
+        launch_plong_w<CharT, 8>(a, nblocks, s);
+    else
+        launch_plong_w<CharT, 16>(a, nblocks, s);
+}
+
+hipError_t launch_plong(int char_width, uint32_t needle_words, const PartialLongArgs& a, uint32_t nblocks,
+                        hipStream_t s)
+{
+    if (needle_words > PARTIAL_MAX_WORDS) return hipErrorInvalidValue;
+    if (char_width == 1)
+        launch_plong_c<uint8_t>(needle_words, a, nblocks, s);
+    else if (char_width == 2)
+        launch_plong_c<uint16_t>(needle_words, a, nblocks, s);
+    else
+        launch_plong_c<uint32_t>(needle_words, a, nblocks, s);
+    return hipGetLastError();
+}
+
+} // namespace rfa

@@ -495,11 +495,12 @@ static void validate_topk(int metric, int kind)
     validate_kind(metric, kind, RFA_CACHED);
     const bool sim_double = kind == RFA_NORMALIZED_SIMILARITY ||
                             (kind == RFA_SIMILARITY &&
-                             (metric == RFA_JARO || metric == RFA_JARO_WINKLER || metric == RFA_RATIO));
+                             (metric == RFA_JARO || metric == RFA_JARO_WINKLER || metric == RFA_RATIO ||
+                              metric == RFA_PARTIAL_RATIO));
     if (!sim_double)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT,
                        "top-k ranks double similarities: use a normalized similarity, or the similarity of "
-                       "Jaro, Jaro-Winkler or ratio");
+                       "Jaro, Jaro-Winkler, ratio or partial_ratio");
 }
 
 void topk_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, double cutoff, uint32_t k, int64_t* out_index,
@@ -857,7 +858,7 @@ rfa_status rfa_server_create(rfa_context* ctx, const rfa_strings* choices, const
 {
     if (!ctx || !choices || !cfg || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, choices, cfg or out is NULL");
     return guarded([&] {
-        if (cfg->metric < RFA_LEVENSHTEIN || cfg->metric > RFA_POSTFIX)
+        if (cfg->metric < RFA_LEVENSHTEIN || cfg->metric > RFA_PARTIAL_RATIO)
             throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         validate_topk(cfg->metric, cfg->kind);
         if (cfg->k == 0) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "k must be positive");

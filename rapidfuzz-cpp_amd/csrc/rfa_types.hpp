@@ -26,8 +26,9 @@ enum Algo : int32_t {
     ALGO_OSA = 2,  // Hyyro 2003 with transpositions (OSA_impl.hpp:34-74, 176-249)
     ALGO_JARO = 3, // greedy window flagging + transposition walk (Jaro_impl.hpp:94-315)
     ALGO_MIXED = 4, // per-item metric (Levenshtein / Indel / LCSseq / OSA), chosen at run time
-    ALGO_POSIT = 5  // positional comparison: Hamming (Hamming_impl.hpp:23-35), common prefix / suffix
+    ALGO_POSIT = 5, // positional comparison: Hamming (Hamming_impl.hpp:23-35), common prefix / suffix
                     // (Prefix_impl.hpp:23-27, Postfix_impl.hpp:23-27)
+    ALGO_PARTIAL = 6 // LCS of the shorter string against every window of the longer one (fuzz_impl.hpp:57-279)
 };
 
 // A packed string set on the device.

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "librapidfuzz_amd.so")
 
-LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX = range(10)
+LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX, PARTIAL_RATIO = range(11)
 DISTANCE, SIMILARITY, NORMALIZED_DISTANCE, NORMALIZED_SIMILARITY = range(4)
 CACHED, MULTI = 0, 1
 SCHEDULE_BUCKETED, SCHEDULE_PLAIN = 0, 1
@@ -280,7 +280,7 @@ class Scorer:
         """(index, score), each [nq, k]: the extract() results >= cutoff_d ordered by
         (score desc, index asc); missing entries are index -1, score 0."""
         if kind is None:
-            kind = SIMILARITY if self.metric in (JARO, JARO_WINKLER, RATIO) else NORMALIZED_SIMILARITY
+            kind = SIMILARITY if self.metric in (JARO, JARO_WINKLER, RATIO, PARTIAL_RATIO) else NORMALIZED_SIMILARITY
         idx = np.zeros((self.nq, k), dtype=np.int64)
         sc = np.zeros((self.nq, k), dtype=np.float64)
         _check(_lib.rfa_topk(self.h, choices.h, kind, cutoff_d, k, idx.ctypes.data, sc.ctypes.data))
@@ -293,7 +293,7 @@ class Server:
     def __init__(self, ctx: Context, choices: Strings, metric=RATIO, kind=None, score_cutoff=0.0, k=5,
                  max_batch=64, slots=4, candidate_cap=65536, prefix_weight=0.1, weights=(1, 1, 1)):
         if kind is None:
-            kind = SIMILARITY if metric in (JARO, JARO_WINKLER, RATIO) else NORMALIZED_SIMILARITY
+            kind = SIMILARITY if metric in (JARO, JARO_WINKLER, RATIO, PARTIAL_RATIO) else NORMALIZED_SIMILARITY
         self.ctx = ctx
         self.k = k
         self.width = choices.width
