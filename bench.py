@@ -581,6 +581,121 @@ def workload_positional(args, emit=True):
     return line
 
 
+PQ, PC = 1024, 65536
+
+
+def gen_partial(rng, nq=PQ, nc=PC):
+    """partial_ratio workload: queries of U{8..32} and choices of U{32..128} units over a-z and space
+    (space 1 in 6); a quarter of the choices carry a query with 0-2 substitutions written over a
+    random position (so high scores occur)."""
+    def text(n):
+        t = rng.integers(97, 123, size=n).astype(np.uint8)
+        t[rng.random(n) < 1 / 6] = 32
+        return t
+
+    ql = rng.integers(8, 33, size=nq)
+    cl = rng.integers(32, 129, size=nc)
+    qs = [text(int(m)) for m in ql]
+    cs = [text(int(n)) for n in cl]
+    for j in rng.choice(nc, size=nc // 4, replace=False):
+        q = qs[int(rng.integers(0, nq))].copy()
+        for _ in range(int(rng.integers(0, 3))):
+            q[int(rng.integers(0, len(q)))] = int(rng.integers(97, 123))
+        at = int(rng.integers(0, len(cs[j]) - len(q) + 1))
+        cs[j][at:at + len(q)] = q
+    qc, qo = pack_u8(qs)
+    cc, co = pack_u8(cs)
+    return qc, qo, ql.astype(np.int64), cc, co, cl.astype(np.int64)
+
+
+def workload_partial(args, emit=True):
+    """fuzz::partial_ratio cdist, 1024 word-like queries (8-32 units) x 65,536 text choices (32-128
+    units), uint8, no cutoff: the best ratio of each query against every window of each choice.
+    Queries of at most 32 units run the tile kernel with 32-bit words."""
+    import rapidfuzz_amd as rfa
+    from rapidfuzz_amd import hip
+
+    rng = np.random.default_rng(SEED + 7)
+    qc, qo, ql, cc, co, cl = gen_partial(rng)
+    nq, nc = len(ql), len(cl)
+    ctx = rfa.Context(0)
+    qs = rfa.Strings(ctx, chars=qc, offsets=qo, width=1)
+    cs = rfa.Strings(ctx, chars=cc, offsets=co, width=1)
+    out = hip.DeviceBuffer(nq * nc * 8)
+    sc = rfa.Scorer(ctx, rfa.PARTIAL_RATIO, qs)
+    for _ in range(max(1, args.warmup)):
+        sc.cdist(cs, rfa.SIMILARITY, out_device_ptr=out.ptr.value)
+    ctx.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    steps = max(1, args.steps)
+    e0.record(ctx.stream)
+    for _ in range(steps):
+        sc.cdist(cs, rfa.SIMILARITY, out_device_ptr=out.ptr.value)
+    e1.record(ctx.stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    pairs = float(nq) * float(nc)
+    # window steps each pair needs (query as the needle: sum over window starts of
+    # min(m, n - p); choice as the needle: m windows of n steps)
+    mh = np.bincount(ql, minlength=129).astype(np.float64)
+    nh = np.bincount(cl, minlength=129).astype(np.float64)
+    mv = np.arange(129, dtype=np.float64)
+    M, N = np.meshgrid(mv, mv, indexing="ij")
+    s1 = np.where(M <= N, M * (M + 1) / 2 + M * (N - M), 0.0)
+    s2 = np.where(M >= N, M * N, 0.0)
+    wsteps = float(np.sum(np.outer(mh, nh) * (s1 + s2)))
+    ops = wsteps * 5.0  # one 32-bit LCS word-step = half of the 10 INT32 ops of a 64-bit one
+    tops = ops / (ms / 1e3) / 1e12
+    peak = 39.322
+    print(f"# partial_ratio: {ms:.3f} ms/step, {pairs / (ms / 1e3) / 1e9:.2f} Gpairs/s, "
+          f"{wsteps / (ms / 1e3) / 1e12:.2f} T window-steps/s", file=sys.stderr, flush=True)
+    line = {"metric": "partial_ratio cdist 1024 x 65536, Gpairs/s", "value": round(pairs / (ms / 1e3) / 1e9, 3),
+            "unit": "Gpairs/s", "n_gpus": 1, "steps": steps, "warmup": max(1, args.warmup),
+            "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u32", "data": "synthetic",
+            "config": {"workload": f"fuzz::CachedPartialRatio cdist {nq} x {nc}, uint8 a-z and space, queries "
+                                   f"8-32 units, choices 32-128 units (a quarter embed a query), no cutoff"},
+            "roofline": {"bound": "valu", "achieved": round(tops, 3), "peak": peak, "unit": "TOP/s",
+                         "frac": round(tops / peak, 4), "traffic": None,
+                         "ops_model": "window word-steps x 5 INT32 ops (32-bit LCS word-step); "
+                                      f"{wsteps:.4g} window steps per call",
+                         "kernel": "k_ptile<u8, 32-bit words>"}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        if oracle_py.reference_available():
+            ref = oracle_py.Reference()
+            cores = _cores()
+            k = 4
+            while True:
+                idx = np.linspace(0, nq - 1, k).astype(np.int64)
+                sq, so = pack_u8([qc[int(qo[i]):int(qo[i + 1])] for i in idx])
+                t0 = time.perf_counter()
+                r = ref.cdist_u8_mt(rfa.PARTIAL_RATIO, rfa.SIMILARITY, sq, so, cc, co, threads=cores)
+                dt = time.perf_counter() - t0
+                if dt > 3 or k >= nq:
+                    break
+                k = min(nq, k * 4)
+            rows = np.zeros(nc, np.float64)
+            bad = 0
+            for k_, i in enumerate(idx):
+                out.to_host(rows, offset=int(i) * nc * 8)
+                bad += int(np.count_nonzero(rows.view(np.uint64) != r[k_].view(np.uint64)))
+            line["cpu_baseline"] = {"value": round(k * nc / dt / 1e9, 5), "unit": "Gpairs/s", "cores": cores,
+                                    "kind": "reference",
+                                    "sample": f"{k} of {nq} queries x all {nc} choices, "
+                                              f"fuzz::CachedPartialRatio ({dt:.1f} s)"}
+            line["check"] = "bit-exact vs reference on the CPU sample" if bad == 0 else f"{bad} MISMATCHES"
+    sc.close()
+    out.free()
+    for x in (qs, cs):
+        x.close()
+    ctx.close()
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
+
+
 def _mutate_dna(rng, s):
     """Per position: substitution 1% (to another base), insertion 0.5% (a random base before it),
     deletion 0.5%."""
@@ -719,7 +834,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="cdist",
-                    choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "copy"],
+                    choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "partial", "copy"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
                          "uneven: 1M independent pairs; serve: the serving leg alone; latency: 1 x 10,000 "
                          "Levenshtein; dna: 256 x 16,384 long sequences with band cutoffs; copy: HBM counter "
@@ -752,6 +867,8 @@ def main():
         return workload_dna(args)
     if args.workload == "positional":
         return workload_positional(args)
+    if args.workload == "partial":
+        return workload_partial(args)
 
     import rapidfuzz_amd as rfa
     from rapidfuzz_amd import hip
@@ -904,6 +1021,9 @@ def main():
         pos = workload_positional(args, emit=False)
         res["configs"]["positional_4096x65536"] = {k: pos[k] for k in ("metric", "value", "unit", "ms_per_step",
                                                                         "hamming", "prefix", "postfix")}
+        par = workload_partial(args, emit=False)
+        res["configs"]["partial_ratio_1024x65536"] = {**{k: par.get(k) for k in keep},
+                                                      "config": par["config"]["workload"]}
         une = workload_uneven(args, emit=False)
         res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
                                              "speedup_vs_plain": une["speedup_vs_plain"],

@@ -179,28 +179,66 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
             const bool on1 = qact && m > 0 && m <= n && cut <= 100;
             const uint32_t mm = wave_max_u32(on1 ? m : 0u);
             if (mm) {
+                // steps below the shortest participating query need no per-lane length mask
+                const uint32_t mmin = 64 - wave_max_u32(on1 ? 64 - m : 0u);
                 uint32_t bestd = 0xffffffffu;
-                for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-                    uint32_t rowA = 0, rowB = 0;
-                    if (k0 + lane < n) rowA = look((uint32_t)c[co + k0 + lane]);
-                    if (k0 + 64 + lane < n) rowB = look((uint32_t)c[co + k0 + 64 + lane]);
-                    const uint32_t pend = min(n, k0 + 64);
-                    for (uint32_t p = k0; p < pend; ++p) {
-                        const uint32_t steps = min(mm, n - p);
+                // one register holds the rows of c[k0, k0 + 64): windows k0 .. k0 + 64 - mm
+                // never read past it
+                const uint32_t wstep = 65 - mm;
+                uint32_t unit_next = lane < n ? (uint32_t)c[co + lane] : 0u;
+                for (uint32_t k0 = 0; k0 < n; k0 += wstep) {
+                    const uint32_t row = k0 + lane < n ? look(unit_next) : 0u;
+                    // the next batch's unit, loaded while this batch runs
+                    const uint32_t k1 = k0 + wstep;
+                    if (k1 + lane < n) unit_next = (uint32_t)c[co + k1 + lane];
+                    const uint32_t pend = min(n, k0 + wstep);
+                    uint32_t p = k0;
+                    if (p == 0) {
+                        // window 0 step by step: the prefix windows c[0, j + 1) on the way
+                        // (each needs its last unit in the query)
                         Lcs1<W> S;
                         S.reset();
+                        const uint32_t steps = min(mm, n);
                         bool first_in = false;
                         for (uint32_t j = 0; j < steps; ++j) {
-                            const uint32_t idx = p - k0 + j; // < 127
-                            const uint32_t r = idx < 64 ? readlane(rowA, idx) : readlane(rowB, idx - 64);
-                            W X = col[r * 64];
+                            W X = col[readlane(row, j) * 64];
                             if (j == 0) first_in = X != 0;
                             X = j < m ? X : (W)0;
                             S.step(X);
-                            if (p == 0) {
-                                // prefix window c[0, j + 1): its last unit must occur in the query
-                                if (on1 && j + 1 < m && X != 0) offer(b, indel, m, j + 1, S.count(), cut);
-                            }
+                            if (on1 && j + 1 < m && X != 0) offer(b, indel, m, j + 1, S.count(), cut);
+                        }
+                        if (on1) {
+                            const uint32_t lcs = S.count();
+                            if (m < n)
+                                bestd = min(bestd, 2 * (m - lcs));
+                            else if (first_in)
+                                offer(b, indel, m, n, lcs, cut);
+                        }
+                        ++p;
+                    }
+                    for (; p < pend; ++p) {
+                        const uint32_t steps = min(mm, n - p);
+                        const uint32_t free_steps = min(mmin, steps);
+                        const uint32_t i0 = p - k0;
+                        Lcs1<W> S;
+                        S.reset();
+                        // two rows in flight: Xa for step j, Xb for step j + 1
+                        W Xa = col[readlane(row, i0) * 64];
+                        W Xb = col[readlane(row, min(i0 + 1, 63u)) * 64];
+                        const bool first_in = Xa != 0;
+                        uint32_t j = 0;
+                        for (; j + 2 <= free_steps; j += 2) {
+                            const W A = Xa, B = Xb;
+                            Xa = col[readlane(row, min(i0 + j + 2, 63u)) * 64];
+                            Xb = col[readlane(row, min(i0 + j + 3, 63u)) * 64];
+                            S.step(A);
+                            S.step(B);
+                        }
+                        for (; j < steps; ++j) {
+                            const W X = Xa;
+                            Xa = Xb;
+                            Xb = col[readlane(row, min(i0 + j + 2, 63u)) * 64];
+                            S.step(j < m ? X : (W)0);
                         }
                         if (on1) {
                             const uint32_t lcs = S.count();
@@ -228,8 +266,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                 for (uint32_t p = 0; p < mM; ++p) {
                     Lcs1<W> S;
                     S.reset();
+                    W Xn = col[readlane(rowC, 0) * 64];
                     for (uint32_t j = 0; j < n; ++j) {
-                        const W Xq = col[readlane(rowC, j) * 64];
+                        const W Xq = Xn;
+                        Xn = col[readlane(rowC, min(j + 1, 63u)) * 64];
                         S.step((W)(Xq >> p) & maskn);
                     }
                     if (on2 && p < m) {
