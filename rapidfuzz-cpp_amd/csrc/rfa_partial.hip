@@ -27,7 +27,8 @@
 // the choice's code units uniform across the wave):
 //   query = needle (m <= n): for every window start p of the choice, the query's
 //     PM against c[p + j], j < min(m, n - p); the window's LCS is popcount(~S).
-//     Window 0's running counts give the prefix windows.
+//     Window 0's running counts give the prefix windows (scored last, once the
+//     full windows' best bounds them).
 //   choice = needle (m >= n): the pattern of the window q[p, p + n) is the
 //     query's PM shifted right by p and masked to n bits, the text is the
 //     choice; the final state of window 0 gives every prefix window
@@ -193,29 +194,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                     if (k1 + lane < n) unit_next = (uint32_t)c[co + k1 + lane];
                     const uint32_t pend = min(n, k0 + wstep);
                     uint32_t p = k0;
-                    if (p == 0) {
-                        // window 0 step by step: the prefix windows c[0, j + 1) on the way
-                        // (each needs its last unit in the query)
-                        Lcs1<W> S;
-                        S.reset();
-                        const uint32_t steps = min(mm, n);
-                        bool first_in = false;
-                        for (uint32_t j = 0; j < steps; ++j) {
-                            W X = col[readlane(row, j) * 64];
-                            if (j == 0) first_in = X != 0;
-                            X = j < m ? X : (W)0;
-                            S.step(X);
-                            if (on1 && j + 1 < m && X != 0) offer(b, indel, m, j + 1, S.count(), cut);
-                        }
-                        if (on1) {
-                            const uint32_t lcs = S.count();
-                            if (m < n)
-                                bestd = min(bestd, 2 * (m - lcs));
-                            else if (first_in)
-                                offer(b, indel, m, n, lcs, cut);
-                        }
-                        ++p;
-                    }
                     for (; p < pend; ++p) {
                         const uint32_t steps = min(mm, n - p);
                         const uint32_t free_steps = min(mmin, steps);
@@ -242,14 +220,32 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                         }
                         if (on1) {
                             const uint32_t lcs = S.count();
-                            if (p + m < n)
+                            if (p + m < n) {
                                 bestd = min(bestd, 2 * (m - lcs));
-                            else if (first_in)
-                                offer(b, indel, m, n - p, lcs, cut); // suffix window c[p, n)
+                            }
+                            else {
+                                // this lane's full windows are done: their best first, so that
+                                // it bounds the suffix and prefix windows
+                                if (p + m == n) offer_full(b, m, bestd, cut);
+                                if (first_in) offer(b, indel, m, n - p, lcs, cut); // suffix window c[p, n)
+                            }
                         }
                     }
                 }
-                if (on1) offer_full(b, m, bestd, cut);
+                // prefix windows c[0, i), 0 < i < m (each needs its last unit in the query):
+                // window 0 again, step by step
+                if (wave_max_u32(on1 && m > 1 ? 1u : 0u)) {
+                    const uint32_t row = lane < n ? look((uint32_t)c[co + lane]) : 0u;
+                    Lcs1<W> S;
+                    S.reset();
+                    const uint32_t steps = min(mm, n) - 1;
+                    for (uint32_t j = 0; j < steps; ++j) {
+                        W X = col[readlane(row, j) * 64];
+                        X = j < m ? X : (W)0;
+                        S.step(X);
+                        if (on1 && j + 1 < m && X != 0) offer(b, indel, m, j + 1, S.count(), cut);
+                    }
+                }
             }
 
             // ---- the choice is the needle (and the swapped pass of equal lengths) ----
@@ -263,6 +259,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                 W U = 0; // bit k: q[k] occurs in the choice
                 for (uint32_t j = 0; j < n; ++j) U |= col[readlane(rowC, j) * 64];
                 uint32_t bestd = 0xffffffffu;
+                Lcs1<W> S0; // final state of window 0
+                S0.reset();
                 for (uint32_t p = 0; p < mM; ++p) {
                     Lcs1<W> S;
                     S.reset();
@@ -272,23 +270,26 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                         Xn = col[readlane(rowC, min(j + 1, 63u)) * 64];
                         S.step((W)(Xq >> p) & maskn);
                     }
+                    if (p == 0) S0 = S;
                     if (on2 && p < m) {
                         const uint32_t lcs = S.count();
-                        if (p + n < m)
+                        if (p + n < m) {
                             bestd = min(bestd, 2 * (n - lcs));
-                        else if ((U >> p) & 1)
-                            offer(b, indel, n, m - p, lcs, cut); // suffix window q[p, m)
-                        if (p == 0) {
-                            // prefix windows q[0, i): LCS = zero bits of S below i
-                            uint32_t cum = 0;
-                            for (uint32_t i = 1; i < n; ++i) {
-                                cum += S.zero_bit(i - 1);
-                                if ((U >> (i - 1)) & 1) offer(b, indel, n, i, cum, cut);
-                            }
+                        }
+                        else {
+                            if (p + n == m) offer_full(b, n, bestd, cut); // the full windows first
+                            if ((U >> p) & 1) offer(b, indel, n, m - p, lcs, cut); // suffix window q[p, m)
                         }
                     }
                 }
-                if (on2) offer_full(b, n, bestd, cut);
+                if (on2) {
+                    // prefix windows q[0, i), 0 < i < n: LCS = zero bits of window 0's state below i
+                    uint32_t cum = 0;
+                    for (uint32_t i = 1; i < n; ++i) {
+                        cum += S0.zero_bit(i - 1);
+                        if ((U >> (i - 1)) & 1) offer(b, indel, n, i, cum, cut);
+                    }
+                }
             }
 
             double v;
