@@ -548,18 +548,19 @@ explicit CachedPostfix(const Sentence1& s1_) -> CachedPostfix<detail::sentence_c
 
 namespace fuzz {
 
-template <typename CharT1>
-struct CachedRatio {
+namespace detail {
+using namespace rapidfuzz_amd::detail;
+
+// the fuzz scorers on one cached query: similarity in [0, 100] (fuzz.hpp:82-118, 169-200)
+template <int Metric>
+struct CachedFuzz {
     template <typename InputIt1>
-    CachedRatio(InputIt1 first1, InputIt1 last1)
+    CachedFuzz(InputIt1 first1, InputIt1 last1)
     {
-        detail::Packed q;
+        Packed q;
         q.add(first1, last1);
-        set_ = std::make_shared<detail::ScorerSet>(RFA_RATIO, std::move(q), rfa_weights{1, 1, 1}, 0.1);
+        set_ = std::make_shared<ScorerSet>(Metric, std::move(q), rfa_weights{1, 1, 1}, 0.1);
     }
-    template <typename Sentence1>
-    CachedRatio(const Sentence1& s1) : CachedRatio(detail::to_begin(s1), detail::to_end(s1))
-    {}
 
     template <typename InputIt2>
     double similarity(InputIt2 first2, InputIt2 last2, double score_cutoff = 0.0, double score_hint = 0.0) const
@@ -606,11 +607,38 @@ struct CachedRatio {
     }
 
 private:
-    std::shared_ptr<detail::ScorerSet> set_;
+    std::shared_ptr<ScorerSet> set_;
+};
+
+} // namespace detail
+
+// fuzz::CachedRatio (fuzz.hpp:82-118)
+template <typename CharT1>
+struct CachedRatio : detail::CachedFuzz<RFA_RATIO> {
+    template <typename InputIt1>
+    CachedRatio(InputIt1 first1, InputIt1 last1) : detail::CachedFuzz<RFA_RATIO>(first1, last1)
+    {}
+    template <typename Sentence1>
+    CachedRatio(const Sentence1& s1) : CachedRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+};
+
+// fuzz::CachedPartialRatio (fuzz.hpp:169-200): the best ratio of the shorter string
+// against the windows of the longer one
+template <typename CharT1>
+struct CachedPartialRatio : detail::CachedFuzz<RFA_PARTIAL_RATIO> {
+    template <typename InputIt1>
+    CachedPartialRatio(InputIt1 first1, InputIt1 last1) : detail::CachedFuzz<RFA_PARTIAL_RATIO>(first1, last1)
+    {}
+    template <typename Sentence1>
+    CachedPartialRatio(const Sentence1& s1) : CachedPartialRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
 };
 
 template <typename Sentence1>
 CachedRatio(const Sentence1& s1) -> CachedRatio<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+CachedPartialRatio(const Sentence1& s1) -> CachedPartialRatio<detail::sentence_char_t<Sentence1>>;
 
 } // namespace fuzz
 

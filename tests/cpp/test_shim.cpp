@@ -90,6 +90,15 @@ int main()
             rfa::fuzz::CachedRatio<char> ratio(std::string("new york mets"));
             CHECK(near(ratio.similarity(std::string("new york mets")), 100.0));
             CHECK(near(ratio.similarity(std::string("the wonderful new york mets")), 65.0));
+            // tests-fuzz.cpp:77-79, 157-158
+            rfa::fuzz::CachedPartialRatio<char> partial(std::string("new york mets"));
+            CHECK(near(partial.similarity(std::string("new york mets")), 100.0));
+            CHECK(near(partial.similarity(std::string("the wonderful new york mets")), 100.0));
+            rfa::fuzz::CachedPartialRatio p001(std::string("001"));
+            CHECK(near(p001.similarity(std::string("220222")), 33.3333333));
+            CHECK(near(rfa::fuzz::CachedPartialRatio<char>(std::string("physics 2 vid"))
+                           .similarity(std::string("study physics physics 2 video")),
+                       100.0));
         }
         {
             // deduction guides and wide code units against narrow ones
@@ -110,12 +119,14 @@ int main()
             rfa::CachedOSA<char> osa(q);
             rfa::CachedJaroWinkler<char> jw(q);
             rfa::fuzz::CachedRatio<char> ratio(q);
+            rfa::fuzz::CachedPartialRatio<char> partial(q);
             const auto d = lev.distance_many(set);
             const auto dc = lev.distance_many(set, 5);
             const auto ns = indel.normalized_similarity_many(set, 0.6);
             const auto od = osa.distance_many(set);
             const auto js = jw.similarity_many(set);
             const auto rs = ratio.similarity_many(set, 50.0);
+            const auto ps = partial.similarity_many(set, 50.0);
             for (size_t c = 0; c < choices.size(); c += 7) {
                 CHECK(d[c] == lev.distance(choices[c]));
                 CHECK(dc[c] == lev.distance(choices[c], 5));
@@ -123,6 +134,7 @@ int main()
                 CHECK(od[c] == osa.distance(choices[c]));
                 CHECK(js[c] == jw.similarity(choices[c]));
                 CHECK(rs[c] == ratio.similarity(choices[c], 50.0));
+                CHECK(ps[c] == partial.similarity(choices[c], 50.0));
             }
             // extract(): the k best by (score desc, index asc) among scores >= cutoff
             const auto top = ratio.extract_topk(set, 5, 40.0);
