@@ -19,16 +19,19 @@
 //   distance, and distances of equal-length windows are even), so the minimum
 //   over all full windows equals the one it finds whenever that passes.
 //
-// Doubles are computed only for windows whose ratio 2 lcs / (l1 + L) is at
-// least that of the best window so far (a smaller fraction rounds to a
-// smaller or equal double).
+// A prefix or suffix window's score is a function of x = d / M alone (d its
+// Indel distance, M = l1 + L): the reference's test d <= ceil(M * c') rejects
+// only windows that its test fl(x) <= c' rejects too (for M < 2^19, M * c'
+// cannot fall a whole unit below d while x rounds to c'), and the other tests
+// compare fl(x).  So the best such window is the one with the largest ratio
+// 2 lcs / M; the kernels compare those fractions in integers and compute the
+// doubles once per pair: that window's score and the full windows' score.
 //
 // Tile kernel (queries of at most 64 units, one per lane, PM[row][lane] in LDS,
 // the choice's code units uniform across the wave):
 //   query = needle (m <= n): for every window start p of the choice, the query's
 //     PM against c[p + j], j < min(m, n - p); the window's LCS is popcount(~S).
-//     Window 0's running counts give the prefix windows (scored last, once the
-//     full windows' best bounds them).
+//     Window 0's running counts give the prefix windows.
 //   choice = needle (m >= n): the pattern of the window q[p, p + n) is the
 //     query's PM shifted right by p and masked to n bits, the text is the
 //     choice; the final state of window 0 gives every prefix window
@@ -46,49 +49,40 @@
 
 namespace rfa {
 
-// best window so far: its score (0: none) and its ratio num / den = 2 lcs / (l1 + L)
-struct PBest {
-    double v;
+// best prefix / suffix window so far by its ratio num / den = 2 lcs / (l1 + L); num = 0: none
+struct WBest {
     uint32_t num, den;
 };
 
-// CachedRatio(s1).similarity(window, cutoff) from the window's LCS (fuzz_impl.hpp:43-47)
-__device__ __forceinline__ double window_ratio(const ScoreSpec& indel, uint32_t l1, uint32_t L, uint32_t lcs,
-                                               double cut)
+__device__ __forceinline__ void offer(WBest& b, uint32_t l1, uint32_t L, uint32_t lcs)
 {
-    IntPair p{&indel, l1, L, lcs};
+    const uint32_t num = 2 * lcs, den = l1 + L;
+    if ((uint64_t)num * b.den > (uint64_t)b.num * den) {
+        b.num = num;
+        b.den = den;
+    }
+}
+
+// CachedRatio(s1).similarity(window, cutoff) of the best window (fuzz_impl.hpp:43-47,
+// 151-175): Indel distance den - num against the maximum den
+__device__ __forceinline__ double window_score(const ScoreSpec& indel, const WBest& b, double cut)
+{
+    if (b.num == 0) return 0.0;
+    IntPair p{&indel, b.den, 0, b.num / 2};
     return int_api_norm_similarity(p, cut / 100) * 100;
 }
 
-// a prefix or suffix window (fuzz_impl.hpp:151-175)
-__device__ __forceinline__ void offer(PBest& b, const ScoreSpec& indel, uint32_t l1, uint32_t L, uint32_t lcs,
-                                      double cut)
+// the full windows' best distance d (fuzz_impl.hpp:81-148); 0 when it does not count
+__device__ __forceinline__ double full_score(uint32_t l1, uint32_t d, double cut)
 {
-    if (lcs == 0) return;
-    if ((uint64_t)(2 * lcs) * b.den < (uint64_t)b.num * (l1 + L)) return;
-    const double v = window_ratio(indel, l1, L, lcs, cut);
-    if (v > b.v) {
-        b.v = v;
-        b.num = 2 * lcs;
-        b.den = l1 + L;
-    }
-}
-
-// the full windows' best distance d (fuzz_impl.hpp:81-148)
-__device__ __forceinline__ void offer_full(PBest& b, uint32_t l1, uint32_t d, double cut)
-{
-    if (d == 0xffffffffu) return;
+    if (d == 0xffffffffu) return 0.0;
     const uint64_t maximum = 2ull * l1;
     const double ncs = dmin(1.0, 1.0 - cut / 100 + 0.00001); // NormSim_to_NormDist
     const uint64_t cutoff_dist = (uint64_t)ceil((double)maximum * ncs);
-    if ((uint64_t)d >= cutoff_dist) return;
+    if ((uint64_t)d >= cutoff_dist) return 0.0;
     double score = 1.0 - (double)d / (double)maximum;
     score *= 100;
-    if (score >= cut && score > b.v) {
-        b.v = score;
-        b.num = (uint32_t)(maximum - d);
-        b.den = (uint32_t)maximum;
-    }
+    return score >= cut ? score : 0.0;
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
@@ -103,13 +97,6 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, 64));
     return uniform(v);
-}
-
-__device__ __forceinline__ double wave_max_f64(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
-    return v;
 }
 
 // one-word LCS state (LCSseq_impl.hpp:219-252); 64-bit words as two halves
@@ -174,7 +161,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
             const uint64_t co0 = a.coff[cidx];
             const uint32_t n = uniform((uint32_t)(a.coff[cidx + 1] - co0));
             const uint64_t co = ((uint64_t)uniform((uint32_t)(co0 >> 32)) << 32) | uniform((uint32_t)co0);
-            PBest b{0.0, 0u, 1u};
+            WBest b{0u, 1u};
+            uint32_t bestd1 = 0xffffffffu, bestd2 = 0xffffffffu; // full windows of each case
 
             // ---- the query is the needle: windows of the choice ----
             const bool on1 = qact && m > 0 && m <= n && cut <= 100;
@@ -182,7 +170,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
             if (mm) {
                 // steps below the shortest participating query need no per-lane length mask
                 const uint32_t mmin = 64 - wave_max_u32(on1 ? 64 - m : 0u);
-                uint32_t bestd = 0xffffffffu;
                 // one register holds the rows of c[k0, k0 + 64): windows k0 .. k0 + 64 - mm
                 // never read past it
                 const uint32_t wstep = 65 - mm;
@@ -220,15 +207,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                         }
                         if (on1) {
                             const uint32_t lcs = S.count();
-                            if (p + m < n) {
-                                bestd = min(bestd, 2 * (m - lcs));
-                            }
-                            else {
-                                // this lane's full windows are done: their best first, so that
-                                // it bounds the suffix and prefix windows
-                                if (p + m == n) offer_full(b, m, bestd, cut);
-                                if (first_in) offer(b, indel, m, n - p, lcs, cut); // suffix window c[p, n)
-                            }
+                            if (p + m < n)
+                                bestd1 = min(bestd1, 2 * (m - lcs));
+                            else if (first_in)
+                                offer(b, m, n - p, lcs); // suffix window c[p, n)
                         }
                     }
                 }
@@ -243,7 +225,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                         W X = col[readlane(row, j) * 64];
                         X = j < m ? X : (W)0;
                         S.step(X);
-                        if (on1 && j + 1 < m && X != 0) offer(b, indel, m, j + 1, S.count(), cut);
+                        if (on1 && j + 1 < m && X != 0) offer(b, m, j + 1, S.count());
                     }
                 }
             }
@@ -258,7 +240,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                 const W maskn = n >= BITS ? ~(W)0 : (((W)1 << n) - 1);
                 W U = 0; // bit k: q[k] occurs in the choice
                 for (uint32_t j = 0; j < n; ++j) U |= col[readlane(rowC, j) * 64];
-                uint32_t bestd = 0xffffffffu;
                 Lcs1<W> S0; // final state of window 0
                 S0.reset();
                 for (uint32_t p = 0; p < mM; ++p) {
@@ -273,13 +254,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                     if (p == 0) S0 = S;
                     if (on2 && p < m) {
                         const uint32_t lcs = S.count();
-                        if (p + n < m) {
-                            bestd = min(bestd, 2 * (n - lcs));
-                        }
-                        else {
-                            if (p + n == m) offer_full(b, n, bestd, cut); // the full windows first
-                            if ((U >> p) & 1) offer(b, indel, n, m - p, lcs, cut); // suffix window q[p, m)
-                        }
+                        if (p + n < m)
+                            bestd2 = min(bestd2, 2 * (n - lcs));
+                        else if ((U >> p) & 1)
+                            offer(b, n, m - p, lcs); // suffix window q[p, m)
                     }
                 }
                 if (on2) {
@@ -287,7 +265,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
                     uint32_t cum = 0;
                     for (uint32_t i = 1; i < n; ++i) {
                         cum += S0.zero_bit(i - 1);
-                        if ((U >> (i - 1)) & 1) offer(b, indel, n, i, cum, cut);
+                        if ((U >> (i - 1)) & 1) offer(b, n, i, cum);
                     }
                 }
             }
@@ -298,7 +276,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
             else if (m == 0 || n == 0)
                 v = (m == n) ? 100.0 : 0.0;
             else
-                v = b.v;
+                v = dmax(window_score(indel, b, cut), dmax(full_score(m, bestd1, cut), full_score(n, bestd2, cut)));
             union {
                 double d;
                 uint64_t u;
@@ -356,11 +334,10 @@ __device__ __forceinline__ uint32_t zeros(const uint64_t (&S)[WMAX], uint32_t nw
 // the query (m units) is the needle, m <= n: window p = lane + 64k of the choice
 template <int WMAX, typename CharT, bool LDS>
 __device__ void plong_query_needle(const Pattern<CharT, LDS>& P, uint32_t m, const CharT* __restrict__ T, uint32_t n,
-                                   const ScoreSpec& indel, double cut, PBest& b)
+                                   WBest& b, uint32_t& bestd)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nw = (m + 63) / 64;
-    uint32_t bestd = 0xffffffffu;
     for (uint32_t p0 = 0; p0 < n; p0 += 64) {
         const uint32_t p = p0 + lane;
         const bool act = p < n;
@@ -378,29 +355,27 @@ __device__ void plong_query_needle(const Pattern<CharT, LDS>& P, uint32_t m, con
             for (int w = 0; w < WMAX; ++w) X[w] = (uint32_t)w < nw ? row[w] : 0ull;
             lcs_words<WMAX>(S, X, nw);
             // prefix window c[0, j + 1) (lane 0 holds window 0)
-            if (p == 0 && j + 1 < m && r != 0) offer(b, indel, m, j + 1, zeros<WMAX>(S, nw), cut);
+            if (p == 0 && j + 1 < m && r != 0) offer(b, m, j + 1, zeros<WMAX>(S, nw));
         }
         if (act) {
             const uint32_t lcs = zeros<WMAX>(S, nw);
             if (p + m < n)
                 bestd = min(bestd, 2 * (m - lcs));
             else if (first_in)
-                offer(b, indel, m, n - p, lcs, cut);
+                offer(b, m, n - p, lcs);
         }
     }
-    offer_full(b, m, wave_min_u32(bestd), cut);
 }
 
 // the choice (n units) is the needle, n <= m: window q[p, p + n) with p = lane + 64k,
 // its pattern the query's PM from bit p, masked to n bits; the text is the choice
 template <int WMAX, typename CharT, bool LDS>
 __device__ void plong_choice_needle(const Pattern<CharT, LDS>& P, uint32_t m, const CharT* __restrict__ T, uint32_t n,
-                                    const ScoreSpec& indel, double cut, PBest& b)
+                                    WBest& b, uint32_t& bestd)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nw = (n + 63) / 64, qw = P.W;
     const uint64_t last = (n & 63) ? ((1ull << (n & 63)) - 1) : ~0ull;
-    uint32_t bestd = 0xffffffffu;
     for (uint32_t p0 = 0; p0 < m; p0 += 64) {
         const uint32_t p = p0 + lane;
         const bool act = p < m;
@@ -434,7 +409,7 @@ __device__ void plong_choice_needle(const Pattern<CharT, LDS>& P, uint32_t m, co
             if (p + n < m)
                 bestd = min(bestd, 2 * (n - lcs));
             else if (U[0] & 1)
-                offer(b, indel, n, m - p, lcs, cut); // suffix window q[p, m), q[p] in the choice
+                offer(b, n, m - p, lcs); // suffix window q[p, m), q[p] in the choice
             if (p == 0) {
                 // prefix windows q[0, i), 0 < i < n
                 uint32_t cum = 0;
@@ -446,14 +421,13 @@ __device__ void plong_choice_needle(const Pattern<CharT, LDS>& P, uint32_t m, co
                             const uint32_t i = (uint32_t)w * 64 + k + 1;
                             if (i >= n) break;
                             cum += (uint32_t)((z >> k) & 1);
-                            if ((u >> k) & 1) offer(b, indel, n, i, cum, cut);
+                            if ((u >> k) & 1) offer(b, n, i, cum);
                         }
                     }
                 }
             }
         }
     }
-    offer_full(b, n, wave_min_u32(bestd), cut);
 }
 
 template <typename CharT, int WMAX, bool LDS>
@@ -492,10 +466,22 @@ __global__ __launch_bounds__(64) void k_plong(PartialLongArgs a)
             v = (m == n) ? 100.0 : 0.0;
         }
         else {
-            PBest b{0.0, 0u, 1u};
-            if (m <= n) plong_query_needle<WMAX>(P, m, C + co, n, indel, cut, b);
-            if (m >= n) plong_choice_needle<WMAX>(P, m, C + co, n, indel, cut, b);
-            v = wave_max_f64(b.v);
+            WBest b{0u, 1u};
+            uint32_t bestd1 = 0xffffffffu, bestd2 = 0xffffffffu;
+            if (m <= n) plong_query_needle<WMAX>(P, m, C + co, n, b, bestd1);
+            if (m >= n) plong_choice_needle<WMAX>(P, m, C + co, n, b, bestd2);
+            // the wave's best window ratio and full-window distances
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint32_t on = (uint32_t)__shfl_xor(b.num, off, 64), od = (uint32_t)__shfl_xor(b.den, off, 64);
+                if ((uint64_t)on * b.den > (uint64_t)b.num * od) {
+                    b.num = on;
+                    b.den = od;
+                }
+            }
+            bestd1 = wave_min_u32(bestd1);
+            bestd2 = wave_min_u32(bestd2);
+            v = dmax(window_score(indel, b, cut), dmax(full_score(m, bestd1, cut), full_score(n, bestd2, cut)));
         }
         if (lane == 0) a.out[(uint64_t)qi * a.out_stride + ci] = v;
     }

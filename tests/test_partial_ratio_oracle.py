@@ -136,3 +136,32 @@ def test_each_window_rule_is_pinned(oracle):
 
 def test_swapped_pass_vs_reference(reference):
     assert float(reference.cdist(PARTIAL, SIM, [b"aaa"], [b"aba"])[0, 0]) == 80.0
+
+
+def test_window_score_depends_on_the_fraction_only(oracle):
+    """The kernels score one window per pair: the one with the largest 2 lcs / (l1 + L).
+    That is exact when CachedRatio's score of a window is a non-decreasing function of
+    x = 1 - d / M alone (d = l1 + L - 2 lcs, M = l1 + L), so that equal fractions score
+    equal doubles and a smaller fraction never scores more.  Checked on every
+    (l1, L, lcs) with l1, L <= 18 and cutoffs that put 1 - c/100 + 1e-5 on and beside
+    such fractions."""
+    cuts = [0.0, 25.0, 50.0, 60.0, 75.0, 80.0, 100.0 * (1 - 1 / 3) + 0.001,
+            100.0 * (1 - 0.3) - 0.001, 100.0 * (1 - 0.25 + 0.00001), 100.0 * (1 - 7 / 15 + 0.00001)]
+    cases = []
+    for l1 in range(1, 19):
+        for L in range(1, 19):
+            for lcs in range(0, min(l1, L) + 1):
+                cases.append((l1, L, lcs))
+    queries = [np.array([97] * lcs + [98] * (l1 - lcs), np.uint32) for l1, L, lcs in cases]
+    windows = [np.array([97] * lcs + [99] * (L - lcs), np.uint32) for l1, L, lcs in cases]
+    from fractions import Fraction
+    for cut in cuts:
+        by_fraction = {}
+        for (l1, L, lcs), a, b in zip(cases, queries, windows):
+            v = float(oracle.score(RATIO, SIM, a, b, cutoff_d=cut))
+            by_fraction.setdefault(Fraction(2 * lcs, l1 + L), set()).add(v)
+        fracs = sorted(by_fraction)
+        for f in fracs:
+            assert len(by_fraction[f]) == 1, (cut, f, by_fraction[f])
+        vals = [next(iter(by_fraction[f])) for f in fracs]
+        assert all(x <= y for x, y in zip(vals, vals[1:])), cut
