@@ -53,8 +53,13 @@ typedef enum {
     RFA_HAMMING = 7,      /* CachedHamming (Hamming.hpp:131-161) with pad = true; Cached semantics only */
     RFA_PREFIX = 8,       /* CachedPrefix (Prefix.hpp:65-94); Cached semantics only */
     RFA_POSTFIX = 9,      /* CachedPostfix (Postfix.hpp:65-94); Cached semantics only */
-    RFA_PARTIAL_RATIO = 10 /* fuzz::CachedPartialRatio (fuzz.hpp:169-200, fuzz_impl.hpp:57-279): best ratio of
-                              the shorter string against the longer one's windows; similarity, Cached only */
+    RFA_PARTIAL_RATIO = 10, /* fuzz::CachedPartialRatio (fuzz.hpp:169-200, fuzz_impl.hpp:57-279): best ratio of
+                               the shorter string against the longer one's windows; similarity, Cached only */
+    /* ratio / partial_ratio of the token-sorted strings (each string split at whitespace, tokens
+     * sorted, joined with single spaces: detail::sorted_split(...).join()); similarity, Cached only */
+    RFA_TOKEN_SORT_RATIO = 11,         /* fuzz::CachedTokenSortRatio (fuzz.hpp:282-304) */
+    RFA_PARTIAL_TOKEN_SORT_RATIO = 12, /* fuzz::CachedPartialTokenSortRatio (fuzz.hpp:342-364) */
+    RFA_QRATIO = 13                    /* fuzz::CachedQRatio (fuzz.hpp:776-796): ratio, 0 if either string is empty */
 } rfa_metric;
 
 /* Scorer method (details/distance.hpp:211-301, 326-356). */
@@ -137,6 +142,12 @@ uint64_t rfa_strings_count(const rfa_strings* set);
 rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* queries, const rfa_weights* weights,
                              double prefix_weight, rfa_scorer** out);
 rfa_status rfa_scorer_destroy(rfa_scorer* scorer);
+/* Scorer options (no reference counterpart).
+ * RFA_SCORER_SIGNED_UNITS (default 0): 1 = 8-bit code units order as signed values when
+ * tokens are sorted (the reference's comparison for `char` strings, Range.hpp:214-218);
+ * 0 = as unsigned values (uint8_t strings, and every 16/32-bit width). */
+#define RFA_SCORER_SIGNED_UNITS 1
+rfa_status rfa_scorer_set_option(rfa_scorer* scorer, int option, int64_t value);
 
 /* All queries against all choices: out[q * n_choices + c] is
  *   Cached<Metric>(query q).<kind>(choice c, cutoff)        (semantics RFA_CACHED)

@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "librf_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "librfref.so")
 
-LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX, PARTIAL_RATIO = range(11)
+(LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX, PARTIAL_RATIO, TOKEN_SORT_RATIO,
+ PARTIAL_TOKEN_SORT_RATIO, QRATIO) = range(14)
 DISTANCE, SIMILARITY, NORM_DISTANCE, NORM_SIMILARITY = range(4)
 INT64_MAX = (1 << 63) - 1
 
@@ -75,6 +76,10 @@ class Oracle:
         L.orc_pairs.restype = ctypes.c_int
         L.orc_pairs.argtypes = [_i32p, ctypes.c_int, _u32p, _u64p, _u32p, _u64p, ctypes.c_size_t,
                                 ctypes.c_uint64, ctypes.c_double, _u64p, _f64p]
+        L.orc_set_units.restype = None
+        L.orc_set_units.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_sorted_join.restype = ctypes.c_size_t
+        L.orc_sorted_join.argtypes = [_u32p, ctypes.c_size_t, _u32p]
         L.orc_jaro_raw.restype = None
         L.orc_jaro_raw.argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _u64p, _u64p]
         for name in ("orc_levenshtein_raw",):
@@ -86,7 +91,10 @@ class Oracle:
             getattr(L, name).argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t]
 
     def cdist(self, metric, kind, queries, choices, weights=(1, 1, 1), prefix_weight=0.1,
-              cutoff_u=None, cutoff_d=None, multi=False):
+              cutoff_u=None, cutoff_d=None, multi=False, char_width=4, signed8=False):
+        """char_width / signed8: the code units the token scorers split and order (the
+        reference's whitespace set of 8-bit units, and `char`'s signed order)."""
+        self.lib.orc_set_units(char_width, int(signed8))
         if cutoff_u is None:
             cutoff_u = default_int_cutoff(kind)
         qc, qo = pack(queries)
@@ -105,6 +113,14 @@ class Oracle:
 
     def score(self, metric, kind, s1, s2, **kw):
         return self.cdist(metric, kind, [s1], [s2], **kw)[0, 0]
+
+    def sorted_join(self, s, char_width=4, signed8=False):
+        """detail::sorted_split(s).join() as uint32 code units."""
+        self.lib.orc_set_units(char_width, int(signed8))
+        a = np.ascontiguousarray(to_codes(s), dtype=np.uint32)
+        out = np.zeros(max(1, len(a)), dtype=np.uint32)
+        n = self.lib.orc_sorted_join(_p(a, _u32p), len(a), _p(out, _u32p))
+        return out[:n]
 
     def pairs(self, metrics, a, b, kind=DISTANCE, cutoff_u=INT64_MAX, cutoff_d=1.0):
         ac, ao = pack(a)

@@ -25,7 +25,7 @@
 
 namespace {
 
-enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10 };
+enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10, TOKEN_SORT = 11, PARTIAL_TOKEN_SORT = 12, QRATIO = 13 };
 enum { DIST = 0, SIM = 1, NDIST = 2, NSIM = 3 };
 
 thread_local std::string g_err;
@@ -80,6 +80,9 @@ struct AnyCached {
     rapidfuzz::CachedPrefix<CharT>* pre = nullptr;
     rapidfuzz::CachedPostfix<CharT>* post = nullptr;
     rapidfuzz::fuzz::CachedPartialRatio<CharT>* partial = nullptr;
+    rapidfuzz::fuzz::CachedTokenSortRatio<CharT>* tsort = nullptr;
+    rapidfuzz::fuzz::CachedPartialTokenSortRatio<CharT>* ptsort = nullptr;
+    rapidfuzz::fuzz::CachedQRatio<CharT>* qratio = nullptr;
 
     AnyCached(int m, std::vector<CharT> q, rapidfuzz::LevenshteinWeightTable w_, double pw_)
         : metric(m), s1(std::move(q)), w(w_), pw(pw_)
@@ -96,6 +99,9 @@ struct AnyCached {
         case PREFIX: pre = new rapidfuzz::CachedPrefix<CharT>(s1); break;
         case POSTFIX: post = new rapidfuzz::CachedPostfix<CharT>(s1); break;
         case PARTIAL: partial = new rapidfuzz::fuzz::CachedPartialRatio<CharT>(s1); break;
+        case TOKEN_SORT: tsort = new rapidfuzz::fuzz::CachedTokenSortRatio<CharT>(s1); break;
+        case PARTIAL_TOKEN_SORT: ptsort = new rapidfuzz::fuzz::CachedPartialTokenSortRatio<CharT>(s1); break;
+        case QRATIO: qratio = new rapidfuzz::fuzz::CachedQRatio<CharT>(s1); break;
         default: throw std::invalid_argument("unknown metric");
         }
     }
@@ -105,6 +111,9 @@ struct AnyCached {
         delete pre;
         delete post;
         delete partial;
+        delete tsort;
+        delete ptsort;
+        delete qratio;
         delete lev;
         delete indel;
         delete lcs;
@@ -133,6 +142,18 @@ struct AnyCached {
         case PARTIAL:
             if (kind != SIM) return -1;
             *od = partial->similarity(b, e, cd);
+            return 1;
+        case TOKEN_SORT:
+            if (kind != SIM) return -1;
+            *od = tsort->similarity(b, e, cd);
+            return 1;
+        case PARTIAL_TOKEN_SORT:
+            if (kind != SIM) return -1;
+            *od = ptsort->similarity(b, e, cd);
+            return 1;
+        case QRATIO:
+            if (kind != SIM) return -1;
+            *od = qratio->similarity(b, e, cd);
             return 1;
         }
         return -1;
@@ -379,13 +400,16 @@ extern "C" {
 
 const char* ref_last_error() { return g_err.c_str(); }
 
-// Cached* cdist over arbitrary code units; char_width picks CharT (1, 2 or 4).
+// Cached* cdist over arbitrary code units; char_width picks CharT (1, 2 or 4; -1 for char).
 int ref_cdist_cached(int metric, int kind, int char_width, const uint32_t* q, const uint64_t* qoff,
                      size_t nq, const uint32_t* c, const uint64_t* coff, size_t nc, uint64_t ins,
                      uint64_t del, uint64_t rep, double pw, uint64_t cu, double cd, uint64_t* ou,
                      double* od)
 {
     try {
+        if (char_width == -1) // `char`: signed 8-bit units
+            return cdist_cached_impl<char>(metric, kind, q, qoff, nq, c, coff, nc, ins, del, rep, pw, cu, cd, ou,
+                                           od);
         if (char_width == 1)
             return cdist_cached_impl<uint8_t>(metric, kind, q, qoff, nq, c, coff, nc, ins, del, rep, pw, cu,
                                               cd, ou, od);

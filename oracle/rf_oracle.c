@@ -580,6 +580,87 @@ static double partial_ratio_impl(const uint32_t* s1, size_t len1, const uint32_t
     return res;
 }
 
+/* ------------------------------------------------------------------------ */
+/* token sort (common_impl.hpp:84-179, SplittedSentenceView.hpp:70-85)      */
+/* ------------------------------------------------------------------------ */
+
+static int g_unit_width = 4;
+static int g_signed8 = 0;
+
+void orc_set_units(int width, int signed8)
+{
+    g_unit_width = width;
+    g_signed8 = signed8;
+}
+
+/* detail::is_space for the configured unit width */
+static int orc_is_space(uint32_t c)
+{
+    switch (c) {
+    case 0x0009: case 0x000A: case 0x000B: case 0x000C: case 0x000D:
+    case 0x001C: case 0x001D: case 0x001E: case 0x001F: case 0x0020: return 1;
+    }
+    if (g_unit_width == 1) return 0;
+    switch (c) {
+    case 0x0085: case 0x00A0: case 0x1680: case 0x2000: case 0x2001: case 0x2002: case 0x2003:
+    case 0x2004: case 0x2005: case 0x2006: case 0x2007: case 0x2008: case 0x2009: case 0x200A:
+    case 0x2028: case 0x2029: case 0x202F: case 0x205F: case 0x3000: return 1;
+    }
+    return 0;
+}
+
+static long long unit_key(uint32_t c) { return (g_unit_width == 1 && g_signed8) ? (long long)(signed char)c : (long long)c; }
+
+/* Range operator< (Range.hpp:214-218): std::lexicographical_compare of the units */
+static int tok_less(const uint32_t* a, size_t la, const uint32_t* b, size_t lb)
+{
+    size_t l = la < lb ? la : lb;
+    for (size_t k = 0; k < l; ++k) {
+        long long x = unit_key(a[k]), y = unit_key(b[k]);
+        if (x != y) return x < y;
+    }
+    return la < lb;
+}
+
+size_t orc_sorted_join(const uint32_t* s, size_t n, uint32_t* out)
+{
+    size_t* st = (size_t*)malloc((n + 1) * sizeof(size_t));
+    size_t* ln = (size_t*)malloc((n + 1) * sizeof(size_t));
+    size_t T = 0;
+    /* the split loop: tokens are the runs of non-space units */
+    size_t i = 0;
+    while (i < n) {
+        while (i < n && orc_is_space(s[i])) ++i;
+        if (i == n) break;
+        size_t e = i;
+        while (e < n && !orc_is_space(s[e])) ++e;
+        st[T] = i;
+        ln[T] = e - i;
+        ++T;
+        i = e;
+    }
+    /* std::sort: equal tokens are identical, so any correct sort gives the same join */
+    for (size_t a = 1; a < T; ++a) {
+        size_t ks = st[a], kl = ln[a], b = a;
+        while (b > 0 && tok_less(s + ks, kl, s + st[b - 1], ln[b - 1])) {
+            st[b] = st[b - 1];
+            ln[b] = ln[b - 1];
+            --b;
+        }
+        st[b] = ks;
+        ln[b] = kl;
+    }
+    size_t len = 0;
+    for (size_t t = 0; t < T; ++t) {
+        if (t) out[len++] = 0x20;
+        memcpy(out + len, s + st[t], ln[t] * sizeof(uint32_t));
+        len += ln[t];
+    }
+    free(st);
+    free(ln);
+    return len;
+}
+
 /* partial_ratio_alignment(...).score (fuzz_impl.hpp:193-228); CachedPartialRatio::similarity
  * (fuzz_impl.hpp:255-279) returns the same value. */
 double orc_partial_ratio(const uint32_t* s1, size_t len1, const uint32_t* s2, size_t len2, double score_cutoff)
@@ -637,6 +718,41 @@ int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const 
     case ORC_PARTIAL_RATIO:
         if (kind != ORC_SIMILARITY) return -1;
         *out_d = orc_partial_ratio(s1, l1, s2, l2, cutoff_d);
+        return 1;
+    case ORC_TOKEN_SORT_RATIO:
+    case ORC_PARTIAL_TOKEN_SORT_RATIO: {
+        /* CachedTokenSortRatio / CachedPartialTokenSortRatio::similarity (fuzz_impl.hpp:314-322,
+         * 353-362): ratio / partial_ratio of the sorted-token joins */
+        if (kind != ORC_SIMILARITY) return -1;
+        if (cutoff_d > 100) {
+            *out_d = 0;
+            return 1;
+        }
+        uint32_t* a = (uint32_t*)malloc((l1 + 1) * sizeof(uint32_t));
+        uint32_t* b = (uint32_t*)malloc((l2 + 1) * sizeof(uint32_t));
+        size_t la = orc_sorted_join(s1, l1, a), lb = orc_sorted_join(s2, l2, b);
+        if (metric == ORC_TOKEN_SORT_RATIO) {
+            Pair q = {ORC_INDEL, a, b, la, lb, 1, 1, 2, 0.0};
+            *out_d = int_api_norm_similarity(&q, cutoff_d / 100) * 100;
+        }
+        else {
+            *out_d = orc_partial_ratio(a, la, b, lb, cutoff_d);
+        }
+        free(a);
+        free(b);
+        return 1;
+    }
+    case ORC_QRATIO:
+        /* CachedQRatio::similarity (fuzz_impl.hpp:912-922) */
+        if (kind != ORC_SIMILARITY) return -1;
+        if (!l1 || !l2) {
+            *out_d = 0;
+            return 1;
+        }
+        p.metric = ORC_INDEL;
+        p.ins = p.del = 1;
+        p.rep = 2;
+        *out_d = int_api_norm_similarity(&p, cutoff_d / 100) * 100;
         return 1;
     }
     return -1;

@@ -37,7 +37,10 @@ enum {
     ORC_HAMMING = 7, /* CachedHamming with pad = true */
     ORC_PREFIX = 8,
     ORC_POSTFIX = 9,
-    ORC_PARTIAL_RATIO = 10 /* fuzz::CachedPartialRatio: similarity only */
+    ORC_PARTIAL_RATIO = 10, /* fuzz::CachedPartialRatio: similarity only */
+    ORC_TOKEN_SORT_RATIO = 11,
+    ORC_PARTIAL_TOKEN_SORT_RATIO = 12,
+    ORC_QRATIO = 13
 };
 
 enum { ORC_DISTANCE = 0, ORC_SIMILARITY = 1, ORC_NORM_DISTANCE = 2, ORC_NORM_SIMILARITY = 3 };
@@ -56,6 +59,12 @@ void orc_jaro_raw(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, ui
 double orc_jaro_similarity(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl, double cutoff);
 double orc_jaro_winkler_similarity(const uint32_t* P, size_t Pl, const uint32_t* T, size_t Tl,
                                    double prefix_weight, double cutoff);
+
+/* Code units of the token scorers: width 1, 2 or 4 (1 uses the reference's 8-bit whitespace
+ * set) and, for width 1, signed comparison (the reference's `char`).  Default 4, unsigned. */
+void orc_set_units(int width, int signed8);
+/* detail::sorted_split(s).join() into out (>= n units); returns its length */
+size_t orc_sorted_join(const uint32_t* s, size_t n, uint32_t* out);
 
 /* fuzz::partial_ratio(s1, s2, cutoff): the reference's window search, step by step */
 double orc_partial_ratio(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, double cutoff);

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -54,7 +55,10 @@ int algo_for(int metric, const rfa_weights& w)
     case RFA_HAMMING:
     case RFA_PREFIX:
     case RFA_POSTFIX: return ALGO_POSIT;
-    case RFA_PARTIAL_RATIO: return ALGO_PARTIAL;
+    case RFA_PARTIAL_RATIO:
+    case RFA_PARTIAL_TOKEN_SORT_RATIO: return ALGO_PARTIAL;
+    case RFA_TOKEN_SORT_RATIO:
+    case RFA_QRATIO: return ALGO_LCS;
     case RFA_JARO:
     case RFA_JARO_WINKLER: return ALGO_JARO;
     }
@@ -171,10 +175,12 @@ rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const 
 {
     if (width != 1 && width != 2 && width != 4) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "char_width must be 1, 2 or 4");
     if (!off && count) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "offsets is NULL");
+    static std::atomic<uint64_t> next_id{1};
     auto* s = new rfa_strings();
     s->ctx = ctx;
     s->width = width;
     s->count = count;
+    s->id = next_id.fetch_add(1);
     s->h_off.assign(off, off + count + 1);
     if (count == 0) s->h_off.assign(1, 0);
     for (uint64_t i = 0; i < count; ++i) {
@@ -215,16 +221,110 @@ void free_strings(rfa_strings* s)
     delete s;
 }
 
+rfa_strings* token_sorted(rfa_context* ctx, const rfa_strings* in, bool signed8)
+{
+    hipStream_t s = ctx->stream;
+    const uint64_t n = in->count, w = (uint64_t)in->width;
+    const uint64_t units = in->h_off.back();
+    const size_t scan = tokens_scan_bytes(n);
+    // one allocation: out_len, ntok, out_off, tok_off (n + 1 each), tok_start, tok_len, out units, scan temp
+    const size_t cnt = (size_t)(n + 1);
+    const size_t tok = (size_t)std::max<uint64_t>(1, units);
+    size_t bytes = 4 * cnt * 8 + 2 * tok * 4 + tok * w + 64 + 256 + scan;
+    uint8_t* base = nullptr;
+    if (hipMalloc(&base, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        throw ApiError(RFA_ERR_OUT_OF_MEMORY, "device allocation for the token-sorted strings failed");
+    }
+    TokenArgs a{};
+    a.chars = in->d_chars;
+    a.off = in->d_off;
+    a.count = n;
+    a.out_len = (uint64_t*)base;
+    a.ntok = a.out_len + cnt;
+    a.out_off = a.ntok + cnt;
+    a.tok_off = a.out_off + cnt;
+    a.tok_start = (uint32_t*)(a.tok_off + cnt);
+    a.tok_len = a.tok_start + tok;
+    a.out = (void*)(a.tok_len + tok);
+    a.scan_temp = (void*)(((uintptr_t)((uint8_t*)a.out + tok * w + 64) + 255) & ~(uintptr_t)255);
+    a.scan_bytes = scan;
+    a.signed8 = signed8 ? 1 : 0;
+    a.num_cus = (uint32_t)ctx->num_cus;
+    std::vector<uint64_t> h_off(cnt);
+    std::vector<uint8_t> h_chars;
+    hipError_t e = hipMemsetAsync(a.out_len + n, 0, 8, s);
+    if (e == hipSuccess) e = hipMemsetAsync(a.ntok + n, 0, 8, s);
+    if (e == hipSuccess) e = launch_tokens(in->width, a, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_off.data(), a.out_off, cnt * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) {
+        h_chars.resize(h_off[n] * w);
+        if (!h_chars.empty()) e = hipMemcpy(h_chars.data(), a.out, h_chars.size(), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(base);
+    if (e != hipSuccess) throw ApiError(RFA_ERR_HIP, std::string("token sort: ") + hipGetErrorString(e));
+    return make_strings(ctx, h_chars.data(), in->width, h_off.data(), n);
+}
+
+const rfa_strings* scorer_choices(rfa_scorer* sc, const rfa_strings* ch)
+{
+    if (!sc->token_sort) return ch;
+    if (!sc->tok_choices || sc->tok_choices_id != ch->id || sc->tok_choices_signed8 != sc->signed8) {
+        rfa_strings* t = token_sorted(sc->ctx, ch, sc->signed8);
+        free_strings(sc->tok_choices);
+        sc->tok_choices = t;
+        sc->tok_choices_id = ch->id;
+        sc->tok_choices_signed8 = sc->signed8;
+    }
+    return sc->tok_choices;
+}
+
+// drops a scorer's plan (its query set and tile / block lists)
+static void clear_plan(rfa_scorer* sc)
+{
+    free_strings(sc->q);
+    sc->q = nullptr;
+    if (sc->d_tile_order) (void)hipFree(sc->d_tile_order);
+    if (sc->d_block_queries) (void)hipFree(sc->d_block_queries);
+    sc->d_tile_order = nullptr;
+    sc->d_block_queries = nullptr;
+    sc->tile_order.clear();
+    sc->classes.clear();
+    sc->block_queries.clear();
+    sc->block_max_len = sc->block_min_len = 0;
+    sc->block_u8_rows = 0;
+    sc->hash_bits = 6;
+    sc->few_plan.reset();
+}
+
 rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries, const rfa_weights& w,
                         double prefix_weight)
 {
     auto* sc = new rfa_scorer();
     sc->ctx = ctx;
+    sc->api_metric = metric;
     sc->metric = metric;
     sc->w = w;
     sc->prefix_weight = prefix_weight;
+    // token-sort scorers run ratio / partial_ratio on token-sorted copies, QRatio runs ratio
+    if (metric == RFA_TOKEN_SORT_RATIO || metric == RFA_PARTIAL_TOKEN_SORT_RATIO) {
+        sc->metric = metric == RFA_TOKEN_SORT_RATIO ? RFA_RATIO : RFA_PARTIAL_RATIO;
+        sc->token_sort = true;
+    }
+    else if (metric == RFA_QRATIO) {
+        sc->metric = RFA_RATIO;
+        sc->empty_zero = true;
+    }
     try {
-        sc->q = make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
+        if (sc->token_sort) {
+            sc->q_raw =
+                make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
+            sc->q = token_sorted(ctx, sc->q_raw, false);
+        }
+        else {
+            sc->q = make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
+        }
         plan_scorer(sc);
     }
     catch (...) {
@@ -237,10 +337,21 @@ rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries
 void free_scorer(rfa_scorer* sc)
 {
     if (!sc) return;
-    free_strings(sc->q);
-    if (sc->d_tile_order) (void)hipFree(sc->d_tile_order);
-    if (sc->d_block_queries) (void)hipFree(sc->d_block_queries);
+    clear_plan(sc);
+    free_strings(sc->q_raw);
+    free_strings(sc->tok_choices);
     delete sc;
+}
+
+// RFA_SCORER_SIGNED_UNITS on a token-sort scorer: the queries are sorted again
+void scorer_set_signed(rfa_scorer* sc, bool on)
+{
+    if (sc->signed8 == on) return;
+    sc->signed8 = on;
+    if (!sc->token_sort) return;
+    clear_plan(sc);
+    sc->q = token_sorted(sc->ctx, sc->q_raw, on);
+    plan_scorer(sc);
 }
 
 ScoreSpec make_spec_m(int metric, const rfa_weights& w, double pw, int kind, int semantics, uint64_t cu, double cd,
@@ -276,6 +387,7 @@ ScoreSpec make_spec(const rfa_scorer* sc, int kind, int semantics, uint64_t cu, 
     s.algo = algo;
     s.cutoff_u = cu;
     s.cutoff_d = cd;
+    s.empty_zero = sc->empty_zero ? 1 : 0;
     s.ins = sc->w.insert_cost;
     s.del = sc->w.delete_cost;
     s.rep = sc->w.replace_cost;
@@ -438,6 +550,8 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
 {
     rfa_context* ctx = sc->ctx;
     validate_kind(sc->metric, kind, semantics);
+    if (semantics == RFA_MULTI && sc->api_metric != sc->metric)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "token_sort_ratio, partial_token_sort_ratio and QRatio have no Multi* scorer");
     const uint64_t* d_qcut = nullptr;
     if (qcut_host) {
         if (!is_int_kind(sc->metric, kind))
@@ -929,8 +1043,7 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
 {
     if (!ctx || !queries || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, queries or out is NULL");
     return guarded([&] {
-        if (metric < RFA_LEVENSHTEIN || metric > RFA_PARTIAL_RATIO)
-            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
+        if (metric < RFA_LEVENSHTEIN || metric > RFA_QRATIO) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
         if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};
         // weights MultiLevenshtein accepts are kept even when Cached* scoring cannot use them
@@ -940,6 +1053,19 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_CHECK(hipSetDevice(ctx->device));
         *out = make_scorer(ctx, metric, queries, w, prefix_weight);
+    });
+}
+
+rfa_status rfa_scorer_set_option(rfa_scorer* sc, int option, int64_t value)
+{
+    if (!sc) return fail(RFA_ERR_INVALID_ARGUMENT, "scorer is NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(sc->ctx->mu);
+        HIP_CHECK(hipSetDevice(sc->ctx->device));
+        if (option != RFA_SCORER_SIGNED_UNITS) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown scorer option");
+        if (value != 0 && value != 1) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "RFA_SCORER_SIGNED_UNITS is 0 or 1");
+        begin_call(sc->ctx);
+        scorer_set_signed(sc, value == 1);
     });
 }
 
@@ -963,7 +1089,9 @@ rfa_status rfa_cdist(rfa_scorer* sc, const rfa_strings* choices, int kind, int s
         std::lock_guard<std::mutex> lk(sc->ctx->mu);
         HIP_CHECK(hipSetDevice(sc->ctx->device));
         begin_call(sc->ctx);
-        cdist_impl(sc, choices, kind, semantics, cutoff_u, cutoff_d, out, out_on_device);
+        if (choices->width != sc->q->width)
+            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
+        cdist_impl(sc, scorer_choices(sc, choices), kind, semantics, cutoff_u, cutoff_d, out, out_on_device);
     });
 }
 
@@ -977,7 +1105,9 @@ rfa_status rfa_cdist_cutoffs(rfa_scorer* sc, const rfa_strings* choices, int kin
         std::lock_guard<std::mutex> lk(sc->ctx->mu);
         HIP_CHECK(hipSetDevice(sc->ctx->device));
         begin_call(sc->ctx);
-        cdist_impl(sc, choices, kind, semantics, 0, 0.0, out, out_on_device, cutoffs_u);
+        if (choices->width != sc->q->width)
+            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
+        cdist_impl(sc, scorer_choices(sc, choices), kind, semantics, 0, 0.0, out, out_on_device, cutoffs_u);
     });
 }
 

@@ -231,6 +231,7 @@ struct rfa_strings {
     std::vector<uint8_t> h_chars;
     std::vector<uint64_t> h_off;
     uint64_t max_len = 0;
+    uint64_t id = 0;   // unique per set of the process (caches of derived sets key on it)
     bool view = false; // borrows d_chars/d_off of another set
     std::shared_ptr<rfa::ServeIndex> index; // built on first top-k use
 
@@ -292,6 +293,16 @@ struct rfa_scorer {
     uint64_t block_max_len = 0, block_min_len = 0;
     std::shared_ptr<void> few_plan; // cached plan of the one-pair-per-lane path (last choice set)
     uint32_t block_u8_rows = 0; // distinct symbols + 1 of the 8-bit block queries (0: unknown)
+    // metric the caller asked for (metric holds the kernels' metric: token-sort
+    // scorers run ratio / partial_ratio on token-sorted copies, QRatio runs ratio)
+    int api_metric = 0;
+    bool token_sort = false;
+    bool empty_zero = false;
+    bool signed8 = false;            // RFA_SCORER_SIGNED_UNITS
+    rfa_strings* q_raw = nullptr;    // token-sort scorers: the queries as given
+    rfa_strings* tok_choices = nullptr; // token-sorted copy of the last choice set
+    uint64_t tok_choices_id = 0;
+    bool tok_choices_signed8 = false;
 };
 
 namespace rfa {
@@ -301,6 +312,10 @@ int algo_for(int metric, const rfa_weights& w);
 bool multi_levenshtein_weights(const rfa_weights& w);
 bool is_int_kind(int metric, int kind);
 rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const uint64_t* off, uint64_t count);
+// token-sorted copy of a set (device transform on the context's stream)
+rfa_strings* token_sorted(rfa_context* ctx, const rfa_strings* in, bool signed8);
+// the choice set a scorer's kernels read: its token-sorted copy for token-sort scorers
+const rfa_strings* scorer_choices(rfa_scorer* sc, const rfa_strings* ch);
 void free_strings(rfa_strings* s);
 rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries, const rfa_weights& w,
                         double prefix_weight);

@@ -166,6 +166,28 @@ struct PartialLongArgs {
     uint32_t lds_bytes;      // 0: patterns live in pm_scratch
 };
 
+// Token-sorted copy of a string set (rfa_tokens.hip): out_len, ntok, out_off and tok_off
+// hold count + 1 entries (out_len[count] and ntok[count] zeroed by the caller);
+// tok_start / tok_len and out hold at least as many entries as the set has units.
+struct TokenArgs {
+    const void* chars;
+    const uint64_t* off;
+    uint64_t count;
+    uint64_t* out_len;
+    uint64_t* ntok;
+    uint64_t* out_off;
+    uint64_t* tok_off;
+    uint32_t* tok_start;
+    uint32_t* tok_len;
+    void* out;
+    void* scan_temp;
+    size_t scan_bytes;
+    int signed8; // 8-bit units compare as signed (the reference's `char`)
+    uint32_t num_cus;
+};
+size_t tokens_scan_bytes(uint64_t count);
+hipError_t launch_tokens(int char_width, const TokenArgs& a, hipStream_t s);
+
 // Widest shorter string (in 64-unit words) the long path takes.
 constexpr uint32_t PARTIAL_MAX_WORDS = 16;
 hipError_t launch_plong(int char_width, uint32_t needle_words, const PartialLongArgs& a, uint32_t nblocks,

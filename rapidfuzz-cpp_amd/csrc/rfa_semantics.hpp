@@ -15,6 +15,7 @@
 //   Jaro-Winkler cutoff and boost ....... rapidfuzz/distance/JaroWinkler_impl.hpp:41-71
 //   MultiJaroWinkler .................... rapidfuzz/distance/JaroWinkler.hpp:122-145
 //   fuzz::ratio / CachedRatio ........... rapidfuzz/fuzz_impl.hpp:28-47, fuzz.hpp:82-89
+//   fuzz::QRatio / CachedQRatio ......... rapidfuzz/fuzz_impl.hpp:897-922
 //
 // Compile with -ffp-contract=off: the JW boost must not become an FMA.
 #pragma once
@@ -312,6 +313,10 @@ __device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64
     }
     IntPair p{&s, l1, l2, r.v};
     if (s.metric == RFA_RATIO) {
+        if (s.empty_zero && (l1 == 0 || l2 == 0)) {
+            od[idx] = 0.0; // CachedQRatio::similarity
+            return;
+        }
         // ratio = Indel normalized similarity * 100 (Cached: with the +1e-5 widening)
         ScoreSpec t = s;
         t.metric = RFA_INDEL;

@@ -510,10 +510,12 @@ void topk_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, double cutoff, u
     validate_topk(sc->metric, kind);
     if (ch->width != sc->q->width)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
+    ch = scorer_choices(sc, ch);
     const uint64_t nq = sc->q->count, nc = ch->count;
     if (!nq || !k) return;
     if (nc >= 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 - 1 choices");
-    const FusedKind f = fused_kind(sc->metric, sc->w, kind, cutoff, sc->q->width);
+    // (QRatio's empty-string rule is applied by the dense path's epilogue only)
+    const FusedKind f = sc->empty_zero ? FusedKind{} : fused_kind(sc->metric, sc->w, kind, cutoff, sc->q->width);
     std::vector<uint32_t> fused, dense;
     for (uint64_t q = 0; q < nq; ++q) {
         if (f.ok && sc->q->len(q) <= 256 && nc)

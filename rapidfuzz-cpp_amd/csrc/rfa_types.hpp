@@ -17,6 +17,8 @@ struct ScoreSpec {
     uint64_t ins, del, rep;
     double prefix_weight;
     uint64_t band_k; // exact answer needed only up to band_k (used to skip pairs by length)
+    int32_t empty_zero; // QRatio: 0 when either string is empty (fuzz_impl.hpp:912-922)
+    int32_t reserved;
 };
 
 // Raw recurrences.

@@ -175,7 +175,10 @@ private:
 // rfa_scorer over packed queries, one per code-unit width (built on demand).
 class ScorerSet {
 public:
-    ScorerSet(int metric, Packed q, rfa_weights w, double pw) : metric_(metric), q_(std::move(q)), w_(w), pw_(pw) {}
+    // signed8: 8-bit units order as signed values when tokens are sorted (`char` queries)
+    ScorerSet(int metric, Packed q, rfa_weights w, double pw, bool signed8 = false)
+        : metric_(metric), q_(std::move(q)), w_(w), pw_(pw), signed8_(signed8)
+    {}
     ~ScorerSet()
     {
         for (auto& kv : scorers_) rfa_scorer_destroy(kv.second);
@@ -195,6 +198,13 @@ public:
         DeviceStrings qs(ctx, q_, width);
         rfa_scorer* sc = nullptr;
         check(rfa_scorer_create(ctx, metric_, qs.get(), &w_, pw_, &sc));
+        if (signed8_ && width == 1) {
+            const rfa_status st = rfa_scorer_set_option(sc, RFA_SCORER_SIGNED_UNITS, 1);
+            if (st != RFA_OK) {
+                rfa_scorer_destroy(sc);
+                check(st);
+            }
+        }
         scorers_[width] = sc;
         return sc;
     }
@@ -204,6 +214,7 @@ private:
     Packed q_;
     rfa_weights w_;
     double pw_;
+    bool signed8_;
     mutable std::mutex mu_;
     mutable std::map<int, rfa_scorer*> scorers_;
 };
@@ -555,11 +566,11 @@ using namespace rapidfuzz_amd::detail;
 template <int Metric>
 struct CachedFuzz {
     template <typename InputIt1>
-    CachedFuzz(InputIt1 first1, InputIt1 last1)
+    CachedFuzz(InputIt1 first1, InputIt1 last1, bool signed8 = false)
     {
         Packed q;
         q.add(first1, last1);
-        set_ = std::make_shared<ScorerSet>(Metric, std::move(q), rfa_weights{1, 1, 1}, 0.1);
+        set_ = std::make_shared<ScorerSet>(Metric, std::move(q), rfa_weights{1, 1, 1}, 0.1, signed8);
     }
 
     template <typename InputIt2>
@@ -635,10 +646,55 @@ struct CachedPartialRatio : detail::CachedFuzz<RFA_PARTIAL_RATIO> {
     {}
 };
 
+// fuzz::CachedTokenSortRatio (fuzz.hpp:282-304): ratio of the token-sorted strings.
+// Tokens of `char` strings order as signed values, like the reference's comparison.
+template <typename CharT1>
+struct CachedTokenSortRatio : detail::CachedFuzz<RFA_TOKEN_SORT_RATIO> {
+    template <typename InputIt1>
+    CachedTokenSortRatio(InputIt1 first1, InputIt1 last1)
+        : detail::CachedFuzz<RFA_TOKEN_SORT_RATIO>(first1, last1,
+                                                    std::is_signed<CharT1>::value && sizeof(CharT1) == 1)
+    {}
+    template <typename Sentence1>
+    CachedTokenSortRatio(const Sentence1& s1) : CachedTokenSortRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+};
+
+// fuzz::CachedPartialTokenSortRatio (fuzz.hpp:342-364): partial_ratio of the token-sorted strings
+template <typename CharT1>
+struct CachedPartialTokenSortRatio : detail::CachedFuzz<RFA_PARTIAL_TOKEN_SORT_RATIO> {
+    template <typename InputIt1>
+    CachedPartialTokenSortRatio(InputIt1 first1, InputIt1 last1)
+        : detail::CachedFuzz<RFA_PARTIAL_TOKEN_SORT_RATIO>(first1, last1,
+                                                            std::is_signed<CharT1>::value && sizeof(CharT1) == 1)
+    {}
+    template <typename Sentence1>
+    CachedPartialTokenSortRatio(const Sentence1& s1)
+        : CachedPartialTokenSortRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+};
+
+// fuzz::CachedQRatio (fuzz.hpp:776-796): ratio, 0 when either string is empty
+template <typename CharT1>
+struct CachedQRatio : detail::CachedFuzz<RFA_QRATIO> {
+    template <typename InputIt1>
+    CachedQRatio(InputIt1 first1, InputIt1 last1) : detail::CachedFuzz<RFA_QRATIO>(first1, last1)
+    {}
+    template <typename Sentence1>
+    CachedQRatio(const Sentence1& s1) : CachedQRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+};
+
 template <typename Sentence1>
 CachedRatio(const Sentence1& s1) -> CachedRatio<detail::sentence_char_t<Sentence1>>;
 template <typename Sentence1>
 CachedPartialRatio(const Sentence1& s1) -> CachedPartialRatio<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+CachedTokenSortRatio(const Sentence1& s1) -> CachedTokenSortRatio<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+CachedPartialTokenSortRatio(const Sentence1& s1) -> CachedPartialTokenSortRatio<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+CachedQRatio(const Sentence1& s1) -> CachedQRatio<detail::sentence_char_t<Sentence1>>;
 
 } // namespace fuzz
 

@@ -14,11 +14,13 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "librapidfuzz_amd.so")
 
-LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX, PARTIAL_RATIO = range(11)
+(LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX, PARTIAL_RATIO, TOKEN_SORT_RATIO,
+ PARTIAL_TOKEN_SORT_RATIO, QRATIO) = range(14)
 DISTANCE, SIMILARITY, NORMALIZED_DISTANCE, NORMALIZED_SIMILARITY = range(4)
 CACHED, MULTI = 0, 1
 SCHEDULE_BUCKETED, SCHEDULE_PLAIN = 0, 1
 OPTION_PAIRS_GRAPH, OPTION_PAIRS_CHAIN = 1, 2
+SCORER_SIGNED_UNITS = 1
 INT64_MAX = (1 << 63) - 1
 
 _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED", 3: "RFA_ERR_HIP",
@@ -27,7 +29,7 @@ _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED",
 # every symbol include/rapidfuzz_amd.h declares
 EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
             "rfa_context_synchronize", "rfa_context_stream", "rfa_context_set_option", "rfa_strings_create", "rfa_strings_destroy",
-            "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_cdist_cutoffs",
+            "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_scorer_set_option", "rfa_cdist", "rfa_cdist_cutoffs",
             "rfa_pairs", "rfa_pairs_autotune", "rfa_topk",
             "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
             "rfa_server_get_stats"]
@@ -85,6 +87,7 @@ def load_library(path: str = LIB_PATH):
     L.rfa_scorer_create.argtypes = [_vp, ctypes.c_int, _vp, ctypes.POINTER(Weights), ctypes.c_double,
                                     ctypes.POINTER(_vp)]
     L.rfa_scorer_destroy.argtypes = [_vp]
+    L.rfa_scorer_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int64]
     L.rfa_cdist.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _vp,
                             ctypes.c_int]
     L.rfa_cdist_cutoffs.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int]
@@ -100,7 +103,7 @@ def load_library(path: str = LIB_PATH):
     L.rfa_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     for name in ("rfa_context_create", "rfa_context_destroy", "rfa_context_synchronize", "rfa_strings_create",
                  "rfa_strings_destroy", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_cdist_cutoffs",
-                 "rfa_pairs", "rfa_pairs_autotune", "rfa_context_set_option",
+                 "rfa_pairs", "rfa_pairs_autotune", "rfa_context_set_option", "rfa_scorer_set_option",
                  "rfa_topk", "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait",
                  "rfa_server_replay", "rfa_server_get_stats"):
         getattr(L, name).restype = ctypes.c_int
@@ -114,6 +117,10 @@ def _check(status):
         if status == 1:
             raise InvalidArgument(status, msg)
         raise RfaError(status, msg)
+
+
+# metrics whose top-k ranks the plain similarity (0-100 or 0-1 doubles)
+_SIM_METRICS = (JARO, JARO_WINKLER, RATIO, PARTIAL_RATIO, TOKEN_SORT_RATIO, PARTIAL_TOKEN_SORT_RATIO, QRATIO)
 
 
 def is_int_result(metric, kind):
@@ -248,6 +255,10 @@ class Scorer:
             _check(_lib.rfa_scorer_destroy(self.h))
         self.h = None
 
+    def set_option(self, option, value):
+        """rfa_scorer_set_option (SCORER_SIGNED_UNITS: 8-bit units order as signed `char`)."""
+        _check(_lib.rfa_scorer_set_option(self.h, option, ctypes.c_int64(value)))
+
     def __del__(self):
         try:
             self.close()
@@ -280,7 +291,7 @@ class Scorer:
         """(index, score), each [nq, k]: the extract() results >= cutoff_d ordered by
         (score desc, index asc); missing entries are index -1, score 0."""
         if kind is None:
-            kind = SIMILARITY if self.metric in (JARO, JARO_WINKLER, RATIO, PARTIAL_RATIO) else NORMALIZED_SIMILARITY
+            kind = SIMILARITY if self.metric in _SIM_METRICS else NORMALIZED_SIMILARITY
         idx = np.zeros((self.nq, k), dtype=np.int64)
         sc = np.zeros((self.nq, k), dtype=np.float64)
         _check(_lib.rfa_topk(self.h, choices.h, kind, cutoff_d, k, idx.ctypes.data, sc.ctypes.data))
@@ -293,7 +304,7 @@ class Server:
     def __init__(self, ctx: Context, choices: Strings, metric=RATIO, kind=None, score_cutoff=0.0, k=5,
                  max_batch=64, slots=4, candidate_cap=65536, prefix_weight=0.1, weights=(1, 1, 1)):
         if kind is None:
-            kind = SIMILARITY if metric in (JARO, JARO_WINKLER, RATIO, PARTIAL_RATIO) else NORMALIZED_SIMILARITY
+            kind = SIMILARITY if metric in _SIM_METRICS else NORMALIZED_SIMILARITY
         self.ctx = ctx
         self.k = k
         self.width = choices.width

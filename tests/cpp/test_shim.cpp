@@ -99,6 +99,24 @@ int main()
             CHECK(near(rfa::fuzz::CachedPartialRatio<char>(std::string("physics 2 vid"))
                            .similarity(std::string("study physics physics 2 video")),
                        100.0));
+            // tests-fuzz.cpp:84-85, 125-151
+            rfa::fuzz::CachedTokenSortRatio<char> ts(std::string("new york mets"));
+            CHECK(near(ts.similarity(std::string("new york mets")), 100.0));
+            CHECK(near(ts.similarity(std::string("mets york new")), 100.0));
+            CHECK(near(rfa::fuzz::CachedTokenSortRatio<char>(std::string("")).similarity(std::string("")), 100.0));
+            CHECK(near(rfa::fuzz::CachedPartialTokenSortRatio<char>(std::string("")).similarity(std::string("")),
+                       100.0));
+            CHECK(near(rfa::fuzz::CachedQRatio<char>(std::string("")).similarity(std::string("")), 0.0));
+            CHECK(near(rfa::fuzz::CachedQRatio<char>(std::string("test")).similarity(std::string("")), 0.0));
+            // `char` tokens order as signed bytes: "\xe9x" sorts before "ab", so the join is
+            // "\xe9x ab" (88.9 against "\xe9xab"; unsigned order would give "ab \xe9x", 44.4)
+            rfa::fuzz::CachedTokenSortRatio<char> acc(std::string("\xe9x ab"));
+            CHECK(near(acc.similarity(std::string("\xe9xab")), 88.8888889));
+            rfa::fuzz::CachedTokenSortRatio<unsigned char> accu(std::basic_string<unsigned char>(
+                reinterpret_cast<const unsigned char*>("\xe9x ab")));
+            CHECK(near(accu.similarity(std::basic_string<unsigned char>(
+                           reinterpret_cast<const unsigned char*>("\xe9xab"))),
+                       44.4444444));
         }
         {
             // deduction guides and wide code units against narrow ones

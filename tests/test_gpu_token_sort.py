@@ -1,0 +1,120 @@
+"""GPU parity of token_sort_ratio, partial_token_sort_ratio and QRatio (rfa_cdist /
+rfa_topk) against the oracle, bit for bit: the device token sort for 8/16/32-bit
+units and signed 8-bit order, and the cached token-sorted choice set."""
+import numpy as np
+import pytest
+
+import golden_cases as G
+
+pytestmark = pytest.mark.gpu
+
+TSR, PTSR, QR, SIM = 11, 12, 13, 1
+
+
+def _text(rng, n, width):
+    pool = [97, 98, 99, 100, 101, 0x20, 0x20, 0x20, 0x09, 0x0A, 0x1F, 0x85, 0xA0, 0xC3, 0xE9, 0xFF]
+    if width > 1:
+        pool += [0x3000, 0x2003, 0x1680, 0x4E2D, 0x0101]
+    if width == 4:
+        pool += [0x1F600]
+    return np.array([pool[int(i)] for i in rng.integers(0, len(pool), size=n)], dtype=np.uint32)
+
+
+def _sets(rng, width, nq, nc, qmax, cmax):
+    q = [_text(rng, int(rng.integers(0, qmax)), width) for _ in range(nq)]
+    c = [_text(rng, int(rng.integers(0, cmax)), width) for _ in range(nc)]
+    for s in q[:20]:  # token permutations of queries
+        toks = "".join(chr(x) for x in s).split(" ")
+        rng.shuffle(toks)
+        c.append(np.array([ord(ch) for ch in " ".join(toks)], np.uint32))
+    return q, c
+
+
+def _check(ctx, oracle, metric, q, c, width, signed8=False, cuts=(0.0, 60.0, 90.0)):
+    import rapidfuzz_amd as rfa
+
+    sc = rfa.Scorer(ctx, metric, ctx.strings(q, width=width))
+    if signed8:
+        sc.set_option(rfa.SCORER_SIGNED_UNITS, 1)
+    cs = ctx.strings(c, width=width)
+    for cut in cuts:
+        got = sc.cdist(cs, SIM, cutoff_d=cut)
+        exp = oracle.cdist(metric, SIM, q, c, cutoff_d=cut, char_width=width, signed8=signed8)
+        bad = np.argwhere(got.view(np.uint64) != exp.view(np.uint64))
+        assert len(bad) == 0, (metric, width, signed8, cut, len(bad), bad[:3].tolist())
+    return sc, cs
+
+
+def test_known_answers(ctx):
+    import rapidfuzz_amd as rfa
+
+    for c in G.load()["cases"]:
+        if c["metric"] not in (TSR, PTSR, QR):
+            continue
+        sc = rfa.Scorer(ctx, c["metric"], ctx.strings([G.expand(c["s1"])]))
+        got = float(sc.cdist(ctx.strings([G.expand(c["s2"])]), SIM, cutoff_d=c.get("cutoff", 0.0))[0, 0])
+        assert abs(got - c["expected"]) <= c["tol"], (G.case_id(c), got)
+
+
+@pytest.mark.parametrize("metric", [TSR, PTSR, QR])
+@pytest.mark.parametrize("width,signed8", [(1, False), (1, True), (2, False), (4, False)])
+def test_short(ctx, oracle, metric, width, signed8):
+    """Queries of 0-40 units (tile path) against choices of 0-90 units."""
+    rng = np.random.default_rng(500 + metric * 11 + width + 5 * signed8)
+    q, c = _sets(rng, width, 80, 60, 40, 90)
+    _check(ctx, oracle, metric, q, c, width, signed8)
+
+
+@pytest.mark.parametrize("metric", [TSR, PTSR])
+def test_long(ctx, oracle, metric):
+    """Queries of 70-200 units (block / long partial paths) and many tokens per string."""
+    rng = np.random.default_rng(560 + metric)
+    q, c = _sets(rng, 1, 6, 12, 200, 260)
+    q = [s for s in q if len(s) > 64] + [_text(rng, 150, 1)]
+    _check(ctx, oracle, metric, q, c, 1, cuts=(0.0, 70.0))
+
+
+def test_cached_choices_and_option_switch(ctx, oracle):
+    """A second choice set, the first one again, then the signed order: every call
+    scores the right token-sorted copy."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(590)
+    q, c1 = _sets(rng, 1, 30, 40, 30, 60)
+    _, c2 = _sets(rng, 1, 30, 40, 30, 60)
+    sc = rfa.Scorer(ctx, TSR, ctx.strings(q, width=1))
+    s1, s2 = ctx.strings(c1, width=1), ctx.strings(c2, width=1)
+    for chs, cs in ((c1, s1), (c2, s2), (c1, s1)):
+        got = sc.cdist(cs, SIM)
+        assert np.array_equal(got.view(np.uint64), oracle.cdist(TSR, SIM, q, chs, char_width=1).view(np.uint64))
+    sc.set_option(rfa.SCORER_SIGNED_UNITS, 1)
+    got = sc.cdist(s1, SIM)
+    exp = oracle.cdist(TSR, SIM, q, c1, char_width=1, signed8=True)
+    assert np.array_equal(got.view(np.uint64), exp.view(np.uint64))
+
+
+@pytest.mark.parametrize("metric", [TSR, QR])
+def test_topk(ctx, oracle, metric):
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(600 + metric)
+    q, c = _sets(rng, 1, 8, 400, 25, 60)
+    sc = rfa.Scorer(ctx, metric, ctx.strings(q, width=1))
+    idx, score = sc.topk(ctx.strings(c, width=1), 5, cutoff_d=50.0)
+    full = oracle.cdist(metric, SIM, q, c, cutoff_d=50.0, char_width=1)
+    for i in range(len(q)):
+        order = sorted(((-full[i, j], j) for j in range(len(c)) if full[i, j] >= 50.0))[:5]
+        assert list(idx[i]) == [j for _, j in order] + [-1] * (5 - len(order))
+        assert np.array_equal(score[i], np.array([-s for s, _ in order] + [0.0] * (5 - len(order))))
+
+
+def test_rejected(ctx):
+    import rapidfuzz_amd as rfa
+
+    sc = rfa.Scorer(ctx, TSR, ctx.strings([b"a b"]))
+    with pytest.raises(rfa.InvalidArgument):
+        sc.cdist(ctx.strings([b"b a"]), 0)
+    with pytest.raises(rfa.InvalidArgument):
+        sc.cdist(ctx.strings([b"b a"]), SIM, semantics=1)
+    with pytest.raises(rfa.InvalidArgument):
+        sc.set_option(99, 1)
