@@ -696,6 +696,94 @@ def workload_partial(args, emit=True):
     return line
 
 
+def workload_tokens(args, emit=True):
+    """token_sort_ratio, partial_token_sort_ratio and QRatio cdist on the partial_ratio data (1024
+    word-like queries x 65,536 texts over a-z and space), no cutoff.  A token-sort scorer sorts
+    the tokens of the choice set on the device on its first call and reuses that copy: the
+    steady-state time is `ms`, the one-time sort is `choice_sort_ms`."""
+    import rapidfuzz_amd as rfa
+    from rapidfuzz_amd import hip
+
+    rng = np.random.default_rng(SEED + 7)
+    qc, qo, ql, cc, co, cl = gen_partial(rng)
+    nq, nc = len(ql), len(cl)
+    ctx = rfa.Context(0)
+    qs = rfa.Strings(ctx, chars=qc, offsets=qo, width=1)
+    cs = rfa.Strings(ctx, chars=cc, offsets=co, width=1)
+    out = hip.DeviceBuffer(nq * nc * 8)
+    pairs = float(nq) * float(nc)
+    res = {}
+    for name, metric in (("token_sort_ratio", rfa.TOKEN_SORT_RATIO),
+                         ("partial_token_sort_ratio", rfa.PARTIAL_TOKEN_SORT_RATIO), ("qratio", rfa.QRATIO)):
+        sc = rfa.Scorer(ctx, metric, qs)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sc.cdist(cs, rfa.SIMILARITY, out_device_ptr=out.ptr.value)
+        ctx.synchronize()
+        first_ms = (time.perf_counter() - t0) * 1e3
+        for _ in range(max(0, args.warmup - 1)):
+            sc.cdist(cs, rfa.SIMILARITY, out_device_ptr=out.ptr.value)
+        ctx.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        steps = max(1, args.steps)
+        t0 = time.perf_counter()
+        e0.record(ctx.stream)
+        for _ in range(steps):
+            sc.cdist(cs, rfa.SIMILARITY, out_device_ptr=out.ptr.value)
+        e1.record(ctx.stream)
+        e1.synchronize()
+        wall = (time.perf_counter() - t0) / steps * 1e3
+        ms = e0.elapsed_ms(e1) / steps
+        entry = {"ms": round(ms, 3), "gpairs_per_s": round(pairs / (ms / 1e3) / 1e9, 3)}
+        if metric != rfa.QRATIO:
+            entry["choice_sort_ms"] = round(max(0.0, first_ms - wall), 2)
+        print(f"# {name}: {ms:.3f} ms/step, {entry['gpairs_per_s']} Gpairs/s"
+              + (f", first call +{entry['choice_sort_ms']} ms (choice-set token sort)" if "choice_sort_ms" in entry
+                 else ""), file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle_py
+            if oracle_py.reference_available():
+                ref = oracle_py.Reference()
+                cores = _cores()
+                k = 4
+                while True:
+                    idx = np.linspace(0, nq - 1, k).astype(np.int64)
+                    sq, so = pack_u8([qc[int(qo[i]):int(qo[i + 1])] for i in idx])
+                    t0 = time.perf_counter()
+                    r = ref.cdist_u8_mt(metric, rfa.SIMILARITY, sq, so, cc, co, threads=cores)
+                    dt = time.perf_counter() - t0
+                    if dt > 2 or k >= nq:
+                        break
+                    k = min(nq, k * 4)
+                rows = np.zeros(nc, np.float64)
+                bad = 0
+                for k_, i in enumerate(idx):
+                    out.to_host(rows, offset=int(i) * nc * 8)
+                    bad += int(np.count_nonzero(rows.view(np.uint64) != r[k_].view(np.uint64)))
+                entry["cpu_baseline"] = {"value": round(k * nc / dt / 1e9, 5), "unit": "Gpairs/s", "cores": cores,
+                                         "kind": "reference",
+                                         "sample": f"{k} of {nq} queries x all {nc} choices ({dt:.1f} s)"}
+                entry["check"] = "bit-exact vs reference on the CPU sample" if bad == 0 else f"{bad} MISMATCHES"
+        sc.close()
+        res[name] = entry
+    line = {"metric": "token_sort_ratio cdist 1024 x 65536, Gpairs/s", "value": res["token_sort_ratio"]["gpairs_per_s"],
+            "unit": "Gpairs/s", "n_gpus": 1, "steps": max(1, args.steps), "warmup": max(1, args.warmup),
+            "ms_per_step": res["token_sort_ratio"]["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+            "config": {"workload": f"fuzz::CachedTokenSortRatio / CachedPartialTokenSortRatio / CachedQRatio cdist "
+                                   f"{nq} x {nc}, uint8 a-z and space, queries 8-32 units, choices 32-128 units, "
+                                   f"no cutoff"},
+            **res}
+    out.free()
+    for x in (qs, cs):
+        x.close()
+    ctx.close()
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
+
+
 def _mutate_dna(rng, s):
     """Per position: substitution 1% (to another base), insertion 0.5% (a random base before it),
     deletion 0.5%."""
@@ -834,7 +922,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="cdist",
-                    choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "partial", "copy"],
+                    choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "partial", "tokens", "copy"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
                          "uneven: 1M independent pairs; serve: the serving leg alone; latency: 1 x 10,000 "
                          "Levenshtein; dna: 256 x 16,384 long sequences with band cutoffs; copy: HBM counter "
@@ -869,6 +957,8 @@ def main():
         return workload_positional(args)
     if args.workload == "partial":
         return workload_partial(args)
+    if args.workload == "tokens":
+        return workload_tokens(args)
 
     import rapidfuzz_amd as rfa
     from rapidfuzz_amd import hip
@@ -1024,6 +1114,10 @@ def main():
         par = workload_partial(args, emit=False)
         res["configs"]["partial_ratio_1024x65536"] = {**{k: par.get(k) for k in keep},
                                                       "config": par["config"]["workload"]}
+        tok = workload_tokens(args, emit=False)
+        res["configs"]["token_scorers_1024x65536"] = {k: tok[k] for k in ("metric", "value", "unit", "ms_per_step",
+                                                                           "token_sort_ratio",
+                                                                           "partial_token_sort_ratio", "qratio")}
         une = workload_uneven(args, emit=False)
         res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
                                              "speedup_vs_plain": une["speedup_vs_plain"],
