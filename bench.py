@@ -697,7 +697,8 @@ def workload_partial(args, emit=True):
 
 
 def workload_tokens(args, emit=True):
-    """token_sort_ratio, partial_token_sort_ratio and QRatio cdist on the partial_ratio data (1024
+    """token_sort_ratio, partial_token_sort_ratio, QRatio, token_set_ratio and token_ratio cdist on the
+    partial_ratio data (1024
     word-like queries x 65,536 texts over a-z and space), no cutoff.  A token-sort scorer sorts
     the tokens of the choice set on the device on its first call and reuses that copy: the
     steady-state time is `ms`, the one-time sort is `choice_sort_ms`."""
@@ -714,7 +715,8 @@ def workload_tokens(args, emit=True):
     pairs = float(nq) * float(nc)
     res = {}
     for name, metric in (("token_sort_ratio", rfa.TOKEN_SORT_RATIO),
-                         ("partial_token_sort_ratio", rfa.PARTIAL_TOKEN_SORT_RATIO), ("qratio", rfa.QRATIO)):
+                         ("partial_token_sort_ratio", rfa.PARTIAL_TOKEN_SORT_RATIO), ("qratio", rfa.QRATIO),
+                         ("token_set_ratio", rfa.TOKEN_SET_RATIO), ("token_ratio", rfa.TOKEN_RATIO)):
         sc = rfa.Scorer(ctx, metric, qs)
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -771,7 +773,8 @@ def workload_tokens(args, emit=True):
             "unit": "Gpairs/s", "n_gpus": 1, "steps": max(1, args.steps), "warmup": max(1, args.warmup),
             "ms_per_step": res["token_sort_ratio"]["ms"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-            "config": {"workload": f"fuzz::CachedTokenSortRatio / CachedPartialTokenSortRatio / CachedQRatio cdist "
+            "config": {"workload": f"fuzz::CachedTokenSortRatio / CachedPartialTokenSortRatio / CachedQRatio / "
+                                   f"CachedTokenSetRatio / CachedTokenRatio cdist "
                                    f"{nq} x {nc}, uint8 a-z and space, queries 8-32 units, choices 32-128 units, "
                                    f"no cutoff"},
             **res}
@@ -1117,7 +1120,8 @@ def main():
         tok = workload_tokens(args, emit=False)
         res["configs"]["token_scorers_1024x65536"] = {k: tok[k] for k in ("metric", "value", "unit", "ms_per_step",
                                                                            "token_sort_ratio",
-                                                                           "partial_token_sort_ratio", "qratio")}
+                                                                           "partial_token_sort_ratio", "qratio",
+                                                                           "token_set_ratio", "token_ratio")}
         une = workload_uneven(args, emit=False)
         res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
                                              "speedup_vs_plain": une["speedup_vs_plain"],

@@ -59,7 +59,11 @@ typedef enum {
      * sorted, joined with single spaces: detail::sorted_split(...).join()); similarity, Cached only */
     RFA_TOKEN_SORT_RATIO = 11,         /* fuzz::CachedTokenSortRatio (fuzz.hpp:282-304) */
     RFA_PARTIAL_TOKEN_SORT_RATIO = 12, /* fuzz::CachedPartialTokenSortRatio (fuzz.hpp:342-364) */
-    RFA_QRATIO = 13                    /* fuzz::CachedQRatio (fuzz.hpp:776-796): ratio, 0 if either string is empty */
+    RFA_QRATIO = 13,                   /* fuzz::CachedQRatio (fuzz.hpp:776-796): ratio, 0 if either string is empty */
+    /* token sets (deduplicated sorted tokens): rfa_cdist only; queries whose deduplicated token join
+     * has at most 64 code units */
+    RFA_TOKEN_SET_RATIO = 14,          /* fuzz::CachedTokenSetRatio (fuzz.hpp:412-434) */
+    RFA_TOKEN_RATIO = 15               /* fuzz::CachedTokenRatio (fuzz.hpp:530-557): max of the token-sort and token-set ratios */
 } rfa_metric;
 
 /* Scorer method (details/distance.hpp:211-301, 326-356). */

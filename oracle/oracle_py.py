@@ -19,7 +19,7 @@ ORACLE_SO = os.path.join(HERE, "librf_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "librfref.so")
 
 (LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX, PARTIAL_RATIO, TOKEN_SORT_RATIO,
- PARTIAL_TOKEN_SORT_RATIO, QRATIO) = range(14)
+ PARTIAL_TOKEN_SORT_RATIO, QRATIO, TOKEN_SET_RATIO, TOKEN_RATIO) = range(16)
 DISTANCE, SIMILARITY, NORM_DISTANCE, NORM_SIMILARITY = range(4)
 INT64_MAX = (1 << 63) - 1
 

@@ -25,7 +25,7 @@
 
 namespace {
 
-enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10, TOKEN_SORT = 11, PARTIAL_TOKEN_SORT = 12, QRATIO = 13 };
+enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10, TOKEN_SORT = 11, PARTIAL_TOKEN_SORT = 12, QRATIO = 13, TOKEN_SET = 14, TOKEN_RATIO = 15 };
 enum { DIST = 0, SIM = 1, NDIST = 2, NSIM = 3 };
 
 thread_local std::string g_err;
@@ -83,6 +83,8 @@ struct AnyCached {
     rapidfuzz::fuzz::CachedTokenSortRatio<CharT>* tsort = nullptr;
     rapidfuzz::fuzz::CachedPartialTokenSortRatio<CharT>* ptsort = nullptr;
     rapidfuzz::fuzz::CachedQRatio<CharT>* qratio = nullptr;
+    rapidfuzz::fuzz::CachedTokenSetRatio<CharT>* tset = nullptr;
+    rapidfuzz::fuzz::CachedTokenRatio<CharT>* tratio = nullptr;
 
     AnyCached(int m, std::vector<CharT> q, rapidfuzz::LevenshteinWeightTable w_, double pw_)
         : metric(m), s1(std::move(q)), w(w_), pw(pw_)
@@ -102,6 +104,8 @@ struct AnyCached {
         case TOKEN_SORT: tsort = new rapidfuzz::fuzz::CachedTokenSortRatio<CharT>(s1); break;
         case PARTIAL_TOKEN_SORT: ptsort = new rapidfuzz::fuzz::CachedPartialTokenSortRatio<CharT>(s1); break;
         case QRATIO: qratio = new rapidfuzz::fuzz::CachedQRatio<CharT>(s1); break;
+        case TOKEN_SET: tset = new rapidfuzz::fuzz::CachedTokenSetRatio<CharT>(s1); break;
+        case TOKEN_RATIO: tratio = new rapidfuzz::fuzz::CachedTokenRatio<CharT>(s1); break;
         default: throw std::invalid_argument("unknown metric");
         }
     }
@@ -114,6 +118,8 @@ struct AnyCached {
         delete tsort;
         delete ptsort;
         delete qratio;
+        delete tset;
+        delete tratio;
         delete lev;
         delete indel;
         delete lcs;
@@ -154,6 +160,14 @@ struct AnyCached {
         case QRATIO:
             if (kind != SIM) return -1;
             *od = qratio->similarity(b, e, cd);
+            return 1;
+        case TOKEN_SET:
+            if (kind != SIM) return -1;
+            *od = tset->similarity(b, e, cd);
+            return 1;
+        case TOKEN_RATIO:
+            if (kind != SIM) return -1;
+            *od = tratio->similarity(b, e, cd);
             return 1;
         }
         return -1;

@@ -661,6 +661,181 @@ size_t orc_sorted_join(const uint32_t* s, size_t n, uint32_t* out)
     return len;
 }
 
+/* ------------------------------------------------------------------------ */
+/* token sets (common_impl.hpp:11-35, fuzz_impl.hpp:370-418, 518-565)       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const uint32_t* p;
+    size_t n;
+} Tok;
+
+/* sorted_split: the tokens in sorted order; returns the count */
+static size_t sorted_tokens(const uint32_t* s, size_t n, Tok* out)
+{
+    size_t T = 0, i = 0;
+    while (i < n) {
+        while (i < n && orc_is_space(s[i])) ++i;
+        if (i == n) break;
+        size_t e = i;
+        while (e < n && !orc_is_space(s[e])) ++e;
+        out[T].p = s + i;
+        out[T].n = e - i;
+        ++T;
+        i = e;
+    }
+    for (size_t a = 1; a < T; ++a) {
+        Tok k = out[a];
+        size_t b = a;
+        while (b > 0 && tok_less(k.p, k.n, out[b - 1].p, out[b - 1].n)) {
+            out[b] = out[b - 1];
+            --b;
+        }
+        out[b] = k;
+    }
+    return T;
+}
+
+static int tok_eq(Tok a, Tok b) { return a.n == b.n && (a.n == 0 || memcmp(a.p, b.p, a.n * sizeof(uint32_t)) == 0); }
+
+/* SplittedSentenceView::dedupe (std::unique) */
+static size_t dedupe(Tok* t, size_t T)
+{
+    size_t w = 0;
+    for (size_t i = 0; i < T; ++i)
+        if (w == 0 || !tok_eq(t[w - 1], t[i])) t[w++] = t[i];
+    return w;
+}
+
+/* SplittedSentenceView::size() of a token list: units plus one space between tokens */
+static size_t joined_len(const Tok* t, size_t T)
+{
+    size_t n = T ? T - 1 : 0;
+    for (size_t i = 0; i < T; ++i) n += t[i].n;
+    return n;
+}
+
+static size_t join_into(const Tok* t, size_t T, uint32_t* out)
+{
+    size_t len = 0;
+    for (size_t i = 0; i < T; ++i) {
+        if (i) out[len++] = 0x20;
+        memcpy(out + len, t[i].p, t[i].n * sizeof(uint32_t));
+        len += t[i].n;
+    }
+    return len;
+}
+
+/* fuzz_detail::norm_distance (fuzz_impl.hpp:55-61) */
+static double fz_norm_distance(size_t dist, size_t lensum, double score_cutoff)
+{
+    double score = (lensum > 0) ? (100.0 - 100.0 * (double)dist / (double)lensum) : 100.0;
+    return (score >= score_cutoff) ? score : 0;
+}
+
+/* The set part shared by token_set_ratio and token_ratio.  *perfect: one token set
+ * holds the other (and they share a token).  Returns max(result of the diff
+ * joins, sect+ab / sect+ba ratios); ratio_in seeds `result` (token_ratio passes the
+ * sorted-join ratio, token_set_ratio 0). */
+static double token_set_part(Tok* ta, size_t na, Tok* tb, size_t nb, double score_cutoff, double ratio_in,
+                             int* perfect)
+{
+    /* set_decomposition (common_impl.hpp:11-35) */
+    na = dedupe(ta, na);
+    nb = dedupe(tb, nb);
+    Tok* inter = (Tok*)malloc((na + 1) * sizeof(Tok));
+    Tok* dab = (Tok*)malloc((na + 1) * sizeof(Tok));
+    Tok* dba = (Tok*)malloc((nb + 1) * sizeof(Tok));
+    size_t ni = 0, nab = 0, nba = nb;
+    memcpy(dba, tb, nb * sizeof(Tok));
+    for (size_t i = 0; i < na; ++i) {
+        size_t j = 0;
+        while (j < nba && !tok_eq(dba[j], ta[i])) ++j;
+        if (j < nba) {
+            memmove(dba + j, dba + j + 1, (nba - j - 1) * sizeof(Tok));
+            --nba;
+            inter[ni++] = ta[i];
+        }
+        else {
+            dab[nab++] = ta[i];
+        }
+    }
+    *perfect = ni && (!nab || !nba);
+    double result = ratio_in;
+    if (!*perfect) {
+        const size_t ab_len = joined_len(dab, nab), ba_len = joined_len(dba, nba), sect_len = joined_len(inter, ni);
+        uint32_t* a = (uint32_t*)malloc((ab_len + 1) * sizeof(uint32_t));
+        uint32_t* b = (uint32_t*)malloc((ba_len + 1) * sizeof(uint32_t));
+        join_into(dab, nab, a);
+        join_into(dba, nba, b);
+        const size_t sect_ab_len = sect_len + (sect_len != 0) + ab_len;
+        const size_t sect_ba_len = sect_len + (sect_len != 0) + ba_len;
+        /* score_cutoff_to_distance (fuzz_impl.hpp:63-66) */
+        const size_t cutoff_distance =
+            (size_t)ceil((double)(sect_ab_len + sect_ba_len) * (1.0 - score_cutoff / 100));
+        /* indel_distance(diff_ab_joined, diff_ba_joined, cutoff_distance) */
+        size_t dist = ab_len + ba_len - 2 * (size_t)orc_lcs_raw(a, ab_len, b, ba_len);
+        if (dist > cutoff_distance) dist = cutoff_distance + 1;
+        if (dist <= cutoff_distance) {
+            const double v = fz_norm_distance(dist, sect_ab_len + sect_ba_len, score_cutoff);
+            result = result < v ? v : result;
+        }
+        if (sect_len) {
+            const double r1 = fz_norm_distance((sect_len != 0) + ab_len, sect_len + sect_ab_len, score_cutoff);
+            const double r2 = fz_norm_distance((sect_len != 0) + ba_len, sect_len + sect_ba_len, score_cutoff);
+            result = result < r1 ? r1 : result; /* std::max({result, r1, r2}) */
+            result = result < r2 ? r2 : result;
+        }
+        free(a);
+        free(b);
+    }
+    free(inter);
+    free(dab);
+    free(dba);
+    return result;
+}
+
+/* fuzz::token_set_ratio (fuzz_impl.hpp:370-426) */
+static double orc_token_set_ratio(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, double cutoff)
+{
+    if (cutoff > 100) return 0;
+    Tok* ta = (Tok*)malloc((l1 + 1) * sizeof(Tok));
+    Tok* tb = (Tok*)malloc((l2 + 1) * sizeof(Tok));
+    size_t na = sorted_tokens(s1, l1, ta), nb = sorted_tokens(s2, l2, tb);
+    double r = 0;
+    if (na && nb) {
+        int perfect = 0;
+        r = token_set_part(ta, na, tb, nb, cutoff, 0.0, &perfect);
+        if (perfect) r = 100;
+    }
+    free(ta);
+    free(tb);
+    return r;
+}
+
+/* fuzz::token_ratio / CachedTokenRatio::similarity (fuzz_impl.hpp:518-565, 574-622, 683-691) */
+static double orc_token_ratio(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, double cutoff)
+{
+    if (cutoff > 100) return 0;
+    Tok* ta = (Tok*)malloc((l1 + 1) * sizeof(Tok));
+    Tok* tb = (Tok*)malloc((l2 + 1) * sizeof(Tok));
+    size_t na = sorted_tokens(s1, l1, ta), nb = sorted_tokens(s2, l2, tb);
+    /* ratio(tokens_a.join(), tokens_b.join()) before the tokens are deduplicated */
+    uint32_t* a = (uint32_t*)malloc((l1 + 1) * sizeof(uint32_t));
+    uint32_t* b = (uint32_t*)malloc((l2 + 1) * sizeof(uint32_t));
+    size_t la = join_into(ta, na, a), lb = join_into(tb, nb, b);
+    Pair q = {ORC_INDEL, a, b, la, lb, 1, 1, 2, 0.0};
+    const double ratio = int_api_norm_similarity(&q, cutoff / 100) * 100;
+    int perfect = 0;
+    double r = token_set_part(ta, na, tb, nb, cutoff, ratio, &perfect);
+    if (perfect) r = 100;
+    free(a);
+    free(b);
+    free(ta);
+    free(tb);
+    return r;
+}
+
 /* partial_ratio_alignment(...).score (fuzz_impl.hpp:193-228); CachedPartialRatio::similarity
  * (fuzz_impl.hpp:255-279) returns the same value. */
 double orc_partial_ratio(const uint32_t* s1, size_t len1, const uint32_t* s2, size_t len2, double score_cutoff)
@@ -742,6 +917,14 @@ int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const 
         free(b);
         return 1;
     }
+    case ORC_TOKEN_SET_RATIO:
+        if (kind != ORC_SIMILARITY) return -1;
+        *out_d = orc_token_set_ratio(s1, l1, s2, l2, cutoff_d);
+        return 1;
+    case ORC_TOKEN_RATIO:
+        if (kind != ORC_SIMILARITY) return -1;
+        *out_d = orc_token_ratio(s1, l1, s2, l2, cutoff_d);
+        return 1;
     case ORC_QRATIO:
         /* CachedQRatio::similarity (fuzz_impl.hpp:912-922) */
         if (kind != ORC_SIMILARITY) return -1;

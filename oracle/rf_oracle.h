@@ -40,7 +40,9 @@ enum {
     ORC_PARTIAL_RATIO = 10, /* fuzz::CachedPartialRatio: similarity only */
     ORC_TOKEN_SORT_RATIO = 11,
     ORC_PARTIAL_TOKEN_SORT_RATIO = 12,
-    ORC_QRATIO = 13
+    ORC_QRATIO = 13,
+    ORC_TOKEN_SET_RATIO = 14,
+    ORC_TOKEN_RATIO = 15
 };
 
 enum { ORC_DISTANCE = 0, ORC_SIMILARITY = 1, ORC_NORM_DISTANCE = 2, ORC_NORM_SIMILARITY = 3 };

@@ -59,6 +59,8 @@ int algo_for(int metric, const rfa_weights& w)
     case RFA_PARTIAL_TOKEN_SORT_RATIO: return ALGO_PARTIAL;
     case RFA_TOKEN_SORT_RATIO:
     case RFA_QRATIO: return ALGO_LCS;
+    case RFA_TOKEN_SET_RATIO:
+    case RFA_TOKEN_RATIO: return ALGO_TSET;
     case RFA_JARO:
     case RFA_JARO_WINKLER: return ALGO_JARO;
     }
@@ -221,16 +223,18 @@ void free_strings(rfa_strings* s)
     delete s;
 }
 
-rfa_strings* token_sorted(rfa_context* ctx, const rfa_strings* in, bool signed8)
+// token-sorted copy of a set; with `table`, deduplicated, and the token table filled in
+static rfa_strings* token_transform(rfa_context* ctx, const rfa_strings* in, bool signed8, TokenTable* table)
 {
     hipStream_t s = ctx->stream;
     const uint64_t n = in->count, w = (uint64_t)in->width;
     const uint64_t units = in->h_off.back();
     const size_t scan = tokens_scan_bytes(n);
-    // one allocation: out_len, ntok, out_off, tok_off (n + 1 each), tok_start, tok_len, out units, scan temp
+    // one allocation: out_len, ntok, out_off, tok_off (n + 1 each), tok_start, tok_len, dup, out units,
+    // scan temp; the token table (set_off, hash, start, len) in its own allocation
     const size_t cnt = (size_t)(n + 1);
     const size_t tok = (size_t)std::max<uint64_t>(1, units);
-    size_t bytes = 4 * cnt * 8 + 2 * tok * 4 + tok * w + 64 + 256 + scan;
+    size_t bytes = 4 * cnt * 8 + 2 * tok * 4 + tok + tok * w + 64 + 256 + scan + 256;
     uint8_t* base = nullptr;
     if (hipMalloc(&base, bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -246,11 +250,30 @@ rfa_strings* token_sorted(rfa_context* ctx, const rfa_strings* in, bool signed8)
     a.tok_off = a.out_off + cnt;
     a.tok_start = (uint32_t*)(a.tok_off + cnt);
     a.tok_len = a.tok_start + tok;
-    a.out = (void*)(a.tok_len + tok);
+    a.dup = (uint8_t*)(a.tok_len + tok);
+    a.out = (void*)(((uintptr_t)(a.dup + tok) + 7) & ~(uintptr_t)7);
     a.scan_temp = (void*)(((uintptr_t)((uint8_t*)a.out + tok * w + 64) + 255) & ~(uintptr_t)255);
     a.scan_bytes = scan;
     a.signed8 = signed8 ? 1 : 0;
     a.num_cus = (uint32_t)ctx->num_cus;
+    if (table) {
+        // kept tokens: at most (units + 1) / 2
+        const size_t kt = (size_t)(units + 1) / 2 + 1;
+        if (hipMalloc(&table->d_blob, cnt * 8 + kt * 16) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipFree(base);
+            throw ApiError(RFA_ERR_OUT_OF_MEMORY, "device allocation for the token table failed");
+        }
+        table->d_off = (uint64_t*)table->d_blob;
+        table->d_hash = table->d_off + cnt;
+        table->d_start = (uint32_t*)(table->d_hash + kt);
+        table->d_len = table->d_start + kt;
+        a.dedup = 1;
+        a.set_off = table->d_off;
+        a.set_hash = table->d_hash;
+        a.set_start = table->d_start;
+        a.set_len = table->d_len;
+    }
     std::vector<uint64_t> h_off(cnt);
     std::vector<uint8_t> h_chars;
     hipError_t e = hipMemsetAsync(a.out_len + n, 0, 8, s);
@@ -267,8 +290,34 @@ rfa_strings* token_sorted(rfa_context* ctx, const rfa_strings* in, bool signed8)
     return make_strings(ctx, h_chars.data(), in->width, h_off.data(), n);
 }
 
+rfa_strings* token_sorted(rfa_context* ctx, const rfa_strings* in, bool signed8)
+{
+    return token_transform(ctx, in, signed8, nullptr);
+}
+
+TokenTable::~TokenTable()
+{
+    free_strings(joined);
+    if (d_blob) (void)hipFree(d_blob);
+}
+
+std::shared_ptr<TokenTable> token_table(rfa_context* ctx, const rfa_strings* in, bool signed8)
+{
+    auto t = std::make_shared<TokenTable>();
+    t->joined = token_transform(ctx, in, signed8, t.get());
+    return t;
+}
+
 const rfa_strings* scorer_choices(rfa_scorer* sc, const rfa_strings* ch)
 {
+    if (sc->token_set) {
+        if (!sc->c_tokens || sc->c_tokens_id != ch->id || sc->c_tokens_signed8 != sc->signed8) {
+            sc->c_tokens = token_table(sc->ctx, ch, sc->signed8);
+            sc->c_tokens_id = ch->id;
+            sc->c_tokens_signed8 = sc->signed8;
+        }
+        return sc->c_tokens->joined;
+    }
     if (!sc->token_sort) return ch;
     if (!sc->tok_choices || sc->tok_choices_id != ch->id || sc->tok_choices_signed8 != sc->signed8) {
         rfa_strings* t = token_sorted(sc->ctx, ch, sc->signed8);
@@ -316,8 +365,19 @@ rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries
         sc->metric = RFA_RATIO;
         sc->empty_zero = true;
     }
+    else if (metric == RFA_TOKEN_SET_RATIO || metric == RFA_TOKEN_RATIO) {
+        sc->token_set = true;
+    }
     try {
-        if (sc->token_sort) {
+        if (sc->token_set) {
+            sc->q_raw =
+                make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
+            sc->q_tokens = token_table(ctx, sc->q_raw, false);
+            sc->q = sc->q_tokens->joined; // the plan runs over the deduplicated joins
+            sc->q_tokens->joined = nullptr;
+            if (metric == RFA_TOKEN_RATIO) sc->sort_sub = make_scorer(ctx, RFA_TOKEN_SORT_RATIO, queries, w, prefix_weight);
+        }
+        else if (sc->token_sort) {
             sc->q_raw =
                 make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
             sc->q = token_sorted(ctx, sc->q_raw, false);
@@ -340,6 +400,7 @@ void free_scorer(rfa_scorer* sc)
     clear_plan(sc);
     free_strings(sc->q_raw);
     free_strings(sc->tok_choices);
+    free_scorer(sc->sort_sub);
     delete sc;
 }
 
@@ -348,6 +409,15 @@ void scorer_set_signed(rfa_scorer* sc, bool on)
 {
     if (sc->signed8 == on) return;
     sc->signed8 = on;
+    if (sc->sort_sub) scorer_set_signed(sc->sort_sub, on);
+    if (sc->token_set) {
+        clear_plan(sc);
+        sc->q_tokens = token_table(sc->ctx, sc->q_raw, on);
+        sc->q = sc->q_tokens->joined;
+        sc->q_tokens->joined = nullptr;
+        plan_scorer(sc);
+        return;
+    }
     if (!sc->token_sort) return;
     clear_plan(sc);
     sc->q = token_sorted(sc->ctx, sc->q_raw, on);
@@ -461,6 +531,76 @@ void run_block(rfa_context* ctx, int algo, int width, BlockArgs a, uint64_t max_
     // are taken in batches of consecutive items, about 8 grabs per workgroup
     a.batch = a.nitems_dev ? 1u : (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, a.nitems / (8ull * nblocks)));
     HIP_CHECK(launch_block(algo, width, a, nblocks, slots, s));
+}
+
+// token_set_ratio / token_ratio over the raw choice set: the deduplicated joins and token
+// tables of both sides, then the set kernel; token_ratio first writes the token-sort
+// ratio (its own scorer) and the set kernel combines.
+void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics, double cd, void* out,
+                int out_on_device)
+{
+    rfa_context* ctx = sc->ctx;
+    hipStream_t s = ctx->stream;
+    if (kind != RFA_SIMILARITY) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "token ratios only provide similarity");
+    if (semantics != RFA_CACHED)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "token_set_ratio and token_ratio have no Multi* scorer");
+    if (raw->width != sc->q->width)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
+    if (!sc->block_queries.empty())
+        throw ApiError(RFA_ERR_UNSUPPORTED,
+                       "token_set_ratio / token_ratio take queries whose deduplicated token join has at most 64 "
+                       "code units (and tiles of at most 254 distinct symbols)");
+    const uint64_t nq = sc->q->count, nc = raw->count;
+    if (nq == 0 || nc == 0) return;
+    if (nc > 0xffffffffull || nq > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 strings");
+    const size_t bytes = (size_t)nq * nc * 8;
+    void* dout = out_on_device ? out : ctx->out_tmp.get(bytes);
+    const rfa_strings* ch = scorer_choices(sc, raw);
+    const TokenTable& ct = *sc->c_tokens;
+    int mode = 0;
+    if (sc->sort_sub) {
+        cdist_impl(sc->sort_sub, scorer_choices(sc->sort_sub, raw), RFA_SIMILARITY, RFA_CACHED, 0, cd, dout, 1);
+        mode = 1;
+    }
+    if (!sc->classes.empty()) {
+        TsetArgs a{};
+        TileArgs& ta = a.t;
+        ta.qchars = sc->q->d_chars;
+        ta.qoff = sc->q->d_off;
+        ta.qorder = sc->d_tile_order;
+        ta.nq = (uint32_t)sc->tile_order.size();
+        ta.cchars = ch->d_chars;
+        ta.coff = ch->d_off;
+        ta.ncsel = (uint32_t)nc;
+        ta.out_stride = nc;
+        ta.hash_bits = sc->hash_bits;
+        ta.spec = make_spec(sc, RFA_SIMILARITY, RFA_CACHED, 0, cd, ALGO_TSET);
+        ta.out = dout;
+        a.q_set_off = sc->q_tokens->d_off;
+        a.q_set_start = sc->q_tokens->d_start;
+        a.q_set_len = sc->q_tokens->d_len;
+        a.q_set_hash = sc->q_tokens->d_hash;
+        a.c_set_off = ct.d_off;
+        a.c_set_start = ct.d_start;
+        a.c_set_len = ct.d_len;
+        a.c_set_hash = ct.d_hash;
+        a.mode = mode;
+        for (const TileClass& tc : sc->classes) {
+            ta.tile_base = tc.first_tile;
+            ta.rows_cap = tc.rows_cap;
+            uint32_t target_blocks = (uint32_t)ctx->num_cus * 12;
+            uint32_t nbx = std::max<uint32_t>(1, (target_blocks + tc.ntiles - 1) / tc.ntiles);
+            uint32_t cpb = std::max<uint32_t>(8, (ta.ncsel + nbx - 1) / nbx);
+            cpb = ((cpb + 7) / 8) * 8; // whole rounds of the 8 waves
+            ta.choices_per_block = cpb;
+            nbx = (ta.ncsel + cpb - 1) / cpb;
+            HIP_CHECK(launch_tset(sc->q->width, tc.word_bits, a, tc.ntiles, nbx, s));
+        }
+    }
+    if (!out_on_device) {
+        HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
 }
 
 // partial_ratio: tile-planned queries (<= 64 units) through the tile kernel, the
@@ -1043,7 +1183,8 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
 {
     if (!ctx || !queries || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, queries or out is NULL");
     return guarded([&] {
-        if (metric < RFA_LEVENSHTEIN || metric > RFA_QRATIO) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
+        if (metric < RFA_LEVENSHTEIN || metric > RFA_TOKEN_RATIO)
+            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
         if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};
         // weights MultiLevenshtein accepts are kept even when Cached* scoring cannot use them
@@ -1091,7 +1232,10 @@ rfa_status rfa_cdist(rfa_scorer* sc, const rfa_strings* choices, int kind, int s
         begin_call(sc->ctx);
         if (choices->width != sc->q->width)
             throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
-        cdist_impl(sc, scorer_choices(sc, choices), kind, semantics, cutoff_u, cutoff_d, out, out_on_device);
+        if (sc->token_set)
+            cdist_tset(sc, choices, kind, semantics, cutoff_d, out, out_on_device);
+        else
+            cdist_impl(sc, scorer_choices(sc, choices), kind, semantics, cutoff_u, cutoff_d, out, out_on_device);
     });
 }
 

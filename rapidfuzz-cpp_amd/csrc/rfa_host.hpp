@@ -269,6 +269,17 @@ struct rfa_strings {
 
 namespace rfa {
 
+// deduplicated token-sorted copy of a string set and its token table (device)
+struct TokenTable {
+    rfa_strings* joined = nullptr; // the joins (owned unless handed over)
+    uint8_t* d_blob = nullptr;
+    uint64_t* d_off = nullptr;     // count + 1: the tokens of string i are [d_off[i], d_off[i + 1])
+    uint64_t* d_hash = nullptr;
+    uint32_t* d_start = nullptr;   // start in the string's join
+    uint32_t* d_len = nullptr;
+    ~TokenTable();
+};
+
 struct TileClass {
     uint32_t first_tile = 0, ntiles = 0;
     int word_bits = 64;
@@ -299,7 +310,15 @@ struct rfa_scorer {
     bool token_sort = false;
     bool empty_zero = false;
     bool signed8 = false;            // RFA_SCORER_SIGNED_UNITS
-    rfa_strings* q_raw = nullptr;    // token-sort scorers: the queries as given
+    rfa_strings* q_raw = nullptr;    // token-sort / token-set scorers: the queries as given
+    // token-set scorers: the queries' token table (q holds their deduplicated joins), the
+    // choices' (cached per set), and token_ratio's token-sort scorer
+    bool token_set = false;
+    std::shared_ptr<rfa::TokenTable> q_tokens;
+    std::shared_ptr<rfa::TokenTable> c_tokens;
+    uint64_t c_tokens_id = 0;
+    bool c_tokens_signed8 = false;
+    rfa_scorer* sort_sub = nullptr;
     rfa_strings* tok_choices = nullptr; // token-sorted copy of the last choice set
     uint64_t tok_choices_id = 0;
     bool tok_choices_signed8 = false;
@@ -314,6 +333,8 @@ bool is_int_kind(int metric, int kind);
 rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const uint64_t* off, uint64_t count);
 // token-sorted copy of a set (device transform on the context's stream)
 rfa_strings* token_sorted(rfa_context* ctx, const rfa_strings* in, bool signed8);
+// deduplicated token-sorted copy and token table
+std::shared_ptr<rfa::TokenTable> token_table(rfa_context* ctx, const rfa_strings* in, bool signed8);
 // the choice set a scorer's kernels read: its token-sorted copy for token-sort scorers
 const rfa_strings* scorer_choices(rfa_scorer* sc, const rfa_strings* ch);
 void free_strings(rfa_strings* s);

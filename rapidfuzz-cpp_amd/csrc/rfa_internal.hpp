@@ -184,9 +184,37 @@ struct TokenArgs {
     size_t scan_bytes;
     int signed8; // 8-bit units compare as signed (the reference's `char`)
     uint32_t num_cus;
+    // deduplicated form (token sets): dup (one byte per token), set_off (count + 1) and the
+    // kept tokens' start (in the output string), length and hash, in sorted order
+    int dedup;
+    uint8_t* dup;
+    uint64_t* set_off;
+    uint32_t* set_start;
+    uint32_t* set_len;
+    uint64_t* set_hash;
 };
 size_t tokens_scan_bytes(uint64_t count);
 hipError_t launch_tokens(int char_width, const TokenArgs& a, hipStream_t s);
+
+// token_set_ratio / token_ratio (rfa_tset.hip): query tiles of deduplicated token
+// joins (TileArgs over them) against choices' joins; the token tables give each
+// string's tokens in sorted order (start in its join, length, hash).  mode 0 writes
+// token_set_ratio; mode 1 combines the token-sort ratio already in out into token_ratio.
+struct TsetArgs {
+    TileArgs t;
+    const uint64_t* q_set_off;
+    const uint32_t* q_set_start;
+    const uint32_t* q_set_len;
+    const uint64_t* q_set_hash;
+    const uint64_t* c_set_off;
+    const uint32_t* c_set_start;
+    const uint32_t* c_set_len;
+    const uint64_t* c_set_hash;
+    int mode;
+    uint32_t tok_lds_u64; // set by launch_tset
+};
+hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
+                       hipStream_t s);
 
 // Widest shorter string (in 64-unit words) the long path takes.
 constexpr uint32_t PARTIAL_MAX_WORDS = 16;

@@ -19,6 +19,10 @@
 //                   (ties by position: equal tokens are identical) and sums the
 //                   lengths of the tokens before it, which is its output offset;
 //                   no lane waits for another.
+// The deduplicated form (SplittedSentenceView::dedupe, token_set_ratio's sets)
+// runs k_tok_dedup after the split: a token equal to an earlier one is dropped,
+// and the output length and token count are those of the kept tokens; the write
+// then also emits each kept token's (start, length, hash) in sorted order.
 #include <hipcub/hipcub.hpp>
 
 #include "rfa_device.hpp"
@@ -103,6 +107,19 @@ __global__ __launch_bounds__(TOK_THREADS) void k_tok_split(const CharT* __restri
     }
 }
 
+// 64-bit FNV-1a of a token's code units (equal tokens hash equal; users compare the
+// units of tokens whose hashes match)
+template <typename CharT>
+__device__ __forceinline__ uint64_t tok_hash(const CharT* __restrict__ p, uint32_t n)
+{
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (uint32_t k = 0; k < n; ++k) {
+        h ^= (uint64_t)(uint32_t)p[k];
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}
+
 // code unit as the comparison sees it
 template <typename CharT, bool SIGNED8>
 __device__ __forceinline__ int64_t tok_key(CharT c)
@@ -126,14 +143,64 @@ __device__ __forceinline__ int tok_cmp(const CharT* __restrict__ a, uint32_t la,
     return la == lb ? 0 : (la < lb ? -1 : 1);
 }
 
-template <typename CharT, bool SIGNED8>
+// dup[t] = 1 for a token equal to an earlier token of its string; out_len / ntok of the kept tokens
+template <typename CharT>
+__global__ __launch_bounds__(TOK_THREADS) void k_tok_dedup(const CharT* __restrict__ chars,
+                                                           const uint64_t* __restrict__ off, uint64_t count,
+                                                           const uint64_t* __restrict__ tok_off,
+                                                           const uint32_t* __restrict__ tok_start,
+                                                           const uint32_t* __restrict__ tok_len,
+                                                           uint8_t* __restrict__ dup, uint64_t* __restrict__ out_len,
+                                                           uint64_t* __restrict__ ntok)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * TOK_THREADS + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * TOK_THREADS) >> 6;
+    for (uint64_t s = wave0; s < count; s += nwaves) {
+        const CharT* src = chars + off[s];
+        const uint64_t t0 = tok_off[s], T = tok_off[s + 1] - t0;
+        uint64_t units = 0, kept = 0;
+        for (uint64_t a = lane; a < T; a += 64) {
+            const uint32_t sa = tok_start[t0 + a], la = tok_len[t0 + a];
+            bool d = false;
+            for (uint64_t b = 0; b < a && !d; ++b) {
+                const uint32_t lb = tok_len[t0 + b];
+                if (lb != la) continue;
+                const uint32_t sb = tok_start[t0 + b];
+                uint32_t k = 0;
+                while (k < la && src[sa + k] == src[sb + k]) ++k;
+                d = k == la;
+            }
+            dup[t0 + a] = d ? 1 : 0;
+            if (!d) {
+                units += la;
+                ++kept;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            units += __shfl_xor(units, o, 64);
+            kept += __shfl_xor(kept, o, 64);
+        }
+        if (lane == 0) {
+            out_len[s] = units + (kept ? kept - 1 : 0);
+            ntok[s] = kept;
+        }
+    }
+}
+
+template <typename CharT, bool SIGNED8, bool DEDUP>
 __global__ __launch_bounds__(TOK_THREADS) void k_tok_write(const CharT* __restrict__ chars,
                                                            const uint64_t* __restrict__ off, uint64_t count,
                                                            const uint64_t* __restrict__ tok_off,
                                                            const uint32_t* __restrict__ tok_start,
                                                            const uint32_t* __restrict__ tok_len,
                                                            const uint64_t* __restrict__ out_off,
-                                                           CharT* __restrict__ out)
+                                                           CharT* __restrict__ out, const uint8_t* __restrict__ dup,
+                                                           const uint64_t* __restrict__ set_off,
+                                                           uint32_t* __restrict__ set_start,
+                                                           uint32_t* __restrict__ set_len,
+                                                           uint64_t* __restrict__ set_hash)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t wave0 = ((uint64_t)blockIdx.x * TOK_THREADS + threadIdx.x) >> 6;
@@ -142,11 +209,13 @@ __global__ __launch_bounds__(TOK_THREADS) void k_tok_write(const CharT* __restri
         const CharT* src = chars + off[s];
         const uint64_t t0 = tok_off[s], T = tok_off[s + 1] - t0;
         CharT* dst = out + out_off[s];
+        const uint64_t Tout = DEDUP ? set_off[s + 1] - set_off[s] : T;
         for (uint64_t a = lane; a < T; a += 64) {
+            if (DEDUP && dup[t0 + a]) continue;
             const uint32_t sa = tok_start[t0 + a], la = tok_len[t0 + a];
             uint64_t rank = 0, pos = 0;
             for (uint64_t b = 0; b < T; ++b) {
-                if (b == a) continue;
+                if (b == a || (DEDUP && dup[t0 + b])) continue;
                 const uint32_t sb = tok_start[t0 + b], lb = tok_len[t0 + b];
                 const int c = tok_cmp<CharT, SIGNED8>(src + sb, lb, src + sa, la);
                 if (c < 0 || (c == 0 && b < a)) {
@@ -155,7 +224,13 @@ __global__ __launch_bounds__(TOK_THREADS) void k_tok_write(const CharT* __restri
                 }
             }
             for (uint32_t k = 0; k < la; ++k) dst[pos + k] = src[sa + k];
-            if (rank + 1 < T) dst[pos + la] = (CharT)0x20;
+            if (rank + 1 < Tout) dst[pos + la] = (CharT)0x20;
+            if constexpr (DEDUP) {
+                const uint64_t e = set_off[s] + rank;
+                set_start[e] = (uint32_t)pos;
+                set_len[e] = la;
+                set_hash[e] = tok_hash<CharT>(src + sa, la);
+            }
         }
     }
 }
@@ -178,19 +253,43 @@ static hipError_t run_tokens(const TokenArgs& a, hipStream_t s)
     hipLaunchKernelGGL(k_tok_count<CharT>, dim3(blocks), dim3(TOK_THREADS), 0, s, in, a.off, a.count, a.out_len,
                        a.ntok);
     size_t sb = a.scan_bytes;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(a.scan_temp, sb, a.out_len, a.out_off, (int)(a.count + 1), s);
-    if (e != hipSuccess) return e;
+    hipError_t e = hipSuccess;
+    if (!a.dedup) {
+        e = hipcub::DeviceScan::ExclusiveSum(a.scan_temp, sb, a.out_len, a.out_off, (int)(a.count + 1), s);
+        if (e != hipSuccess) return e;
+    }
     sb = a.scan_bytes;
     e = hipcub::DeviceScan::ExclusiveSum(a.scan_temp, sb, a.ntok, a.tok_off, (int)(a.count + 1), s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_tok_split<CharT>, dim3(blocks), dim3(TOK_THREADS), 0, s, in, a.off, a.count, a.tok_off,
                        a.tok_start, a.tok_len);
-    if (a.signed8)
-        hipLaunchKernelGGL((k_tok_write<CharT, true>), dim3(blocks), dim3(TOK_THREADS), 0, s, in, a.off, a.count,
-                           a.tok_off, a.tok_start, a.tok_len, a.out_off, (CharT*)a.out);
+    if (a.dedup) {
+        // kept tokens: the output length and the token count again (ntok is free after its scan)
+        hipLaunchKernelGGL(k_tok_dedup<CharT>, dim3(blocks), dim3(TOK_THREADS), 0, s, in, a.off, a.count, a.tok_off,
+                           a.tok_start, a.tok_len, a.dup, a.out_len, a.ntok);
+        sb = a.scan_bytes;
+        e = hipcub::DeviceScan::ExclusiveSum(a.scan_temp, sb, a.out_len, a.out_off, (int)(a.count + 1), s);
+        if (e != hipSuccess) return e;
+        sb = a.scan_bytes;
+        e = hipcub::DeviceScan::ExclusiveSum(a.scan_temp, sb, a.ntok, a.set_off, (int)(a.count + 1), s);
+        if (e != hipSuccess) return e;
+        if (a.signed8)
+            hipLaunchKernelGGL((k_tok_write<CharT, true, true>), dim3(blocks), dim3(TOK_THREADS), 0, s, in, a.off,
+                               a.count, a.tok_off, a.tok_start, a.tok_len, a.out_off, (CharT*)a.out, a.dup, a.set_off,
+                               a.set_start, a.set_len, a.set_hash);
+        else
+            hipLaunchKernelGGL((k_tok_write<CharT, false, true>), dim3(blocks), dim3(TOK_THREADS), 0, s, in, a.off,
+                               a.count, a.tok_off, a.tok_start, a.tok_len, a.out_off, (CharT*)a.out, a.dup, a.set_off,
+                               a.set_start, a.set_len, a.set_hash);
+    }
+    else if (a.signed8)
+        hipLaunchKernelGGL((k_tok_write<CharT, true, false>), dim3(blocks), dim3(TOK_THREADS), 0, s, in, a.off,
+                           a.count, a.tok_off, a.tok_start, a.tok_len, a.out_off, (CharT*)a.out, nullptr, nullptr,
+                           nullptr, nullptr, nullptr);
     else
-        hipLaunchKernelGGL((k_tok_write<CharT, false>), dim3(blocks), dim3(TOK_THREADS), 0, s, in, a.off, a.count,
-                           a.tok_off, a.tok_start, a.tok_len, a.out_off, (CharT*)a.out);
+        hipLaunchKernelGGL((k_tok_write<CharT, false, false>), dim3(blocks), dim3(TOK_THREADS), 0, s, in, a.off,
+                           a.count, a.tok_off, a.tok_start, a.tok_len, a.out_off, (CharT*)a.out, nullptr, nullptr,
+                           nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 

@@ -1,0 +1,255 @@
+// rfa_tset.hip -- fuzz::token_set_ratio and the set part of fuzz::token_ratio
+// for cdist (fuzz_impl.hpp:370-426, 518-565; set_decomposition common_impl.hpp:11-35).
+//
+// Both strings are taken as their deduplicated, sorted token lists U (joined
+// with single spaces) plus a token table (start in U, length, hash).  Per pair:
+//   sect = tokens in both, ab = the query's other tokens, ba = the choice's;
+//   dist = indel_distance(join(ab), join(ba)).
+// join(ab) is U_q with the sect tokens (and the spaces they leave doubled)
+// removed; a unit that can match nothing does not change an LCS, so
+// LCS(join(ab), join(ba)) is the bit-parallel LCS of U_q's pattern with the
+// removed positions masked out against U_c with the removed units stepped as
+// "no match".  No per-pair string is built.
+//
+// Query tiles (the scorer's tile plan over the queries' U, at most 64 units,
+// one query per lane, PM[row][lane] in LDS) against choices uniform across the
+// wave.  Each lane keeps its query's token table in LDS columns and a 64-bit
+// Bloom mask of its token hashes; a choice token is compared only when the
+// Bloom bit is set, and tokens whose hashes match are compared unit by unit.
+#include <cmath>
+
+#include "rfa_device.hpp"
+#include "rfa_internal.hpp"
+#include "rfa_semantics.hpp"
+#include "rfa_steps.hpp"
+#include "rfa_tile_pm.hpp"
+
+namespace rfa {
+
+constexpr uint32_t TSET_MAX_QTOK = 32; // a join of <= 64 units has <= 32 tokens
+
+// fuzz_detail::norm_distance (fuzz_impl.hpp:55-61)
+__device__ __forceinline__ double fz_norm_distance(uint64_t dist, uint64_t lensum, double cut)
+{
+    const double score = (lensum > 0) ? (100.0 - 100.0 * (double)dist / (double)lensum) : 100.0;
+    return (score >= cut) ? score : 0.0;
+}
+
+template <typename W>
+struct TsLcs;
+
+template <>
+struct TsLcs<uint32_t> {
+    uint32_t S;
+    __device__ __forceinline__ void reset() { S = ~0u; }
+    __device__ __forceinline__ void step(uint32_t X) { lcs_step(X, S); }
+    __device__ __forceinline__ uint32_t count() const { return (uint32_t)__popc(~S); }
+};
+
+template <>
+struct TsLcs<uint64_t> {
+    uint32_t Sl, Sh;
+    __device__ __forceinline__ void reset() { Sl = Sh = ~0u; }
+    __device__ __forceinline__ void step(uint64_t X) { lcs_step(X, Sl, Sh); }
+    __device__ __forceinline__ uint32_t count() const { return (uint32_t)(__popc(~Sl) + __popc(~Sh)); }
+};
+
+template <typename CharT, typename W, int MODE>
+__global__ __launch_bounds__(TILE_THREADS) void k_tset(TsetArgs a)
+{
+    extern __shared__ uint64_t smem_u64[];
+    const TileArgs& ta = a.t;
+    W* pm = (W*)smem_u64;
+    uint8_t* extra = (uint8_t*)(pm + (size_t)ta.rows_cap * 64);
+    // the query token tables after the tile's PM rows and symbol map
+    uint64_t* qhash = (uint64_t*)(smem_u64 + a.tok_lds_u64);
+    uint8_t* qst = (uint8_t*)(qhash + TSET_MAX_QTOK * 64);
+    uint8_t* qln = qst + TSET_MAX_QTOK * 64;
+    const CharT* __restrict__ q = (const CharT*)ta.qchars;
+    const CharT* __restrict__ c = (const CharT*)ta.cchars;
+
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t tile = blockIdx.y + ta.tile_base;
+    const uint32_t qpos = tile * 64 + lane;
+    const bool qact = qpos < ta.nq;
+    const uint32_t qidx = qact ? ta.qorder[qpos] : 0;
+    const uint64_t qo = qact ? ta.qoff[qidx] : 0;
+    const uint32_t m = qact ? (uint32_t)(ta.qoff[qidx + 1] - qo) : 0;
+
+    SymLookup<CharT> look = build_tile_pm<CharT, W>(q, qo, m, pm, extra, ta);
+    const W* __restrict__ col = pm + lane;
+    const double cut = ta.spec.cutoff_d;
+
+    // this lane's query tokens: hash, start and length in LDS columns (each wave holds
+    // the same tile, so every wave writes the same values into its own lane's column)
+    uint32_t Tq = 0;
+    uint64_t bloom = 0;
+    if (qact) {
+        const uint64_t t0 = a.q_set_off[qidx];
+        Tq = (uint32_t)(a.q_set_off[qidx + 1] - t0);
+        for (uint32_t k = 0; k < Tq; ++k) {
+            const uint64_t h = a.q_set_hash[t0 + k];
+            if (wave == 0) {
+                qhash[k * 64 + lane] = h;
+                qst[k * 64 + lane] = (uint8_t)a.q_set_start[t0 + k];
+                qln[k * 64 + lane] = (uint8_t)a.q_set_len[t0 + k];
+            }
+            bloom |= 1ull << (h & 63);
+        }
+    }
+    __syncthreads();
+
+    // is choice token (h, len, units at p) one of this lane's query tokens?  -1: no, else its index
+    auto find_q = [&](uint64_t h, uint32_t len, const CharT* __restrict__ p) -> int {
+        if (!qact || !((bloom >> (h & 63)) & 1)) return -1;
+        for (uint32_t k = 0; k < Tq; ++k) {
+            if (qhash[k * 64 + lane] != h || qln[k * 64 + lane] != len) continue;
+            const CharT* qs = q + qo + qst[k * 64 + lane];
+            uint32_t i = 0;
+            while (i < len && qs[i] == p[i]) ++i;
+            if (i == len) return (int)k;
+        }
+        return -1;
+    };
+
+    const uint32_t it0 = blockIdx.x * ta.choices_per_block;
+    const uint32_t it1 = min(ta.ncsel, it0 + ta.choices_per_block);
+    for (uint32_t cidx = it0 + uniform(wave); cidx < it1; cidx += TILE_WAVES) {
+        const uint64_t co0 = ta.coff[cidx];
+        const uint64_t co = ((uint64_t)uniform((uint32_t)(co0 >> 32)) << 32) | uniform((uint32_t)co0);
+        const uint64_t tc0 = a.c_set_off[cidx];
+        const uint32_t Tc = uniform((uint32_t)(a.c_set_off[cidx + 1] - tc0));
+        double v = 0.0;
+        bool perfect = false;
+        if (cut <= 100) {
+            // ---- pass 1: the intersection ----
+            uint32_t inter_q = 0, n_inter = 0, n_kept_c = 0;
+            uint64_t ba_units = 0, kept_c_lo = 0;
+            for (uint32_t t = 0; t < Tc; ++t) {
+                const uint64_t h = a.c_set_hash[tc0 + t];
+                const uint32_t st = a.c_set_start[tc0 + t], ln = a.c_set_len[tc0 + t];
+                const int k = find_q(h, ln, c + co + st);
+                if (k >= 0) {
+                    inter_q |= 1u << k;
+                    ++n_inter;
+                }
+                else {
+                    ba_units += ln;
+                    ++n_kept_c;
+                    if (t < 64) kept_c_lo |= 1ull << t;
+                }
+            }
+            // ---- the query side: ab, sect and the pattern mask of join(ab) ----
+            uint64_t ab_units = 0, sect_units = 0;
+            uint32_t n_kept_q = 0;
+            W mask = 0;
+            for (uint32_t k = 0; k < Tq; ++k) {
+                const uint32_t st = qst[k * 64 + lane], ln = qln[k * 64 + lane];
+                if ((inter_q >> k) & 1) {
+                    sect_units += ln;
+                }
+                else {
+                    ab_units += ln;
+                    const W bits = (ln >= sizeof(W) * 8) ? ~(W)0 : (((W)1 << ln) - 1);
+                    mask |= bits << st;
+                    if (n_kept_q) mask |= (W)1 << (st - 1); // the space before it
+                    ++n_kept_q;
+                }
+            }
+            const uint64_t ab_len = ab_units + (n_kept_q ? n_kept_q - 1 : 0);
+            const uint64_t ba_len = ba_units + (n_kept_c ? n_kept_c - 1 : 0);
+            const uint64_t sect_len = sect_units + (n_inter ? n_inter - 1 : 0);
+            perfect = n_inter > 0 && (n_kept_q == 0 || n_kept_c == 0);
+            // ---- pass 2: LCS(join(ab), join(ba)) over U_c ----
+            const bool need = qact && !perfect;
+            uint32_t lcs = 0;
+            if (ballot(need)) {
+                TsLcs<W> S;
+                S.reset();
+                bool seen = false;
+                for (uint32_t t = 0; t < Tc; ++t) {
+                    const uint32_t st = a.c_set_start[tc0 + t], ln = a.c_set_len[tc0 + t];
+                    bool kept;
+                    if (t < 64)
+                        kept = (kept_c_lo >> t) & 1;
+                    else
+                        kept = find_q(a.c_set_hash[tc0 + t], ln, c + co + st) < 0;
+                    if (t > 0) {
+                        const W X = col[look((uint32_t)c[co + st - 1]) * 64];
+                        S.step(kept && seen ? (X & mask) : (W)0);
+                    }
+                    for (uint32_t i = 0; i < ln; ++i) {
+                        const W X = col[look((uint32_t)c[co + st + i]) * 64];
+                        S.step(kept ? (X & mask) : (W)0);
+                    }
+                    seen = seen || kept;
+                }
+                lcs = S.count();
+            }
+            if (need) {
+                const uint64_t dist = ab_len + ba_len - 2 * (uint64_t)lcs;
+                const uint64_t sect_ab_len = sect_len + (sect_len != 0) + ab_len;
+                const uint64_t sect_ba_len = sect_len + (sect_len != 0) + ba_len;
+                const uint64_t lensum = sect_ab_len + sect_ba_len;
+                const uint64_t cutoff_distance = (uint64_t)ceil((double)lensum * (1.0 - cut / 100));
+                if (dist <= cutoff_distance) v = fz_norm_distance(dist, lensum, cut);
+                if (sect_len) {
+                    v = dmax(v, fz_norm_distance((sect_len != 0) + ab_len, sect_len + sect_ab_len, cut));
+                    v = dmax(v, fz_norm_distance((sect_len != 0) + ba_len, sect_len + sect_ba_len, cut));
+                }
+            }
+        }
+        if (qact) {
+            double* dst = (double*)ta.out + (uint64_t)qidx * ta.out_stride + cidx;
+            double r;
+            if (cut > 100)
+                r = 0.0;
+            else if (MODE == 0)
+                r = (Tq == 0 || Tc == 0) ? 0.0 : (perfect ? 100.0 : v);
+            else
+                r = perfect ? 100.0 : dmax(*dst, v); // token_ratio: out holds the sorted-join ratio
+            *dst = r;
+        }
+    }
+}
+
+size_t tset_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t hash_bits, uint32_t* tok_lds_u64)
+{
+    const size_t tile = (tile_lds_bytes(char_width, word_bits, rows_cap, hash_bits) + 7) & ~(size_t)7;
+    *tok_lds_u64 = (uint32_t)(tile / 8);
+    return tile + TSET_MAX_QTOK * 64 * (8 + 2);
+}
+
+template <typename CharT, typename W>
+static void launch_tset_w(const TsetArgs& a, uint32_t ntiles, uint32_t nbx, size_t lds, hipStream_t s)
+{
+    if (a.mode == 0)
+        hipLaunchKernelGGL((k_tset<CharT, W, 0>), dim3(nbx, ntiles), dim3(TILE_THREADS), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_tset<CharT, W, 1>), dim3(nbx, ntiles), dim3(TILE_THREADS), lds, s, a);
+}
+
+hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
+                       hipStream_t s)
+{
+    const size_t lds = tset_lds_bytes(char_width, word_bits, a.t.rows_cap, a.t.hash_bits, &a.tok_lds_u64);
+    if (word_bits == 32) {
+        if (char_width == 1)
+            launch_tset_w<uint8_t, uint32_t>(a, ntiles, nblocks_x, lds, s);
+        else if (char_width == 2)
+            launch_tset_w<uint16_t, uint32_t>(a, ntiles, nblocks_x, lds, s);
+        else
+            launch_tset_w<uint32_t, uint32_t>(a, ntiles, nblocks_x, lds, s);
+    }
+    else {
+        if (char_width == 1)
+            launch_tset_w<uint8_t, uint64_t>(a, ntiles, nblocks_x, lds, s);
+        else if (char_width == 2)
+            launch_tset_w<uint16_t, uint64_t>(a, ntiles, nblocks_x, lds, s);
+        else
+            launch_tset_w<uint32_t, uint64_t>(a, ntiles, nblocks_x, lds, s);
+    }
+    return hipGetLastError();
+}
+
+} // namespace rfa

@@ -30,7 +30,8 @@ enum Algo : int32_t {
     ALGO_MIXED = 4, // per-item metric (Levenshtein / Indel / LCSseq / OSA), chosen at run time
     ALGO_POSIT = 5, // positional comparison: Hamming (Hamming_impl.hpp:23-35), common prefix / suffix
                     // (Prefix_impl.hpp:23-27, Postfix_impl.hpp:23-27)
-    ALGO_PARTIAL = 6 // LCS of the shorter string against every window of the longer one (fuzz_impl.hpp:57-279)
+    ALGO_PARTIAL = 6, // LCS of the shorter string against every window of the longer one (fuzz_impl.hpp:57-279)
+    ALGO_TSET = 7     // masked LCS of two token-set differences (fuzz_impl.hpp:370-426)
 };
 
 // A packed string set on the device.

@@ -685,6 +685,35 @@ struct CachedQRatio : detail::CachedFuzz<RFA_QRATIO> {
     {}
 };
 
+// fuzz::CachedTokenSetRatio (fuzz.hpp:412-434) and fuzz::CachedTokenRatio (fuzz.hpp:530-557):
+// token sets; similarity / similarity_many (no extract_topk).  Queries whose deduplicated token
+// join exceeds 64 code units throw.
+template <typename CharT1>
+struct CachedTokenSetRatio : detail::CachedFuzz<RFA_TOKEN_SET_RATIO> {
+    template <typename InputIt1>
+    CachedTokenSetRatio(InputIt1 first1, InputIt1 last1)
+        : detail::CachedFuzz<RFA_TOKEN_SET_RATIO>(first1, last1, std::is_signed<CharT1>::value && sizeof(CharT1) == 1)
+    {}
+    template <typename Sentence1>
+    CachedTokenSetRatio(const Sentence1& s1) : CachedTokenSetRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+};
+
+template <typename CharT1>
+struct CachedTokenRatio : detail::CachedFuzz<RFA_TOKEN_RATIO> {
+    template <typename InputIt1>
+    CachedTokenRatio(InputIt1 first1, InputIt1 last1)
+        : detail::CachedFuzz<RFA_TOKEN_RATIO>(first1, last1, std::is_signed<CharT1>::value && sizeof(CharT1) == 1)
+    {}
+    template <typename Sentence1>
+    CachedTokenRatio(const Sentence1& s1) : CachedTokenRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+};
+
+template <typename Sentence1>
+CachedTokenSetRatio(const Sentence1& s1) -> CachedTokenSetRatio<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+CachedTokenRatio(const Sentence1& s1) -> CachedTokenRatio<detail::sentence_char_t<Sentence1>>;
 template <typename Sentence1>
 CachedRatio(const Sentence1& s1) -> CachedRatio<detail::sentence_char_t<Sentence1>>;
 template <typename Sentence1>

@@ -110,6 +110,13 @@ int main()
             CHECK(near(rfa::fuzz::CachedQRatio<char>(std::string("test")).similarity(std::string("")), 0.0));
             // `char` tokens order as signed bytes: "\xe9x" sorts before "ab", so the join is
             // "\xe9x ab" (88.9 against "\xe9xab"; unsigned order would give "ab \xe9x", 44.4)
+            // tests-fuzz.cpp:88-94, 125-135
+            const std::string s4 = "new york mets vs atlanta braves", s5 = "atlanta braves vs new york mets";
+            CHECK(near(rfa::fuzz::CachedTokenSetRatio<char>(s4).similarity(s5), 100.0));
+            CHECK(near(rfa::fuzz::CachedTokenSetRatio<char>(std::string("a{")).similarity(std::string("{b")), 50.0));
+            CHECK(near(rfa::fuzz::CachedTokenSetRatio<char>(std::string("")).similarity(std::string("")), 0.0));
+            CHECK(near(rfa::fuzz::CachedTokenRatio<char>(std::string("")).similarity(std::string("")), 100.0));
+            CHECK(near(rfa::fuzz::CachedTokenRatio<char>(s4).similarity(s5), 100.0));
             rfa::fuzz::CachedTokenSortRatio<char> acc(std::string("\xe9x ab"));
             CHECK(near(acc.similarity(std::string("\xe9xab")), 88.8888889));
             rfa::fuzz::CachedTokenSortRatio<unsigned char> accu(std::basic_string<unsigned char>(

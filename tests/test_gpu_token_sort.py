@@ -1,6 +1,7 @@
-"""GPU parity of token_sort_ratio, partial_token_sort_ratio and QRatio (rfa_cdist /
-rfa_topk) against the oracle, bit for bit: the device token sort for 8/16/32-bit
-units and signed 8-bit order, and the cached token-sorted choice set."""
+"""GPU parity of token_sort_ratio, partial_token_sort_ratio, QRatio (rfa_cdist / rfa_topk),
+token_set_ratio and token_ratio (rfa_cdist) against the oracle, bit for bit: the device
+token sort (plain and deduplicated) for 8/16/32-bit units and signed 8-bit order, the
+cached token-sorted choice sets and the masked-LCS set kernel."""
 import numpy as np
 import pytest
 
@@ -8,7 +9,7 @@ import golden_cases as G
 
 pytestmark = pytest.mark.gpu
 
-TSR, PTSR, QR, SIM = 11, 12, 13, 1
+TSR, PTSR, QR, TSET, TRATIO, SIM = 11, 12, 13, 14, 15, 1
 
 
 def _text(rng, n, width):
@@ -49,14 +50,14 @@ def test_known_answers(ctx):
     import rapidfuzz_amd as rfa
 
     for c in G.load()["cases"]:
-        if c["metric"] not in (TSR, PTSR, QR):
+        if c["metric"] not in (TSR, PTSR, QR, TSET, TRATIO):
             continue
         sc = rfa.Scorer(ctx, c["metric"], ctx.strings([G.expand(c["s1"])]))
         got = float(sc.cdist(ctx.strings([G.expand(c["s2"])]), SIM, cutoff_d=c.get("cutoff", 0.0))[0, 0])
         assert abs(got - c["expected"]) <= c["tol"], (G.case_id(c), got)
 
 
-@pytest.mark.parametrize("metric", [TSR, PTSR, QR])
+@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO])
 @pytest.mark.parametrize("width,signed8", [(1, False), (1, True), (2, False), (4, False)])
 def test_short(ctx, oracle, metric, width, signed8):
     """Queries of 0-40 units (tile path) against choices of 0-90 units."""
@@ -106,6 +107,41 @@ def test_topk(ctx, oracle, metric):
         order = sorted(((-full[i, j], j) for j in range(len(c)) if full[i, j] >= 50.0))[:5]
         assert list(idx[i]) == [j for _, j in order] + [-1] * (5 - len(order))
         assert np.array_equal(score[i], np.array([-s for s, _ in order] + [0.0] * (5 - len(order))))
+
+
+def test_token_sets_many_choice_tokens(ctx, oracle):
+    """Choices of up to 400 units (more than 64 distinct tokens: the set kernel recomputes
+    membership past its 64-token mask), shared tokens, 32- and 64-bit query tiles."""
+    rng = np.random.default_rng(620)
+    words = [np.array([ord(ch) for ch in w], np.uint32) for w in
+             ["a", "b", "ab", "ba", "abc", "cab", "x", "yy", "zz", "q", "r", "st", "uv", "w", "aa", "bb"]]
+    def sentence(n):
+        parts = [words[int(i)] for i in rng.integers(0, len(words), size=n)]
+        out = []
+        for i, p in enumerate(parts):
+            if i:
+                out.append(32)
+            out.extend(int(x) for x in p)
+        return np.array(out, np.uint32)
+    q = [sentence(int(rng.integers(0, 12))) for _ in range(70)]
+    q = [x[:64] for x in q]
+    c = [sentence(int(rng.integers(0, 20))) for _ in range(50)]
+    c += [np.concatenate([rng.integers(200, 255, size=300).astype(np.uint32) * 0 + 32 +
+                          (np.arange(300, dtype=np.uint32) % 2) * (np.arange(300, dtype=np.uint32) + 1) % 200,
+                          sentence(5)]) for _ in range(4)]
+    for metric in (TSET, TRATIO):
+        _check(ctx, oracle, metric, q, c, 1, cuts=(0.0, 70.0, 100.5))
+
+
+def test_token_set_limits(ctx):
+    import rapidfuzz_amd as rfa
+
+    long_q = rfa.Scorer(ctx, TSET, ctx.strings([b"alpha beta gamma delta epsilon zeta eta theta iota kappa lambda mu"]))
+    with pytest.raises(rfa.RfaError):
+        long_q.cdist(ctx.strings([b"beta"]), SIM)  # deduplicated join over 64 units
+    sc = rfa.Scorer(ctx, TRATIO, ctx.strings([b"a b"]))
+    with pytest.raises(rfa.RfaError):
+        sc.topk(ctx.strings([b"b a"]), 1)
 
 
 def test_rejected(ctx):

@@ -1,4 +1,5 @@
-"""token_sort_ratio, partial_token_sort_ratio and QRatio: pins of the oracle.
+"""token_sort_ratio, partial_token_sort_ratio, QRatio, token_set_ratio and token_ratio:
+pins of the oracle.
 
 * The reference's assertions (tests-fuzz.cpp) are in tests/golden/known_answers.json
   (metrics 11-13) and run in test_oracle_golden.
@@ -11,7 +12,7 @@
 import numpy as np
 import pytest
 
-TSR, PTSR, QR, SIM = 11, 12, 13, 1
+TSR, PTSR, QR, TSET, TRATIO, SIM = 11, 12, 13, 14, 15, 1
 NARROW_WS = [0x09, 0x0A, 0x0B, 0x0C, 0x0D, 0x1C, 0x1D, 0x1E, 0x1F, 0x20]
 WIDE_WS = NARROW_WS + [0x85, 0xA0, 0x1680] + list(range(0x2000, 0x200B)) + [0x2028, 0x2029, 0x202F, 0x205F, 0x3000]
 
@@ -72,7 +73,7 @@ def _sets(rng, width, nq=14, nc=18):
     return q, c
 
 
-@pytest.mark.parametrize("metric", [TSR, PTSR, QR])
+@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO])
 @pytest.mark.parametrize("width,signed8", [(1, False), (1, True), (2, False), (4, False)])
 def test_vs_reference(oracle, reference, metric, width, signed8):
     rng = np.random.default_rng(70 + metric * 7 + width + 3 * signed8)
@@ -92,7 +93,7 @@ def test_signed_order_matters(oracle):
         [0x61, 0x62, 0x63, 0x20, 0xE9, 0x74, 0xE9]
 
 
-@pytest.mark.parametrize("metric", [TSR, PTSR, QR])
+@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO])
 @pytest.mark.parametrize("s1,s2", [(b"South Korea", b"North Korea"), (b"bc", b"bca")])
 def test_cutoff_boundary_issues(oracle, metric, s1, s2):
     # tests-fuzz.cpp:161-184
