@@ -716,7 +716,9 @@ def workload_tokens(args, emit=True):
     res = {}
     for name, metric in (("token_sort_ratio", rfa.TOKEN_SORT_RATIO),
                          ("partial_token_sort_ratio", rfa.PARTIAL_TOKEN_SORT_RATIO), ("qratio", rfa.QRATIO),
-                         ("token_set_ratio", rfa.TOKEN_SET_RATIO), ("token_ratio", rfa.TOKEN_RATIO)):
+                         ("token_set_ratio", rfa.TOKEN_SET_RATIO), ("token_ratio", rfa.TOKEN_RATIO),
+                         ("partial_token_set_ratio", rfa.PARTIAL_TOKEN_SET_RATIO),
+                         ("partial_token_ratio", rfa.PARTIAL_TOKEN_RATIO)):
         sc = rfa.Scorer(ctx, metric, qs)
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -774,7 +776,7 @@ def workload_tokens(args, emit=True):
             "ms_per_step": res["token_sort_ratio"]["ms"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u32", "data": "synthetic",
             "config": {"workload": f"fuzz::CachedTokenSortRatio / CachedPartialTokenSortRatio / CachedQRatio / "
-                                   f"CachedTokenSetRatio / CachedTokenRatio cdist "
+                                   f"CachedTokenSetRatio / CachedTokenRatio / CachedPartialTokenSetRatio / CachedPartialTokenRatio cdist "
                                    f"{nq} x {nc}, uint8 a-z and space, queries 8-32 units, choices 32-128 units, "
                                    f"no cutoff"},
             **res}
@@ -1121,7 +1123,9 @@ def main():
         res["configs"]["token_scorers_1024x65536"] = {k: tok[k] for k in ("metric", "value", "unit", "ms_per_step",
                                                                            "token_sort_ratio",
                                                                            "partial_token_sort_ratio", "qratio",
-                                                                           "token_set_ratio", "token_ratio")}
+                                                                           "token_set_ratio", "token_ratio",
+                                                                           "partial_token_set_ratio",
+                                                                           "partial_token_ratio")}
         une = workload_uneven(args, emit=False)
         res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
                                              "speedup_vs_plain": une["speedup_vs_plain"],

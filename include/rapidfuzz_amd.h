@@ -63,7 +63,9 @@ typedef enum {
     /* token sets (deduplicated sorted tokens): rfa_cdist only; queries whose deduplicated token join
      * has at most 64 code units */
     RFA_TOKEN_SET_RATIO = 14,          /* fuzz::CachedTokenSetRatio (fuzz.hpp:412-434) */
-    RFA_TOKEN_RATIO = 15               /* fuzz::CachedTokenRatio (fuzz.hpp:530-557): max of the token-sort and token-set ratios */
+    RFA_TOKEN_RATIO = 15,              /* fuzz::CachedTokenRatio (fuzz.hpp:530-557): max of the token-sort and token-set ratios */
+    RFA_PARTIAL_TOKEN_SET_RATIO = 16,  /* fuzz::CachedPartialTokenSetRatio (fuzz.hpp:471-493) */
+    RFA_PARTIAL_TOKEN_RATIO = 17       /* fuzz::CachedPartialTokenRatio (fuzz.hpp:595-620) */
 } rfa_metric;
 
 /* Scorer method (details/distance.hpp:211-301, 326-356). */

@@ -19,7 +19,8 @@ ORACLE_SO = os.path.join(HERE, "librf_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "librfref.so")
 
 (LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX, PARTIAL_RATIO, TOKEN_SORT_RATIO,
- PARTIAL_TOKEN_SORT_RATIO, QRATIO, TOKEN_SET_RATIO, TOKEN_RATIO) = range(16)
+ PARTIAL_TOKEN_SORT_RATIO, QRATIO, TOKEN_SET_RATIO, TOKEN_RATIO, PARTIAL_TOKEN_SET_RATIO,
+ PARTIAL_TOKEN_RATIO) = range(18)
 DISTANCE, SIMILARITY, NORM_DISTANCE, NORM_SIMILARITY = range(4)
 INT64_MAX = (1 << 63) - 1
 

@@ -25,7 +25,7 @@
 
 namespace {
 
-enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10, TOKEN_SORT = 11, PARTIAL_TOKEN_SORT = 12, QRATIO = 13, TOKEN_SET = 14, TOKEN_RATIO = 15 };
+enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10, TOKEN_SORT = 11, PARTIAL_TOKEN_SORT = 12, QRATIO = 13, TOKEN_SET = 14, TOKEN_RATIO = 15, PTOKEN_SET = 16, PTOKEN_RATIO = 17 };
 enum { DIST = 0, SIM = 1, NDIST = 2, NSIM = 3 };
 
 thread_local std::string g_err;
@@ -85,6 +85,8 @@ struct AnyCached {
     rapidfuzz::fuzz::CachedQRatio<CharT>* qratio = nullptr;
     rapidfuzz::fuzz::CachedTokenSetRatio<CharT>* tset = nullptr;
     rapidfuzz::fuzz::CachedTokenRatio<CharT>* tratio = nullptr;
+    rapidfuzz::fuzz::CachedPartialTokenSetRatio<CharT>* ptset = nullptr;
+    rapidfuzz::fuzz::CachedPartialTokenRatio<CharT>* ptratio = nullptr;
 
     AnyCached(int m, std::vector<CharT> q, rapidfuzz::LevenshteinWeightTable w_, double pw_)
         : metric(m), s1(std::move(q)), w(w_), pw(pw_)
@@ -106,6 +108,8 @@ struct AnyCached {
         case QRATIO: qratio = new rapidfuzz::fuzz::CachedQRatio<CharT>(s1); break;
         case TOKEN_SET: tset = new rapidfuzz::fuzz::CachedTokenSetRatio<CharT>(s1); break;
         case TOKEN_RATIO: tratio = new rapidfuzz::fuzz::CachedTokenRatio<CharT>(s1); break;
+        case PTOKEN_SET: ptset = new rapidfuzz::fuzz::CachedPartialTokenSetRatio<CharT>(s1); break;
+        case PTOKEN_RATIO: ptratio = new rapidfuzz::fuzz::CachedPartialTokenRatio<CharT>(s1); break;
         default: throw std::invalid_argument("unknown metric");
         }
     }
@@ -120,6 +124,8 @@ struct AnyCached {
         delete qratio;
         delete tset;
         delete tratio;
+        delete ptset;
+        delete ptratio;
         delete lev;
         delete indel;
         delete lcs;
@@ -168,6 +174,14 @@ struct AnyCached {
         case TOKEN_RATIO:
             if (kind != SIM) return -1;
             *od = tratio->similarity(b, e, cd);
+            return 1;
+        case PTOKEN_SET:
+            if (kind != SIM) return -1;
+            *od = ptset->similarity(b, e, cd);
+            return 1;
+        case PTOKEN_RATIO:
+            if (kind != SIM) return -1;
+            *od = ptratio->similarity(b, e, cd);
             return 1;
         }
         return -1;

@@ -836,6 +836,83 @@ static double orc_token_ratio(const uint32_t* s1, size_t l1, const uint32_t* s2,
     return r;
 }
 
+/* fuzz_detail::partial_token_set_ratio (fuzz_impl.hpp:460-478) behind CachedPartialTokenSetRatio
+ * (fuzz_impl.hpp:497-505) */
+static double orc_partial_token_set_ratio(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2,
+                                          double cutoff)
+{
+    if (cutoff > 100) return 0;
+    Tok* ta = (Tok*)malloc((l1 + 1) * sizeof(Tok));
+    Tok* tb = (Tok*)malloc((l2 + 1) * sizeof(Tok));
+    size_t na = sorted_tokens(s1, l1, ta), nb = sorted_tokens(s2, l2, tb);
+    double r = 0;
+    if (na && nb) {
+        na = dedupe(ta, na);
+        nb = dedupe(tb, nb);
+        /* set_decomposition: any common token -> 100, else the differences are the sets */
+        int common = 0;
+        for (size_t i = 0; i < na && !common; ++i)
+            for (size_t j = 0; j < nb && !common; ++j) common = tok_eq(ta[i], tb[j]);
+        if (common) {
+            r = 100;
+        }
+        else {
+            uint32_t* a = (uint32_t*)malloc((l1 + 1) * sizeof(uint32_t));
+            uint32_t* b = (uint32_t*)malloc((l2 + 1) * sizeof(uint32_t));
+            size_t la = join_into(ta, na, a), lb = join_into(tb, nb, b);
+            r = orc_partial_ratio(a, la, b, lb, cutoff);
+            free(a);
+            free(b);
+        }
+    }
+    free(ta);
+    free(tb);
+    return r;
+}
+
+/* fuzz_detail::partial_token_ratio (fuzz_impl.hpp:740-770) behind CachedPartialTokenRatio */
+static double orc_partial_token_ratio(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, double cutoff)
+{
+    if (cutoff > 100) return 0;
+    Tok* ta = (Tok*)malloc((l1 + 1) * sizeof(Tok));
+    Tok* tb = (Tok*)malloc((l2 + 1) * sizeof(Tok));
+    Tok* da = (Tok*)malloc((l1 + 1) * sizeof(Tok));
+    Tok* db = (Tok*)malloc((l2 + 1) * sizeof(Tok));
+    const size_t na = sorted_tokens(s1, l1, ta), nb = sorted_tokens(s2, l2, tb);
+    memcpy(da, ta, na * sizeof(Tok));
+    memcpy(db, tb, nb * sizeof(Tok));
+    const size_t nda = dedupe(da, na), ndb = dedupe(db, nb);
+    int common = 0;
+    for (size_t i = 0; i < nda && !common; ++i)
+        for (size_t j = 0; j < ndb && !common; ++j) common = tok_eq(da[i], db[j]);
+    double r;
+    if (common) {
+        r = 100;
+    }
+    else {
+        uint32_t* a = (uint32_t*)malloc((l1 + 1) * sizeof(uint32_t));
+        uint32_t* b = (uint32_t*)malloc((l2 + 1) * sizeof(uint32_t));
+        size_t la = join_into(ta, na, a), lb = join_into(tb, nb, b);
+        r = orc_partial_ratio(a, la, b, lb, cutoff); /* partial_ratio(s1_sorted, tokens_b.join()) */
+        /* with no common token the differences are the deduplicated sets; when neither side
+         * had duplicates they equal the sorted tokens and the reference skips the second call */
+        if (!(na == nda && nb == ndb)) {
+            const double c2 = (cutoff < r) ? r : cutoff; /* std::max(score_cutoff, result) */
+            la = join_into(da, nda, a);
+            lb = join_into(db, ndb, b);
+            const double r2 = orc_partial_ratio(a, la, b, lb, c2);
+            r = (r < r2) ? r2 : r;
+        }
+        free(a);
+        free(b);
+    }
+    free(ta);
+    free(tb);
+    free(da);
+    free(db);
+    return r;
+}
+
 /* partial_ratio_alignment(...).score (fuzz_impl.hpp:193-228); CachedPartialRatio::similarity
  * (fuzz_impl.hpp:255-279) returns the same value. */
 double orc_partial_ratio(const uint32_t* s1, size_t len1, const uint32_t* s2, size_t len2, double score_cutoff)
@@ -924,6 +1001,14 @@ int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const 
     case ORC_TOKEN_RATIO:
         if (kind != ORC_SIMILARITY) return -1;
         *out_d = orc_token_ratio(s1, l1, s2, l2, cutoff_d);
+        return 1;
+    case ORC_PARTIAL_TOKEN_SET_RATIO:
+        if (kind != ORC_SIMILARITY) return -1;
+        *out_d = orc_partial_token_set_ratio(s1, l1, s2, l2, cutoff_d);
+        return 1;
+    case ORC_PARTIAL_TOKEN_RATIO:
+        if (kind != ORC_SIMILARITY) return -1;
+        *out_d = orc_partial_token_ratio(s1, l1, s2, l2, cutoff_d);
         return 1;
     case ORC_QRATIO:
         /* CachedQRatio::similarity (fuzz_impl.hpp:912-922) */

@@ -42,7 +42,9 @@ enum {
     ORC_PARTIAL_TOKEN_SORT_RATIO = 12,
     ORC_QRATIO = 13,
     ORC_TOKEN_SET_RATIO = 14,
-    ORC_TOKEN_RATIO = 15
+    ORC_TOKEN_RATIO = 15,
+    ORC_PARTIAL_TOKEN_SET_RATIO = 16,
+    ORC_PARTIAL_TOKEN_RATIO = 17
 };
 
 enum { ORC_DISTANCE = 0, ORC_SIMILARITY = 1, ORC_NORM_DISTANCE = 2, ORC_NORM_SIMILARITY = 3 };

@@ -60,7 +60,9 @@ int algo_for(int metric, const rfa_weights& w)
     case RFA_TOKEN_SORT_RATIO:
     case RFA_QRATIO: return ALGO_LCS;
     case RFA_TOKEN_SET_RATIO:
-    case RFA_TOKEN_RATIO: return ALGO_TSET;
+    case RFA_TOKEN_RATIO:
+    case RFA_PARTIAL_TOKEN_SET_RATIO:
+    case RFA_PARTIAL_TOKEN_RATIO: return ALGO_TSET;
     case RFA_JARO:
     case RFA_JARO_WINKLER: return ALGO_JARO;
     }
@@ -365,7 +367,8 @@ rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries
         sc->metric = RFA_RATIO;
         sc->empty_zero = true;
     }
-    else if (metric == RFA_TOKEN_SET_RATIO || metric == RFA_TOKEN_RATIO) {
+    else if (metric == RFA_TOKEN_SET_RATIO || metric == RFA_TOKEN_RATIO || metric == RFA_PARTIAL_TOKEN_SET_RATIO ||
+             metric == RFA_PARTIAL_TOKEN_RATIO) {
         sc->token_set = true;
     }
     try {
@@ -376,6 +379,10 @@ rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries
             sc->q = sc->q_tokens->joined; // the plan runs over the deduplicated joins
             sc->q_tokens->joined = nullptr;
             if (metric == RFA_TOKEN_RATIO) sc->sort_sub = make_scorer(ctx, RFA_TOKEN_SORT_RATIO, queries, w, prefix_weight);
+            if (metric == RFA_PARTIAL_TOKEN_RATIO)
+                sc->sort_sub = make_scorer(ctx, RFA_PARTIAL_TOKEN_SORT_RATIO, queries, w, prefix_weight);
+            if (metric == RFA_PARTIAL_TOKEN_SET_RATIO || metric == RFA_PARTIAL_TOKEN_RATIO)
+                sc->pr_dedup = make_scorer(ctx, RFA_PARTIAL_RATIO, sc->q, w, prefix_weight);
         }
         else if (sc->token_sort) {
             sc->q_raw =
@@ -401,6 +408,7 @@ void free_scorer(rfa_scorer* sc)
     free_strings(sc->q_raw);
     free_strings(sc->tok_choices);
     free_scorer(sc->sort_sub);
+    free_scorer(sc->pr_dedup);
     delete sc;
 }
 
@@ -416,6 +424,11 @@ void scorer_set_signed(rfa_scorer* sc, bool on)
         sc->q = sc->q_tokens->joined;
         sc->q_tokens->joined = nullptr;
         plan_scorer(sc);
+        if (sc->pr_dedup) {
+            free_scorer(sc->pr_dedup);
+            sc->pr_dedup = nullptr;
+            sc->pr_dedup = make_scorer(sc->ctx, RFA_PARTIAL_RATIO, sc->q, sc->w, sc->prefix_weight);
+        }
         return;
     }
     if (!sc->token_sort) return;
@@ -548,8 +561,8 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
     if (!sc->block_queries.empty())
         throw ApiError(RFA_ERR_UNSUPPORTED,
-                       "token_set_ratio / token_ratio take queries whose deduplicated token join has at most 64 "
-                       "code units (and tiles of at most 254 distinct symbols)");
+                       "the token-set ratios take queries whose deduplicated token join has at most 64 code units "
+                       "(and tiles of at most 254 distinct symbols)");
     const uint64_t nq = sc->q->count, nc = raw->count;
     if (nq == 0 || nc == 0) return;
     if (nc > 0xffffffffull || nq > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 strings");
@@ -558,9 +571,25 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
     const rfa_strings* ch = scorer_choices(sc, raw);
     const TokenTable& ct = *sc->c_tokens;
     int mode = 0;
-    if (sc->sort_sub) {
+    const double* aux = nullptr;
+    switch (sc->api_metric) {
+    case RFA_TOKEN_RATIO:
         cdist_impl(sc->sort_sub, scorer_choices(sc->sort_sub, raw), RFA_SIMILARITY, RFA_CACHED, 0, cd, dout, 1);
         mode = 1;
+        break;
+    case RFA_PARTIAL_TOKEN_SET_RATIO:
+        cdist_impl(sc->pr_dedup, ch, RFA_SIMILARITY, RFA_CACHED, 0, cd, dout, 1);
+        mode = 2;
+        break;
+    case RFA_PARTIAL_TOKEN_RATIO: {
+        cdist_impl(sc->sort_sub, scorer_choices(sc->sort_sub, raw), RFA_SIMILARITY, RFA_CACHED, 0, cd, dout, 1);
+        double* d2 = (double*)ctx->aux_tmp.get(bytes);
+        cdist_impl(sc->pr_dedup, ch, RFA_SIMILARITY, RFA_CACHED, 0, cd, d2, 1);
+        aux = d2;
+        mode = 3;
+        break;
+    }
+    default: break;
     }
     if (!sc->classes.empty()) {
         TsetArgs a{};
@@ -585,6 +614,7 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
         a.c_set_len = ct.d_len;
         a.c_set_hash = ct.d_hash;
         a.mode = mode;
+        a.aux = aux;
         for (const TileClass& tc : sc->classes) {
             ta.tile_base = tc.first_tile;
             ta.rows_cap = tc.rows_cap;
@@ -1107,7 +1137,7 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamSynchronize(ctx->stream2);
         for (DevBuf* b : {&ctx->queue, &ctx->pm_scratch, &ctx->tflag, &ctx->items_a, &ctx->items_b, &ctx->items_idx,
-                          &ctx->items_swap, &ctx->out_tmp, &ctx->misc, &ctx->lane_arena, &ctx->lane_items,
+                          &ctx->items_swap, &ctx->out_tmp, &ctx->aux_tmp, &ctx->misc, &ctx->lane_arena, &ctx->lane_items,
                           &ctx->lane_swap, &ctx->lane_metric, &ctx->lane_out, &ctx->block_items[0],
                           &ctx->block_items[1], &ctx->block_items[2], &ctx->block_items[3], &ctx->plan_keys,
                           &ctx->plan_vals, &ctx->plan_temp, &ctx->topk_scores, &ctx->topk_run[0],
@@ -1183,7 +1213,7 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
 {
     if (!ctx || !queries || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, queries or out is NULL");
     return guarded([&] {
-        if (metric < RFA_LEVENSHTEIN || metric > RFA_TOKEN_RATIO)
+        if (metric < RFA_LEVENSHTEIN || metric > RFA_PARTIAL_TOKEN_RATIO)
             throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
         if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};

@@ -209,7 +209,7 @@ struct rfa_context {
     std::mutex mu;
     int num_cus = 256;
     size_t lds_per_cu = 160 * 1024;
-    rfa::DevBuf queue, pm_scratch, tflag, items_a, items_b, items_idx, items_swap, out_tmp, misc;
+    rfa::DevBuf queue, pm_scratch, tflag, items_a, items_b, items_idx, items_swap, out_tmp, aux_tmp, misc;
     rfa::DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4], plan_keys, plan_vals,
         plan_temp;
     rfa::DevBuf topk_scores, topk_run[2], topk_cand_s, topk_cand_i, topk_counts, topk_q, topk_out, qcut;
@@ -318,7 +318,8 @@ struct rfa_scorer {
     std::shared_ptr<rfa::TokenTable> c_tokens;
     uint64_t c_tokens_id = 0;
     bool c_tokens_signed8 = false;
-    rfa_scorer* sort_sub = nullptr;
+    rfa_scorer* sort_sub = nullptr; // token_ratio: token-sort ratio; partial_token_ratio: partial token-sort
+    rfa_scorer* pr_dedup = nullptr; // partial token ratios: partial_ratio over the deduplicated joins
     rfa_strings* tok_choices = nullptr; // token-sorted copy of the last choice set
     uint64_t tok_choices_id = 0;
     bool tok_choices_signed8 = false;

@@ -199,7 +199,8 @@ hipError_t launch_tokens(int char_width, const TokenArgs& a, hipStream_t s);
 // token_set_ratio / token_ratio (rfa_tset.hip): query tiles of deduplicated token
 // joins (TileArgs over them) against choices' joins; the token tables give each
 // string's tokens in sorted order (start in its join, length, hash).  mode 0 writes
-// token_set_ratio; mode 1 combines the token-sort ratio already in out into token_ratio.
+// token_set_ratio; mode 1 combines the token-sort ratio already in out into token_ratio;
+// modes 2 / 3 finish partial_token_set_ratio / partial_token_ratio from partial_ratio values.
 struct TsetArgs {
     TileArgs t;
     const uint64_t* q_set_off;
@@ -210,8 +211,9 @@ struct TsetArgs {
     const uint32_t* c_set_start;
     const uint32_t* c_set_len;
     const uint64_t* c_set_hash;
-    int mode;
+    int mode; // 0 token_set_ratio, 1 token_ratio, 2 partial_token_set_ratio, 3 partial_token_ratio
     uint32_t tok_lds_u64; // set by launch_tset
+    const double* aux;    // mode 3: partial_ratio of the deduplicated joins (out holds the sorted joins')
 };
 hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
                        hipStream_t s);

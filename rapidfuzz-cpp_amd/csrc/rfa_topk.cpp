@@ -492,8 +492,9 @@ static void topk_dense_subset(rfa_context* ctx, int metric, const rfa_weights& w
 
 static void validate_topk(int metric, int kind)
 {
-    if (metric == RFA_TOKEN_SET_RATIO || metric == RFA_TOKEN_RATIO)
-        throw ApiError(RFA_ERR_UNSUPPORTED, "top-k over token_set_ratio / token_ratio is not accelerated (use rfa_cdist)");
+    if (metric == RFA_TOKEN_SET_RATIO || metric == RFA_TOKEN_RATIO || metric == RFA_PARTIAL_TOKEN_SET_RATIO ||
+        metric == RFA_PARTIAL_TOKEN_RATIO)
+        throw ApiError(RFA_ERR_UNSUPPORTED, "top-k over the token-set ratios is not accelerated (use rfa_cdist)");
     validate_kind(metric, kind, RFA_CACHED);
     const bool sim_double = kind == RFA_NORMALIZED_SIMILARITY ||
                             (kind == RFA_SIMILARITY &&

@@ -11,6 +11,13 @@
 // removed positions masked out against U_c with the removed units stepped as
 // "no match".  No per-pair string is built.
 //
+// partial_token_set_ratio and partial_token_ratio (fuzz_impl.hpp:460-478, 740-770)
+// return 100 when the sets share a token; otherwise their differences are the
+// whole deduplicated sets, so they are partial_ratio of the deduplicated joins
+// (and, for partial_token_ratio, of the sorted joins: the larger, the two being
+// equal when no token repeats).  The host computes those with the partial_ratio
+// path; modes 2 and 3 here only decide "shares a token".
+//
 // Query tiles (the scorer's tile plan over the queries' U, at most 64 units,
 // one query per lane, PM[row][lane] in LDS) against choices uniform across the
 // wave.  Each lane keeps its query's token table in LDS columns and a 64-bit
@@ -120,7 +127,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tset(TsetArgs a)
         const uint64_t tc0 = a.c_set_off[cidx];
         const uint32_t Tc = uniform((uint32_t)(a.c_set_off[cidx + 1] - tc0));
         double v = 0.0;
-        bool perfect = false;
+        bool perfect = false, common = false;
         if (cut <= 100) {
             // ---- pass 1: the intersection ----
             uint32_t inter_q = 0, n_inter = 0, n_kept_c = 0;
@@ -160,8 +167,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tset(TsetArgs a)
             const uint64_t ba_len = ba_units + (n_kept_c ? n_kept_c - 1 : 0);
             const uint64_t sect_len = sect_units + (n_inter ? n_inter - 1 : 0);
             perfect = n_inter > 0 && (n_kept_q == 0 || n_kept_c == 0);
-            // ---- pass 2: LCS(join(ab), join(ba)) over U_c ----
-            const bool need = qact && !perfect;
+            common = n_inter > 0;
+            // ---- pass 2: LCS(join(ab), join(ba)) over U_c (token_set_ratio / token_ratio) ----
+            const bool need = MODE < 2 && qact && !perfect;
             uint32_t lcs = 0;
             if (ballot(need)) {
                 TsLcs<W> S;
@@ -206,8 +214,12 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tset(TsetArgs a)
                 r = 0.0;
             else if (MODE == 0)
                 r = (Tq == 0 || Tc == 0) ? 0.0 : (perfect ? 100.0 : v);
-            else
+            else if (MODE == 1)
                 r = perfect ? 100.0 : dmax(*dst, v); // token_ratio: out holds the sorted-join ratio
+            else if (MODE == 2)
+                r = (Tq == 0 || Tc == 0) ? 0.0 : (common ? 100.0 : *dst); // out: partial_ratio of the sets
+            else
+                r = common ? 100.0 : dmax(*dst, a.aux[(uint64_t)qidx * ta.out_stride + cidx]);
             *dst = r;
         }
     }
@@ -223,10 +235,12 @@ size_t tset_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t
 template <typename CharT, typename W>
 static void launch_tset_w(const TsetArgs& a, uint32_t ntiles, uint32_t nbx, size_t lds, hipStream_t s)
 {
-    if (a.mode == 0)
-        hipLaunchKernelGGL((k_tset<CharT, W, 0>), dim3(nbx, ntiles), dim3(TILE_THREADS), lds, s, a);
-    else
-        hipLaunchKernelGGL((k_tset<CharT, W, 1>), dim3(nbx, ntiles), dim3(TILE_THREADS), lds, s, a);
+    switch (a.mode) {
+    case 0: hipLaunchKernelGGL((k_tset<CharT, W, 0>), dim3(nbx, ntiles), dim3(TILE_THREADS), lds, s, a); break;
+    case 1: hipLaunchKernelGGL((k_tset<CharT, W, 1>), dim3(nbx, ntiles), dim3(TILE_THREADS), lds, s, a); break;
+    case 2: hipLaunchKernelGGL((k_tset<CharT, W, 2>), dim3(nbx, ntiles), dim3(TILE_THREADS), lds, s, a); break;
+    default: hipLaunchKernelGGL((k_tset<CharT, W, 3>), dim3(nbx, ntiles), dim3(TILE_THREADS), lds, s, a); break;
+    }
 }
 
 hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,

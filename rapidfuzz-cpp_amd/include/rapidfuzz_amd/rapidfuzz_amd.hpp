@@ -710,6 +710,36 @@ struct CachedTokenRatio : detail::CachedFuzz<RFA_TOKEN_RATIO> {
     {}
 };
 
+// fuzz::CachedPartialTokenSetRatio (fuzz.hpp:471-493), fuzz::CachedPartialTokenRatio (fuzz.hpp:595-620)
+template <typename CharT1>
+struct CachedPartialTokenSetRatio : detail::CachedFuzz<RFA_PARTIAL_TOKEN_SET_RATIO> {
+    template <typename InputIt1>
+    CachedPartialTokenSetRatio(InputIt1 first1, InputIt1 last1)
+        : detail::CachedFuzz<RFA_PARTIAL_TOKEN_SET_RATIO>(first1, last1,
+                                                           std::is_signed<CharT1>::value && sizeof(CharT1) == 1)
+    {}
+    template <typename Sentence1>
+    CachedPartialTokenSetRatio(const Sentence1& s1)
+        : CachedPartialTokenSetRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+};
+
+template <typename CharT1>
+struct CachedPartialTokenRatio : detail::CachedFuzz<RFA_PARTIAL_TOKEN_RATIO> {
+    template <typename InputIt1>
+    CachedPartialTokenRatio(InputIt1 first1, InputIt1 last1)
+        : detail::CachedFuzz<RFA_PARTIAL_TOKEN_RATIO>(first1, last1,
+                                                       std::is_signed<CharT1>::value && sizeof(CharT1) == 1)
+    {}
+    template <typename Sentence1>
+    CachedPartialTokenRatio(const Sentence1& s1) : CachedPartialTokenRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+};
+
+template <typename Sentence1>
+CachedPartialTokenSetRatio(const Sentence1& s1) -> CachedPartialTokenSetRatio<detail::sentence_char_t<Sentence1>>;
+template <typename Sentence1>
+CachedPartialTokenRatio(const Sentence1& s1) -> CachedPartialTokenRatio<detail::sentence_char_t<Sentence1>>;
 template <typename Sentence1>
 CachedTokenSetRatio(const Sentence1& s1) -> CachedTokenSetRatio<detail::sentence_char_t<Sentence1>>;
 template <typename Sentence1>

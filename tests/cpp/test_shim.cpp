@@ -117,6 +117,12 @@ int main()
             CHECK(near(rfa::fuzz::CachedTokenSetRatio<char>(std::string("")).similarity(std::string("")), 0.0));
             CHECK(near(rfa::fuzz::CachedTokenRatio<char>(std::string("")).similarity(std::string("")), 100.0));
             CHECK(near(rfa::fuzz::CachedTokenRatio<char>(s4).similarity(s5), 100.0));
+            // tests-fuzz.cpp:96-99, 125-135
+            CHECK(near(rfa::fuzz::CachedPartialTokenSetRatio<char>(s4).similarity(
+                           std::string("new york city mets - atlanta braves")),
+                       100.0));
+            CHECK(near(rfa::fuzz::CachedPartialTokenSetRatio<char>(std::string("")).similarity(std::string("")), 0.0));
+            CHECK(near(rfa::fuzz::CachedPartialTokenRatio<char>(std::string("")).similarity(std::string("")), 100.0));
             rfa::fuzz::CachedTokenSortRatio<char> acc(std::string("\xe9x ab"));
             CHECK(near(acc.similarity(std::string("\xe9xab")), 88.8888889));
             rfa::fuzz::CachedTokenSortRatio<unsigned char> accu(std::basic_string<unsigned char>(

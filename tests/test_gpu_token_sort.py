@@ -1,5 +1,6 @@
 """GPU parity of token_sort_ratio, partial_token_sort_ratio, QRatio (rfa_cdist / rfa_topk),
-token_set_ratio and token_ratio (rfa_cdist) against the oracle, bit for bit: the device
+token_set_ratio, token_ratio, partial_token_set_ratio and partial_token_ratio (rfa_cdist)
+against the oracle, bit for bit: the device
 token sort (plain and deduplicated) for 8/16/32-bit units and signed 8-bit order, the
 cached token-sorted choice sets and the masked-LCS set kernel."""
 import numpy as np
@@ -9,7 +10,7 @@ import golden_cases as G
 
 pytestmark = pytest.mark.gpu
 
-TSR, PTSR, QR, TSET, TRATIO, SIM = 11, 12, 13, 14, 15, 1
+TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO, SIM = 11, 12, 13, 14, 15, 16, 17, 1
 
 
 def _text(rng, n, width):
@@ -50,14 +51,14 @@ def test_known_answers(ctx):
     import rapidfuzz_amd as rfa
 
     for c in G.load()["cases"]:
-        if c["metric"] not in (TSR, PTSR, QR, TSET, TRATIO):
+        if c["metric"] not in (TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO):
             continue
         sc = rfa.Scorer(ctx, c["metric"], ctx.strings([G.expand(c["s1"])]))
         got = float(sc.cdist(ctx.strings([G.expand(c["s2"])]), SIM, cutoff_d=c.get("cutoff", 0.0))[0, 0])
         assert abs(got - c["expected"]) <= c["tol"], (G.case_id(c), got)
 
 
-@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO])
+@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO])
 @pytest.mark.parametrize("width,signed8", [(1, False), (1, True), (2, False), (4, False)])
 def test_short(ctx, oracle, metric, width, signed8):
     """Queries of 0-40 units (tile path) against choices of 0-90 units."""
@@ -129,7 +130,7 @@ def test_token_sets_many_choice_tokens(ctx, oracle):
     c += [np.concatenate([rng.integers(200, 255, size=300).astype(np.uint32) * 0 + 32 +
                           (np.arange(300, dtype=np.uint32) % 2) * (np.arange(300, dtype=np.uint32) + 1) % 200,
                           sentence(5)]) for _ in range(4)]
-    for metric in (TSET, TRATIO):
+    for metric in (TSET, TRATIO, PTSET, PTRATIO):
         _check(ctx, oracle, metric, q, c, 1, cuts=(0.0, 70.0, 100.5))
 
 

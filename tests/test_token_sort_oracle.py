@@ -12,7 +12,7 @@ pins of the oracle.
 import numpy as np
 import pytest
 
-TSR, PTSR, QR, TSET, TRATIO, SIM = 11, 12, 13, 14, 15, 1
+TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO, SIM = 11, 12, 13, 14, 15, 16, 17, 1
 NARROW_WS = [0x09, 0x0A, 0x0B, 0x0C, 0x0D, 0x1C, 0x1D, 0x1E, 0x1F, 0x20]
 WIDE_WS = NARROW_WS + [0x85, 0xA0, 0x1680] + list(range(0x2000, 0x200B)) + [0x2028, 0x2029, 0x202F, 0x205F, 0x3000]
 
@@ -73,7 +73,7 @@ def _sets(rng, width, nq=14, nc=18):
     return q, c
 
 
-@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO])
+@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO])
 @pytest.mark.parametrize("width,signed8", [(1, False), (1, True), (2, False), (4, False)])
 def test_vs_reference(oracle, reference, metric, width, signed8):
     rng = np.random.default_rng(70 + metric * 7 + width + 3 * signed8)
@@ -93,10 +93,30 @@ def test_signed_order_matters(oracle):
         [0x61, 0x62, 0x63, 0x20, 0xE9, 0x74, 0xE9]
 
 
-@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO])
+@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO])
 @pytest.mark.parametrize("s1,s2", [(b"South Korea", b"North Korea"), (b"bc", b"bca")])
 def test_cutoff_boundary_issues(oracle, metric, s1, s2):
     # tests-fuzz.cpp:161-184
     score = float(oracle.score(metric, SIM, s1, s2, cutoff_d=0.0))
     assert float(oracle.score(metric, SIM, s1, s2, cutoff_d=score + 0.0001)) == 0.0
     assert abs(float(oracle.score(metric, SIM, s1, s2, cutoff_d=score - 0.0001)) - score) <= 1e-6
+
+
+@pytest.mark.parametrize("metric", [TSET, TRATIO, PTSET, PTRATIO])
+def test_disjoint_token_sets_with_repeats(oracle, reference, metric):
+    """No token in common but shared characters, and repeated tokens on each side: the
+    paths where partial_token_ratio's second partial_ratio (deduplicated differences)
+    and token_set_ratio's difference LCS decide the score."""
+    rng = np.random.default_rng(80 + metric)
+    qv, cv = ["ab", "ba", "aab", "bba", "abab"], ["a", "b", "abb", "bab", "baa", "aaab"]
+
+    def sent(vocab, n):
+        return np.array([ord(ch) for ch in " ".join(vocab[int(i)] for i in rng.integers(0, len(vocab), size=n))],
+                        np.uint32)
+
+    q = [sent(qv, int(rng.integers(1, 7))) for _ in range(14)]
+    c = [sent(cv, int(rng.integers(1, 9))) for _ in range(16)]
+    for cut in (0.0, 40.0, 75.0):
+        exp = reference.cdist(metric, SIM, q, c, cutoff_d=cut, char_width=1)
+        got = oracle.cdist(metric, SIM, q, c, cutoff_d=cut, char_width=1)
+        assert np.array_equal(got.view(np.uint64), exp.view(np.uint64)), (metric, cut)
