@@ -47,7 +47,7 @@ int algo_for(int metric, const rfa_weights& w)
             if (w.insert_cost == 0 || w.insert_cost == w.replace_cost) return ALGO_LEV;
             if (w.replace_cost >= w.insert_cost + w.delete_cost) return ALGO_LCS;
         }
-        return -1;
+        return ALGO_WLEV;
     case RFA_INDEL:
     case RFA_LCSSEQ:
     case RFA_RATIO: return ALGO_LCS;
@@ -715,6 +715,82 @@ void cdist_partial(rfa_scorer* sc, const rfa_strings* ch, double cd, double* dou
     }
 }
 
+// Levenshtein with general weights: tile-planned queries (<= 64 units) through the
+// tile kernel, the rest one wave per pair.  The long kernel holds up to
+// WLEV_LONG_MAX pattern rows across a wave: the query, or the choice when the
+// query is longer.
+void cdist_wlev(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, void* dout, const uint64_t* d_qcut)
+{
+    rfa_context* ctx = sc->ctx;
+    hipStream_t s = ctx->stream;
+    const uint64_t nc = ch->count;
+    const int width = sc->q->width;
+    // every value of the recurrence stays below (|s1| + |s2| + 1) * max weight, with the
+    // replace weight clamped to insert + delete (a larger one is never used)
+    constexpr uint64_t W24 = 1ull << 24;
+    if (spec.ins >= W24 || spec.del >= W24 || std::min<uint64_t>(spec.rep, spec.ins + spec.del) >= W24)
+        throw ApiError(RFA_ERR_UNSUPPORTED, "Levenshtein weights of 2^24 or more are not accelerated");
+    const uint64_t rep = std::min<uint64_t>(spec.rep, spec.ins + spec.del);
+    const uint64_t wmax = std::max<uint64_t>(std::max<uint64_t>(spec.ins, spec.del), rep);
+    const uint64_t lens = sc->q->max_len + ch->max_len + 1;
+    if (wmax && (lens >= (1ull << 30) || wmax * lens >= (1ull << 30)))
+        throw ApiError(RFA_ERR_UNSUPPORTED,
+                       "Levenshtein with general weights: weights times string lengths must stay below 2^30");
+    if (!sc->classes.empty()) {
+        TileArgs ta{};
+        ta.qchars = sc->q->d_chars;
+        ta.qoff = sc->q->d_off;
+        ta.qorder = sc->d_tile_order;
+        ta.nq = (uint32_t)sc->tile_order.size();
+        ta.cchars = ch->d_chars;
+        ta.coff = ch->d_off;
+        ta.corder = nullptr;
+        ta.ncsel = (uint32_t)nc;
+        ta.out_stride = nc;
+        ta.hash_bits = sc->hash_bits;
+        ta.spec = spec;
+        ta.out = dout;
+        ta.qcut = d_qcut;
+        for (const TileClass& tc : sc->classes) {
+            ta.tile_base = tc.first_tile;
+            ta.rows_cap = tc.rows_cap;
+            uint32_t target_blocks = (uint32_t)ctx->num_cus * 12;
+            uint32_t nbx = std::max<uint32_t>(1, (target_blocks + tc.ntiles - 1) / tc.ntiles);
+            uint32_t cpb = std::max<uint32_t>(32, (ta.ncsel + nbx - 1) / nbx);
+            cpb = ((cpb + 31) / 32) * 32; // whole 4-choice runs for each of the 8 waves
+            ta.choices_per_block = cpb;
+            nbx = (ta.ncsel + cpb - 1) / cpb;
+            HIP_CHECK(launch_wtile(width, tc.word_bits, ta, tc.ntiles, nbx, s));
+        }
+    }
+    if (!sc->block_queries.empty()) {
+        uint64_t pattern = sc->block_max_len;
+        if (pattern > WLEV_LONG_MAX) {
+            if (ch->max_len > WLEV_LONG_MAX)
+                throw ApiError(RFA_ERR_UNSUPPORTED,
+                               "Levenshtein with general weights of two strings longer than 1024 code units");
+            pattern = WLEV_LONG_MAX;
+        }
+        WlevLongArgs wa{};
+        wa.qchars = sc->q->d_chars;
+        wa.qoff = sc->q->d_off;
+        wa.qlist = sc->d_block_queries;
+        wa.nql = (uint32_t)sc->block_queries.size();
+        wa.cchars = ch->d_chars;
+        wa.coff = ch->d_off;
+        wa.nc = (uint32_t)nc;
+        wa.out_stride = nc;
+        wa.spec = spec;
+        wa.out = dout;
+        wa.qcut = d_qcut;
+        const uint32_t cells = (uint32_t)std::max<uint64_t>(1, (pattern + 63) / 64);
+        const uint64_t items = (uint64_t)wa.nql * nc;
+        const uint32_t nblocks = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>((items + 3) / 4, (uint64_t)ctx->num_cus * 8));
+        if (items) HIP_CHECK(launch_wlong(width, cells, wa, nblocks, s));
+    }
+}
+
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
                 int out_on_device, const uint64_t* qcut_host)
 {
@@ -739,9 +815,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         if (!multi_levenshtein_weights(sc->w)) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unsupported weights");
         algo = ALGO_LEV;
     }
-    if (algo < 0)
-        throw ApiError(RFA_ERR_UNSUPPORTED,
-                       "Levenshtein weights other than {w,w,w} or Indel-like {w,w,>=2w} are not accelerated");
+    if (algo < 0) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
     hipStream_t s = ctx->stream;
     const uint64_t nq = sc->q->count, nc = ch->count;
     const size_t bytes = (size_t)nq * nc * 8;
@@ -752,6 +826,14 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
     const int width = sc->q->width;
     if (algo == ALGO_PARTIAL) {
         cdist_partial(sc, ch, cd, (double*)dout);
+        if (!out_on_device) {
+            HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
+        return;
+    }
+    if (algo == ALGO_WLEV) {
+        cdist_wlev(sc, ch, spec, dout, d_qcut);
         if (!out_on_device) {
             HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
@@ -1217,10 +1299,6 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
             throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
         if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};
-        // weights MultiLevenshtein accepts are kept even when Cached* scoring cannot use them
-        if (algo_for(metric, w) < 0 && !(metric == RFA_LEVENSHTEIN && multi_levenshtein_weights(w)))
-            throw ApiError(RFA_ERR_UNSUPPORTED,
-                           "Levenshtein weights other than {w,w,w} or Indel-like {w,w,>=2w} are not accelerated");
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_CHECK(hipSetDevice(ctx->device));
         *out = make_scorer(ctx, metric, queries, w, prefix_weight);

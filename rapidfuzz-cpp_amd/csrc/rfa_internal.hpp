@@ -218,6 +218,30 @@ struct TsetArgs {
 hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
                        hipStream_t s);
 
+// Levenshtein with general weights (rfa_wlev.hip).  Query tiles of at most 64 units run
+// the tile kernel (TileArgs); longer queries run one wave per (query, choice) pair with
+// the pattern's rows across the lanes, up to WLEV_LONG_MAX rows (the choice becomes the
+// pattern when the query is longer).  Every value of the recurrence must stay below 2^30.
+hipError_t launch_wtile(int char_width, int word_bits, const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x,
+                        hipStream_t s);
+
+struct WlevLongArgs {
+    const void* qchars;
+    const uint64_t* qoff;
+    const uint32_t* qlist; // the long queries
+    uint32_t nql;
+    const void* cchars;
+    const uint64_t* coff;
+    uint32_t nc;
+    uint64_t out_stride;
+    ScoreSpec spec;
+    void* out;            // out[q * out_stride + c]
+    const uint64_t* qcut; // integer cutoff per query (nullptr: spec.cutoff_u)
+};
+constexpr uint32_t WLEV_LONG_MAX = 1024;
+hipError_t launch_wlong(int char_width, uint32_t cells_per_lane, const WlevLongArgs& a, uint32_t nblocks,
+                        hipStream_t s);
+
 // Widest shorter string (in 64-unit words) the long path takes.
 constexpr uint32_t PARTIAL_MAX_WORDS = 16;
 hipError_t launch_plong(int char_width, uint32_t needle_words, const PartialLongArgs& a, uint32_t nblocks,

@@ -74,6 +74,8 @@ __device__ __forceinline__ uint64_t int_distance_core(const IntPair& p, uint64_t
 {
     const ScoreSpec& s = *p.s;
     if (s.metric == RFA_LEVENSHTEIN) {
+        // general weights: the exact weighted distance, cut (Levenshtein_impl.hpp:141-152)
+        if (s.algo == ALGO_WLEV) return p.raw <= c ? p.raw : c + 1;
         // CachedLevenshtein::_distance: uniform or Indel, scaled by the common weight
         if (s.ins == s.del) {
             if (s.ins == 0) return 0;
@@ -90,7 +92,7 @@ __device__ __forceinline__ uint64_t int_distance_core(const IntPair& p, uint64_t
             d *= s.ins;
             return d <= c ? d : c + 1;
         }
-        return c + 1; // general weights are rejected before launch
+        return c + 1; // unreachable: other weights run ALGO_WLEV
     }
     uint64_t exact = (s.metric == RFA_INDEL) ? indel_from_lcs(p) : p.raw;
     return exact <= c ? exact : c + 1;

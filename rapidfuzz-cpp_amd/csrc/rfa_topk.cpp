@@ -875,9 +875,9 @@ rfa_status rfa_server_create(rfa_context* ctx, const rfa_strings* choices, const
         rfa_weights w = cfg->metric == RFA_LEVENSHTEIN ? cfg->weights : rfa_weights{1, 1, 1};
         if (cfg->metric == RFA_LEVENSHTEIN && w.insert_cost == 0 && w.delete_cost == 0 && w.replace_cost == 0)
             w = rfa_weights{1, 1, 1}; // zero-initialized config: the reference default
-        if (algo_for(cfg->metric, w) < 0)
-            throw ApiError(RFA_ERR_UNSUPPORTED,
-                           "Levenshtein weights other than {w,w,w} or Indel-like {w,w,>=2w} are not accelerated");
+        if (algo_for(cfg->metric, w) < 0 || algo_for(cfg->metric, w) == ALGO_WLEV)
+            throw ApiError(RFA_ERR_UNSUPPORTED, "the server takes Levenshtein weights {w,w,w} or Indel-like "
+                                                "{w,w,>=2w} (score other weights with rfa_topk or rfa_cdist)");
         auto* srv = new rfa_server();
         srv->ctx = ctx;
         srv->cfg = *cfg;

@@ -64,6 +64,13 @@ int main()
             rfa::CachedLevenshtein<char> aabc("aabc");
             CHECK(aabc.distance("cccd") == 4);
             CHECK(aabc.distance("cccd", 2) == 3);
+            // general weights {insert 1, delete 2, replace 3}: two replacements and an insertion,
+            // out of a maximum of 6 * 2 + 7 * 1 = 19
+            rfa::CachedLevenshtein<char> w123(std::string("kitten"), rfa::LevenshteinWeightTable{1, 2, 3});
+            CHECK(w123.distance(std::string("sitting")) == 7);
+            CHECK(w123.distance(std::string("sitting"), 5) == 6);
+            CHECK(w123.distance(std::string("kitchen")) == 4);
+            CHECK(std::fabs(w123.normalized_similarity(std::string("sitting")) - (1.0 - 7.0 / 19.0)) < 1e-12);
         }
         {
             rfa::CachedIndel<char> indel(std::string("aaaa"));

@@ -105,12 +105,9 @@ def test_levenshtein_weights(ctx, oracle, weights):
     check(ctx, oracle, 0, queries, choices, weights=weights)
 
 
-def test_unsupported_weights(ctx):
-    import rapidfuzz_amd as rfa
-
-    qs = ctx.strings([b"abc"])
-    with pytest.raises(rfa.RfaError):
-        rfa.Scorer(ctx, 0, qs, weights=(1, 2, 3))
+def test_general_weights_accepted(ctx, oracle):
+    """Weights off the uniform and Indel-like forms take the weighted Wagner-Fischer kernels."""
+    check(ctx, oracle, 0, [b"abc", b"", b"kitten"], [b"abd", b"sitting", b""], width=1, weights=(1, 2, 3))
 
 
 def test_ratio_rejects_distance(ctx):
@@ -156,8 +153,8 @@ def test_multi_levenshtein_rejects_weights(ctx):
     with pytest.raises(rfa.InvalidArgument):
         sc.cdist(ctx.strings([b"abd"]), rfa.DISTANCE, semantics=rfa.MULTI)
     sc0 = rfa.Scorer(ctx, 0, ctx.strings([b"abc"]), weights=(1, 1, 0))  # Multi-only weights
-    with pytest.raises(rfa.RfaError):
-        sc0.cdist(ctx.strings([b"abd"]), rfa.DISTANCE)  # Cached scoring with them is not accelerated
+    # Cached scoring with them runs the general-weight recurrence: a free replacement
+    assert sc0.cdist(ctx.strings([b"abd"]), rfa.DISTANCE)[0, 0] == 0
 
 
 def _dna_case(seed, nq=6, nc=40, lo=300, hi=1500):
