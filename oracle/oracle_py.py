@@ -190,6 +190,11 @@ class Reference:
             f.argtypes = [ctypes.c_int, ctypes.c_int, _u8p, _u64p, ctypes.c_size_t, _u8p, _u64p,
                           ctypes.c_size_t, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, _u64p, _f64p,
                           ctypes.c_int]
+        L.ref_cdist_cached_u8_mt_w.restype = ctypes.c_int
+        L.ref_cdist_cached_u8_mt_w.argtypes = [ctypes.c_int, ctypes.c_int, _u8p, _u64p, ctypes.c_size_t, _u8p,
+                                               _u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64,
+                                               ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
+                                               _u64p, _f64p, ctypes.c_int]
         L.ref_pairs_u32_mt.restype = ctypes.c_int
         L.ref_pairs_u32_mt.argtypes = [_i32p, _u32p, _u64p, _u32p, _u64p, ctypes.c_size_t, _u64p,
                                        ctypes.c_int]
@@ -229,8 +234,9 @@ class Reference:
         return (ou if r == 0 else od).reshape(nq, nc)
 
     def cdist_u8_mt(self, metric, kind, qchars, qoff, cchars, coff, cutoff_u=None, cutoff_d=None,
-                    prefix_weight=0.1, threads=None, multi=False):
-        """Threaded u8 cdist (the CPU baseline); inputs already packed as uint8."""
+                    prefix_weight=0.1, threads=None, multi=False, weights=None):
+        """Threaded u8 cdist (the CPU baseline); inputs already packed as uint8.  weights:
+        Levenshtein (insert, delete, replace) of the Cached* scorers."""
         if cutoff_u is None:
             cutoff_u = default_int_cutoff(kind)
         nq, nc = len(qoff) - 1, len(coff) - 1
@@ -239,9 +245,14 @@ class Reference:
         threads = threads or os.cpu_count() or 1
         ou = np.zeros(nq * nc, dtype=np.uint64)
         od = np.zeros(nq * nc, dtype=np.float64)
-        f = self.lib.ref_cdist_multi_u8_mt if multi else self.lib.ref_cdist_cached_u8_mt
-        r = f(metric, kind, _p(qchars, _u8p), _p(qoff, _u64p), nq, _p(cchars, _u8p), _p(coff, _u64p), nc,
-              prefix_weight, cutoff_u, cutoff_d, _p(ou, _u64p), _p(od, _f64p), threads)
+        if weights is not None and not multi:
+            r = self.lib.ref_cdist_cached_u8_mt_w(metric, kind, _p(qchars, _u8p), _p(qoff, _u64p), nq,
+                                                  _p(cchars, _u8p), _p(coff, _u64p), nc, *weights, prefix_weight,
+                                                  cutoff_u, cutoff_d, _p(ou, _u64p), _p(od, _f64p), threads)
+        else:
+            f = self.lib.ref_cdist_multi_u8_mt if multi else self.lib.ref_cdist_cached_u8_mt
+            r = f(metric, kind, _p(qchars, _u8p), _p(qoff, _u64p), nq, _p(cchars, _u8p), _p(coff, _u64p), nc,
+                  prefix_weight, cutoff_u, cutoff_d, _p(ou, _u64p), _p(od, _f64p), threads)
         if r < 0:
             raise ValueError(f"reference error {r}: {self._err()}")
         return (ou if is_int_result(metric, kind) else od).reshape(nq, nc)

@@ -350,8 +350,8 @@ struct U8Set {
     size_t n;
 };
 
-int cdist_cached_u8_threaded(int metric, int kind, U8Set q, U8Set c, double pw, uint64_t cu, double cd,
-                             uint64_t* ou, double* od, int nthreads)
+int cdist_cached_u8_threaded(int metric, int kind, U8Set q, U8Set c, rapidfuzz::LevenshteinWeightTable w, double pw,
+                             uint64_t cu, double cd, uint64_t* ou, double* od, int nthreads)
 {
     std::atomic<size_t> next(0);
     std::atomic<int> status(0);
@@ -360,7 +360,7 @@ int cdist_cached_u8_threaded(int metric, int kind, U8Set q, U8Set c, double pw, 
             size_t i = next.fetch_add(1);
             if (i >= q.n) break;
             std::vector<uint8_t> s1(q.chars + q.off[i], q.chars + q.off[i + 1]);
-            AnyCached<uint8_t> sc(metric, s1, {1, 1, 1}, pw);
+            AnyCached<uint8_t> sc(metric, s1, w, pw);
             for (size_t j = 0; j < c.n; ++j) {
                 const uint8_t* b = c.chars + c.off[j];
                 const uint8_t* e = c.chars + c.off[j + 1];
@@ -493,8 +493,24 @@ int ref_cdist_cached_u8_mt(int metric, int kind, const uint8_t* q, const uint64_
                            double cd, uint64_t* ou, double* od, int nthreads)
 {
     try {
-        return cdist_cached_u8_threaded(metric, kind, {q, qoff, nq}, {c, coff, nc}, pw, cu, cd, ou, od,
-                                        nthreads);
+        return cdist_cached_u8_threaded(metric, kind, {q, qoff, nq}, {c, coff, nc}, {1, 1, 1}, pw, cu, cd, ou,
+                                        od, nthreads);
+    }
+    catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    }
+}
+
+// The same with Levenshtein weights.
+int ref_cdist_cached_u8_mt_w(int metric, int kind, const uint8_t* q, const uint64_t* qoff, size_t nq,
+                             const uint8_t* c, const uint64_t* coff, size_t nc, uint64_t ins, uint64_t del,
+                             uint64_t rep, double pw, uint64_t cu, double cd, uint64_t* ou, double* od,
+                             int nthreads)
+{
+    try {
+        return cdist_cached_u8_threaded(metric, kind, {q, qoff, nq}, {c, coff, nc}, {ins, del, rep}, pw, cu, cd,
+                                        ou, od, nthreads);
     }
     catch (const std::exception& e) {
         g_err = e.what();

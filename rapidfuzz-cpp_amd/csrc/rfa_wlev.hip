@@ -43,24 +43,52 @@ __device__ __forceinline__ uint32_t wl_wave_max(uint32_t v)
     return uniform(v);
 }
 
+// a * b + c on the 24-bit multiplier (a is a 0/1 match bit, b < 2^24).  Written as
+// the instruction itself: the compiler would otherwise turn the multiply by a bit
+// into a 64-bit compare and a select.
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+
+// bit i of the 64 (or 32) mismatch bits held as two 32-bit halves
+template <uint32_t I>
+__device__ __forceinline__ uint32_t miss_bit(uint32_t ylo, uint32_t yhi)
+{
+    return __builtin_amdgcn_ubfe(I < 32 ? ylo : yhi, I & 31, 1);
+}
+
 // One column step of the tile kernel: D(., j-1) -> D(., j).  Y has bit i - 1
 // set where s1[i-1] differs from the choice unit; rows above mt are left alone
-// (no query of the wave reaches them).
+// (no query of the wave reaches them).  Per cell: a bit extract, a multiply-add,
+// two adds and a three-way minimum.  The diagonal term of row i + 1 is taken from
+// D(i, j-1) before D(i, j) overwrites it, so every row updates its own register
+// (no rotation of the column, no copies where the step ends early).
+template <typename W, uint32_t BITS, uint32_t I = 1>
+__device__ __forceinline__ void wf_rows(uint32_t (&D)[BITS + 1], uint32_t ylo, uint32_t yhi, uint32_t ins,
+                                        uint32_t del, uint32_t rep, uint32_t mt, uint32_t sub)
+{
+    if constexpr (I <= BITS) {
+        if ((I & 3) == 1 && I > mt) return;
+        const uint32_t up = D[I];
+        uint32_t nsub = 0;
+        if constexpr (I < BITS) nsub = mad_u24(miss_bit<I>(ylo, yhi), rep, up);
+        D[I] = min(min(up + ins, sub), D[I - 1] + del);
+        wf_rows<W, BITS, I + 1>(D, ylo, yhi, ins, del, rep, mt, nsub);
+    }
+}
+
 template <typename W, uint32_t BITS>
 __device__ __forceinline__ void wf_step(uint32_t (&D)[BITS + 1], W Y, uint32_t ins, uint32_t del, uint32_t rep,
                                         uint32_t mt)
 {
-    uint32_t diag = D[0];
-    D[0] = diag + ins;
-#pragma unroll
-    for (uint32_t i = 1; i <= BITS; ++i) {
-        if ((i & 7) == 1 && i > mt) break;
-        const uint32_t up = D[i];
-        const uint32_t miss = (uint32_t)(Y >> (i - 1)) & 1u;
-        const uint32_t sub = __umul24(miss, rep) + diag;
-        D[i] = min(min(up + ins, sub), D[i - 1] + del);
-        diag = up;
-    }
+    const uint32_t ylo = (uint32_t)Y;
+    const uint32_t yhi = BITS > 32 ? (uint32_t)((uint64_t)Y >> 32) : 0u;
+    const uint32_t sub = mad_u24(miss_bit<0>(ylo, yhi), rep, D[0]);
+    D[0] += ins;
+    wf_rows<W, BITS>(D, ylo, yhi, ins, del, rep, mt, sub);
 }
 
 } // namespace
