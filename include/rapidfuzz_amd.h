@@ -35,7 +35,7 @@ extern "C" {
 typedef enum {
     RFA_OK = 0,
     RFA_ERR_INVALID_ARGUMENT = 1, /* the reference throws std::invalid_argument */
-    RFA_ERR_UNSUPPORTED = 2,      /* outside the accelerated path (e.g. general weights) */
+    RFA_ERR_UNSUPPORTED = 2,      /* outside the accelerated path (e.g. size limits of a kernel) */
     RFA_ERR_HIP = 3,              /* HIP runtime error */
     RFA_ERR_OUT_OF_MEMORY = 4,
     RFA_ERR_NO_DEVICE = 5
@@ -143,8 +143,13 @@ uint64_t rfa_strings_count(const rfa_strings* set);
  * experimental::Multi<Metric><MaxLen>(count) after count inserts
  * (Levenshtein.hpp:334-380).  weights may be NULL ({1,1,1}); only Levenshtein
  * reads it.  prefix_weight is read by Jaro-Winkler only.
- * Levenshtein weights must be uniform ({w,w,w}) or Indel-like (ins == del,
- * replace >= ins + del); other weights return RFA_ERR_UNSUPPORTED. */
+ * Levenshtein takes any weights (CachedLevenshtein::_distance,
+ * Levenshtein.hpp:445-477): weights other than uniform ({w,w,w}) or Indel-like
+ * (ins == del, replace >= ins + del) run the generalized Wagner-Fischer
+ * recurrence (Levenshtein_impl.hpp:85-155), where rfa_cdist returns
+ * RFA_ERR_UNSUPPORTED for a weight of 2^24 or more, for (query length +
+ * choice length + 1) x largest weight of 2^30 or more, and for pairs in
+ * which both strings exceed 1,024 code units. */
 rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* queries, const rfa_weights* weights,
                              double prefix_weight, rfa_scorer** out);
 rfa_status rfa_scorer_destroy(rfa_scorer* scorer);
