@@ -65,7 +65,9 @@ typedef enum {
     RFA_TOKEN_SET_RATIO = 14,          /* fuzz::CachedTokenSetRatio (fuzz.hpp:412-434) */
     RFA_TOKEN_RATIO = 15,              /* fuzz::CachedTokenRatio (fuzz.hpp:530-557): max of the token-sort and token-set ratios */
     RFA_PARTIAL_TOKEN_SET_RATIO = 16,  /* fuzz::CachedPartialTokenSetRatio (fuzz.hpp:471-493) */
-    RFA_PARTIAL_TOKEN_RATIO = 17       /* fuzz::CachedPartialTokenRatio (fuzz.hpp:595-620) */
+    RFA_PARTIAL_TOKEN_RATIO = 17,      /* fuzz::CachedPartialTokenRatio (fuzz.hpp:595-620) */
+    RFA_DAMERAU_LEVENSHTEIN = 18       /* experimental::CachedDamerauLevenshtein (DamerauLevenshtein.hpp:112-140):
+                                          Cached semantics, every kind; queries of at most 64 code units */
 } rfa_metric;
 
 /* Scorer method (details/distance.hpp:211-301, 326-356). */

@@ -20,7 +20,7 @@ REF_SO = os.path.join(HERE, "_ref", "librfref.so")
 
 (LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX, PARTIAL_RATIO, TOKEN_SORT_RATIO,
  PARTIAL_TOKEN_SORT_RATIO, QRATIO, TOKEN_SET_RATIO, TOKEN_RATIO, PARTIAL_TOKEN_SET_RATIO,
- PARTIAL_TOKEN_RATIO) = range(18)
+ PARTIAL_TOKEN_RATIO, DAMERAU_LEVENSHTEIN) = range(19)
 DISTANCE, SIMILARITY, NORM_DISTANCE, NORM_SIMILARITY = range(4)
 INT64_MAX = (1 << 63) - 1
 
@@ -58,7 +58,8 @@ def _p(a, t):
 
 
 def is_int_result(metric: int, kind: int) -> bool:
-    return metric in (LEVENSHTEIN, INDEL, LCSSEQ, OSA, HAMMING, PREFIX, POSTFIX) and kind in (DISTANCE, SIMILARITY)
+    return metric in (LEVENSHTEIN, INDEL, LCSSEQ, OSA, HAMMING, PREFIX, POSTFIX, DAMERAU_LEVENSHTEIN) and \
+        kind in (DISTANCE, SIMILARITY)
 
 
 class Oracle:

@@ -25,7 +25,7 @@
 
 namespace {
 
-enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10, TOKEN_SORT = 11, PARTIAL_TOKEN_SORT = 12, QRATIO = 13, TOKEN_SET = 14, TOKEN_RATIO = 15, PTOKEN_SET = 16, PTOKEN_RATIO = 17 };
+enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10, TOKEN_SORT = 11, PARTIAL_TOKEN_SORT = 12, QRATIO = 13, TOKEN_SET = 14, TOKEN_RATIO = 15, PTOKEN_SET = 16, PTOKEN_RATIO = 17, DAMERAU = 18 };
 enum { DIST = 0, SIM = 1, NDIST = 2, NSIM = 3 };
 
 thread_local std::string g_err;
@@ -87,6 +87,7 @@ struct AnyCached {
     rapidfuzz::fuzz::CachedTokenRatio<CharT>* tratio = nullptr;
     rapidfuzz::fuzz::CachedPartialTokenSetRatio<CharT>* ptset = nullptr;
     rapidfuzz::fuzz::CachedPartialTokenRatio<CharT>* ptratio = nullptr;
+    rapidfuzz::experimental::CachedDamerauLevenshtein<CharT>* dl = nullptr;
 
     AnyCached(int m, std::vector<CharT> q, rapidfuzz::LevenshteinWeightTable w_, double pw_)
         : metric(m), s1(std::move(q)), w(w_), pw(pw_)
@@ -110,6 +111,7 @@ struct AnyCached {
         case TOKEN_RATIO: tratio = new rapidfuzz::fuzz::CachedTokenRatio<CharT>(s1); break;
         case PTOKEN_SET: ptset = new rapidfuzz::fuzz::CachedPartialTokenSetRatio<CharT>(s1); break;
         case PTOKEN_RATIO: ptratio = new rapidfuzz::fuzz::CachedPartialTokenRatio<CharT>(s1); break;
+        case DAMERAU: dl = new rapidfuzz::experimental::CachedDamerauLevenshtein<CharT>(s1); break;
         default: throw std::invalid_argument("unknown metric");
         }
     }
@@ -126,6 +128,7 @@ struct AnyCached {
         delete tratio;
         delete ptset;
         delete ptratio;
+        delete dl;
         delete lev;
         delete indel;
         delete lcs;
@@ -145,6 +148,7 @@ struct AnyCached {
         case HAMMING: return run_int_metric(*ham, kind, b, e, cu, cd, ou, od);
         case PREFIX: return run_int_metric(*pre, kind, b, e, cu, cd, ou, od);
         case POSTFIX: return run_int_metric(*post, kind, b, e, cu, cd, ou, od);
+        case DAMERAU: return run_int_metric(*dl, kind, b, e, cu, cd, ou, od);
         case JARO: return run_fp_metric(*jaro, kind, b, e, cd, od);
         case JW: return run_fp_metric(*jw, kind, b, e, cd, od);
         case RATIO:

@@ -91,6 +91,74 @@ uint64_t orc_osa_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l
     return d;
 }
 
+/* Damerau-Levenshtein after Zhao and Sahni, as damerau_levenshtein_distance_zhao
+ * runs it (DamerauLevenshtein_impl.hpp:36-103): rows over s1, one row of s2
+ * positions at a time, R / R1 the current and previous rows (R holds the row
+ * before R1 until overwritten), FR the value saved at the last match of each s2
+ * position, T the value saved at the last match of the current row.  The
+ * reference's last_row_id map (s1 character -> last row it occurred in, read at
+ * s2[j]) is kept here per s2 position: lastk[j] = last row i with s1[i] == s2[j],
+ * written when that row finishes, which is the value the map returns.  maxVal is
+ * max(l1, l2) + 1 as there. */
+uint64_t orc_damerau_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2)
+{
+    const int64_t maxv = (int64_t)(l1 > l2 ? l1 : l2) + 1;
+    const size_t size = l2 + 2;
+    int64_t* buf = (int64_t*)malloc(3 * size * sizeof(int64_t));
+    int64_t* lastk = (int64_t*)malloc((l2 + 1) * sizeof(int64_t));
+    int64_t *FRa = buf, *R1a = buf + size, *Ra = buf + 2 * size;
+    for (size_t j = 0; j < size; ++j) FRa[j] = R1a[j] = maxv;
+    Ra[0] = maxv;
+    for (size_t j = 1; j < size; ++j) Ra[j] = (int64_t)j - 1;
+    for (size_t j = 0; j <= l2; ++j) lastk[j] = -1;
+    int64_t* R = Ra + 1; /* R[-1] is the sentinel column */
+    int64_t* R1 = R1a + 1;
+    int64_t* FR = FRa + 1;
+    for (int64_t i = 1; i <= (int64_t)l1; ++i) {
+        int64_t* t = R;
+        R = R1;
+        R1 = t;
+        int64_t last_col_id = -1;
+        int64_t last_i2l1 = R[0];
+        R[0] = i;
+        int64_t T = maxv;
+        const uint32_t a = s1[i - 1];
+        for (int64_t j = 1; j <= (int64_t)l2; ++j) {
+            const uint32_t b = s2[j - 1];
+            int64_t diag = R1[j - 1] + (a != b);
+            int64_t left = R[j - 1] + 1;
+            int64_t up = R1[j] + 1;
+            int64_t temp = diag < left ? diag : left;
+            temp = up < temp ? up : temp;
+            if (a == b) {
+                last_col_id = j;
+                FR[j] = R1[j - 2];
+                T = last_i2l1;
+            }
+            else {
+                int64_t k = lastk[j];
+                int64_t l = last_col_id;
+                if (j - l == 1) {
+                    int64_t tr = FR[j] + (i - k);
+                    temp = tr < temp ? tr : temp;
+                }
+                else if (i - k == 1) {
+                    int64_t tr = T + (j - l);
+                    temp = tr < temp ? tr : temp;
+                }
+            }
+            last_i2l1 = R[j];
+            R[j] = temp;
+        }
+        for (size_t j = 1; j <= l2; ++j)
+            if (s2[j - 1] == a) lastk[j] = i;
+    }
+    uint64_t d = (uint64_t)R[l2];
+    free(buf);
+    free(lastk);
+    return d;
+}
+
 /* rapidfuzz_reference/Jaro.hpp:22-56: greedy matching window and
  * transposition walk.  Writes the common-character count and the number of
  * mismatched flagged pairs (before the halving of the Jaro formula). */
@@ -329,6 +397,16 @@ static uint64_t lev_cached_distance(const Pair* p, uint64_t c)
     return d <= c ? d : c + 1;
 }
 
+/* damerau_levenshtein_distance (DamerauLevenshtein_impl.hpp:105-121): the length
+ * difference bound, then the exact distance, cut. */
+static uint64_t damerau_cached_distance(const Pair* p, uint64_t c)
+{
+    uint64_t min_edits = p->l1 > p->l2 ? p->l1 - p->l2 : p->l2 - p->l1;
+    if (min_edits > c) return c + 1;
+    uint64_t d = orc_damerau_raw(p->s1, p->l1, p->s2, p->l2);
+    return d <= c ? d : c + 1;
+}
+
 /* CachedOSA::_distance (OSA.hpp:253-267). */
 static uint64_t osa_cached_distance(const Pair* p, uint64_t c)
 {
@@ -353,7 +431,8 @@ static uint64_t int_maximum(const Pair* p)
     case ORC_OSA:
     case ORC_HAMMING:
     case ORC_PREFIX:
-    case ORC_POSTFIX: return u64max(p->l1, p->l2);
+    case ORC_POSTFIX:
+    case ORC_DAMERAU_LEVENSHTEIN: return u64max(p->l1, p->l2); /* DamerauLevenshtein.hpp:127-131 */
     }
     return 0;
 }
@@ -374,6 +453,7 @@ static uint64_t int_distance_core(const Pair* p, uint64_t c)
     case ORC_INDEL: return indel_cached_distance(p, c);
     case ORC_OSA: return osa_cached_distance(p, c);
     case ORC_HAMMING: return hamming_cached_distance(p, c);
+    case ORC_DAMERAU_LEVENSHTEIN: return damerau_cached_distance(p, c);
     }
     return 0;
 }
@@ -942,6 +1022,7 @@ int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const 
     case ORC_HAMMING:
     case ORC_PREFIX:
     case ORC_POSTFIX:
+    case ORC_DAMERAU_LEVENSHTEIN:
         if (metric != ORC_LEVENSHTEIN) p.ins = p.del = 1, p.rep = (metric == ORC_INDEL) ? 2 : 1;
         switch (kind) {
         case ORC_DISTANCE: *out_u = int_api_distance(&p, cutoff_u); return 0;

@@ -44,7 +44,8 @@ enum {
     ORC_TOKEN_SET_RATIO = 14,
     ORC_TOKEN_RATIO = 15,
     ORC_PARTIAL_TOKEN_SET_RATIO = 16,
-    ORC_PARTIAL_TOKEN_RATIO = 17
+    ORC_PARTIAL_TOKEN_RATIO = 17,
+    ORC_DAMERAU_LEVENSHTEIN = 18 /* experimental::CachedDamerauLevenshtein */
 };
 
 enum { ORC_DISTANCE = 0, ORC_SIMILARITY = 1, ORC_NORM_DISTANCE = 2, ORC_NORM_SIMILARITY = 3 };
@@ -54,6 +55,7 @@ uint64_t orc_levenshtein_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, 
                              uint64_t ins, uint64_t del, uint64_t rep);
 uint64_t orc_lcs_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 uint64_t orc_osa_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
+uint64_t orc_damerau_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 /* positional metrics: Hamming distance with padding, common prefix / suffix length */
 uint64_t orc_hamming_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 uint64_t orc_prefix_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);

@@ -65,6 +65,7 @@ int algo_for(int metric, const rfa_weights& w)
     case RFA_PARTIAL_TOKEN_RATIO: return ALGO_TSET;
     case RFA_JARO:
     case RFA_JARO_WINKLER: return ALGO_JARO;
+    case RFA_DAMERAU_LEVENSHTEIN: return ALGO_DL;
     }
     return -1;
 }
@@ -77,7 +78,8 @@ bool multi_levenshtein_weights(const rfa_weights& w)
 bool is_int_kind(int metric, int kind)
 {
     return (metric == RFA_LEVENSHTEIN || metric == RFA_INDEL || metric == RFA_LCSSEQ || metric == RFA_OSA ||
-            metric == RFA_HAMMING || metric == RFA_PREFIX || metric == RFA_POSTFIX) &&
+            metric == RFA_HAMMING || metric == RFA_PREFIX || metric == RFA_POSTFIX ||
+            metric == RFA_DAMERAU_LEVENSHTEIN) &&
            (kind == RFA_DISTANCE || kind == RFA_SIMILARITY);
 }
 
@@ -495,6 +497,8 @@ void validate_kind(int metric, int kind, int semantics)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "partial_ratio has no Multi* scorer");
     if (semantics == RFA_MULTI && (metric == RFA_HAMMING || metric == RFA_PREFIX || metric == RFA_POSTFIX))
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "Hamming, Prefix and Postfix have no Multi* scorer");
+    if (semantics == RFA_MULTI && metric == RFA_DAMERAU_LEVENSHTEIN)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "Damerau-Levenshtein has no Multi* scorer");
 }
 
 // Block-kernel launch over a cross product (queries x choices) or explicit items.
@@ -791,6 +795,45 @@ void cdist_wlev(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, vo
     }
 }
 
+// Damerau-Levenshtein: the tile kernel over the scorer's query tiles; queries the
+// tile plan does not take (longer than 64 units, or tiles of too many distinct
+// symbols) are not accelerated.
+void cdist_dl(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, void* dout, const uint64_t* d_qcut)
+{
+    rfa_context* ctx = sc->ctx;
+    if (!sc->block_queries.empty())
+        throw ApiError(RFA_ERR_UNSUPPORTED, "Damerau-Levenshtein queries longer than 64 code units (or query tiles "
+                                            "of more than 255 distinct symbols) are not accelerated");
+    if (ch->max_len >= (1ull << 29))
+        throw ApiError(RFA_ERR_UNSUPPORTED, "Damerau-Levenshtein choices of 2^29 or more code units");
+    const uint64_t nc = ch->count;
+    TileArgs ta{};
+    ta.qchars = sc->q->d_chars;
+    ta.qoff = sc->q->d_off;
+    ta.qorder = sc->d_tile_order;
+    ta.nq = (uint32_t)sc->tile_order.size();
+    ta.cchars = ch->d_chars;
+    ta.coff = ch->d_off;
+    ta.corder = nullptr;
+    ta.ncsel = (uint32_t)nc;
+    ta.out_stride = nc;
+    ta.hash_bits = sc->hash_bits;
+    ta.spec = spec;
+    ta.out = dout;
+    ta.qcut = d_qcut;
+    for (const TileClass& tc : sc->classes) {
+        ta.tile_base = tc.first_tile;
+        ta.rows_cap = tc.rows_cap;
+        uint32_t target_blocks = (uint32_t)ctx->num_cus * 12;
+        uint32_t nbx = std::max<uint32_t>(1, (target_blocks + tc.ntiles - 1) / tc.ntiles);
+        uint32_t cpb = std::max<uint32_t>(32, (ta.ncsel + nbx - 1) / nbx);
+        cpb = ((cpb + 31) / 32) * 32; // whole 4-choice runs for each of the 8 waves
+        ta.choices_per_block = cpb;
+        nbx = (ta.ncsel + cpb - 1) / cpb;
+        HIP_CHECK(launch_dltile(sc->q->width, tc.word_bits, ta, tc.ntiles, nbx, ctx->stream));
+    }
+}
+
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
                 int out_on_device, const uint64_t* qcut_host)
 {
@@ -832,8 +875,11 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         }
         return;
     }
-    if (algo == ALGO_WLEV) {
-        cdist_wlev(sc, ch, spec, dout, d_qcut);
+    if (algo == ALGO_WLEV || algo == ALGO_DL) {
+        if (algo == ALGO_WLEV)
+            cdist_wlev(sc, ch, spec, dout, d_qcut);
+        else
+            cdist_dl(sc, ch, spec, dout, d_qcut);
         if (!out_on_device) {
             HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
@@ -1295,7 +1341,7 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
 {
     if (!ctx || !queries || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, queries or out is NULL");
     return guarded([&] {
-        if (metric < RFA_LEVENSHTEIN || metric > RFA_PARTIAL_TOKEN_RATIO)
+        if (metric < RFA_LEVENSHTEIN || metric > RFA_DAMERAU_LEVENSHTEIN)
             throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
         if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};

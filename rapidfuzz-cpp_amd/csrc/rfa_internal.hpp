@@ -242,6 +242,11 @@ constexpr uint32_t WLEV_LONG_MAX = 1024;
 hipError_t launch_wlong(int char_width, uint32_t cells_per_lane, const WlevLongArgs& a, uint32_t nblocks,
                         hipStream_t s);
 
+// Damerau-Levenshtein (rfa_dl.hip): query tiles of at most 64 units (TileArgs, the
+// scorer's tile plan), integer or double results like k_tile.
+hipError_t launch_dltile(int char_width, int word_bits, const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x,
+                         hipStream_t s);
+
 // Widest shorter string (in 64-unit words) the long path takes.
 constexpr uint32_t PARTIAL_MAX_WORDS = 16;
 hipError_t launch_plong(int char_width, uint32_t needle_words, const PartialLongArgs& a, uint32_t nblocks,

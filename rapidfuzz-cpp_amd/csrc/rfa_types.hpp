@@ -32,7 +32,8 @@ enum Algo : int32_t {
                     // (Prefix_impl.hpp:23-27, Postfix_impl.hpp:23-27)
     ALGO_PARTIAL = 6, // LCS of the shorter string against every window of the longer one (fuzz_impl.hpp:57-279)
     ALGO_TSET = 7,    // masked LCS of two token-set differences (fuzz_impl.hpp:370-426)
-    ALGO_WLEV = 8     // Wagner-Fischer with general weights (Levenshtein_impl.hpp:85-111)
+    ALGO_WLEV = 8,    // Wagner-Fischer with general weights (Levenshtein_impl.hpp:85-111)
+    ALGO_DL = 9       // Damerau-Levenshtein, Zhao and Sahni (DamerauLevenshtein_impl.hpp:36-103)
 };
 
 // A packed string set on the device.

@@ -951,6 +951,26 @@ struct MultiJaroWinkler : detail::MultiBase<RFA_JARO_WINKLER, MaxLen, double, 0,
     {}
 };
 
+// experimental::CachedDamerauLevenshtein (DamerauLevenshtein.hpp:112-140): queries of at
+// most 64 code units (longer ones throw, RFA_ERR_UNSUPPORTED)
+template <typename CharT1>
+struct CachedDamerauLevenshtein
+    : rapidfuzz_amd::detail::CachedBase<RFA_DAMERAU_LEVENSHTEIN, size_t, 0, rapidfuzz_amd::detail::kIntMax> {
+    template <typename Sentence1>
+    explicit CachedDamerauLevenshtein(const Sentence1& s1_)
+        : CachedDamerauLevenshtein(rapidfuzz_amd::detail::to_begin(s1_), rapidfuzz_amd::detail::to_end(s1_))
+    {}
+    template <typename InputIt1>
+    CachedDamerauLevenshtein(InputIt1 first1, InputIt1 last1)
+    {
+        this->init(first1, last1, {1, 1, 1}, 0.1);
+    }
+};
+
+template <typename Sentence1>
+explicit CachedDamerauLevenshtein(const Sentence1& s1_)
+    -> CachedDamerauLevenshtein<rapidfuzz_amd::detail::sentence_char_t<Sentence1>>;
+
 } // namespace experimental
 
 namespace fuzz {

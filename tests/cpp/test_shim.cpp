@@ -73,6 +73,16 @@ int main()
             CHECK(std::fabs(w123.normalized_similarity(std::string("sitting")) - (1.0 - 7.0 / 19.0)) < 1e-12);
         }
         {
+            // tests-DamerauLevenshtein.cpp:106-130
+            rfa::experimental::CachedDamerauLevenshtein dl(std::string("abaa"));
+            CHECK(dl.distance(std::string("baaa")) == 1);
+            CHECK(std::fabs(dl.normalized_similarity(std::string("baaa")) - 0.75) < 1e-12);
+            rfa::experimental::CachedDamerauLevenshtein<char> ca(std::string("CA"));
+            CHECK(ca.distance(std::string("ABC")) == 2);
+            CHECK(ca.distance(std::string("ABC"), 1) == 2);
+            CHECK(std::fabs(ca.normalized_similarity(std::string("ABC")) - 1.0 / 3.0) < 1e-12);
+        }
+        {
             rfa::CachedIndel<char> indel(std::string("aaaa"));
             CHECK(indel.distance(std::string("aaaa")) == 0);
             CHECK(indel.similarity(std::string("aaaa")) == 8);
