@@ -782,6 +782,89 @@ def workload_wlev(args, emit=True):
     return line
 
 
+DL_OPS_PER_CELL = 16
+
+
+def workload_dl(args, emit=True):
+    """experimental::CachedDamerauLevenshtein (Zhao and Sahni, DamerauLevenshtein_impl.hpp:36-121)
+    on the headline cdist shape: 4096 x 65,536 printable uint8 strings of 1-64 units, distance,
+    no cutoff."""
+    import rapidfuzz_amd as rfa
+    from rapidfuzz_amd import hip
+
+    rng = np.random.default_rng(SEED)
+    qc, qo, ql = gen(rng, NQ)
+    cc, co, cl = gen(rng, NC)
+    cells = float(ql.sum()) * float(cl.sum())
+    ctx = rfa.Context(0)
+    qs = rfa.Strings(ctx, chars=qc, offsets=qo, width=1)
+    cs = rfa.Strings(ctx, chars=cc, offsets=co, width=1)
+    out = hip.DeviceBuffer(NQ * NC * 8)
+    sc = rfa.Scorer(ctx, rfa.DAMERAU_LEVENSHTEIN, qs)
+    for _ in range(max(1, args.warmup)):
+        sc.cdist(cs, rfa.DISTANCE, out_device_ptr=out.ptr.value)
+    ctx.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    steps = max(1, args.steps)
+    e0.record(ctx.stream)
+    for _ in range(steps):
+        sc.cdist(cs, rfa.DISTANCE, out_device_ptr=out.ptr.value)
+    e1.record(ctx.stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    gcups = cells / (ms / 1e3) / 1e9
+    cus = hip.device_attribute(hip.ATTR_CU_COUNT)
+    clk_khz = hip.device_attribute(hip.ATTR_CLOCK_RATE_KHZ)
+    peak = cus * 64 * clk_khz * 1e3 / 1e12
+    tops = cells * DL_OPS_PER_CELL / (ms / 1e3) / 1e12
+    print(f"# damerau-levenshtein: {ms:.3f} ms/step, {gcups:.1f} GCUPS", file=sys.stderr, flush=True)
+    line = {"metric": "Damerau-Levenshtein cdist 4096 x 65536, GCUPS", "value": round(gcups, 1), "unit": "GCUPS",
+            "n_gpus": 1, "steps": steps, "warmup": max(1, args.warmup), "ms_per_step": round(ms, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+            "config": {"workload": f"experimental::CachedDamerauLevenshtein cdist {NQ} x {NC}, uint8 printable, "
+                                   f"lengths {LMIN}-{LMAX}, distance, no cutoff", "cells": cells},
+            "roofline": {"bound": "valu", "achieved": round(tops, 3), "peak": round(peak, 3), "unit": "TOP/s",
+                         "frac": round(tops / peak, 4), "traffic": None,
+                         "ops_model": f"{DL_OPS_PER_CELL} INT32 ops per DP cell (4 for the Levenshtein terms, 7 "
+                                      "for the two transposition terms, 5 for the match bookkeeping)",
+                         "kernel": "k_dltile<u8, 64/32-bit PM words>"}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        if oracle_py.reference_available():
+            ref = oracle_py.Reference()
+            cores = _cores()
+            k = 16
+            while True:
+                idx = np.linspace(0, NQ - 1, k).astype(np.int64)
+                sq, so = pack_u8([qc[int(qo[i]):int(qo[i + 1])] for i in idx])
+                t0 = time.perf_counter()
+                r = ref.cdist_u8_mt(rfa.DAMERAU_LEVENSHTEIN, rfa.DISTANCE, sq, so, cc, co, threads=cores)
+                dt = time.perf_counter() - t0
+                if dt > 3 or k >= NQ:
+                    break
+                k = min(NQ, k * 4)
+            rows = np.zeros(NC, np.uint64)
+            bad = 0
+            for k_, i in enumerate(idx):
+                out.to_host(rows, offset=int(i) * NC * 8)
+                bad += int(np.count_nonzero(rows != r[k_]))
+            sample_cells = float(sum(ql[int(i)] for i in idx)) * float(cl.sum())
+            line["cpu_baseline"] = {"value": round(sample_cells / dt / 1e9, 3), "unit": "GCUPS", "cores": cores,
+                                    "kind": "reference",
+                                    "sample": f"{k} of {NQ} queries x all {NC} choices, "
+                                              f"CachedDamerauLevenshtein ({dt:.1f} s)"}
+            line["check"] = "bit-exact vs reference on the CPU sample" if bad == 0 else f"{bad} MISMATCHES"
+    sc.close()
+    out.free()
+    for x in (qs, cs):
+        x.close()
+    ctx.close()
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
+
+
 def workload_tokens(args, emit=True):
     """token_sort_ratio, partial_token_sort_ratio, QRatio, token_set_ratio and token_ratio cdist on the
     partial_ratio data (1024
@@ -1014,7 +1097,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="cdist",
                     choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "partial", "tokens", "wlev",
-                             "copy"],
+                             "dl", "copy"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
                          "uneven: 1M independent pairs; serve: the serving leg alone; latency: 1 x 10,000 "
                          "Levenshtein; dna: 256 x 16,384 long sequences with band cutoffs; copy: HBM counter "
@@ -1053,6 +1136,8 @@ def main():
         return workload_tokens(args)
     if args.workload == "wlev":
         return workload_wlev(args)
+    if args.workload == "dl":
+        return workload_dl(args)
 
     import rapidfuzz_amd as rfa
     from rapidfuzz_amd import hip
@@ -1218,6 +1303,9 @@ def main():
         wl = workload_wlev(args, emit=False)
         res["configs"]["weighted_levenshtein_4096x65536"] = {**{k: wl.get(k) for k in keep},
                                                              "config": wl["config"]["workload"]}
+        dlv = workload_dl(args, emit=False)
+        res["configs"]["damerau_levenshtein_4096x65536"] = {**{k: dlv.get(k) for k in keep},
+                                                            "config": dlv["config"]["workload"]}
         une = workload_uneven(args, emit=False)
         res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
                                              "speedup_vs_plain": une["speedup_vs_plain"],
