@@ -80,6 +80,8 @@ class Oracle:
                                 ctypes.c_uint64, ctypes.c_double, _u64p, _f64p]
         L.orc_set_units.restype = None
         L.orc_set_units.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_editops.restype = ctypes.c_size_t
+        L.orc_editops.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _i32p, _u64p, _u64p]
         L.orc_sorted_join.restype = ctypes.c_size_t
         L.orc_sorted_join.argtypes = [_u32p, ctypes.c_size_t, _u32p]
         L.orc_jaro_raw.restype = None
@@ -115,6 +117,18 @@ class Oracle:
 
     def score(self, metric, kind, s1, s2, **kw):
         return self.cdist(metric, kind, [s1], [s2], **kw)[0, 0]
+
+    def editops(self, metric, s1, s2):
+        """(type, src_pos, dest_pos) arrays of levenshtein_editops (metric 0) or
+        lcs_seq_editops / indel_editops (metric 1 or 2)."""
+        a, b = to_codes(s1), to_codes(s2)
+        n = len(a) + len(b)
+        t = np.zeros(max(1, n), np.int32)
+        sp = np.zeros(max(1, n), np.uint64)
+        dp = np.zeros(max(1, n), np.uint64)
+        k = self.lib.orc_editops(metric, _p(a, _u32p), len(a), _p(b, _u32p), len(b), _p(t, _i32p), _p(sp, _u64p),
+                                 _p(dp, _u64p))
+        return t[:k], sp[:k], dp[:k]
 
     def sorted_join(self, s, char_width=4, signed8=False):
         """detail::sorted_split(s).join() as uint32 code units."""
@@ -196,6 +210,9 @@ class Reference:
                                                _u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64,
                                                ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
                                                _u64p, _f64p, ctypes.c_int]
+        L.ref_editops.restype = ctypes.c_int64
+        L.ref_editops.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _i32p, _u64p,
+                                  _u64p]
         L.ref_pairs_u32_mt.restype = ctypes.c_int
         L.ref_pairs_u32_mt.argtypes = [_i32p, _u32p, _u64p, _u32p, _u64p, ctypes.c_size_t, _u64p,
                                        ctypes.c_int]
@@ -257,6 +274,19 @@ class Reference:
         if r < 0:
             raise ValueError(f"reference error {r}: {self._err()}")
         return (ou if is_int_result(metric, kind) else od).reshape(nq, nc)
+
+    def editops(self, metric, s1, s2):
+        a = np.ascontiguousarray(to_codes(s1), dtype=np.uint32)
+        b = np.ascontiguousarray(to_codes(s2), dtype=np.uint32)
+        n = len(a) + len(b)
+        t = np.zeros(max(1, n), np.int32)
+        sp = np.zeros(max(1, n), np.uint64)
+        dp = np.zeros(max(1, n), np.uint64)
+        k = self.lib.ref_editops(metric, _p(a, _u32p), len(a), _p(b, _u32p), len(b), _p(t, _i32p), _p(sp, _u64p),
+                                 _p(dp, _u64p))
+        if k < 0:
+            raise ValueError(f"reference error {k}: {self._err()}")
+        return t[:k], sp[:k], dp[:k]
 
     def pairs_mt(self, metrics, achars, aoff, bchars, boff, threads=None):
         n = len(aoff) - 1

@@ -644,4 +644,25 @@ int ref_extract_ratio_u8_mt(const uint8_t* q, const uint64_t* qoff, size_t nq, c
     return 0;
 }
 
+// levenshtein_editops (metric 0) or lcs_seq_editops (otherwise) of one pair of 32-bit
+// strings; returns the count, writes at most l1 + l2 operations.
+int64_t ref_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, int32_t* type,
+                    uint64_t* src_pos, uint64_t* dest_pos)
+{
+    try {
+        std::vector<uint32_t> a(s1, s1 + l1), b(s2, s2 + l2);
+        rapidfuzz::Editops ops = metric == 0 ? rapidfuzz::levenshtein_editops(a, b) : rapidfuzz::lcs_seq_editops(a, b);
+        for (size_t i = 0; i < ops.size(); ++i) {
+            type[i] = static_cast<int32_t>(ops[i].type);
+            src_pos[i] = ops[i].src_pos;
+            dest_pos[i] = ops[i].dest_pos;
+        }
+        return static_cast<int64_t>(ops.size());
+    }
+    catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    }
+}
+
 } // extern "C"

@@ -397,6 +397,113 @@ static uint64_t lev_cached_distance(const Pair* p, uint64_t c)
     return d <= c ? d : c + 1;
 }
 
+/* ---- editops --------------------------------------------------------------
+ * levenshtein_editops (Levenshtein_impl.hpp:1229-1257) and lcs_seq_editops
+ * (LCSseq_impl.hpp:528-534) strip the common prefix and suffix, record the
+ * bit-parallel matrix of the remaining strings and walk it back
+ * (recover_alignment, Levenshtein_impl.hpp:959-1015, LCSseq_impl.hpp:440-506).
+ * The bits are restated here from plain DP tables: for s1 = a (columns) and
+ * s2 = b (rows), VP / VN of row r and column c mark D[c][r] - D[c-1][r] = +1 / -1
+ * (D the Levenshtein table over prefixes), and the LCS state S has the bit set
+ * where L[c][r] == L[c-1][r] (no LCS step at column c).  Levenshtein's walk is
+ * the reference's only while the stripped s1 has at most 64 units (longer
+ * ones record banded or Hirschberg matrices); LCS's holds for every length. */
+enum { ED_REPLACE = 1, ED_INSERT = 2, ED_DELETE = 3 };
+
+size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, int32_t* type,
+                   uint64_t* src_pos, uint64_t* dest_pos)
+{
+    size_t pre = 0;
+    while (pre < l1 && pre < l2 && s1[pre] == s2[pre]) ++pre;
+    size_t suf = 0;
+    while (suf < l1 - pre && suf < l2 - pre && s1[l1 - 1 - suf] == s2[l2 - 1 - suf]) ++suf;
+    const uint32_t* a = s1 + pre;
+    const uint32_t* b = s2 + pre;
+    const size_t la = l1 - pre - suf, lb = l2 - pre - suf;
+    const size_t W = la + 1;
+    uint64_t* T = (uint64_t*)malloc((la + 1) * (lb + 1) * sizeof(uint64_t)); /* T[r * W + c] */
+    const int lev = metric == ORC_LEVENSHTEIN;
+    for (size_t c = 0; c <= la; ++c) T[c] = lev ? c : 0;
+    for (size_t r = 1; r <= lb; ++r) {
+        T[r * W] = lev ? r : 0;
+        for (size_t c = 1; c <= la; ++c) {
+            uint64_t v;
+            if (lev) {
+                v = T[(r - 1) * W + c - 1] + (a[c - 1] != b[r - 1]);
+                if (T[(r - 1) * W + c] + 1 < v) v = T[(r - 1) * W + c] + 1;
+                if (T[r * W + c - 1] + 1 < v) v = T[r * W + c - 1] + 1;
+            }
+            else {
+                if (a[c - 1] == b[r - 1])
+                    v = T[(r - 1) * W + c - 1] + 1;
+                else
+                    v = T[(r - 1) * W + c] > T[r * W + c - 1] ? T[(r - 1) * W + c] : T[r * W + c - 1];
+            }
+            T[r * W + c] = v;
+        }
+    }
+    size_t dist = lev ? (size_t)T[lb * W + la] : la + lb - 2 * (size_t)T[lb * W + la];
+    const size_t total = dist;
+    size_t col = la, row = lb;
+#define ED_PUT(t)                                                                                                    \
+    do {                                                                                                         \
+        --dist;                                                                                                  \
+        type[dist] = (t);                                                                                        \
+        src_pos[dist] = col + pre;                                                                               \
+        dest_pos[dist] = row + pre;                                                                              \
+    } while (0)
+    if (lev) {
+        while (row && col) {
+            /* VP bit (row - 1, col - 1): D[col][row] - D[col-1][row] == +1 */
+            if (T[row * W + col] == T[row * W + col - 1] + 1) {
+                --col;
+                ED_PUT(ED_DELETE);
+            }
+            else {
+                --row;
+                /* VN bit (row - 1, col - 1) of the row above: D[col][row] - D[col-1][row] == -1 */
+                if (row && T[row * W + col] + 1 == T[row * W + col - 1]) {
+                    ED_PUT(ED_INSERT);
+                }
+                else {
+                    --col;
+                    if (a[col] != b[row]) ED_PUT(ED_REPLACE);
+                }
+            }
+        }
+    }
+    else {
+        while (row && col) {
+            /* S bit (row - 1, col - 1): no LCS step at column col */
+            if (T[row * W + col] == T[row * W + col - 1]) {
+                --col;
+                ED_PUT(ED_DELETE);
+            }
+            else {
+                --row;
+                /* S bit (row - 1, col - 1) clear: an LCS step at column col already by this row */
+                if (row && T[row * W + col] != T[row * W + col - 1]) {
+                    ED_PUT(ED_INSERT);
+                }
+                else {
+                    --col;
+                }
+            }
+        }
+    }
+    while (col) {
+        --col;
+        ED_PUT(ED_DELETE);
+    }
+    while (row) {
+        --row;
+        ED_PUT(ED_INSERT);
+    }
+#undef ED_PUT
+    free(T);
+    return total;
+}
+
 /* damerau_levenshtein_distance (DamerauLevenshtein_impl.hpp:105-121): the length
  * difference bound, then the exact distance, cut. */
 static uint64_t damerau_cached_distance(const Pair* p, uint64_t c)

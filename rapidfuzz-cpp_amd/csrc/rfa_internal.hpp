@@ -247,6 +247,24 @@ hipError_t launch_wlong(int char_width, uint32_t cells_per_lane, const WlevLongA
 hipError_t launch_dltile(int char_width, int word_bits, const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x,
                          hipStream_t s);
 
+// Editops bit matrices (rfa_editops.hip): item i pairs the stripped first string
+// A[a_start, a_start + a_len) (a_len <= 64) with B[b_start, b_start + b_len); its
+// rows start at rows[row_off] (Levenshtein: VP, VN per text unit; LCS: S); score
+// receives the distance (Levenshtein) or the LCS length.
+struct EditopsArgs {
+    const void* achars;
+    const void* bchars;
+    const uint64_t* a_start;
+    const uint32_t* a_len;
+    const uint64_t* b_start;
+    const uint32_t* b_len;
+    const uint64_t* row_off;
+    uint64_t* rows;
+    uint32_t* score;
+    uint32_t n;
+};
+hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, uint32_t nblocks, hipStream_t s);
+
 // Widest shorter string (in 64-unit words) the long path takes.
 constexpr uint32_t PARTIAL_MAX_WORDS = 16;
 hipError_t launch_plong(int char_width, uint32_t needle_words, const PartialLongArgs& a, uint32_t nblocks,

@@ -557,6 +557,119 @@ explicit CachedPrefix(const Sentence1& s1_) -> CachedPrefix<detail::sentence_cha
 template <typename Sentence1>
 explicit CachedPostfix(const Sentence1& s1_) -> CachedPostfix<detail::sentence_char_t<Sentence1>>;
 
+// ---- edit operations (details/types.hpp:28-60, 167-355: the parts a caller reads) ----
+enum class EditType { None = RFA_EDIT_NONE, Replace = RFA_EDIT_REPLACE, Insert = RFA_EDIT_INSERT, Delete = RFA_EDIT_DELETE };
+
+struct EditOp {
+    EditType type = EditType::None;
+    size_t src_pos = 0;
+    size_t dest_pos = 0;
+    EditOp() = default;
+    EditOp(EditType t, size_t s, size_t d) : type(t), src_pos(s), dest_pos(d) {}
+    friend bool operator==(const EditOp& a, const EditOp& b)
+    {
+        return a.type == b.type && a.src_pos == b.src_pos && a.dest_pos == b.dest_pos;
+    }
+    friend bool operator!=(const EditOp& a, const EditOp& b) { return !(a == b); }
+};
+
+class Editops : private std::vector<EditOp> {
+    using Base = std::vector<EditOp>;
+
+public:
+    using Base::begin;
+    using Base::empty;
+    using Base::end;
+    using Base::size;
+    using Base::operator[];
+    using Base::push_back;
+    using Base::reserve;
+    size_t get_src_len() const { return src_len_; }
+    size_t get_dest_len() const { return dest_len_; }
+    void set_src_len(size_t n) { src_len_ = n; }
+    void set_dest_len(size_t n) { dest_len_ = n; }
+    friend bool operator==(const Editops& a, const Editops& b)
+    {
+        return a.src_len_ == b.src_len_ && a.dest_len_ == b.dest_len_ &&
+               static_cast<const Base&>(a) == static_cast<const Base&>(b);
+    }
+
+private:
+    size_t src_len_ = 0, dest_len_ = 0;
+};
+
+namespace detail {
+
+// One pair through rfa_editops (sizes first, then the operations).
+inline Editops editops_of(int metric, Packed a, Packed b)
+{
+    const int w = std::max(a.width, b.width);
+    rfa_context* ctx = context();
+    DeviceStrings da(ctx, a, w), db(ctx, b, w);
+    uint64_t off[2] = {0, 0};
+    check(rfa_editops(ctx, da.get(), db.get(), metric, off, nullptr, 0));
+    std::vector<rfa_editop> raw(std::max<uint64_t>(1, off[1]));
+    check(rfa_editops(ctx, da.get(), db.get(), metric, off, raw.data(), raw.size()));
+    Editops ops;
+    ops.reserve(off[1]);
+    for (uint64_t i = 0; i < off[1]; ++i)
+        ops.push_back(EditOp(static_cast<EditType>(raw[i].type), raw[i].src_pos, raw[i].dest_pos));
+    ops.set_src_len(a.off[1]);
+    ops.set_dest_len(b.off[1]);
+    return ops;
+}
+
+template <typename It1, typename It2>
+Editops editops_of(int metric, It1 first1, It1 last1, It2 first2, It2 last2)
+{
+    Packed a, b;
+    a.add(first1, last1);
+    b.add(first2, last2);
+    return editops_of(metric, std::move(a), std::move(b));
+}
+
+} // namespace detail
+
+// levenshtein_editops (Levenshtein.hpp:282-295): pairs whose first string has at most 64
+// code units once the common prefix and suffix are removed; score_hint does not change
+// the result
+template <typename InputIt1, typename InputIt2>
+Editops levenshtein_editops(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2,
+                            size_t score_hint = std::numeric_limits<size_t>::max())
+{
+    (void)score_hint;
+    return detail::editops_of(RFA_LEVENSHTEIN, first1, last1, first2, last2);
+}
+template <typename Sentence1, typename Sentence2>
+Editops levenshtein_editops(const Sentence1& s1, const Sentence2& s2,
+                            size_t score_hint = std::numeric_limits<size_t>::max())
+{
+    return levenshtein_editops(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2), detail::to_end(s2),
+                               score_hint);
+}
+
+// lcs_seq_editops (LCSseq.hpp:65-75) and indel_editops (Indel.hpp:65-75, the same script)
+template <typename InputIt1, typename InputIt2>
+Editops lcs_seq_editops(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2)
+{
+    return detail::editops_of(RFA_LCSSEQ, first1, last1, first2, last2);
+}
+template <typename Sentence1, typename Sentence2>
+Editops lcs_seq_editops(const Sentence1& s1, const Sentence2& s2)
+{
+    return lcs_seq_editops(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2), detail::to_end(s2));
+}
+template <typename InputIt1, typename InputIt2>
+Editops indel_editops(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2)
+{
+    return lcs_seq_editops(first1, last1, first2, last2);
+}
+template <typename Sentence1, typename Sentence2>
+Editops indel_editops(const Sentence1& s1, const Sentence2& s2)
+{
+    return lcs_seq_editops(s1, s2);
+}
+
 namespace fuzz {
 
 namespace detail {

@@ -31,7 +31,7 @@ _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED",
 EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
             "rfa_context_synchronize", "rfa_context_stream", "rfa_context_set_option", "rfa_strings_create", "rfa_strings_destroy",
             "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_scorer_set_option", "rfa_cdist", "rfa_cdist_cutoffs",
-            "rfa_pairs", "rfa_pairs_autotune", "rfa_topk",
+            "rfa_pairs", "rfa_pairs_autotune", "rfa_topk", "rfa_editops",
             "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
             "rfa_server_get_stats"]
 
@@ -96,6 +96,7 @@ def load_library(path: str = LIB_PATH):
                             _vp, ctypes.c_int]
     L.rfa_pairs_autotune.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _vp]
     L.rfa_topk.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp]
+    L.rfa_editops.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64]
     L.rfa_server_create.argtypes = [_vp, _vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
     L.rfa_server_destroy.argtypes = [_vp]
     L.rfa_server_submit.argtypes = [_vp, _vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
@@ -106,7 +107,7 @@ def load_library(path: str = LIB_PATH):
                  "rfa_strings_destroy", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_cdist_cutoffs",
                  "rfa_pairs", "rfa_pairs_autotune", "rfa_context_set_option", "rfa_scorer_set_option",
                  "rfa_topk", "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait",
-                 "rfa_server_replay", "rfa_server_get_stats"):
+                 "rfa_server_replay", "rfa_server_get_stats", "rfa_editops"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -356,6 +357,20 @@ class Server:
         st = ServerStats()
         _check(_lib.rfa_server_get_stats(self.h, ctypes.byref(st)))
         return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+
+EDIT_NONE, EDIT_REPLACE, EDIT_INSERT, EDIT_DELETE = range(4)
+EDITOP_DTYPE = np.dtype([("type", np.int32), ("reserved", np.int32), ("src_pos", np.uint64), ("dest_pos", np.uint64)])
+
+
+def editops(ctx: Context, a: Strings, b: Strings, metric=LEVENSHTEIN):
+    """Edit scripts of the pairs (a[i], b[i]): (offsets [count + 1], ops structured array);
+    pair i's operations are ops[offsets[i]:offsets[i + 1]]."""
+    off = np.zeros(a.count + 1, dtype=np.uint64)
+    _check(_lib.rfa_editops(ctx.h, a.h, b.h, metric, off.ctypes.data, None, 0))
+    ops = np.zeros(max(1, int(off[-1])), dtype=EDITOP_DTYPE)
+    _check(_lib.rfa_editops(ctx.h, a.h, b.h, metric, off.ctypes.data, ops.ctypes.data, len(ops)))
+    return off, ops[:int(off[-1])]
 
 
 def pairs(ctx: Context, a: Strings, b: Strings, metrics, kind=DISTANCE, cutoff_u=INT64_MAX, cutoff_d=1.0,

@@ -40,6 +40,26 @@ static bool throws_invalid(F&& f)
     return false;
 }
 
+// editops_apply_str (details/types.hpp) restated for the checks below
+static std::string apply_editops(const rfa::Editops& ops, const std::string& s1, const std::string& s2)
+{
+    std::string out;
+    size_t src = 0;
+    for (size_t k = 0; k < ops.size(); ++k) {
+        while (src < ops[k].src_pos) out.push_back(s1[src++]);
+        if (ops[k].type == rfa::EditType::Delete)
+            ++src;
+        else if (ops[k].type == rfa::EditType::Insert)
+            out.push_back(s2[ops[k].dest_pos]);
+        else if (ops[k].type == rfa::EditType::Replace) {
+            out.push_back(s2[ops[k].dest_pos]);
+            ++src;
+        }
+    }
+    while (src < s1.size()) out.push_back(s1[src++]);
+    return out;
+}
+
 static std::string random_string(std::mt19937& rng, size_t lo, size_t hi)
 {
     std::uniform_int_distribution<size_t> len(lo, hi);
@@ -71,6 +91,22 @@ int main()
             CHECK(w123.distance(std::string("sitting"), 5) == 6);
             CHECK(w123.distance(std::string("kitchen")) == 4);
             CHECK(std::fabs(w123.normalized_similarity(std::string("sitting")) - (1.0 - 7.0 / 19.0)) < 1e-12);
+        }
+        {
+            // tests-Levenshtein.cpp:235-244, 294-309 and tests-Indel.cpp:275-285 (apply-based)
+            const std::string s = "Lorem ipsum.", d = "XYZLorem ABC iPsum";
+            rfa::Editops ops = rfa::levenshtein_editops(s, d);
+            CHECK(apply_editops(ops, s, d) == d);
+            CHECK(ops.get_src_len() == s.size());
+            CHECK(ops.get_dest_len() == d.size());
+            CHECK(ops.size() == rfa::CachedLevenshtein<char>(s).distance(d));
+            const std::string b1 = "b", b2 = "aaaaaaaaaaaaaaaabbaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaa";
+            CHECK(apply_editops(rfa::levenshtein_editops(b1, b2), b1, b2) == b2);
+            CHECK(apply_editops(rfa::levenshtein_editops(std::string("aa"), std::string("abb")), "aa", "abb") == "abb");
+            rfa::Editops ind = rfa::indel_editops(s, d);
+            CHECK(apply_editops(ind, s, d) == d);
+            CHECK(ind.size() == rfa::CachedIndel<char>(s).distance(d));
+            CHECK(ind == rfa::lcs_seq_editops(s, d));
         }
         {
             // tests-DamerauLevenshtein.cpp:106-130

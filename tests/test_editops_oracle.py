@@ -1,0 +1,104 @@
+"""The oracle's editops (levenshtein_editops, Levenshtein_impl.hpp:959-1015 and
+1229-1257; lcs_seq_editops = indel_editops, LCSseq_impl.hpp:440-534), pinned:
+
+* against the compiled reference, operation for operation: Levenshtein pairs whose
+  first string has at most 64 units after the common affix is removed (where the
+  reference walks the full single-word matrix), LCS pairs of any length;
+* by what any correct script satisfies: applied to s1 it gives s2 (the reference's
+  own editops tests, tests-Levenshtein.cpp:235-316 and tests-Indel.cpp:275-285, are
+  of this form and their strings are used here), its length is the distance, and
+  positions never decrease.
+"""
+import numpy as np
+import pytest
+
+import _gen
+
+LEV, INDEL, LCS = 0, 1, 2
+REPLACE, INSERT, DELETE = 1, 2, 3
+
+
+def apply_ops(ops, s1, s2):
+    """editops_apply (types.hpp) restated: copy s1, applying each op at its positions."""
+    t, sp, dp = ops
+    out = []
+    src = 0
+    for k in range(len(t)):
+        while src < sp[k]:
+            out.append(int(s1[src]))
+            src += 1
+        if t[k] == DELETE:
+            src += 1
+        elif t[k] == INSERT:
+            out.append(int(s2[dp[k]]))
+        elif t[k] == REPLACE:
+            out.append(int(s2[dp[k]]))
+            src += 1
+    while src < len(s1):
+        out.append(int(s1[src]))
+        src += 1
+    return out
+
+
+def _codes(s):
+    return np.frombuffer(s.encode(), np.uint8).astype(np.uint32) if isinstance(s, str) else s
+
+
+REF_STRINGS = [
+    ("Lorem ipsum.", "XYZLorem ABC iPsum"),
+    ("b", "aaaaaaaaaaaaaaaabbaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaa"),
+    ("aa", "abb"),
+]
+
+
+@pytest.mark.parametrize("metric", [LEV, LCS])
+def test_apply_and_count(oracle, metric):
+    rng = np.random.default_rng(5 + metric)
+    pairs = [(_codes(a), _codes(b)) for a, b in REF_STRINGS]
+    pairs += [(_gen.random_strings(rng, 1, 0, 40, alpha_n=3)[0], _gen.random_strings(rng, 1, 0, 50, alpha_n=3)[0])
+              for _ in range(300)]
+    for s1, s2 in pairs:
+        ops = oracle.editops(metric, s1, s2)
+        assert apply_ops(ops, s1, s2) == [int(x) for x in s2]
+        kind_metric = LEV if metric == LEV else INDEL
+        assert len(ops[0]) == int(oracle.score(kind_metric, 0, s1, s2))
+        assert np.all(np.diff(ops[1].astype(np.int64)) >= 0) and np.all(np.diff(ops[2].astype(np.int64)) >= 0)
+
+
+def _check_vs_reference(oracle, reference, metric, pairs):
+    for s1, s2 in pairs:
+        got = oracle.editops(metric, s1, s2)
+        exp = reference.editops(metric, s1, s2)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e), f"metric {metric} {list(s1)} {list(s2)}: {got} vs {exp}"
+
+
+def test_levenshtein_vs_reference(oracle, reference):
+    rng = np.random.default_rng(21)
+    pairs = [(_codes(a), _codes(b)) for a, b in REF_STRINGS]
+    for alpha in (2, 3, 6):
+        for _ in range(150):
+            s1 = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 65))).astype(np.uint32)
+            s2 = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 150))).astype(np.uint32)
+            pairs.append((s1, s2))
+            pairs.append((s1, _gen.mutate(rng, s1, int(rng.integers(0, 8)), alpha_n=alpha)))
+    _check_vs_reference(oracle, reference, LEV, pairs)
+
+
+def test_lcs_vs_reference(oracle, reference):
+    rng = np.random.default_rng(22)
+    pairs = [(_codes(a), _codes(b)) for a, b in REF_STRINGS]
+    for alpha in (2, 4, 8):
+        for _ in range(120):
+            s1 = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 200))).astype(np.uint32)
+            s2 = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 200))).astype(np.uint32)
+            pairs.append((s1, s2))
+            pairs.append((s1, _gen.mutate(rng, s1, int(rng.integers(0, 8)), alpha_n=alpha)))
+    # tests-Indel.cpp:275-285
+    pairs.append((_codes("bbbdbbmbbbbbbbbbBbfbbbbbbbbbbbbbbbbbbbrbbbbbrbbbbbdbnbbbjbhbbbbbbbbbhbbbbbCbobb"
+                         "bxbbbbbkbbbAbxbbwbbbtbcbbbbebbiblbbbbqbbbbbbpbbbbbbubbbkbbDbbbhbkbCbbgbbrbbbbbb"
+                         "bbbbbkbyvbbsbAbbbbz"),
+                  _codes("jaaagaaqyaaaanrCfwaaxaeahtaaaCzaaaspaaBkvaaaaqDaacndaaeolwiaaauaaaaaaamA")))
+    _check_vs_reference(oracle, reference, LCS, pairs)
+    for s1, s2 in pairs[-1:]:
+        assert len(oracle.editops(LCS, s1, s2)[0]) == 231

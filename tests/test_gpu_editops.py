@@ -1,0 +1,82 @@
+"""GPU parity of rfa_editops (levenshtein_editops, lcs_seq_editops / indel_editops)
+against the oracle, operation for operation: random pairs around the 64-unit limit
+of the stripped first string, long second strings (several 64-unit chunks of rows),
+common affixes, empty strings, 8/16/32-bit units, and the reference's editops test
+strings; every script also rebuilds the second string."""
+import numpy as np
+import pytest
+
+import _gen
+from test_editops_oracle import REF_STRINGS, _codes, apply_ops
+
+pytestmark = pytest.mark.gpu
+
+LEV, INDEL, LCS = 0, 1, 2
+
+
+def _pairs(rng, alpha, n=120):
+    out = [(_codes(a), _codes(b)) for a, b in REF_STRINGS]
+    out += [(np.zeros(0, np.uint32), np.zeros(0, np.uint32)), (np.zeros(0, np.uint32), _codes("abc")),
+            (_codes("abc"), np.zeros(0, np.uint32)), (_codes("abc"), _codes("abc"))]
+    for _ in range(n):
+        s1 = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 65))).astype(np.uint32)
+        s2 = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 300))).astype(np.uint32)
+        out.append((s1, s2))
+        out.append((s1, _gen.mutate(rng, s1, int(rng.integers(0, 8)), alpha_n=alpha)))
+    # a common affix around a 64-unit core
+    core = rng.integers(97, 97 + alpha, size=64).astype(np.uint32)
+    pre, suf = _codes("prefix-"), _codes("-suffix")
+    out.append((np.concatenate([pre, core, suf]), np.concatenate([pre, _gen.mutate(rng, core, 5), suf])))
+    return out
+
+
+def check(ctx, oracle, metric, pairs, width=1):
+    import rapidfuzz_amd as rfa
+
+    print(f"metric {metric} width {width} pairs {len(pairs)}", flush=True)
+    a = ctx.strings([p[0] for p in pairs], width=width)
+    b = ctx.strings([p[1] for p in pairs], width=width)
+    off, ops = rfa.editops(ctx, a, b, metric)
+    for i, (s1, s2) in enumerate(pairs):
+        got = ops[int(off[i]):int(off[i + 1])]
+        exp = oracle.editops(metric, s1, s2)
+        assert np.array_equal(got["type"], exp[0]) and np.array_equal(got["src_pos"], exp[1]) and \
+            np.array_equal(got["dest_pos"], exp[2]), f"pair {i} ({len(s1)}, {len(s2)}): {got} vs {exp}"
+        assert apply_ops((got["type"], got["src_pos"], got["dest_pos"]), s1, s2) == [int(x) for x in s2]
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("metric", [LEV, INDEL, LCS])
+@pytest.mark.parametrize("alpha", [2, 5])
+def test_random(ctx, oracle, metric, alpha):
+    check(ctx, oracle, metric, _pairs(np.random.default_rng(10 * metric + alpha), alpha))
+
+
+@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("metric", [LEV, LCS])
+def test_wide_units(ctx, oracle, width, metric):
+    rng = np.random.default_rng(30 + width)
+    lo = 0x4E00 if width == 2 else 0x1F300
+    pairs = [(rng.integers(lo, lo + 6, size=int(rng.integers(0, 65))).astype(np.uint32),
+              rng.integers(lo, lo + 6, size=int(rng.integers(0, 130))).astype(np.uint32)) for _ in range(80)]
+    check(ctx, oracle, metric, pairs, width=width)
+
+
+def test_sizes_then_ops_and_limits(ctx, oracle):
+    import rapidfuzz_amd as rfa
+
+    a = ctx.strings([b"Lorem ipsum.", b"aa"])
+    b = ctx.strings([b"XYZLorem ABC iPsum", b"abb"])
+    off = np.zeros(3, np.uint64)
+    assert rfa._lib.rfa_editops(ctx.h, a.h, b.h, 0, off.ctypes.data, None, 0) == 0
+    small = np.zeros(1, dtype=rfa.EDITOP_DTYPE)
+    assert rfa._lib.rfa_editops(ctx.h, a.h, b.h, 0, off.ctypes.data, small.ctypes.data, 1) == 1  # too small
+    assert int(off[2]) == len(oracle.editops(0, b"Lorem ipsum.", b"XYZLorem ABC iPsum")[0]) + \
+        len(oracle.editops(0, b"aa", b"abb")[0])
+    long_a = ctx.strings([np.full(70, 97, np.uint32)], width=1)
+    long_b = ctx.strings([np.full(70, 98, np.uint32)], width=1)
+    with pytest.raises(rfa.RfaError):
+        rfa.editops(ctx, long_a, long_b, 0)
+    with pytest.raises(rfa.InvalidArgument):
+        rfa.editops(ctx, a, b, 3)  # OSA has no editops
