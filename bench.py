@@ -865,6 +865,84 @@ def workload_dl(args, emit=True):
     return line
 
 
+ED_PAIRS = 262144
+
+
+def workload_editops(args, emit=True):
+    """levenshtein_editops over 262,144 pairs (Levenshtein_impl.hpp:1229-1257): printable uint8
+    first strings of 1-64 units, each against a copy with 0-8 random edits.  The call hands back
+    host arrays, so the timed region is the whole rfa_editops call: item upload, the bit-row
+    kernel, the rows' copy back and the host walk."""
+    import rapidfuzz_amd as rfa
+    from rapidfuzz_amd import hip
+
+    rng = np.random.default_rng(SEED + 11)
+    la = rng.integers(1, 65, size=ED_PAIRS)
+    a_strs, b_strs = [], []
+    for m in la:
+        t = rng.integers(0x20, 0x7F, size=int(m)).astype(np.uint8)
+        u = list(t)
+        for _ in range(int(rng.integers(0, 9))):
+            op = int(rng.integers(0, 3))
+            if op == 0 and u:
+                u[int(rng.integers(0, len(u)))] = int(rng.integers(0x20, 0x7F))
+            elif op == 1:
+                u.insert(int(rng.integers(0, len(u) + 1)), int(rng.integers(0x20, 0x7F)))
+            elif u:
+                del u[int(rng.integers(0, len(u)))]
+        a_strs.append(t)
+        b_strs.append(np.array(u, dtype=np.uint8))
+    ac, ao = pack_u8(a_strs)
+    bc, bo = pack_u8(b_strs)
+    ctx = rfa.Context(0)
+    A = rfa.Strings(ctx, chars=ac, offsets=ao, width=1)
+    B = rfa.Strings(ctx, chars=bc, offsets=bo, width=1)
+    off, ops = rfa.editops(ctx, A, B, rfa.LEVENSHTEIN)
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        off, ops = rfa.editops(ctx, A, B, rfa.LEVENSHTEIN)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    rows = float(np.sum(bo[1:] - bo[:-1]))
+    print(f"# editops: {ms:.3f} ms per call (host-inclusive), {ED_PAIRS / (ms / 1e3) / 1e6:.2f} Mpairs/s, "
+          f"{int(off[-1])} operations", file=sys.stderr, flush=True)
+    line = {"metric": "levenshtein_editops 262144 pairs, Mpairs/s", "value": round(ED_PAIRS / (ms / 1e3) / 1e6, 3),
+            "unit": "Mpairs/s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": round(ms, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+            "config": {"workload": f"levenshtein_editops of {ED_PAIRS} pairs, uint8 printable, first strings 1-64 "
+                                   "units, second = first with 0-8 random edits; whole call (host-inclusive)",
+                       "operations": int(off[-1]), "text_rows": rows},
+            "roofline": None}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        if oracle_py.reference_available():
+            ref = oracle_py.Reference()
+            cores = _cores()
+            t0 = time.perf_counter()
+            counts = ref.editops_counts_u8_mt(0, ac, ao, bc, bo, threads=cores)
+            dt = time.perf_counter() - t0
+            bad = int(np.count_nonzero(counts != (off[1:] - off[:-1])))
+            # operation-for-operation on a sample
+            for i in range(0, ED_PAIRS, ED_PAIRS // 512):
+                exp = ref.editops(0, a_strs[i].astype(np.uint32), b_strs[i].astype(np.uint32))
+                got = ops[int(off[i]):int(off[i + 1])]
+                if not (np.array_equal(got["type"], exp[0]) and np.array_equal(got["src_pos"], exp[1]) and
+                        np.array_equal(got["dest_pos"], exp[2])):
+                    bad += 1
+            line["cpu_baseline"] = {"value": round(ED_PAIRS / dt / 1e6, 3), "unit": "Mpairs/s", "cores": cores,
+                                    "kind": "reference",
+                                    "sample": f"all {ED_PAIRS} pairs, rapidfuzz::levenshtein_editops ({dt:.2f} s)"}
+            line["check"] = ("operation counts of every pair and 512 whole scripts equal the reference"
+                             if bad == 0 else f"{bad} MISMATCHES")
+    A.close()
+    B.close()
+    ctx.close()
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
+
+
 def workload_tokens(args, emit=True):
     """token_sort_ratio, partial_token_sort_ratio, QRatio, token_set_ratio and token_ratio cdist on the
     partial_ratio data (1024
@@ -1097,7 +1175,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="cdist",
                     choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "partial", "tokens", "wlev",
-                             "dl", "copy"],
+                             "dl", "editops", "copy"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
                          "uneven: 1M independent pairs; serve: the serving leg alone; latency: 1 x 10,000 "
                          "Levenshtein; dna: 256 x 16,384 long sequences with band cutoffs; copy: HBM counter "
@@ -1138,6 +1216,8 @@ def main():
         return workload_wlev(args)
     if args.workload == "dl":
         return workload_dl(args)
+    if args.workload == "editops":
+        return workload_editops(args)
 
     import rapidfuzz_amd as rfa
     from rapidfuzz_amd import hip
@@ -1306,6 +1386,10 @@ def main():
         dlv = workload_dl(args, emit=False)
         res["configs"]["damerau_levenshtein_4096x65536"] = {**{k: dlv.get(k) for k in keep},
                                                             "config": dlv["config"]["workload"]}
+        edo = workload_editops(args, emit=False)
+        res["configs"]["levenshtein_editops_262144"] = {k: edo.get(k) for k in ("metric", "value", "unit",
+                                                                                 "ms_per_step", "cpu_baseline",
+                                                                                 "check")}
         une = workload_uneven(args, emit=False)
         res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
                                              "speedup_vs_plain": une["speedup_vs_plain"],

@@ -213,6 +213,8 @@ class Reference:
         L.ref_editops.restype = ctypes.c_int64
         L.ref_editops.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _i32p, _u64p,
                                   _u64p]
+        L.ref_editops_u8_mt.restype = ctypes.c_int
+        L.ref_editops_u8_mt.argtypes = [ctypes.c_int, _u8p, _u64p, _u8p, _u64p, ctypes.c_size_t, _u64p, ctypes.c_int]
         L.ref_pairs_u32_mt.restype = ctypes.c_int
         L.ref_pairs_u32_mt.argtypes = [_i32p, _u32p, _u64p, _u32p, _u64p, ctypes.c_size_t, _u64p,
                                        ctypes.c_int]
@@ -287,6 +289,14 @@ class Reference:
         if k < 0:
             raise ValueError(f"reference error {k}: {self._err()}")
         return t[:k], sp[:k], dp[:k]
+
+    def editops_counts_u8_mt(self, metric, achars, aoff, bchars, boff, threads=None):
+        """Threaded levenshtein_editops / lcs_seq_editops over 8-bit pairs: operations per pair."""
+        n = len(aoff) - 1
+        out = np.zeros(n, dtype=np.uint64)
+        self.lib.ref_editops_u8_mt(metric, _p(achars, _u8p), _p(aoff, _u64p), _p(bchars, _u8p), _p(boff, _u64p), n,
+                                   _p(out, _u64p), threads or os.cpu_count() or 1)
+        return out
 
     def pairs_mt(self, metrics, achars, aoff, bchars, boff, threads=None):
         n = len(aoff) - 1

@@ -665,4 +665,28 @@ int64_t ref_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s
     }
 }
 
+// CPU baseline: editops of pairs (a[i], b[i]) of 8-bit strings over nthreads threads;
+// counts[i] receives pair i's number of operations.
+int ref_editops_u8_mt(int metric, const uint8_t* a, const uint64_t* aoff, const uint8_t* b, const uint64_t* boff,
+                      size_t n, uint64_t* counts, int nthreads)
+{
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+        for (;;) {
+            size_t i0 = next.fetch_add(256);
+            if (i0 >= n) break;
+            for (size_t i = i0; i < std::min(n, i0 + 256); ++i) {
+                std::basic_string<uint8_t> s1(a + aoff[i], a + aoff[i + 1]), s2(b + boff[i], b + boff[i + 1]);
+                rapidfuzz::Editops ops =
+                    metric == 0 ? rapidfuzz::levenshtein_editops(s1, s2) : rapidfuzz::lcs_seq_editops(s1, s2);
+                counts[i] = ops.size();
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < std::max(1, nthreads); ++t) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    return 0;
+}
+
 } // extern "C"

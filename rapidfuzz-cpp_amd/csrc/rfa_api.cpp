@@ -1270,7 +1270,7 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
                           &ctx->block_items[1], &ctx->block_items[2], &ctx->block_items[3], &ctx->plan_keys,
                           &ctx->plan_vals, &ctx->plan_temp, &ctx->topk_scores, &ctx->topk_run[0],
                           &ctx->topk_run[1], &ctx->topk_cand_s, &ctx->topk_cand_i, &ctx->topk_counts, &ctx->topk_q,
-                          &ctx->topk_out, &ctx->qcut, &ctx->ed_items, &ctx->ed_rows})
+                          &ctx->topk_out, &ctx->qcut, &ctx->ed_items, &ctx->ed_rows, &ctx->ed_ops})
             b->release();
         ctx->staging.release();
         ctx->pairs_graph.release();

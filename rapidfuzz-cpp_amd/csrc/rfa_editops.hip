@@ -1,41 +1,98 @@
-// rfa_editops.hip -- the bit matrices behind levenshtein_editops and
-// lcs_seq_editops / indel_editops (Levenshtein_impl.hpp:1229-1257,
-// LCSseq_impl.hpp:528-534).  The host strips each pair's common affix and walks
-// the recorded rows back into edit operations; the device records the rows.
+// rfa_editops.hip -- levenshtein_editops and lcs_seq_editops / indel_editops on
+// the device (Levenshtein_impl.hpp:959-1015 and 1229-1257, LCSseq_impl.hpp:440-534).
 //
-// One wave per pair: lane l holds s1[l] (the stripped first string, at most 64
-// units), so the match mask of a text unit is one ballot, and the single-word
-// recurrences run on wave-uniform 64-bit values:
-//   Levenshtein (Hyyro 2003, Levenshtein_impl.hpp:259-309): VP / VN after every
-//     text unit, plus the final distance;
-//   LCS (Hyyro 2004, LCSseq_impl.hpp:205-250): S after every text unit, plus the
-//     LCS length.
-// Text units arrive 64 at a time (one per lane, read back with v_readlane); lane
-// j keeps row j of the chunk and the chunk's rows are stored together.
+// Two kernels per batch of pairs (a[i], b[i]):
+//   k_ed_rows: one wave per pair.  The common prefix and suffix are found 64 units
+//     at a time by ballot; lane l then holds the stripped s1[l] (at most 64 units),
+//     so the match mask of a text unit is one ballot and the single-word
+//     recurrence runs on wave-uniform 64-bit values:
+//       Levenshtein (Hyyro 2003, Levenshtein_impl.hpp:259-309): VP / VN after
+//         every text unit;
+//       LCS (Hyyro 2004, LCSseq_impl.hpp:205-250): S after every text unit.
+//     Text units arrive 64 at a time (one per lane, read back with v_readlane);
+//     lane j keeps row j of the chunk and the chunk's rows are stored together.
+//     The pair's rows start where its second string starts in b's offsets (times
+//     two for VP / VN), so no host-side layout is needed.  Per pair it writes the
+//     operation count, the stripped lengths and the prefix length.
+//   k_ed_walk: one lane per pair.  From the bottom-right corner it takes, at
+//     every cell, the operation the reference prefers, so the script is the
+//     reference's:
+//       Levenshtein: a deletion where VP marks the cell, else one row up an
+//         insertion where VN marks it, else a diagonal (a replacement if the
+//         units differ);
+//       LCS: a deletion where S marks the cell (no LCS step), else one row up an
+//         insertion where S is clear, else a diagonal match;
+//     then the rest of either string as deletions, then insertions.  Operations
+//     are written at the pair's offset (the prefix sum of the counts).
+// Levenshtein's script is the reference's when the stripped first string has at
+// most 64 units: the reference then records the full single-word matrix, which
+// depends only on the strings (longer ones use banded or Hirschberg matrices and
+// are flagged here).
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
 
 namespace rfa {
 
 template <typename CharT, bool LEV>
-__global__ __launch_bounds__(256) void k_editops(EditopsArgs a)
+__global__ __launch_bounds__(256) void k_ed_rows(EditopsArgs a)
 {
     const CharT* __restrict__ A = (const CharT*)a.achars;
     const CharT* __restrict__ B = (const CharT*)a.bchars;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave0 = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t nwaves = gridDim.x * 4;
-    for (uint32_t it = wave0; it < a.n; it += nwaves) {
-        const uint32_t la = uniform(a.a_len[it]), lb = uniform(a.b_len[it]);
-        const uint64_t as = a.a_start[it], bs = a.b_start[it], ro = a.row_off[it];
+    const uint64_t wave0 = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * 4;
+    for (uint64_t it = wave0; it < a.n; it += nwaves) {
+        const uint64_t a0 = a.aoff[it], a1 = a.aoff[it + 1], b0 = a.boff[it], b1 = a.boff[it + 1];
+        const uint64_t la0 = a1 - a0, lb0 = b1 - b0;
+        const uint64_t common = la0 < lb0 ? la0 : lb0;
+        // common prefix
+        uint64_t pre = 0;
+        for (;;) {
+            const uint64_t k = pre + lane;
+            const bool eq = k < common && A[a0 + k] == B[b0 + k];
+            const uint64_t m = ballot(eq);
+            if (m == ~0ull) {
+                pre += 64;
+                continue;
+            }
+            pre += (uint64_t)__builtin_ctzll(~m);
+            break;
+        }
+        // common suffix of the rest
+        const uint64_t rest = common - pre;
+        uint64_t suf = 0;
+        for (;;) {
+            const uint64_t k = suf + lane;
+            const bool eq = k < rest && A[a1 - 1 - k] == B[b1 - 1 - k];
+            const uint64_t m = ballot(eq);
+            if (m == ~0ull) {
+                suf += 64;
+                continue;
+            }
+            suf += (uint64_t)__builtin_ctzll(~m);
+            break;
+        }
+        const uint64_t la = la0 - pre - suf, lb = lb0 - pre - suf;
+        EdInfo info;
+        info.pre = pre;
+        info.la = (uint32_t)la;
+        info.lb = (uint32_t)lb;
+        info.flag = la > 64 ? 1u : 0u;
+        info.count = (uint32_t)(la + lb);
+        if (la == 0 || lb == 0 || la > 64) {
+            if (lane == 0) a.info[it] = info;
+            continue;
+        }
+        const uint64_t as = a0 + pre, bs = b0 + pre;
+        const uint64_t ro = (LEV ? 2 : 1) * (b0 - a.b_base);
         const bool in = lane < la;
         const uint32_t my = in ? (uint32_t)A[as + lane] : 0u;
         uint64_t VP = ~0ull, VN = 0, S = ~0ull;
-        uint32_t dist = la;
-        const uint64_t mask = la ? 1ull << (la - 1) : 0;
-        for (uint32_t j0 = 0; j0 < lb; j0 += 64) {
+        uint32_t dist = (uint32_t)la;
+        const uint64_t mask = 1ull << (la - 1);
+        for (uint64_t j0 = 0; j0 < lb; j0 += 64) {
             const uint32_t tb = j0 + lane < lb ? (uint32_t)B[bs + j0 + lane] : 0u;
-            const uint32_t steps = min(64u, lb - j0);
+            const uint32_t steps = (uint32_t)(lb - j0 < 64 ? lb - j0 : 64);
             uint64_t r0 = 0, r1 = 0;
             for (uint32_t s = 0; s < steps; ++s) {
                 const uint32_t c = readlane(tb, s);
@@ -64,40 +121,103 @@ __global__ __launch_bounds__(256) void k_editops(EditopsArgs a)
             }
             if (lane < steps) {
                 if constexpr (LEV) {
-                    a.rows[ro + 2 * (uint64_t)(j0 + lane)] = r0;
-                    a.rows[ro + 2 * (uint64_t)(j0 + lane) + 1] = r1;
+                    a.rows[ro + 2 * (j0 + lane)] = r0;
+                    a.rows[ro + 2 * (j0 + lane) + 1] = r1;
                 }
                 else {
                     a.rows[ro + j0 + lane] = r0;
                 }
             }
         }
-        if (lane == 0) {
-            if constexpr (LEV)
-                a.score[it] = dist;
-            else
-                a.score[it] = (uint32_t)__popcll(~S & (la >= 64 ? ~0ull : ((1ull << la) - 1)));
+        if constexpr (LEV)
+            info.count = dist;
+        else
+            info.count = (uint32_t)(la + lb - 2 * (uint64_t)__popcll(~S & (la >= 64 ? ~0ull : ((1ull << la) - 1))));
+        if (lane == 0) a.info[it] = info;
+    }
+}
+
+template <typename CharT, bool LEV>
+__global__ __launch_bounds__(256) void k_ed_walk(EditopsArgs a)
+{
+    const CharT* __restrict__ A = (const CharT*)a.achars;
+    const CharT* __restrict__ B = (const CharT*)a.bchars;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const EdInfo info = a.info[i];
+    uint64_t dist = info.count;
+    if (dist == 0) return;
+    EdOp* out = a.ops + a.offsets[i];
+    const uint64_t pre = info.pre;
+    uint64_t col = info.la, row = info.lb;
+    auto put = [&](int32_t type) {
+        --dist;
+        EdOp o;
+        o.type = type;
+        o.reserved = 0;
+        o.src_pos = col + pre;
+        o.dest_pos = row + pre;
+        out[dist] = o;
+    };
+    if (info.la && info.lb) {
+        const uint64_t a0 = a.aoff[i] + pre, b0 = a.boff[i] + pre;
+        const uint64_t* __restrict__ r = a.rows + (LEV ? 2 : 1) * (a.boff[i] - a.b_base);
+        while (row && col) {
+            const uint64_t c = col - 1;
+            const bool del = LEV ? ((r[2 * (row - 1)] >> c) & 1) : ((r[row - 1] >> c) & 1);
+            if (del) {
+                --col;
+                put(RFA_EDIT_DELETE);
+                continue;
+            }
+            --row;
+            const bool ins = row && (LEV ? ((r[2 * (row - 1) + 1] >> c) & 1) : !((r[row - 1] >> c) & 1));
+            if (ins) {
+                put(RFA_EDIT_INSERT);
+                continue;
+            }
+            --col;
+            if (LEV && A[a0 + col] != B[b0 + row]) put(RFA_EDIT_REPLACE);
         }
+    }
+    while (col) {
+        --col;
+        put(RFA_EDIT_DELETE);
+    }
+    while (row) {
+        --row;
+        put(RFA_EDIT_INSERT);
     }
 }
 
 template <typename CharT>
-static void launch_editops_c(const EditopsArgs& a, bool lev, uint32_t nblocks, hipStream_t s)
+static void launch_ed_c(const EditopsArgs& a, bool lev, bool walk, uint32_t nblocks, hipStream_t s)
 {
-    if (lev)
-        hipLaunchKernelGGL((k_editops<CharT, true>), dim3(nblocks), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_editops<CharT, false>), dim3(nblocks), dim3(256), 0, s, a);
+    if (walk) {
+        const uint32_t nb = (uint32_t)((a.n + 255) / 256);
+        if (lev)
+            hipLaunchKernelGGL((k_ed_walk<CharT, true>), dim3(nb), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_ed_walk<CharT, false>), dim3(nb), dim3(256), 0, s, a);
+    }
+    else {
+        if (lev)
+            hipLaunchKernelGGL((k_ed_rows<CharT, true>), dim3(nblocks), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_ed_rows<CharT, false>), dim3(nblocks), dim3(256), 0, s, a);
+    }
 }
 
-hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, uint32_t nblocks, hipStream_t s)
+hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, bool walk, uint32_t nblocks,
+                          hipStream_t s)
 {
+    if (a.n == 0) return hipSuccess;
     if (char_width == 1)
-        launch_editops_c<uint8_t>(a, lev, nblocks, s);
+        launch_ed_c<uint8_t>(a, lev, walk, nblocks, s);
     else if (char_width == 2)
-        launch_editops_c<uint16_t>(a, lev, nblocks, s);
+        launch_ed_c<uint16_t>(a, lev, walk, nblocks, s);
     else
-        launch_editops_c<uint32_t>(a, lev, nblocks, s);
+        launch_ed_c<uint32_t>(a, lev, walk, nblocks, s);
     return hipGetLastError();
 }
 

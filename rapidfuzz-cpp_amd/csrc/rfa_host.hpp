@@ -213,7 +213,7 @@ struct rfa_context {
     rfa::DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4], plan_keys, plan_vals,
         plan_temp;
     rfa::DevBuf topk_scores, topk_run[2], topk_cand_s, topk_cand_i, topk_counts, topk_q, topk_out, qcut;
-    rfa::DevBuf ed_items, ed_rows;
+    rfa::DevBuf ed_items, ed_rows, ed_ops;
     size_t lane_arena_bytes_zeroed = 0;
     uint32_t gen_next = 1;
     rfa::Keep keep; // host sources of in-flight uploads

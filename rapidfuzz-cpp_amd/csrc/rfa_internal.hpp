@@ -247,23 +247,36 @@ hipError_t launch_wlong(int char_width, uint32_t cells_per_lane, const WlevLongA
 hipError_t launch_dltile(int char_width, int word_bits, const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x,
                          hipStream_t s);
 
-// Editops bit matrices (rfa_editops.hip): item i pairs the stripped first string
-// A[a_start, a_start + a_len) (a_len <= 64) with B[b_start, b_start + b_len); its
-// rows start at rows[row_off] (Levenshtein: VP, VN per text unit; LCS: S); score
-// receives the distance (Levenshtein) or the LCS length.
+// Editops (rfa_editops.hip).  Pair i is (a[aoff[i], aoff[i+1]), b[boff[i], boff[i+1]));
+// its rows start at rows[(1 or 2) * (boff[i] - b_base)].  k_ed_rows (walk = false)
+// fills info; k_ed_walk (walk = true) writes pair i's operations at ops[offsets[i]].
+struct EdInfo {
+    uint64_t pre;   // common prefix length
+    uint32_t la;    // stripped lengths
+    uint32_t lb;
+    uint32_t count; // operations
+    uint32_t flag;  // 1: the stripped first string exceeds 64 units
+};
+struct EdOp { // the layout of rfa_editop
+    int32_t type;
+    int32_t reserved;
+    uint64_t src_pos;
+    uint64_t dest_pos;
+};
 struct EditopsArgs {
     const void* achars;
     const void* bchars;
-    const uint64_t* a_start;
-    const uint32_t* a_len;
-    const uint64_t* b_start;
-    const uint32_t* b_len;
-    const uint64_t* row_off;
+    const uint64_t* aoff;
+    const uint64_t* boff;
+    uint64_t b_base;
     uint64_t* rows;
-    uint32_t* score;
-    uint32_t n;
+    EdInfo* info;
+    const uint64_t* offsets;
+    EdOp* ops;
+    uint64_t n;
 };
-hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, uint32_t nblocks, hipStream_t s);
+hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, bool walk, uint32_t nblocks,
+                          hipStream_t s);
 
 // Widest shorter string (in 64-unit words) the long path takes.
 constexpr uint32_t PARTIAL_MAX_WORDS = 16;

@@ -600,15 +600,15 @@ private:
 
 namespace detail {
 
-// One pair through rfa_editops (sizes first, then the operations).
+// One pair through rfa_editops.
 inline Editops editops_of(int metric, Packed a, Packed b)
 {
     const int w = std::max(a.width, b.width);
     rfa_context* ctx = context();
     DeviceStrings da(ctx, a, w), db(ctx, b, w);
     uint64_t off[2] = {0, 0};
-    check(rfa_editops(ctx, da.get(), db.get(), metric, off, nullptr, 0));
-    std::vector<rfa_editop> raw(std::max<uint64_t>(1, off[1]));
+    // a script never holds more operations than the two strings have units
+    std::vector<rfa_editop> raw(std::max<uint64_t>(1, a.off[1] + b.off[1]));
     check(rfa_editops(ctx, da.get(), db.get(), metric, off, raw.data(), raw.size()));
     Editops ops;
     ops.reserve(off[1]);

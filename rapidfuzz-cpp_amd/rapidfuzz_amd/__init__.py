@@ -367,9 +367,10 @@ def editops(ctx: Context, a: Strings, b: Strings, metric=LEVENSHTEIN):
     """Edit scripts of the pairs (a[i], b[i]): (offsets [count + 1], ops structured array);
     pair i's operations are ops[offsets[i]:offsets[i + 1]]."""
     off = np.zeros(a.count + 1, dtype=np.uint64)
-    _check(_lib.rfa_editops(ctx.h, a.h, b.h, metric, off.ctypes.data, None, 0))
-    ops = np.zeros(max(1, int(off[-1])), dtype=EDITOP_DTYPE)
-    _check(_lib.rfa_editops(ctx.h, a.h, b.h, metric, off.ctypes.data, ops.ctypes.data, len(ops)))
+    # a script never holds more operations than the two strings have units
+    cap = max(1, int(a.lengths.sum()) + int(b.lengths.sum()))
+    ops = np.empty(cap, dtype=EDITOP_DTYPE)
+    _check(_lib.rfa_editops(ctx.h, a.h, b.h, metric, off.ctypes.data, ops.ctypes.data, cap))
     return off, ops[:int(off[-1])]
 
 
