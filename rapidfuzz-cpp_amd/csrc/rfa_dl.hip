@@ -42,36 +42,53 @@ __device__ __forceinline__ uint32_t dl_wave_max(uint32_t v)
 }
 
 template <uint32_t I>
-__device__ __forceinline__ bool bit_of(uint32_t lo, uint32_t hi)
+__device__ __forceinline__ uint32_t half_of(uint32_t lo, uint32_t hi)
 {
-    return (((I < 32 ? lo : hi) >> (I & 31)) & 1u) != 0;
+    return I < 32 ? lo : hi;
+}
+
+// (m & a) | (~m & b), written as the instruction: the compiler would otherwise turn a
+// 0 / -1 mask back into a compare and a select, whose masks it then spills.
+__device__ __forceinline__ uint32_t bitsel(uint32_t m, uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+    return r;
 }
 
 // Rows I..BITS of column c, written into A over column c - 2 (B holds column c - 1).
-// x: this unit's match bits, p: the previous unit's; li = column c - 2's value of
-// row I - 1 before it was overwritten.
+// (xlo, xhi): this unit's match bits; (ylo, yhi) their complement; (qlo, qhi): the
+// complement of the previous unit's; li = column c - 2's value of row I - 1 before
+// it was overwritten.  Every choice is arithmetic, so no per-row condition masks
+// are kept: the match bit as 0 / -1 (v_bfe_i32), a term that does not apply
+// carries the sentinel bit (v_lshl_or_b32), the saved values update under the
+// match mask (v_bfi_b32).  Real values stay below 2^29, so a term with bit 30 set
+// never wins.
 template <uint32_t BITS, uint32_t I = 1>
 __device__ __forceinline__ void dl_rows(uint32_t (&A)[BITS + 1], const uint32_t (&B)[BITS + 1],
-                                        uint32_t (&G)[BITS + 1], uint32_t xlo, uint32_t xhi, uint32_t plo,
-                                        uint32_t phi, uint32_t c, uint32_t mt, uint32_t li, uint32_t U)
+                                        uint32_t (&G)[BITS + 1], uint32_t xlo, uint32_t xhi, uint32_t ylo,
+                                        uint32_t yhi, uint32_t qlo, uint32_t qhi, uint32_t c, uint32_t mt,
+                                        uint32_t li, uint32_t U)
 {
     if constexpr (I <= BITS) {
         if ((I & 3) == 1 && I > mt) return;
         const uint32_t old = A[I];
-        const bool match = bit_of<I - 1>(xlo, xhi);
-        uint32_t t = min(min(B[I - 1] + (match ? 0u : 1u), A[I - 1] + 1u), B[I] + 1u);
-        uint32_t t1 = DL_BIG, t2 = DL_BIG;
-        if constexpr (I >= 2) {
-            if (bit_of<I - 2>(xlo, xhi)) t1 = G[I] + c; // row I - 1 matched this unit
-        }
-        if (bit_of<I - 1>(plo, phi)) t2 = U + I; // the previous unit matched row I
+        const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)half_of<I - 1>(xlo, xhi), (I - 1) & 31, 1);
+        const uint32_t miss = m + 1u;
+        uint32_t t = min(min(B[I - 1] + miss, A[I - 1] + 1u), B[I] + 1u);
+        // row I - 1 matched this unit: the transposition through G[I]
+        const uint32_t n1 = I >= 2 ? __builtin_amdgcn_ubfe(half_of<(I >= 2 ? I - 2 : 0)>(ylo, yhi), (I - 2) & 31, 1)
+                                   : 1u;
+        const uint32_t t1 = (G[I] + c) | (n1 << 30);
+        // the previous unit matched row I: the transposition through U
+        const uint32_t n2 = __builtin_amdgcn_ubfe(half_of<I - 1>(qlo, qhi), (I - 1) & 31, 1);
+        const uint32_t t2 = (U + I) | (n2 << 30);
         t = min(t, min(t1, t2));
-        if (match) {
-            G[I] = (I >= 2 ? B[I - 2] : DL_BIG) - c;
-            U = li - I;
-        }
+        const uint32_t bm2 = I >= 2 ? B[I >= 2 ? I - 2 : 0] : DL_BIG;
+        G[I] = bitsel(m, bm2 - c, G[I]);
+        U = bitsel(m, li - I, U);
         A[I] = t;
-        dl_rows<BITS, I + 1>(A, B, G, xlo, xhi, plo, phi, c, mt, old, U);
+        dl_rows<BITS, I + 1>(A, B, G, xlo, xhi, ylo, yhi, qlo, qhi, c, mt, old, U);
     }
 }
 
@@ -79,12 +96,13 @@ template <typename W, uint32_t BITS>
 __device__ __forceinline__ void dl_step(uint32_t (&A)[BITS + 1], const uint32_t (&B)[BITS + 1],
                                         uint32_t (&G)[BITS + 1], W X, W P, uint32_t c, uint32_t mt)
 {
-    const uint32_t xlo = (uint32_t)X, plo = (uint32_t)P;
+    const uint32_t xlo = (uint32_t)X;
     const uint32_t xhi = BITS > 32 ? (uint32_t)((uint64_t)X >> 32) : 0u;
-    const uint32_t phi = BITS > 32 ? (uint32_t)((uint64_t)P >> 32) : 0u;
+    const uint32_t qlo = ~(uint32_t)P;
+    const uint32_t qhi = BITS > 32 ? ~(uint32_t)((uint64_t)P >> 32) : ~0u;
     const uint32_t li = A[0];
     A[0] = c;
-    dl_rows<BITS>(A, B, G, xlo, xhi, plo, phi, c, mt, li, DL_BIG);
+    dl_rows<BITS>(A, B, G, xlo, xhi, ~xlo, ~xhi, qlo, qhi, c, mt, li, DL_BIG);
 }
 
 } // namespace
