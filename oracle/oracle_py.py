@@ -213,6 +213,8 @@ class Reference:
         L.ref_editops.restype = ctypes.c_int64
         L.ref_editops.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _i32p, _u64p,
                                   _u64p]
+        L.ref_opcodes.restype = ctypes.c_int64
+        L.ref_opcodes.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _u64p]
         L.ref_editops_u8_mt.restype = ctypes.c_int
         L.ref_editops_u8_mt.argtypes = [ctypes.c_int, _u8p, _u64p, _u8p, _u64p, ctypes.c_size_t, _u64p, ctypes.c_int]
         L.ref_pairs_u32_mt.restype = ctypes.c_int
@@ -289,6 +291,16 @@ class Reference:
         if k < 0:
             raise ValueError(f"reference error {k}: {self._err()}")
         return t[:k], sp[:k], dp[:k]
+
+    def opcodes(self, metric, s1, s2):
+        """(n, 5) array: type, src_begin, src_end, dest_begin, dest_end of the reference's Opcodes."""
+        a = np.ascontiguousarray(to_codes(s1), dtype=np.uint32)
+        b = np.ascontiguousarray(to_codes(s2), dtype=np.uint32)
+        out = np.zeros(5 * (2 * (len(a) + len(b)) + 1), np.uint64)
+        k = self.lib.ref_opcodes(metric, _p(a, _u32p), len(a), _p(b, _u32p), len(b), _p(out, _u64p))
+        if k < 0:
+            raise ValueError(f"reference error {k}: {self._err()}")
+        return out[:5 * k].reshape(k, 5)
 
     def editops_counts_u8_mt(self, metric, achars, aoff, bchars, boff, threads=None):
         """Threaded levenshtein_editops / lcs_seq_editops over 8-bit pairs: operations per pair."""

@@ -665,6 +665,30 @@ int64_t ref_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s
     }
 }
 
+// The reference's Opcodes of its own editops of one pair (details/types.hpp:526-565):
+// five values per opcode (type, src_begin, src_end, dest_begin, dest_end) into out,
+// at most 2 * (l1 + l2) + 1 opcodes; returns their count.
+int64_t ref_opcodes(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, uint64_t* out)
+{
+    try {
+        std::vector<uint32_t> a(s1, s1 + l1), b(s2, s2 + l2);
+        rapidfuzz::Editops ops = metric == 0 ? rapidfuzz::levenshtein_editops(a, b) : rapidfuzz::lcs_seq_editops(a, b);
+        rapidfuzz::Opcodes oc(ops);
+        for (size_t i = 0; i < oc.size(); ++i) {
+            out[5 * i] = static_cast<uint64_t>(oc[i].type);
+            out[5 * i + 1] = oc[i].src_begin;
+            out[5 * i + 2] = oc[i].src_end;
+            out[5 * i + 3] = oc[i].dest_begin;
+            out[5 * i + 4] = oc[i].dest_end;
+        }
+        return static_cast<int64_t>(oc.size());
+    }
+    catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    }
+}
+
 // CPU baseline: editops of pairs (a[i], b[i]) of 8-bit strings over nthreads threads;
 // counts[i] receives pair i's number of operations.
 int ref_editops_u8_mt(int metric, const uint8_t* a, const uint64_t* aoff, const uint8_t* b, const uint64_t* boff,

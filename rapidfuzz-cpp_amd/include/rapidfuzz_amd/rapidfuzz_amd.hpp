@@ -598,6 +598,140 @@ private:
     size_t src_len_ = 0, dest_len_ = 0;
 };
 
+// details/types.hpp:81-100, 357-565: runs of edit operations with their source and
+// destination ranges (EditType::None for the unchanged runs between them)
+struct Opcode {
+    EditType type = EditType::None;
+    size_t src_begin = 0, src_end = 0, dest_begin = 0, dest_end = 0;
+    Opcode() = default;
+    Opcode(EditType t, size_t sb, size_t se, size_t db, size_t de)
+        : type(t), src_begin(sb), src_end(se), dest_begin(db), dest_end(de)
+    {}
+    friend bool operator==(const Opcode& a, const Opcode& b)
+    {
+        return a.type == b.type && a.src_begin == b.src_begin && a.src_end == b.src_end &&
+               a.dest_begin == b.dest_begin && a.dest_end == b.dest_end;
+    }
+    friend bool operator!=(const Opcode& a, const Opcode& b) { return !(a == b); }
+};
+
+class Opcodes : private std::vector<Opcode> {
+    using Base = std::vector<Opcode>;
+
+public:
+    using Base::begin;
+    using Base::empty;
+    using Base::end;
+    using Base::size;
+    using Base::operator[];
+    using Base::push_back;
+    Opcodes() = default;
+    // Editops -> Opcodes: consecutive operations of one type that continue each other
+    // become one run; the stretches between runs (and after the last) are None runs
+    explicit Opcodes(const Editops& ops)
+    {
+        src_len_ = ops.get_src_len();
+        dest_len_ = ops.get_dest_len();
+        size_t sp = 0, dp = 0, i = 0;
+        while (i < ops.size()) {
+            if (sp < ops[i].src_pos || dp < ops[i].dest_pos) {
+                push_back(Opcode(EditType::None, sp, ops[i].src_pos, dp, ops[i].dest_pos));
+                sp = ops[i].src_pos;
+                dp = ops[i].dest_pos;
+            }
+            const size_t sb = sp, db = dp;
+            const EditType t = ops[i].type;
+            for (;;) {
+                sp += (t == EditType::Replace || t == EditType::Delete) ? 1 : 0;
+                dp += (t == EditType::Replace || t == EditType::Insert) ? 1 : 0;
+                ++i;
+                if (i >= ops.size() || ops[i].type != t || ops[i].src_pos != sp || ops[i].dest_pos != dp) break;
+            }
+            push_back(Opcode(t, sb, sp, db, dp));
+        }
+        if (sp < src_len_ || dp < dest_len_) push_back(Opcode(EditType::None, sp, src_len_, dp, dest_len_));
+    }
+    size_t get_src_len() const { return src_len_; }
+    size_t get_dest_len() const { return dest_len_; }
+    void set_src_len(size_t n) { src_len_ = n; }
+    void set_dest_len(size_t n) { dest_len_ = n; }
+    Editops as_editops() const
+    {
+        Editops out;
+        out.set_src_len(src_len_);
+        out.set_dest_len(dest_len_);
+        for (const Opcode& op : *this) {
+            switch (op.type) {
+            case EditType::None: break;
+            case EditType::Replace:
+                for (size_t k = 0; k < op.src_end - op.src_begin; ++k)
+                    out.push_back(EditOp(EditType::Replace, op.src_begin + k, op.dest_begin + k));
+                break;
+            case EditType::Insert:
+                for (size_t k = 0; k < op.dest_end - op.dest_begin; ++k)
+                    out.push_back(EditOp(EditType::Insert, op.src_begin, op.dest_begin + k));
+                break;
+            case EditType::Delete:
+                for (size_t k = 0; k < op.src_end - op.src_begin; ++k)
+                    out.push_back(EditOp(EditType::Delete, op.src_begin + k, op.dest_begin));
+                break;
+            }
+        }
+        return out;
+    }
+    friend bool operator==(const Opcodes& a, const Opcodes& b)
+    {
+        return a.src_len_ == b.src_len_ && a.dest_len_ == b.dest_len_ &&
+               static_cast<const Base&>(a) == static_cast<const Base&>(b);
+    }
+
+private:
+    size_t src_len_ = 0, dest_len_ = 0;
+};
+
+inline Opcodes as_opcodes(const Editops& ops) { return Opcodes(ops); }
+
+// editops_apply / opcodes_apply (details/types.hpp): s1 with the operations applied,
+// taking inserted and replacing units from s2
+template <typename CharT, typename Sentence1, typename Sentence2>
+std::basic_string<CharT> editops_apply_str(const Editops& ops, const Sentence1& s1, const Sentence2& s2)
+{
+    std::basic_string<CharT> out;
+    size_t src = 0;
+    const auto b1 = std::begin(s1);
+    const auto b2 = std::begin(s2);
+    const size_t n1 = static_cast<size_t>(std::distance(std::begin(s1), std::end(s1)));
+    for (const EditOp& op : ops) {
+        for (; src < op.src_pos; ++src) out.push_back(static_cast<CharT>(b1[src]));
+        switch (op.type) {
+        case EditType::None: break;
+        case EditType::Replace:
+            out.push_back(static_cast<CharT>(b2[op.dest_pos]));
+            ++src;
+            break;
+        case EditType::Insert: out.push_back(static_cast<CharT>(b2[op.dest_pos])); break;
+        case EditType::Delete: ++src; break;
+        }
+    }
+    for (; src < n1; ++src) out.push_back(static_cast<CharT>(b1[src]));
+    return out;
+}
+
+template <typename CharT, typename Sentence1, typename Sentence2>
+std::basic_string<CharT> opcodes_apply_str(const Opcodes& ops, const Sentence1& s1, const Sentence2& s2)
+{
+    std::basic_string<CharT> out;
+    const auto b1 = std::begin(s1);
+    const auto b2 = std::begin(s2);
+    for (const Opcode& op : ops) {
+        if (op.type == EditType::None)
+            for (size_t k = op.src_begin; k < op.src_end; ++k) out.push_back(static_cast<CharT>(b1[k]));
+        else if (op.type == EditType::Replace || op.type == EditType::Insert)
+            for (size_t k = op.dest_begin; k < op.dest_end; ++k) out.push_back(static_cast<CharT>(b2[k]));
+    }
+    return out;
+}
+
 namespace detail {
 
 // One pair through rfa_editops.

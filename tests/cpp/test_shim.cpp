@@ -107,6 +107,12 @@ int main()
             CHECK(apply_editops(ind, s, d) == d);
             CHECK(ind.size() == rfa::CachedIndel<char>(s).distance(d));
             CHECK(ind == rfa::lcs_seq_editops(s, d));
+            // Opcodes (details/types.hpp:357-565): round trip and apply
+            rfa::Opcodes oc(ops);
+            CHECK(oc.as_editops() == ops);
+            CHECK(rfa::opcodes_apply_str<char>(oc, s, d) == d);
+            CHECK(rfa::editops_apply_str<char>(ops, s, d) == d);
+            CHECK(oc.get_src_len() == s.size() && oc.get_dest_len() == d.size());
         }
         {
             // tests-DamerauLevenshtein.cpp:106-130
