@@ -67,7 +67,7 @@ typedef enum {
     RFA_PARTIAL_TOKEN_SET_RATIO = 16,  /* fuzz::CachedPartialTokenSetRatio (fuzz.hpp:471-493) */
     RFA_PARTIAL_TOKEN_RATIO = 17,      /* fuzz::CachedPartialTokenRatio (fuzz.hpp:595-620) */
     RFA_DAMERAU_LEVENSHTEIN = 18       /* experimental::CachedDamerauLevenshtein (DamerauLevenshtein.hpp:112-140):
-                                          Cached semantics, every kind; queries of at most 64 code units */
+                                          Cached semantics, every kind; strings below 2^29 code units */
 } rfa_metric;
 
 /* Scorer method (details/distance.hpp:211-301, 326-356). */

@@ -797,15 +797,34 @@ void cdist_wlev(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, vo
 
 // Damerau-Levenshtein: the tile kernel over the scorer's query tiles; queries the
 // tile plan does not take (longer than 64 units, or tiles of too many distinct
-// symbols) are not accelerated.
+// symbols) one lane per pair with the arrays in scratch.
 void cdist_dl(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, void* dout, const uint64_t* d_qcut)
 {
     rfa_context* ctx = sc->ctx;
-    if (!sc->block_queries.empty())
-        throw ApiError(RFA_ERR_UNSUPPORTED, "Damerau-Levenshtein queries longer than 64 code units (or query tiles "
-                                            "of more than 255 distinct symbols) are not accelerated");
-    if (ch->max_len >= (1ull << 29))
-        throw ApiError(RFA_ERR_UNSUPPORTED, "Damerau-Levenshtein choices of 2^29 or more code units");
+    if (ch->max_len >= (1ull << 29) || sc->q->max_len >= (1ull << 29))
+        throw ApiError(RFA_ERR_UNSUPPORTED, "Damerau-Levenshtein strings of 2^29 or more code units");
+    if (!sc->block_queries.empty() && ch->count) {
+        DlLongArgs la{};
+        la.qchars = sc->q->d_chars;
+        la.qoff = sc->q->d_off;
+        la.qlist = sc->d_block_queries;
+        la.nql = (uint32_t)sc->block_queries.size();
+        la.max_rows = (uint32_t)sc->block_max_len;
+        la.cchars = ch->d_chars;
+        la.coff = ch->d_off;
+        la.nc = ch->count;
+        la.out_stride = ch->count;
+        la.spec = spec;
+        la.out = dout;
+        la.qcut = d_qcut;
+        const uint64_t per_block = 3ull * (la.max_rows + 1) * 64 * 4;
+        constexpr uint64_t SCRATCH_TOTAL = 1ull << 30;
+        const uint64_t items = (uint64_t)la.nql * ((ch->count + 63) / 64);
+        const uint64_t nb = std::max<uint64_t>(
+            1, std::min<uint64_t>(std::min<uint64_t>(items, (uint64_t)ctx->num_cus * 8), SCRATCH_TOTAL / per_block));
+        la.scratch = (uint32_t*)ctx->pm_scratch.get(nb * per_block);
+        HIP_CHECK(launch_dllong(sc->q->width, la, (uint32_t)nb, ctx->stream));
+    }
     const uint64_t nc = ch->count;
     TileArgs ta{};
     ta.qchars = sc->q->d_chars;

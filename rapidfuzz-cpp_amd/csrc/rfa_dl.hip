@@ -197,6 +197,112 @@ __global__ __launch_bounds__(TILE_THREADS) void k_dltile(TileArgs a)
     }
 }
 
+// ---- long queries: one lane per (query, choice) pair ---------------------------
+//
+// A wave takes one query and 64 consecutive choices.  The query's units and the row
+// loop are uniform across the wave; each lane streams its own choice.  The three
+// arrays of the tile kernel (two columns and G) live in the wave's slice of global
+// scratch, lane-interleaved ([row][lane], so every access of the wave is one 256-byte
+// segment); the cell is the tile kernel's, with the match tests as compares.
+template <typename CharT, bool INTK>
+__global__ __launch_bounds__(64) void k_dllong(DlLongArgs a)
+{
+    const CharT* __restrict__ Q = (const CharT*)a.qchars;
+    const CharT* __restrict__ C = (const CharT*)a.cchars;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t stride = (uint64_t)(a.max_rows + 1) * 64;
+    uint32_t* __restrict__ S0 = a.scratch + (uint64_t)blockIdx.x * 3 * stride + lane;
+    uint32_t* __restrict__ S1 = S0 + stride;
+    uint32_t* __restrict__ Gs = S1 + stride;
+    const uint64_t nchunks = (a.nc + 63) / 64;
+    const uint64_t items = (uint64_t)a.nql * nchunks;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t qi = a.qlist[it / nchunks];
+        const uint64_t cj = (it % nchunks) * 64 + lane;
+        const bool active = cj < a.nc;
+        const uint64_t qo = a.qoff[qi];
+        const uint32_t m = uniform((uint32_t)(a.qoff[qi + 1] - qo));
+        const uint64_t co = active ? a.coff[cj] : 0;
+        const uint32_t n = active ? (uint32_t)(a.coff[cj + 1] - co) : 0u;
+        uint32_t nmax = n;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor(nmax, off, 64));
+        nmax = uniform(nmax);
+        // S0: column -1 (all sentinel), S1: column 0 (D(0, r) = r)
+        for (uint32_t r = 0; r <= m; ++r) {
+            S0[(uint64_t)r * 64] = DL_BIG;
+            S1[(uint64_t)r * 64] = r;
+            Gs[(uint64_t)r * 64] = DL_BIG;
+        }
+        uint32_t prev_ch = 0;
+        bool have_prev = false;
+        for (uint32_t c = 1; c <= nmax; ++c) {
+            const bool act = c <= n;
+            const uint32_t ch = act ? (uint32_t)C[co + c - 1] : 0u;
+            // odd columns overwrite S0 (column c - 2), even ones S1
+            uint32_t* __restrict__ X = (c & 1) ? S0 : S1;
+            const uint32_t* __restrict__ Y = (c & 1) ? S1 : S0;
+            if (act) {
+                uint32_t li = X[0];
+                X[0] = c;
+                uint32_t U = DL_BIG;
+                uint32_t yprev = Y[0], xprev = c, bm2 = DL_BIG;
+                uint32_t qprev = 0;
+                for (uint32_t r = 1; r <= m; ++r) {
+                    const uint64_t o = (uint64_t)r * 64;
+                    const uint32_t old = X[o], yr = Y[o], g = Gs[o];
+                    const uint32_t qr = (uint32_t)Q[qo + r - 1];
+                    const bool match = qr == ch;
+                    uint32_t t = min(min(yprev + (match ? 0u : 1u), xprev + 1u), yr + 1u);
+                    if (r >= 2 && qprev == ch) t = min(t, g + c);             // row r - 1 matched this unit
+                    if (have_prev && qr == prev_ch) t = min(t, U + r);       // the previous unit matched row r
+                    if (match) {
+                        Gs[o] = bm2 - c;
+                        U = li - r;
+                    }
+                    X[o] = t;
+                    li = old;
+                    bm2 = yprev;
+                    yprev = yr;
+                    xprev = t;
+                    qprev = qr;
+                }
+            }
+            prev_ch = ch;
+            have_prev = true;
+        }
+        if (active) {
+            const uint32_t res = ((n & 1) ? S0 : S1)[(uint64_t)m * 64];
+            ScoreSpec spec = a.spec;
+            if (a.qcut) spec.cutoff_u = a.qcut[qi];
+            RawResult rr{};
+            rr.v = res;
+            const uint64_t bits = INTK ? finalize_int(spec, m, n, res) : finalize_bits(spec, m, n, rr);
+            ((uint64_t*)a.out)[(uint64_t)qi * a.out_stride + cj] = bits;
+        }
+    }
+}
+
+template <typename CharT>
+static void launch_dllong_c(const DlLongArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    if (a.spec.kind == RFA_DISTANCE || a.spec.kind == RFA_SIMILARITY)
+        hipLaunchKernelGGL((k_dllong<CharT, true>), dim3(nblocks), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_dllong<CharT, false>), dim3(nblocks), dim3(64), 0, s, a);
+}
+
+hipError_t launch_dllong(int char_width, const DlLongArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    if (char_width == 1)
+        launch_dllong_c<uint8_t>(a, nblocks, s);
+    else if (char_width == 2)
+        launch_dllong_c<uint16_t>(a, nblocks, s);
+    else
+        launch_dllong_c<uint32_t>(a, nblocks, s);
+    return hipGetLastError();
+}
+
 template <typename CharT, typename W>
 static void launch_dltile_w(const TileArgs& a, uint32_t ntiles, uint32_t nbx, size_t lds, hipStream_t s)
 {

@@ -246,6 +246,24 @@ hipError_t launch_wlong(int char_width, uint32_t cells_per_lane, const WlevLongA
 // scorer's tile plan), integer or double results like k_tile.
 hipError_t launch_dltile(int char_width, int word_bits, const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x,
                          hipStream_t s);
+// Longer queries: one wave per (query, 64 choices), one lane per pair, the arrays in
+// scratch (3 * (max_rows + 1) * 64 words per block; nblocks slices).
+struct DlLongArgs {
+    const void* qchars;
+    const uint64_t* qoff;
+    const uint32_t* qlist;
+    uint32_t nql;
+    uint32_t max_rows; // longest query of qlist
+    const void* cchars;
+    const uint64_t* coff;
+    uint64_t nc;
+    uint64_t out_stride;
+    ScoreSpec spec;
+    void* out;
+    const uint64_t* qcut;
+    uint32_t* scratch;
+};
+hipError_t launch_dllong(int char_width, const DlLongArgs& a, uint32_t nblocks, hipStream_t s);
 
 // Editops (rfa_editops.hip).  Pair i is (a[aoff[i], aoff[i+1]), b[boff[i], boff[i+1]));
 // its rows start at rows[(1 or 2) * (boff[i] - b_base)].  k_ed_rows (walk = false)

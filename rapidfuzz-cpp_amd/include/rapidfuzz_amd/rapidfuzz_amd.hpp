@@ -1198,8 +1198,7 @@ struct MultiJaroWinkler : detail::MultiBase<RFA_JARO_WINKLER, MaxLen, double, 0,
     {}
 };
 
-// experimental::CachedDamerauLevenshtein (DamerauLevenshtein.hpp:112-140): queries of at
-// most 64 code units (longer ones throw, RFA_ERR_UNSUPPORTED)
+// experimental::CachedDamerauLevenshtein (DamerauLevenshtein.hpp:112-140)
 template <typename CharT1>
 struct CachedDamerauLevenshtein
     : rapidfuzz_amd::detail::CachedBase<RFA_DAMERAU_LEVENSHTEIN, size_t, 0, rapidfuzz_amd::detail::kIntMax> {

@@ -106,13 +106,30 @@ def test_topk(ctx, oracle, cutoff):
         assert np.array_equal(gi, exp[0]) and np.array_equal(gs.view(np.uint64), exp[1].view(np.uint64))
 
 
+@pytest.mark.parametrize("alpha", [3, 26])
+def test_long_queries(ctx, oracle, alpha):
+    """Queries over 64 units: one lane per pair with the arrays in scratch."""
+    rng = np.random.default_rng(900 + alpha)
+    queries = [rng.integers(97, 97 + alpha, size=n).astype(np.uint32) for n in (65, 90, 128, 129, 200, 700, 1100)]
+    queries += _gen.random_strings(rng, 5, 0, 64, alpha_n=alpha)  # beside the long ones: the tile path
+    choices = _gen.random_strings(rng, 40, 0, 300, alpha_n=alpha) + [np.zeros(0, np.uint32)]
+    choices += [_swapped(rng, q, 3) for q in queries[:7]]
+    choices += [_gen.mutate(rng, q, 6, alpha_n=alpha) for q in queries[:7]]
+    check(ctx, oracle, queries, choices)
+
+
+def test_tiles_with_many_symbols(ctx, oracle):
+    """Query tiles of more than 255 distinct 32-bit symbols leave the tile plan."""
+    rng = np.random.default_rng(77)
+    queries = _gen.random_strings(rng, 64, 20, 64, alpha_lo=0x10000, alpha_n=5000)
+    choices = [_swapped(rng, q, 3) for q in queries[:20]] + _gen.random_strings(rng, 30, 0, 80, alpha_lo=0x10000,
+                                                                                    alpha_n=5000)
+    check(ctx, oracle, queries, choices, width=4, kinds=(0, 3))
+
+
 def test_limits(ctx):
     import rapidfuzz_amd as rfa
 
-    sc = rfa.Scorer(ctx, DL, ctx.strings([np.full(65, 97, np.uint32)], width=1))
-    with pytest.raises(rfa.RfaError):
-        sc.cdist(ctx.strings([b"abc"]), 0)  # a query longer than 64 units
-    sc.close()
     sc = rfa.Scorer(ctx, DL, ctx.strings([b"abc"]))
     with pytest.raises(rfa.InvalidArgument):
         sc.cdist(ctx.strings([b"abd"]), 0, semantics=rfa.MULTI)
