@@ -34,12 +34,30 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
     ea.aoff = a->d_off;
     ea.boff = b->d_off;
     ea.b_base = b->h_off[0];
-    const uint64_t row_words = (lev ? 2ull : 1ull) * (b->h_off[n] - b->h_off[0]);
+    // rows: Levenshtein two words per text unit (laid out by b's offsets); LCS
+    // ceil(|a[i]| / 64) words per text unit (an upper bound for the stripped strings)
+    uint64_t row_words = 0;
+    std::vector<uint64_t> row_base;
+    if (lev) {
+        row_words = 2ull * (b->h_off[n] - b->h_off[0]);
+    }
+    else {
+        row_base.resize(n);
+        for (uint64_t i = 0; i < n; ++i) {
+            row_base[i] = row_words;
+            const uint64_t la = a->len(i), lb = b->len(i);
+            row_words += std::min<uint64_t>((la + 63) / 64, EDITOPS_LCS_MAX / 64) * lb;
+        }
+    }
     ea.rows = (uint64_t*)ctx->ed_rows.get(std::max<uint64_t>(8, row_words * 8));
-    uint8_t* items = (uint8_t*)ctx->ed_items.get(n * sizeof(EdInfo) + (n + 1) * 8);
+    uint8_t* items = (uint8_t*)ctx->ed_items.get(n * sizeof(EdInfo) + (n + 1) * 8 + (lev ? 0 : n * 8));
     ea.info = (EdInfo*)items;
     ea.offsets = (const uint64_t*)(items + n * sizeof(EdInfo));
     ea.n = n;
+    if (!lev) {
+        ea.row_base = (const uint64_t*)(items + n * sizeof(EdInfo) + (n + 1) * 8);
+        HIP_CHECK(hipMemcpyAsync((void*)ea.row_base, row_base.data(), n * 8, hipMemcpyHostToDevice, s));
+    }
     const uint32_t nblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 3) / 4, (uint64_t)ctx->num_cus * 16));
     HIP_CHECK(launch_editops(a->width, ea, lev, false, nblocks, s));
     std::vector<EdInfo> info(n);
@@ -47,8 +65,11 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
     HIP_CHECK(hipStreamSynchronize(s));
     for (uint64_t i = 0; i < n; ++i) {
         if (info[i].flag)
-            throw ApiError(RFA_ERR_UNSUPPORTED, "editops of a pair whose first string has more than 64 code units "
-                                                "without the common prefix and suffix");
+            throw ApiError(RFA_ERR_UNSUPPORTED,
+                           lev ? "Levenshtein editops of a pair whose first string has more than 64 code units "
+                                 "without the common prefix and suffix"
+                               : "LCS editops of a pair whose first string has more than 4096 code units without "
+                                 "the common prefix and suffix");
         offsets[i + 1] = offsets[i] + info[i].count;
     }
     if (!ops) return;

@@ -27,7 +27,10 @@
 // Levenshtein's script is the reference's when the stripped first string has at
 // most 64 units: the reference then records the full single-word matrix, which
 // depends only on the strings (longer ones use banded or Hirschberg matrices and
-// are flagged here).
+// are flagged here).  LCS records the full matrix at every length; stripped first
+// strings of up to 4096 units run here, over several words (lane l holding
+// s1[64 k + l], so a word's match mask is one ballot; the words' additions carry
+// across lanes by carry-lookahead on two ballots).
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
 
@@ -77,14 +80,59 @@ __global__ __launch_bounds__(256) void k_ed_rows(EditopsArgs a)
         info.pre = pre;
         info.la = (uint32_t)la;
         info.lb = (uint32_t)lb;
-        info.flag = la > 64 ? 1u : 0u;
+        info.flag = la > (LEV ? 64u : EDITOPS_LCS_MAX) ? 1u : 0u;
         info.count = (uint32_t)(la + lb);
-        if (la == 0 || lb == 0 || la > 64) {
+        if (la == 0 || lb == 0 || info.flag) {
             if (lane == 0) a.info[it] = info;
             continue;
         }
         const uint64_t as = a0 + pre, bs = b0 + pre;
-        const uint64_t ro = (LEV ? 2 : 1) * (b0 - a.b_base);
+        const uint64_t ro = LEV ? 2 * (b0 - a.b_base) : a.row_base[it];
+        if constexpr (!LEV) {
+            if (la > 64) {
+                // several words: lane l holds s1[64 k + l], word k's match mask is one ballot
+                // and goes to lane k; the additions carry across lanes by carry-lookahead
+                const uint32_t W = (uint32_t)((la + 63) / 64);
+                uint32_t su[EDITOPS_LCS_MAX / 64];
+#pragma unroll
+                for (uint32_t k = 0; k < EDITOPS_LCS_MAX / 64; ++k)
+                    su[k] = (k < W && k * 64 + lane < la) ? (uint32_t)A[as + k * 64 + lane] : 0u;
+                const bool act = lane < W;
+                uint64_t Sw = ~0ull;
+                for (uint64_t j0 = 0; j0 < lb; j0 += 64) {
+                    const uint32_t tb = j0 + lane < lb ? (uint32_t)B[bs + j0 + lane] : 0u;
+                    const uint32_t steps = (uint32_t)(lb - j0 < 64 ? lb - j0 : 64);
+                    for (uint32_t s = 0; s < steps; ++s) {
+                        const uint32_t c = readlane(tb, s);
+                        uint64_t X = 0;
+#pragma unroll
+                        for (uint32_t k = 0; k < EDITOPS_LCS_MAX / 64; ++k) {
+                            if (k < W) {
+                                const uint64_t mk = ballot(k * 64 + lane < la && su[k] == c);
+                                X = lane == k ? mk : X;
+                            }
+                        }
+                        const uint64_t u = Sw & X;
+                        uint64_t x = Sw + u;
+                        const bool g = act && (x < Sw);
+                        const bool p = act && !g && (x == ~0ull);
+                        const uint64_t G = ballot(g), Pp = ballot(p);
+                        const uint64_t sum = (G | Pp) + G;
+                        x += ((sum ^ Pp) >> lane) & 1u;
+                        if (act) {
+                            Sw = x | (Sw - u);
+                            a.rows[ro + (j0 + s) * W + lane] = Sw;
+                        }
+                    }
+                }
+                const uint64_t valid = la - 64ull * lane >= 64 ? ~0ull : ((1ull << (la - 64ull * lane)) - 1);
+                uint32_t cnt = act ? (uint32_t)__popcll(~Sw & valid) : 0u;
+                for (int d = 32; d >= 1; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
+                info.count = (uint32_t)(la + lb - 2ull * cnt);
+                if (lane == 0) a.info[it] = info;
+                continue;
+            }
+        }
         const bool in = lane < la;
         const uint32_t my = in ? (uint32_t)A[as + lane] : 0u;
         uint64_t VP = ~0ull, VN = 0, S = ~0ull;
@@ -161,17 +209,19 @@ __global__ __launch_bounds__(256) void k_ed_walk(EditopsArgs a)
     };
     if (info.la && info.lb) {
         const uint64_t a0 = a.aoff[i] + pre, b0 = a.boff[i] + pre;
-        const uint64_t* __restrict__ r = a.rows + (LEV ? 2 : 1) * (a.boff[i] - a.b_base);
+        const uint64_t* __restrict__ r = a.rows + (LEV ? 2 * (a.boff[i] - a.b_base) : a.row_base[i]);
+        const uint64_t W = (info.la + 63) / 64; // LCS row words
         while (row && col) {
             const uint64_t c = col - 1;
-            const bool del = LEV ? ((r[2 * (row - 1)] >> c) & 1) : ((r[row - 1] >> c) & 1);
+            const bool del = LEV ? ((r[2 * (row - 1)] >> c) & 1) : ((r[(row - 1) * W + c / 64] >> (c % 64)) & 1);
             if (del) {
                 --col;
                 put(RFA_EDIT_DELETE);
                 continue;
             }
             --row;
-            const bool ins = row && (LEV ? ((r[2 * (row - 1) + 1] >> c) & 1) : !((r[row - 1] >> c) & 1));
+            const bool ins =
+                row && (LEV ? ((r[2 * (row - 1) + 1] >> c) & 1) : !((r[(row - 1) * W + c / 64] >> (c % 64)) & 1));
             if (ins) {
                 put(RFA_EDIT_INSERT);
                 continue;

@@ -266,7 +266,7 @@ struct DlLongArgs {
 hipError_t launch_dllong(int char_width, const DlLongArgs& a, uint32_t nblocks, hipStream_t s);
 
 // Editops (rfa_editops.hip).  Pair i is (a[aoff[i], aoff[i+1]), b[boff[i], boff[i+1]));
-// its rows start at rows[(1 or 2) * (boff[i] - b_base)].  k_ed_rows (walk = false)
+// its rows start at rows[2 * (boff[i] - b_base)] (Levenshtein) or rows[row_base[i]] (LCS).  k_ed_rows (walk = false)
 // fills info; k_ed_walk (walk = true) writes pair i's operations at ops[offsets[i]].
 struct EdInfo {
     uint64_t pre;   // common prefix length
@@ -292,7 +292,9 @@ struct EditopsArgs {
     const uint64_t* offsets;
     EdOp* ops;
     uint64_t n;
+    const uint64_t* row_base; // LCS: first row word of each pair (rows of ceil(la / 64) words)
 };
+constexpr uint32_t EDITOPS_LCS_MAX = 4096; // stripped first string, LCS editops
 hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, bool walk, uint32_t nblocks,
                           hipStream_t s);
 

@@ -782,7 +782,8 @@ Editops levenshtein_editops(const Sentence1& s1, const Sentence2& s2,
                                score_hint);
 }
 
-// lcs_seq_editops (LCSseq.hpp:65-75) and indel_editops (Indel.hpp:65-75, the same script)
+// lcs_seq_editops (LCSseq.hpp:65-75) and indel_editops (Indel.hpp:65-75, the same script):
+// pairs whose first string has at most 4096 code units once the common affixes are removed
 template <typename InputIt1, typename InputIt2>
 Editops lcs_seq_editops(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2)
 {

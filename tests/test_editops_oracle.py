@@ -3,7 +3,8 @@
 
 * against the compiled reference, operation for operation: Levenshtein pairs whose
   first string has at most 64 units after the common affix is removed (where the
-  reference walks the full single-word matrix), LCS pairs of any length;
+  reference walks the full single-word matrix), LCS pairs up to 4096 units (its
+  single-word, unrolled and blockwise matrices);
 * by what any correct script satisfies: applied to s1 it gives s2 (the reference's
   own editops tests, tests-Levenshtein.cpp:235-316 and tests-Indel.cpp:275-285, are
   of this form and their strings are used here), its length is the distance, and
@@ -94,6 +95,11 @@ def test_lcs_vs_reference(oracle, reference):
             s2 = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 200))).astype(np.uint32)
             pairs.append((s1, s2))
             pairs.append((s1, _gen.mutate(rng, s1, int(rng.integers(0, 8)), alpha_n=alpha)))
+    # several words, through the reference's unrolled (<= 8 words) and blockwise LCS paths
+    for n1 in (65, 300, 520, 1500, 3000, 4096):
+        s1 = rng.integers(97, 101, size=n1).astype(np.uint32)
+        pairs.append((s1, _gen.mutate(rng, s1, int(rng.integers(1, 40)))))
+        pairs.append((s1, rng.integers(97, 101, size=int(rng.integers(0, 300))).astype(np.uint32)))
     # tests-Indel.cpp:275-285
     pairs.append((_codes("bbbdbbmbbbbbbbbbBbfbbbbbbbbbbbbbbbbbbbrbbbbbrbbbbbdbnbbbjbhbbbbbbbbbhbbbbbCbobb"
                          "bxbbbbbkbbbAbxbbwbbbtbcbbbbebbiblbbbbqbbbbbbpbbbbbbubbbkbbDbbbhbkbCbbgbbrbbbbbb"

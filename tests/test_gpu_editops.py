@@ -1,6 +1,7 @@
 """GPU parity of rfa_editops (levenshtein_editops, lcs_seq_editops / indel_editops)
 against the oracle, operation for operation: random pairs around the 64-unit limit
-of the stripped first string, long second strings (several 64-unit chunks of rows),
+of the stripped first string (Levenshtein; LCS up to 4096 units over several words),
+long second strings (several 64-unit chunks of rows),
 common affixes, empty strings, 8/16/32-bit units, and the reference's editops test
 strings; every script also rebuilds the second string."""
 import numpy as np
@@ -63,6 +64,21 @@ def test_wide_units(ctx, oracle, width, metric):
     check(ctx, oracle, metric, pairs, width=width)
 
 
+@pytest.mark.parametrize("metric", [INDEL, LCS])
+def test_lcs_long_first_strings(ctx, oracle, metric):
+    """LCS editops over several words: stripped first strings of 65-4096 units."""
+    rng = np.random.default_rng(50 + metric)
+    pairs = []
+    for n1 in (65, 100, 128, 129, 300, 700, 1500, 4096):
+        s1 = rng.integers(97, 101, size=n1).astype(np.uint32)
+        pairs.append((s1, _gen.mutate(rng, s1, int(rng.integers(1, 30)))))
+        pairs.append((s1, rng.integers(97, 101, size=int(rng.integers(0, 400))).astype(np.uint32)))
+    pairs += [(rng.integers(97, 99, size=int(rng.integers(65, 200))).astype(np.uint32),
+               rng.integers(97, 99, size=int(rng.integers(0, 200))).astype(np.uint32)) for _ in range(30)]
+    pairs += _pairs(rng, 3, n=20)
+    check(ctx, oracle, metric, pairs)
+
+
 def test_sizes_then_ops_and_limits(ctx, oracle):
     import rapidfuzz_amd as rfa
 
@@ -77,6 +93,10 @@ def test_sizes_then_ops_and_limits(ctx, oracle):
     long_a = ctx.strings([np.full(70, 97, np.uint32)], width=1)
     long_b = ctx.strings([np.full(70, 98, np.uint32)], width=1)
     with pytest.raises(rfa.RfaError):
-        rfa.editops(ctx, long_a, long_b, 0)
+        rfa.editops(ctx, long_a, long_b, 0)  # Levenshtein: more than 64 units after the affixes
+    huge_a = ctx.strings([np.full(4097, 97, np.uint32)], width=1)
+    huge_b = ctx.strings([np.full(10, 98, np.uint32)], width=1)
+    with pytest.raises(rfa.RfaError):
+        rfa.editops(ctx, huge_a, huge_b, 2)  # LCS: more than 4096 units after the affixes
     with pytest.raises(rfa.InvalidArgument):
         rfa.editops(ctx, a, b, 3)  # OSA has no editops
