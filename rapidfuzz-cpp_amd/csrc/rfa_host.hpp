@@ -345,6 +345,8 @@ rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries
 void free_scorer(rfa_scorer* sc);
 void validate_kind(int metric, int kind, int semantics);
 // qcut_host: optional per-query integer cutoffs (host memory, one per query)
+void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics, double cd, void* out,
+                int out_on_device);
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
                 int out_on_device, const uint64_t* qcut_host = nullptr);
 

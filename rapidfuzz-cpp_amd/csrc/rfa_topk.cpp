@@ -463,6 +463,41 @@ static void topk_dense(rfa_scorer* sc, const rfa_strings* ch, int kind, double c
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Top-k of the token-set ratios: their choice token tables are cached per string set,
+// so the whole set is scored at once (no chunk views), then selected.
+static void topk_tset(rfa_scorer* sc, const rfa_strings* ch, int kind, double cutoff, uint32_t k, int64_t* out_index,
+                      double* out_score)
+{
+    rfa_context* ctx = sc->ctx;
+    const uint64_t nq = sc->q->count, nc = ch->count;
+    if (!nq || !k) return;
+    if (nc == 0) {
+        std::fill(out_index, out_index + nq * k, (int64_t)-1);
+        std::fill(out_score, out_score + nq * k, 0.0);
+        return;
+    }
+    constexpr uint64_t DENSE_MAX = 16ull << 30;
+    if (nq * nc * 8 > DENSE_MAX)
+        throw ApiError(RFA_ERR_UNSUPPORTED, "top-k of a token-set ratio over more than 2^31 query-choice pairs");
+    hipStream_t s = ctx->stream;
+    double* dense = (double*)ctx->topk_scores.get((size_t)nq * nc * 8);
+    cdist_tset(sc, ch, kind, RFA_CACHED, cutoff, dense, 1);
+    uint8_t* base = (uint8_t*)ctx->topk_run[0].get((size_t)nq * k * 16);
+    SelectArgs sa{};
+    sa.dscore = dense;
+    sa.dstride = nc;
+    sa.dn = nc;
+    sa.dbase = 0;
+    sa.cutoff = cutoff;
+    sa.k = k;
+    sa.out_score = (double*)base;
+    sa.out_index = (int64_t*)(base + (size_t)nq * k * 8);
+    HIP_CHECK(launch_select(sa, (uint32_t)nq, s));
+    HIP_CHECK(hipMemcpyAsync(out_score, sa.out_score, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(out_index, sa.out_index, (size_t)nq * k * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
 // Top-k for a subset of a scorer's queries through the dense path.
 static void topk_dense_subset(rfa_context* ctx, int metric, const rfa_weights& w, double pw,
                               const std::vector<std::pair<const uint8_t*, uint32_t>>& qs, int width,
@@ -490,16 +525,21 @@ static void topk_dense_subset(rfa_context* ctx, int metric, const rfa_weights& w
     free_strings(tmp);
 }
 
+static bool is_token_set(int metric)
+{
+    return metric == RFA_TOKEN_SET_RATIO || metric == RFA_TOKEN_RATIO || metric == RFA_PARTIAL_TOKEN_SET_RATIO ||
+           metric == RFA_PARTIAL_TOKEN_RATIO;
+}
+
 static void validate_topk(int metric, int kind)
 {
-    if (metric == RFA_TOKEN_SET_RATIO || metric == RFA_TOKEN_RATIO || metric == RFA_PARTIAL_TOKEN_SET_RATIO ||
-        metric == RFA_PARTIAL_TOKEN_RATIO)
-        throw ApiError(RFA_ERR_UNSUPPORTED, "top-k over the token-set ratios is not accelerated (use rfa_cdist)");
     validate_kind(metric, kind, RFA_CACHED);
+    if (is_token_set(metric) && kind != RFA_SIMILARITY)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "token ratios only provide similarity");
     const bool sim_double = kind == RFA_NORMALIZED_SIMILARITY ||
                             (kind == RFA_SIMILARITY &&
                              (metric == RFA_JARO || metric == RFA_JARO_WINKLER || metric == RFA_RATIO ||
-                              metric == RFA_PARTIAL_RATIO));
+                              metric == RFA_PARTIAL_RATIO || is_token_set(metric)));
     if (!sim_double)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT,
                        "top-k ranks double similarities: use a normalized similarity, or the similarity of "
@@ -513,6 +553,10 @@ void topk_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, double cutoff, u
     validate_topk(sc->metric, kind);
     if (ch->width != sc->q->width)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
+    if (sc->token_set) {
+        topk_tset(sc, ch, kind, cutoff, k, out_index, out_score);
+        return;
+    }
     ch = scorer_choices(sc, ch);
     const uint64_t nq = sc->q->count, nc = ch->count;
     if (!nq || !k) return;

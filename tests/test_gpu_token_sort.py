@@ -95,7 +95,7 @@ def test_cached_choices_and_option_switch(ctx, oracle):
     assert np.array_equal(got.view(np.uint64), exp.view(np.uint64))
 
 
-@pytest.mark.parametrize("metric", [TSR, QR])
+@pytest.mark.parametrize("metric", [TSR, QR, TSET, TRATIO, PTSET, PTRATIO])
 def test_topk(ctx, oracle, metric):
     import rapidfuzz_amd as rfa
 
@@ -141,8 +141,8 @@ def test_token_set_limits(ctx):
     with pytest.raises(rfa.RfaError):
         long_q.cdist(ctx.strings([b"beta"]), SIM)  # deduplicated join over 64 units
     sc = rfa.Scorer(ctx, TRATIO, ctx.strings([b"a b"]))
-    with pytest.raises(rfa.RfaError):
-        sc.topk(ctx.strings([b"b a"]), 1)
+    with pytest.raises(rfa.InvalidArgument):
+        sc.topk(ctx.strings([b"b a"]), 1, kind=3)  # similarity only
 
 
 def test_rejected(ctx):
