@@ -213,14 +213,16 @@ rfa_status rfa_topk(rfa_scorer* scorer, const rfa_strings* choices, int kind, do
  * rfa_editops: the edit script of every pair (a[i], b[i]), as
  * levenshtein_editops(a[i], b[i]) (metric RFA_LEVENSHTEIN; Levenshtein.hpp:282-295,
  * Levenshtein_impl.hpp:1229-1257) or lcs_seq_editops / indel_editops (RFA_LCSSEQ or
- * RFA_INDEL; LCSseq.hpp:65-75, Indel.hpp:65-75).  Operations have the reference's
+ * RFA_INDEL; LCSseq.hpp:65-75, Indel.hpp:65-75) or hamming_editops with pad (RFA_HAMMING;
+ * Hamming.hpp:77-90).  Operations have the reference's
  * EditOp layout (type, src_pos, dest_pos; details/types.hpp:28-60); pair i's run
  * is ops[offsets[i], offsets[i + 1]), its src_len / dest_len are the string lengths.
  * offsets (count + 1 entries) is always written.  ops may be NULL to ask for the
  * sizes only; otherwise capacity (in operations) must be at least offsets[count]
  * (else RFA_ERR_INVALID_ARGUMENT, with offsets written).  A pair whose first string
  * has more than 64 (Levenshtein) or 4096 (LCSseq / Indel) code units once the
- * common prefix and suffix are removed returns RFA_ERR_UNSUPPORTED. */
+ * common prefix and suffix are removed returns RFA_ERR_UNSUPPORTED (Hamming has no
+ * limit). */
 #define RFA_EDIT_NONE 0
 #define RFA_EDIT_REPLACE 1
 #define RFA_EDIT_INSERT 2

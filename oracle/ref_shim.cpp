@@ -651,7 +651,9 @@ int64_t ref_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s
 {
     try {
         std::vector<uint32_t> a(s1, s1 + l1), b(s2, s2 + l2);
-        rapidfuzz::Editops ops = metric == 0 ? rapidfuzz::levenshtein_editops(a, b) : rapidfuzz::lcs_seq_editops(a, b);
+        rapidfuzz::Editops ops = metric == 0   ? rapidfuzz::levenshtein_editops(a, b)
+                                 : metric == 7 ? rapidfuzz::hamming_editops(a, b)
+                                               : rapidfuzz::lcs_seq_editops(a, b);
         for (size_t i = 0; i < ops.size(); ++i) {
             type[i] = static_cast<int32_t>(ops[i].type);
             src_pos[i] = ops[i].src_pos;

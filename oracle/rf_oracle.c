@@ -413,6 +413,17 @@ enum { ED_REPLACE = 1, ED_INSERT = 2, ED_DELETE = 3 };
 size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, int32_t* type,
                    uint64_t* src_pos, uint64_t* dest_pos)
 {
+    if (metric == ORC_HAMMING) {
+        /* hamming_editops with pad (Hamming_impl.hpp:40-59): a replacement at every
+         * differing position of the common length, then the longer string's tail */
+        size_t k = 0, i = 0;
+        const size_t mn = l1 < l2 ? l1 : l2;
+        for (; i < mn; ++i)
+            if (s1[i] != s2[i]) type[k] = ED_REPLACE, src_pos[k] = i, dest_pos[k] = i, ++k;
+        for (; i < l1; ++i) type[k] = ED_DELETE, src_pos[k] = i, dest_pos[k] = l2, ++k;
+        for (; i < l2; ++i) type[k] = ED_INSERT, src_pos[k] = l1, dest_pos[k] = i, ++k;
+        return k;
+    }
     size_t pre = 0;
     while (pre < l1 && pre < l2 && s1[pre] == s2[pre]) ++pre;
     size_t suf = 0;

@@ -57,7 +57,8 @@ uint64_t orc_lcs_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l
 uint64_t orc_osa_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 uint64_t orc_damerau_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 /* Editops of levenshtein_editops (metric 0) / lcs_seq_editops = indel_editops (metric 1
- * or 2) for one pair; writes at most l1 + l2 operations, returns their count. */
+ * or 2) / hamming_editops with pad (metric 7) for one pair; writes at most l1 + l2
+ * operations, returns their count. */
 size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, int32_t* type,
                    uint64_t* src_pos, uint64_t* dest_pos);
 /* positional metrics: Hamming distance with padding, common prefix / suffix length */

@@ -19,14 +19,15 @@ static_assert(sizeof(EdOp) == sizeof(rfa_editop), "EdOp mirrors rfa_editop");
 void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, int metric, uint64_t* offsets,
                   rfa_editop* ops, uint64_t capacity)
 {
-    if (metric != RFA_LEVENSHTEIN && metric != RFA_INDEL && metric != RFA_LCSSEQ)
-        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "editops are for Levenshtein, Indel and LCSseq");
+    if (metric != RFA_LEVENSHTEIN && metric != RFA_INDEL && metric != RFA_LCSSEQ && metric != RFA_HAMMING)
+        throw ApiError(RFA_ERR_INVALID_ARGUMENT, "editops are for Levenshtein, Indel, LCSseq and Hamming");
     if (a->count != b->count) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "a and b must hold the same number of strings");
     if (a->width != b->width) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "a and b must have the same char width");
     const uint64_t n = a->count;
     offsets[0] = 0;
     if (n == 0) return;
     const bool lev = metric == RFA_LEVENSHTEIN;
+    const bool ham = metric == RFA_HAMMING;
     hipStream_t s = ctx->stream;
     EditopsArgs ea{};
     ea.achars = a->d_chars;
@@ -38,7 +39,10 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
     // ceil(|a[i]| / 64) words per text unit (an upper bound for the stripped strings)
     uint64_t row_words = 0;
     std::vector<uint64_t> row_base;
-    if (lev) {
+    if (ham) {
+        // no rows
+    }
+    else if (lev) {
         row_words = 2ull * (b->h_off[n] - b->h_off[0]);
     }
     else {
@@ -50,16 +54,19 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
         }
     }
     ea.rows = (uint64_t*)ctx->ed_rows.get(std::max<uint64_t>(8, row_words * 8));
-    uint8_t* items = (uint8_t*)ctx->ed_items.get(n * sizeof(EdInfo) + (n + 1) * 8 + (lev ? 0 : n * 8));
+    uint8_t* items = (uint8_t*)ctx->ed_items.get(n * sizeof(EdInfo) + (n + 1) * 8 + (lev || ham ? 0 : n * 8));
     ea.info = (EdInfo*)items;
     ea.offsets = (const uint64_t*)(items + n * sizeof(EdInfo));
     ea.n = n;
-    if (!lev) {
+    if (!lev && !ham) {
         ea.row_base = (const uint64_t*)(items + n * sizeof(EdInfo) + (n + 1) * 8);
         HIP_CHECK(hipMemcpyAsync((void*)ea.row_base, row_base.data(), n * 8, hipMemcpyHostToDevice, s));
     }
     const uint32_t nblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 3) / 4, (uint64_t)ctx->num_cus * 16));
-    HIP_CHECK(launch_editops(a->width, ea, lev, false, nblocks, s));
+    if (ham)
+        HIP_CHECK(launch_editops_hamming(a->width, ea, false, nblocks, s));
+    else
+        HIP_CHECK(launch_editops(a->width, ea, lev, false, nblocks, s));
     std::vector<EdInfo> info(n);
     HIP_CHECK(hipMemcpyAsync(info.data(), ea.info, n * sizeof(EdInfo), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -80,7 +87,10 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
     if (total == 0) return;
     ea.ops = (EdOp*)ctx->ed_ops.get(total * sizeof(EdOp));
     HIP_CHECK(hipMemcpyAsync((void*)ea.offsets, offsets, (n + 1) * 8, hipMemcpyHostToDevice, s));
-    HIP_CHECK(launch_editops(a->width, ea, lev, true, 0, s));
+    if (ham)
+        HIP_CHECK(launch_editops_hamming(a->width, ea, true, nblocks, s));
+    else
+        HIP_CHECK(launch_editops(a->width, ea, lev, true, 0, s));
     HIP_CHECK(hipMemcpyAsync(ops, ea.ops, total * sizeof(EdOp), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
 }

@@ -240,6 +240,92 @@ __global__ __launch_bounds__(256) void k_ed_walk(EditopsArgs a)
     }
 }
 
+// hamming_editops with pad (Hamming_impl.hpp:40-59): a replacement at every differing
+// position of the common length, then deletions of s1's tail or insertions of s2's.
+// One wave per pair; the count is a ballot popcount per 64 positions, and the write
+// pass ranks each differing position among the wave's by mbcnt, so the operations are
+// written in parallel and in order.
+template <typename CharT, bool WRITE>
+__global__ __launch_bounds__(256) void k_ed_hamming(EditopsArgs a)
+{
+    const CharT* __restrict__ A = (const CharT*)a.achars;
+    const CharT* __restrict__ B = (const CharT*)a.bchars;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave0 = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * 4;
+    for (uint64_t it = wave0; it < a.n; it += nwaves) {
+        const uint64_t a0 = a.aoff[it], b0 = a.boff[it];
+        const uint64_t la = a.aoff[it + 1] - a0, lb = a.boff[it + 1] - b0;
+        const uint64_t mn = la < lb ? la : lb;
+        EdOp* out = WRITE ? a.ops + a.offsets[it] : nullptr;
+        uint64_t k = 0;
+        for (uint64_t i0 = 0; i0 < mn; i0 += 64) {
+            const uint64_t i = i0 + lane;
+            const bool diff = i < mn && A[a0 + i] != B[b0 + i];
+            const uint64_t m = ballot(diff);
+            if constexpr (WRITE) {
+                if (diff) {
+                    EdOp o;
+                    o.type = RFA_EDIT_REPLACE;
+                    o.reserved = 0;
+                    o.src_pos = i;
+                    o.dest_pos = i;
+                    out[k + __popcll(m & ((1ull << lane) - 1))] = o;
+                }
+            }
+            k += (uint64_t)__popcll(m);
+        }
+        if constexpr (WRITE) {
+            for (uint64_t i = mn + lane; i < la; i += 64) {
+                EdOp o;
+                o.type = RFA_EDIT_DELETE;
+                o.reserved = 0;
+                o.src_pos = i;
+                o.dest_pos = lb;
+                out[k + (i - mn)] = o;
+            }
+            for (uint64_t i = mn + lane; i < lb; i += 64) {
+                EdOp o;
+                o.type = RFA_EDIT_INSERT;
+                o.reserved = 0;
+                o.src_pos = la;
+                o.dest_pos = i;
+                out[k + (i - mn)] = o;
+            }
+        }
+        else if (lane == 0) {
+            EdInfo info;
+            info.pre = 0;
+            info.la = (uint32_t)la;
+            info.lb = (uint32_t)lb;
+            info.flag = 0;
+            info.count = (uint32_t)(k + (la - mn) + (lb - mn));
+            a.info[it] = info;
+        }
+    }
+}
+
+template <typename CharT>
+static void launch_ed_hamming_c(const EditopsArgs& a, bool walk, uint32_t nblocks, hipStream_t s)
+{
+    if (walk)
+        hipLaunchKernelGGL((k_ed_hamming<CharT, true>), dim3(nblocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_ed_hamming<CharT, false>), dim3(nblocks), dim3(256), 0, s, a);
+}
+
+hipError_t launch_editops_hamming(int char_width, const EditopsArgs& a, bool walk, uint32_t nblocks, hipStream_t s)
+{
+    if (a.n == 0) return hipSuccess;
+    if (char_width == 1)
+        launch_ed_hamming_c<uint8_t>(a, walk, nblocks, s);
+    else if (char_width == 2)
+        launch_ed_hamming_c<uint16_t>(a, walk, nblocks, s);
+    else
+        launch_ed_hamming_c<uint32_t>(a, walk, nblocks, s);
+    return hipGetLastError();
+}
+
 template <typename CharT>
 static void launch_ed_c(const EditopsArgs& a, bool lev, bool walk, uint32_t nblocks, hipStream_t s)
 {

@@ -297,6 +297,7 @@ struct EditopsArgs {
 constexpr uint32_t EDITOPS_LCS_MAX = 4096; // stripped first string, LCS editops
 hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, bool walk, uint32_t nblocks,
                           hipStream_t s);
+hipError_t launch_editops_hamming(int char_width, const EditopsArgs& a, bool walk, uint32_t nblocks, hipStream_t s);
 
 // Widest shorter string (in 64-unit words) the long path takes.
 constexpr uint32_t PARTIAL_MAX_WORDS = 16;

@@ -782,6 +782,25 @@ Editops levenshtein_editops(const Sentence1& s1, const Sentence2& s2,
                                score_hint);
 }
 
+// hamming_editops (Hamming.hpp:77-90): with pad = false, strings of different lengths
+// throw std::invalid_argument; score_hint does not change the result
+template <typename InputIt1, typename InputIt2>
+Editops hamming_editops(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2, bool pad_ = true,
+                        size_t score_hint = std::numeric_limits<size_t>::max())
+{
+    (void)score_hint;
+    if (!pad_ && std::distance(first1, last1) != std::distance(first2, last2))
+        throw std::invalid_argument("Sequences are not the same length.");
+    return detail::editops_of(RFA_HAMMING, first1, last1, first2, last2);
+}
+template <typename Sentence1, typename Sentence2>
+Editops hamming_editops(const Sentence1& s1, const Sentence2& s2, bool pad_ = true,
+                        size_t score_hint = std::numeric_limits<size_t>::max())
+{
+    return hamming_editops(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2), detail::to_end(s2), pad_,
+                           score_hint);
+}
+
 // lcs_seq_editops (LCSseq.hpp:65-75) and indel_editops (Indel.hpp:65-75, the same script):
 // pairs whose first string has at most 4096 code units once the common affixes are removed
 template <typename InputIt1, typename InputIt2>

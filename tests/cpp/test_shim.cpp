@@ -113,6 +113,13 @@ int main()
             CHECK(rfa::opcodes_apply_str<char>(oc, s, d) == d);
             CHECK(rfa::editops_apply_str<char>(ops, s, d) == d);
             CHECK(oc.get_src_len() == s.size() && oc.get_dest_len() == d.size());
+            // tests-Hamming.cpp:112-127 (pad = true)
+            rfa::Editops hops = rfa::hamming_editops(s, d);
+            CHECK(rfa::editops_apply_str<char>(hops, s, d) == d);
+            CHECK(hops.get_src_len() == s.size() && hops.get_dest_len() == d.size());
+            CHECK(rfa::editops_apply_str<char>(rfa::hamming_editops(d, s), d, s) == s);
+            CHECK(hops.size() == rfa::CachedHamming<char>(s).distance(d));
+            CHECK(throws_invalid([&] { rfa::hamming_editops(s, d, false); }));
         }
         {
             // tests-DamerauLevenshtein.cpp:106-130

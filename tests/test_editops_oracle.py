@@ -1,5 +1,6 @@
 """The oracle's editops (levenshtein_editops, Levenshtein_impl.hpp:959-1015 and
-1229-1257; lcs_seq_editops = indel_editops, LCSseq_impl.hpp:440-534), pinned:
+1229-1257; lcs_seq_editops = indel_editops, LCSseq_impl.hpp:440-534; hamming_editops,
+Hamming_impl.hpp:40-59), pinned:
 
 * against the compiled reference, operation for operation: Levenshtein pairs whose
   first string has at most 64 units after the common affix is removed (where the
@@ -15,7 +16,7 @@ import pytest
 
 import _gen
 
-LEV, INDEL, LCS = 0, 1, 2
+LEV, INDEL, LCS, HAMMING = 0, 1, 2, 7
 REPLACE, INSERT, DELETE = 1, 2, 3
 
 
@@ -52,7 +53,7 @@ REF_STRINGS = [
 ]
 
 
-@pytest.mark.parametrize("metric", [LEV, LCS])
+@pytest.mark.parametrize("metric", [LEV, LCS, HAMMING])
 def test_apply_and_count(oracle, metric):
     rng = np.random.default_rng(5 + metric)
     pairs = [(_codes(a), _codes(b)) for a, b in REF_STRINGS]
@@ -61,7 +62,7 @@ def test_apply_and_count(oracle, metric):
     for s1, s2 in pairs:
         ops = oracle.editops(metric, s1, s2)
         assert apply_ops(ops, s1, s2) == [int(x) for x in s2]
-        kind_metric = LEV if metric == LEV else INDEL
+        kind_metric = metric if metric in (LEV, HAMMING) else INDEL
         assert len(ops[0]) == int(oracle.score(kind_metric, 0, s1, s2))
         assert np.all(np.diff(ops[1].astype(np.int64)) >= 0) and np.all(np.diff(ops[2].astype(np.int64)) >= 0)
 
@@ -84,6 +85,19 @@ def test_levenshtein_vs_reference(oracle, reference):
             pairs.append((s1, s2))
             pairs.append((s1, _gen.mutate(rng, s1, int(rng.integers(0, 8)), alpha_n=alpha)))
     _check_vs_reference(oracle, reference, LEV, pairs)
+
+
+def test_hamming_vs_reference(oracle, reference):
+    """hamming_editops with pad (Hamming_impl.hpp:40-59), tests-Hamming.cpp:112-127's strings included."""
+    rng = np.random.default_rng(23)
+    pairs = [(_codes("Lorem ipsum."), _codes("XYZLorem ABC iPsum")), (_codes("XYZLorem ABC iPsum"),
+                                                                      _codes("Lorem ipsum.")),
+             (_codes(""), _codes("ab"))]
+    for _ in range(200):
+        s1 = rng.integers(97, 100, size=int(rng.integers(0, 80))).astype(np.uint32)
+        s2 = rng.integers(97, 100, size=int(rng.integers(0, 80))).astype(np.uint32)
+        pairs.append((s1, s2))
+    _check_vs_reference(oracle, reference, HAMMING, pairs)
 
 
 def test_lcs_vs_reference(oracle, reference):

@@ -12,7 +12,7 @@ from test_editops_oracle import REF_STRINGS, _codes, apply_ops
 
 pytestmark = pytest.mark.gpu
 
-LEV, INDEL, LCS = 0, 1, 2
+LEV, INDEL, LCS, HAMMING = 0, 1, 2, 7
 
 
 def _pairs(rng, alpha, n=120):
@@ -48,14 +48,14 @@ def check(ctx, oracle, metric, pairs, width=1):
     b.close()
 
 
-@pytest.mark.parametrize("metric", [LEV, INDEL, LCS])
+@pytest.mark.parametrize("metric", [LEV, INDEL, LCS, HAMMING])
 @pytest.mark.parametrize("alpha", [2, 5])
 def test_random(ctx, oracle, metric, alpha):
     check(ctx, oracle, metric, _pairs(np.random.default_rng(10 * metric + alpha), alpha))
 
 
 @pytest.mark.parametrize("width", [2, 4])
-@pytest.mark.parametrize("metric", [LEV, LCS])
+@pytest.mark.parametrize("metric", [LEV, LCS, HAMMING])
 def test_wide_units(ctx, oracle, width, metric):
     rng = np.random.default_rng(30 + width)
     lo = 0x4E00 if width == 2 else 0x1F300
@@ -77,6 +77,14 @@ def test_lcs_long_first_strings(ctx, oracle, metric):
                rng.integers(97, 99, size=int(rng.integers(0, 200))).astype(np.uint32)) for _ in range(30)]
     pairs += _pairs(rng, 3, n=20)
     check(ctx, oracle, metric, pairs)
+
+
+def test_hamming_long(ctx, oracle):
+    """Hamming editops have no length limit: strings of up to 5000 units."""
+    rng = np.random.default_rng(61)
+    pairs = [(rng.integers(97, 99, size=n1).astype(np.uint32), rng.integers(97, 99, size=n2).astype(np.uint32))
+             for n1, n2 in ((5000, 5000), (4000, 100), (70, 3000), (0, 129), (129, 0))]
+    check(ctx, oracle, HAMMING, pairs)
 
 
 def test_sizes_then_ops_and_limits(ctx, oracle):
