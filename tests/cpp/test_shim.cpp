@@ -244,6 +244,18 @@ int main()
                              [](const auto& a, const auto& b) { return a.second > b.second; });
             if (expect.size() > 5) expect.resize(5);
             CHECK(top == expect);
+            // the same for a token-set ratio (scored over the whole set, then selected)
+            rfa::fuzz::CachedTokenSetRatio<char> tset(choices[3]);
+            const auto ttop = tset.extract_topk(set, 3, 30.0);
+            std::vector<std::pair<int64_t, double>> texpect;
+            const auto tall = tset.similarity_many(set, 30.0);
+            for (size_t c = 0; c < choices.size(); ++c)
+                if (tall[c] >= 30.0) texpect.emplace_back((int64_t)c, tall[c]);
+            std::stable_sort(texpect.begin(), texpect.end(),
+                             [](const auto& a, const auto& b) { return a.second > b.second; });
+            if (texpect.size() > 3) texpect.resize(3);
+            CHECK(ttop == texpect);
+            CHECK(!ttop.empty() && ttop[0].second == 100.0);
         }
 
         // ---- experimental::Multi* ----
