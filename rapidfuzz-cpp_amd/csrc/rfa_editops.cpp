@@ -54,31 +54,40 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
         }
     }
     ea.rows = (uint64_t*)ctx->ed_rows.get(std::max<uint64_t>(8, row_words * 8));
-    uint8_t* items = (uint8_t*)ctx->ed_items.get(n * sizeof(EdInfo) + (n + 1) * 8 + (lev || ham ? 0 : n * 8));
-    ea.info = (EdInfo*)items;
-    ea.offsets = (const uint64_t*)(items + n * sizeof(EdInfo));
+    // items: info, counts (n + 1), offsets (n + 1), row bases (LCS), the error word, scan temp
+    const size_t scan_bytes = editops_scan_bytes(n);
+    const size_t o_info = 0, o_counts = n * sizeof(EdInfo), o_offsets = o_counts + (n + 1) * 8,
+                 o_base = o_offsets + (n + 1) * 8, o_err = o_base + (lev || ham ? 0 : n * 8),
+                 o_temp = (o_err + 4 + 255) / 256 * 256;
+    uint8_t* items = (uint8_t*)ctx->ed_items.get(o_temp + scan_bytes);
+    ea.info = (EdInfo*)(items + o_info);
+    ea.counts = (uint64_t*)(items + o_counts);
+    uint64_t* d_offsets = (uint64_t*)(items + o_offsets);
+    ea.offsets = d_offsets;
+    ea.err = (uint32_t*)(items + o_err);
     ea.n = n;
     if (!lev && !ham) {
-        ea.row_base = (const uint64_t*)(items + n * sizeof(EdInfo) + (n + 1) * 8);
+        ea.row_base = (const uint64_t*)(items + o_base);
         HIP_CHECK(hipMemcpyAsync((void*)ea.row_base, row_base.data(), n * 8, hipMemcpyHostToDevice, s));
     }
+    HIP_CHECK(hipMemsetAsync(ea.counts + n, 0, 8, s));
+    HIP_CHECK(hipMemsetAsync(ea.err, 0, 4, s));
     const uint32_t nblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 3) / 4, (uint64_t)ctx->num_cus * 16));
     if (ham)
         HIP_CHECK(launch_editops_hamming(a->width, ea, false, nblocks, s));
     else
         HIP_CHECK(launch_editops(a->width, ea, lev, false, nblocks, s));
-    std::vector<EdInfo> info(n);
-    HIP_CHECK(hipMemcpyAsync(info.data(), ea.info, n * sizeof(EdInfo), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(editops_offsets(ea.counts, d_offsets, n, items + o_temp, scan_bytes, s));
+    uint32_t err = 0;
+    HIP_CHECK(hipMemcpyAsync(&err, ea.err, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(offsets, d_offsets, (n + 1) * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    for (uint64_t i = 0; i < n; ++i) {
-        if (info[i].flag)
-            throw ApiError(RFA_ERR_UNSUPPORTED,
-                           lev ? "Levenshtein editops of a pair whose first string has more than 64 code units "
-                                 "without the common prefix and suffix"
-                               : "LCS editops of a pair whose first string has more than 4096 code units without "
-                                 "the common prefix and suffix");
-        offsets[i + 1] = offsets[i] + info[i].count;
-    }
+    if (err)
+        throw ApiError(RFA_ERR_UNSUPPORTED,
+                       lev ? "Levenshtein editops of a pair whose first string has more than 64 code units "
+                             "without the common prefix and suffix"
+                           : "LCS editops of a pair whose first string has more than 4096 code units without "
+                             "the common prefix and suffix");
     if (!ops) return;
     const uint64_t total = offsets[n];
     if (capacity < total)
@@ -86,7 +95,6 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
                        "ops holds " + std::to_string(capacity) + " operations, " + std::to_string(total) + " needed");
     if (total == 0) return;
     ea.ops = (EdOp*)ctx->ed_ops.get(total * sizeof(EdOp));
-    HIP_CHECK(hipMemcpyAsync((void*)ea.offsets, offsets, (n + 1) * 8, hipMemcpyHostToDevice, s));
     if (ham)
         HIP_CHECK(launch_editops_hamming(a->width, ea, true, nblocks, s));
     else

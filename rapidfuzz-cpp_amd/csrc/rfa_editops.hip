@@ -31,10 +31,26 @@
 // strings of up to 4096 units run here, over several words (lane l holding
 // s1[64 k + l], so a word's match mask is one ballot; the words' additions carry
 // across lanes by carry-lookahead on two ballots).
+#include <hipcub/hipcub.hpp>
+
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
 
 namespace rfa {
+
+size_t editops_scan_bytes(uint64_t n)
+{
+    size_t b = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr, (int)(n + 1));
+    return b;
+}
+
+hipError_t editops_offsets(const uint64_t* counts, uint64_t* offsets, uint64_t n, void* temp, size_t bytes,
+                           hipStream_t s)
+{
+    size_t b = bytes;
+    return hipcub::DeviceScan::ExclusiveSum(temp, b, counts, offsets, (int)(n + 1), s);
+}
 
 template <typename CharT, bool LEV>
 __global__ __launch_bounds__(256) void k_ed_rows(EditopsArgs a)
@@ -83,7 +99,11 @@ __global__ __launch_bounds__(256) void k_ed_rows(EditopsArgs a)
         info.flag = la > (LEV ? 64u : EDITOPS_LCS_MAX) ? 1u : 0u;
         info.count = (uint32_t)(la + lb);
         if (la == 0 || lb == 0 || info.flag) {
-            if (lane == 0) a.info[it] = info;
+            if (lane == 0) {
+                a.info[it] = info;
+                a.counts[it] = info.flag ? 0 : info.count;
+                if (info.flag) atomicOr(a.err, 1u);
+            }
             continue;
         }
         const uint64_t as = a0 + pre, bs = b0 + pre;
@@ -129,7 +149,10 @@ __global__ __launch_bounds__(256) void k_ed_rows(EditopsArgs a)
                 uint32_t cnt = act ? (uint32_t)__popcll(~Sw & valid) : 0u;
                 for (int d = 32; d >= 1; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
                 info.count = (uint32_t)(la + lb - 2ull * cnt);
-                if (lane == 0) a.info[it] = info;
+                if (lane == 0) {
+                    a.info[it] = info;
+                    a.counts[it] = info.count;
+                }
                 continue;
             }
         }
@@ -181,7 +204,10 @@ __global__ __launch_bounds__(256) void k_ed_rows(EditopsArgs a)
             info.count = dist;
         else
             info.count = (uint32_t)(la + lb - 2 * (uint64_t)__popcll(~S & (la >= 64 ? ~0ull : ((1ull << la) - 1))));
-        if (lane == 0) a.info[it] = info;
+        if (lane == 0) {
+            a.info[it] = info;
+            a.counts[it] = info.count;
+        }
     }
 }
 
@@ -301,6 +327,7 @@ __global__ __launch_bounds__(256) void k_ed_hamming(EditopsArgs a)
             info.flag = 0;
             info.count = (uint32_t)(k + (la - mn) + (lb - mn));
             a.info[it] = info;
+            a.counts[it] = info.count;
         }
     }
 }

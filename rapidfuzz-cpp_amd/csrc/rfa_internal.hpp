@@ -293,7 +293,13 @@ struct EditopsArgs {
     EdOp* ops;
     uint64_t n;
     const uint64_t* row_base; // LCS: first row word of each pair (rows of ceil(la / 64) words)
+    uint64_t* counts;         // n + 1 operation counts (the last one zero), scanned into offsets
+    uint32_t* err;            // set when a pair exceeds the length limit
 };
+size_t editops_scan_bytes(uint64_t n);
+// offsets[0..n] = exclusive prefix sum of counts[0..n] (counts[n] = 0)
+hipError_t editops_offsets(const uint64_t* counts, uint64_t* offsets, uint64_t n, void* temp, size_t bytes,
+                           hipStream_t s);
 constexpr uint32_t EDITOPS_LCS_MAX = 4096; // stripped first string, LCS editops
 hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, bool walk, uint32_t nblocks,
                           hipStream_t s);
