@@ -818,10 +818,11 @@ void cdist_dl(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, void
         la.out = dout;
         la.qcut = d_qcut;
         const uint64_t per_block = 3ull * (la.max_rows + 1) * 64 * 4;
-        constexpr uint64_t SCRATCH_TOTAL = 1ull << 30;
+        // one-wave blocks, up to 8 per SIMD: the row loop waits on scratch, so it needs the waves
+        constexpr uint64_t SCRATCH_TOTAL = 4ull << 30;
         const uint64_t items = (uint64_t)la.nql * ((ch->count + 63) / 64);
         const uint64_t nb = std::max<uint64_t>(
-            1, std::min<uint64_t>(std::min<uint64_t>(items, (uint64_t)ctx->num_cus * 8), SCRATCH_TOTAL / per_block));
+            1, std::min<uint64_t>(std::min<uint64_t>(items, (uint64_t)ctx->num_cus * 32), SCRATCH_TOTAL / per_block));
         la.scratch = (uint32_t*)ctx->pm_scratch.get(nb * per_block);
         HIP_CHECK(launch_dllong(sc->q->width, la, (uint32_t)nb, ctx->stream));
     }
