@@ -411,6 +411,7 @@ void free_scorer(rfa_scorer* sc)
     free_strings(sc->tok_choices);
     free_scorer(sc->sort_sub);
     free_scorer(sc->pr_dedup);
+    free_scorer(sc->pr_dedup_dupq);
     delete sc;
 }
 
@@ -426,6 +427,10 @@ void scorer_set_signed(rfa_scorer* sc, bool on)
         sc->q = sc->q_tokens->joined;
         sc->q_tokens->joined = nullptr;
         plan_scorer(sc);
+        free_scorer(sc->pr_dedup_dupq);
+        sc->pr_dedup_dupq = nullptr;
+        sc->dup_queries.clear();
+        sc->dup_queries_ready = false;
         if (sc->pr_dedup) {
             free_scorer(sc->pr_dedup);
             sc->pr_dedup = nullptr;
@@ -586,9 +591,83 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
         mode = 2;
         break;
     case RFA_PARTIAL_TOKEN_RATIO: {
-        cdist_impl(sc->sort_sub, scorer_choices(sc->sort_sub, raw), RFA_SIMILARITY, RFA_CACHED, 0, cd, dout, 1);
+        const rfa_strings* sorted_ch = scorer_choices(sc->sort_sub, raw);
+        cdist_impl(sc->sort_sub, sorted_ch, RFA_SIMILARITY, RFA_CACHED, 0, cd, dout, 1);
+        // The deduplicated join equals the sorted join unless the string repeats a token
+        // (then it is shorter), so the partial_ratio of the deduplicated joins differs from
+        // the sorted joins' only in the rows of queries and the columns of choices that
+        // repeat one: start from the sorted joins' values and compute just those.
+        if (!sc->dup_queries_ready) {
+            const rfa_strings* qs = sc->sort_sub->q;
+            for (uint64_t q = 0; q < nq; ++q)
+                if (sc->q->len(q) < qs->len(q)) sc->dup_queries.push_back((uint32_t)q);
+            if (!sc->dup_queries.empty()) {
+                std::vector<uint8_t> chars;
+                std::vector<uint64_t> off{0};
+                const int w = sc->q->width;
+                for (uint32_t q : sc->dup_queries) {
+                    const uint8_t* p = sc->q->h_chars.data() + sc->q->h_off[q] * w;
+                    chars.insert(chars.end(), p, p + sc->q->len(q) * w);
+                    off.push_back(off.back() + sc->q->len(q));
+                }
+                rfa_strings* sub = make_strings(ctx, chars.data(), w, off.data(), sc->dup_queries.size());
+                try {
+                    sc->pr_dedup_dupq = make_scorer(ctx, RFA_PARTIAL_RATIO, sub, sc->w, sc->prefix_weight);
+                }
+                catch (...) {
+                    free_strings(sub);
+                    throw;
+                }
+                free_strings(sub);
+            }
+            sc->dup_queries_ready = true;
+        }
+        std::vector<uint32_t> dup_choices;
+        for (uint64_t c = 0; c < nc; ++c)
+            if (ch->len(c) < sorted_ch->len(c)) dup_choices.push_back((uint32_t)c);
         double* d2 = (double*)ctx->aux_tmp.get(bytes);
-        cdist_impl(sc->pr_dedup, ch, RFA_SIMILARITY, RFA_CACHED, 0, cd, d2, 1);
+        if (sc->dup_queries.size() * 4 > nq || dup_choices.size() * 4 > nc) {
+            // repeats are common: the whole matrix
+            cdist_impl(sc->pr_dedup, ch, RFA_SIMILARITY, RFA_CACHED, 0, cd, d2, 1);
+        }
+        else {
+            HIP_CHECK(hipMemcpyAsync(d2, dout, bytes, hipMemcpyDeviceToDevice, s));
+            if (!dup_choices.empty()) {
+                // all queries x the choices that repeat a token
+                std::vector<uint8_t> chars;
+                std::vector<uint64_t> off{0};
+                const int w = ch->width;
+                for (uint32_t c : dup_choices) {
+                    const uint8_t* p = ch->h_chars.data() + ch->h_off[c] * w;
+                    chars.insert(chars.end(), p, p + ch->len(c) * w);
+                    off.push_back(off.back() + ch->len(c));
+                }
+                rfa_strings* sub = make_strings(ctx, chars.data(), w, off.data(), dup_choices.size());
+                try {
+                    const size_t sb = (size_t)nq * dup_choices.size() * 8;
+                    double* d3 = (double*)ctx->tset_cols.get(sb + dup_choices.size() * 4);
+                    uint32_t* cols = (uint32_t*)((uint8_t*)d3 + sb);
+                    HIP_CHECK(hipMemcpyAsync(cols, dup_choices.data(), dup_choices.size() * 4, hipMemcpyHostToDevice, s));
+                    cdist_impl(sc->pr_dedup, sub, RFA_SIMILARITY, RFA_CACHED, 0, cd, d3, 1);
+                    HIP_CHECK(launch_scatter_cols(d2, nc, d3, cols, dup_choices.size(), nq, s));
+                    HIP_CHECK(hipStreamSynchronize(s)); // the subset set is freed next
+                }
+                catch (...) {
+                    free_strings(sub);
+                    throw;
+                }
+                free_strings(sub);
+            }
+            if (!sc->dup_queries.empty()) {
+                // the queries that repeat a token x all choices, row by row
+                const uint64_t nd = sc->dup_queries.size();
+                double* d4 = (double*)ctx->tset_rows.get((size_t)nd * nc * 8);
+                cdist_impl(sc->pr_dedup_dupq, ch, RFA_SIMILARITY, RFA_CACHED, 0, cd, d4, 1);
+                for (uint64_t r = 0; r < nd; ++r)
+                    HIP_CHECK(hipMemcpyAsync(d2 + (uint64_t)sc->dup_queries[r] * nc, d4 + r * nc, nc * 8,
+                                             hipMemcpyDeviceToDevice, s));
+            }
+        }
         aux = d2;
         mode = 3;
         break;
@@ -1290,7 +1369,8 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
                           &ctx->block_items[1], &ctx->block_items[2], &ctx->block_items[3], &ctx->plan_keys,
                           &ctx->plan_vals, &ctx->plan_temp, &ctx->topk_scores, &ctx->topk_run[0],
                           &ctx->topk_run[1], &ctx->topk_cand_s, &ctx->topk_cand_i, &ctx->topk_counts, &ctx->topk_q,
-                          &ctx->topk_out, &ctx->qcut, &ctx->ed_items, &ctx->ed_rows, &ctx->ed_ops})
+                          &ctx->topk_out, &ctx->qcut, &ctx->ed_items, &ctx->ed_rows, &ctx->ed_ops, &ctx->tset_cols,
+                          &ctx->tset_rows})
             b->release();
         ctx->staging.release();
         ctx->pairs_graph.release();

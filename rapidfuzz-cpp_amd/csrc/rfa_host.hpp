@@ -214,6 +214,7 @@ struct rfa_context {
         plan_temp;
     rfa::DevBuf topk_scores, topk_run[2], topk_cand_s, topk_cand_i, topk_counts, topk_q, topk_out, qcut;
     rfa::DevBuf ed_items, ed_rows, ed_ops;
+    rfa::DevBuf tset_cols, tset_rows; // partial_token_ratio: the repeat columns and rows
     size_t lane_arena_bytes_zeroed = 0;
     uint32_t gen_next = 1;
     rfa::Keep keep; // host sources of in-flight uploads
@@ -321,6 +322,11 @@ struct rfa_scorer {
     bool c_tokens_signed8 = false;
     rfa_scorer* sort_sub = nullptr; // token_ratio: token-sort ratio; partial_token_ratio: partial token-sort
     rfa_scorer* pr_dedup = nullptr; // partial token ratios: partial_ratio over the deduplicated joins
+    // partial_token_ratio: the queries that repeat a token (their deduplicated join is shorter
+    // than their sorted join) and a partial_ratio scorer over just their deduplicated joins
+    std::vector<uint32_t> dup_queries;
+    rfa_scorer* pr_dedup_dupq = nullptr;
+    bool dup_queries_ready = false;
     rfa_strings* tok_choices = nullptr; // token-sorted copy of the last choice set
     uint64_t tok_choices_id = 0;
     bool tok_choices_signed8 = false;

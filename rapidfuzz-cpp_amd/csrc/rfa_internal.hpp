@@ -215,6 +215,8 @@ struct TsetArgs {
     uint32_t tok_lds_u64; // set by launch_tset
     const double* aux;    // mode 3: partial_ratio of the deduplicated joins (out holds the sorted joins')
 };
+hipError_t launch_scatter_cols(double* dst, uint64_t stride, const double* src, const uint32_t* cols, uint64_t ncols,
+                               uint64_t nq, hipStream_t s);
 hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
                        hipStream_t s);
 

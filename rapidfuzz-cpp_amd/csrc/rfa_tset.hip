@@ -243,6 +243,27 @@ static void launch_tset_w(const TsetArgs& a, uint32_t ntiles, uint32_t nbx, size
     }
 }
 
+// dst[q * stride + cols[j]] = src[q * ncols + j] for every q < nq, j < ncols
+__global__ __launch_bounds__(256) void k_scatter_cols(double* dst, uint64_t stride, const double* src,
+                                                      const uint32_t* cols, uint64_t ncols, uint64_t nq)
+{
+    const uint64_t total = nq * ncols;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t q = i / ncols, j = i % ncols;
+        dst[q * stride + cols[j]] = src[i];
+    }
+}
+
+hipError_t launch_scatter_cols(double* dst, uint64_t stride, const double* src, const uint32_t* cols, uint64_t ncols,
+                               uint64_t nq, hipStream_t s)
+{
+    const uint64_t total = nq * ncols;
+    if (!total) return hipSuccess;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_scatter_cols, dim3(blocks), dim3(256), 0, s, dst, stride, src, cols, ncols, nq);
+    return hipGetLastError();
+}
+
 hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
                        hipStream_t s)
 {

@@ -134,6 +134,33 @@ def test_token_sets_many_choice_tokens(ctx, oracle):
         _check(ctx, oracle, metric, q, c, 1, cuts=(0.0, 70.0, 100.5))
 
 
+@pytest.mark.parametrize("signed8", [False, True])
+def test_partial_token_ratio_few_repeats(ctx, oracle, signed8):
+    """partial_token_ratio where few strings repeat a token: the deduplicated joins' partial_ratio
+    is computed only for the rows and columns of the strings that repeat one (about 10% of the
+    choices, 2 of 40 queries), the rest taken from the sorted joins'."""
+    rng = np.random.default_rng(640 + signed8)
+    vocab = [np.array([int(x) for x in rng.integers(97, 123, size=int(rng.integers(2, 6)))], np.uint32)
+             for _ in range(400)]
+    if signed8:
+        vocab += [np.array([0xE9, 97 + i % 3], np.uint32) for i in range(20)]
+    def sentence(n, repeat):
+        idx = list(rng.choice(len(vocab), size=n, replace=False))
+        if repeat and n:
+            idx.append(idx[int(rng.integers(0, len(idx)))])
+            rng.shuffle(idx)
+        out = []
+        for i, t in enumerate(idx):
+            if i:
+                out.append(32)
+            out.extend(int(x) for x in vocab[int(t)])
+        return np.array(out, np.uint32)
+    q = [sentence(int(rng.integers(1, 5)), i in (5, 17))[:64] for i in range(40)]
+    c = [sentence(int(rng.integers(0, 12)), rng.random() < 0.1) for _ in range(300)]
+    c += [q[i] for i in range(0, 40, 4)]
+    _check(ctx, oracle, PTRATIO, q, c, 1, signed8=signed8, cuts=(0.0, 60.0, 95.0))
+
+
 def test_token_set_limits(ctx):
     import rapidfuzz_amd as rfa
 
