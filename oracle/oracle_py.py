@@ -115,6 +115,27 @@ class Oracle:
             raise ValueError(f"unsupported metric/kind {metric}/{kind}")
         return (ou if r == 0 else od).reshape(nq, nc)
 
+    def cdist_mt(self, metric, kind, queries, choices, threads=None, **kw):
+        """cdist over row blocks of the queries on a thread pool (ctypes releases the GIL).
+        Per-query cutoffs: cutoffs_u=[...] scores row i with cutoff_u=cutoffs_u[i]."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        cut = kw.pop("cutoffs_u", None)
+        threads = threads or min(16, os.cpu_count() or 1)
+        if cut is not None:
+            rows = [(i, i + 1) for i in range(len(queries))]
+        else:
+            step = max(1, (len(queries) + threads - 1) // threads)
+            rows = [(i, min(len(queries), i + step)) for i in range(0, len(queries), step)]
+
+        def part(r):
+            extra = {} if cut is None else {"cutoff_u": int(cut[r[0]])}
+            return self.cdist(metric, kind, queries[r[0]:r[1]], choices, **kw, **extra)
+
+        with ThreadPoolExecutor(threads) as ex:
+            parts = list(ex.map(part, rows))
+        return np.concatenate(parts, axis=0) if parts else np.zeros((0, len(choices)))
+
     def score(self, metric, kind, s1, s2, **kw):
         return self.cdist(metric, kind, [s1], [s2], **kw)[0, 0]
 
