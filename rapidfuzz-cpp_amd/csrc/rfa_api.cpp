@@ -201,6 +201,11 @@ rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const 
         delete s;
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "chars is NULL");
     }
+    if (s->max_len > 0xffffffffull) {
+        // the kernels count a string's units in 32 bits
+        delete s;
+        throw ApiError(RFA_ERR_UNSUPPORTED, "strings longer than 2^32 - 1 code units");
+    }
     s->h_chars.resize(nunits * width);
     if (nunits) std::memcpy(s->h_chars.data(), chars, nunits * width);
     // 64 bytes of tail padding: kernels load code units in short vectors and may read
@@ -212,8 +217,14 @@ rfa_strings* make_strings(rfa_context* ctx, const void* chars, int width, const 
         delete s;
         throw ApiError(RFA_ERR_OUT_OF_MEMORY, "device allocation for strings failed");
     }
-    if (nunits) HIP_CHECK(hipMemcpy(s->d_chars, s->h_chars.data(), nunits * width, hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(s->d_off, s->h_off.data(), s->h_off.size() * 8, hipMemcpyHostToDevice));
+    try {
+        if (nunits) HIP_CHECK(hipMemcpy(s->d_chars, s->h_chars.data(), nunits * width, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(s->d_off, s->h_off.data(), s->h_off.size() * 8, hipMemcpyHostToDevice));
+    }
+    catch (...) {
+        free_strings(s);
+        throw;
+    }
     return s;
 }
 
@@ -490,6 +501,21 @@ ScoreSpec make_spec(const rfa_scorer* sc, int kind, int semantics, uint64_t cu, 
             s.band_k = cu;
     }
     return s;
+}
+
+// Whether the cutoff can decide a result from the two lengths alone.  The
+// reference's defaults (no integer cutoff, similarity 0, normalized 1.0 / 0.0)
+// never do, so those calls skip the per-pair test.
+bool cutoff_may_settle(const ScoreSpec& s, bool per_query)
+{
+    if (per_query) return true;
+    const bool fp = s.metric == RFA_JARO || s.metric == RFA_JARO_WINKLER || s.metric == RFA_RATIO;
+    switch (s.kind) {
+    case RFA_DISTANCE: return fp ? s.cutoff_d < 1.0 : s.cutoff_u < (1ull << 62);
+    case RFA_SIMILARITY: return fp ? s.cutoff_d > 0.0 : s.cutoff_u > 0;
+    case RFA_NORMALIZED_DISTANCE: return s.cutoff_d < 1.0;
+    default: return s.cutoff_d > 0.0;
+    }
 }
 
 void validate_kind(int metric, int kind, int semantics)
@@ -1043,6 +1069,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ta.spec = spec;
         ta.out = dout;
         ta.qcut = d_qcut;
+        ta.settle = cutoff_may_settle(spec, d_qcut != nullptr) ? 1u : 0u;
         if (ta.ncsel) {
             for (const TileClass& tc : sc->classes) {
                 ta.tile_base = tc.first_tile;

@@ -25,6 +25,7 @@ struct TileArgs {
     ScoreSpec spec;
     void* out;
     const uint64_t* qcut; // per-query integer cutoff (nullptr: spec.cutoff_u)
+    uint32_t settle;      // the cutoffs may decide results by length alone: test before each recurrence
 };
 
 size_t tile_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t hash_bits);

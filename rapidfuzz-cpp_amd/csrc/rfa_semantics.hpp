@@ -79,7 +79,7 @@ __device__ __forceinline__ uint64_t int_distance_core(const IntPair& p, uint64_t
         // CachedLevenshtein::_distance: uniform or Indel, scaled by the common weight
         if (s.ins == s.del) {
             if (s.ins == 0) return 0;
-            uint64_t nc = ceil_div_u64(c, s.ins);
+            uint64_t nc = s.ins == 1 ? c : ceil_div_u64(c, s.ins);
             uint64_t d;
             if (s.ins == s.rep) {
                 nc = u64min(nc, u64max(p.l1, p.l2)); // uniform_levenshtein_distance clamp
@@ -155,34 +155,33 @@ __device__ __forceinline__ uint64_t multi_full_distance(const IntPair& p)
 
 // --- Jaro / Jaro-Winkler ---------------------------------------------------
 
-// jaro_similarity(PM, P, T, cutoff) with the library's filters, from counts.
+// jaro_similarity(PM, P, T, cutoff) from the counts (Jaro_impl.hpp:32-71, 409-454).
+//
+// The reference checks two upper bounds of Sim against the cutoff before the
+// formula: the length bound (min/P + min/T + 1) / 3 and the common-character
+// bound (cc/P + cc/T + 1) / 3.  Each bound adds terms at least as large as the
+// formula's (cc <= min(P, T), and (cc - t) / cc <= 1), in the same order, and
+// rounded division and addition are monotone, so each computed bound is at
+// least the computed Sim.  A failed bound therefore means Sim < cutoff, and the
+// result is 0 either way: the formula and its final comparison decide alone.
+// (1 x 1 strings reach their rule whenever cutoff <= 1, where the length bound
+// is exactly 1.0.)  Lengths and counts are below 2^32 (strings are capped there),
+// so they convert to double through 32 bits.
 __device__ __forceinline__ double jaro_from_counts(uint64_t Pl, uint64_t Tl, uint64_t cc, uint64_t tr, bool eq11,
                                                    double cutoff)
 {
     if (cutoff > 1.0) return 0.0;
     if (!Pl && !Tl) return 1.0;
     if (!Tl || !Pl) return 0.0;
-    {
-        double min_len = (double)(Pl < Tl ? Pl : Tl);
-        double Sim = min_len / (double)Pl + min_len / (double)Tl + 1.0;
-        Sim /= 3.0;
-        if (!(Sim >= cutoff)) return 0.0;
-    }
+    if (cutoff != cutoff) return 0.0; // a NaN cutoff fails the length bound
     if (Pl == 1 && Tl == 1) return eq11 ? 1.0 : 0.0;
     if (!cc) return 0.0;
-    {
-        double Sim = 0;
-        Sim += (double)cc / (double)Pl;
-        Sim += (double)cc / (double)Tl;
-        Sim += 1.0;
-        Sim /= 3.0;
-        if (!(Sim >= cutoff)) return 0.0;
-    }
-    uint64_t t = tr / 2;
+    const uint32_t t = (uint32_t)tr / 2;
+    const double dcc = (double)(uint32_t)cc;
     double Sim = 0;
-    Sim += (double)cc / (double)Pl;
-    Sim += (double)cc / (double)Tl;
-    Sim += ((double)cc - (double)t) / (double)cc;
+    Sim += dcc / (double)(uint32_t)Pl;
+    Sim += dcc / (double)(uint32_t)Tl;
+    Sim += (dcc - (double)t) / dcc;
     Sim /= 3.0;
     return (Sim >= cutoff) ? Sim : 0;
 }
@@ -261,6 +260,50 @@ __device__ __forceinline__ uint64_t band_from_cutoff(const ScoreSpec& s)
     return ~0ull;
 }
 
+// --- results decided by the lengths -------------------------------------------
+
+// The raw value of a pair of lengths m, n lies in [raw_range_lo, raw_range_hi]:
+// a distance (Levenshtein, OSA) in [|m - n|, max(m, n)], an LCS length in
+// [0, min(m, n)].
+template <int ALGO>
+__device__ __forceinline__ uint64_t raw_range_lo(uint64_t m, uint64_t n)
+{
+    if constexpr (ALGO == ALGO_LCS)
+        return 0;
+    else
+        return m > n ? m - n : n - m;
+}
+
+template <int ALGO>
+__device__ __forceinline__ uint64_t raw_range_hi(uint64_t m, uint64_t n)
+{
+    if constexpr (ALGO == ALGO_LCS)
+        return m < n ? m : n;
+    else
+        return m > n ? m : n;
+}
+
+__device__ __forceinline__ uint64_t finalize_int(const ScoreSpec& s, uint64_t l1, uint64_t l2, uint64_t raw);
+template <int ALGO>
+__device__ __forceinline__ uint64_t finalize_bits_algo(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r);
+
+// True when the result of (m, n) is the same for every raw value of its range,
+// so the recurrence can be skipped (the reference's length filters:
+// Levenshtein_impl.hpp:921-922, LCSseq_impl.hpp:384-389, OSA's |m - n| bound).
+// Every result chain here is monotone in the raw value (the cut values c + 1,
+// 1.0 and 0 lie beyond every uncut value, and the double steps round
+// monotonically), so equal results at both ends mean equal results throughout.
+template <int ALGO, bool INTK>
+__device__ __forceinline__ bool settled_by_length(const ScoreSpec& s, uint64_t m, uint64_t n)
+{
+    const uint64_t lo = raw_range_lo<ALGO>(m, n), hi = raw_range_hi<ALGO>(m, n);
+    if constexpr (INTK)
+        return finalize_int(s, m, n, lo) == finalize_int(s, m, n, hi);
+    else
+        return finalize_bits_algo<ALGO>(s, m, n, RawResult{lo, 0, 0}) ==
+               finalize_bits_algo<ALGO>(s, m, n, RawResult{hi, 0, 0});
+}
+
 // --- dispatch ----------------------------------------------------------------
 
 // Integer-valued kinds: distance / similarity of Levenshtein, Indel, LCSseq, OSA.
@@ -279,40 +322,43 @@ __device__ __forceinline__ uint64_t finalize_int(const ScoreSpec& s, uint64_t l1
     return s.kind == RFA_DISTANCE ? int_api_distance(p, s.cutoff_u) : int_api_similarity(p, s.cutoff_u);
 }
 
-// Writes the reference API value for one pair into out[idx] (uint64 or double).
-__device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r, void* out,
-                                         uint64_t idx)
+// Jaro / Jaro-Winkler: the reference API value of one pair.
+__device__ __forceinline__ double finalize_jaro(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r)
+{
+    FpPair p{&s, l1, l2, r};
+    double v;
+    if (s.semantics == RFA_MULTI) {
+        if (s.kind == RFA_SIMILARITY)
+            v = multi_fp_similarity(p, s.cutoff_d);
+        else {
+            double dist = 1.0 - multi_fp_similarity(p, 0.0);
+            if (s.kind != RFA_DISTANCE) dist = dist <= 1.0 ? dist : 1.0;
+            if (s.kind == RFA_NORMALIZED_SIMILARITY) {
+                double ns = 1.0 - dist;
+                v = ns >= s.cutoff_d ? ns : 0.0;
+            }
+            else
+                v = dist <= s.cutoff_d ? dist : 1.0;
+        }
+    }
+    else {
+        switch (s.kind) {
+        case RFA_DISTANCE: v = fp_distance(p, s.cutoff_d); break;
+        case RFA_SIMILARITY: v = fp_similarity(p, s.cutoff_d); break;
+        case RFA_NORMALIZED_DISTANCE: v = fp_norm_distance(p, s.cutoff_d); break;
+        default: v = fp_norm_similarity(p, s.cutoff_d); break;
+        }
+    }
+    return v;
+}
+
+// The integer-valued metrics (Levenshtein, Indel, LCSseq, OSA, ratio, the
+// positional metrics): writes the value for one pair into out[idx] (uint64 or double).
+__device__ __forceinline__ void finalize_intmetric(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r, void* out,
+                                                   uint64_t idx)
 {
     uint64_t* ou = (uint64_t*)out;
     double* od = (double*)out;
-    if (s.metric == RFA_JARO || s.metric == RFA_JARO_WINKLER) {
-        FpPair p{&s, l1, l2, r};
-        double v;
-        if (s.semantics == RFA_MULTI) {
-            if (s.kind == RFA_SIMILARITY)
-                v = multi_fp_similarity(p, s.cutoff_d);
-            else {
-                double dist = 1.0 - multi_fp_similarity(p, 0.0);
-                if (s.kind != RFA_DISTANCE) dist = dist <= 1.0 ? dist : 1.0;
-                if (s.kind == RFA_NORMALIZED_SIMILARITY) {
-                    double ns = 1.0 - dist;
-                    v = ns >= s.cutoff_d ? ns : 0.0;
-                }
-                else
-                    v = dist <= s.cutoff_d ? dist : 1.0;
-            }
-        }
-        else {
-            switch (s.kind) {
-            case RFA_DISTANCE: v = fp_distance(p, s.cutoff_d); break;
-            case RFA_SIMILARITY: v = fp_similarity(p, s.cutoff_d); break;
-            case RFA_NORMALIZED_DISTANCE: v = fp_norm_distance(p, s.cutoff_d); break;
-            default: v = fp_norm_similarity(p, s.cutoff_d); break;
-            }
-        }
-        od[idx] = v;
-        return;
-    }
     IntPair p{&s, l1, l2, r.v};
     if (s.metric == RFA_RATIO) {
         if (s.empty_zero && (l1 == 0 || l2 == 0)) {
@@ -363,6 +409,16 @@ __device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64
         od[idx] = int_api_norm_similarity(p, s.cutoff_d);
 }
 
+// Writes the reference API value for one pair into out[idx] (uint64 or double).
+__device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r, void* out,
+                                         uint64_t idx)
+{
+    if (s.metric == RFA_JARO || s.metric == RFA_JARO_WINKLER)
+        ((double*)out)[idx] = finalize_jaro(s, l1, l2, r);
+    else
+        finalize_intmetric(s, l1, l2, r, out, idx);
+}
+
 // The same value as its 64 result bits (uint64 or the double's bit pattern).
 __device__ __forceinline__ uint64_t finalize_bits(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r)
 {
@@ -372,6 +428,23 @@ __device__ __forceinline__ uint64_t finalize_bits(const ScoreSpec& s, uint64_t l
     } v;
     v.u = 0;
     finalize(s, l1, l2, r, &v, 0);
+    return v.u;
+}
+
+// finalize_bits for a kernel whose recurrence fixes the metric family: the Jaro
+// epilogue only in Jaro kernels, the integer-metric epilogue everywhere else.
+template <int ALGO>
+__device__ __forceinline__ uint64_t finalize_bits_algo(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r)
+{
+    union {
+        uint64_t u;
+        double d;
+    } v;
+    v.u = 0;
+    if constexpr (ALGO == ALGO_JARO)
+        v.d = finalize_jaro(s, l1, l2, r);
+    else
+        finalize_intmetric(s, l1, l2, r, &v, 0);
     return v.u;
 }
 

@@ -35,6 +35,16 @@ struct SymLookup<uint32_t> {
 constexpr uint32_t TILE_WAVES = 8;
 constexpr uint32_t TILE_THREADS = TILE_WAVES * 64;
 
+// LDS after the PM rows: the symbol map (row table or hash) and the scan scratch.
+// A multiple of 8 bytes for every hash size.
+__host__ __device__ constexpr uint32_t tile_extra_bytes(int char_width, uint32_t hash_bits)
+{
+    return char_width == 1 ? 256 * 4 + 256 + 4 * TILE_WAVES : (1u << hash_bits) * 10 + 16 + 4 * TILE_WAVES;
+}
+
+// LDS after the symbol map: two 64-bit words per lane of length-settled flags.
+constexpr uint32_t TILE_SETTLE_BYTES = 64 * 2 * 8;
+
 // Builds the tile's PM rows in LDS.  Returns the lookup functor.
 template <typename CharT, typename W>
 __device__ __forceinline__ SymLookup<CharT> build_tile_pm(const CharT* __restrict__ q, uint64_t qo, uint32_t m,
