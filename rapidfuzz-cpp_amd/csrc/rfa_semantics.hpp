@@ -354,17 +354,62 @@ __device__ __forceinline__ double finalize_jaro(const ScoreSpec& s, uint64_t l1,
 
 // The integer-valued metrics (Levenshtein, Indel, LCSseq, OSA, ratio, the
 // positional metrics): writes the value for one pair into out[idx] (uint64 or double).
-__device__ __forceinline__ void finalize_intmetric(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r, void* out,
-                                                   uint64_t idx)
+// Normalized scores (and ratio) of Cached scorers whose maximum is max(l1, l2)
+// or l1 + l2 (Levenshtein with unit weights, OSA, Indel, LCSseq, ratio) for a
+// cutoff c' in [0, 1] in distance units: the chain of int_api_norm_distance /
+// int_api_norm_similarity with 32-bit conversions and the cut decided before
+// the division.  A cut distance d > cd = ceil(max * c') makes the reference
+// divide cd + 1 (or more) by max; cd + 1 exceeds max * c' by nearly 1, so the
+// quotient exceeds c' by far more than its rounding error and the distance
+// becomes 1.0 (the similarity 0.0).  Returns false when the case is not covered.
+__device__ __forceinline__ bool norm_fast(const ScoreSpec& s, uint64_t l1, uint64_t l2, uint64_t raw, double& out)
 {
-    uint64_t* ou = (uint64_t*)out;
-    double* od = (double*)out;
+    if (s.semantics != RFA_CACHED) return false;
+    const bool ratio = s.metric == RFA_RATIO;
+    if (!ratio && s.kind != RFA_NORMALIZED_DISTANCE && s.kind != RFA_NORMALIZED_SIMILARITY) return false;
+    const bool lcs_raw = ratio || s.metric == RFA_INDEL || s.metric == RFA_LCSSEQ;
+    const bool sum_max = ratio || s.metric == RFA_INDEL;
+    const bool lev_unit = s.metric == RFA_LEVENSHTEIN && s.ins == 1 && s.del == 1 && s.rep == 1;
+    if (!(lcs_raw || lev_unit || s.metric == RFA_OSA)) return false;
+    const double c_in = ratio ? s.cutoff_d / 100 : s.cutoff_d;
+    const bool sim = ratio || s.kind == RFA_NORMALIZED_SIMILARITY;
+    const double cp = sim ? normsim_to_normdist(c_in) : c_in;
+    if (!(cp >= 0.0 && cp <= 1.0)) return false;
+    const uint64_t M64 = sum_max ? l1 + l2 : u64max(l1, l2);
+    if (M64 > 0xffffffffull) return false;
+    const uint32_t M = (uint32_t)M64;
+    // the exact distance: the raw value, or the maximum less one or two LCS lengths
+    const uint32_t r32 = (uint32_t)raw;
+    const uint32_t d = lcs_raw ? M - (sum_max ? r32 + r32 : r32) : r32;
+    const uint32_t cd = (uint32_t)ceil((double)M * cp);
+    double norm = 1.0;
+    if (d <= cd) {
+        norm = M ? (double)d / (double)M : 0.0;
+        norm = (norm <= cp) ? norm : 1.0;
+    }
+    if (!sim) {
+        out = norm;
+        return true;
+    }
+    const double ns = 1.0 - norm;
+    const double v = (ns >= c_in) ? ns : 0.0;
+    out = ratio ? v * 100 : v;
+    return true;
+}
+
+__device__ __forceinline__ uint64_t dbits(double d) { return __builtin_bit_cast(uint64_t, d); }
+
+__device__ __forceinline__ uint64_t finalize_intmetric(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r)
+{
     IntPair p{&s, l1, l2, r.v};
+    if (s.metric == RFA_RATIO && s.empty_zero && (l1 == 0 || l2 == 0)) {
+        return dbits(0.0); // CachedQRatio::similarity
+    }
+    {
+        double v;
+        if (norm_fast(s, l1, l2, r.v, v)) return dbits(v);
+    }
     if (s.metric == RFA_RATIO) {
-        if (s.empty_zero && (l1 == 0 || l2 == 0)) {
-            od[idx] = 0.0; // CachedQRatio::similarity
-            return;
-        }
         // ratio = Indel normalized similarity * 100 (Cached: with the +1e-5 widening)
         ScoreSpec t = s;
         t.metric = RFA_INDEL;
@@ -375,60 +420,44 @@ __device__ __forceinline__ void finalize_intmetric(const ScoreSpec& s, uint64_t 
             norm = norm <= 1.0 ? norm : 1.0;
             double ns = 1.0 - norm;
             double c = s.cutoff_d / 100.0;
-            od[idx] = (ns >= c ? ns : 0.0) * 100.0;
+            return dbits((ns >= c ? ns : 0.0) * 100.0);
         }
-        else
-            od[idx] = int_api_norm_similarity(q, s.cutoff_d / 100) * 100;
-        return;
+        return dbits(int_api_norm_similarity(q, s.cutoff_d / 100) * 100);
     }
-    if (s.kind == RFA_DISTANCE || s.kind == RFA_SIMILARITY) {
-        ou[idx] = finalize_int(s, l1, l2, r.v);
-        return;
-    }
+    if (s.kind == RFA_DISTANCE || s.kind == RFA_SIMILARITY) return finalize_int(s, l1, l2, r.v);
     if (s.semantics == RFA_MULTI) {
         uint64_t maximum = int_maximum(p);
         uint64_t d = multi_full_distance(p);
         switch (s.kind) {
         case RFA_NORMALIZED_DISTANCE: {
             double norm = maximum ? (double)d / (double)maximum : 0.0;
-            od[idx] = norm <= s.cutoff_d ? norm : 1.0;
-            return;
+            return dbits(norm <= s.cutoff_d ? norm : 1.0);
         }
         default: {
             double norm = maximum ? (double)d / (double)maximum : 0.0;
             norm = norm <= 1.0 ? norm : 1.0;
             double ns = 1.0 - norm;
-            od[idx] = ns >= s.cutoff_d ? ns : 0.0;
-            return;
+            return dbits(ns >= s.cutoff_d ? ns : 0.0);
         }
         }
     }
-    if (s.kind == RFA_NORMALIZED_DISTANCE)
-        od[idx] = int_api_norm_distance(p, s.cutoff_d);
-    else
-        od[idx] = int_api_norm_similarity(p, s.cutoff_d);
+    if (s.kind == RFA_NORMALIZED_DISTANCE) return dbits(int_api_norm_distance(p, s.cutoff_d));
+    return dbits(int_api_norm_similarity(p, s.cutoff_d));
 }
 
-// Writes the reference API value for one pair into out[idx] (uint64 or double).
+// The reference API value of one pair as its 64 result bits (uint64 or the
+// double's bit pattern).
+__device__ __forceinline__ uint64_t finalize_bits(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r)
+{
+    if (s.metric == RFA_JARO || s.metric == RFA_JARO_WINKLER) return dbits(finalize_jaro(s, l1, l2, r));
+    return finalize_intmetric(s, l1, l2, r);
+}
+
+// Writes the value into out[idx] (8-byte elements: uint64 or double).
 __device__ __forceinline__ void finalize(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r, void* out,
                                          uint64_t idx)
 {
-    if (s.metric == RFA_JARO || s.metric == RFA_JARO_WINKLER)
-        ((double*)out)[idx] = finalize_jaro(s, l1, l2, r);
-    else
-        finalize_intmetric(s, l1, l2, r, out, idx);
-}
-
-// The same value as its 64 result bits (uint64 or the double's bit pattern).
-__device__ __forceinline__ uint64_t finalize_bits(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r)
-{
-    union {
-        uint64_t u;
-        double d;
-    } v;
-    v.u = 0;
-    finalize(s, l1, l2, r, &v, 0);
-    return v.u;
+    ((uint64_t*)out)[idx] = finalize_bits(s, l1, l2, r);
 }
 
 // finalize_bits for a kernel whose recurrence fixes the metric family: the Jaro
@@ -436,16 +465,10 @@ __device__ __forceinline__ uint64_t finalize_bits(const ScoreSpec& s, uint64_t l
 template <int ALGO>
 __device__ __forceinline__ uint64_t finalize_bits_algo(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r)
 {
-    union {
-        uint64_t u;
-        double d;
-    } v;
-    v.u = 0;
     if constexpr (ALGO == ALGO_JARO)
-        v.d = finalize_jaro(s, l1, l2, r);
+        return dbits(finalize_jaro(s, l1, l2, r));
     else
-        finalize_intmetric(s, l1, l2, r, &v, 0);
-    return v.u;
+        return finalize_intmetric(s, l1, l2, r);
 }
 
 } // namespace rfa

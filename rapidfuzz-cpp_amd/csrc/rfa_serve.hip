@@ -230,8 +230,7 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a)
                 if (cidx != 0xffffffffu) a.dense_out[(uint64_t)qr * a.dense_stride + cidx] = finalize_bits(sp, qm[q], n, r);
                 continue;
             }
-            double score;
-            finalize(a.spec, qm[q], n, r, &score, 0);
+            const double score = __builtin_bit_cast(double, finalize_bits(a.spec, qm[q], n, r));
             const bool pass = cidx != 0xffffffffu && score >= a.cutoff;
             const uint64_t bal = ballot(pass);
             if (!bal) continue;

@@ -249,3 +249,35 @@ def test_normalization_tile_grid(ctx, oracle, metric, kind, c):
     got = sc.cdist(ctx.strings(strs, width=1), kind, cutoff_d=c)
     exp = oracle.cdist(metric, kind, strs, strs, cutoff_d=c)
     assert _same(got, exp)
+
+
+@pytest.mark.parametrize("metric", [0, 1, 3, 6])
+@pytest.mark.parametrize("kind,c", [(2, 0.2), (2, 1.0 / 3.0), (2, 1.0), (3, 0.8), (3, 2.0 / 3.0), (3, 0.0),
+                                    (3, 1.0)])
+def test_normalization_levenshtein_family(ctx, metric, kind, c):
+    """The normalized epilogue of Levenshtein / OSA (d = |i - j|, maximum max(i, j)), Indel
+    (maximum i + j) and ratio over 'a'^i x 'a'^j, i, j <= 1024: every (d, maximum) those give,
+    through the tile kernels (<= 64 units) and the block kernel (longer queries)."""
+    import rapidfuzz_amd as rfa
+
+    if metric == 6 and kind != 3:
+        pytest.skip("ratio is a similarity")
+    n = 1024
+    strs = [np.full(i, 97, dtype=np.uint8) for i in range(n + 1)]
+    s = ctx.strings(strs, width=1)
+    sc = rfa.Scorer(ctx, metric, s)
+    i = np.arange(n + 1, dtype=np.int64)
+    d = np.abs(i[:, None] - i[None, :]).astype(np.uint64)
+    M = (i[:, None] + i[None, :] if metric in (1, 6) else np.maximum(i[:, None], i[None, :])).astype(np.uint64)
+    if metric == 6:
+        got = sc.cdist(s, 1, cutoff_d=c * 100)
+        exp = _norm.norm_similarity(d, M, (c * 100) / 100) * 100
+    else:
+        got = sc.cdist(s, kind, cutoff_d=c)
+        exp = (_norm.norm_distance if kind == 2 else _norm.norm_similarity)(d, M, c)
+    sc.close()
+    s.close()
+    if not np.array_equal(got.view(np.uint64), exp.view(np.uint64)):
+        bad = np.argwhere(got.view(np.uint64) != exp.view(np.uint64))
+        a, b = bad[0]
+        raise AssertionError(f"{len(bad)} differ; first i={a} j={b}: got {got[a, b]!r} expected {exp[a, b]!r}")

@@ -70,3 +70,25 @@ def test_grid_catches_plausible_mistakes(oracle):
     assert not _same(sim_without_widening(0.8), exp_sim)
     assert not _same(dist_strict_cut(0.2), exp_dist)
     assert not _same(dist_empty_as_one(0.2), exp_dist)
+
+
+@pytest.mark.parametrize("metric", [0, 1, 3, 6])
+@pytest.mark.parametrize("kind,c", [(2, 0.2), (2, 1.0 / 3.0), (3, 0.8), (3, 2.0 / 3.0), (3, 0.0)])
+def test_levenshtein_family_closed_form(oracle, metric, kind, c):
+    """The same closed forms for 'a'^i against 'a'^j under Levenshtein / OSA (d = |i - j|, maximum
+    max(i, j)), Indel (d = |i - j|, maximum i + j) and ratio (Indel normalized similarity at c / 100,
+    times 100): the expected values of the GPU's normalized epilogue."""
+    n = 120
+    strs, d, M = _grid(n)
+    if metric in (1, 6):
+        i = np.arange(n + 1, dtype=np.uint64)
+        M = i[:, None] + i[None, :]
+    if metric == 6:
+        if kind != 3:
+            pytest.skip("ratio is a similarity")
+        exp = oracle.cdist(6, 1, strs, strs, cutoff_d=c * 100)
+        assert _same(_norm.norm_similarity(d, M, c) * 100, exp)
+        return
+    exp = oracle.cdist(metric, kind, strs, strs, cutoff_d=c)
+    f = _norm.norm_distance if kind == 2 else _norm.norm_similarity
+    assert _same(f(d, M, c), exp)
