@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "rfa_host.hpp"
+#include "rfa_semantics.hpp"
 
 namespace rfa {
 
@@ -1070,6 +1071,11 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ta.out = dout;
         ta.qcut = d_qcut;
         ta.settle = cutoff_may_settle(spec, d_qcut != nullptr) ? 1u : 0u;
+        {
+            // the fast normalized epilogue computes maxima in 32 bits
+            NormEpi ne;
+            ta.norm_epi = norm_epi_setup(spec, ne) && (uint64_t)64 + ch->max_len <= 0xffffffffull ? 1u : 0u;
+        }
         if (ta.ncsel) {
             for (const TileClass& tc : sc->classes) {
                 ta.tile_base = tc.first_tile;

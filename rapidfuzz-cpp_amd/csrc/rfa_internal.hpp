@@ -26,6 +26,7 @@ struct TileArgs {
     void* out;
     const uint64_t* qcut; // per-query integer cutoff (nullptr: spec.cutoff_u)
     uint32_t settle;      // the cutoffs may decide results by length alone: test before each recurrence
+    uint32_t norm_epi;    // the normalized kinds' fast epilogue covers the launch (norm_epi_setup)
 };
 
 size_t tile_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t hash_bits);

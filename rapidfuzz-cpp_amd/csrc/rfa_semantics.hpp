@@ -133,7 +133,12 @@ __device__ __forceinline__ double int_api_norm_distance(const IntPair& p, double
     return (norm <= c) ? norm : 1.0;
 }
 
-__device__ __forceinline__ double normsim_to_normdist(double c) { return dmin(1.0, 1.0 - c + 0.00001); }
+// NormSim_to_NormDist (common.hpp:50-53): std::min(1.0, 1 - c + 1e-5), NaN giving 1.0 as there
+__device__ __forceinline__ double normsim_to_normdist(double c)
+{
+    const double w = 1.0 - c + 0.00001;
+    return (w < 1.0) ? w : 1.0;
+}
 
 __device__ __forceinline__ double int_api_norm_similarity(const IntPair& p, double c)
 {
@@ -283,26 +288,6 @@ __device__ __forceinline__ uint64_t raw_range_hi(uint64_t m, uint64_t n)
         return m > n ? m : n;
 }
 
-__device__ __forceinline__ uint64_t finalize_int(const ScoreSpec& s, uint64_t l1, uint64_t l2, uint64_t raw);
-template <int ALGO>
-__device__ __forceinline__ uint64_t finalize_bits_algo(const ScoreSpec& s, uint64_t l1, uint64_t l2, RawResult r);
-
-// True when the result of (m, n) is the same for every raw value of its range,
-// so the recurrence can be skipped (the reference's length filters:
-// Levenshtein_impl.hpp:921-922, LCSseq_impl.hpp:384-389, OSA's |m - n| bound).
-// Every result chain here is monotone in the raw value (the cut values c + 1,
-// 1.0 and 0 lie beyond every uncut value, and the double steps round
-// monotonically), so equal results at both ends mean equal results throughout.
-template <int ALGO, bool INTK>
-__device__ __forceinline__ bool settled_by_length(const ScoreSpec& s, uint64_t m, uint64_t n)
-{
-    const uint64_t lo = raw_range_lo<ALGO>(m, n), hi = raw_range_hi<ALGO>(m, n);
-    if constexpr (INTK)
-        return finalize_int(s, m, n, lo) == finalize_int(s, m, n, hi);
-    else
-        return finalize_bits_algo<ALGO>(s, m, n, RawResult{lo, 0, 0}) ==
-               finalize_bits_algo<ALGO>(s, m, n, RawResult{hi, 0, 0});
-}
 
 // --- dispatch ----------------------------------------------------------------
 
@@ -361,40 +346,108 @@ __device__ __forceinline__ double finalize_jaro(const ScoreSpec& s, uint64_t l1,
 // the division.  A cut distance d > cd = ceil(max * c') makes the reference
 // divide cd + 1 (or more) by max; cd + 1 exceeds max * c' by nearly 1, so the
 // quotient exceeds c' by far more than its rounding error and the distance
-// becomes 1.0 (the similarity 0.0).  Returns false when the case is not covered.
-__device__ __forceinline__ bool norm_fast(const ScoreSpec& s, uint64_t l1, uint64_t l2, uint64_t raw, double& out)
+// becomes 1.0 (the similarity 0.0).  NormEpi holds the launch-level choices;
+// the per-pair steps are straight-line selects.
+struct NormEpi {
+    bool lcs_raw; // the raw value is an LCS length (Indel, LCSseq, ratio), else a distance
+    bool sum_max; // the maximum is l1 + l2 (Indel, ratio), else max(l1, l2)
+    bool sim;     // normalized similarity (or ratio), else normalized distance
+    bool ratio;   // times 100
+    double cp;    // the cutoff in distance units
+    double c_in;  // the similarity cutoff (sim)
+};
+
+// Whether the fast chain covers the launch; fills e.  (Also used on the host.)
+__device__ __host__ inline bool norm_epi_setup(const ScoreSpec& s, NormEpi& e)
 {
     if (s.semantics != RFA_CACHED) return false;
-    const bool ratio = s.metric == RFA_RATIO;
-    if (!ratio && s.kind != RFA_NORMALIZED_DISTANCE && s.kind != RFA_NORMALIZED_SIMILARITY) return false;
-    const bool lcs_raw = ratio || s.metric == RFA_INDEL || s.metric == RFA_LCSSEQ;
-    const bool sum_max = ratio || s.metric == RFA_INDEL;
+    e.ratio = s.metric == RFA_RATIO;
+    if (!e.ratio && s.kind != RFA_NORMALIZED_DISTANCE && s.kind != RFA_NORMALIZED_SIMILARITY) return false;
+    if (e.ratio && s.empty_zero) return false;
+    e.lcs_raw = e.ratio || s.metric == RFA_INDEL || s.metric == RFA_LCSSEQ;
+    e.sum_max = e.ratio || s.metric == RFA_INDEL;
     const bool lev_unit = s.metric == RFA_LEVENSHTEIN && s.ins == 1 && s.del == 1 && s.rep == 1;
-    if (!(lcs_raw || lev_unit || s.metric == RFA_OSA)) return false;
-    const double c_in = ratio ? s.cutoff_d / 100 : s.cutoff_d;
-    const bool sim = ratio || s.kind == RFA_NORMALIZED_SIMILARITY;
-    const double cp = sim ? normsim_to_normdist(c_in) : c_in;
-    if (!(cp >= 0.0 && cp <= 1.0)) return false;
-    const uint64_t M64 = sum_max ? l1 + l2 : u64max(l1, l2);
-    if (M64 > 0xffffffffull) return false;
-    const uint32_t M = (uint32_t)M64;
-    // the exact distance: the raw value, or the maximum less one or two LCS lengths
-    const uint32_t r32 = (uint32_t)raw;
-    const uint32_t d = lcs_raw ? M - (sum_max ? r32 + r32 : r32) : r32;
-    const uint32_t cd = (uint32_t)ceil((double)M * cp);
+    if (!(e.lcs_raw || lev_unit || s.metric == RFA_OSA)) return false;
+    e.c_in = e.ratio ? s.cutoff_d / 100 : s.cutoff_d;
+    e.sim = e.ratio || s.kind == RFA_NORMALIZED_SIMILARITY;
+    // NormSim_to_NormDist (common.hpp:50-53)
+    const double widened = 1.0 - e.c_in + 0.00001;
+    e.cp = e.sim ? (widened < 1.0 ? widened : 1.0) : e.c_in;
+    return e.cp >= 0.0 && e.cp <= 1.0;
+}
+
+__device__ __forceinline__ uint32_t norm_epi_max(const NormEpi& e, uint32_t l1, uint32_t l2)
+{
+    return e.sum_max ? l1 + l2 : max(l1, l2);
+}
+
+// the largest distance that is not cut
+__device__ __forceinline__ uint32_t norm_epi_cut(const NormEpi& e, uint32_t M)
+{
+    return (uint32_t)ceil((double)M * e.cp);
+}
+
+// the exact distance: the raw value, or the maximum less one or two LCS lengths
+__device__ __forceinline__ uint32_t norm_epi_dist(const NormEpi& e, uint32_t M, uint32_t raw)
+{
+    return e.lcs_raw ? M - (e.sum_max ? raw + raw : raw) : raw;
+}
+
+__device__ __forceinline__ double norm_epi_value(const NormEpi& e, uint32_t M, uint32_t d, uint32_t cd)
+{
     double norm = 1.0;
     if (d <= cd) {
         norm = M ? (double)d / (double)M : 0.0;
-        norm = (norm <= cp) ? norm : 1.0;
+        norm = (norm <= e.cp) ? norm : 1.0;
     }
-    if (!sim) {
-        out = norm;
-        return true;
-    }
+    if (!e.sim) return norm;
     const double ns = 1.0 - norm;
-    const double v = (ns >= c_in) ? ns : 0.0;
-    out = ratio ? v * 100 : v;
+    const double v = (ns >= e.c_in) ? ns : 0.0;
+    return e.ratio ? v * 100 : v;
+}
+
+__device__ __forceinline__ bool norm_fast(const ScoreSpec& s, uint64_t l1, uint64_t l2, uint64_t raw, double& out)
+{
+    NormEpi e;
+    if (!norm_epi_setup(s, e)) return false;
+    const uint64_t M64 = e.sum_max ? l1 + l2 : u64max(l1, l2);
+    if (M64 > 0xffffffffull) return false;
+    const uint32_t M = (uint32_t)M64;
+    out = norm_epi_value(e, M, norm_epi_dist(e, M, (uint32_t)raw), norm_epi_cut(e, M));
     return true;
+}
+
+// For an LCS-valued pair of lengths m, n (Indel, ratio, LCSseq; Cached
+// semantics): a length T such that every LCS below T gives the same result,
+// the cut value (an Indel distance M - 2 LCS above the integer cutoff c, an
+// LCSseq distance max - LCS above c, a similarity below its cutoff), or 0 when
+// no such bound applies.  The normalized kinds use the integer cutoff of
+// norm_fast, where a cut distance gives 1.0 / 0.0.
+__device__ __forceinline__ uint32_t lcs_cut_below(const ScoreSpec& s, uint64_t m, uint64_t n)
+{
+    if (s.semantics != RFA_CACHED) return 0;
+    const bool ratio = s.metric == RFA_RATIO;
+    if (!ratio && s.metric != RFA_INDEL && s.metric != RFA_LCSSEQ) return 0;
+    if (ratio && s.empty_zero) return 0;
+    const bool sum_max = s.metric != RFA_LCSSEQ;
+    const uint64_t M = sum_max ? m + n : u64max(m, n);
+    uint64_t c; // the largest distance that is not cut
+    if (!ratio && s.kind == RFA_DISTANCE) {
+        c = s.cutoff_u;
+    }
+    else if (!ratio && s.kind == RFA_SIMILARITY) {
+        if (s.metric == RFA_LCSSEQ) return (uint32_t)u64min(s.cutoff_u, 0xffffffffull); // LCS < c gives 0
+        if (s.cutoff_u > M) return 0;
+        c = M - s.cutoff_u; // int_api_similarity: cut where d > M - c
+    }
+    else {
+        NormEpi e;
+        if (!norm_epi_setup(s, e) || M > 0xffffffffull) return 0;
+        c = norm_epi_cut(e, (uint32_t)M);
+    }
+    if (M <= c) return 0;
+    // cut where the distance exceeds c: M - 2 LCS > c (Indel), max - LCS > c (LCSseq)
+    return (uint32_t)(sum_max ? (M - c + 1) / 2 : M - c);
 }
 
 __device__ __forceinline__ uint64_t dbits(double d) { return __builtin_bit_cast(uint64_t, d); }

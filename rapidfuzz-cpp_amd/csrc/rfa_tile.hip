@@ -1,7 +1,7 @@
 // rfa_tile.hip -- query-tile kernel: 64 short queries (one per lane) against a
 // stream of choices that is uniform across the wave.
 //
-// Shape.  A workgroup of 4 waves owns one tile of up to 64 queries, each at
+// Shape.  A workgroup of 8 waves owns one tile of up to 64 queries, each at
 // most 64 code units (the reference's single-word paths:
 // Levenshtein_impl.hpp:259-322, LCSseq_impl.hpp:205-260 with N=1,
 // OSA_impl.hpp:34-74, Jaro_impl.hpp:94-132 and 258-276).  The tile's
@@ -28,8 +28,12 @@ namespace rfa {
 // scheduler kept from pulling a load down to its use), so an LDS read's latency
 // overlaps two steps of the wave's own work; the last 0-3 positions follow.
 // Loads past cnt read row 63's word and are never stepped.
-template <typename W, typename Step>
-__device__ __forceinline__ void pipelined_rows(const W* __restrict__ col, uint32_t row, uint32_t cnt, Step& step)
+// stop(done) is asked after each whole group with the number of positions
+// done so far; true ends the pass (the rest of the positions are not stepped)
+// and the function returns true.
+template <typename W, typename Step, typename Stop>
+__device__ __forceinline__ bool pipelined_rows(const W* __restrict__ col, uint32_t row, uint32_t cnt, Step& step,
+                                               Stop& stop)
 {
     auto ld = [&](uint32_t i) -> W { return col[readlane(row, i < 63u ? i : 63u) * 64]; };
     W x0 = ld(0), x1 = ld(1);
@@ -45,6 +49,7 @@ __device__ __forceinline__ void pipelined_rows(const W* __restrict__ col, uint32
         __builtin_amdgcn_sched_barrier(0);
         step(x2, i + 2);
         step(x3, i + 3);
+        if (stop(i + 4)) return true;
     }
     if (i < cnt) {
         step(x0, i);
@@ -53,6 +58,14 @@ __device__ __forceinline__ void pipelined_rows(const W* __restrict__ col, uint32
             if (i + 2 < cnt) step(ld(i + 2), i + 2);
         }
     }
+    return false;
+}
+
+template <typename W, typename Step>
+__device__ __forceinline__ void pipelined_rows(const W* __restrict__ col, uint32_t row, uint32_t cnt, Step& step)
+{
+    auto never = [](uint32_t) { return false; };
+    pipelined_rows<W>(col, row, cnt, step, never);
 }
 
 constexpr uint32_t T_AND_ANDN = (TA & TB & ~TC) & 0xFF; // a & b & ~c
@@ -138,9 +151,35 @@ __device__ __forceinline__ RawResult jaro_word(const W* __restrict__ col, uint32
     return res;
 }
 
-// INTK: the result kind is integer-valued (the integer epilogue alone is
-// compiled in, which keeps the kernel's register footprint small).
-template <int ALGO, typename CharT, typename W, bool INTK>
+// EPI selects the epilogue compiled in: EPI_INT for the integer-valued kinds
+// (which keeps the register footprint small), EPI_NORM for the normalized
+// kinds whose launch-level choices NormEpi makes once (Cached Levenshtein with
+// unit weights, OSA, Indel, LCSseq, ratio), EPI_ANY for the rest.
+enum : int { EPI_INT = 0, EPI_NORM = 1, EPI_ANY = 2 };
+
+template <int ALGO, int EPI>
+__device__ __forceinline__ uint64_t tile_result(const ScoreSpec& spec, const NormEpi& ne, uint32_t m, uint32_t n,
+                                                RawResult r)
+{
+    if constexpr (EPI == EPI_INT)
+        return finalize_int(spec, m, n, r.v);
+    else if constexpr (EPI == EPI_NORM) {
+        const uint32_t M = norm_epi_max(ne, m, n);
+        return dbits(norm_epi_value(ne, M, norm_epi_dist(ne, M, (uint32_t)r.v), norm_epi_cut(ne, M)));
+    }
+    else
+        return finalize_bits_algo<ALGO>(spec, m, n, r);
+}
+
+template <int ALGO, int EPI>
+__device__ __forceinline__ bool tile_settled(const ScoreSpec& spec, const NormEpi& ne, uint32_t m, uint32_t n)
+{
+    const uint64_t lo = raw_range_lo<ALGO>(m, n), hi = raw_range_hi<ALGO>(m, n);
+    return tile_result<ALGO, EPI>(spec, ne, m, n, RawResult{lo, 0, 0}) ==
+           tile_result<ALGO, EPI>(spec, ne, m, n, RawResult{hi, 0, 0});
+}
+
+template <int ALGO, typename CharT, typename W, int EPI>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
 {
     extern __shared__ uint64_t smem_u64[];
@@ -160,6 +199,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
     SymLookup<CharT> look = build_tile_pm<CharT, W>(q, qo, m, pm, extra, a);
     ScoreSpec spec = a.spec;
     if (a.qcut && qact) spec.cutoff_u = a.qcut[qidx];
+    NormEpi ne{};
+    if constexpr (EPI == EPI_NORM) norm_epi_setup(a.spec, ne); // the host checked that it applies
 
     // Choice lengths whose result this lane's length alone decides (bit n of
     // settle_lo for n < 64, settle_64 for n = 64; longer choices are tested
@@ -176,9 +217,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
             uint64_t mine = 0;
 #pragma unroll 1
             for (uint32_t n = wave; n < 64; n += TILE_WAVES)
-                mine |= (uint64_t)settled_by_length<ALGO, INTK>(spec, m, n) << n;
+                mine |= (uint64_t)tile_settled<ALGO, EPI>(spec, ne, m, n) << n;
             atomicOr((unsigned long long*)&sl[lane], (unsigned long long)mine);
-            if (wave == TILE_WAVES - 1) sl[64 + lane] = settled_by_length<ALGO, INTK>(spec, m, 64);
+            if (wave == TILE_WAVES - 1) sl[64 + lane] = tile_settled<ALGO, EPI>(spec, ne, m, 64);
             __syncthreads();
             settle_lo = sl[lane];
             settle_64 = sl[64 + lane] != 0;
@@ -200,7 +241,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
     // holds choice t's index, offset and length (loaded a run ahead), choice
     // t + 1's first 64 code units are loaded while choice t runs, and its PM
     // rows are looked up before choice t's epilogue.
-    constexpr uint32_t G = INTK ? 16 : 4;
+    constexpr uint32_t G = EPI == EPI_INT ? 8 : 4;
     const bool direct = a.corder != nullptr;
     uint32_t g0 = it0 + uniform(wave) * G;
     if (g0 >= it1) return;
@@ -257,7 +298,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
             if constexpr (ALGO != ALGO_JARO) {
                 if (a.settle) {
                     const bool st = n < 64 ? ((settle_lo >> n) & 1) != 0
-                                           : (n == 64 ? settle_64 : settled_by_length<ALGO, INTK>(spec, m, n));
+                                           : (n == 64 ? settle_64 : tile_settled<ALGO, EPI>(spec, ne, m, n));
                     skip = ballot(!st) == 0;
                     if (skip) res.v = raw_range_lo<ALGO>(m, n); // any value of the range gives this result
                 }
@@ -286,11 +327,51 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
                             lcs_step((uint64_t)X, VPl, VPh);
                     }
                 };
-                if (n) pipelined_rows<W>(col, row, min(64u, n), step);
-                for (uint32_t k0 = 64; k0 < n; k0 += 64) {
+                // LCS: when no lane can reach its threshold any more (the LCS so far
+                // plus the positions left), every result is already the cut value
+                uint32_t need = 0;
+                bool can_stop = false;
+                if constexpr (ALGO == ALGO_LCS) {
+                    if (a.settle) {
+                        if constexpr (EPI == EPI_NORM) {
+                            // an LCS below this gives a distance above the cut
+                            const uint32_t M = norm_epi_max(ne, m, n), cd = norm_epi_cut(ne, M);
+                            need = M <= cd ? 0u : (ne.sum_max ? (M - cd + 1) / 2 : M - cd);
+                        }
+                        else {
+                            need = lcs_cut_below(spec, m, n);
+                        }
+                        if (!qact) need = ~0u; // idle lanes never block a stop
+                        can_stop = ballot(need == 0) == 0;
+                    }
+                }
+                uint32_t k0 = 0;
+                auto stop = [&](uint32_t done) -> bool {
+                    if constexpr (ALGO == ALGO_LCS) {
+                        if (!can_stop) return false;
+                        const W S = (sizeof(W) == 8) ? (W)(((uint64_t)VPh << 32) | VPl) : (W)VPl;
+                        const uint32_t bound = (uint32_t)popc<W>(~S) + (n - k0 - done);
+                        return ballot(bound >= need) == 0;
+                    }
+                    else {
+                        return false;
+                    }
+                };
+                // (the test sits in its own copy of the loop, so runs without it pay nothing)
+                bool stopped = false;
+                if (can_stop) {
+                    if (n) stopped = pipelined_rows<W>(col, row, min(64u, n), step, stop);
+                }
+                else if (n) {
+                    pipelined_rows<W>(col, row, min(64u, n), step);
+                }
+                for (k0 = 64; k0 < n && !stopped; k0 += 64) {
                     const uint32_t cnt = uniform(min(64u, n - k0));
                     const uint32_t rk = text_row(text_unit(co + k0, cnt), cnt);
-                    pipelined_rows<W>(col, rk, cnt, step);
+                    if (can_stop)
+                        stopped = pipelined_rows<W>(col, rk, cnt, step, stop);
+                    else
+                        pipelined_rows<W>(col, rk, cnt, step);
                 }
                 const W VP = (sizeof(W) == 8) ? (W)(((uint64_t)VPh << 32) | VPl) : (W)VPl;
                 const W VN = (sizeof(W) == 8) ? (W)(((uint64_t)VNh << 32) | VNl) : (W)VNl;
@@ -307,11 +388,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
                 res = jaro_word<W>(col, row, m, n);
             }
             const uint32_t row_nx = text_row(u_nx, n_nx);
-            uint64_t bits;
-            if constexpr (INTK)
-                bits = finalize_int(spec, m, n, res.v);
-            else
-                bits = finalize_bits_algo<ALGO>(spec, m, n, res);
+            const uint64_t bits = tile_result<ALGO, EPI>(spec, ne, m, n, res);
             if (direct) {
                 if (qact) ((uint64_t*)a.out)[(uint64_t)qidx * a.out_stride + cidx] = bits;
             }
@@ -351,11 +428,15 @@ static void launch_one(const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x, s
     const bool intk = (a.spec.kind == RFA_DISTANCE || a.spec.kind == RFA_SIMILARITY) && a.spec.metric <= RFA_OSA;
     if constexpr (ALGO != ALGO_JARO) {
         if (intk) {
-            hipLaunchKernelGGL((k_tile<ALGO, CharT, W, true>), grid, dim3(TILE_THREADS), lds, s, a);
+            hipLaunchKernelGGL((k_tile<ALGO, CharT, W, EPI_INT>), grid, dim3(TILE_THREADS), lds, s, a);
+            return;
+        }
+        if (a.norm_epi) {
+            hipLaunchKernelGGL((k_tile<ALGO, CharT, W, EPI_NORM>), grid, dim3(TILE_THREADS), lds, s, a);
             return;
         }
     }
-    hipLaunchKernelGGL((k_tile<ALGO, CharT, W, false>), grid, dim3(TILE_THREADS), lds, s, a);
+    hipLaunchKernelGGL((k_tile<ALGO, CharT, W, EPI_ANY>), grid, dim3(TILE_THREADS), lds, s, a);
 }
 
 template <typename CharT, typename W>

@@ -281,3 +281,36 @@ def test_normalization_levenshtein_family(ctx, metric, kind, c):
         bad = np.argwhere(got.view(np.uint64) != exp.view(np.uint64))
         a, b = bad[0]
         raise AssertionError(f"{len(bad)} differ; first i={a} j={b}: got {got[a, b]!r} expected {exp[a, b]!r}")
+
+
+@pytest.mark.parametrize("metric", [1, 2, 6])
+def test_lcs_cutoff_sweep(ctx, oracle, printable_case, metric):
+    """Indel, LCSseq and ratio at many cutoffs of every kind on the printable tiles: the LCS
+    recurrence stops once no lane can reach the LCS its cutoff needs, and the results must not
+    change.  Choices near a query put exact values on both sides of each threshold."""
+    import rapidfuzz_amd as rfa
+
+    queries, choices = printable_case
+    queries, choices = queries[:128], choices[:1536]
+    qs, cs = ctx.strings(queries, width=1), ctx.strings(choices, width=1)
+    sc = rfa.Scorer(ctx, metric, qs)
+    runs = []
+    if metric == 6:
+        runs = [(1, None, c) for c in (0.0, 30.0, 50.0, 70.0, 85.0, 95.0, 100.0)]
+    else:
+        runs += [(0, c, None) for c in (0, 1, 3, 8, 15, 30, 60)]
+        runs += [(1, c, None) for c in (1, 5, 10, 20, 40)]
+        runs += [(2, None, c) for c in (0.05, 0.15, 0.3, 0.6)]
+        runs += [(3, None, c) for c in (0.4, 0.7, 0.85, 0.95)]
+    for kind, cu, cd in runs:
+        got = sc.cdist(cs, kind, cutoff_u=cu, cutoff_d=cd)
+        exp = oracle.cdist_mt(metric, kind, queries, choices, cutoff_u=cu, cutoff_d=cd)
+        assert _same(got, exp), f"kind {kind} cutoff {cu}/{cd}: " + _first_diff(got, exp, queries, choices)
+    if metric != 6:
+        rng = np.random.default_rng(metric)
+        for kind in (0, 1):
+            cuts = rng.integers(0, 40, size=len(queries)).astype(np.uint64)
+            got = sc.cdist(cs, kind, cutoffs_u=cuts)
+            exp = oracle.cdist_mt(metric, kind, queries, choices, cutoffs_u=cuts)
+            assert _same(got, exp), f"per-query kind {kind}: " + _first_diff(got, exp, queries, choices)
+    sc.close()
