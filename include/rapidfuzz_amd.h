@@ -191,6 +191,21 @@ rfa_status rfa_cdist_cutoffs(rfa_scorer* scorer, const rfa_strings* choices, int
  * number of strings and the same char width. */
 rfa_status rfa_pairs(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics, int kind,
                      uint64_t cutoff_u, double cutoff_d, int schedule, void* out, int out_on_device);
+/* rfa_pairs without any allocation (no reference counterpart): for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  `metrics` (one code per
+ * pair: RFA_LEVENSHTEIN, RFA_INDEL, RFA_LCSSEQ or RFA_OSA) and `out` (8 bytes
+ * per pair) are device memory; the call only enqueues kernels on the context's
+ * stream (rfa_context_stream): no allocation, no copy, no synchronization.
+ * The bucketed schedule's plan lives in a static per-GPU workspace of 48 MiB
+ * held from library load to process end, so one call takes at most 2^20 pairs,
+ * and every pair's shorter string must have at most 4096 units (64 when
+ * has_osa is nonzero: `metrics` may then hold RFA_OSA); larger calls fail with
+ * RFA_ERR_UNSUPPORTED.  Codes outside 0..3 give unspecified values.  Calls that
+ * overlap on one GPU (from other contexts or threads without ordering) share
+ * that workspace and race; calls on one context are ordered by its stream. */
+rfa_status rfa_pairs_noalloc(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const uint8_t* metrics,
+                             int has_osa, int kind, uint64_t cutoff_u, double cutoff_d, void* out);
 /* Autotuner of the bucketed pairs schedule (no reference counterpart): times
  * the schedule on this batch for each candidate RFA_OPTION_PAIRS_CHAIN value
  * (128 .. 2048, and 4097 = the shorter string is always the pattern), keeps the

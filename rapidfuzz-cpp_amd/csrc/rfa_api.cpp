@@ -1167,6 +1167,81 @@ uint64_t* lane_arena(rfa_context* ctx, uint32_t maxw, uint32_t nlanes, uint64_t 
     return arena;
 }
 
+// The bucketed pairs schedule after the metric upload: the device plan, then the
+// segmented kernel (and the short-OSA lane kernels, and the block kernel with its
+// queue on the first stream when some pattern needs it).  Every buffer is the
+// caller's; the function allocates nothing unless need_block (the block kernel's
+// pattern arenas).
+static void pairs_bucketed_launch(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const uint8_t* dmet,
+                                  uint64_t n, const ScoreSpec& base, bool has_osa, bool need_block, uint64_t maxm,
+                                  uint64_t maxn, uint32_t nb, uint32_t* bins, uint32_t* vals, void* temp,
+                                  size_t temp_bytes, uint32_t* counters, void* dout, int kind, uint64_t cu)
+{
+    hipStream_t s = ctx->stream, s2 = ctx->stream2;
+    const int width = a->width;
+    const uint32_t* packed = vals + n;
+    LaneArgs la{};
+    la.achars = a->d_chars;
+    la.aoff = a->d_off;
+    la.bchars = b->d_chars;
+    la.boff = b->d_off;
+    la.spec = base;
+    la.out = dout;
+    HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, ctx->seg_chain_steps, bins, vals, vals + n, temp,
+                          temp_bytes, counters, s));
+    // with block work, the segmented and lane kernels (second stream) overlap the block
+    // queue (first stream); without, everything stays on the first stream
+    hipStream_t sk = s;
+    if (need_block) {
+        HIP_CHECK(hipEventRecord(ctx->fork, s));
+        HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
+        sk = s2;
+        BlockArgs ba{};
+        ba.achars = a->d_chars;
+        ba.aoff = a->d_off;
+        ba.bchars = b->d_chars;
+        ba.boff = b->d_off;
+        ba.item_packed = packed;
+        ba.nitems_dev = counters + 1;
+        ba.pair_metric = dmet;
+        ba.nitems = (uint32_t)n; // upper bound, for the grid size
+        ba.spec = base;
+        ba.out = dout;
+        ba.qgram = kind == RFA_DISTANCE && cu < (1ull << 40) && 7 * cu + 7 < maxn ? 1u : 0u;
+        run_block(ctx, ALGO_MIXED, width, ba, maxm, maxn, false, counters);
+    }
+    SegArgs sa{};
+    sa.achars = a->d_chars;
+    sa.aoff = a->d_off;
+    sa.bchars = b->d_chars;
+    sa.boff = b->d_off;
+    sa.item_packed = packed;
+    sa.pair_metric = dmet;
+    sa.counters = counters;
+    sa.spec = base;
+    sa.out = dout;
+    // no more waves than pairs (each wave holds up to 64 pairs, at least one)
+    const uint32_t max_blocks = (uint32_t)std::min<uint64_t>((n + 3) / 4, 1u << 20);
+    HIP_CHECK(launch_seg(width, nb, sa, (uint32_t)ctx->num_cus, max_blocks, sk));
+    if (has_osa) {
+        la.item_packed = packed;
+        la.pair_metric = dmet;
+        la.nitems = n;
+        // OSA 33-64 unit range [counters[1], counters[2]), then 1-32 units [counters[2], counters[3])
+        const uint32_t nblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 6, (n + 63) / 64);
+        la.begin_dev = counters + 1;
+        la.end_dev = counters + 2;
+        HIP_CHECK(launch_lane_short(width, 64, la, nblk, sk));
+        la.begin_dev = counters + 2;
+        la.end_dev = counters + 3;
+        HIP_CHECK(launch_lane_short(width, 32, la, nblk, sk));
+    }
+    if (need_block) {
+        HIP_CHECK(hipEventRecord(ctx->join, s2));
+        HIP_CHECK(hipStreamWaitEvent(s, ctx->join, 0));
+    }
+}
+
 void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics, int kind,
                 uint64_t cu, double cd, int schedule, void* out, int out_on_device)
 {
@@ -1191,7 +1266,7 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         staged[i] = (uint8_t)m;
     }
     if (bad_metric) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "pair metrics must be Levenshtein, Indel, LCSseq or OSA");
-    hipStream_t s = ctx->stream, s2 = ctx->stream2;
+    hipStream_t s = ctx->stream;
     const size_t bytes = (size_t)n * 8;
     void* dout = out_on_device ? out : ctx->out_tmp.get(bytes);
     const int width = a->width;
@@ -1231,7 +1306,6 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         const size_t temp_bytes = plan_temp_bytes(n);
         void* temp = ctx->plan_temp.get(temp_bytes);
         uint32_t* counters = (uint32_t*)ctx->queue.get(128); // 18 range ends and the block queue, written by the plan
-        const uint32_t* packed = vals + n;
         const uint64_t maxm = std::min(a->max_len, b->max_len);
         const uint64_t maxn = std::max(a->max_len, b->max_len);
         const bool need_block = maxm > 4096 || (has_osa && maxm > 64);
@@ -1239,59 +1313,8 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
         // the launch sequence after the metric upload; everything it reads on the host is in
         // the graph key below
         auto enqueue = [&]() {
-            HIP_CHECK(launch_plan(a->d_off, b->d_off, dmet, n, ctx->seg_chain_steps, bins, vals, vals + n, temp,
-                                  temp_bytes, counters, s));
-            // with block work, the segmented and lane kernels (second stream) overlap the block
-            // queue (first stream); without, everything stays on the first stream
-            hipStream_t sk = s;
-            if (need_block) {
-                HIP_CHECK(hipEventRecord(ctx->fork, s));
-                HIP_CHECK(hipStreamWaitEvent(s2, ctx->fork, 0));
-                sk = s2;
-                BlockArgs ba{};
-                ba.achars = a->d_chars;
-                ba.aoff = a->d_off;
-                ba.bchars = b->d_chars;
-                ba.boff = b->d_off;
-                ba.item_packed = packed;
-                ba.nitems_dev = counters + 1;
-                ba.pair_metric = dmet;
-                ba.nitems = (uint32_t)n; // upper bound, for the grid size
-                ba.spec = base;
-                ba.out = dout;
-                ba.qgram = kind == RFA_DISTANCE && cu < (1ull << 40) && 7 * cu + 7 < maxn ? 1u : 0u;
-                run_block(ctx, ALGO_MIXED, width, ba, maxm, maxn, false, counters);
-            }
-            SegArgs sa{};
-            sa.achars = a->d_chars;
-            sa.aoff = a->d_off;
-            sa.bchars = b->d_chars;
-            sa.boff = b->d_off;
-            sa.item_packed = packed;
-            sa.pair_metric = dmet;
-            sa.counters = counters;
-            sa.spec = base;
-            sa.out = dout;
-            // no more waves than pairs (each wave holds up to 64 pairs, at least one)
-            const uint32_t max_blocks = (uint32_t)std::min<uint64_t>((n + 3) / 4, 1u << 20);
-            HIP_CHECK(launch_seg(width, nb, sa, (uint32_t)ctx->num_cus, max_blocks, sk));
-            if (has_osa) {
-                la.item_packed = packed;
-                la.pair_metric = dmet;
-                la.nitems = n;
-                // OSA 33-64 unit range [counters[1], counters[2]), then 1-32 units [counters[2], counters[3])
-                const uint32_t nblk = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 6, (n + 63) / 64);
-                la.begin_dev = counters + 1;
-                la.end_dev = counters + 2;
-                HIP_CHECK(launch_lane_short(width, 64, la, nblk, sk));
-                la.begin_dev = counters + 2;
-                la.end_dev = counters + 3;
-                HIP_CHECK(launch_lane_short(width, 32, la, nblk, sk));
-            }
-            if (need_block) {
-                HIP_CHECK(hipEventRecord(ctx->join, s2));
-                HIP_CHECK(hipStreamWaitEvent(s, ctx->join, 0));
-            }
+            pairs_bucketed_launch(ctx, a, b, dmet, n, base, has_osa, need_block, maxm, maxn, nb, bins, vals, temp,
+                                  temp_bytes, counters, dout, kind, cu);
         };
         // every host value the launch sequence bakes in: same key, same launches
         auto key_now = [&]() {
@@ -1354,6 +1377,46 @@ void pairs_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, co
     }
 }
 
+// rfa_pairs_noalloc: the bucketed schedule of pairs whose shorter string has at
+// most 4096 units (64 when OSA codes may occur), with the plan in this device's
+// static workspace.  Allocates nothing, copies nothing, does not synchronize.
+void pairs_noalloc_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const uint8_t* d_metrics,
+                        int has_osa, int kind, uint64_t cu, double cd, void* d_out)
+{
+    if (a->count != b->count) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "a and b must hold the same number of strings");
+    if (a->width != b->width) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "a and b must have the same char width");
+    if (kind < RFA_DISTANCE || kind > RFA_NORMALIZED_SIMILARITY) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown kind");
+    const uint64_t n = a->count;
+    if (n == 0) return;
+    if (!d_metrics || !d_out) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "metrics or out is NULL");
+    if (n > NOALLOC_MAX_PAIRS)
+        throw ApiError(RFA_ERR_UNSUPPORTED, "rfa_pairs_noalloc takes at most 2^20 pairs per call");
+    if (!ctx->noalloc_scratch) throw ApiError(RFA_ERR_NO_DEVICE, "the static workspace has no device address");
+    // every pair must run in the segmented kernel (or, for OSA, the short lane kernel)
+    const uint64_t cap = has_osa ? 64 : 4096;
+    if (std::min(a->max_len, b->max_len) > cap)
+        for (uint64_t i = 0; i < n; ++i)
+            if (std::min(a->len(i), b->len(i)) > cap)
+                throw ApiError(RFA_ERR_UNSUPPORTED, has_osa ? "rfa_pairs_noalloc with OSA takes pairs whose shorter "
+                                                              "string has at most 64 units"
+                                                            : "rfa_pairs_noalloc takes pairs whose shorter string has "
+                                                              "at most 4096 units");
+    // workspace: counters | bins (4 B per pair) | items (8 B per pair) | plan temporaries
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t temp_bytes = plan_temp_bytes(n);
+    const size_t off_bins = 256, off_vals = off_bins + align(n * 4), off_temp = off_vals + align(n * 8);
+    if (off_temp + temp_bytes > NOALLOC_SCRATCH_BYTES)
+        throw ApiError(RFA_ERR_UNSUPPORTED, "the plan does not fit the static workspace");
+    uint8_t* ws = (uint8_t*)ctx->noalloc_scratch;
+    const rfa_weights unit{1, 1, 1};
+    const ScoreSpec base = make_spec_m(RFA_LEVENSHTEIN, unit, 0.1, kind, RFA_CACHED, cu, cd, ALGO_LEV);
+    const uint64_t maxm = std::min(a->max_len, b->max_len), maxn = std::max(a->max_len, b->max_len);
+    const uint32_t nb = seg_symbol_bits(a->width, std::max(a->max_unit(), b->max_unit()));
+    pairs_bucketed_launch(ctx, a, b, d_metrics, n, base, has_osa != 0, false, maxm, maxn, nb,
+                          (uint32_t*)(ws + off_bins), (uint32_t*)(ws + off_vals), ws + off_temp, temp_bytes,
+                          (uint32_t*)ws, d_out, kind, cu);
+}
+
 } // namespace rfa
 
 using namespace rfa;
@@ -1375,7 +1438,11 @@ rfa_status rfa_context_create(int device, rfa_context** out)
         }
         if (device < 0 || device >= n) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "device index out of range");
         HIP_CHECK(hipSetDevice(device));
+        // the static workspace of rfa_pairs_noalloc: its address on this device (now current)
+        void* ws = nullptr;
+        HIP_CHECK(noalloc_scratch_address(device, &ws));
         auto* c = new rfa_context();
+        c->noalloc_scratch = ws;
         c->device = device;
         hipDeviceProp_t prop;
         HIP_CHECK(hipGetDeviceProperties(&prop, device));
@@ -1555,6 +1622,17 @@ rfa_status rfa_pairs(rfa_context* ctx, const rfa_strings* a, const rfa_strings* 
     });
 }
 
+
+rfa_status rfa_pairs_noalloc(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const uint8_t* metrics,
+                             int has_osa, int kind, uint64_t cutoff_u, double cutoff_d, void* out)
+{
+    if (!ctx || !a || !b) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, a or b is NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_CHECK(hipSetDevice(ctx->device));
+        pairs_noalloc_impl(ctx, a, b, metrics, has_osa, kind, cutoff_u, cutoff_d, out);
+    });
+}
 
 rfa_status rfa_pairs_autotune(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics,
                               int kind, uint64_t cutoff_u, double cutoff_d, uint32_t* chosen)

@@ -222,6 +222,7 @@ struct rfa_context {
     rfa::PairsGraph pairs_graph;
     bool pairs_graph_on = false; // RFA_OPTION_PAIRS_GRAPH
     uint32_t seg_chain_steps = rfa::SEG_CHAIN_STEPS; // RFA_OPTION_PAIRS_CHAIN, set by rfa_pairs_autotune
+    void* noalloc_scratch = nullptr; // this device's static workspace (rfa_pairs_noalloc)
 };
 
 struct rfa_strings {

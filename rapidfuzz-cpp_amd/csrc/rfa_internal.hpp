@@ -29,6 +29,15 @@ struct TileArgs {
     uint32_t norm_epi;    // the normalized kinds' fast epilogue covers the launch (norm_epi_setup)
 };
 
+// rfa_pairs_noalloc's workspace: a static device array per GPU (rfa_noalloc.hip).
+// 48 MiB holds the plan of 2^20 pairs (per-pair bins and items, 128 workgroup
+// histograms over the plan's bins, the scan's temporary storage).
+constexpr size_t NOALLOC_SCRATCH_BYTES = 48ull << 20;
+constexpr uint64_t NOALLOC_MAX_PAIRS = 1ull << 20;
+constexpr uint32_t NOALLOC_MAX_DEVICES = 64;
+// Device address of `device`'s workspace (taken once per device with that device current).
+hipError_t noalloc_scratch_address(int device, void** ptr);
+
 size_t tile_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t hash_bits);
 hipError_t launch_tile(int algo, int char_width, int word_bits, const TileArgs& a, uint32_t ntiles,
                        uint32_t nblocks_x, hipStream_t s);

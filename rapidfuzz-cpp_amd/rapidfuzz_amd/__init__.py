@@ -31,7 +31,7 @@ _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED",
 EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
             "rfa_context_synchronize", "rfa_context_stream", "rfa_context_set_option", "rfa_strings_create", "rfa_strings_destroy",
             "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_scorer_set_option", "rfa_cdist", "rfa_cdist_cutoffs",
-            "rfa_pairs", "rfa_pairs_autotune", "rfa_topk", "rfa_editops",
+            "rfa_pairs", "rfa_pairs_noalloc", "rfa_pairs_autotune", "rfa_topk", "rfa_editops",
             "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
             "rfa_server_get_stats"]
 
@@ -94,6 +94,8 @@ def load_library(path: str = LIB_PATH):
     L.rfa_cdist_cutoffs.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int]
     L.rfa_pairs.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                             _vp, ctypes.c_int]
+    L.rfa_pairs_noalloc.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double,
+                                    _vp]
     L.rfa_pairs_autotune.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _vp]
     L.rfa_topk.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp]
     L.rfa_editops.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64]
@@ -384,6 +386,13 @@ def pairs(ctx: Context, a: Strings, b: Strings, metrics, kind=DISTANCE, cutoff_u
     out = np.zeros(a.count, dtype=dt)
     _check(_lib.rfa_pairs(ctx.h, a.h, b.h, m.ctypes.data, kind, cutoff_u, cutoff_d, schedule, out.ctypes.data, 0))
     return out
+
+
+def pairs_noalloc(ctx: Context, a: Strings, b: Strings, metrics_ptr, has_osa, out_ptr, kind=DISTANCE,
+                  cutoff_u=INT64_MAX, cutoff_d=1.0):
+    """rfa_pairs_noalloc: metrics_ptr (one uint8 code per pair) and out_ptr (8 bytes per pair) are
+    device addresses; the call only enqueues work on the context's stream."""
+    _check(_lib.rfa_pairs_noalloc(ctx.h, a.h, b.h, metrics_ptr, int(has_osa), kind, cutoff_u, cutoff_d, out_ptr))
 
 
 def pairs_autotune(ctx: Context, a: Strings, b: Strings, metrics, kind=DISTANCE, cutoff_u=INT64_MAX, cutoff_d=1.0):

@@ -37,6 +37,10 @@ class DeviceBuffer:
         _ok(lib().hipMalloc(ctypes.byref(self.ptr), nbytes), "hipMalloc")
         self.nbytes = nbytes
 
+    def from_host(self, arr, offset=0):
+        """Copy numpy array arr to byte offset of the buffer."""
+        _ok(lib().hipMemcpy(self.ptr.value + offset, arr.ctypes.data, arr.nbytes, 1), "hipMemcpy H2D")
+
     def to_host(self, arr, nbytes=None, offset=0):
         """Copy nbytes (default: arr.nbytes) starting at byte offset into numpy array arr."""
         n = arr.nbytes if nbytes is None else nbytes
