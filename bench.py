@@ -38,6 +38,48 @@ OPS = {0: 44, 1: 10, 3: 54, 5: 22}
 NAMES = {0: "levenshtein", 1: "indel", 3: "osa", 5: "jaro_winkler"}
 
 
+def _cores():
+    """Threads for the CPU baselines: the host cores this process may use, at most 16 -- the CPU
+    share of one GPU on the GPU box (its nproc shows the whole machine)."""
+    try:
+        return min(len(os.sched_getaffinity(0)), 16)
+    except Exception:
+        return min(os.cpu_count() or 1, 16)
+
+
+def host_cpu():
+    """Host facts every CPU baseline reports: nproc, the usable cores, the threads used, the model."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except Exception:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "threads": _cores(), "cpu_model": model,
+            "thread_cap": "16 threads: the CPU share of one GPU on the GPU box"}
+
+
+def annotate_cpu(obj, info):
+    """Adds the host facts to every cpu_baseline object of a result line."""
+    if isinstance(obj, dict):
+        for k, v in list(obj.items()):
+            if k == "cpu_baseline" and isinstance(v, dict):
+                v.setdefault("nproc", info["nproc"])
+                v.setdefault("cpu_model", info["cpu_model"])
+            else:
+                annotate_cpu(v, info)
+    elif isinstance(obj, list):
+        for v in obj:
+            annotate_cpu(v, info)
+    return obj
+
+
 def gen(rng, n):
     lens = rng.integers(LMIN, LMAX + 1, size=n).astype(np.uint64)
     off = np.zeros(n + 1, dtype=np.uint64)
@@ -51,11 +93,7 @@ def cpu_baseline(qc, qo, cc, co, ql, cl, target_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except Exception:
-        cores = os.cpu_count() or 1
-    cores = min(cores, 16)
+    cores = _cores()
     sum_n = float(cl.sum())
     if oracle_py.reference_available():
         ref = oracle_py.Reference()
@@ -209,7 +247,7 @@ def workload_uneven(args, emit=True):
         import oracle_py
         if oracle_py.reference_available():
             ref = oracle_py.Reference()
-            cores = min(len(os.sched_getaffinity(0)), 16)
+            cores = _cores()
             k = 20000
             while True:
                 t0 = time.perf_counter()
@@ -245,7 +283,7 @@ def workload_uneven(args, emit=True):
         "roofline": roofline, "cpu_baseline": cpu, "check": check,
     }
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
 
 
@@ -367,7 +405,7 @@ def workload_serve(args, emit=True):
         import oracle_py
         if oracle_py.reference_available():
             ref = oracle_py.Reference()
-            cores = min(len(os.sched_getaffinity(0)), 16)
+            cores = _cores()
             ns = cores * 2
             while True:
                 qc_s, qo_s = pack_u8(queries[:ns])
@@ -407,15 +445,8 @@ def workload_serve(args, emit=True):
     cs.close()
     ctx.close()
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
-
-
-def _cores():
-    try:
-        return min(len(os.sched_getaffinity(0)), 16)
-    except Exception:
-        return min(os.cpu_count() or 1, 16)
 
 
 def workload_latency(args, emit=True):
@@ -473,18 +504,26 @@ def workload_latency(args, emit=True):
         if oracle_py.reference_available():
             ref = oracle_py.Reference()
             cut = np.array([(1 << 63) - 1], np.uint64)
-            r = ref.lev_cutoff_mt(qc, qo, cc, co, cut, threads=1)
-            t1 = []
-            for _ in range(20):
-                t0 = time.perf_counter()
-                ref.lev_cutoff_mt(qc, qo, cc, co, cut, threads=1)
-                t1.append(time.perf_counter() - t0)
-            cpu_us = float(np.median(t1)) * 1e6
+            cores = _cores()
+
+            def median_us(threads, reps=20):
+                t = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    ref.lev_cutoff_mt(qc, qo, cc, co, cut, threads=threads)
+                    t.append(time.perf_counter() - t0)
+                return float(np.median(t)) * 1e6
+
+            r = ref.lev_cutoff_mt(qc, qo, cc, co, cut, threads=cores)
+            cpu_us = median_us(cores)
+            one_us = median_us(1)
             bad = int(np.count_nonzero(r[0] != got[0]))
             check = "bit-exact vs reference" if bad == 0 else f"{bad} MISMATCHES vs reference"
-            cpu = {"value": round(cpu_us, 1), "unit": "us per 1x10000 call", "cores": 1, "kind": "reference",
-                   "sample": "the full workload, CachedLevenshtein<uint8_t> loop on one core, median of 20",
-                   "gcups": round(cells / (cpu_us / 1e6) / 1e9, 3)}
+            cpu = {"value": round(cpu_us, 1), "unit": "us per 1x10000 call", "cores": cores, "kind": "reference",
+                   "sample": f"the full workload, CachedLevenshtein<uint8_t> over {cores} std::threads (chunks of "
+                             f"256 candidates; thread start-up included), median of 20",
+                   "gcups": round(cells / (cpu_us / 1e6) / 1e9, 3),
+                   "one_core_us": round(one_us, 1)}
     line = {"metric": "1-vs-10,000 Levenshtein distance latency (host round trip, results in host memory)",
             "value": round(p50, 2), "unit": "us (p50)", "n_gpus": 1, "steps": reps, "warmup": 20,
             "ms_per_step": round(p50 / 1e3, 4), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
@@ -496,7 +535,7 @@ def workload_latency(args, emit=True):
     out.free()
     ctx.close()
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
 
 
@@ -577,7 +616,7 @@ def workload_positional(args, emit=True):
         x.close()
     ctx.close()
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
 
 
@@ -692,7 +731,7 @@ def workload_partial(args, emit=True):
         x.close()
     ctx.close()
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
 
 
@@ -778,7 +817,7 @@ def workload_wlev(args, emit=True):
         x.close()
     ctx.close()
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
 
 
@@ -861,7 +900,7 @@ def workload_dl(args, emit=True):
         x.close()
     ctx.close()
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
 
 
@@ -939,7 +978,7 @@ def workload_editops(args, emit=True):
     B.close()
     ctx.close()
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
 
 
@@ -1032,7 +1071,7 @@ def workload_tokens(args, emit=True):
         x.close()
     ctx.close()
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
 
 
@@ -1146,7 +1185,7 @@ def workload_dna(args, emit=True):
     out.free()
     ctx.close()
     if emit:
-        print(json.dumps(line), flush=True)
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
     return line
 
 
@@ -1287,6 +1326,15 @@ def main():
 
     scorers = {}
     if not args.no_scorers:
+        # every row is checked against the reference library on 8 query rows x all choices
+        ref = None
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle_py
+            if oracle_py.reference_available():
+                ref = oracle_py.Reference()
+        sample = np.linspace(0, NQ - 1, 8).astype(np.int64)
+        sq, so = pack_u8([qc[int(qo[i]):int(qo[i + 1])] for i in sample])
         for metric in (0, 1, 3, 5):
             sc = lev if metric == 0 else rfa.Scorer(ctx, metric, qs, prefix_weight=0.1)
             for mode in ("no_cutoff", "norm_sim_0.8"):
@@ -1306,14 +1354,37 @@ def main():
                 e1.record(stream)
                 e1.synchronize()
                 ms = e0.elapsed_ms(e1) / reps
+                row_check = "not checked"
+                if ref is not None:
+                    exp = ref.cdist_u8_mt(metric, kind, sq, so, cc, co, cutoff_d=kw.get("cd"), threads=_cores())
+                    bad = 0
+                    got_row = np.zeros(NC, dtype=np.uint64)
+                    for k, i in enumerate(sample):
+                        out.to_host(got_row, offset=int(i) * NC * 8)
+                        bad += int(np.count_nonzero(got_row != exp[k].view(np.uint64)))
+                    row_check = ("bit-exact vs reference on 8 query rows x all choices" if bad == 0
+                                 else f"{bad} MISMATCHES vs reference")
                 scorers[f"{NAMES[metric]}/{mode}"] = {
                     "ms": round(ms, 3), "gcups": round(cells / (ms / 1e3) / 1e9, 1),
-                    "int_tops": round(OPS[metric] * word_steps / (ms / 1e3) / 1e12, 2),
-                    "int_frac": round(OPS[metric] * word_steps / (ms / 1e3) / 1e12 / peak_tops, 4)}
+                    "x86_priced_int_tops": round(OPS[metric] * word_steps / (ms / 1e3) / 1e12, 2),
+                    "check": row_check}
             if sc is not lev:
                 sc.close()
 
     prof = read_profile_facts(os.path.join(ROOT, "profiles", "tile_levenshtein_cdist_4096x65536.md"))
+    # The roofline prices a word-step with the VALU instructions gfx950 needs for the Hyyro-2003
+    # recurrence: 21 per 64-bit word (two 32-bit halves: bitop3 folds the NOR/OR/XOR terms,
+    # v_lshl_add_u64 the 64-bit add, v_alignbit the carried shifts) and 11 per 32-bit word (tiles of
+    # queries <= 32 units), read off this build's inner loop; the PM-row broadcast (v_readlane), the
+    # LDS address and the per-choice epilogue are not counted, so the fraction is a lower bound on
+    # the VALU issue rate.  Peak: CUs x 64 lanes x clock lane-operations per second.  The survey's
+    # x86 pricing (44 INT32 ops per 64-cell word-step) counts operations gfx950 fuses, so it is kept
+    # beside it (x86_priced), not as the fraction.
+    GFX_OPS_64, GFX_OPS_32 = 21, 11
+    lane_steps_64 = n64 * float(cl.sum())
+    lane_steps_32 = (NQ - n64) * float(cl.sum())
+    gfx_ops = GFX_OPS_64 * lane_steps_64 + GFX_OPS_32 * lane_steps_32
+    gfx_tops = gfx_ops / (ms_per_step / 1e3) / 1e12
     res = {
         "metric": "Levenshtein 1-vs-N GCUPS + % INT roofline; served p50/p99 latency at QPS",
         "value": round(gcups, 1),
@@ -1330,14 +1401,19 @@ def main():
         "config": {"workload": f"cdist {NQ} queries x {NC} choices, uint8 printable ASCII, lengths uniform "
                                f"{LMIN}-{LMAX}, CachedLevenshtein distance, no cutoff (single-word path)",
                    "queries": NQ, "choices": NC, "cells": cells, "seed": SEED},
-        "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 3),
-                     "unit": "TOP/s", "frac": round(achieved_tops / peak_tops, 4),
+        "roofline": {"bound": "valu", "achieved": round(gfx_tops, 3), "peak": round(peak_tops, 3),
+                     "unit": "TOP/s (INT32 VALU lane-operations)", "frac": round(gfx_tops / peak_tops, 4),
                      "traffic": prof.get("traffic_bytes_per_step"),
                      "traffic_unit": "HBM bytes per step (the 64-bit + 32-bit launch pair), from FETCH_SIZE / "
                                      "WRITE_SIZE scaled by a 1 GiB copy",
-                     "ops_per_launch_pair": ops,
-                     "ops_model": "44 INT32 ops per 64-cell word-step (2 x the 22 64-bit ops of "
-                                  "Levenshtein_impl.hpp:285-302); 32-bit-word tiles count half a word-step",
+                     "ops_per_step": gfx_ops,
+                     "ops_model": f"{GFX_OPS_64} VALU instructions per 64-bit word-step and {GFX_OPS_32} per "
+                                  "32-bit word-step (the gfx950 Hyyro-2003 recurrence of the inner loop) x lane "
+                                  "word-steps",
+                     "x86_priced": {"achieved": round(achieved_tops, 3), "ops_per_step": ops,
+                                    "ops_model": "44 INT32 ops per 64-cell word-step (2 x the 22 64-bit ops of "
+                                                 "Levenshtein_impl.hpp:285-302); 32-bit-word tiles count half a "
+                                                 "word-step", "frac_of_peak": round(achieved_tops / peak_tops, 4)},
                      "kernel": "k_tile<LEV,u8,64/32-bit words,integer epilogue>",
                      "valu_issue_utilization": {"w64": prof.get("valu_issue_utilization_64"),
                                                 "w32": prof.get("valu_issue_utilization_32"),
@@ -1395,7 +1471,9 @@ def main():
                                              "speedup_vs_plain": une["speedup_vs_plain"],
                                              "autotuned_chain_threshold": une["autotuned_chain_threshold"],
                                              "config": une["config"]["workload"]}
-    print(json.dumps(res), flush=True)
+    info = host_cpu()
+    res["host_cpu"] = info
+    print(json.dumps(annotate_cpu(res, info)), flush=True)
 
 
 if __name__ == "__main__":

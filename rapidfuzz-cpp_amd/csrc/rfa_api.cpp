@@ -1076,6 +1076,14 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
             NormEpi ne;
             ta.norm_epi = norm_epi_setup(spec, ne) && (uint64_t)64 + ch->max_len <= 0xffffffffull ? 1u : 0u;
         }
+        {
+            // a Cached integer result is a distance, similarity or cutoff + 1 no larger than the
+            // maximum, (query + choice length) x the largest weight (Multi* similarities can
+            // wrap around below zero, as the reference's do, so they keep 64 bits)
+            const uint64_t wmax = std::max({spec.ins, spec.del, spec.rep, (uint64_t)1});
+            const uint64_t len = 64 + ch->max_len;
+            ta.narrow_int = semantics == RFA_CACHED && len < (0xffffffffull / wmax) ? 1u : 0u;
+        }
         if (ta.ncsel) {
             for (const TileClass& tc : sc->classes) {
                 ta.tile_base = tc.first_tile;

@@ -27,6 +27,7 @@ struct TileArgs {
     const uint64_t* qcut; // per-query integer cutoff (nullptr: spec.cutoff_u)
     uint32_t settle;      // the cutoffs may decide results by length alone: test before each recurrence
     uint32_t norm_epi;    // the normalized kinds' fast epilogue covers the launch (norm_epi_setup)
+    uint32_t narrow_int;  // integer results stay below 2^32 (held as 32 bits)
 };
 
 // rfa_pairs_noalloc's workspace: a static device array per GPU (rfa_noalloc.hip).

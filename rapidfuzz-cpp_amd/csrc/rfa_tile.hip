@@ -14,6 +14,8 @@
 //
 // Word width W is 32 bits for tiles whose queries all fit in 32 units and 64
 // bits otherwise (a 64-bit op is two VALU ops on CDNA).
+#include <type_traits>
+
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
 #include "rfa_semantics.hpp"
@@ -241,7 +243,12 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
     // holds choice t's index, offset and length (loaded a run ahead), choice
     // t + 1's first 64 code units are loaded while choice t runs, and its PM
     // rows are looked up before choice t's epilogue.
-    constexpr uint32_t G = EPI == EPI_INT ? 8 : 4;
+    // Integer results are held as 32 bits (the host routes launches whose values could
+    // reach 2^32 to EPI_ANY), 16 per lane: one 128-byte row segment per run.  Doubles
+    // are held 16 per lane in the normalized variant (128-byte segments again), 8 in the
+    // general one.
+    constexpr uint32_t G = EPI == EPI_ANY ? 8 : 16;
+    using OutT = std::conditional_t<EPI == EPI_INT, uint32_t, uint64_t>;
     const bool direct = a.corder != nullptr;
     uint32_t g0 = it0 + uniform(wave) * G;
     if (g0 >= it1) return;
@@ -274,7 +281,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
         const uint32_t g1 = g0 + TILE_WAVES * G; // this wave's next run
         uint32_t x_ci, x_olo, x_ohi, x_len;
         run_load(g1, x_ci, x_olo, x_ohi, x_len);
-        uint64_t outv[G];
+        OutT outv[G];
         for (uint32_t t = 0; t < gn; ++t) {
             const uint32_t cidx = uniform(readlane(r_ci, t));
             const uint32_t n = n_cur;
@@ -393,7 +400,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
                 if (qact) ((uint64_t*)a.out)[(uint64_t)qidx * a.out_stride + cidx] = bits;
             }
             else {
-                outv[t] = bits;
+                outv[t] = (OutT)bits;
             }
             n_cur = n_nx, co_cur = co_nx, row_cur = row_nx;
         }
@@ -403,15 +410,15 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
                 uint4* d4 = (uint4*)dst;
 #pragma unroll
                 for (uint32_t t = 0; t < G; t += 2)
-                    d4[t / 2] = make_uint4((uint32_t)outv[t], (uint32_t)(outv[t] >> 32), (uint32_t)outv[t + 1],
-                                           (uint32_t)(outv[t + 1] >> 32));
+                    d4[t / 2] = make_uint4((uint32_t)outv[t], (uint32_t)((uint64_t)outv[t] >> 32),
+                                           (uint32_t)outv[t + 1], (uint32_t)((uint64_t)outv[t + 1] >> 32));
             }
             else if (gn == G) {
 #pragma unroll
-                for (uint32_t t = 0; t < G; ++t) dst[t] = outv[t];
+                for (uint32_t t = 0; t < G; ++t) dst[t] = (uint64_t)outv[t];
             }
             else {
-                for (uint32_t t = 0; t < gn; ++t) dst[t] = outv[t];
+                for (uint32_t t = 0; t < gn; ++t) dst[t] = (uint64_t)outv[t];
             }
         }
         if (g1 >= it1) break;
@@ -427,7 +434,7 @@ static void launch_one(const TileArgs& a, uint32_t ntiles, uint32_t nblocks_x, s
     dim3 grid(nblocks_x, ntiles);
     const bool intk = (a.spec.kind == RFA_DISTANCE || a.spec.kind == RFA_SIMILARITY) && a.spec.metric <= RFA_OSA;
     if constexpr (ALGO != ALGO_JARO) {
-        if (intk) {
+        if (intk && a.narrow_int) {
             hipLaunchKernelGGL((k_tile<ALGO, CharT, W, EPI_INT>), grid, dim3(TILE_THREADS), lds, s, a);
             return;
         }
