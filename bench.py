@@ -1373,14 +1373,14 @@ def main():
 
     prof = read_profile_facts(os.path.join(ROOT, "profiles", "tile_levenshtein_cdist_4096x65536.md"))
     # The roofline prices a word-step with the VALU instructions gfx950 needs for the Hyyro-2003
-    # recurrence: 21 per 64-bit word (two 32-bit halves: bitop3 folds the NOR/OR/XOR terms,
-    # v_lshl_add_u64 the 64-bit add, v_alignbit the carried shifts) and 11 per 32-bit word (tiles of
+    # recurrence: 20 per 64-bit word (two 32-bit halves: bitop3 folds the NOR/OR/XOR terms,
+    # v_lshl_add_u64 the 64-bit add, v_alignbit the carried shifts) and 10 per 32-bit word (tiles of
     # queries <= 32 units), read off this build's inner loop; the PM-row broadcast (v_readlane), the
     # LDS address and the per-choice epilogue are not counted, so the fraction is a lower bound on
     # the VALU issue rate.  Peak: CUs x 64 lanes x clock lane-operations per second.  The survey's
     # x86 pricing (44 INT32 ops per 64-cell word-step) counts operations gfx950 fuses, so it is kept
     # beside it (x86_priced), not as the fraction.
-    GFX_OPS_64, GFX_OPS_32 = 21, 11
+    GFX_OPS_64, GFX_OPS_32 = 20, 10
     lane_steps_64 = n64 * float(cl.sum())
     lane_steps_32 = (NQ - n64) * float(cl.sum())
     gfx_ops = GFX_OPS_64 * lane_steps_64 + GFX_OPS_32 * lane_steps_32

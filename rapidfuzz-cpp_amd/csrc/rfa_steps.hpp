@@ -28,6 +28,7 @@ constexpr uint32_t TA = 0xF0, TB = 0xCC, TC = 0xAA;
 constexpr uint32_t T_XOR_OR = ((TA ^ TB) | TC) & 0xFF;  // (a ^ b) | c
 constexpr uint32_t T_OR_NOR = (TA | ~(TB | TC)) & 0xFF; // a | ~(b | c)
 constexpr uint32_t T_OR_ANDN = (TA | (TB & ~TC)) & 0xFF; // a | (b & ~c)
+constexpr uint32_t T_AND2 = (TA & TB) & 0xFF;             // a & b (c unused)
 
 template <uint32_t IMM>
 __device__ __forceinline__ uint32_t bop3(uint32_t a, uint32_t b, uint32_t c)
@@ -46,10 +47,10 @@ __device__ __forceinline__ void lev_step(uint32_t X, uint32_t& VP, uint32_t& VN)
     const uint32_t D0 = bop3<T_XOR_OR>(s, VP, XV);
     uint32_t HP = bop3<T_OR_NOR>(VN, D0, VP);
     uint32_t HN = D0 & VP;
-    HP = (HP << 1) | 1;
-    HN = HN << 1;
+    HP = (HP << 1) | 1; // one v_lshl_or_b32: VN below reads it through bitop3, which the
+    HN = HN << 1;       // compiler does not fold back into a separate shift and OR
     VP = bop3<T_OR_NOR>(HN, D0, HP);
-    VN = HP & D0;
+    VN = bop3<T_AND2>(HP, D0, 0u);
 }
 
 __device__ __forceinline__ void lev_step(uint64_t X, uint32_t& VPl, uint32_t& VPh, uint32_t& VNl, uint32_t& VNh)
@@ -67,7 +68,7 @@ __device__ __forceinline__ void lev_step(uint64_t X, uint32_t& VPl, uint32_t& VP
     const uint32_t HNh = shl1_hi(HNh0, HNl0), HNl = HNl0 << 1;
     VPl = bop3<T_OR_NOR>(HNl, D0l, HPl);
     VPh = bop3<T_OR_NOR>(HNh, D0h, HPh);
-    VNl = HPl & D0l;
+    VNl = bop3<T_AND2>(HPl, D0l, 0u);
     VNh = HPh & D0h;
 }
 
@@ -99,7 +100,7 @@ __device__ __forceinline__ void osa_step(uint32_t X, uint32_t& VP, uint32_t& VN,
     HP = (HP << 1) | 1;
     HN = HN << 1;
     VP = bop3<T_OR_NOR>(HN, D0, HP);
-    VN = HP & D0;
+    VN = bop3<T_AND2>(HP, D0, 0u);
     PMold = X;
 }
 
