@@ -18,6 +18,7 @@ events on the engine's stream; the CPU baseline is the reference library
 the C restatement when that library is absent.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -1209,12 +1210,51 @@ def workload_copy(args):
           flush=True)
 
 
+def workload_scorer(args):
+    """One scorer of the per-scorer table alone on the headline inputs (4096 x 65536 printable
+    ASCII, lengths 1-64): --metric, --kind and --cutoff pick it.  --warmup calls, then --steps
+    timed calls; the line gives the per-call time (for profiling one kernel family)."""
+    import rapidfuzz_amd as rfa
+    from rapidfuzz_amd import hip
+    rng = np.random.default_rng(SEED)
+    qc, qo, ql = gen(rng, NQ)
+    cc, co, cl = gen(rng, NC)
+    cells = float(ql.sum()) * float(cl.sum())
+    ctx = rfa.Context(0)
+    qs = rfa.Strings(ctx, chars=qc, offsets=qo, width=1)
+    cs = rfa.Strings(ctx, chars=cc, offsets=co, width=1)
+    out = hip.DeviceBuffer(NQ * NC * 8)
+    sc = rfa.Scorer(ctx, args.metric, qs, prefix_weight=0.1)
+
+    def call():
+        sc.cdist(cs, args.kind, cutoff_d=args.cutoff, out_device_ptr=out.ptr.value)
+
+    for _ in range(args.warmup):
+        call()
+    ctx.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(ctx.stream)
+    for _ in range(args.steps):
+        call()
+    e1.record(ctx.stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / max(1, args.steps)
+    print(json.dumps({"metric": f"{NAMES.get(args.metric, args.metric)} cdist", "kind": args.kind,
+                      "cutoff": args.cutoff, "value": round(cells / (ms / 1e3) / 1e9, 1), "unit": "GCUPS",
+                      "ms_per_step": round(ms, 4), "steps": args.steps, "warmup": args.warmup,
+                      "config": {"workload": f"cdist {NQ}x{NC} u8 printable, lengths {LMIN}-{LMAX}"}}),
+          flush=True)
+    for o in (sc, qs, cs):
+        o.close()
+    out.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="cdist",
                     choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "partial", "tokens", "wlev",
-                             "dl", "editops", "copy"],
+                             "dl", "editops", "copy", "scorer"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
                          "uneven: 1M independent pairs; serve: the serving leg alone; latency: 1 x 10,000 "
                          "Levenshtein; dna: 256 x 16,384 long sequences with band cutoffs; copy: HBM counter "
@@ -1227,6 +1267,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the latency, DNA and uneven-batch summaries of the default run")
+    ap.add_argument("--metric", type=int, default=0, help="scorer workload: metric code (0 Levenshtein, "
+                                                             "1 Indel, 3 OSA, 5 Jaro-Winkler)")
+    ap.add_argument("--kind", type=int, default=0, help="scorer workload: 0 distance, 1 similarity, "
+                                                           "2 normalized distance, 3 normalized similarity")
+    ap.add_argument("--cutoff", type=float, default=None, help="scorer workload: cutoff of a normalized kind")
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -1241,6 +1286,8 @@ def main():
         return workload_serve(args)
     if args.workload == "copy":
         return workload_copy(args)
+    if args.workload == "scorer":
+        return workload_scorer(args)
     if args.workload == "latency":
         return workload_latency(args)
     if args.workload == "dna":
@@ -1425,8 +1472,9 @@ def main():
         "scorers": scorers,
     }
     out.free()
-    for s_ in (lev, qs, cs):
+    for s_ in (lev, qs, cs, ctx):
         s_.close()
+    gc.collect()
     if not args.no_serve:
         srv = workload_serve(args, emit=False)
         res["serving"] = {k: srv[k] for k in ("latency_ms", "served_qps", "choices_scored_per_s", "mean_batch",

@@ -8,7 +8,11 @@
 //   * choices are sorted by length and cut into groups of 64 (one wave); a
 //     group's characters are stored lane-interleaved, dword d of lane l at
 //     data[base + d * 64 + l], so one load per wave reads 256 contiguous bytes;
-//   * the PM rows of QPB queries sit in LDS, indexed directly by byte value;
+//   * the PM rows of QPB queries sit in LDS, indexed directly by byte value,
+//     word-major ([query][word][byte]): a wave's gather for one word reads
+//     64 8-byte entries spread over 256 consecutive ones, as with one word
+//     (a [byte][word] layout spaces them 8*WQ bytes apart, so the 64 lanes'
+//     addresses fall on 64/WQ bank pairs and conflict WQ times as often);
 //   * each wave walks its choices' characters once and advances the
 //     recurrence of every query of its workgroup per character (choice
 //     outer, queries inner), so choice bytes leave HBM once per batch;
@@ -56,7 +60,8 @@ struct QState {
         }
     }
 
-    // One text character; row points at the query's PM words for that character.
+    // One text character; row points at the query's PM word 0 for that
+    // character, word w at row[w * 256].
     __device__ __forceinline__ void step(const uint64_t* row)
     {
         if constexpr (WQ == 1) {
@@ -71,7 +76,7 @@ struct QState {
             unsigned long long carry = 0;
 #pragma unroll
             for (int w = 0; w < WQ; ++w) {
-                const uint64_t X = row[w];
+                const uint64_t X = row[w * 256];
                 const uint64_t S = ((uint64_t)a[w][1] << 32) | a[w][0];
                 const uint64_t u = S & X;
                 unsigned long long co;
@@ -89,7 +94,7 @@ struct QState {
             for (int w = 0; w < WQ; ++w) {
                 const uint64_t VP = ((uint64_t)a[w][1] << 32) | a[w][0];
                 const uint64_t VN = ((uint64_t)b[w][1] << 32) | b[w][0];
-                const uint64_t X = row[w] | hn_c;
+                const uint64_t X = row[w * 256] | hn_c;
                 const uint64_t D0 = (((X & VP) + VP) ^ VP) | X | VN;
                 uint64_t HP = VN | ~(D0 | VP);
                 uint64_t HN = D0 & VP;
@@ -141,7 +146,7 @@ struct QState {
 template <int ALGO, int WQ, int QPB, bool DENSE>
 __global__ __launch_bounds__(256) void k_serve(ServeArgs a)
 {
-    extern __shared__ uint64_t pm[]; // [QPB][256][WQ]
+    extern __shared__ uint64_t pm[]; // [QPB][WQ][256]
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t sub = blockIdx.y;
     const uint32_t q0 = a.q_begin + sub * QPB;
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a)
         const uint64_t o = a.qoff[q0 + q];
         const uint32_t m = (uint32_t)(a.qoff[q0 + q + 1] - o);
         for (uint32_t i = tid; i < m; i += 256)
-            atomicOr((unsigned long long*)&pm[((size_t)q * 256 + a.qchars[o + i]) * WQ + (i >> 6)], 1ull << (i & 63));
+            atomicOr((unsigned long long*)&pm[((size_t)q * WQ + (i >> 6)) * 256 + a.qchars[o + i]], 1ull << (i & 63));
     }
     __syncthreads();
 
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a)
             if (d + 1 < D) v = gd[(size_t)(d + 1) * 64];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const uint64_t* row = pm + ((cur >> (8 * b)) & 0xffu) * WQ;
+                const uint64_t* row = pm + ((cur >> (8 * b)) & 0xffu);
 #pragma unroll
                 for (int q = 0; q < QPB; ++q)
                     if (act & (1u << q)) st[q].step(row + (size_t)q * 256 * WQ);
@@ -211,7 +216,7 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a)
             for (int b = 0; b < 4; ++b) {
                 const uint32_t j = d * 4 + b;
                 if (j < n) {
-                    const uint64_t* row = pm + ((cur >> (8 * b)) & 0xffu) * WQ;
+                    const uint64_t* row = pm + ((cur >> (8 * b)) & 0xffu);
 #pragma unroll
                     for (int q = 0; q < QPB; ++q)
                         if (act & (1u << q)) st[q].step(row + (size_t)q * 256 * WQ);
