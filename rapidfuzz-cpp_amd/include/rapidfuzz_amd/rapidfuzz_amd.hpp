@@ -557,6 +557,182 @@ explicit CachedPrefix(const Sentence1& s1_) -> CachedPrefix<detail::sentence_cha
 template <typename Sentence1>
 explicit CachedPostfix(const Sentence1& s1_) -> CachedPostfix<detail::sentence_char_t<Sentence1>>;
 
+// ---- free functions -----------------------------------------------------------
+// levenshtein_* (Levenshtein.hpp:140-265), indel_* (Indel.hpp:12-60), lcs_seq_*
+// (LCSseq.hpp:12-60), osa_* (OSA.hpp:36-108), jaro_* (Jaro.hpp:12-59),
+// jaro_winkler_* (JaroWinkler.hpp:12-75), hamming_* (Hamming.hpp:35-125),
+// prefix_* / postfix_* (Prefix.hpp:12-60, Postfix.hpp:12-60): the same names,
+// parameters and defaults.  Each call scores its one pair through a one-query
+// scorer, so the result is the Cached* scorer's (the reference's free functions
+// and Cached* classes share details/distance.hpp's normalization); a call is a
+// device round trip, and many pairs belong in one pairs_*() call.
+namespace detail {
+template <typename It>
+using iter_char_t = typename std::decay<decltype(*std::declval<It>())>::type;
+} // namespace detail
+
+#define RFA_FREE_KINDS(NAME, CACHED, DECL_EXTRA, CTOR_EXTRA)                                                        \
+    template <typename InputIt1, typename InputIt2>                                                                \
+    size_t NAME##_distance(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2 DECL_EXTRA,           \
+                           size_t score_cutoff = std::numeric_limits<size_t>::max())                               \
+    {                                                                                                              \
+        return CACHED<detail::iter_char_t<InputIt1>>(first1, last1 CTOR_EXTRA).distance(first2, last2, score_cutoff); \
+    }                                                                                                              \
+    template <typename Sentence1, typename Sentence2>                                                              \
+    size_t NAME##_distance(const Sentence1& s1, const Sentence2& s2 DECL_EXTRA,                                   \
+                           size_t score_cutoff = std::numeric_limits<size_t>::max())                               \
+    {                                                                                                              \
+        return NAME##_distance(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2), detail::to_end(s2)  \
+                                   CTOR_EXTRA, score_cutoff);                                                      \
+    }                                                                                                              \
+    template <typename InputIt1, typename InputIt2>                                                                \
+    size_t NAME##_similarity(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2 DECL_EXTRA,         \
+                             size_t score_cutoff = 0)                                                              \
+    {                                                                                                              \
+        return CACHED<detail::iter_char_t<InputIt1>>(first1, last1 CTOR_EXTRA)                                    \
+            .similarity(first2, last2, score_cutoff);                                                              \
+    }                                                                                                              \
+    template <typename Sentence1, typename Sentence2>                                                              \
+    size_t NAME##_similarity(const Sentence1& s1, const Sentence2& s2 DECL_EXTRA, size_t score_cutoff = 0)         \
+    {                                                                                                              \
+        return NAME##_similarity(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2),                   \
+                                 detail::to_end(s2) CTOR_EXTRA, score_cutoff);                                     \
+    }                                                                                                              \
+    template <typename InputIt1, typename InputIt2>                                                                \
+    double NAME##_normalized_distance(InputIt1 first1, InputIt1 last1, InputIt2 first2,                            \
+                                      InputIt2 last2 DECL_EXTRA, double score_cutoff = 1.0)                        \
+    {                                                                                                              \
+        return CACHED<detail::iter_char_t<InputIt1>>(first1, last1 CTOR_EXTRA)                                    \
+            .normalized_distance(first2, last2, score_cutoff);                                                     \
+    }                                                                                                              \
+    template <typename Sentence1, typename Sentence2>                                                              \
+    double NAME##_normalized_distance(const Sentence1& s1, const Sentence2& s2 DECL_EXTRA,                         \
+                                      double score_cutoff = 1.0)                                                   \
+    {                                                                                                              \
+        return NAME##_normalized_distance(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2),          \
+                                          detail::to_end(s2) CTOR_EXTRA, score_cutoff);                            \
+    }                                                                                                              \
+    template <typename InputIt1, typename InputIt2>                                                                \
+    double NAME##_normalized_similarity(InputIt1 first1, InputIt1 last1, InputIt2 first2,                          \
+                                        InputIt2 last2 DECL_EXTRA, double score_cutoff = 0.0)                      \
+    {                                                                                                              \
+        return CACHED<detail::iter_char_t<InputIt1>>(first1, last1 CTOR_EXTRA)                                    \
+            .normalized_similarity(first2, last2, score_cutoff);                                                   \
+    }                                                                                                              \
+    template <typename Sentence1, typename Sentence2>                                                              \
+    double NAME##_normalized_similarity(const Sentence1& s1, const Sentence2& s2 DECL_EXTRA,                       \
+                                        double score_cutoff = 0.0)                                                 \
+    {                                                                                                              \
+        return NAME##_normalized_similarity(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2),        \
+                                            detail::to_end(s2) CTOR_EXTRA, score_cutoff);                          \
+    }
+
+#define RFA_NOTHING
+#define RFA_COMMA_PAD_DECL , bool pad_ = true
+#define RFA_COMMA_PAD , pad_
+RFA_FREE_KINDS(indel, CachedIndel, RFA_NOTHING, RFA_NOTHING)
+RFA_FREE_KINDS(lcs_seq, CachedLCSseq, RFA_NOTHING, RFA_NOTHING)
+RFA_FREE_KINDS(osa, CachedOSA, RFA_NOTHING, RFA_NOTHING)
+RFA_FREE_KINDS(prefix, CachedPrefix, RFA_NOTHING, RFA_NOTHING)
+RFA_FREE_KINDS(postfix, CachedPostfix, RFA_NOTHING, RFA_NOTHING)
+RFA_FREE_KINDS(hamming, CachedHamming, RFA_COMMA_PAD_DECL, RFA_COMMA_PAD)
+#undef RFA_FREE_KINDS
+#undef RFA_COMMA_PAD_DECL
+#undef RFA_COMMA_PAD
+
+// levenshtein_* carry the weight table and score_hint (Levenshtein.hpp:140-265)
+template <typename InputIt1, typename InputIt2>
+size_t levenshtein_distance(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2,
+                            LevenshteinWeightTable weights = {1, 1, 1},
+                            size_t score_cutoff = std::numeric_limits<size_t>::max(),
+                            size_t score_hint = std::numeric_limits<size_t>::max())
+{
+    return CachedLevenshtein<detail::iter_char_t<InputIt1>>(first1, last1, weights)
+        .distance(first2, last2, score_cutoff, score_hint);
+}
+template <typename Sentence1, typename Sentence2>
+size_t levenshtein_distance(const Sentence1& s1, const Sentence2& s2, LevenshteinWeightTable weights = {1, 1, 1},
+                            size_t score_cutoff = std::numeric_limits<size_t>::max(),
+                            size_t score_hint = std::numeric_limits<size_t>::max())
+{
+    return levenshtein_distance(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2), detail::to_end(s2),
+                                weights, score_cutoff, score_hint);
+}
+template <typename InputIt1, typename InputIt2>
+size_t levenshtein_similarity(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2,
+                              LevenshteinWeightTable weights = {1, 1, 1}, size_t score_cutoff = 0,
+                              size_t score_hint = 0)
+{
+    return CachedLevenshtein<detail::iter_char_t<InputIt1>>(first1, last1, weights)
+        .similarity(first2, last2, score_cutoff, score_hint);
+}
+template <typename Sentence1, typename Sentence2>
+size_t levenshtein_similarity(const Sentence1& s1, const Sentence2& s2, LevenshteinWeightTable weights = {1, 1, 1},
+                              size_t score_cutoff = 0, size_t score_hint = 0)
+{
+    return levenshtein_similarity(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2), detail::to_end(s2),
+                                  weights, score_cutoff, score_hint);
+}
+template <typename InputIt1, typename InputIt2>
+double levenshtein_normalized_distance(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2,
+                                       LevenshteinWeightTable weights = {1, 1, 1}, double score_cutoff = 1.0,
+                                       double score_hint = 1.0)
+{
+    return CachedLevenshtein<detail::iter_char_t<InputIt1>>(first1, last1, weights)
+        .normalized_distance(first2, last2, score_cutoff, score_hint);
+}
+template <typename Sentence1, typename Sentence2>
+double levenshtein_normalized_distance(const Sentence1& s1, const Sentence2& s2,
+                                       LevenshteinWeightTable weights = {1, 1, 1}, double score_cutoff = 1.0,
+                                       double score_hint = 1.0)
+{
+    return levenshtein_normalized_distance(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2),
+                                           detail::to_end(s2), weights, score_cutoff, score_hint);
+}
+template <typename InputIt1, typename InputIt2>
+double levenshtein_normalized_similarity(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2,
+                                         LevenshteinWeightTable weights = {1, 1, 1}, double score_cutoff = 0.0,
+                                         double score_hint = 0.0)
+{
+    return CachedLevenshtein<detail::iter_char_t<InputIt1>>(first1, last1, weights)
+        .normalized_similarity(first2, last2, score_cutoff, score_hint);
+}
+template <typename Sentence1, typename Sentence2>
+double levenshtein_normalized_similarity(const Sentence1& s1, const Sentence2& s2,
+                                         LevenshteinWeightTable weights = {1, 1, 1}, double score_cutoff = 0.0,
+                                         double score_hint = 0.0)
+{
+    return levenshtein_normalized_similarity(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2),
+                                             detail::to_end(s2), weights, score_cutoff, score_hint);
+}
+
+// jaro_* and jaro_winkler_* return doubles for every kind (Jaro.hpp:12-59, JaroWinkler.hpp:12-75)
+#define RFA_FREE_DOUBLE(NAME, KIND, CUT, CACHED, DECL_EXTRA, CTOR_EXTRA)                                            \
+    template <typename InputIt1, typename InputIt2>                                                                \
+    double NAME##_##KIND(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2 DECL_EXTRA,             \
+                         double score_cutoff = CUT)                                                                \
+    {                                                                                                              \
+        return CACHED<detail::iter_char_t<InputIt1>>(first1, last1 CTOR_EXTRA).KIND(first2, last2, score_cutoff);   \
+    }                                                                                                              \
+    template <typename Sentence1, typename Sentence2>                                                              \
+    double NAME##_##KIND(const Sentence1& s1, const Sentence2& s2 DECL_EXTRA, double score_cutoff = CUT)          \
+    {                                                                                                              \
+        return NAME##_##KIND(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2),                       \
+                             detail::to_end(s2) CTOR_EXTRA, score_cutoff);                                         \
+    }
+#define RFA_COMMA_PW_DECL , double prefix_weight = 0.1
+#define RFA_COMMA_PW , prefix_weight
+RFA_FREE_DOUBLE(jaro, distance, 1.0, CachedJaro, RFA_NOTHING, RFA_NOTHING)
+RFA_FREE_DOUBLE(jaro, similarity, 0.0, CachedJaro, RFA_NOTHING, RFA_NOTHING)
+RFA_FREE_DOUBLE(jaro, normalized_distance, 1.0, CachedJaro, RFA_NOTHING, RFA_NOTHING)
+RFA_FREE_DOUBLE(jaro, normalized_similarity, 0.0, CachedJaro, RFA_NOTHING, RFA_NOTHING)
+RFA_FREE_DOUBLE(jaro_winkler, distance, 1.0, CachedJaroWinkler, RFA_COMMA_PW_DECL, RFA_COMMA_PW)
+RFA_FREE_DOUBLE(jaro_winkler, similarity, 0.0, CachedJaroWinkler, RFA_COMMA_PW_DECL, RFA_COMMA_PW)
+RFA_FREE_DOUBLE(jaro_winkler, normalized_distance, 1.0, CachedJaroWinkler, RFA_COMMA_PW_DECL, RFA_COMMA_PW)
+RFA_FREE_DOUBLE(jaro_winkler, normalized_similarity, 0.0, CachedJaroWinkler, RFA_COMMA_PW_DECL, RFA_COMMA_PW)
+#undef RFA_COMMA_PW_DECL
+#undef RFA_COMMA_PW
+
 // ---- edit operations (details/types.hpp:28-60, 167-355: the parts a caller reads) ----
 enum class EditType { None = RFA_EDIT_NONE, Replace = RFA_EDIT_REPLACE, Insert = RFA_EDIT_INSERT, Delete = RFA_EDIT_DELETE };
 
@@ -1022,6 +1198,32 @@ CachedPartialTokenSortRatio(const Sentence1& s1) -> CachedPartialTokenSortRatio<
 template <typename Sentence1>
 CachedQRatio(const Sentence1& s1) -> CachedQRatio<detail::sentence_char_t<Sentence1>>;
 
+// fuzz free functions (fuzz.hpp:44-49, 161-167, 229-235, 334-340, 404-410, 463-469, 523-528,
+// 587-593, 713-718): one pair through the matching Cached* scorer
+#define RFA_FREE_FUZZ(NAME, CACHED)                                                                                 \
+    template <typename InputIt1, typename InputIt2>                                                                \
+    double NAME(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2, double score_cutoff = 0)        \
+    {                                                                                                              \
+        return CACHED<rapidfuzz_amd::detail::iter_char_t<InputIt1>>(first1, last1)                                \
+            .similarity(first2, last2, score_cutoff);                                                              \
+    }                                                                                                              \
+    template <typename Sentence1, typename Sentence2>                                                              \
+    double NAME(const Sentence1& s1, const Sentence2& s2, double score_cutoff = 0)                                 \
+    {                                                                                                              \
+        return NAME(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2), detail::to_end(s2),             \
+                    score_cutoff);                                                                                 \
+    }
+RFA_FREE_FUZZ(ratio, CachedRatio)
+RFA_FREE_FUZZ(partial_ratio, CachedPartialRatio)
+RFA_FREE_FUZZ(token_sort_ratio, CachedTokenSortRatio)
+RFA_FREE_FUZZ(partial_token_sort_ratio, CachedPartialTokenSortRatio)
+RFA_FREE_FUZZ(token_set_ratio, CachedTokenSetRatio)
+RFA_FREE_FUZZ(partial_token_set_ratio, CachedPartialTokenSetRatio)
+RFA_FREE_FUZZ(token_ratio, CachedTokenRatio)
+RFA_FREE_FUZZ(partial_token_ratio, CachedPartialTokenRatio)
+RFA_FREE_FUZZ(QRatio, CachedQRatio)
+#undef RFA_FREE_FUZZ
+
 } // namespace fuzz
 
 // ---- experimental::Multi* -------------------------------------------------
@@ -1130,6 +1332,18 @@ public:
             for (size_t t = 0; t < nc; ++t) out[t * rc + i] = tmp[i * nc + t];
     }
 
+    // batch with one cutoff per inserted string (rfa_cdist_cutoffs; integer results only):
+    // out[t * result_count() + i] = the score of string i against text t, cut at score_cutoffs[i]
+    // (padding entries past the inserted strings take the default cutoff)
+    std::vector<ResType> distance_many(const Choices& choices, const std::vector<ResType>& score_cutoffs) const
+    {
+        return many_cutoffs(choices, RFA_DISTANCE, score_cutoffs, static_cast<ResType>(WorstDistance));
+    }
+    std::vector<ResType> similarity_many(const Choices& choices, const std::vector<ResType>& score_cutoffs) const
+    {
+        return many_cutoffs(choices, RFA_SIMILARITY, score_cutoffs, static_cast<ResType>(WorstSimilarity));
+    }
+
 protected:
     int query_width() const { return q_.width; }
 
@@ -1164,6 +1378,25 @@ protected:
             else
                 std::memcpy(&scores[i], &tmp[i], sizeof(double));
         }
+    }
+
+    std::vector<ResType> many_cutoffs(const Choices& choices, int kind, const std::vector<ResType>& cuts,
+                                      ResType pad_cut) const
+    {
+        static_assert(std::is_integral<ResType>::value, "per-string cutoffs apply to integer results");
+        if (cuts.size() != pos_) throw std::invalid_argument("score_cutoffs needs one entry per inserted string");
+        const size_t rc = result_count(), nc = choices.size();
+        std::vector<ResType> out(rc * nc);
+        if (!nc) return out;
+        std::vector<uint64_t> cu(rc, static_cast<uint64_t>(pad_cut));
+        for (size_t i = 0; i < cuts.size(); ++i) cu[i] = static_cast<uint64_t>(cuts[i]);
+        const int w = std::max(query_width(), choices.width());
+        std::vector<uint64_t> tmp(rc * nc);
+        rapidfuzz_amd::detail::check(
+            rfa_cdist_cutoffs(scorer(w), choices.at_width(w), kind, RFA_MULTI, cu.data(), tmp.data(), 0));
+        for (size_t i = 0; i < rc; ++i)
+            for (size_t t = 0; t < nc; ++t) out[t * rc + i] = static_cast<ResType>(tmp[i * nc + t]);
+        return out;
     }
 
     size_t input_count_;
@@ -1237,6 +1470,28 @@ template <typename Sentence1>
 explicit CachedDamerauLevenshtein(const Sentence1& s1_)
     -> CachedDamerauLevenshtein<rapidfuzz_amd::detail::sentence_char_t<Sentence1>>;
 
+// damerau_levenshtein_* (DamerauLevenshtein.hpp:31-112): one pair through a one-query scorer
+#define RFA_FREE_DL(KIND, RES, CUT)                                                                                 \
+    template <typename InputIt1, typename InputIt2>                                                                \
+    RES damerau_levenshtein_##KIND(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2,             \
+                                   RES score_cutoff = CUT)                                                         \
+    {                                                                                                              \
+        return CachedDamerauLevenshtein<rapidfuzz_amd::detail::iter_char_t<InputIt1>>(first1, last1)              \
+            .KIND(first2, last2, score_cutoff);                                                                    \
+    }                                                                                                              \
+    template <typename Sentence1, typename Sentence2>                                                              \
+    RES damerau_levenshtein_##KIND(const Sentence1& s1, const Sentence2& s2, RES score_cutoff = CUT)               \
+    {                                                                                                              \
+        return damerau_levenshtein_##KIND(rapidfuzz_amd::detail::to_begin(s1), rapidfuzz_amd::detail::to_end(s1),  \
+                                          rapidfuzz_amd::detail::to_begin(s2), rapidfuzz_amd::detail::to_end(s2),  \
+                                          score_cutoff);                                                           \
+    }
+RFA_FREE_DL(distance, size_t, std::numeric_limits<size_t>::max())
+RFA_FREE_DL(similarity, size_t, 0)
+RFA_FREE_DL(normalized_distance, double, 1.0)
+RFA_FREE_DL(normalized_similarity, double, 0.0)
+#undef RFA_FREE_DL
+
 } // namespace experimental
 
 namespace fuzz {
@@ -1252,5 +1507,175 @@ struct MultiRatio : rapidfuzz_amd::experimental::detail::MultiBase<RFA_RATIO, Ma
 
 } // namespace experimental
 } // namespace fuzz
+
+// ---- independent pairs in one launch (rfa_pairs) --------------------------------
+// out[i] = <metrics[i]>_<kind>(a[i], b[i], score_cutoff) for the free functions
+// levenshtein_* (weights {1, 1, 1}), indel_*, lcs_seq_* and osa_*: the batched form of
+// per-pair calls (Levenshtein.hpp:140-265, Indel.hpp:12-60, LCSseq.hpp:12-60, OSA.hpp:36-108),
+// pairs of any lengths mixed in one call.
+enum class PairMetric : int32_t {
+    Levenshtein = RFA_LEVENSHTEIN,
+    Indel = RFA_INDEL,
+    LCSseq = RFA_LCSSEQ,
+    OSA = RFA_OSA
+};
+
+namespace detail {
+template <typename Out>
+std::vector<Out> run_pairs(const Choices& a, const Choices& b, const std::vector<int32_t>& metrics, int kind,
+                           uint64_t cu, double cd, int schedule)
+{
+    if (a.size() != b.size()) throw std::invalid_argument("a and b must hold the same number of strings");
+    if (metrics.size() != a.size()) throw std::invalid_argument("metrics needs one entry per pair");
+    std::vector<Out> out(a.size());
+    if (!a.size()) return out;
+    const int w = std::max(a.width(), b.width());
+    check(rfa_pairs(context(), a.at_width(w), b.at_width(w), metrics.data(), kind, cu, cd, schedule, out.data(), 0));
+    return out;
+}
+inline std::vector<int32_t> same_metric(size_t n, PairMetric m) { return std::vector<int32_t>(n, static_cast<int32_t>(m)); }
+} // namespace detail
+
+inline std::vector<size_t> pairs_distance(const Choices& a, const Choices& b, const std::vector<int32_t>& metrics,
+                                          size_t score_cutoff = std::numeric_limits<size_t>::max(),
+                                          int schedule = RFA_SCHEDULE_BUCKETED)
+{
+    const uint64_t cu = std::min<uint64_t>(score_cutoff, static_cast<uint64_t>(detail::kIntMax));
+    auto v = detail::run_pairs<uint64_t>(a, b, metrics, RFA_DISTANCE, cu, 0.0, schedule);
+    return std::vector<size_t>(v.begin(), v.end());
+}
+inline std::vector<size_t> pairs_similarity(const Choices& a, const Choices& b, const std::vector<int32_t>& metrics,
+                                            size_t score_cutoff = 0, int schedule = RFA_SCHEDULE_BUCKETED)
+{
+    auto v = detail::run_pairs<uint64_t>(a, b, metrics, RFA_SIMILARITY, score_cutoff, 0.0, schedule);
+    return std::vector<size_t>(v.begin(), v.end());
+}
+inline std::vector<double> pairs_normalized_distance(const Choices& a, const Choices& b,
+                                                     const std::vector<int32_t>& metrics, double score_cutoff = 1.0,
+                                                     int schedule = RFA_SCHEDULE_BUCKETED)
+{
+    return detail::run_pairs<double>(a, b, metrics, RFA_NORMALIZED_DISTANCE, 0, score_cutoff, schedule);
+}
+inline std::vector<double> pairs_normalized_similarity(const Choices& a, const Choices& b,
+                                                       const std::vector<int32_t>& metrics,
+                                                       double score_cutoff = 0.0,
+                                                       int schedule = RFA_SCHEDULE_BUCKETED)
+{
+    return detail::run_pairs<double>(a, b, metrics, RFA_NORMALIZED_SIMILARITY, 0, score_cutoff, schedule);
+}
+// one metric for every pair
+inline std::vector<size_t> pairs_distance(const Choices& a, const Choices& b, PairMetric m,
+                                          size_t score_cutoff = std::numeric_limits<size_t>::max())
+{
+    return pairs_distance(a, b, detail::same_metric(a.size(), m), score_cutoff);
+}
+inline std::vector<size_t> pairs_similarity(const Choices& a, const Choices& b, PairMetric m, size_t score_cutoff = 0)
+{
+    return pairs_similarity(a, b, detail::same_metric(a.size(), m), score_cutoff);
+}
+inline std::vector<double> pairs_normalized_distance(const Choices& a, const Choices& b, PairMetric m,
+                                                     double score_cutoff = 1.0)
+{
+    return pairs_normalized_distance(a, b, detail::same_metric(a.size(), m), score_cutoff);
+}
+inline std::vector<double> pairs_normalized_similarity(const Choices& a, const Choices& b, PairMetric m,
+                                                       double score_cutoff = 0.0)
+{
+    return pairs_normalized_similarity(a, b, detail::same_metric(a.size(), m), score_cutoff);
+}
+
+// ---- request server (rfa_server_*) -------------------------------------------------
+// A device-resident choice set answering top-k requests with continuous batching: the
+// README's extract() loop (README.md:153-214) as a service.  No reference class counterpart.
+struct ServerConfig {
+    int metric = RFA_RATIO;
+    int kind = RFA_SIMILARITY; // a double similarity: normalized_similarity, or similarity of Jaro / JW / ratio
+    double score_cutoff = 0.0;
+    uint32_t k = 5;
+    uint32_t max_batch = 64;
+    uint32_t slots = 4;
+    uint32_t candidate_cap = 65536;
+    double prefix_weight = 0.1;
+    LevenshteinWeightTable weights = {1, 1, 1};
+};
+
+class Server {
+public:
+    explicit Server(const Choices& choices, const ServerConfig& cfg = ServerConfig())
+        : width_(choices.width()), k_(cfg.k)
+    {
+        rfa_server_config c{};
+        c.metric = cfg.metric;
+        c.kind = cfg.kind;
+        c.score_cutoff = cfg.score_cutoff;
+        c.k = cfg.k;
+        c.max_batch = cfg.max_batch;
+        c.slots = cfg.slots;
+        c.candidate_cap = cfg.candidate_cap;
+        c.prefix_weight = cfg.prefix_weight;
+        c.weights = {cfg.weights.insert_cost, cfg.weights.delete_cost, cfg.weights.replace_cost};
+        detail::check(rfa_server_create(detail::context(), choices.at_width(width_), &c, &h_));
+    }
+    ~Server() { rfa_server_destroy(h_); }
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+
+    // queues one query (code units of at most the choice set's width); returns a ticket at once
+    template <typename InputIt>
+    uint64_t submit(InputIt first, InputIt last) const
+    {
+        detail::Packed p;
+        p.add(first, last);
+        if (p.width > width_) throw std::invalid_argument("query code units are wider than the choice set's");
+        std::vector<uint8_t> bytes(std::max<size_t>(p.units.size() * static_cast<size_t>(width_), 1));
+        for (size_t i = 0; i < p.units.size(); ++i) {
+            const uint32_t u = p.units[i];
+            if (width_ == 1)
+                bytes[i] = static_cast<uint8_t>(u);
+            else if (width_ == 2) {
+                const uint16_t v = static_cast<uint16_t>(u);
+                std::memcpy(&bytes[i * 2], &v, 2);
+            }
+            else
+                std::memcpy(&bytes[i * 4], &u, 4);
+        }
+        uint64_t t = 0;
+        detail::check(rfa_server_submit(h_, bytes.data(), p.units.size(), &t));
+        return t;
+    }
+    template <typename Sentence>
+    uint64_t submit(const Sentence& q) const
+    {
+        return submit(detail::to_begin(q), detail::to_end(q));
+    }
+
+    // blocks until the request is answered: (choice index, score), best first, at most k
+    std::vector<std::pair<int64_t, double>> wait(uint64_t ticket) const
+    {
+        std::vector<int64_t> idx(std::max<uint32_t>(k_, 1));
+        std::vector<double> sc(idx.size());
+        detail::check(rfa_server_wait(h_, ticket, idx.data(), sc.data()));
+        std::vector<std::pair<int64_t, double>> out;
+        for (uint32_t j = 0; j < k_ && idx[j] >= 0; ++j) out.emplace_back(idx[j], sc[j]);
+        return out;
+    }
+    template <typename Sentence>
+    std::vector<std::pair<int64_t, double>> query(const Sentence& q) const
+    {
+        return wait(submit(q));
+    }
+
+    rfa_server_stats stats() const
+    {
+        rfa_server_stats st{};
+        detail::check(rfa_server_get_stats(h_, &st));
+        return st;
+    }
+
+private:
+    int width_;
+    uint32_t k_;
+    rfa_server* h_ = nullptr;
+};
 
 } // namespace rapidfuzz_amd

@@ -326,6 +326,169 @@ int main()
             }
         }
         {
+            // free functions: known answers (tests-Levenshtein.cpp:144-179, tests-Indel.cpp:204-215,
+            // tests-OSA.cpp:62-97, tests-fuzz.cpp:63-99, tests-Hamming.cpp:99-108)
+            CHECK(rfa::levenshtein_distance(std::string("South Korea"), std::string("North Korea")) == 2);
+            CHECK(rfa::levenshtein_distance(std::string("South Korea"), std::string("North Korea"), {1, 1, 2}) == 4);
+            CHECK(rfa::levenshtein_distance(std::string("South Korea"), std::string("North Korea"), {1, 1, 2}, 2) == 3);
+            CHECK(rfa::levenshtein_distance("aabc", "cccd", {1, 1, 1}, 2) == 3);
+            const std::string sk = "South Korea", nk = "North Korea";
+            CHECK(rfa::levenshtein_distance(sk.begin(), sk.end(), nk.begin(), nk.end()) == 2);
+            CHECK(rfa::levenshtein_similarity(sk, nk) == 9);
+            CHECK(near(rfa::levenshtein_normalized_distance(sk, nk), 2.0 / 11.0));
+            CHECK(near(rfa::levenshtein_normalized_similarity(sk, nk), 9.0 / 11.0));
+            CHECK(rfa::levenshtein_normalized_similarity(sk, nk, {1, 1, 1}, 0.9) == 0.0);
+            CHECK(rfa::indel_distance(std::string("aaaa"), std::string("bbbb")) == 8);
+            CHECK(rfa::indel_similarity(std::string("aaaa"), std::string("aaaa")) == 8);
+            CHECK(rfa::lcs_seq_similarity(std::string("aaaa"), std::string("aaab")) == 3);
+            CHECK(rfa::lcs_seq_distance(std::string("aaaa"), std::string("aaab")) == 1);
+            CHECK(rfa::osa_distance(std::string("CA"), std::string("ABC")) == 3);
+            CHECK(rfa::osa_distance(std::string("CA"), std::string("AC")) == 1);
+            CHECK(rfa::osa_distance(std::string("aaaa"), std::string(""), 1) == 2);
+            CHECK(near(rfa::jaro_winkler_similarity(std::string("0"), std::string("00")), 0.85));
+            CHECK(near(rfa::jaro_similarity(std::string("00"), std::string("00")), 1.0));
+            CHECK(near(rfa::jaro_winkler_distance(std::string("0"), std::string("00"), 0.1), 0.15));
+            CHECK(rfa::hamming_distance(std::string("abaa"), std::string("aaba")) == 2);
+            CHECK(rfa::hamming_distance(std::string("aaaa"), std::string("aaaaa")) == 1);
+            CHECK(throws_invalid([] { (void)rfa::hamming_distance(std::string("aaaa"), std::string("aaaaa"), false); }));
+            CHECK(rfa::prefix_similarity(std::string("prefix_sample"), std::string("prefix_other")) == 7);
+            CHECK(rfa::postfix_similarity(std::string("sample_postfix"), std::string("other_postfix")) == 8);
+            CHECK(rfa::experimental::damerau_levenshtein_distance(std::string("CA"), std::string("ABC")) == 2);
+            CHECK(near(rfa::fuzz::ratio(std::string("new york mets"), std::string("the wonderful new york mets")), 65.0));
+            CHECK(near(rfa::fuzz::ratio(std::string("new york mets"), std::string("the wonderful new york mets"), 70.0),
+                       0.0));
+            CHECK(near(rfa::fuzz::partial_ratio(std::string("the wonderful new york mets"), std::string("new york mets")),
+                       100.0));
+            CHECK(near(rfa::fuzz::token_sort_ratio(std::string("new york mets"), std::string("mets york new")), 100.0));
+            const std::string s4 = "new york mets vs atlanta braves", s5 = "atlanta braves vs new york mets";
+            CHECK(near(rfa::fuzz::token_set_ratio(s4, s5), 100.0));
+            CHECK(near(rfa::fuzz::token_ratio(s4, s5), 100.0));
+            CHECK(near(rfa::fuzz::partial_token_set_ratio(s4, std::string("new york city mets - atlanta braves")), 100.0));
+            CHECK(near(rfa::fuzz::QRatio(std::string("test"), std::string("")), 0.0));
+            CHECK(near(rfa::fuzz::partial_token_sort_ratio(std::string(""), std::string("")), 100.0));
+            CHECK(near(rfa::fuzz::partial_token_ratio(std::string(""), std::string("")), 100.0));
+            // free functions on C strings and wide strings
+            CHECK(rfa::levenshtein_distance("kitten", "sitting") == 3);
+            CHECK(rfa::indel_distance(std::u32string(U"k\u00e4se"), std::u32string(U"kase")) == 2);
+        }
+        {
+            // pairs_*: one launch over many pairs == the per-pair free functions
+            std::mt19937 rng(11);
+            std::vector<std::string> as, bs;
+            std::vector<int32_t> mets;
+            for (int i = 0; i < 400; ++i) {
+                as.push_back(random_string(rng, 0, i % 50 == 0 ? 700 : 90));
+                bs.push_back(random_string(rng, 0, 90));
+                mets.push_back(i % 4);
+            }
+            const rfa::Choices A(as), B(bs);
+            const auto pd = rfa::pairs_distance(A, B, mets);
+            const auto pdc = rfa::pairs_distance(A, B, mets, 7);
+            const auto ps = rfa::pairs_similarity(A, B, mets);
+            const auto pnd = rfa::pairs_normalized_distance(A, B, mets, 0.4);
+            const auto pns = rfa::pairs_normalized_similarity(A, B, mets, 0.6);
+            const auto lev = rfa::pairs_distance(A, B, rfa::PairMetric::Levenshtein);
+            const auto plain = rfa::pairs_distance(A, B, mets, std::numeric_limits<size_t>::max(), RFA_SCHEDULE_PLAIN);
+            for (size_t i = 0; i < as.size(); i += 3) {
+                size_t d, dc, sim;
+                double nd, ns;
+                switch (mets[i]) {
+                case 0:
+                    d = rfa::levenshtein_distance(as[i], bs[i]);
+                    dc = rfa::levenshtein_distance(as[i], bs[i], {1, 1, 1}, 7);
+                    sim = rfa::levenshtein_similarity(as[i], bs[i]);
+                    nd = rfa::levenshtein_normalized_distance(as[i], bs[i], {1, 1, 1}, 0.4);
+                    ns = rfa::levenshtein_normalized_similarity(as[i], bs[i], {1, 1, 1}, 0.6);
+                    break;
+                case 1:
+                    d = rfa::indel_distance(as[i], bs[i]);
+                    dc = rfa::indel_distance(as[i], bs[i], 7);
+                    sim = rfa::indel_similarity(as[i], bs[i]);
+                    nd = rfa::indel_normalized_distance(as[i], bs[i], 0.4);
+                    ns = rfa::indel_normalized_similarity(as[i], bs[i], 0.6);
+                    break;
+                case 2:
+                    d = rfa::lcs_seq_distance(as[i], bs[i]);
+                    dc = rfa::lcs_seq_distance(as[i], bs[i], 7);
+                    sim = rfa::lcs_seq_similarity(as[i], bs[i]);
+                    nd = rfa::lcs_seq_normalized_distance(as[i], bs[i], 0.4);
+                    ns = rfa::lcs_seq_normalized_similarity(as[i], bs[i], 0.6);
+                    break;
+                default:
+                    d = rfa::osa_distance(as[i], bs[i]);
+                    dc = rfa::osa_distance(as[i], bs[i], 7);
+                    sim = rfa::osa_similarity(as[i], bs[i]);
+                    nd = rfa::osa_normalized_distance(as[i], bs[i], 0.4);
+                    ns = rfa::osa_normalized_similarity(as[i], bs[i], 0.6);
+                    break;
+                }
+                CHECK(pd[i] == d);
+                CHECK(pdc[i] == dc);
+                CHECK(ps[i] == sim);
+                CHECK(pnd[i] == nd);
+                CHECK(pns[i] == ns);
+                CHECK(plain[i] == d);
+                CHECK(lev[i] == rfa::levenshtein_distance(as[i], bs[i]));
+            }
+            CHECK(throws_invalid([&] { (void)rfa::pairs_distance(A, rfa::Choices(std::vector<std::string>{"x"}), mets); }));
+        }
+        {
+            // Server: every answer equals extract_topk of the same query on the same set
+            std::mt19937 rng(13);
+            std::vector<std::string> cs;
+            for (int i = 0; i < 2000; ++i) cs.push_back(random_string(rng, 4, 40));
+            const rfa::Choices set(cs);
+            rfa::ServerConfig cfg;
+            cfg.metric = RFA_RATIO;
+            cfg.kind = RFA_SIMILARITY;
+            cfg.score_cutoff = 50.0;
+            cfg.k = 5;
+            rfa::Server srv(set, cfg);
+            std::vector<std::string> qs;
+            std::vector<uint64_t> tickets;
+            for (int i = 0; i < 24; ++i) {
+                qs.push_back(random_string(rng, 4, 40));
+                tickets.push_back(srv.submit(qs.back()));
+            }
+            for (size_t i = 0; i < qs.size(); ++i) {
+                const auto got = srv.wait(tickets[i]);
+                const auto exp = rfa::fuzz::CachedRatio<char>(qs[i]).extract_topk(set, 5, 50.0);
+                CHECK(got == exp);
+            }
+            const auto self = srv.query(cs[17]);
+            CHECK(!self.empty() && self[0].second == 100.0);
+            CHECK(srv.stats().requests >= 25);
+            CHECK(throws_invalid([&] { (void)srv.submit(std::u32string(U"\u4e2d")); }));
+        }
+        {
+            // Multi* with one cutoff per inserted string == one cutoff at a time
+            std::mt19937 rng(17);
+            rfa::experimental::MultiLevenshtein<32> ml(5);
+            rfa::experimental::MultiIndel<32> mi(5);
+            std::vector<std::string> qs;
+            for (int i = 0; i < 5; ++i) {
+                qs.push_back(random_string(rng, 1, 32));
+                ml.insert(qs.back());
+                mi.insert(qs.back());
+            }
+            std::vector<std::string> ts;
+            for (int i = 0; i < 50; ++i) ts.push_back(random_string(rng, 0, 40));
+            const rfa::Choices tc(ts);
+            const std::vector<size_t> cuts = {0, 1, 3, 6, 40};
+            const auto dm = ml.distance_many(tc, cuts);
+            const auto sm = mi.similarity_many(tc, cuts);
+            const size_t rc = ml.result_count();
+            std::vector<size_t> one(rc);
+            for (size_t t = 0; t < ts.size(); ++t)
+                for (size_t i = 0; i < qs.size(); ++i) {
+                    ml.distance(one.data(), rc, ts[t], cuts[i]);
+                    CHECK(dm[t * rc + i] == one[i]);
+                    mi.similarity(one.data(), mi.result_count(), ts[t], cuts[i]);
+                    CHECK(sm[t * mi.result_count() + i] == one[i]);
+                }
+            CHECK(throws_invalid([&] { (void)ml.distance_many(tc, std::vector<size_t>{1, 2}); }));
+        }
+        {
             rfa::experimental::MultiIndel<64> mi(1);
             mi.insert(std::string(64, 'a'));
             std::vector<size_t> s(mi.result_count());
