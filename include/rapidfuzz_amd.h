@@ -121,6 +121,11 @@ void* rfa_context_stream(rfa_context* ctx);
  * its words), and pairs whose text has this many units or more are scheduled
  * first.  rfa_pairs_autotune sets it by measurement. */
 #define RFA_OPTION_PAIRS_CHAIN 2
+/* RFA_OPTION_CDIST_BAND (default 1): cdist of 8-bit queries over 64 units under a
+ * Levenshtein distance cutoff k of at most 4094 runs the diagonal band kernel
+ * (a window of ceil((k + 2) / 64) words per pair, several pairs per wave); 0 =
+ * the one-wave-per-pair block kernel for every long query (same results). */
+#define RFA_OPTION_CDIST_BAND 3
 rfa_status rfa_context_set_option(rfa_context* ctx, int option, int64_t value);
 /* Message of the last failed call on this thread ("" if none). */
 const char* rfa_last_error(void);

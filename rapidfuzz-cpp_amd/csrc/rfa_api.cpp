@@ -155,12 +155,14 @@ void plan_scorer(rfa_scorer* sc)
     for (uint32_t qi : sc->block_queries) sc->block_min_len = std::min(sc->block_min_len, q->len(qi));
     if (q->width == 1) { // 8-bit block patterns are sized by their distinct symbols
         uint32_t rows = 0;
+        sc->q_rows.assign(q->count, 0);
         for (uint32_t qi : sc->block_queries) {
             bool seen[256] = {};
             uint32_t d = 0;
             for (uint64_t pos = q->h_off[qi]; pos < q->h_off[qi + 1]; ++pos)
                 if (!seen[q->h_chars[pos]]) seen[q->h_chars[pos]] = true, ++d;
             rows = std::max(rows, d + 1);
+            sc->q_rows[qi] = (uint16_t)(d + 1);
         }
         sc->block_u8_rows = rows;
     }
@@ -359,6 +361,7 @@ static void clear_plan(rfa_scorer* sc)
     sc->block_queries.clear();
     sc->block_max_len = sc->block_min_len = 0;
     sc->block_u8_rows = 0;
+    sc->q_rows.clear();
     sc->hash_bits = 6;
     sc->few_plan.reset();
 }
@@ -960,6 +963,58 @@ void cdist_dl(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, void
     }
 }
 
+// Long 8-bit queries under a narrow Levenshtein distance cutoff run the diagonal band
+// kernel: Cached distance with uniform weights, band_words(k) <= 64 and the query's
+// padded PM rows within BAND_LDS_MAX.  The other block queries are returned in `rest`.
+void run_band_queries(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, int algo, int kind, int semantics,
+                      const uint64_t* qcut_host, const uint64_t* d_qcut, void* dout, std::vector<uint32_t>& rest)
+{
+    constexpr uint64_t BAND_LDS_MAX = 64 * 1024;
+    rfa_context* ctx = sc->ctx;
+    rest.clear();
+    const bool kernel_ok = kind == RFA_DISTANCE && semantics == RFA_CACHED && algo == ALGO_LEV && sc->q->width == 1 &&
+                           spec.ins > 0 && sc->q_rows.size() == sc->q->count && !ctx->no_band;
+    std::vector<uint32_t> bandq;
+    uint64_t lds = 0;
+    bool multi = false;
+    for (uint32_t qi : sc->block_queries) {
+        const uint64_t m = sc->q->len(qi);
+        const uint64_t cut = qcut_host ? qcut_host[qi] : spec.cutoff_u;
+        const uint64_t k = host_ceil_div(cut, spec.ins);
+        const bool ok = kernel_ok && k < m && band_words((uint32_t)std::min<uint64_t>(k, 1u << 20)) <= BAND_MAX_WORDS &&
+                        band_lds_bytes((uint32_t)m, (uint32_t)k, sc->q_rows[qi]) <= BAND_LDS_MAX;
+        if (!ok) {
+            rest.push_back(qi);
+            continue;
+        }
+        bandq.push_back(qi);
+        lds = std::max(lds, band_lds_bytes((uint32_t)m, (uint32_t)k, sc->q_rows[qi]));
+        multi = multi || band_words((uint32_t)k) > 1;
+    }
+    if (bandq.empty() || ch->count == 0) return;
+    BandArgs a{};
+    a.qchars = (const uint8_t*)sc->q->d_chars;
+    a.qoff = sc->q->d_off;
+    a.queries = upload(ctx->keep, ctx->band_q, bandq, ctx->stream);
+    a.cchars = (const uint8_t*)ch->d_chars;
+    a.coff = ch->d_off;
+    a.nc = (uint32_t)ch->count;
+    // about 8 workgroups per CU over the whole launch, at least 256 choices each
+    const uint64_t target = (uint64_t)ctx->num_cus * 8;
+    const uint64_t per_q = std::max<uint64_t>(1, (target + bandq.size() - 1) / bandq.size());
+    uint64_t chunk = std::max<uint64_t>(256, (ch->count + per_q - 1) / per_q);
+    chunk = (chunk + 63) / 64 * 64;
+    a.chunk = (uint32_t)chunk;
+    a.chunks_per_query = (uint32_t)((ch->count + chunk - 1) / chunk);
+    a.out_stride = ch->count;
+    a.spec = spec;
+    a.out = dout;
+    a.qcut = d_qcut;
+    const uint64_t nblocks = (uint64_t)bandq.size() * a.chunks_per_query;
+    if (nblocks > 0x7fffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "too many banded query chunks in one call");
+    HIP_CHECK(launch_band(a, (uint32_t)nblocks, (uint32_t)lds, multi, ctx->stream));
+}
+
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
                 int out_on_device, const uint64_t* qcut_host)
 {
@@ -1099,20 +1154,24 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
             }
         }
     }
-    // ---- block kernel: long (or large-alphabet) queries x all choices ----
-    if (!sc->block_queries.empty()) {
+    // ---- long (or large-alphabet) queries x all choices: the diagonal band kernel for
+    // narrow distance cutoffs, the block kernel for the rest ----
+    std::vector<uint32_t> rest_queries;
+    if (!sc->block_queries.empty()) run_band_queries(sc, ch, spec, algo, kind, semantics, qcut_host, d_qcut, dout, rest_queries);
+    if (!rest_queries.empty()) {
         BlockArgs ba{};
         ba.achars = sc->q->d_chars;
         ba.aoff = sc->q->d_off;
         ba.bchars = ch->d_chars;
         ba.boff = ch->d_off;
-        ba.item_a = sc->d_block_queries;
+        ba.item_a = rest_queries.size() == sc->block_queries.size() ? sc->d_block_queries
+                                                                    : upload(ctx->keep, ctx->items_a, rest_queries, s);
         std::vector<uint32_t> all(nc);
         std::iota(all.begin(), all.end(), 0u);
         ba.item_b = upload(ctx->keep, ctx->items_b, all, s);
         ba.cross_nb = (uint32_t)nc;
         ba.out_stride = nc;
-        uint64_t ni = (uint64_t)sc->block_queries.size() * nc;
+        uint64_t ni = (uint64_t)rest_queries.size() * nc;
         if (ni > BLOCK_MAX_ITEMS) throw ApiError(RFA_ERR_UNSUPPORTED, "too many long-query pairs in one call");
         ba.nitems = (uint32_t)ni;
         ba.spec = spec;
@@ -1120,7 +1179,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ba.qcut = d_qcut;
         // the q-gram filter can reject pairs only where 7k + 7 < m
         if (kind == RFA_DISTANCE)
-            for (uint32_t qi : sc->block_queries) {
+            for (uint32_t qi : rest_queries) {
                 const uint64_t kq = qcut_host ? qcut_host[qi] : cu;
                 if (kq < (1ull << 40) && 7 * kq + 7 < sc->q->len(qi)) {
                     ba.qgram = 1;
@@ -1507,6 +1566,11 @@ rfa_status rfa_context_set_option(rfa_context* ctx, int option, int64_t value)
             if (value < 1 || value > (int64_t)1 << 31)
                 throw ApiError(RFA_ERR_INVALID_ARGUMENT, "RFA_OPTION_PAIRS_CHAIN must be in [1, 2^31]");
             ctx->seg_chain_steps = (uint32_t)value;
+            return;
+        }
+        if (option == RFA_OPTION_CDIST_BAND) {
+            if (value != 0 && value != 1) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "RFA_OPTION_CDIST_BAND is 0 or 1");
+            ctx->no_band = value == 0;
             return;
         }
         if (option != RFA_OPTION_PAIRS_GRAPH) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown option");

@@ -1,0 +1,271 @@
+// rfa_band.hip -- banded Levenshtein distance of long queries (> 64 units,
+// 8-bit units) against a choice set under an integer cutoff k: the calls
+// CachedLevenshtein<uint8_t>(q).distance(s2, k) of a cdist with a cutoff
+// (Levenshtein_impl.hpp:641-850 is the reference's banded block recurrence,
+// :419-505 its one-word diagonal band, which this kernel widens to many words).
+//
+// Band.  A pair (query P of m units, choice T of n units, d = m - n, |d| <= k)
+// reaches a distance <= k only through cells on the diagonals i - j in
+// [dlo, dhi] = [ceil((d - k) / 2), floor((d + k) / 2)]: a path pays |i - j| to
+// reach cell (i, j) and |d - (i - j)| to leave it.
+//
+// Diagonal window.  The pair keeps B = ceil((k + 2) / 64) 64-bit words that
+// slide one row down per text unit: before text unit j (0-based) bit b of the
+// window (word b / 64 in lane b / 64 of the pair's lanes) holds the vertical
+// delta D[r][j] - D[r - 1][j] of row r = j + dlo + b.  Text unit j evaluates
+// column j + 1 on the same rows (the reference's one-word band step: D0 from
+// the addition over the window, HP and HN unshifted, D0 shifted down one row
+// for the next window), with the addition's carry passed across the words.
+// The window's top row takes no carry from above and the row entering at the
+// bottom takes no diagonal zero; both only over-estimate cells on diagonals
+// dlo - 1 and dlo + 64 B - 1, outside the band, so every cell on a path of
+// cost <= k stays exact.  Rows at or above row 0 start with vertical delta 0
+// and never match, so they copy row 0 (D[0][j] = j).
+//
+// Score and early exit.  D[m][n] lies on diagonal d; the lane holding that
+// diagonal adds 1 - D0 of it per text unit.  Every lane also follows the
+// bottom cell of its word (diagonal dL).  A word's cells on diagonals
+// dL - 63 .. dL are at least bottom - (dL - diagonal) and still owe
+// |d - diagonal| to reach (m, n); along any path that sum never decreases, so
+// once it exceeds k for every word the distance is > k and the pair ends with
+// k + 1 (checked every 8 text units).
+//
+// Layout.  One workgroup per (query, chunk of choices).  The query's PM rows
+// live in LDS as dwords, padded with zero words on both sides so any window
+// position reads three consecutive dwords and two funnel shifts
+// (v_alignbit_b32); a byte -> row-offset map selects the row.  A wave runs
+// floor(64 / B) pairs at once, B lanes each: the addition carries between a
+// pair's words resolve by ballot carry-lookahead, and the bit D0 shifts down
+// from the next word comes from one ballot.  A pair that ends hands its lanes
+// to the next candidate choice of the chunk (choices are length-filtered 64 at
+// a time), so early exits and uneven lengths keep the wave busy.
+#include "rfa_block_impl.hpp"
+#include "rfa_device.hpp"
+#include "rfa_internal.hpp"
+#include "rfa_semantics.hpp"
+#include "rfa_steps.hpp"
+
+namespace rfa {
+
+// One text unit: the window's match bits are three dwords of the symbol's PM row
+// at bit offset g (two funnel shifts), then the band step.  `count` false leaves
+// the score as it is (the pair's text ended earlier in the chunk).
+template <bool MULTI>
+__device__ __forceinline__ void band_step(const uint32_t* __restrict__ pmd, uint32_t row, uint32_t g, uint64_t LASTM,
+                                          uint32_t lane, uint64_t& VP, uint64_t& VN, int32_t& sc, int32_t& score,
+                                          uint32_t bitD, bool count)
+{
+    const uint32_t dw = row + (g >> 5);
+    const uint32_t w0 = pmd[dw], w1 = pmd[dw + 1], w2 = pmd[dw + 2];
+    const uint32_t Xl = __builtin_amdgcn_alignbit(w1, w0, g), Xh = __builtin_amdgcn_alignbit(w2, w1, g);
+    const uint32_t VPl = (uint32_t)VP, VPh = (uint32_t)(VP >> 32), VNl = (uint32_t)VN, VNh = (uint32_t)(VN >> 32);
+    uint64_t sum = ((((uint64_t)(Xh & VPh)) << 32) | (Xl & VPl)) + VP;
+    if constexpr (MULTI) {
+        // carry into each word: generated where the word's own add overflows,
+        // propagated where its sum is all ones (no overflow is possible then)
+        const uint64_t G = ballot(sum < VP) & ~LASTM, Pp = ballot(sum == ~0ull) & ~LASTM;
+        sum += lane_bit(((G | Pp) + G) ^ Pp, lane);
+    }
+    const uint32_t D0l = bop3<T_XOR_OR>((uint32_t)sum, VPl, Xl | VNl);
+    const uint32_t D0h = bop3<T_XOR_OR>((uint32_t)(sum >> 32), VPh, Xh | VNh);
+    const uint32_t HPl = bop3<T_OR_NOR>(VNl, D0l, VPl), HPh = bop3<T_OR_NOR>(VNh, D0h, VPh);
+    const uint32_t HNl = D0l & VPl, HNh = D0h & VPh;
+    // D0 one row down: bit 0 of the next word enters at the top
+    uint32_t in = 0;
+    if constexpr (MULTI) in = lane_bit((ballot((D0l & 1) != 0) >> 1) & ~LASTM, lane);
+    const uint32_t Sl = __builtin_amdgcn_alignbit(D0h, D0l, 1), Sh = __builtin_amdgcn_alignbit(in, D0h, 1);
+    VP = ((uint64_t)bop3<T_OR_NOR>(HNh, Sh, HPh) << 32) | bop3<T_OR_NOR>(HNl, Sl, HPl);
+    VN = ((uint64_t)(Sh & HPh) << 32) | (Sl & HPl);
+    sc += 1 + ((int32_t)D0h >> 31);
+    const uint32_t zb = (uint32_t)((((uint64_t)D0h << 32) | D0l) >> bitD) & 1u;
+    score += count ? 1 - (int32_t)zb : 0;
+}
+
+// The diagonal window above is worth its bookkeeping while it stays narrow:
+// band_words(k) <= 64 words, one lane each.
+template <bool MULTI>
+__global__ __launch_bounds__(BAND_THREADS) void k_band(BandArgs a)
+{
+    extern __shared__ uint32_t smem_band[];
+    uint32_t* rowmap = smem_band;    // [256] dword offset of each symbol's PM row (row 0: zeros)
+    uint32_t* pmd = smem_band + 256; // PM rows, 2 * WP dwords each
+    __shared__ uint32_t s_next, s_rows;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t qslot = blockIdx.x / a.chunks_per_query;
+    const uint32_t chunk = blockIdx.x - qslot * a.chunks_per_query;
+    const uint32_t qi = a.queries[qslot];
+    const uint64_t qo = a.qoff[qi];
+    const uint32_t m = (uint32_t)(a.qoff[qi + 1] - qo);
+    const uint8_t* __restrict__ P = a.qchars + qo;
+    ScoreSpec spec = a.spec;
+    if (a.qcut) {
+        spec.cutoff_u = a.qcut[qi];
+        spec.band_k = band_from_cutoff(spec);
+    }
+    const uint32_t k = (uint32_t)spec.band_k; // the host launches queries with band_words(k) <= 64 only
+    const uint32_t B = band_words(k), PAD = band_pad(k), WP = band_row_words(m, k);
+    const uint32_t c0 = chunk * a.chunk, c1 = min(a.nc, c0 + a.chunk);
+    uint64_t* __restrict__ out = (uint64_t*)a.out;
+
+    // ---- the query's PM rows ----
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) rowmap[i] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) rowmap[P[i]] = 1;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t v[4], cnt = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = rowmap[lane * 4 + q];
+            cnt += v[q];
+        }
+        uint32_t total;
+        uint32_t ex = wave_excl_scan(cnt, &total);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            rowmap[lane * 4 + q] = v[q] ? (ex + 1) * 2 * WP : 0u;
+            ex += v[q];
+        }
+        if (lane == 0) {
+            s_rows = total + 1;
+            s_next = c0;
+        }
+    }
+    __syncthreads();
+    const uint32_t ndw = s_rows * 2 * WP;
+    for (uint32_t i = threadIdx.x; i < ndw; i += blockDim.x) pmd[i] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t bit = 64 * PAD + i;
+        atomicOr(&pmd[rowmap[P[i]] + (bit >> 5)], 1u << (bit & 31));
+    }
+    __syncthreads();
+
+    // ---- pairs of B lanes ----
+    const uint32_t S = B, nseg = 64 / S;
+    const uint32_t seg = lane / S, L = lane - seg * S;
+    const bool lane_ok = seg < nseg;
+    const uint64_t LASTM = ballot(lane_ok && L == S - 1);
+    const uint32_t seg_lane0 = lane_ok ? seg * S : lane;
+    const uint64_t seg_mask = S == 64 ? ~0ull : ((1ull << S) - 1);
+    const uint64_t lt_mask = (1ull << lane) - 1;
+
+    bool busy = false;
+    uint32_t ci = 0, n = 0, jt = 0, bitD = 0;
+    bool isD = false;
+    const uint8_t* T = P;
+    int32_t g0 = 0, sc = 0, scmax = 0, score = 0;
+    uint64_t VP = 0, VN = 0;
+    // candidates of the chunk: lane r holds the r-th pending one (wave-uniform counts)
+    uint32_t pend_ci = 0, pend_n = 0, pend_off = 0;
+    bool exhausted = false;
+
+    for (;;) {
+        // ---- hand idle lanes the next candidates ----
+        for (;;) {
+            const uint64_t idle = ballot(lane_ok && L == 0 && !busy);
+            if (idle == 0) break;
+            if (pend_off >= pend_n) {
+                if (exhausted) break;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&s_next, 64u);
+                base = readlane(base, 0);
+                if (base >= c1) {
+                    exhausted = true;
+                    break;
+                }
+                // 64 choices: the lengths settle most of them
+                const uint32_t cj = base + lane;
+                bool cand = false;
+                if (cj < c1) {
+                    const uint64_t o = a.coff[cj];
+                    const uint32_t nn = (uint32_t)(a.coff[cj + 1] - o);
+                    // the reference's length filter (Levenshtein_impl.hpp:921-922); an
+                    // empty choice is m deletions
+                    const uint32_t diff = m > nn ? m - nn : nn - m;
+                    const bool decided = diff > k || nn == 0;
+                    if (decided) out[(uint64_t)qi * a.out_stride + cj] = finalize_int(spec, m, nn, diff > k ? (uint64_t)k + 1 : m);
+                    cand = !decided;
+                }
+                const uint64_t cm = ballot(cand);
+                const uint32_t np = (uint32_t)__popcll(cm);
+                const uint32_t dst = cand ? (uint32_t)__popcll(cm & lt_mask) : np + (uint32_t)__popcll(~cm & lt_mask);
+                pend_ci = (uint32_t)__builtin_amdgcn_ds_permute((int)(dst * 4), (int)cj);
+                pend_n = uniform(np);
+                pend_off = 0;
+                continue;
+            }
+            const uint32_t navail = pend_n - pend_off;
+            const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
+            const bool take = ((idle >> lane) & 1) && rank < navail;
+            uint32_t got = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((pend_off + (take ? rank : 0u)) * 4), (int)pend_ci);
+            got = take ? got : ~0u;
+            got = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(seg_lane0 * 4), (int)got);
+            pend_off = uniform(pend_off + min(navail, (uint32_t)__popcll(idle)));
+            if (lane_ok && !busy && got != ~0u) {
+                ci = got;
+                const uint64_t o = a.coff[ci];
+                n = (uint32_t)(a.coff[ci + 1] - o);
+                T = a.cchars + o;
+                const int32_t d = (int32_t)m - (int32_t)n;
+                const int32_t s = -(int32_t)((uint32_t)((int32_t)k - d) >> 1); // dlo
+                g0 = s - 1 + 64 * (int32_t)(L + PAD);
+                const int32_t dL = s + 64 * (int32_t)L + 62, dt = dL - 63;
+                const int32_t gmin = dt <= d ? d : 2 * dt - d;
+                scmax = (int32_t)k + dL - gmin;
+                sc = dL > 0 ? dL : 0;
+                const int32_t sh = 1 - s - 64 * (int32_t)L; // first window bit at row >= 1
+                VP = sh <= 0 ? ~0ull : (sh >= 64 ? 0ull : (~0ull << sh));
+                VN = 0;
+                const int32_t bD = d - s + 1;
+                isD = (uint32_t)(bD >> 6) == L;
+                bitD = (uint32_t)bD & 63u;
+                score = d > 0 ? d : 0;
+                jt = 0;
+                busy = true;
+            }
+        }
+        if (ballot(busy) == 0) break;
+
+        // ---- 8 text units of every running pair ----
+        uint32_t u[8], ro[8];
+        load_units<uint8_t, 8>(busy ? T + jt : P, u);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ro[q] = rowmap[u[q]];
+        // idle lanes step too (their segments' state is reset when they take a pair, and
+        // LASTM keeps their carries inside their segments); only a pair that ends inside
+        // this chunk must stop counting its score
+        if (ballot(busy && jt + 8 > n) == 0) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                band_step<MULTI>(pmd, ro[q], (uint32_t)(g0 + (int32_t)jt + q), LASTM, lane, VP, VN, sc, score, bitD, true);
+        }
+        else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                band_step<MULTI>(pmd, ro[q], (uint32_t)(g0 + (int32_t)jt + q), LASTM, lane, VP, VN, sc, score, bitD,
+                                 jt + (uint32_t)q < n);
+        }
+        if (busy) jt += 8;
+        const uint64_t alive = ballot(busy && sc <= scmax);
+        const bool seg_alive = ((alive >> seg_lane0) & seg_mask) != 0;
+        const bool fin = busy && jt >= n;
+        const bool dead = busy && !fin && !seg_alive;
+        if ((fin && isD) || (dead && L == 0)) {
+            const uint64_t raw = fin && score <= (int32_t)k ? (uint64_t)score : (uint64_t)k + 1;
+            out[(uint64_t)qi * a.out_stride + ci] = finalize_int(spec, m, n, raw);
+        }
+        if (fin || dead) busy = false;
+    }
+}
+
+hipError_t launch_band(const BandArgs& a, uint32_t nblocks, uint32_t lds_bytes, bool multi, hipStream_t s)
+{
+    if (multi)
+        hipLaunchKernelGGL((k_band<true>), dim3(nblocks), dim3(BAND_THREADS), lds_bytes, s, a);
+    else
+        hipLaunchKernelGGL((k_band<false>), dim3(nblocks), dim3(BAND_THREADS), lds_bytes, s, a);
+    return hipGetLastError();
+}
+
+} // namespace rfa

@@ -213,7 +213,7 @@ struct rfa_context {
     rfa::DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4], plan_keys, plan_vals,
         plan_temp;
     rfa::DevBuf topk_scores, topk_run[2], topk_cand_s, topk_cand_i, topk_counts, topk_q, topk_out, qcut;
-    rfa::DevBuf ed_items, ed_rows, ed_ops;
+    rfa::DevBuf ed_items, ed_rows, ed_ops, band_q;
     rfa::DevBuf tset_cols, tset_rows; // partial_token_ratio: the repeat columns and rows
     size_t lane_arena_bytes_zeroed = 0;
     uint32_t gen_next = 1;
@@ -222,6 +222,7 @@ struct rfa_context {
     rfa::PairsGraph pairs_graph;
     bool pairs_graph_on = false; // RFA_OPTION_PAIRS_GRAPH
     uint32_t seg_chain_steps = rfa::SEG_CHAIN_STEPS; // RFA_OPTION_PAIRS_CHAIN, set by rfa_pairs_autotune
+    bool no_band = false; // RFA_OPTION_CDIST_BAND = 0
     void* noalloc_scratch = nullptr; // this device's static workspace (rfa_pairs_noalloc)
 };
 
@@ -307,6 +308,7 @@ struct rfa_scorer {
     uint64_t block_max_len = 0, block_min_len = 0;
     std::shared_ptr<void> few_plan; // cached plan of the one-pair-per-lane path (last choice set)
     uint32_t block_u8_rows = 0; // distinct symbols + 1 of the 8-bit block queries (0: unknown)
+    std::vector<uint16_t> q_rows; // 8-bit queries: distinct symbols + 1 of each block query (others 0)
     // metric the caller asked for (metric holds the kernels' metric: token-sort
     // scorers run ratio / partial_ratio on token-sorted copies, QRatio runs ratio)
     int api_metric = 0;

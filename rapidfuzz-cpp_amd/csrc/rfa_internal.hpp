@@ -87,6 +87,39 @@ constexpr uint64_t BLOCK_MAX_ITEMS = 0xffffffffull - (1ull << 26);
 uint64_t block_arena_bytes(int char_width, uint32_t max_m, uint32_t rows = 0);
 hipError_t launch_block(int algo, int char_width, const BlockArgs& a, uint32_t nblocks, uint32_t slots, hipStream_t s);
 
+// Banded Levenshtein of long 8-bit queries x a choice set (rfa_band.hip): one
+// workgroup per (query, chunk of choices), a diagonal window of band_words(k)
+// 64-bit words per pair, floor(64 / band_words(k)) pairs per wave.
+constexpr uint32_t BAND_THREADS = 256;
+constexpr uint32_t BAND_MAX_WORDS = 64;
+__host__ __device__ inline uint32_t band_words(uint32_t k) { return (k + 2 + 63) / 64; }
+__host__ __device__ inline uint32_t band_pad(uint32_t k) { return (k + 1 + 63) / 64 + 1; }
+// uint64 words of one zero-padded PM row of a query of m units under cutoff k
+__host__ __device__ inline uint32_t band_row_words(uint32_t m, uint32_t k)
+{
+    return band_pad(k) + (m + 63) / 64 + (k + 64 * band_words(k) + 63) / 64 + 2;
+}
+// dynamic LDS of one workgroup: the byte -> row map and `rows` PM rows
+inline uint64_t band_lds_bytes(uint32_t m, uint32_t k, uint32_t rows)
+{
+    return 1024 + (uint64_t)rows * band_row_words(m, k) * 8;
+}
+struct BandArgs {
+    const uint8_t* qchars;
+    const uint64_t* qoff;
+    const uint32_t* queries; // query per slot; workgroup b runs slot b / chunks_per_query
+    const uint8_t* cchars;
+    const uint64_t* coff;
+    uint32_t nc;             // choices
+    uint32_t chunk;          // choices per workgroup
+    uint32_t chunks_per_query;
+    uint64_t out_stride;
+    ScoreSpec spec;          // kind RFA_DISTANCE, Levenshtein; band_k the raw cutoff
+    void* out;               // out[q * out_stride + c]
+    const uint64_t* qcut;    // integer cutoff per query (nullptr: spec.cutoff_u)
+};
+hipError_t launch_band(const BandArgs& a, uint32_t nblocks, uint32_t lds_bytes, bool multi, hipStream_t s);
+
 // Lane kernel (rfa_lane.hip): one pair per lane, per-lane 128-slot maps per 64-unit block.
 struct LaneArgs {
     const void* achars;

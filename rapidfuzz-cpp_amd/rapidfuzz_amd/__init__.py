@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "librapidfuzz_amd.so")
 DISTANCE, SIMILARITY, NORMALIZED_DISTANCE, NORMALIZED_SIMILARITY = range(4)
 CACHED, MULTI = 0, 1
 SCHEDULE_BUCKETED, SCHEDULE_PLAIN = 0, 1
-OPTION_PAIRS_GRAPH, OPTION_PAIRS_CHAIN = 1, 2
+OPTION_PAIRS_GRAPH, OPTION_PAIRS_CHAIN, OPTION_CDIST_BAND = 1, 2, 3
 SCORER_SIGNED_UNITS = 1
 INT64_MAX = (1 << 63) - 1
 
