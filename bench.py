@@ -36,6 +36,10 @@ SEED = 20261203
 # recurrence lines): Levenshtein_impl.hpp:285-302 (22), LCSseq_impl.hpp:226-229 (5),
 # OSA_impl.hpp:51-70 (27), Jaro_impl.hpp:114-128 per text char (11)
 OPS = {0: 44, 1: 10, 3: 54, 5: 22}
+# VALU instructions per 64-bit window word-step of the band kernel's inner loop (gfx950): the
+# funnel read of the PM window, the addition with its ballot carry-lookahead, the D0 / HP / HN /
+# VP / VN terms as 32-bit halves (bitop3), the D0 shift across words, and the two score counters
+BAND_GFX_OPS = 29
 NAMES = {0: "levenshtein", 1: "indel", 3: "osa", 5: "jaro_winkler"}
 
 
@@ -1149,6 +1153,33 @@ def workload_dna(args, emit=True):
         entry = {"ms": round(ms, 2), "gcups": round(cells / (ms / 1e3) / 1e9, 1), "pairs_within_cutoff": within}
         print(f"# dna cutoff {frac:.0%}: {ms:.2f} ms, {entry['gcups']} GCUPS (full-matrix cells), "
               f"{within} pairs within the cutoff", file=sys.stderr, flush=True)
+        # the band kernel's own count of the word-steps it evaluated (one more, untimed call)
+        ctx.set_option(rfa.OPTION_COUNT_WORD_STEPS, 1)
+        step()
+        wsteps = ctx.counter(rfa.COUNTER_BAND_WORD_STEPS)
+        npairs = ctx.counter(rfa.COUNTER_BAND_PAIRS)
+        ctx.set_option(rfa.OPTION_COUNT_WORD_STEPS, 0)
+        cus = hip.device_attribute(hip.ATTR_CU_COUNT)
+        peak_tops = cus * 64 * hip.device_attribute(hip.ATTR_CLOCK_RATE_KHZ) * 1e3 / 1e12
+        gfx_tops = BAND_GFX_OPS * wsteps / (ms / 1e3) / 1e12
+        x86_tops = 42 * wsteps / (ms / 1e3) / 1e12
+        prof = read_profile_facts(os.path.join(ROOT, "profiles", "band_dna_256x16384.md"))
+        entry["roofline"] = {
+            "bound": "valu", "achieved": round(gfx_tops, 3), "peak": round(peak_tops, 3),
+            "unit": "TOP/s (INT32 VALU lane-operations)", "frac": round(gfx_tops / peak_tops, 4),
+            "traffic": prof.get(f"traffic_bytes_{int(frac * 100)}pct"),
+            "traffic_unit": "HBM bytes per call, from FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy",
+            "word_steps": wsteps, "pairs_run": npairs,
+            "ops_model": f"{BAND_GFX_OPS} VALU instructions per 64-bit window word-step (the gfx950 band step of "
+                         "the inner loop) x the word-steps the kernel counted on the device",
+            "x86_priced": {"achieved": round(x86_tops, 3), "frac_of_peak": round(x86_tops / peak_tops, 4),
+                           "ops_model": "42 INT32 ops per word-step (2 x the 21 64-bit ops of the reference's block "
+                                        "word, Levenshtein_impl.hpp:700-736)"},
+            "kernel": "k_band<multi-word> (diagonal window, floor(64 / words) pairs per wave)",
+            "valu_issue_utilization": prof.get(f"valu_issue_utilization_{int(frac * 100)}pct"),
+            "source": "profiles/band_dna_256x16384.md"}
+        print(f"# dna cutoff {frac:.0%}: {wsteps:.4g} word-steps over {npairs} pairs run, "
+              f"{gfx_tops:.2f} TOP/s ({gfx_tops / peak_tops:.3f} of peak)", file=sys.stderr, flush=True)
         if not args.no_cpu:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import oracle_py
@@ -1181,6 +1212,8 @@ def workload_dna(args, emit=True):
             "config": {"workload": f"{NQ3} ACGT queries len 1000-10000 x {NC3} candidates (half derived: 1% sub, "
                                    f"0.5% ins, 0.5% del; half unrelated), cutoff floor(f*m), f in 5%, 20%",
                        "cells": cells},
+            "roofline": res["cutoff_5pct"].get("roofline"), "cpu_baseline": res["cutoff_5pct"].get("cpu_baseline"),
+            "check": res["cutoff_5pct"].get("check"),
             **res}
     scorer.close()
     out.free()

@@ -126,7 +126,17 @@ void* rfa_context_stream(rfa_context* ctx);
  * (a window of ceil((k + 2) / 64) words per pair, several pairs per wave); 0 =
  * the one-wave-per-pair block kernel for every long query (same results). */
 #define RFA_OPTION_CDIST_BAND 3
+/* RFA_OPTION_COUNT_WORD_STEPS (default 0): 1 = the band kernel counts, on the
+ * device, the 64-bit word-steps it evaluates (text units run x window words of
+ * every pair) and the pairs it runs, for rfa_context_counter; setting it
+ * (to 0 or 1) clears both counters. */
+#define RFA_OPTION_COUNT_WORD_STEPS 4
 rfa_status rfa_context_set_option(rfa_context* ctx, int option, int64_t value);
+/* Device counters of the context (no reference counterpart): waits for the
+ * context's work, then stores the counter in *value. */
+#define RFA_COUNTER_BAND_WORD_STEPS 1
+#define RFA_COUNTER_BAND_PAIRS 2
+rfa_status rfa_context_counter(rfa_context* ctx, int counter, uint64_t* value);
 /* Message of the last failed call on this thread ("" if none). */
 const char* rfa_last_error(void);
 int rfa_abi_version(void);

@@ -1010,6 +1010,7 @@ void run_band_queries(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& sp
     a.spec = spec;
     a.out = dout;
     a.qcut = d_qcut;
+    a.counters = (unsigned long long*)ctx->d_counters;
     const uint64_t nblocks = (uint64_t)bandq.size() * a.chunks_per_query;
     if (nblocks > 0x7fffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "too many banded query chunks in one call");
     HIP_CHECK(launch_band(a, (uint32_t)nblocks, (uint32_t)lds, multi, ctx->stream));
@@ -1541,6 +1542,7 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
             b->release();
         ctx->staging.release();
         ctx->pairs_graph.release();
+        if (ctx->d_counters) (void)hipFree(ctx->d_counters);
         (void)hipEventDestroy(ctx->fork);
         (void)hipEventDestroy(ctx->join);
         (void)hipStreamDestroy(ctx->stream2);
@@ -1557,6 +1559,20 @@ rfa_status rfa_context_synchronize(rfa_context* ctx)
 
 void* rfa_context_stream(rfa_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+rfa_status rfa_context_counter(rfa_context* ctx, int counter, uint64_t* value)
+{
+    if (!ctx || !value) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx or value is NULL");
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (counter != RFA_COUNTER_BAND_WORD_STEPS && counter != RFA_COUNTER_BAND_PAIRS)
+            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown counter");
+        *value = 0;
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->d_counters)
+            HIP_CHECK(hipMemcpy(value, ctx->d_counters + (counter - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
+    });
+}
+
 rfa_status rfa_context_set_option(rfa_context* ctx, int option, int64_t value)
 {
     if (!ctx) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx is NULL");
@@ -1566,6 +1582,17 @@ rfa_status rfa_context_set_option(rfa_context* ctx, int option, int64_t value)
             if (value < 1 || value > (int64_t)1 << 31)
                 throw ApiError(RFA_ERR_INVALID_ARGUMENT, "RFA_OPTION_PAIRS_CHAIN must be in [1, 2^31]");
             ctx->seg_chain_steps = (uint32_t)value;
+            return;
+        }
+        if (option == RFA_OPTION_COUNT_WORD_STEPS) {
+            if (value != 0 && value != 1) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "RFA_OPTION_COUNT_WORD_STEPS is 0 or 1");
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            if (value && !ctx->d_counters) HIP_CHECK(hipMalloc(&ctx->d_counters, 8 * sizeof(uint64_t)));
+            if (ctx->d_counters) HIP_CHECK(hipMemset(ctx->d_counters, 0, 8 * sizeof(uint64_t)));
+            if (!value && ctx->d_counters) {
+                HIP_CHECK(hipFree(ctx->d_counters));
+                ctx->d_counters = nullptr;
+            }
             return;
         }
         if (option == RFA_OPTION_CDIST_BAND) {

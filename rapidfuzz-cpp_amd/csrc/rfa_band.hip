@@ -48,37 +48,68 @@
 namespace rfa {
 
 // One text unit: the window's match bits are three dwords of the symbol's PM row
-// at bit offset g (two funnel shifts), then the band step.  `count` false leaves
-// the score as it is (the pair's text ended earlier in the chunk).
-template <bool MULTI>
-__device__ __forceinline__ void band_step(const uint32_t* __restrict__ pmd, uint32_t row, uint32_t g, uint64_t LASTM,
-                                          uint32_t lane, uint64_t& VP, uint64_t& VN, int32_t& sc, int32_t& score,
-                                          uint32_t bitD, bool count)
+// at bit offset g (two funnel shifts), then the band step.  `upd` false leaves the
+// pair's state as it is (its text ended earlier in the chunk).
+template <bool MULTI, bool PRED>
+__device__ __forceinline__ void band_step(const uint8_t* __restrict__ lds, uint32_t row, uint32_t g, uint64_t LASTM,
+                                          uint32_t lane, bool last, uint64_t& VP, uint64_t& VN, int32_t& sc, bool upd)
 {
-    const uint32_t dw = row + (g >> 5);
-    const uint32_t w0 = pmd[dw], w1 = pmd[dw + 1], w2 = pmd[dw + 2];
+    // row: byte address of the symbol's PM row in the workgroup's LDS
+    const uint32_t* __restrict__ p = (const uint32_t*)(lds + row + ((g >> 5) << 2));
+    const uint32_t w0 = p[0], w1 = p[1], w2 = p[2];
     const uint32_t Xl = __builtin_amdgcn_alignbit(w1, w0, g), Xh = __builtin_amdgcn_alignbit(w2, w1, g);
     const uint32_t VPl = (uint32_t)VP, VPh = (uint32_t)(VP >> 32), VNl = (uint32_t)VN, VNh = (uint32_t)(VN >> 32);
-    uint64_t sum = ((((uint64_t)(Xh & VPh)) << 32) | (Xl & VPl)) + VP;
+    uint32_t sl, sh;
     if constexpr (MULTI) {
-        // carry into each word: generated where the word's own add overflows,
-        // propagated where its sum is all ones (no overflow is possible then)
-        const uint64_t G = ballot(sum < VP) & ~LASTM, Pp = ballot(sum == ~0ull) & ~LASTM;
-        sum += lane_bit(((G | Pp) + G) ^ Pp, lane);
+        // (X & VP) + VP per word with its carry-out mask; then the carry into each word,
+        // generated where the word's own add overflows and propagated where its sum is all
+        // ones (no overflow is possible then), added in as the carry-in of one more add
+        const uint32_t Al = Xl & VPl, Ah = Xh & VPh;
+        uint64_t ga, t;
+        asm("v_add_co_u32_e64 %0, %2, %3, %4\n\tv_addc_co_u32_e64 %1, %2, %5, %6, %2"
+            : "=&v"(sl), "=&v"(sh), "=&s"(ga)
+            : "v"(Al), "v"(VPl), "v"(Ah), "v"(VPh));
+        const uint64_t G = ga & ~LASTM, Pp = ballot((sl & sh) == ~0u) & ~LASTM;
+        const uint64_t cin = ((G | Pp) + G) ^ Pp;
+        uint32_t fl, fh;
+        asm("v_addc_co_u32_e64 %0, %2, %3, 0, %4\n\tv_addc_co_u32_e64 %1, %2, %5, 0, %2"
+            : "=&v"(fl), "=&v"(fh), "=&s"(t)
+            : "v"(sl), "s"(cin), "v"(sh));
+        sl = fl;
+        sh = fh;
     }
+    else {
+        const uint64_t X = ((uint64_t)Xh << 32) | Xl;
+        const uint64_t sum = (X & VP) + VP;
+        sl = (uint32_t)sum;
+        sh = (uint32_t)(sum >> 32);
+    }
+    const uint64_t sum = ((uint64_t)sh << 32) | sl;
     const uint32_t D0l = bop3<T_XOR_OR>((uint32_t)sum, VPl, Xl | VNl);
     const uint32_t D0h = bop3<T_XOR_OR>((uint32_t)(sum >> 32), VPh, Xh | VNh);
     const uint32_t HPl = bop3<T_OR_NOR>(VNl, D0l, VPl), HPh = bop3<T_OR_NOR>(VNh, D0h, VPh);
     const uint32_t HNl = D0l & VPl, HNh = D0h & VPh;
-    // D0 one row down: bit 0 of the next word enters at the top
+    // D0 one row down: bit 0 of the next word (the next lane, DPP wave_shl:1) enters at
+    // the top; the last word of a pair takes none
     uint32_t in = 0;
-    if constexpr (MULTI) in = lane_bit((ballot((D0l & 1) != 0) >> 1) & ~LASTM, lane);
+    if constexpr (MULTI) {
+        const uint32_t nb = (uint32_t)__builtin_amdgcn_mov_dpp((int)D0l, 0x130, 0xF, 0xF, true); // lane 63: 0
+        in = last ? 0u : nb;
+    }
     const uint32_t Sl = __builtin_amdgcn_alignbit(D0h, D0l, 1), Sh = __builtin_amdgcn_alignbit(in, D0h, 1);
-    VP = ((uint64_t)bop3<T_OR_NOR>(HNh, Sh, HPh) << 32) | bop3<T_OR_NOR>(HNl, Sl, HPl);
-    VN = ((uint64_t)(Sh & HPh) << 32) | (Sl & HPl);
-    sc += 1 + ((int32_t)D0h >> 31);
-    const uint32_t zb = (uint32_t)((((uint64_t)D0h << 32) | D0l) >> bitD) & 1u;
-    score += count ? 1 - (int32_t)zb : 0;
+    const uint64_t nVP = ((uint64_t)bop3<T_OR_NOR>(HNh, Sh, HPh) << 32) | bop3<T_OR_NOR>(HNl, Sl, HPl);
+    const uint64_t nVN = ((uint64_t)(Sh & HPh) << 32) | (Sl & HPl);
+    const int32_t nsc = sc + 1 + ((int32_t)D0h >> 31);
+    if constexpr (PRED) {
+        VP = upd ? nVP : VP;
+        VN = upd ? nVN : VN;
+        sc = upd ? nsc : sc;
+    }
+    else {
+        VP = nVP;
+        VN = nVN;
+        sc = nsc;
+    }
 }
 
 // The diagonal window above is worth its bookkeeping while it stays narrow:
@@ -87,7 +118,7 @@ template <bool MULTI>
 __global__ __launch_bounds__(BAND_THREADS) void k_band(BandArgs a)
 {
     extern __shared__ uint32_t smem_band[];
-    uint32_t* rowmap = smem_band;    // [256] dword offset of each symbol's PM row (row 0: zeros)
+    uint32_t* rowmap = smem_band;    // [256] LDS byte address of each symbol's PM row (row 0: zeros)
     uint32_t* pmd = smem_band + 256; // PM rows, 2 * WP dwords each
     __shared__ uint32_t s_next, s_rows;
     const uint32_t lane = threadIdx.x & 63;
@@ -123,7 +154,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band(BandArgs a)
         uint32_t ex = wave_excl_scan(cnt, &total);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            rowmap[lane * 4 + q] = v[q] ? (ex + 1) * 2 * WP : 0u;
+            rowmap[lane * 4 + q] = 1024u + 4u * (v[q] ? (ex + 1) * 2 * WP : 0u);
             ex += v[q];
         }
         if (lane == 0) {
@@ -137,7 +168,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band(BandArgs a)
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
         const uint32_t bit = 64 * PAD + i;
-        atomicOr(&pmd[rowmap[P[i]] + (bit >> 5)], 1u << (bit & 31));
+        atomicOr(&pmd[(rowmap[P[i]] - 1024u) / 4 + (bit >> 5)], 1u << (bit & 31));
     }
     __syncthreads();
 
@@ -146,6 +177,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band(BandArgs a)
     const uint32_t seg = lane / S, L = lane - seg * S;
     const bool lane_ok = seg < nseg;
     const uint64_t LASTM = ballot(lane_ok && L == S - 1);
+    const bool last = !lane_ok || L == S - 1;
     const uint32_t seg_lane0 = lane_ok ? seg * S : lane;
     const uint64_t seg_mask = S == 64 ? ~0ull : ((1ull << S) - 1);
     const uint64_t lt_mask = (1ull << lane) - 1;
@@ -154,11 +186,13 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band(BandArgs a)
     uint32_t ci = 0, n = 0, jt = 0, bitD = 0;
     bool isD = false;
     const uint8_t* T = P;
-    int32_t g0 = 0, sc = 0, scmax = 0, score = 0;
+    int32_t g0 = 0, sc = 0, scmax = 0;
+    uint64_t dmask = 0; // bits of the lane's word between diagonal d and the word's bottom
     uint64_t VP = 0, VN = 0;
     // candidates of the chunk: lane r holds the r-th pending one (wave-uniform counts)
     uint32_t pend_ci = 0, pend_n = 0, pend_off = 0;
     bool exhausted = false;
+    uint64_t cnt_steps = 0, cnt_pairs = 0; // RFA_OPTION_COUNT_WORD_STEPS
 
     for (;;) {
         // ---- hand idle lanes the next candidates ----
@@ -220,7 +254,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band(BandArgs a)
                 const int32_t bD = d - s + 1;
                 isD = (uint32_t)(bD >> 6) == L;
                 bitD = (uint32_t)bD & 63u;
-                score = d > 0 ? d : 0;
+                dmask = (~0ull << bitD) & (~0ull >> 1);
                 jt = 0;
                 busy = true;
             }
@@ -233,18 +267,18 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band(BandArgs a)
 #pragma unroll
         for (int q = 0; q < 8; ++q) ro[q] = rowmap[u[q]];
         // idle lanes step too (their segments' state is reset when they take a pair, and
-        // LASTM keeps their carries inside their segments); only a pair that ends inside
-        // this chunk must stop counting its score
+        // the carries and the shifted bit stay inside segments); only a pair that ends
+        // inside this chunk must keep its state at its last text unit
         if (ballot(busy && jt + 8 > n) == 0) {
 #pragma unroll
             for (int q = 0; q < 8; ++q)
-                band_step<MULTI>(pmd, ro[q], (uint32_t)(g0 + (int32_t)jt + q), LASTM, lane, VP, VN, sc, score, bitD, true);
+                band_step<MULTI, false>((const uint8_t*)smem_band, ro[q], (uint32_t)(g0 + (int32_t)jt + q), LASTM, lane, last, VP, VN, sc, true);
         }
         else {
 #pragma unroll
             for (int q = 0; q < 8; ++q)
-                band_step<MULTI>(pmd, ro[q], (uint32_t)(g0 + (int32_t)jt + q), LASTM, lane, VP, VN, sc, score, bitD,
-                                 jt + (uint32_t)q < n);
+                band_step<MULTI, true>((const uint8_t*)smem_band, ro[q], (uint32_t)(g0 + (int32_t)jt + q), LASTM, lane, last, VP, VN, sc,
+                                       jt + (uint32_t)q < n);
         }
         if (busy) jt += 8;
         const uint64_t alive = ballot(busy && sc <= scmax);
@@ -252,10 +286,28 @@ __global__ __launch_bounds__(BAND_THREADS) void k_band(BandArgs a)
         const bool fin = busy && jt >= n;
         const bool dead = busy && !fin && !seg_alive;
         if ((fin && isD) || (dead && L == 0)) {
+            // D[m][n]: the word's bottom cell less the vertical deltas of the rows below row m
+            // (window bits bitD .. 62 of the final column)
+            const int32_t score = sc - __popcll(VP & dmask) + __popcll(VN & dmask);
             const uint64_t raw = fin && score <= (int32_t)k ? (uint64_t)score : (uint64_t)k + 1;
             out[(uint64_t)qi * a.out_stride + ci] = finalize_int(spec, m, n, raw);
         }
+        if ((fin || dead) && L == 0) {
+            cnt_steps += (uint64_t)min(jt, n) * B;
+            ++cnt_pairs;
+        }
         if (fin || dead) busy = false;
+    }
+    if (a.counters) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            cnt_steps += __shfl_xor(cnt_steps, off, 64);
+            cnt_pairs += __shfl_xor(cnt_pairs, off, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.counters[0], (unsigned long long)cnt_steps);
+            atomicAdd(&a.counters[1], (unsigned long long)cnt_pairs);
+        }
     }
 }
 

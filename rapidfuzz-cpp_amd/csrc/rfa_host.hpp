@@ -223,6 +223,7 @@ struct rfa_context {
     bool pairs_graph_on = false; // RFA_OPTION_PAIRS_GRAPH
     uint32_t seg_chain_steps = rfa::SEG_CHAIN_STEPS; // RFA_OPTION_PAIRS_CHAIN, set by rfa_pairs_autotune
     bool no_band = false; // RFA_OPTION_CDIST_BAND = 0
+    uint64_t* d_counters = nullptr; // RFA_OPTION_COUNT_WORD_STEPS: band word-steps, band pairs
     void* noalloc_scratch = nullptr; // this device's static workspace (rfa_pairs_noalloc)
 };
 

@@ -117,6 +117,7 @@ struct BandArgs {
     ScoreSpec spec;          // kind RFA_DISTANCE, Levenshtein; band_k the raw cutoff
     void* out;               // out[q * out_stride + c]
     const uint64_t* qcut;    // integer cutoff per query (nullptr: spec.cutoff_u)
+    unsigned long long* counters; // nullptr, or [word-steps, pairs] accumulated by the launch
 };
 hipError_t launch_band(const BandArgs& a, uint32_t nblocks, uint32_t lds_bytes, bool multi, hipStream_t s);
 

@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "librapidfuzz_amd.so")
 DISTANCE, SIMILARITY, NORMALIZED_DISTANCE, NORMALIZED_SIMILARITY = range(4)
 CACHED, MULTI = 0, 1
 SCHEDULE_BUCKETED, SCHEDULE_PLAIN = 0, 1
-OPTION_PAIRS_GRAPH, OPTION_PAIRS_CHAIN, OPTION_CDIST_BAND = 1, 2, 3
+OPTION_PAIRS_GRAPH, OPTION_PAIRS_CHAIN, OPTION_CDIST_BAND, OPTION_COUNT_WORD_STEPS = 1, 2, 3, 4
+COUNTER_BAND_WORD_STEPS, COUNTER_BAND_PAIRS = 1, 2
 SCORER_SIGNED_UNITS = 1
 INT64_MAX = (1 << 63) - 1
 
@@ -29,7 +30,7 @@ _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED",
 
 # every symbol include/rapidfuzz_amd.h declares
 EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
-            "rfa_context_synchronize", "rfa_context_stream", "rfa_context_set_option", "rfa_strings_create", "rfa_strings_destroy",
+            "rfa_context_synchronize", "rfa_context_stream", "rfa_context_set_option", "rfa_context_counter", "rfa_strings_create", "rfa_strings_destroy",
             "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_scorer_set_option", "rfa_cdist", "rfa_cdist_cutoffs",
             "rfa_pairs", "rfa_pairs_noalloc", "rfa_pairs_autotune", "rfa_topk", "rfa_editops",
             "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
@@ -81,6 +82,7 @@ def load_library(path: str = LIB_PATH):
     L.rfa_context_stream.argtypes = [_vp]
     L.rfa_context_stream.restype = _vp
     L.rfa_context_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int64]
+    L.rfa_context_counter.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
     L.rfa_strings_create.argtypes = [_vp, _vp, ctypes.c_int, _vp, ctypes.c_uint64, ctypes.POINTER(_vp)]
     L.rfa_strings_destroy.argtypes = [_vp]
     L.rfa_strings_count.argtypes = [_vp]
@@ -194,6 +196,13 @@ class Context:
     def set_option(self, option: int, value: int):
         """rfa_context_set_option, e.g. OPTION_PAIRS_GRAPH: replay repeated pairs calls from a HIP graph."""
         _check(_lib.rfa_context_set_option(self.h, option, value))
+
+    def counter(self, which: int) -> int:
+        """rfa_context_counter (COUNTER_BAND_WORD_STEPS / COUNTER_BAND_PAIRS; counting is on
+        while OPTION_COUNT_WORD_STEPS is 1)."""
+        v = ctypes.c_uint64(0)
+        _check(_lib.rfa_context_counter(self.h, which, ctypes.byref(v)))
+        return int(v.value)
 
     def close(self):
         """Destroys the context; strings and scorers made from it become unusable
