@@ -66,8 +66,11 @@ typedef enum {
     RFA_TOKEN_RATIO = 15,              /* fuzz::CachedTokenRatio (fuzz.hpp:530-557): max of the token-sort and token-set ratios */
     RFA_PARTIAL_TOKEN_SET_RATIO = 16,  /* fuzz::CachedPartialTokenSetRatio (fuzz.hpp:471-493) */
     RFA_PARTIAL_TOKEN_RATIO = 17,      /* fuzz::CachedPartialTokenRatio (fuzz.hpp:595-620) */
-    RFA_DAMERAU_LEVENSHTEIN = 18       /* experimental::CachedDamerauLevenshtein (DamerauLevenshtein.hpp:112-140):
+    RFA_DAMERAU_LEVENSHTEIN = 18,      /* experimental::CachedDamerauLevenshtein (DamerauLevenshtein.hpp:112-140):
                                           Cached semantics, every kind; strings below 2^29 code units */
+    RFA_WRATIO = 19                    /* fuzz::CachedWRatio (fuzz.hpp:659-682, fuzz_impl.hpp:790-876): ratio, then
+                                          token_ratio or partial_ratio and partial_token_ratio with cutoffs raised per
+                                          pair; similarity, Cached only; the token ratios' query limit applies */
 } rfa_metric;
 
 /* Scorer method (details/distance.hpp:211-301, 326-356). */

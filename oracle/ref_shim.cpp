@@ -25,7 +25,7 @@
 
 namespace {
 
-enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10, TOKEN_SORT = 11, PARTIAL_TOKEN_SORT = 12, QRATIO = 13, TOKEN_SET = 14, TOKEN_RATIO = 15, PTOKEN_SET = 16, PTOKEN_RATIO = 17, DAMERAU = 18 };
+enum { LEV = 0, INDEL = 1, LCS = 2, OSA = 3, JARO = 4, JW = 5, RATIO = 6, HAMMING = 7, PREFIX = 8, POSTFIX = 9, PARTIAL = 10, TOKEN_SORT = 11, PARTIAL_TOKEN_SORT = 12, QRATIO = 13, TOKEN_SET = 14, TOKEN_RATIO = 15, PTOKEN_SET = 16, PTOKEN_RATIO = 17, DAMERAU = 18, WRATIO = 19 };
 enum { DIST = 0, SIM = 1, NDIST = 2, NSIM = 3 };
 
 thread_local std::string g_err;
@@ -88,6 +88,7 @@ struct AnyCached {
     rapidfuzz::fuzz::CachedPartialTokenSetRatio<CharT>* ptset = nullptr;
     rapidfuzz::fuzz::CachedPartialTokenRatio<CharT>* ptratio = nullptr;
     rapidfuzz::experimental::CachedDamerauLevenshtein<CharT>* dl = nullptr;
+    rapidfuzz::fuzz::CachedWRatio<CharT>* wratio = nullptr;
 
     AnyCached(int m, std::vector<CharT> q, rapidfuzz::LevenshteinWeightTable w_, double pw_)
         : metric(m), s1(std::move(q)), w(w_), pw(pw_)
@@ -112,6 +113,7 @@ struct AnyCached {
         case PTOKEN_SET: ptset = new rapidfuzz::fuzz::CachedPartialTokenSetRatio<CharT>(s1); break;
         case PTOKEN_RATIO: ptratio = new rapidfuzz::fuzz::CachedPartialTokenRatio<CharT>(s1); break;
         case DAMERAU: dl = new rapidfuzz::experimental::CachedDamerauLevenshtein<CharT>(s1); break;
+        case WRATIO: wratio = new rapidfuzz::fuzz::CachedWRatio<CharT>(s1); break;
         default: throw std::invalid_argument("unknown metric");
         }
     }
@@ -129,6 +131,7 @@ struct AnyCached {
         delete ptset;
         delete ptratio;
         delete dl;
+        delete wratio;
         delete lev;
         delete indel;
         delete lcs;
@@ -186,6 +189,10 @@ struct AnyCached {
         case PTOKEN_RATIO:
             if (kind != SIM) return -1;
             *od = ptratio->similarity(b, e, cd);
+            return 1;
+        case WRATIO:
+            if (kind != SIM) return -1;
+            *od = wratio->similarity(b, e, cd);
             return 1;
         }
         return -1;

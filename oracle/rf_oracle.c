@@ -19,6 +19,8 @@ static uint64_t u64min(uint64_t a, uint64_t b) { return a < b ? a : b; }
 static uint64_t u64max(uint64_t a, uint64_t b) { return a > b ? a : b; }
 static double dmin(double a, double b) { return a < b ? a : b; }
 static double dmax(double a, double b) { return a > b ? a : b; }
+/* std::max(a, b): a unless a < b */
+static double std_max(double a, double b) { return (a < b) ? b : a; }
 
 /* rapidfuzz/details/intrinsics.hpp ceil_div */
 static uint64_t ceil_div(uint64_t a, uint64_t b) { return a / b + (uint64_t)(a % b != 0); }
@@ -1209,6 +1211,35 @@ int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const 
         if (kind != ORC_SIMILARITY) return -1;
         *out_d = orc_partial_token_ratio(s1, l1, s2, l2, cutoff_d);
         return 1;
+    case ORC_WRATIO: {
+        /* CachedWRatio::similarity (fuzz_impl.hpp:838-876): ratio, then token_ratio (length
+         * ratio < 1.5) or partial_ratio and partial_token_ratio, each with the cutoff raised */
+        if (kind != ORC_SIMILARITY) return -1;
+        double c = cutoff_d;
+        if (c > 100 || !l1 || !l2) {
+            *out_d = 0;
+            return 1;
+        }
+        const double UNBASE_SCALE = 0.95;
+        const double len_ratio = (l1 > l2) ? (double)l1 / (double)l2 : (double)l2 / (double)l1;
+        p.metric = ORC_INDEL;
+        p.ins = p.del = 1;
+        p.rep = 2;
+        double end_ratio = int_api_norm_similarity(&p, c / 100) * 100;
+        if (len_ratio < 1.5) {
+            c = std_max(c, end_ratio) / UNBASE_SCALE;
+            const double r = orc_token_ratio(s1, l1, s2, l2, c);
+            *out_d = std_max(end_ratio, r * UNBASE_SCALE);
+            return 1;
+        }
+        const double PARTIAL_SCALE = (len_ratio <= 8.0) ? 0.9 : 0.6;
+        c = std_max(c, end_ratio) / PARTIAL_SCALE;
+        end_ratio = std_max(end_ratio, orc_partial_ratio(s1, l1, s2, l2, c) * PARTIAL_SCALE);
+        c = std_max(c, end_ratio) / UNBASE_SCALE;
+        const double r = orc_partial_token_ratio(s1, l1, s2, l2, c);
+        *out_d = std_max(end_ratio, r * UNBASE_SCALE * PARTIAL_SCALE);
+        return 1;
+    }
     case ORC_QRATIO:
         /* CachedQRatio::similarity (fuzz_impl.hpp:912-922) */
         if (kind != ORC_SIMILARITY) return -1;

@@ -45,7 +45,8 @@ enum {
     ORC_TOKEN_RATIO = 15,
     ORC_PARTIAL_TOKEN_SET_RATIO = 16,
     ORC_PARTIAL_TOKEN_RATIO = 17,
-    ORC_DAMERAU_LEVENSHTEIN = 18 /* experimental::CachedDamerauLevenshtein */
+    ORC_DAMERAU_LEVENSHTEIN = 18, /* experimental::CachedDamerauLevenshtein */
+    ORC_WRATIO = 19               /* fuzz::CachedWRatio */
 };
 
 enum { ORC_DISTANCE = 0, ORC_SIMILARITY = 1, ORC_NORM_DISTANCE = 2, ORC_NORM_SIMILARITY = 3 };

@@ -67,6 +67,7 @@ int algo_for(int metric, const rfa_weights& w)
     case RFA_JARO:
     case RFA_JARO_WINKLER: return ALGO_JARO;
     case RFA_DAMERAU_LEVENSHTEIN: return ALGO_DL;
+    case RFA_WRATIO: return ALGO_LCS;
     }
     return -1;
 }
@@ -388,8 +389,17 @@ rfa_scorer* make_scorer(rfa_context* ctx, int metric, const rfa_strings* queries
              metric == RFA_PARTIAL_TOKEN_RATIO) {
         sc->token_set = true;
     }
+    else if (metric == RFA_WRATIO) {
+        sc->metric = RFA_RATIO; // the kinds and semantics of a ratio scorer
+        sc->wratio = true;
+    }
     try {
-        if (sc->token_set) {
+        if (sc->wratio) {
+            static const int subs[4] = {RFA_RATIO, RFA_TOKEN_RATIO, RFA_PARTIAL_RATIO, RFA_PARTIAL_TOKEN_RATIO};
+            for (int i = 0; i < 4; ++i) sc->wr_sub[i] = make_scorer(ctx, subs[i], queries, w, prefix_weight);
+            sc->q = make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
+        }
+        else if (sc->token_set) {
             sc->q_raw =
                 make_strings(ctx, queries->h_chars.data(), queries->width, queries->h_off.data(), queries->count);
             sc->q_tokens = token_table(ctx, sc->q_raw, false);
@@ -427,6 +437,7 @@ void free_scorer(rfa_scorer* sc)
     free_scorer(sc->sort_sub);
     free_scorer(sc->pr_dedup);
     free_scorer(sc->pr_dedup_dupq);
+    for (rfa_scorer* sub : sc->wr_sub) free_scorer(sub);
     delete sc;
 }
 
@@ -435,6 +446,8 @@ void scorer_set_signed(rfa_scorer* sc, bool on)
 {
     if (sc->signed8 == on) return;
     sc->signed8 = on;
+    for (rfa_scorer* sub : sc->wr_sub)
+        if (sub) scorer_set_signed(sub, on);
     if (sc->sort_sub) scorer_set_signed(sc->sort_sub, on);
     if (sc->token_set) {
         clear_plan(sc);
@@ -656,8 +669,8 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
         for (uint64_t c = 0; c < nc; ++c)
             if (ch->len(c) < sorted_ch->len(c)) dup_choices.push_back((uint32_t)c);
         double* d2 = (double*)ctx->aux_tmp.get(bytes);
-        if (sc->dup_queries.size() * 4 > nq || dup_choices.size() * 4 > nc) {
-            // repeats are common: the whole matrix
+        if (sc->dup_queries.size() * 4 > nq || dup_choices.size() * 4 > nc || ctx->pcut) {
+            // repeats are common (or the cutoffs are per pair): the whole matrix
             cdist_impl(sc->pr_dedup, ch, RFA_SIMILARITY, RFA_CACHED, 0, cd, d2, 1);
         }
         else {
@@ -718,6 +731,7 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
         ta.hash_bits = sc->hash_bits;
         ta.spec = make_spec(sc, RFA_SIMILARITY, RFA_CACHED, 0, cd, ALGO_TSET);
         ta.out = dout;
+        ta.pcut = ctx->pcut;
         a.q_set_off = sc->q_tokens->d_off;
         a.q_set_start = sc->q_tokens->d_start;
         a.q_set_len = sc->q_tokens->d_len;
@@ -739,6 +753,55 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
             nbx = (ta.ncsel + cpb - 1) / cpb;
             HIP_CHECK(launch_tset(sc->q->width, tc.word_bits, a, tc.ntiles, nbx, s));
         }
+    }
+    if (!out_on_device) {
+        HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+// fuzz::WRatio over the raw choice set (fuzz_impl.hpp:838-876): the ratio, then the token
+// ratio, partial ratio and partial token ratio of every pair as whole cdist launches that
+// read the cutoffs raised per pair (rfa_wratio.hip computes them and the final maximum).
+void cdist_wratio(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics, double cd, void* out,
+                  int out_on_device)
+{
+    rfa_context* ctx = sc->ctx;
+    hipStream_t s = ctx->stream;
+    if (kind != RFA_SIMILARITY) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "WRatio only provides similarity");
+    if (semantics != RFA_CACHED) throw ApiError(RFA_ERR_INVALID_ARGUMENT, "WRatio has no Multi* scorer");
+    const uint64_t nq = sc->q->count, nc = raw->count;
+    if (nq == 0 || nc == 0) return;
+    if (nc > 0xffffffffull || nq > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 strings");
+    const size_t bytes = (size_t)nq * nc * 8;
+    double* dout = (double*)(out_on_device ? out : ctx->out_tmp.get(bytes));
+    if (cd > 100) {
+        HIP_CHECK(hipMemsetAsync(dout, 0, bytes, s)); // the reference returns 0 for every pair
+    }
+    else {
+        WRatioArgs a{};
+        a.qoff = sc->q->d_off;
+        a.coff = raw->d_off;
+        a.nq = (uint32_t)nq;
+        a.nc = (uint32_t)nc;
+        a.cutoff = cd;
+        a.R = (double*)ctx->wr_buf[0].get(bytes);
+        a.C = (double*)ctx->wr_buf[1].get(bytes);
+        double* P = (double*)ctx->wr_buf[2].get(bytes);
+        a.out = dout;
+        cdist_impl(sc->wr_sub[0], scorer_choices(sc->wr_sub[0], raw), RFA_SIMILARITY, RFA_CACHED, 0, cd, a.R, 1);
+        HIP_CHECK(launch_wratio(0, a, (uint32_t)ctx->num_cus, s));
+        struct PcutScope {
+            rfa_context* c;
+            ~PcutScope() { c->pcut = nullptr; }
+        } scope{ctx};
+        ctx->pcut = a.C;
+        cdist_tset(sc->wr_sub[1], raw, RFA_SIMILARITY, RFA_CACHED, cd, dout, 1);
+        cdist_impl(sc->wr_sub[2], scorer_choices(sc->wr_sub[2], raw), RFA_SIMILARITY, RFA_CACHED, 0, cd, P, 1);
+        a.X = P;
+        HIP_CHECK(launch_wratio(1, a, (uint32_t)ctx->num_cus, s));
+        cdist_tset(sc->wr_sub[3], raw, RFA_SIMILARITY, RFA_CACHED, cd, P, 1);
+        HIP_CHECK(launch_wratio(2, a, (uint32_t)ctx->num_cus, s));
     }
     if (!out_on_device) {
         HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
@@ -781,6 +844,7 @@ void cdist_partial(rfa_scorer* sc, const rfa_strings* ch, double cd, double* dou
         ta.hash_bits = sc->hash_bits;
         ta.spec = spec;
         ta.out = dout;
+        ta.pcut = ctx->pcut;
         for (const TileClass& tc : sc->classes) {
             ta.tile_base = tc.first_tile;
             ta.rows_cap = tc.rows_cap;
@@ -805,6 +869,7 @@ void cdist_partial(rfa_scorer* sc, const rfa_strings* ch, double cd, double* dou
         pa.out_stride = nc;
         pa.out = dout;
         pa.cutoff = cd;
+        pa.pcut = ctx->pcut;
         const uint32_t needle_words =
             (uint32_t)std::max<uint64_t>(1, (std::min<uint64_t>(sc->block_max_len, ch->max_len) + 63) / 64);
         constexpr uint64_t LDS_BUDGET = 40 * 1024;
@@ -1088,8 +1153,11 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         }
         return;
     }
+    // per-pair cutoffs (WRatio's sub-scores) run on the tile kernels only
+    if (ctx->pcut && (algo != ALGO_LCS || !sc->block_queries.empty() || semantics != RFA_CACHED))
+        throw ApiError(RFA_ERR_UNSUPPORTED, "WRatio takes queries of at most 64 code units");
     // a few queries against a set: one pair per lane (choices across lanes)
-    if (cdist_few_queries(sc, ch, spec, algo, dout, d_qcut)) {
+    if (!ctx->pcut && cdist_few_queries(sc, ch, spec, algo, dout, d_qcut)) {
         if (!out_on_device) {
             HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
@@ -1126,12 +1194,13 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ta.spec = spec;
         ta.out = dout;
         ta.qcut = d_qcut;
-        ta.settle = cutoff_may_settle(spec, d_qcut != nullptr) ? 1u : 0u;
+        ta.settle = cutoff_may_settle(spec, d_qcut != nullptr) && !ctx->pcut ? 1u : 0u;
         {
             // the fast normalized epilogue computes maxima in 32 bits
             NormEpi ne;
-            ta.norm_epi = norm_epi_setup(spec, ne) && (uint64_t)64 + ch->max_len <= 0xffffffffull ? 1u : 0u;
+            ta.norm_epi = norm_epi_setup(spec, ne) && (uint64_t)64 + ch->max_len <= 0xffffffffull && !ctx->pcut ? 1u : 0u;
         }
+        ta.pcut = ctx->pcut;
         {
             // a Cached integer result is a distance, similarity or cutoff + 1 no larger than the
             // maximum, (query + choice length) x the largest weight (Multi* similarities can
@@ -1640,7 +1709,7 @@ rfa_status rfa_scorer_create(rfa_context* ctx, int metric, const rfa_strings* qu
 {
     if (!ctx || !queries || !out) return fail(RFA_ERR_INVALID_ARGUMENT, "ctx, queries or out is NULL");
     return guarded([&] {
-        if (metric < RFA_LEVENSHTEIN || metric > RFA_DAMERAU_LEVENSHTEIN)
+        if (metric < RFA_LEVENSHTEIN || metric > RFA_WRATIO)
             throw ApiError(RFA_ERR_INVALID_ARGUMENT, "unknown metric");
         rfa_weights w = weights ? *weights : rfa_weights{1, 1, 1};
         if (metric != RFA_LEVENSHTEIN) w = rfa_weights{1, 1, 1};
@@ -1685,7 +1754,9 @@ rfa_status rfa_cdist(rfa_scorer* sc, const rfa_strings* choices, int kind, int s
         begin_call(sc->ctx);
         if (choices->width != sc->q->width)
             throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
-        if (sc->token_set)
+        if (sc->wratio)
+            cdist_wratio(sc, choices, kind, semantics, cutoff_d, out, out_on_device);
+        else if (sc->token_set)
             cdist_tset(sc, choices, kind, semantics, cutoff_d, out, out_on_device);
         else
             cdist_impl(sc, scorer_choices(sc, choices), kind, semantics, cutoff_u, cutoff_d, out, out_on_device);

@@ -223,6 +223,8 @@ struct rfa_context {
     bool pairs_graph_on = false; // RFA_OPTION_PAIRS_GRAPH
     uint32_t seg_chain_steps = rfa::SEG_CHAIN_STEPS; // RFA_OPTION_PAIRS_CHAIN, set by rfa_pairs_autotune
     bool no_band = false; // RFA_OPTION_CDIST_BAND = 0
+    const double* pcut = nullptr; // per-pair cutoffs of the cdist launches in progress (WRatio)
+    rfa::DevBuf wr_buf[3];         // WRatio's intermediate matrices
     uint64_t* d_counters = nullptr; // RFA_OPTION_COUNT_WORD_STEPS: band word-steps, band pairs
     void* noalloc_scratch = nullptr; // this device's static workspace (rfa_pairs_noalloc)
 };
@@ -332,6 +334,10 @@ struct rfa_scorer {
     rfa_scorer* pr_dedup_dupq = nullptr;
     bool dup_queries_ready = false;
     rfa_strings* tok_choices = nullptr; // token-sorted copy of the last choice set
+    // fuzz::WRatio: its sub-scorers over the raw queries (ratio, token_ratio, partial_ratio,
+    // partial_token_ratio), run as whole cdist launches with per-pair raised cutoffs
+    bool wratio = false;
+    rfa_scorer* wr_sub[4] = {nullptr, nullptr, nullptr, nullptr};
     uint64_t tok_choices_id = 0;
     bool tok_choices_signed8 = false;
 };
@@ -357,6 +363,8 @@ void validate_kind(int metric, int kind, int semantics);
 // qcut_host: optional per-query integer cutoffs (host memory, one per query)
 void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics, double cd, void* out,
                 int out_on_device);
+void cdist_wratio(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics, double cd, void* out,
+                  int out_on_device);
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
                 int out_on_device, const uint64_t* qcut_host = nullptr);
 

@@ -28,6 +28,9 @@ struct TileArgs {
     uint32_t settle;      // the cutoffs may decide results by length alone: test before each recurrence
     uint32_t norm_epi;    // the normalized kinds' fast epilogue covers the launch (norm_epi_setup)
     uint32_t narrow_int;  // integer results stay below 2^32 (held as 32 bits)
+    // per-pair double cutoff pcut[q * out_stride + c] in place of spec.cutoff_d (fuzz::WRatio's
+    // raised cutoffs); nullptr: spec.cutoff_d.  Launches that set it run the general epilogue.
+    const double* pcut;
 };
 
 // rfa_pairs_noalloc's workspace: a static device array per GPU (rfa_noalloc.hip).
@@ -207,6 +210,7 @@ struct PartialLongArgs {
     uint64_t out_stride;
     double* out;             // out[q * out_stride + c]
     double cutoff;
+    const double* pcut;      // per-pair cutoff [q * out_stride + c] (nullptr: cutoff)
     uint64_t* pm_scratch;    // global pattern arenas (when the pattern does not fit in LDS)
     uint64_t pm_scratch_words;
     uint32_t lds_bytes;      // 0: patterns live in pm_scratch
@@ -265,6 +269,21 @@ hipError_t launch_scatter_cols(double* dst, uint64_t stride, const double* src, 
                                uint64_t nq, hipStream_t s);
 hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
                        hipStream_t s);
+
+// fuzz::WRatio's combining steps over an nq x nc matrix (rfa_wratio.hip): stage 0 turns the
+// ratio R into the first raised cutoffs C; stage 1 folds partial_ratio X into R and C for
+// pairs with a length ratio >= 1.5; stage 2 combines into out (token_ratio on entry).
+struct WRatioArgs {
+    const uint64_t* qoff; // the raw queries' offsets
+    const uint64_t* coff; // the raw choices' offsets
+    uint32_t nq, nc;
+    double cutoff;
+    double* R;
+    double* C;
+    const double* X;
+    double* out;
+};
+hipError_t launch_wratio(int stage, const WRatioArgs& a, uint32_t num_cus, hipStream_t s);
 
 // Levenshtein with general weights (rfa_wlev.hip).  Query tiles of at most 64 units run
 // the tile kernel (TileArgs); longer queries run one wave per (query, choice) pair with

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
     SymLookup<CharT> look = build_tile_pm<CharT, W>(q, qo, m, pm, extra, a);
     ScoreSpec indel = a.spec;
     indel.metric = RFA_INDEL;
-    const double cut = a.spec.cutoff_d;
+    const double cut_all = a.spec.cutoff_d;
     const W* __restrict__ col = pm + lane;
 
     const uint32_t it0 = blockIdx.x * a.choices_per_block;
@@ -158,6 +158,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_ptile(TileArgs a)
         uint64_t outv[G];
         for (uint32_t t = 0; t < gn; ++t) {
             const uint32_t cidx = g0 + t;
+            const double cut = a.pcut && qact ? a.pcut[(uint64_t)qidx * a.out_stride + cidx] : cut_all;
             const uint64_t co0 = a.coff[cidx];
             const uint32_t n = uniform((uint32_t)(a.coff[cidx + 1] - co0));
             const uint64_t co = ((uint64_t)uniform((uint32_t)(co0 >> 32)) << 32) | uniform((uint32_t)co0);
@@ -441,7 +442,6 @@ __global__ __launch_bounds__(64) void k_plong(PartialLongArgs a)
     ScoreSpec indel{};
     indel.metric = RFA_INDEL;
     indel.kind = RFA_NORMALIZED_SIMILARITY;
-    const double cut = a.cutoff;
     // a contiguous share of the (query, choice) items: the pattern changes rarely
     const uint64_t total = (uint64_t)a.nql * a.nc;
     const uint64_t lo = total * blockIdx.x / gridDim.x, hi = total * (blockIdx.x + 1) / gridDim.x;
@@ -458,6 +458,7 @@ __global__ __launch_bounds__(64) void k_plong(PartialLongArgs a)
         }
         const uint64_t co = a.coff[ci];
         const uint32_t n = (uint32_t)(a.coff[ci + 1] - co);
+        const double cut = a.pcut ? a.pcut[(uint64_t)qi * a.out_stride + ci] : a.cutoff;
         double v;
         if (cut > 100) {
             v = 0.0;

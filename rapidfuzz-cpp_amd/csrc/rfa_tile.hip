@@ -395,7 +395,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileArgs a)
                 res = jaro_word<W>(col, row, m, n);
             }
             const uint32_t row_nx = text_row(u_nx, n_nx);
-            const uint64_t bits = tile_result<ALGO, EPI>(spec, ne, m, n, res);
+            uint64_t bits;
+            if constexpr (EPI == EPI_ANY) {
+                if (a.pcut) {
+                    ScoreSpec sp = spec;
+                    if (qact) sp.cutoff_d = a.pcut[(uint64_t)qidx * a.out_stride + cidx];
+                    bits = tile_result<ALGO, EPI>(sp, ne, m, n, res);
+                }
+                else {
+                    bits = tile_result<ALGO, EPI>(spec, ne, m, n, res);
+                }
+            }
+            else {
+                bits = tile_result<ALGO, EPI>(spec, ne, m, n, res);
+            }
             if (direct) {
                 if (qact) ((uint64_t*)a.out)[(uint64_t)qidx * a.out_stride + cidx] = bits;
             }

@@ -478,10 +478,13 @@ static void topk_tset(rfa_scorer* sc, const rfa_strings* ch, int kind, double cu
     }
     constexpr uint64_t DENSE_MAX = 16ull << 30;
     if (nq * nc * 8 > DENSE_MAX)
-        throw ApiError(RFA_ERR_UNSUPPORTED, "top-k of a token-set ratio over more than 2^31 query-choice pairs");
+        throw ApiError(RFA_ERR_UNSUPPORTED, "top-k of a token-set ratio or WRatio over more than 2^31 query-choice pairs");
     hipStream_t s = ctx->stream;
     double* dense = (double*)ctx->topk_scores.get((size_t)nq * nc * 8);
-    cdist_tset(sc, ch, kind, RFA_CACHED, cutoff, dense, 1);
+    if (sc->wratio)
+        cdist_wratio(sc, ch, kind, RFA_CACHED, cutoff, dense, 1);
+    else
+        cdist_tset(sc, ch, kind, RFA_CACHED, cutoff, dense, 1);
     uint8_t* base = (uint8_t*)ctx->topk_run[0].get((size_t)nq * k * 16);
     SelectArgs sa{};
     sa.dscore = dense;
@@ -553,7 +556,7 @@ void topk_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, double cutoff, u
     validate_topk(sc->metric, kind);
     if (ch->width != sc->q->width)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
-    if (sc->token_set) {
+    if (sc->token_set || sc->wratio) {
         topk_tset(sc, ch, kind, cutoff, k, out_index, out_score);
         return;
     }

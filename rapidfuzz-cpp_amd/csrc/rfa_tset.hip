@@ -85,7 +85,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tset(TsetArgs a)
 
     SymLookup<CharT> look = build_tile_pm<CharT, W>(q, qo, m, pm, extra, ta);
     const W* __restrict__ col = pm + lane;
-    const double cut = ta.spec.cutoff_d;
+    const double cut_all = ta.spec.cutoff_d;
 
     // this lane's query tokens: hash, start and length in LDS columns (each wave holds
     // the same tile, so every wave writes the same values into its own lane's column)
@@ -126,9 +126,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tset(TsetArgs a)
         const uint64_t co = ((uint64_t)uniform((uint32_t)(co0 >> 32)) << 32) | uniform((uint32_t)co0);
         const uint64_t tc0 = a.c_set_off[cidx];
         const uint32_t Tc = uniform((uint32_t)(a.c_set_off[cidx + 1] - tc0));
+        const double cut = ta.pcut && qact ? ta.pcut[(uint64_t)qidx * ta.out_stride + cidx] : cut_all;
         double v = 0.0;
         bool perfect = false, common = false;
-        if (cut <= 100) {
+        if (ballot(cut <= 100)) {
             // ---- pass 1: the intersection ----
             uint32_t inter_q = 0, n_inter = 0, n_kept_c = 0;
             uint64_t ba_units = 0, kept_c_lo = 0;

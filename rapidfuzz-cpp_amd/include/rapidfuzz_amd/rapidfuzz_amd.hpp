@@ -1179,6 +1179,22 @@ struct CachedPartialTokenRatio : detail::CachedFuzz<RFA_PARTIAL_TOKEN_RATIO> {
     {}
 };
 
+// fuzz::CachedWRatio (fuzz.hpp:659-682, fuzz_impl.hpp:838-876): ratio, then token_ratio or
+// partial_ratio and partial_token_ratio with cutoffs raised per pair; the token ratios' query
+// limit applies.
+template <typename CharT1>
+struct CachedWRatio : detail::CachedFuzz<RFA_WRATIO> {
+    template <typename InputIt1>
+    CachedWRatio(InputIt1 first1, InputIt1 last1)
+        : detail::CachedFuzz<RFA_WRATIO>(first1, last1, std::is_signed<CharT1>::value && sizeof(CharT1) == 1)
+    {}
+    template <typename Sentence1>
+    CachedWRatio(const Sentence1& s1) : CachedWRatio(detail::to_begin(s1), detail::to_end(s1))
+    {}
+};
+
+template <typename Sentence1>
+CachedWRatio(const Sentence1& s1) -> CachedWRatio<detail::sentence_char_t<Sentence1>>;
 template <typename Sentence1>
 CachedPartialTokenSetRatio(const Sentence1& s1) -> CachedPartialTokenSetRatio<detail::sentence_char_t<Sentence1>>;
 template <typename Sentence1>
@@ -1199,7 +1215,7 @@ template <typename Sentence1>
 CachedQRatio(const Sentence1& s1) -> CachedQRatio<detail::sentence_char_t<Sentence1>>;
 
 // fuzz free functions (fuzz.hpp:44-49, 161-167, 229-235, 334-340, 404-410, 463-469, 523-528,
-// 587-593, 713-718): one pair through the matching Cached* scorer
+// 587-593, 641-657, 713-718): one pair through the matching Cached* scorer
 #define RFA_FREE_FUZZ(NAME, CACHED)                                                                                 \
     template <typename InputIt1, typename InputIt2>                                                                \
     double NAME(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2, double score_cutoff = 0)        \
@@ -1222,6 +1238,7 @@ RFA_FREE_FUZZ(partial_token_set_ratio, CachedPartialTokenSetRatio)
 RFA_FREE_FUZZ(token_ratio, CachedTokenRatio)
 RFA_FREE_FUZZ(partial_token_ratio, CachedPartialTokenRatio)
 RFA_FREE_FUZZ(QRatio, CachedQRatio)
+RFA_FREE_FUZZ(WRatio, CachedWRatio)
 #undef RFA_FREE_FUZZ
 
 } // namespace fuzz

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "librapidfuzz_amd.so")
 
 (LEVENSHTEIN, INDEL, LCSSEQ, OSA, JARO, JARO_WINKLER, RATIO, HAMMING, PREFIX, POSTFIX, PARTIAL_RATIO, TOKEN_SORT_RATIO,
  PARTIAL_TOKEN_SORT_RATIO, QRATIO, TOKEN_SET_RATIO, TOKEN_RATIO, PARTIAL_TOKEN_SET_RATIO,
- PARTIAL_TOKEN_RATIO, DAMERAU_LEVENSHTEIN) = range(19)
+ PARTIAL_TOKEN_RATIO, DAMERAU_LEVENSHTEIN, WRATIO) = range(20)
 DISTANCE, SIMILARITY, NORMALIZED_DISTANCE, NORMALIZED_SIMILARITY = range(4)
 CACHED, MULTI = 0, 1
 SCHEDULE_BUCKETED, SCHEDULE_PLAIN = 0, 1
@@ -127,7 +127,7 @@ def _check(status):
 
 # metrics whose top-k ranks the plain similarity (0-100 or 0-1 doubles)
 _SIM_METRICS = (JARO, JARO_WINKLER, RATIO, PARTIAL_RATIO, TOKEN_SORT_RATIO, PARTIAL_TOKEN_SORT_RATIO, QRATIO,
-                TOKEN_SET_RATIO, TOKEN_RATIO, PARTIAL_TOKEN_SET_RATIO, PARTIAL_TOKEN_RATIO)
+                TOKEN_SET_RATIO, TOKEN_RATIO, PARTIAL_TOKEN_SET_RATIO, PARTIAL_TOKEN_RATIO, WRATIO)
 
 
 def is_int_result(metric, kind):

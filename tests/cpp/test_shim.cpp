@@ -367,6 +367,20 @@ int main()
             CHECK(near(rfa::fuzz::QRatio(std::string("test"), std::string("")), 0.0));
             CHECK(near(rfa::fuzz::partial_token_sort_ratio(std::string(""), std::string("")), 100.0));
             CHECK(near(rfa::fuzz::partial_token_ratio(std::string(""), std::string("")), 100.0));
+            // WRatio (tests-fuzz.cpp:101-135): equal, partial (x 0.9), misordered (x 0.95), issue 452
+            const std::string s1 = "new york mets", s3 = "the wonderful new york mets";
+            CHECK(near(rfa::fuzz::WRatio(s1, s1), 100.0));
+            CHECK(near(rfa::fuzz::WRatio(s1, s3), 90.0));
+            CHECK(near(rfa::fuzz::WRatio(s4, s5), 95.0));
+            std::string hb = "hello";
+            for (int i = 0; i < 7; ++i) hb += "abcde";
+            CHECK(near(rfa::fuzz::WRatio(std::string("hello"), hb), 90.0));
+            CHECK(near(rfa::fuzz::WRatio(std::string(""), std::string("")), 0.0));
+            CHECK(near(rfa::fuzz::CachedWRatio<char>(s1).similarity(s3, 90.5), 0.0));
+            {
+                rfa::fuzz::CachedWRatio<char> wr(s4);
+                CHECK(near(wr.similarity(s5), rfa::fuzz::WRatio(s4, s5)));
+            }
             // free functions on C strings and wide strings
             CHECK(rfa::levenshtein_distance("kitten", "sitting") == 3);
             CHECK(rfa::indel_distance(std::u32string(U"k\u00e4se"), std::u32string(U"kase")) == 2);

@@ -1,5 +1,5 @@
-"""token_sort_ratio, partial_token_sort_ratio, QRatio, token_set_ratio and token_ratio:
-pins of the oracle.
+"""token_sort_ratio, partial_token_sort_ratio, QRatio, token_set_ratio, token_ratio and
+WRatio: pins of the oracle.
 
 * The reference's assertions (tests-fuzz.cpp) are in tests/golden/known_answers.json
   (metrics 11-13) and run in test_oracle_golden.
@@ -12,7 +12,7 @@ pins of the oracle.
 import numpy as np
 import pytest
 
-TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO, SIM = 11, 12, 13, 14, 15, 16, 17, 1
+TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO, WRATIO, SIM = 11, 12, 13, 14, 15, 16, 17, 19, 1
 NARROW_WS = [0x09, 0x0A, 0x0B, 0x0C, 0x0D, 0x1C, 0x1D, 0x1E, 0x1F, 0x20]
 WIDE_WS = NARROW_WS + [0x85, 0xA0, 0x1680] + list(range(0x2000, 0x200B)) + [0x2028, 0x2029, 0x202F, 0x205F, 0x3000]
 
@@ -73,7 +73,7 @@ def _sets(rng, width, nq=14, nc=18):
     return q, c
 
 
-@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO])
+@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO, WRATIO])
 @pytest.mark.parametrize("width,signed8", [(1, False), (1, True), (2, False), (4, False)])
 def test_vs_reference(oracle, reference, metric, width, signed8):
     rng = np.random.default_rng(70 + metric * 7 + width + 3 * signed8)
@@ -93,7 +93,7 @@ def test_signed_order_matters(oracle):
         [0x61, 0x62, 0x63, 0x20, 0xE9, 0x74, 0xE9]
 
 
-@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO])
+@pytest.mark.parametrize("metric", [TSR, PTSR, QR, TSET, TRATIO, PTSET, PTRATIO, WRATIO])
 @pytest.mark.parametrize("s1,s2", [(b"South Korea", b"North Korea"), (b"bc", b"bca")])
 def test_cutoff_boundary_issues(oracle, metric, s1, s2):
     # tests-fuzz.cpp:161-184
@@ -102,7 +102,7 @@ def test_cutoff_boundary_issues(oracle, metric, s1, s2):
     assert abs(float(oracle.score(metric, SIM, s1, s2, cutoff_d=score - 0.0001)) - score) <= 1e-6
 
 
-@pytest.mark.parametrize("metric", [TSET, TRATIO, PTSET, PTRATIO])
+@pytest.mark.parametrize("metric", [TSET, TRATIO, PTSET, PTRATIO, WRATIO])
 def test_disjoint_token_sets_with_repeats(oracle, reference, metric):
     """No token in common but shared characters, and repeated tokens on each side: the
     paths where partial_token_ratio's second partial_ratio (deduplicated differences)
@@ -120,3 +120,47 @@ def test_disjoint_token_sets_with_repeats(oracle, reference, metric):
         exp = reference.cdist(metric, SIM, q, c, cutoff_d=cut, char_width=1)
         got = oracle.cdist(metric, SIM, q, c, cutoff_d=cut, char_width=1)
         assert np.array_equal(got.view(np.uint64), exp.view(np.uint64)), (metric, cut)
+
+
+def _wratio_by_hand(oracle, s1, s2, cut):
+    """WRatio written out from its sub-scores (fuzz_impl.hpp:838-876), each taken from the
+    oracle with the raised cutoff: a check of the oracle's combining steps."""
+    if cut > 100 or not len(s1) or not len(s2):
+        return 0.0
+    l1, l2 = len(s1), len(s2)
+    lr = l1 / l2 if l1 > l2 else l2 / l1
+    end = float(oracle.score(6, SIM, s1, s2, cutoff_d=cut))
+    if lr < 1.5:
+        c = max(cut, end) / 0.95
+        return max(end, float(oracle.score(TRATIO, SIM, s1, s2, cutoff_d=c)) * 0.95)
+    ps = 0.9 if lr <= 8.0 else 0.6
+    c = max(cut, end) / ps
+    end = max(end, float(oracle.score(10, SIM, s1, s2, cutoff_d=c)) * ps)
+    c = max(c, end) / 0.95
+    return max(end, float(oracle.score(PTRATIO, SIM, s1, s2, cutoff_d=c)) * 0.95 * ps)
+
+
+def test_wratio_branches(oracle, reference):
+    """Every length-ratio branch (< 1.5, 1.5 - 8, > 8), cutoffs at the sub-scores' raised
+    cutoffs and next to them, against the reference library and the formula written out."""
+    rng = np.random.default_rng(19)
+    words = ["new", "york", "mets", "atlanta", "braves", "vs", "the", "city", "ny", "a", "b", "ab"]
+
+    def sent(n):
+        return " ".join(words[int(i)] for i in rng.integers(0, len(words), size=n)).encode()
+
+    q = [sent(int(rng.integers(1, 4))) for _ in range(10)]
+    c = [sent(int(rng.integers(1, 12))) for _ in range(24)] + [b"x" * 70, b"new york" * 9, b"mets"]
+    qa = [np.frombuffer(x, np.uint8).astype(np.uint32) for x in q]
+    ca = [np.frombuffer(x, np.uint8).astype(np.uint32) for x in c]
+    base = oracle.cdist(WRATIO, SIM, qa, ca, cutoff_d=0.0, char_width=1)
+    for cut in (0.0, 30.0, 60.0, 85.0, 90.0, 95.0, 100.0, 100.5):
+        exp = reference.cdist(WRATIO, SIM, qa, ca, cutoff_d=cut, char_width=1)
+        got = oracle.cdist(WRATIO, SIM, qa, ca, cutoff_d=cut, char_width=1)
+        assert np.array_equal(got.view(np.uint64), exp.view(np.uint64)), cut
+    for i, s1 in enumerate(qa):
+        for j, s2 in enumerate(ca):
+            v = float(base[i, j])
+            for cut in (0.0, v, v + 1e-9, max(0.0, v - 1e-9), 50.0):
+                got = float(oracle.score(WRATIO, SIM, s1, s2, cutoff_d=cut))
+                assert got == _wratio_by_hand(oracle, s1, s2, cut), (q[i], c[j], cut)
