@@ -988,8 +988,8 @@ def workload_editops(args, emit=True):
 
 
 def workload_tokens(args, emit=True):
-    """token_sort_ratio, partial_token_sort_ratio, QRatio, token_set_ratio and token_ratio cdist on the
-    partial_ratio data (1024
+    """token_sort_ratio, partial_token_sort_ratio, QRatio, token_set_ratio, token_ratio, the partial
+    token ratios and WRatio cdist on the partial_ratio data (1024
     word-like queries x 65,536 texts over a-z and space), no cutoff.  A token-sort scorer sorts
     the tokens of the choice set on the device on its first call and reuses that copy: the
     steady-state time is `ms`, the one-time sort is `choice_sort_ms`."""
@@ -1009,7 +1009,7 @@ def workload_tokens(args, emit=True):
                          ("partial_token_sort_ratio", rfa.PARTIAL_TOKEN_SORT_RATIO), ("qratio", rfa.QRATIO),
                          ("token_set_ratio", rfa.TOKEN_SET_RATIO), ("token_ratio", rfa.TOKEN_RATIO),
                          ("partial_token_set_ratio", rfa.PARTIAL_TOKEN_SET_RATIO),
-                         ("partial_token_ratio", rfa.PARTIAL_TOKEN_RATIO)):
+                         ("partial_token_ratio", rfa.PARTIAL_TOKEN_RATIO), ("wratio", rfa.WRATIO)):
         sc = rfa.Scorer(ctx, metric, qs)
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -1067,7 +1067,8 @@ def workload_tokens(args, emit=True):
             "ms_per_step": res["token_sort_ratio"]["ms"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u32", "data": "synthetic",
             "config": {"workload": f"fuzz::CachedTokenSortRatio / CachedPartialTokenSortRatio / CachedQRatio / "
-                                   f"CachedTokenSetRatio / CachedTokenRatio / CachedPartialTokenSetRatio / CachedPartialTokenRatio cdist "
+                                   f"CachedTokenSetRatio / CachedTokenRatio / CachedPartialTokenSetRatio / CachedPartialTokenRatio / "
+                                   f"CachedWRatio cdist "
                                    f"{nq} x {nc}, uint8 a-z and space, queries 8-32 units, choices 32-128 units, "
                                    f"no cutoff"},
             **res}
@@ -1536,7 +1537,7 @@ def main():
                                                                            "partial_token_sort_ratio", "qratio",
                                                                            "token_set_ratio", "token_ratio",
                                                                            "partial_token_set_ratio",
-                                                                           "partial_token_ratio")}
+                                                                           "partial_token_ratio", "wratio")}
         wl = workload_wlev(args, emit=False)
         res["configs"]["weighted_levenshtein_4096x65536"] = {**{k: wl.get(k) for k in keep},
                                                              "config": wl["config"]["workload"]}

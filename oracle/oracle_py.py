@@ -236,6 +236,10 @@ class Reference:
                                   _u64p]
         L.ref_opcodes.restype = ctypes.c_int64
         L.ref_opcodes.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _u64p]
+        L.ref_editops_algebra.restype = ctypes.c_int64
+        L.ref_editops_algebra.argtypes = [ctypes.c_int, _i64p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_size_t, _i64p,
+                                          ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
         L.ref_editops_u8_mt.restype = ctypes.c_int
         L.ref_editops_u8_mt.argtypes = [ctypes.c_int, _u8p, _u64p, _u8p, _u64p, ctypes.c_size_t, _u64p, ctypes.c_int]
         L.ref_pairs_u32_mt.restype = ctypes.c_int
@@ -312,6 +316,21 @@ class Reference:
         if k < 0:
             raise ValueError(f"reference error {k}: {self._err()}")
         return t[:k], sp[:k], dp[:k]
+
+    def editops_algebra(self, op, ops, src_len, dest_len, a=0, b=0, c=1, sub=None):
+        """The reference's Editops / Opcodes algebra (ref_editops_algebra): ops is an (n, 3) array
+        of (type, src, dest); returns None if the reference throws, else (src_len, dest_len, rows)
+        with rows (n, 3) editops or (n, 5) opcodes."""
+        ops = np.ascontiguousarray(np.asarray(ops, dtype=np.int64).reshape(-1, 3))
+        sub = np.ascontiguousarray(np.asarray(sub if sub is not None else np.zeros((0, 3)), dtype=np.int64).reshape(-1, 3))
+        out = np.zeros(5 * (2 * len(ops) + 2) + 5, np.int64)
+        sl, dl = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        k = self.lib.ref_editops_algebra(op, _p(ops, _i64p), len(ops), src_len, dest_len, a, b, c, _p(sub, _i64p),
+                                         len(sub), _p(out, _i64p), ctypes.byref(sl), ctypes.byref(dl))
+        if k < 0:
+            return None
+        w = 5 if op in (5, 6, 7) else 3
+        return int(sl.value), int(dl.value), out[: k * w].reshape(-1, w)
 
     def opcodes(self, metric, s1, s2):
         """(n, 5) array: type, src_begin, src_end, dest_begin, dest_end of the reference's Opcodes."""

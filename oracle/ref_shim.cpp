@@ -722,4 +722,64 @@ int ref_editops_u8_mt(int metric, const uint8_t* a, const uint64_t* aoff, const 
     return 0;
 }
 
+// The reference's Editops / Opcodes algebra (details/types.hpp:167-565) on one list of
+// operations given as (type, src, dest) triples.  op: 0 slice, 1 remove_slice, 2 reverse,
+// 3 inverse, 4 remove_subsequence(sub), 5 Opcodes(ops).slice, 6 Opcodes(ops).inverse,
+// 7 Opcodes(ops).reverse, 8 Editops(Opcodes(ops).slice(a, b, c)).  Results: editops as
+// triples in out (ops 0-4, 8) or opcodes as 5-tuples (ops 5-7); -1 if the reference throws.
+int64_t ref_editops_algebra(int op, const int64_t* ops, size_t n, size_t src_len, size_t dest_len, int a, int b,
+                            int c, const int64_t* sub, size_t nsub, int64_t* out, size_t* out_src, size_t* out_dest)
+{
+    try {
+        auto make = [](const int64_t* t, size_t k, size_t sl, size_t dl) {
+            rapidfuzz::Editops e;
+            for (size_t i = 0; i < k; ++i)
+                e.emplace_back(static_cast<rapidfuzz::EditType>(t[3 * i]), static_cast<size_t>(t[3 * i + 1]),
+                               static_cast<size_t>(t[3 * i + 2]));
+            e.set_src_len(sl);
+            e.set_dest_len(dl);
+            return e;
+        };
+        rapidfuzz::Editops e = make(ops, n, src_len, dest_len);
+        auto put_ed = [&](const rapidfuzz::Editops& r) {
+            for (size_t i = 0; i < r.size(); ++i) {
+                out[3 * i] = static_cast<int64_t>(r[i].type);
+                out[3 * i + 1] = static_cast<int64_t>(r[i].src_pos);
+                out[3 * i + 2] = static_cast<int64_t>(r[i].dest_pos);
+            }
+            *out_src = r.get_src_len();
+            *out_dest = r.get_dest_len();
+            return static_cast<int64_t>(r.size());
+        };
+        auto put_op = [&](const rapidfuzz::Opcodes& r) {
+            for (size_t i = 0; i < r.size(); ++i) {
+                out[5 * i] = static_cast<int64_t>(r[i].type);
+                out[5 * i + 1] = static_cast<int64_t>(r[i].src_begin);
+                out[5 * i + 2] = static_cast<int64_t>(r[i].src_end);
+                out[5 * i + 3] = static_cast<int64_t>(r[i].dest_begin);
+                out[5 * i + 4] = static_cast<int64_t>(r[i].dest_end);
+            }
+            *out_src = r.get_src_len();
+            *out_dest = r.get_dest_len();
+            return static_cast<int64_t>(r.size());
+        };
+        switch (op) {
+        case 0: return put_ed(e.slice(a, b, c));
+        case 1: e.remove_slice(a, b, c); return put_ed(e);
+        case 2: return put_ed(e.reverse());
+        case 3: return put_ed(e.inverse());
+        case 4: return put_ed(e.remove_subsequence(make(sub, nsub, 0, 0)));
+        case 5: return put_op(rapidfuzz::Opcodes(e).slice(a, b, c));
+        case 6: return put_op(rapidfuzz::Opcodes(e).inverse());
+        case 7: return put_op(rapidfuzz::Opcodes(e).reverse());
+        case 8: return put_ed(rapidfuzz::Editops(rapidfuzz::Opcodes(e).slice(a, b, c)));
+        default: return -1;
+        }
+    }
+    catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
 } // extern "C"

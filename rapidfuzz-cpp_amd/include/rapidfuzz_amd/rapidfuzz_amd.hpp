@@ -749,29 +749,165 @@ struct EditOp {
     friend bool operator!=(const EditOp& a, const EditOp& b) { return !(a == b); }
 };
 
-class Editops : private std::vector<EditOp> {
-    using Base = std::vector<EditOp>;
+namespace detail {
+// Python-style slice bounds of details/types.hpp:108-160: negative indices count from
+// the end, indices are clamped to [0, size]; steps must be positive.  Returns false for
+// an empty range.
+inline bool slice_range(size_t size, int& start, int& stop, int step)
+{
+    if (step == 0) throw std::invalid_argument("slice step cannot be zero");
+    if (step < 0) throw std::invalid_argument("step sizes below 0 lead to an invalid order of editops");
+    const int n = static_cast<int>(size);
+    auto clamp = [n](int i) { return i < 0 ? std::max(i + n, 0) : std::min(i, n); };
+    start = clamp(start);
+    stop = clamp(stop);
+    return start < stop;
+}
+
+// the edit-operation lists: a vector with the two string lengths and the reference's slicing
+template <typename Op, typename Self>
+class OpList : private std::vector<Op> {
+    using Base = std::vector<Op>;
 
 public:
-    using Base::begin;
-    using Base::empty;
-    using Base::end;
-    using Base::size;
+    using typename Base::size_type;
+    OpList() = default;
+    OpList(size_type count, const Op& value) : Base(count, value) {}
+    explicit OpList(size_type count) : Base(count) {}
+
+    using Base::at;
+    using Base::back;
+    using Base::data;
+    using Base::front;
     using Base::operator[];
-    using Base::push_back;
+    using Base::begin;
+    using Base::cbegin;
+    using Base::cend;
+    using Base::crbegin;
+    using Base::crend;
+    using Base::end;
+    using Base::rbegin;
+    using Base::rend;
+    using Base::capacity;
+    using Base::empty;
+    using Base::max_size;
     using Base::reserve;
-    size_t get_src_len() const { return src_len_; }
-    size_t get_dest_len() const { return dest_len_; }
-    void set_src_len(size_t n) { src_len_ = n; }
-    void set_dest_len(size_t n) { dest_len_ = n; }
-    friend bool operator==(const Editops& a, const Editops& b)
+    using Base::shrink_to_fit;
+    using Base::size;
+    using Base::clear;
+    using Base::emplace;
+    using Base::emplace_back;
+    using Base::erase;
+    using Base::insert;
+    using Base::pop_back;
+    using Base::push_back;
+    using Base::resize;
+
+    size_t get_src_len() const noexcept { return src_len_; }
+    size_t get_dest_len() const noexcept { return dest_len_; }
+    void set_src_len(size_t n) noexcept { src_len_ = n; }
+    void set_dest_len(size_t n) noexcept { dest_len_ = n; }
+    void swap(Self& other) noexcept
+    {
+        std::swap(src_len_, other.src_len_);
+        std::swap(dest_len_, other.dest_len_);
+        Base::swap(static_cast<Base&>(other));
+    }
+    // elements start, start + step, ... below stop; the lengths are kept
+    Self slice(int start, int stop, int step = 1) const
+    {
+        Self out;
+        out.set_src_len(src_len_);
+        out.set_dest_len(dest_len_);
+        if (!slice_range(size(), start, stop, step)) return out;
+        out.reserve(static_cast<size_t>((stop - 1 - start) / step + 1));
+        for (int i = start; i < stop; i += step) out.push_back((*this)[static_cast<size_t>(i)]);
+        return out;
+    }
+    // the same elements removed
+    void remove_slice(int start, int stop, int step = 1)
+    {
+        if (!slice_range(size(), start, stop, step)) return;
+        size_t w = static_cast<size_t>(start);
+        for (size_t i = static_cast<size_t>(start); i < size(); ++i) {
+            const int k = static_cast<int>(i);
+            if (k >= stop || (k - start) % step != 0) (*this)[w++] = (*this)[i];
+        }
+        resize(w);
+        shrink_to_fit();
+    }
+    Self reverse() const
+    {
+        Self out = static_cast<const Self&>(*this);
+        std::reverse(out.begin(), out.end());
+        return out;
+    }
+    friend bool operator==(const Self& a, const Self& b)
     {
         return a.src_len_ == b.src_len_ && a.dest_len_ == b.dest_len_ &&
                static_cast<const Base&>(a) == static_cast<const Base&>(b);
     }
+    friend bool operator!=(const Self& a, const Self& b) { return !(a == b); }
+    friend void swap(Self& a, Self& b) noexcept { a.swap(b); }
 
 private:
     size_t src_len_ = 0, dest_len_ = 0;
+};
+} // namespace detail
+
+class Opcodes;
+
+// details/types.hpp:167-355
+class Editops : public detail::OpList<EditOp, Editops> {
+    using List = detail::OpList<EditOp, Editops>;
+
+public:
+    using List::List;
+    Editops() = default;
+    Editops(const Opcodes& ops);
+    // insertions and deletions exchanged, source and destination swapped
+    Editops inverse() const
+    {
+        Editops out = *this;
+        out.set_src_len(get_dest_len());
+        out.set_dest_len(get_src_len());
+        for (EditOp& op : out) {
+            std::swap(op.src_pos, op.dest_pos);
+            if (op.type == EditType::Delete)
+                op.type = EditType::Insert;
+            else if (op.type == EditType::Insert)
+                op.type = EditType::Delete;
+        }
+        return out;
+    }
+    // the operations of `sub` (a subsequence, in order) taken out; the source positions of
+    // the remaining ones move by the insertions (+1) and deletions (-1) removed before them
+    Editops remove_subsequence(const Editops& sub) const
+    {
+        if (sub.size() > size()) throw std::invalid_argument("subsequence is not a subsequence");
+        Editops out;
+        out.set_src_len(get_src_len());
+        out.set_dest_len(get_dest_len());
+        out.reserve(size() - sub.size());
+        ptrdiff_t shift = 0;
+        size_t i = 0;
+        auto keep = [&](const EditOp& op) {
+            EditOp o = op;
+            o.src_pos = static_cast<size_t>(static_cast<ptrdiff_t>(o.src_pos) + shift);
+            out.push_back(o);
+        };
+        for (const EditOp& s : sub) {
+            while (i < size() && (*this)[i] != s) keep((*this)[i++]);
+            if (i == size()) throw std::invalid_argument("subsequence is not a subsequence");
+            if (s.type == EditType::Insert)
+                ++shift;
+            else if (s.type == EditType::Delete)
+                --shift;
+            ++i;
+        }
+        while (i < size()) keep((*this)[i++]);
+        return out;
+    }
 };
 
 // details/types.hpp:81-100, 357-565: runs of edit operations with their source and
@@ -791,23 +927,18 @@ struct Opcode {
     friend bool operator!=(const Opcode& a, const Opcode& b) { return !(a == b); }
 };
 
-class Opcodes : private std::vector<Opcode> {
-    using Base = std::vector<Opcode>;
+class Opcodes : public detail::OpList<Opcode, Opcodes> {
+    using List = detail::OpList<Opcode, Opcodes>;
 
 public:
-    using Base::begin;
-    using Base::empty;
-    using Base::end;
-    using Base::size;
-    using Base::operator[];
-    using Base::push_back;
+    using List::List;
     Opcodes() = default;
     // Editops -> Opcodes: consecutive operations of one type that continue each other
     // become one run; the stretches between runs (and after the last) are None runs
-    explicit Opcodes(const Editops& ops)
+    Opcodes(const Editops& ops)
     {
-        src_len_ = ops.get_src_len();
-        dest_len_ = ops.get_dest_len();
+        set_src_len(ops.get_src_len());
+        set_dest_len(ops.get_dest_len());
         size_t sp = 0, dp = 0, i = 0;
         while (i < ops.size()) {
             if (sp < ops[i].src_pos || dp < ops[i].dest_pos) {
@@ -825,45 +956,51 @@ public:
             }
             push_back(Opcode(t, sb, sp, db, dp));
         }
-        if (sp < src_len_ || dp < dest_len_) push_back(Opcode(EditType::None, sp, src_len_, dp, dest_len_));
+        if (sp < get_src_len() || dp < get_dest_len())
+            push_back(Opcode(EditType::None, sp, get_src_len(), dp, get_dest_len()));
     }
-    size_t get_src_len() const { return src_len_; }
-    size_t get_dest_len() const { return dest_len_; }
-    void set_src_len(size_t n) { src_len_ = n; }
-    void set_dest_len(size_t n) { dest_len_ = n; }
-    Editops as_editops() const
+    Editops as_editops() const { return Editops(*this); }
+    // insertions and deletions exchanged, source and destination ranges swapped
+    Opcodes inverse() const
     {
-        Editops out;
-        out.set_src_len(src_len_);
-        out.set_dest_len(dest_len_);
-        for (const Opcode& op : *this) {
-            switch (op.type) {
-            case EditType::None: break;
-            case EditType::Replace:
-                for (size_t k = 0; k < op.src_end - op.src_begin; ++k)
-                    out.push_back(EditOp(EditType::Replace, op.src_begin + k, op.dest_begin + k));
-                break;
-            case EditType::Insert:
-                for (size_t k = 0; k < op.dest_end - op.dest_begin; ++k)
-                    out.push_back(EditOp(EditType::Insert, op.src_begin, op.dest_begin + k));
-                break;
-            case EditType::Delete:
-                for (size_t k = 0; k < op.src_end - op.src_begin; ++k)
-                    out.push_back(EditOp(EditType::Delete, op.src_begin + k, op.dest_begin));
-                break;
-            }
+        Opcodes out = *this;
+        out.set_src_len(get_dest_len());
+        out.set_dest_len(get_src_len());
+        for (Opcode& op : out) {
+            std::swap(op.src_begin, op.dest_begin);
+            std::swap(op.src_end, op.dest_end);
+            if (op.type == EditType::Delete)
+                op.type = EditType::Insert;
+            else if (op.type == EditType::Insert)
+                op.type = EditType::Delete;
         }
         return out;
     }
-    friend bool operator==(const Opcodes& a, const Opcodes& b)
-    {
-        return a.src_len_ == b.src_len_ && a.dest_len_ == b.dest_len_ &&
-               static_cast<const Base&>(a) == static_cast<const Base&>(b);
-    }
-
-private:
-    size_t src_len_ = 0, dest_len_ = 0;
 };
+
+// Opcodes -> Editops: every unit of a run is one operation (None runs produce none)
+inline Editops::Editops(const Opcodes& ops)
+{
+    set_src_len(ops.get_src_len());
+    set_dest_len(ops.get_dest_len());
+    for (const Opcode& op : ops) {
+        switch (op.type) {
+        case EditType::None: break;
+        case EditType::Replace:
+            for (size_t k = 0; k < op.src_end - op.src_begin; ++k)
+                push_back(EditOp(EditType::Replace, op.src_begin + k, op.dest_begin + k));
+            break;
+        case EditType::Insert:
+            for (size_t k = 0; k < op.dest_end - op.dest_begin; ++k)
+                push_back(EditOp(EditType::Insert, op.src_begin, op.dest_begin + k));
+            break;
+        case EditType::Delete:
+            for (size_t k = 0; k < op.src_end - op.src_begin; ++k)
+                push_back(EditOp(EditType::Delete, op.src_begin + k, op.dest_begin));
+            break;
+        }
+    }
+}
 
 inline Opcodes as_opcodes(const Editops& ops) { return Opcodes(ops); }
 
