@@ -1137,6 +1137,13 @@ def workload_dna(args, emit=True):
         def step():
             scorer.cdist(cs, rfa.DISTANCE, cutoffs_u=cut, out_device_ptr=out.ptr.value)
 
+        # the band kernel's launch settings per window-size bucket, measured untimed on this call
+        t0 = time.perf_counter()
+        chosen = scorer.cdist_autotune(cs, rfa.DISTANCE, cutoffs_u=cut)
+        tune_s = time.perf_counter() - t0
+        tuned = {f"bucket_{b}": {"words_per_lane": int(chosen[b, 0]), "threads": int(chosen[b, 1])}
+                 for b in range(6) if chosen[b, 0]}
+        print(f"# dna cutoff {frac:.0%}: autotuned {tuned} ({tune_s:.2f} s)", file=sys.stderr, flush=True)
         step()
         ctx.synchronize()
         e0, e1 = hip.Event(), hip.Event()
@@ -1151,7 +1158,8 @@ def workload_dna(args, emit=True):
         out.to_host(got)
         got = got.reshape(NQ3, NC3)
         within = int(np.count_nonzero(got <= cut[:, None]))
-        entry = {"ms": round(ms, 2), "gcups": round(cells / (ms / 1e3) / 1e9, 1), "pairs_within_cutoff": within}
+        entry = {"ms": round(ms, 2), "gcups": round(cells / (ms / 1e3) / 1e9, 1), "pairs_within_cutoff": within,
+                 "autotuned": tuned}
         print(f"# dna cutoff {frac:.0%}: {ms:.2f} ms, {entry['gcups']} GCUPS (full-matrix cells), "
               f"{within} pairs within the cutoff", file=sys.stderr, flush=True)
         # the band kernel's own count of the word-steps it evaluated (one more, untimed call)

@@ -232,6 +232,20 @@ rfa_status rfa_pairs_noalloc(rfa_context* ctx, const rfa_strings* a, const rfa_s
 rfa_status rfa_pairs_autotune(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, const int32_t* metrics,
                               int kind, uint64_t cutoff_u, double cutoff_d, uint32_t* chosen);
 
+/* Autotuner of the diagonal band kernel (no reference counterpart: the reference's
+ * banded recurrence, Levenshtein_impl.hpp:641-850, has no launch settings).  Runs the
+ * cdist of the scorer's queries over 64 units against `choices` under their Levenshtein
+ * distance cutoffs (cutoffs_u per query, or cutoff_u for all when NULL) and measures, for
+ * each bucket of window sizes (band words 1, 2-3, 4-7, 8-15, 16-31, 32-64), the words per
+ * lane (1 or 2: lanes per pair = ceil(band words / words per lane), so the pairs one wave
+ * runs), then for the whole launch the workgroup size (64, 128 or 256 threads: the pairs a
+ * workgroup runs against its query); keeps the fastest in the context for later cdist
+ * calls and writes them to chosen[2 * bucket] (words per lane) and chosen[2 * bucket + 1]
+ * (threads), 0 for buckets without such queries (chosen may be NULL).  Results do not
+ * depend on the settings. */
+rfa_status rfa_cdist_autotune(rfa_scorer* sc, const rfa_strings* choices, int kind, uint64_t cutoff_u,
+                              const uint64_t* cutoffs_u, uint32_t* chosen);
+
 /* Top-k extraction.  Replaces the README's extract() loop (README.md:153-214):
  * for every query, score = Cached<Metric>(query).<kind>(choice, cutoff_d) for
  * every choice, results with score >= cutoff_d, ordered by (score descending,

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <map>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -438,6 +439,7 @@ void free_scorer(rfa_scorer* sc)
     free_scorer(sc->pr_dedup);
     free_scorer(sc->pr_dedup_dupq);
     for (rfa_scorer* sub : sc->wr_sub) free_scorer(sub);
+    if (sc->d_dup_queries) (void)hipFree(sc->d_dup_queries);
     delete sc;
 }
 
@@ -458,6 +460,8 @@ void scorer_set_signed(rfa_scorer* sc, bool on)
         free_scorer(sc->pr_dedup_dupq);
         sc->pr_dedup_dupq = nullptr;
         sc->dup_queries.clear();
+        if (sc->d_dup_queries) (void)hipFree(sc->d_dup_queries);
+        sc->d_dup_queries = nullptr;
         sc->dup_queries_ready = false;
         if (sc->pr_dedup) {
             free_scorer(sc->pr_dedup);
@@ -706,9 +710,12 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
                 const uint64_t nd = sc->dup_queries.size();
                 double* d4 = (double*)ctx->tset_rows.get((size_t)nd * nc * 8);
                 cdist_impl(sc->pr_dedup_dupq, ch, RFA_SIMILARITY, RFA_CACHED, 0, cd, d4, 1);
-                for (uint64_t r = 0; r < nd; ++r)
-                    HIP_CHECK(hipMemcpyAsync(d2 + (uint64_t)sc->dup_queries[r] * nc, d4 + r * nc, nc * 8,
-                                             hipMemcpyDeviceToDevice, s));
+                if (!sc->d_dup_queries) {
+                    HIP_CHECK(hipMalloc(&sc->d_dup_queries, nd * sizeof(uint32_t)));
+                    HIP_CHECK(hipMemcpy(sc->d_dup_queries, sc->dup_queries.data(), nd * sizeof(uint32_t),
+                                        hipMemcpyHostToDevice));
+                }
+                HIP_CHECK(launch_scatter_rows(d2, nc, d4, sc->d_dup_queries, nd, s));
             }
         }
         aux = d2;
@@ -1028,57 +1035,107 @@ void cdist_dl(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, void
     }
 }
 
+// Window-size buckets of the band kernel's launch settings: band_words(k) in {1}, {2, 3},
+// {4 .. 7}, {8 .. 15}, {16 .. 31}, {32 .. 64}.
+uint32_t band_bucket(uint32_t k)
+{
+    const uint32_t B = band_words(k);
+    uint32_t b = 0;
+    while (b < 5 && (2u << b) <= B) ++b;
+    return b;
+}
+
+// Words per lane without a measurement: the choice that keeps more of a wave's word slots
+// (64 lanes x words per lane) on window words a pair needs, one word per lane on a tie.
+uint32_t band_rule_wpl(uint32_t k)
+{
+    const uint32_t B = band_words(k);
+    if (B < 2) return 1;
+    // window words per lane-step: one per lane (64 lanes) or two (128 slots)
+    const uint32_t used1 = band_pairs_per_wave(k, 1) * B, used2 = band_pairs_per_wave(k, 2) * B;
+    return used2 > 2 * used1 ? 2u : 1u;
+}
+
 // Long 8-bit queries under a narrow Levenshtein distance cutoff run the diagonal band
 // kernel: Cached distance with uniform weights, band_words(k) <= 64 and the query's
 // padded PM rows within BAND_LDS_MAX.  The other block queries are returned in `rest`.
-void run_band_queries(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, int algo, int kind, int semantics,
-                      const uint64_t* qcut_host, const uint64_t* d_qcut, void* dout, std::vector<uint32_t>& rest)
+// `only`: forced settings for the autotuner's runs -- the band queries of one bucket with
+// words per lane `wpl` (bucket < 6), or all of them with the chosen words per lane (bucket 6);
+// `threads` the workgroup size.
+struct BandForce {
+    uint32_t bucket, wpl, threads;
+};
+
+uint32_t run_band_queries(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, int algo, int kind,
+                          int semantics, const uint64_t* qcut_host, const uint64_t* d_qcut, void* dout,
+                          std::vector<uint32_t>& rest, const BandForce* only = nullptr)
 {
     constexpr uint64_t BAND_LDS_MAX = 64 * 1024;
     rfa_context* ctx = sc->ctx;
     rest.clear();
     const bool kernel_ok = kind == RFA_DISTANCE && semantics == RFA_CACHED && algo == ALGO_LEV && sc->q->width == 1 &&
                            spec.ins > 0 && sc->q_rows.size() == sc->q->count && !ctx->no_band;
-    std::vector<uint32_t> bandq;
+    // one launch; each query runs with its own words per lane
+    std::vector<uint32_t> lq;
+    std::vector<uint8_t> lwpl;
     uint64_t lds = 0;
-    bool multi = false;
+    bool multi = false, wpl2 = false;
+    const uint32_t threads = only ? only->threads : (ctx->band_threads ? ctx->band_threads : BAND_THREADS);
     for (uint32_t qi : sc->block_queries) {
         const uint64_t m = sc->q->len(qi);
         const uint64_t cut = qcut_host ? qcut_host[qi] : spec.cutoff_u;
         const uint64_t k = host_ceil_div(cut, spec.ins);
-        const bool ok = kernel_ok && k < m && band_words((uint32_t)std::min<uint64_t>(k, 1u << 20)) <= BAND_MAX_WORDS &&
-                        band_lds_bytes((uint32_t)m, (uint32_t)k, sc->q_rows[qi]) <= BAND_LDS_MAX;
+        bool ok = kernel_ok && k < m && band_words((uint32_t)std::min<uint64_t>(k, 1u << 20)) <= BAND_MAX_WORDS;
+        uint32_t wpl = 1;
+        if (ok) {
+            const uint32_t b = band_bucket((uint32_t)k);
+            if (only && only->bucket < 6) {
+                if (b != only->bucket) continue; // another bucket: not part of this run
+                wpl = only->wpl;
+            }
+            else {
+                wpl = ctx->band_wpl[b] ? ctx->band_wpl[b] : band_rule_wpl((uint32_t)k);
+            }
+            ok = band_lds_bytes((uint32_t)m, (uint32_t)k, sc->q_rows[qi], wpl) <= BAND_LDS_MAX;
+        }
         if (!ok) {
-            rest.push_back(qi);
+            if (!only) rest.push_back(qi);
             continue;
         }
-        bandq.push_back(qi);
-        lds = std::max(lds, band_lds_bytes((uint32_t)m, (uint32_t)k, sc->q_rows[qi]));
+        lq.push_back(qi);
+        lwpl.push_back((uint8_t)wpl);
+        lds = std::max(lds, band_lds_bytes((uint32_t)m, (uint32_t)k, sc->q_rows[qi], wpl));
         multi = multi || band_words((uint32_t)k) > 1;
+        wpl2 = wpl2 || wpl == 2;
     }
-    if (bandq.empty() || ch->count == 0) return;
-    BandArgs a{};
-    a.qchars = (const uint8_t*)sc->q->d_chars;
-    a.qoff = sc->q->d_off;
-    a.queries = upload(ctx->keep, ctx->band_q, bandq, ctx->stream);
-    a.cchars = (const uint8_t*)ch->d_chars;
-    a.coff = ch->d_off;
-    a.nc = (uint32_t)ch->count;
-    // about 8 workgroups per CU over the whole launch, at least 256 choices each
-    const uint64_t target = (uint64_t)ctx->num_cus * 8;
-    const uint64_t per_q = std::max<uint64_t>(1, (target + bandq.size() - 1) / bandq.size());
-    uint64_t chunk = std::max<uint64_t>(256, (ch->count + per_q - 1) / per_q);
-    chunk = (chunk + 63) / 64 * 64;
-    a.chunk = (uint32_t)chunk;
-    a.chunks_per_query = (uint32_t)((ch->count + chunk - 1) / chunk);
-    a.out_stride = ch->count;
-    a.spec = spec;
-    a.out = dout;
-    a.qcut = d_qcut;
-    a.counters = (unsigned long long*)ctx->d_counters;
-    const uint64_t nblocks = (uint64_t)bandq.size() * a.chunks_per_query;
-    if (nblocks > 0x7fffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "too many banded query chunks in one call");
-    HIP_CHECK(launch_band(a, (uint32_t)nblocks, (uint32_t)lds, multi, ctx->stream));
+    const uint32_t launched = (uint32_t)lq.size();
+    if (ch->count == 0 || lq.empty()) return launched;
+    {
+        BandArgs a{};
+        a.qchars = (const uint8_t*)sc->q->d_chars;
+        a.qoff = sc->q->d_off;
+        a.queries = upload(ctx->keep, ctx->band_q, lq, ctx->stream);
+        a.wpl = upload(ctx->keep, ctx->band_w, lwpl, ctx->stream);
+        a.cchars = (const uint8_t*)ch->d_chars;
+        a.coff = ch->d_off;
+        a.nc = (uint32_t)ch->count;
+        // about 8 workgroups of 4 waves per CU over the launch, at least 256 choices each
+        const uint64_t target = (uint64_t)ctx->num_cus * 8 * 256 / threads;
+        const uint64_t per_q = std::max<uint64_t>(1, (target + lq.size() - 1) / lq.size());
+        uint64_t chunk = std::max<uint64_t>(256, (ch->count + per_q - 1) / per_q);
+        chunk = (chunk + 63) / 64 * 64;
+        a.chunk = (uint32_t)chunk;
+        a.chunks_per_query = (uint32_t)((ch->count + chunk - 1) / chunk);
+        a.out_stride = ch->count;
+        a.spec = spec;
+        a.out = dout;
+        a.qcut = d_qcut;
+        a.counters = (unsigned long long*)ctx->d_counters;
+        const uint64_t nblocks = (uint64_t)lq.size() * a.chunks_per_query;
+        if (nblocks > 0x7fffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "too many banded query chunks in one call");
+        HIP_CHECK(launch_band(a, (uint32_t)nblocks, (uint32_t)lds, multi, wpl2, threads, ctx->stream));
+    }
+    return launched;
 }
 
 void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, uint64_t cu, double cd, void* out,
@@ -1849,6 +1906,92 @@ rfa_status rfa_pairs_autotune(rfa_context* ctx, const rfa_strings* a, const rfa_
         (void)hipEventDestroy(e1);
         ctx->seg_chain_steps = best;
         if (chosen) *chosen = best;
+    });
+}
+
+rfa_status rfa_cdist_autotune(rfa_scorer* sc, const rfa_strings* choices, int kind, uint64_t cutoff_u,
+                              const uint64_t* cutoffs_u, uint32_t* chosen)
+{
+    if (!sc || !choices) return fail(RFA_ERR_INVALID_ARGUMENT, "scorer or choices is NULL");
+    return guarded([&] {
+        rfa_context* ctx = sc->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_CHECK(hipSetDevice(ctx->device));
+        begin_call(ctx);
+        if (chosen) std::fill(chosen, chosen + 12, 0u);
+        validate_kind(sc->metric, kind, RFA_CACHED);
+        if (choices->width != sc->q->width)
+            throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
+        const int algo = algo_for(sc->metric, sc->w);
+        const uint64_t nq = sc->q->count, nc = choices->count;
+        if (nq == 0 || nc == 0 || sc->block_queries.empty()) return;
+        const uint64_t* d_qcut = nullptr;
+        uint64_t cu = cutoff_u;
+        if (cutoffs_u) {
+            std::vector<uint64_t> v(cutoffs_u, cutoffs_u + nq);
+            cu = *std::max_element(v.begin(), v.end());
+            d_qcut = upload(ctx->keep, ctx->qcut, v, ctx->stream);
+        }
+        const ScoreSpec spec = make_spec(sc, kind, RFA_CACHED, cu, 0.0, algo);
+        void* dout = ctx->out_tmp.get((size_t)nq * nc * 8);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        std::vector<uint32_t> rest;
+        try {
+            // 1 warm-up and 2 timed runs of a forced setting; < 0 when no band query runs
+            auto time_runs = [&](const BandForce& f) -> float {
+                if (!run_band_queries(sc, choices, spec, algo, kind, RFA_CACHED, cutoffs_u, d_qcut, dout, rest, &f))
+                    return -1.0f;
+                HIP_CHECK(hipEventRecord(e0, ctx->stream));
+                for (int r = 0; r < 2; ++r)
+                    run_band_queries(sc, choices, spec, algo, kind, RFA_CACHED, cutoffs_u, d_qcut, dout, rest, &f);
+                HIP_CHECK(hipEventRecord(e1, ctx->stream));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                return ms;
+            };
+            // words per lane (so lanes per pair and pairs per wave) for each bucket on its own
+            uint8_t wpl[6] = {0, 0, 0, 0, 0, 0};
+            bool any = false;
+            for (uint32_t b = 0; b < 6; ++b) {
+                const float t1 = time_runs(BandForce{b, 1, BAND_THREADS});
+                if (t1 < 0.0f) continue;
+                any = true;
+                const float t2 = b ? time_runs(BandForce{b, 2, BAND_THREADS}) : -1.0f;
+                wpl[b] = (t2 >= 0.0f && t2 < t1) ? 2 : 1;
+            }
+            if (any) {
+                for (uint32_t b = 0; b < 6; ++b)
+                    if (wpl[b]) ctx->band_wpl[b] = wpl[b];
+                // then the workgroup size (pairs per workgroup) of the whole launch
+                static const uint32_t threads[] = {64, 128, 256};
+                float best_ms = -1.0f;
+                uint32_t best_t = BAND_THREADS;
+                for (uint32_t t : threads) {
+                    const float ms = time_runs(BandForce{6, 0, t});
+                    if (ms >= 0.0f && (best_ms < 0.0f || ms < best_ms)) {
+                        best_ms = ms;
+                        best_t = t;
+                    }
+                }
+                ctx->band_threads = (uint16_t)best_t;
+                if (chosen)
+                    for (uint32_t b = 0; b < 6; ++b)
+                        if (wpl[b]) {
+                            chosen[2 * b] = wpl[b];
+                            chosen[2 * b + 1] = best_t;
+                        }
+            }
+        }
+        catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     });
 }
 

@@ -213,7 +213,7 @@ struct rfa_context {
     rfa::DevBuf lane_arena, lane_items, lane_swap, lane_metric, lane_out, block_items[4], plan_keys, plan_vals,
         plan_temp;
     rfa::DevBuf topk_scores, topk_run[2], topk_cand_s, topk_cand_i, topk_counts, topk_q, topk_out, qcut;
-    rfa::DevBuf ed_items, ed_rows, ed_ops, band_q;
+    rfa::DevBuf ed_items, ed_rows, ed_ops, band_q, band_w;
     rfa::DevBuf tset_cols, tset_rows; // partial_token_ratio: the repeat columns and rows
     size_t lane_arena_bytes_zeroed = 0;
     uint32_t gen_next = 1;
@@ -223,6 +223,11 @@ struct rfa_context {
     bool pairs_graph_on = false; // RFA_OPTION_PAIRS_GRAPH
     uint32_t seg_chain_steps = rfa::SEG_CHAIN_STEPS; // RFA_OPTION_PAIRS_CHAIN, set by rfa_pairs_autotune
     bool no_band = false; // RFA_OPTION_CDIST_BAND = 0
+    // diagonal band kernel launch settings, set by rfa_cdist_autotune: words per lane for each
+    // window-size bucket (band_bucket; 0: the rule of band_rule_wpl) and the workgroup size of
+    // the launch (0: BAND_THREADS)
+    uint8_t band_wpl[6] = {0, 0, 0, 0, 0, 0};
+    uint16_t band_threads = 0;
     const double* pcut = nullptr; // per-pair cutoffs of the cdist launches in progress (WRatio)
     rfa::DevBuf wr_buf[3];         // WRatio's intermediate matrices
     uint64_t* d_counters = nullptr; // RFA_OPTION_COUNT_WORD_STEPS: band word-steps, band pairs
@@ -331,6 +336,7 @@ struct rfa_scorer {
     // partial_token_ratio: the queries that repeat a token (their deduplicated join is shorter
     // than their sorted join) and a partial_ratio scorer over just their deduplicated joins
     std::vector<uint32_t> dup_queries;
+    uint32_t* d_dup_queries = nullptr; // dup_queries on the device (row scatter)
     rfa_scorer* pr_dedup_dupq = nullptr;
     bool dup_queries_ready = false;
     rfa_strings* tok_choices = nullptr; // token-sorted copy of the last choice set

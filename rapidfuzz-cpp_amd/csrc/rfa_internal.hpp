@@ -97,16 +97,23 @@ constexpr uint32_t BAND_THREADS = 256;
 constexpr uint32_t BAND_MAX_WORDS = 64;
 __host__ __device__ inline uint32_t band_words(uint32_t k) { return (k + 2 + 63) / 64; }
 __host__ __device__ inline uint32_t band_pad(uint32_t k) { return (k + 1 + 63) / 64 + 1; }
-// uint64 words of one zero-padded PM row of a query of m units under cutoff k
-__host__ __device__ inline uint32_t band_row_words(uint32_t m, uint32_t k)
+// window words of a pair with WPL words per lane (the last lane's words all run)
+__host__ __device__ inline uint32_t band_window_words(uint32_t k, uint32_t wpl)
 {
-    return band_pad(k) + (m + 63) / 64 + (k + 64 * band_words(k) + 63) / 64 + 2;
+    return (band_words(k) + wpl - 1) / wpl * wpl;
+}
+// uint64 words of one zero-padded PM row of a query of m units under cutoff k
+__host__ __device__ inline uint32_t band_row_words(uint32_t m, uint32_t k, uint32_t wpl)
+{
+    return band_pad(k) + (m + 63) / 64 + (k + 64 * band_window_words(k, wpl) + 63) / 64 + 2;
 }
 // dynamic LDS of one workgroup: the byte -> row map and `rows` PM rows
-inline uint64_t band_lds_bytes(uint32_t m, uint32_t k, uint32_t rows)
+inline uint64_t band_lds_bytes(uint32_t m, uint32_t k, uint32_t rows, uint32_t wpl)
 {
-    return 1024 + (uint64_t)rows * band_row_words(m, k) * 8;
+    return 1024 + (uint64_t)rows * band_row_words(m, k, wpl) * 8;
 }
+// pairs one wave runs: floor(64 / lanes per pair)
+inline uint32_t band_pairs_per_wave(uint32_t k, uint32_t wpl) { return 64 / ((band_words(k) + wpl - 1) / wpl); }
 struct BandArgs {
     const uint8_t* qchars;
     const uint64_t* qoff;
@@ -121,8 +128,12 @@ struct BandArgs {
     void* out;               // out[q * out_stride + c]
     const uint64_t* qcut;    // integer cutoff per query (nullptr: spec.cutoff_u)
     unsigned long long* counters; // nullptr, or [word-steps, pairs] accumulated by the launch
+    const uint8_t* wpl;      // words per lane (1 or 2) of each query slot
 };
-hipError_t launch_band(const BandArgs& a, uint32_t nblocks, uint32_t lds_bytes, bool multi, hipStream_t s);
+// multi: some window has several words; wpl2: some query runs two words per lane;
+// threads: workgroup size (64 .. BAND_THREADS, whole waves)
+hipError_t launch_band(const BandArgs& a, uint32_t nblocks, uint32_t lds_bytes, bool multi, bool wpl2,
+                       uint32_t threads, hipStream_t s);
 
 // Lane kernel (rfa_lane.hip): one pair per lane, per-lane 128-slot maps per 64-unit block.
 struct LaneArgs {
@@ -267,6 +278,8 @@ struct TsetArgs {
 };
 hipError_t launch_scatter_cols(double* dst, uint64_t stride, const double* src, const uint32_t* cols, uint64_t ncols,
                                uint64_t nq, hipStream_t s);
+hipError_t launch_scatter_rows(double* dst, uint64_t stride, const double* src, const uint32_t* rows, uint64_t nrows,
+                               hipStream_t s);
 hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
                        hipStream_t s);
 

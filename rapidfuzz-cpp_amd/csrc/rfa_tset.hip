@@ -265,6 +265,27 @@ hipError_t launch_scatter_cols(double* dst, uint64_t stride, const double* src, 
     return hipGetLastError();
 }
 
+// dst[rows[r] * stride + c] = src[r * stride + c] for every r < nrows, c < stride
+__global__ __launch_bounds__(256) void k_scatter_rows(double* dst, uint64_t stride, const double* src,
+                                                      const uint32_t* rows, uint64_t nrows)
+{
+    const uint64_t total = nrows * stride;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = i / stride, c = i % stride;
+        dst[rows[r] * stride + c] = src[i];
+    }
+}
+
+hipError_t launch_scatter_rows(double* dst, uint64_t stride, const double* src, const uint32_t* rows, uint64_t nrows,
+                               hipStream_t s)
+{
+    const uint64_t total = nrows * stride;
+    if (!total) return hipSuccess;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_scatter_rows, dim3(blocks), dim3(256), 0, s, dst, stride, src, rows, nrows);
+    return hipGetLastError();
+}
+
 hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
                        hipStream_t s)
 {

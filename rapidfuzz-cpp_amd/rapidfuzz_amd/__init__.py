@@ -32,7 +32,7 @@ _STATUS = {0: "RFA_OK", 1: "RFA_ERR_INVALID_ARGUMENT", 2: "RFA_ERR_UNSUPPORTED",
 EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_context_destroy",
             "rfa_context_synchronize", "rfa_context_stream", "rfa_context_set_option", "rfa_context_counter", "rfa_strings_create", "rfa_strings_destroy",
             "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_scorer_set_option", "rfa_cdist", "rfa_cdist_cutoffs",
-            "rfa_pairs", "rfa_pairs_noalloc", "rfa_pairs_autotune", "rfa_topk", "rfa_editops",
+            "rfa_pairs", "rfa_pairs_noalloc", "rfa_pairs_autotune", "rfa_cdist_autotune", "rfa_topk", "rfa_editops",
             "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
             "rfa_server_get_stats"]
 
@@ -99,6 +99,7 @@ def load_library(path: str = LIB_PATH):
     L.rfa_pairs_noalloc.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double,
                                     _vp]
     L.rfa_pairs_autotune.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _vp]
+    L.rfa_cdist_autotune.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_uint64, _vp, _vp]
     L.rfa_topk.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp]
     L.rfa_editops.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64]
     L.rfa_server_create.argtypes = [_vp, _vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
@@ -301,6 +302,16 @@ class Scorer:
         out = np.zeros((self.nq, choices.count), dtype=dt)
         _check(_lib.rfa_cdist(self.h, choices.h, kind, semantics, cu, cd, out.ctypes.data, 0))
         return out
+
+    def cdist_autotune(self, choices: Strings, kind=DISTANCE, cutoff_u=None, cutoffs_u=None):
+        """rfa_cdist_autotune: the band kernel's words per lane and workgroup size for each
+        window-size bucket, measured on this cdist; returns a (6, 2) array (0: bucket unused)."""
+        chosen = np.zeros(12, dtype=np.uint32)
+        cu = default_cutoffs(self.metric, kind)[0] if cutoff_u is None else cutoff_u
+        cut = None if cutoffs_u is None else np.ascontiguousarray(cutoffs_u, dtype=np.uint64)
+        _check(_lib.rfa_cdist_autotune(self.h, choices.h, kind, cu, None if cut is None else cut.ctypes.data,
+                                       chosen.ctypes.data))
+        return chosen.reshape(6, 2)
 
     def topk(self, choices: Strings, k: int, kind=None, cutoff_d=0.0):
         """(index, score), each [nq, k]: the extract() results >= cutoff_d ordered by

@@ -114,3 +114,24 @@ def test_band_single_cutoff_and_weights(ctx, oracle):
             got = sc.cdist(cs, 0, cutoff_u=cu)
             exp = oracle.cdist(0, 0, queries, choices, weights=(w, w, w), cutoff_u=cu)
             assert np.array_equal(got, exp), f"weights {w} cutoff {cu}: {np.count_nonzero(got != exp)} differ"
+
+
+def test_band_autotune(ctx, oracle):
+    """rfa_cdist_autotune measures words per lane x workgroup size per window-size bucket; the
+    settings it keeps give the same results (and so do the rule's, before it runs)."""
+    import rapidfuzz_amd as rfa
+
+    queries, choices = _case(321, 8, 300, 120, 2600, 4)
+    full = oracle.cdist_mt(0, 0, queries, choices)
+    ks = [0, 3, 70, 150, 400, 700, 1300, 2000]
+    cuts = np.array([min(k, len(q) - 1) for k, q in zip(ks, queries)], dtype=np.uint64)
+    exp = np.where(full <= cuts[:, None], full, cuts[:, None] + 1).astype(np.uint64)
+    sc = rfa.Scorer(ctx, 0, ctx.strings(queries, width=1))
+    cs = ctx.strings(choices, width=1)
+    assert np.array_equal(sc.cdist(cs, 0, cutoffs_u=cuts), exp)
+    chosen = sc.cdist_autotune(cs, 0, cutoffs_u=cuts)
+    used = [b for b in range(6) if chosen[b, 0]]
+    assert len(used) >= 4, chosen
+    for b in used:
+        assert chosen[b, 0] in (1, 2) and chosen[b, 1] in (64, 128, 256), chosen
+    assert np.array_equal(sc.cdist(cs, 0, cutoffs_u=cuts), exp)

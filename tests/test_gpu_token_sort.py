@@ -182,3 +182,32 @@ def test_rejected(ctx):
         sc.cdist(ctx.strings([b"b a"]), SIM, semantics=1)
     with pytest.raises(rfa.InvalidArgument):
         sc.set_option(99, 1)
+
+
+def test_partial_token_ratio_signed_switch(ctx, oracle):
+    """partial_token_ratio's per-scorer caches (the repeat rows and their scorer) follow a switch
+    of RFA_SCORER_SIGNED_UNITS: queries and choices that repeat tokens with units >= 0x80,
+    scored unsigned, then signed, then unsigned again on one scorer."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(597)
+    toks = [[0xE9, 0x61], [0x61, 0xE9], [0x62], [0xC3, 0x62, 0x62], [0x7A]]
+
+    def sent(n):
+        out = []
+        for i in rng.integers(0, len(toks), size=n):
+            if out:
+                out.append(0x20)
+            out.extend(toks[int(i)])
+        return np.array(out, np.uint32)
+
+    q = [sent(int(rng.integers(1, 6))) for _ in range(40)]
+    c = [sent(int(rng.integers(1, 9))) for _ in range(70)]
+    sc = rfa.Scorer(ctx, PTRATIO, ctx.strings(q, width=1))
+    cs = ctx.strings(c, width=1)
+    for signed8 in (False, True, False):
+        sc.set_option(rfa.SCORER_SIGNED_UNITS, 1 if signed8 else 0)
+        for cut in (0.0, 70.0):
+            got = sc.cdist(cs, SIM, cutoff_d=cut)
+            exp = oracle.cdist(PTRATIO, SIM, q, c, cutoff_d=cut, char_width=1, signed8=signed8)
+            assert np.array_equal(got.view(np.uint64), exp.view(np.uint64)), (signed8, cut)
