@@ -1185,7 +1185,8 @@ def workload_dna(args, emit=True):
                            "ops_model": "42 INT32 ops per word-step (2 x the 21 64-bit ops of the reference's block "
                                         "word, Levenshtein_impl.hpp:700-736)"},
             "kernel": "k_band<multi-word> (diagonal window, floor(64 / words) pairs per wave)",
-            "valu_issue_utilization": prof.get(f"valu_issue_utilization_{int(frac * 100)}pct"),
+            "valu_instructions_per_cu_cycle": prof.get(f"valu_instructions_per_cu_cycle_{int(frac * 100)}pct"),
+            "lane_instructions_per_word_step": prof.get(f"lane_instructions_per_word_step_{int(frac * 100)}pct"),
             "source": "profiles/band_dna_256x16384.md"}
         print(f"# dna cutoff {frac:.0%}: {wsteps:.4g} word-steps over {npairs} pairs run, "
               f"{gfx_tops:.2f} TOP/s ({gfx_tops / peak_tops:.3f} of peak)", file=sys.stderr, flush=True)

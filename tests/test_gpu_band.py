@@ -135,3 +135,32 @@ def test_band_autotune(ctx, oracle):
     for b in used:
         assert chosen[b, 0] in (1, 2) and chosen[b, 1] in (64, 128, 256), chosen
     assert np.array_equal(sc.cdist(cs, 0, cutoffs_u=cuts), exp)
+
+
+def test_band_long_texts_small_cutoffs(ctx, oracle):
+    """Long queries (2,600-5,200 units) under small cutoffs (1-6% of m): close matches (light
+    mutations of the query) run every text unit, far past the point where unrelated strings and
+    heavy mutations end, in waves whose other pairs come and go."""
+    rng = np.random.default_rng(515)
+    queries = _gen.random_strings(rng, 5, 2600, 5200, alpha_lo=65, alpha_n=4)
+    choices = []
+    for i in range(260):
+        q = queries[i % 5]
+        if i % 3 == 0:
+            choices.append(_gen.mutate(rng, q, int(rng.integers(0, len(q) // 60 + 1)), alpha_lo=65, alpha_n=4))
+        elif i % 3 == 1:
+            choices.append(_gen.mutate(rng, q, int(rng.integers(len(q) // 40, len(q) // 8)), alpha_lo=65, alpha_n=4))
+        else:
+            n = max(1, len(q) + int(rng.integers(-60, 61)))
+            choices.append(rng.integers(65, 69, size=n).astype(np.uint32))
+    full = oracle.cdist_mt(0, 0, queries, choices)
+    for frac in (0.01, 0.03, 0.06):
+        cuts = np.array([int(frac * len(q)) for q in queries], dtype=np.uint64)
+        exp = np.where(full <= cuts[:, None], full, cuts[:, None] + 1).astype(np.uint64)
+        assert int(np.count_nonzero(full <= cuts[:, None])) >= 50
+        got = _run(ctx, queries, choices, cuts, True)
+        bad = np.argwhere(got != exp)
+        assert len(bad) == 0, (frac, len(bad), [(int(i), int(j), int(got[i, j]), int(exp[i, j])) for i, j in bad[:4]])
+    cuts = np.array([int(0.02 * len(q)) for q in queries], dtype=np.uint64)
+    exp = np.where(full <= cuts[:, None], full, cuts[:, None] + 1).astype(np.uint64)
+    assert np.array_equal(_run(ctx, queries, choices, cuts, False), exp)
