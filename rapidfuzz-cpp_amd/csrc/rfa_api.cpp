@@ -1084,7 +1084,7 @@ uint32_t run_band_queries(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec
     for (uint32_t qi : sc->block_queries) {
         const uint64_t m = sc->q->len(qi);
         const uint64_t cut = qcut_host ? qcut_host[qi] : spec.cutoff_u;
-        const uint64_t k = host_ceil_div(cut, spec.ins);
+        const uint64_t k = spec.ins ? host_ceil_div(cut, spec.ins) : ~0ull; // insert weight 0: no band
         bool ok = kernel_ok && k < m && band_words((uint32_t)std::min<uint64_t>(k, 1u << 20)) <= BAND_MAX_WORDS;
         uint32_t wpl = 1;
         if (ok) {
