@@ -20,10 +20,14 @@
 //                 class Levenshtein before the LCS metrics, each by text length
 //                 (exact below 256 units, 32 bins per power of two above),
 //                 longest first, so the lanes of a wave finish together.
-//   passes   : k_bin_count -- bin of every pair, per-workgroup histograms in
-//                             LDS, written bin-major;
-//              exclusive scan of the histograms (one hipcub scan) -- the
-//                             output position of each (bin, workgroup);
+//   passes   : k_bin_count   -- bin of every pair, per-workgroup histograms in
+//                               LDS, each written whole (coalesced);
+//              k_bin_colscan -- per bin, the running count over the workgroups
+//                               (one thread per bin) and the bin's total;
+//              exclusive scan of the bin totals (one hipcub scan) -- the start
+//                               of each bin; a pair's output position is its
+//                               bin's start + the workgroups before it + its
+//                               rank inside its workgroup;
 //              k_bin_scatter -- items placed with LDS cursors;
 //              k_bounds      -- range boundaries for the kernels.
 // The host never reads the plan back, so a call costs O(1) host work.
@@ -98,7 +102,7 @@ __device__ __forceinline__ uint32_t pair_bin(const uint64_t* __restrict__ aoff, 
     return bin;
 }
 
-// Workgroup g bins pairs [g * chunk, min(n, (g + 1) * chunk)); hist[bin * groups + g].
+// Workgroup g bins pairs [g * chunk, min(n, (g + 1) * chunk)); hist[g * NBINS + bin].
 __global__ __launch_bounds__(PLAN_THREADS) void k_bin_count(const uint64_t* __restrict__ aoff,
                                                             const uint64_t* __restrict__ boff,
                                                             const uint8_t* __restrict__ metric, uint64_t n,
@@ -117,27 +121,45 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_bin_count(const uint64_t* __re
         atomicAdd(&h[b], 1u);
     }
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < NBINS; b += PLAN_THREADS) hist[(uint64_t)b * gridDim.x + blockIdx.x] = h[b];
+    for (uint32_t b = threadIdx.x; b < NBINS; b += PLAN_THREADS) hist[(uint64_t)blockIdx.x * NBINS + b] = h[b];
+}
+
+// One thread per bin: pos[g * NBINS + bin] = pairs of the bin in workgroups before g;
+// tot[bin] = the bin's pairs.  Every access of a step is coalesced across the bins.
+__global__ __launch_bounds__(256) void k_bin_colscan(const uint32_t* __restrict__ hist, uint32_t groups,
+                                                    uint32_t* __restrict__ pos, uint32_t* __restrict__ tot)
+{
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= NBINS) return;
+    uint32_t run = 0;
+    for (uint32_t g = 0; g < groups; ++g) {
+        const uint32_t v = hist[(uint64_t)g * NBINS + b];
+        pos[(uint64_t)g * NBINS + b] = run;
+        run += v;
+    }
+    tot[b] = run;
 }
 
 __global__ __launch_bounds__(PLAN_THREADS) void k_bin_scatter(uint64_t n, uint64_t chunk,
                                                               const uint32_t* __restrict__ bins,
                                                               const uint32_t* __restrict__ vals,
                                                               const uint32_t* __restrict__ pos,
+                                                              const uint32_t* __restrict__ start,
                                                               uint32_t* __restrict__ out)
 {
     __shared__ uint32_t cur[NBINS];
-    for (uint32_t b = threadIdx.x; b < NBINS; b += PLAN_THREADS) cur[b] = pos[(uint64_t)b * gridDim.x + blockIdx.x];
+    for (uint32_t b = threadIdx.x; b < NBINS; b += PLAN_THREADS)
+        cur[b] = start[b] + pos[(uint64_t)blockIdx.x * NBINS + b];
     __syncthreads();
     const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = std::min<uint64_t>(n, lo + chunk);
     for (uint64_t i = lo + threadIdx.x; i < hi; i += PLAN_THREADS) out[atomicAdd(&cur[bins[i]], 1u)] = vals[i];
 }
 
 // counters[1..3]: ends of the block and OSA ranges; counters[4 + r]: end of segmented range r
-__global__ void k_bounds(const uint32_t* __restrict__ pos, uint32_t groups, uint64_t n, uint32_t* __restrict__ counters)
+__global__ void k_bounds(const uint32_t* __restrict__ start, uint64_t n, uint32_t* __restrict__ counters)
 {
     const uint32_t t = threadIdx.x;
-    auto start_of = [&](uint32_t bin) { return bin < NBINS ? pos[(uint64_t)bin * groups] : (uint32_t)n; };
+    auto start_of = [&](uint32_t bin) { return bin < NBINS ? start[bin] : (uint32_t)n; };
     if (t == 0) counters[0] = 0;
     if (t == 1) counters[1] = start_of(OSA_BASE);
     if (t == 2) counters[2] = start_of(OSA_BASE + TBINS);
@@ -150,12 +172,13 @@ static uint32_t plan_groups(uint64_t n)
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(PLAN_MAX_GROUPS, (n + 8191) / 8192));
 }
 
+// temp: hist and pos (NBINS per workgroup each), the bin totals and starts, the scan's own
 size_t plan_temp_bytes(uint64_t n)
 {
     const size_t cells = (size_t)NBINS * plan_groups(n);
     size_t scan = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)cells);
-    return 2 * cells * sizeof(uint32_t) + scan + 256;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)NBINS);
+    return (2 * cells + 2 * NBINS) * sizeof(uint32_t) + scan + 256;
 }
 
 hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const uint8_t* metric, uint64_t n,
@@ -167,14 +190,18 @@ hipError_t launch_plan(const uint64_t* aoff, const uint64_t* boff, const uint8_t
     const size_t cells = (size_t)NBINS * groups;
     uint32_t* hist = (uint32_t*)temp;
     uint32_t* pos = hist + cells;
-    void* scan_temp = (void*)(((uintptr_t)(pos + cells) + 255) & ~(uintptr_t)255);
+    uint32_t* tot = pos + cells;
+    uint32_t* start = tot + NBINS;
+    void* scan_temp = (void*)(((uintptr_t)(start + NBINS) + 255) & ~(uintptr_t)255);
     size_t scan_bytes = temp_bytes - ((uint8_t*)scan_temp - (uint8_t*)temp);
     hipLaunchKernelGGL(k_bin_count, dim3(groups), dim3(PLAN_THREADS), 0, s, aoff, boff, metric, n, chain_steps, chunk,
                        bins, vals_in, hist);
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(scan_temp, scan_bytes, hist, pos, (int)cells, s);
+    hipLaunchKernelGGL(k_bin_colscan, dim3((NBINS + 255) / 256), dim3(256), 0, s, hist, groups, pos, tot);
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(scan_temp, scan_bytes, tot, start, (int)NBINS, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_scatter, dim3(groups), dim3(PLAN_THREADS), 0, s, n, chunk, bins, vals_in, pos, vals_out);
-    hipLaunchKernelGGL(k_bounds, dim3(1), dim3(64), 0, s, pos, groups, n, counters);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(groups), dim3(PLAN_THREADS), 0, s, n, chunk, bins, vals_in, pos, start,
+                       vals_out);
+    hipLaunchKernelGGL(k_bounds, dim3(1), dim3(64), 0, s, start, n, counters);
     return hipGetLastError();
 }
 

@@ -15,7 +15,10 @@
 //   PM(c) = AND_b (plane_b XNOR bit_b(c))   over the NB low symbol bits
 // (NB >= bit length of the largest symbol of the launch; a text symbol with a
 // bit set at or above NB matches nothing).  That is NB three-input bit ops per
-// 32-bit half word and no LDS, so occupancy is set by registers alone.
+// 32-bit half word and no LDS, so occupancy is set by registers alone.  The
+// planes of 32 units are one transpose of the 32 x NB bit matrix (units as
+// rows): a byte-permute stage, then masked swaps of 4-, 2- and 1-bit blocks,
+// about 4 VALU per unit.
 //
 // Carries between the words of a pair use the carry-lookahead of the block
 // kernel (generate / propagate ballots resolved by one 64-bit add), with the
@@ -39,47 +42,97 @@
 
 namespace rfa {
 
-// bit i set for every lane i that starts a segment of SEG lanes
-template <uint32_t SEG>
-__device__ __forceinline__ constexpr uint64_t seg_first_mask()
+// bit i set for every lane i that starts a segment of 2^lseg lanes (wave-uniform)
+__device__ __forceinline__ uint64_t seg_first_mask(uint32_t lseg)
 {
-    uint64_t m = 0;
-    for (uint32_t i = 0; i < 64; i += SEG) m |= 1ull << i;
+    uint64_t m = 1;
+    for (uint32_t s = 1u << lseg; s < 64; s <<= 1) m |= m << s;
     return m;
 }
 
 constexpr uint32_t T_AND_XNOR = (TA & ~(TB ^ TC)) & 0xFF; // a & ~(b ^ c)
 
+// Bit-planes of 32 pattern units: pl[b] bit i = bit b of unit i0 + i, from 4 loads of
+// 8 units issued together (a chunk at or past cnt re-reads the pattern's start, so every
+// load stays inside the set).  Units past cnt are whatever the loads return; their bits
+// reach only plane positions >= cnt - i0, which the caller masks off.  The transpose runs
+// in place over r[] (rows = units, columns = symbol bits): blocks of 16 and 8 bits trade
+// places by byte permutes (or a shifted OR where the partner block is known zero: symbols
+// below 2^NB), then 4-, 2- and 1-bit blocks by shift + bit-field insert.
+template <typename CharT, uint32_t NB>
+__device__ __forceinline__ void seg_planes(const CharT* __restrict__ p, uint32_t cnt, uint32_t i0, uint32_t (&pl)[NB])
+{
+    uint32_t r[32];
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c) load_units<CharT, 8>(i0 + 8 * c < cnt ? p + i0 + 8 * c : p, r + 8 * c);
+    if constexpr (NB <= 16) {
+#pragma unroll
+        for (uint32_t i = 0; i < 16; ++i) r[i] |= r[i + 16] << 16;
+    }
+    else {
+#pragma unroll
+        for (uint32_t i = 0; i < 16; ++i) {
+            const uint32_t a = r[i], b = r[i + 16];
+            r[i + 16] = __builtin_amdgcn_perm(a, b, 0x03020706u);
+            r[i] = __builtin_amdgcn_perm(a, b, 0x01000504u);
+        }
+    }
+    constexpr uint32_t R16 = NB <= 16 ? 16 : 32;
+    if constexpr (NB <= 8) {
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i) r[i] |= r[i + 8] << 8;
+    }
+    else {
+#pragma unroll
+        for (uint32_t base = 0; base < R16; base += 16)
+#pragma unroll
+            for (uint32_t i = base; i < base + 8; ++i) {
+                const uint32_t a = r[i], b = r[i + 8];
+                r[i + 8] = __builtin_amdgcn_perm(a, b, 0x03070105u);
+                r[i] = __builtin_amdgcn_perm(a, b, 0x02060004u);
+            }
+    }
+    constexpr uint32_t R = NB <= 8 ? 8 : R16;
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k) {
+        const uint32_t sh = 4u >> k;
+        const uint32_t m = k == 0 ? 0x0F0F0F0Fu : (k == 1 ? 0x33333333u : 0x55555555u);
+#pragma unroll
+        for (uint32_t i = 0; i < R; ++i)
+            if ((i & sh) == 0) {
+                const uint32_t a = r[i], b = r[i + sh];
+                r[i + sh] = (m & (a >> sh)) | (~m & b);
+                r[i] = (~m & (b << sh)) | (m & a);
+            }
+    }
+#pragma unroll
+    for (uint32_t b = 0; b < NB; ++b) pl[b] = r[b];
+}
+
 // Recurrence over the text for one word per lane; returns the lane's share of
 // the pair's raw value (Levenshtein: sum of vertical deltas of the last
 // column; LCS: matched rows).
-template <uint32_t SEG, typename CharT, uint32_t NB>
+// MULTI: segments of 2^lseg >= 2 lanes (the carries cross lanes); otherwise one lane
+// per pair.  All multi-lane classes share one body with their masks as wave-uniform
+// values, which keeps the kernel's code within the instruction cache.
+template <bool MULTI, typename CharT, uint32_t NB>
 __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const CharT* __restrict__ t, uint32_t cnt,
-                                           uint32_t n, bool wact, bool lev, uint32_t nmax, uint32_t lane)
+                                           uint32_t n, bool wact, bool lev, uint32_t nmax, uint32_t lane,
+                                           uint32_t lseg)
 {
-    constexpr uint64_t FIRSTM = seg_first_mask<SEG>();
-    constexpr uint64_t LASTM = FIRSTM << (SEG - 1);
-    // bit-planes of this word's units (lo: units 0-31, hi: units 32-63), built from
-    // loads of CH units; units past cnt only set bits that vml / vmh mask off
+    const uint64_t FIRSTM = MULTI ? seg_first_mask(lseg) : ~0ull;
+    const uint64_t LASTM = MULTI ? FIRSTM << ((1u << lseg) - 1) : ~0ull;
+    // bit-planes of this word's units (lo: units 0-31, hi: units 32-63); units past
+    // cnt only set bits that vml / vmh mask off
     constexpr uint32_t CH = 8;
     uint32_t pl[NB], ph[NB];
-#pragma unroll
-    for (uint32_t b = 0; b < NB; ++b) pl[b] = ph[b] = 0;
-    for (uint32_t i0 = 0; i0 < min(cnt, 32u); i0 += CH) {
-        uint32_t u[CH];
-        load_units<CharT, CH>(p + i0, u);
-#pragma unroll
-        for (uint32_t k = 0; k < CH; ++k)
-#pragma unroll
-            for (uint32_t b = 0; b < NB; ++b) pl[b] |= ((u[k] >> b) & 1u) << (i0 + k);
+    seg_planes<CharT, NB>(p, cnt, 0u, pl);
+    if (ballot(cnt > 32) != 0) {
+        seg_planes<CharT, NB>(p, cnt, 32u, ph);
     }
-    for (uint32_t i0 = 32; i0 < cnt; i0 += CH) {
-        uint32_t u[CH];
-        load_units<CharT, CH>(p + i0, u);
+    else {
 #pragma unroll
-        for (uint32_t k = 0; k < CH; ++k)
-#pragma unroll
-            for (uint32_t b = 0; b < NB; ++b) ph[b] |= ((u[k] >> b) & 1u) << (i0 - 32 + k);
+        for (uint32_t b = 0; b < NB; ++b) ph[b] = 0;
     }
     const uint32_t vml = cnt >= 32 ? ~0u : ((1u << cnt) - 1u);
     const uint32_t vmh = cnt >= 64 ? ~0u : (cnt > 32 ? ((1u << (cnt - 32)) - 1u) : 0u);
@@ -105,7 +158,7 @@ __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const Ch
             const bool al = act && lev;
             const uint64_t vp = VP, vn = VN;
             uint64_t D0, HP, HN;
-            if constexpr (SEG == 1) {
+            if constexpr (!MULTI) {
                 D0 = (((X & vp) + vp) ^ vp) | X | vn;
                 HP = vn | ~(D0 | vp);
                 HN = D0 & vp;
@@ -139,7 +192,7 @@ __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const Ch
             const uint64_t Sv = VP;
             const uint64_t u = Sv & X;
             uint64_t x = Sv + u;
-            if constexpr (SEG > 1) {
+            if constexpr (MULTI) {
                 const bool gg = ac && (x < Sv);
                 const bool pp = ac && !gg && (x == ~0ull);
                 const uint64_t G = ballot(gg) & ~LASTM, Pp = ballot(pp) & ~LASTM;
@@ -175,14 +228,14 @@ __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const Ch
     return __popcll(~VP & M);
 }
 
-// One group of PPW pairs of class SEG, items [first, min(first + PPW, end)).
+// One group of 64 >> lseg pairs (segments of 2^lseg lanes), items [first, min(first + pairs, end)).
 // INTK: integer result kinds (distance, similarity) with the integer epilogue.
-template <uint32_t SEG, typename CharT, uint32_t NB, bool INTK>
-__device__ __forceinline__ void seg_group(const SegArgs& a, uint32_t first, uint32_t end, uint32_t lane)
+template <bool MULTI, typename CharT, uint32_t NB, bool INTK>
+__device__ __forceinline__ void seg_group(const SegArgs& a, uint32_t first, uint32_t end, uint32_t lane, uint32_t lseg)
 {
     const CharT* __restrict__ A = (const CharT*)a.achars;
     const CharT* __restrict__ B = (const CharT*)a.bchars;
-    const uint32_t slot = lane / SEG, w = lane % SEG;
+    const uint32_t slot = lane >> lseg, w = lane & ((1u << lseg) - 1);
     const uint32_t item = first + slot;
     const bool valid = item < end;
     uint32_t pair = 0, la = 0, lb = 0, m = 0, n = 0;
@@ -214,9 +267,9 @@ __device__ __forceinline__ void seg_group(const SegArgs& a, uint32_t first, uint
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor(nmax, off, 64));
     nmax = uniform(nmax);
-    int32_t v = seg_run<SEG, CharT, NB>(p + (wact ? w * 64 : 0u), t, cnt, n, wact, lev, nmax, lane);
-#pragma unroll
-    for (uint32_t off = SEG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    int32_t v = seg_run<MULTI, CharT, NB>(p + (wact ? w * 64 : 0u), t, cnt, n, wact, lev, nmax, lane, lseg);
+    if constexpr (MULTI)
+        for (uint32_t off = 1u << (lseg - 1); off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (valid && w == 0) {
         if (!done) res.v = lev ? (uint64_t)((int64_t)n + v) : (uint64_t)v;
         if constexpr (INTK)
@@ -256,15 +309,11 @@ __global__ __launch_bounds__(256) void k_seg(SegArgs a)
             __builtin_amdgcn_s_setprio(2);
         else
             __builtin_amdgcn_s_setprio(0);
-        switch (k) {
-        case 0: seg_group<64, CharT, NB, INTK>(a, first, end, lane); break;
-        case 1: seg_group<32, CharT, NB, INTK>(a, first, end, lane); break;
-        case 2: seg_group<16, CharT, NB, INTK>(a, first, end, lane); break;
-        case 3: seg_group<8, CharT, NB, INTK>(a, first, end, lane); break;
-        case 4: seg_group<4, CharT, NB, INTK>(a, first, end, lane); break;
-        case 5: seg_group<2, CharT, NB, INTK>(a, first, end, lane); break;
-        default: seg_group<1, CharT, NB, INTK>(a, first, end, lane); break;
-        }
+        // class k: segments of 64 >> k lanes
+        if (k == SEG_CLASSES - 1)
+            seg_group<false, CharT, NB, INTK>(a, first, end, lane, 0u);
+        else
+            seg_group<true, CharT, NB, INTK>(a, first, end, lane, 6u - k);
     }
 }
 
