@@ -161,12 +161,14 @@ struct LaneArgs {
 };
 
 // Segmented kernel (rfa_seg.hip): pairs with patterns of <= 4096 units under the
-// Levenshtein or LCS recurrence.  Ranges r = 0..13 of the plan: r < 7 holds the
-// pairs with texts of >= chain_steps units (the long sequential chains), r >= 7
-// the rest; range r runs SEG = 64 >> (r % 7) lanes per pair.  chain_steps is
-// also the length from which the longer string becomes the pattern (tuned per
-// context by rfa_pairs_autotune; SEG_CHAIN_STEPS by default).
-constexpr uint32_t SEG_CLASSES = 7;
+// Levenshtein or LCS recurrence.  Ranges r = 0..15 of the plan: r < 8 holds the
+// pairs with texts of >= chain_steps units (the long sequential chains), r >= 8
+// the rest; class k = r % 8 runs SEG = 64 >> k lanes per pair for k < 7, and
+// k = 7 one lane per pair whose pattern has at most 32 units (32-bit words).
+// chain_steps is also the length from which the longer string becomes the
+// pattern (tuned per context by rfa_pairs_autotune; SEG_CHAIN_STEPS by default).
+constexpr uint32_t SEG_CLASSES = 8;
+constexpr uint32_t SEG_HALF_CLASS = 7;
 constexpr uint32_t SEG_CHAIN_STEPS = 512;
 struct SegArgs {
     const void* achars;

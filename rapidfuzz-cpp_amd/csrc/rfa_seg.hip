@@ -33,6 +33,7 @@
 // (round-robin over a longest-first list).  No shared queue: a device-wide
 // atomic per group costs more than a short group's whole scan.
 #include <algorithm>
+#include <type_traits>
 
 #include "rfa_block_impl.hpp"
 #include "rfa_device.hpp"
@@ -114,8 +115,9 @@ __device__ __forceinline__ void seg_planes(const CharT* __restrict__ p, uint32_t
 // column; LCS: matched rows).
 // MULTI: segments of 2^lseg >= 2 lanes (the carries cross lanes); otherwise one lane
 // per pair.  All multi-lane classes share one body with their masks as wave-uniform
-// values, which keeps the kernel's code within the instruction cache.
-template <bool MULTI, typename CharT, uint32_t NB>
+// values, which keeps the kernel's code within the instruction cache.  HALF (one lane,
+// patterns of at most 32 units): the word, the match mask and the recurrence are 32-bit.
+template <bool MULTI, bool HALF, typename CharT, uint32_t NB>
 __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const CharT* __restrict__ t, uint32_t cnt,
                                            uint32_t n, bool wact, bool lev, uint32_t nmax, uint32_t lane,
                                            uint32_t lseg)
@@ -124,10 +126,12 @@ __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const Ch
     const uint64_t LASTM = MULTI ? FIRSTM << ((1u << lseg) - 1) : ~0ull;
     // bit-planes of this word's units (lo: units 0-31, hi: units 32-63); units past
     // cnt only set bits that vml / vmh mask off
+    static_assert(!(MULTI && HALF), "half words run one lane per pair");
+    using Wd = std::conditional_t<HALF, uint32_t, uint64_t>;
     constexpr uint32_t CH = 8;
     uint32_t pl[NB], ph[NB];
     seg_planes<CharT, NB>(p, cnt, 0u, pl);
-    if (ballot(cnt > 32) != 0) {
+    if (!HALF && ballot(cnt > 32) != 0) {
         seg_planes<CharT, NB>(p, cnt, 32u, ph);
     }
     else {
@@ -138,7 +142,7 @@ __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const Ch
     const uint32_t vmh = cnt >= 64 ? ~0u : (cnt > 32 ? ((1u << (cnt - 32)) - 1u) : 0u);
 
     const bool any_lev = ballot(wact && lev) != 0, any_lcs = ballot(wact && !lev) != 0;
-    uint64_t VP = ~0ull, VN = 0; // LCS keeps S in VP
+    Wd VP = ~(Wd)0, VN = 0; // LCS keeps S in VP
 
     // one text position: column j, text unit c
     auto step = [&](uint32_t j, uint32_t c) {
@@ -151,13 +155,17 @@ __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const Ch
         for (uint32_t b = 0; b < NB; ++b) {
             const uint32_t sb = (uint32_t)(-(int32_t)((c >> b) & 1u)); // all ones iff bit b of c
             xl = bop3<T_AND_XNOR>(xl, pl[b], sb);
-            xh = bop3<T_AND_XNOR>(xh, ph[b], sb);
+            if constexpr (!HALF) xh = bop3<T_AND_XNOR>(xh, ph[b], sb);
         }
-        const uint64_t X = ((uint64_t)xh << 32) | xl;
+        Wd X;
+        if constexpr (HALF)
+            X = xl;
+        else
+            X = ((uint64_t)xh << 32) | xl;
         if (any_lev) {
             const bool al = act && lev;
-            const uint64_t vp = VP, vn = VN;
-            uint64_t D0, HP, HN;
+            const Wd vp = VP, vn = VN;
+            Wd D0, HP, HN;
             if constexpr (!MULTI) {
                 D0 = (((X & vp) + vp) ^ vp) | X | vn;
                 HP = vn | ~(D0 | vp);
@@ -189,9 +197,9 @@ __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const Ch
         }
         if (any_lcs) {
             const bool ac = act && !lev;
-            const uint64_t Sv = VP;
-            const uint64_t u = Sv & X;
-            uint64_t x = Sv + u;
+            const Wd Sv = VP;
+            const Wd u = Sv & X;
+            Wd x = Sv + u;
             if constexpr (MULTI) {
                 const bool gg = ac && (x < Sv);
                 const bool pp = ac && !gg && (x == ~0ull);
@@ -223,14 +231,20 @@ __device__ __forceinline__ int32_t seg_run(const CharT* __restrict__ p, const Ch
         chunk(ub, j0 + CH);
     }
     if (!wact) return 0;
-    const uint64_t M = ((uint64_t)vmh << 32) | vml;
-    if (lev) return __popcll(VP & M) - __popcll(VN & M);
-    return __popcll(~VP & M);
+    if constexpr (HALF) {
+        if (lev) return __popc(VP & vml) - __popc(VN & vml);
+        return __popc(~VP & vml);
+    }
+    else {
+        const uint64_t M = ((uint64_t)vmh << 32) | vml;
+        if (lev) return __popcll(VP & M) - __popcll(VN & M);
+        return __popcll(~VP & M);
+    }
 }
 
 // One group of 64 >> lseg pairs (segments of 2^lseg lanes), items [first, min(first + pairs, end)).
 // INTK: integer result kinds (distance, similarity) with the integer epilogue.
-template <bool MULTI, typename CharT, uint32_t NB, bool INTK>
+template <bool MULTI, bool HALF, typename CharT, uint32_t NB, bool INTK>
 __device__ __forceinline__ void seg_group(const SegArgs& a, uint32_t first, uint32_t end, uint32_t lane, uint32_t lseg)
 {
     const CharT* __restrict__ A = (const CharT*)a.achars;
@@ -267,7 +281,7 @@ __device__ __forceinline__ void seg_group(const SegArgs& a, uint32_t first, uint
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor(nmax, off, 64));
     nmax = uniform(nmax);
-    int32_t v = seg_run<MULTI, CharT, NB>(p + (wact ? w * 64 : 0u), t, cnt, n, wact, lev, nmax, lane, lseg);
+    int32_t v = seg_run<MULTI, HALF, CharT, NB>(p + (wact ? w * 64 : 0u), t, cnt, n, wact, lev, nmax, lane, lseg);
     if constexpr (MULTI)
         for (uint32_t off = 1u << (lseg - 1); off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (valid && w == 0) {
@@ -279,8 +293,8 @@ __device__ __forceinline__ void seg_group(const SegArgs& a, uint32_t first, uint
     }
 }
 
-// Ranges r = 0..13: counters[3 + r] .. counters[4 + r], SEG = 64 >> (r % 7); ranges
-// r < 7 hold the long-text pairs, whose waves run at raised issue priority so the
+// Ranges r = 0..15: counters[3 + r] .. counters[4 + r], class r % 8; ranges
+// r < 8 hold the long-text pairs, whose waves run at raised issue priority so the
 // longest sequential chains are not slowed by the many short groups around them.
 template <typename CharT, uint32_t NB, bool INTK>
 __global__ __launch_bounds__(256) void k_seg(SegArgs a)
@@ -292,7 +306,8 @@ __global__ __launch_bounds__(256) void k_seg(SegArgs a)
 #pragma unroll
     for (uint32_t r = 0; r < NR; ++r) {
         const uint32_t b = uniform(a.counters[3 + r]), e = uniform(a.counters[4 + r]);
-        const uint32_t ppw = 1u << (r % SEG_CLASSES); // pairs per wave: 64 / SEG
+        const uint32_t k = r % SEG_CLASSES;
+        const uint32_t ppw = k == SEG_HALF_CLASS ? 64u : 1u << k; // pairs per wave: 64 / SEG
         ibeg[r] = b;
         acc += e > b ? (e - b + ppw - 1) / ppw : 0u;
         gend[r] = acc;
@@ -303,17 +318,19 @@ __global__ __launch_bounds__(256) void k_seg(SegArgs a)
         while (r < NR - 1 && g >= gend[r]) ++r;
         const uint32_t g0 = r ? gend[r - 1] : 0u;
         const uint32_t k = r % SEG_CLASSES;
-        const uint32_t first = ibeg[r] + ((g - g0) << k);
+        const uint32_t first = ibeg[r] + (g - g0) * (k == SEG_HALF_CLASS ? 64u : 1u << k);
         const uint32_t end = uniform(a.counters[4 + r]);
         if (r < SEG_CLASSES)
             __builtin_amdgcn_s_setprio(2);
         else
             __builtin_amdgcn_s_setprio(0);
-        // class k: segments of 64 >> k lanes
-        if (k == SEG_CLASSES - 1)
-            seg_group<false, CharT, NB, INTK>(a, first, end, lane, 0u);
+        // class k < 7: segments of 64 >> k lanes; class 7: one lane, 32-bit words
+        if (k == SEG_HALF_CLASS)
+            seg_group<false, true, CharT, NB, INTK>(a, first, end, lane, 0u);
+        else if (k == SEG_HALF_CLASS - 1)
+            seg_group<false, false, CharT, NB, INTK>(a, first, end, lane, 0u);
         else
-            seg_group<true, CharT, NB, INTK>(a, first, end, lane, 6u - k);
+            seg_group<true, false, CharT, NB, INTK>(a, first, end, lane, 6u - k);
     }
 }
 
