@@ -62,6 +62,18 @@ def is_int_result(metric: int, kind: int) -> bool:
         kind in (DISTANCE, SIMILARITY)
 
 
+def _alignment(fn, s1, s2, cutoff):
+    a = np.ascontiguousarray(np.asarray(s1, dtype=np.uint32))
+    b = np.ascontiguousarray(np.asarray(s2, dtype=np.uint32))
+    sc = np.zeros(1, np.float64)
+    al = np.zeros(4, np.uint64)
+    r = fn(a.ctypes.data_as(_u32p), len(a), b.ctypes.data_as(_u32p), len(b), float(cutoff),
+           sc.ctypes.data_as(_f64p), al.ctypes.data_as(_u64p))
+    if r is not None and r != 0:
+        raise RuntimeError("partial_ratio_alignment failed")
+    return (float(sc[0]),) + tuple(int(x) for x in al)
+
+
 class Oracle:
     """The C restatement (rf_oracle.c)."""
 
@@ -93,6 +105,19 @@ class Oracle:
         for name in ("orc_lcs_raw", "orc_osa_raw"):
             getattr(L, name).restype = ctypes.c_uint64
             getattr(L, name).argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t]
+        L.orc_partial_ratio_alignment.restype = None
+        L.orc_partial_ratio_alignment.argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, ctypes.c_double,
+                                                  _f64p, _u64p]
+
+    def partial_ratio_alignment(self, s1, s2, cutoff=0.0):
+        """(score, src_start, src_end, dest_start, dest_end) of fuzz::partial_ratio_alignment."""
+        return _alignment(self.lib.orc_partial_ratio_alignment, s1, s2, cutoff)
+
+    def lcs_raw(self, s1, s2):
+        """Length of the longest common subsequence (plain DP)."""
+        a = np.ascontiguousarray(np.asarray(s1, dtype=np.uint32))
+        b = np.ascontiguousarray(np.asarray(s2, dtype=np.uint32))
+        return int(self.lib.orc_lcs_raw(a.ctypes.data_as(_u32p), len(a), b.ctypes.data_as(_u32p), len(b)))
 
     def cdist(self, metric, kind, queries, choices, weights=(1, 1, 1), prefix_weight=0.1,
               cutoff_u=None, cutoff_d=None, multi=False, char_width=4, signed8=False):
@@ -236,6 +261,9 @@ class Reference:
                                   _u64p]
         L.ref_opcodes.restype = ctypes.c_int64
         L.ref_opcodes.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _u64p]
+        L.ref_partial_ratio_alignment.restype = ctypes.c_int
+        L.ref_partial_ratio_alignment.argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, ctypes.c_double,
+                                                  _f64p, _u64p]
         L.ref_editops_algebra.restype = ctypes.c_int64
         L.ref_editops_algebra.argtypes = [ctypes.c_int, _i64p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_size_t, _i64p,
@@ -316,6 +344,10 @@ class Reference:
         if k < 0:
             raise ValueError(f"reference error {k}: {self._err()}")
         return t[:k], sp[:k], dp[:k]
+
+    def partial_ratio_alignment(self, s1, s2, cutoff=0.0):
+        """(score, src_start, src_end, dest_start, dest_end) from the reference itself."""
+        return _alignment(self.lib.ref_partial_ratio_alignment, s1, s2, cutoff)
 
     def editops_algebra(self, op, ops, src_len, dest_len, a=0, b=0, c=1, sub=None):
         """The reference's Editops / Opcodes algebra (ref_editops_algebra): ops is an (n, 3) array

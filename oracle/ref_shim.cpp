@@ -782,4 +782,25 @@ int64_t ref_editops_algebra(int op, const int64_t* ops, size_t n, size_t src_len
     }
 }
 
+// fuzz::partial_ratio_alignment (fuzz.hpp:129-135) on two strings of 32-bit units:
+// *score and al = {src_start, src_end, dest_start, dest_end}.
+int ref_partial_ratio_alignment(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, double cutoff,
+                                double* score, size_t* al)
+{
+    try {
+        std::basic_string<uint32_t> a(s1, s1 + l1), b(s2, s2 + l2);
+        const rapidfuzz::ScoreAlignment<double> r = rapidfuzz::fuzz::partial_ratio_alignment(a, b, cutoff);
+        *score = r.score;
+        al[0] = r.src_start;
+        al[1] = r.src_end;
+        al[2] = r.dest_start;
+        al[3] = r.dest_end;
+        return 0;
+    }
+    catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
 } // extern "C"

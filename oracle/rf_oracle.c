@@ -693,11 +693,16 @@ typedef struct {
 /* fuzz_detail::partial_ratio_impl (fuzz_impl.hpp:68-179): s1 is the needle, len1 <= len2.
  * The full windows s2[p, p + len1), p < len2 - len1, are searched by bisection of
  * [0, len2 - len1 - 1] with the reference's bound on the distances between two
- * evaluated windows; then the prefix and suffix windows. */
-static double partial_ratio_impl(const uint32_t* s1, size_t len1, const uint32_t* s2, size_t len2,
-                                 double score_cutoff)
+ * evaluated windows; then the prefix and suffix windows.  *ds, *de: the window of s2
+ * the result reports (res.dest_start / dest_end: [0, len1) until a window improves;
+ * a full window that beats the cutoff distance moves it even when its score then
+ * falls below score_cutoff). */
+static double partial_ratio_impl_at(const uint32_t* s1, size_t len1, const uint32_t* s2, size_t len2,
+                                    double score_cutoff, size_t* ds, size_t* de)
 {
     double res = 0;
+    *ds = 0;
+    *de = len1;
     if (len2 > len1) {
         const size_t maximum = len1 * 2;
         const double norm_cutoff_sim = normsim_to_normdist(score_cutoff / 100);
@@ -719,6 +724,8 @@ static double partial_ratio_impl(const uint32_t* s1, size_t len1, const uint32_t
                     scores[f] = indel_distance_full(s1, len1, s2 + f, len1);
                     if (scores[f] < cutoff_dist) {
                         cutoff_dist = best_dist = scores[f];
+                        *ds = f;
+                        *de = f + len1;
                         if (best_dist == 0) perfect = 1;
                     }
                 }
@@ -727,6 +734,8 @@ static double partial_ratio_impl(const uint32_t* s1, size_t len1, const uint32_t
                     scores[sc] = indel_distance_full(s1, len1, s2 + sc, len1);
                     if (scores[sc] < cutoff_dist) {
                         cutoff_dist = best_dist = scores[sc];
+                        *ds = sc;
+                        *de = sc + len1;
                         if (best_dist == 0) perfect = 1;
                     }
                 }
@@ -766,6 +775,8 @@ static double partial_ratio_impl(const uint32_t* s1, size_t len1, const uint32_t
         const double ls = ratio_similarity(s1, len1, s2, i, score_cutoff);
         if (ls > res) {
             score_cutoff = res = ls;
+            *ds = 0;
+            *de = i;
             if (res == 100.0) return res;
         }
     }
@@ -774,10 +785,19 @@ static double partial_ratio_impl(const uint32_t* s1, size_t len1, const uint32_t
         const double ls = ratio_similarity(s1, len1, s2 + i, len2 - i, score_cutoff);
         if (ls > res) {
             score_cutoff = res = ls;
+            *ds = i;
+            *de = len2;
             if (res == 100.0) return res;
         }
     }
     return res;
+}
+
+static double partial_ratio_impl(const uint32_t* s1, size_t len1, const uint32_t* s2, size_t len2,
+                                 double score_cutoff)
+{
+    size_t ds, de;
+    return partial_ratio_impl_at(s1, len1, s2, len2, score_cutoff, &ds, &de);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1127,6 +1147,52 @@ double orc_partial_ratio(const uint32_t* s1, size_t len1, const uint32_t* s2, si
         if (score2 > score) return score2;
     }
     return score;
+}
+
+/* fuzz::partial_ratio_alignment (fuzz_impl.hpp:196-230): out[0] score; src_start, src_end,
+ * dest_start, dest_end in al[0..3] (s1 is the source). */
+void orc_partial_ratio_alignment(const uint32_t* s1, size_t len1, const uint32_t* s2, size_t len2,
+                                 double score_cutoff, double* score, size_t* al)
+{
+    if (len1 > len2) {
+        size_t t[4];
+        orc_partial_ratio_alignment(s2, len2, s1, len1, score_cutoff, score, t);
+        al[0] = t[2];
+        al[1] = t[3];
+        al[2] = t[0];
+        al[3] = t[1];
+        return;
+    }
+    al[0] = 0;
+    al[1] = len1;
+    al[2] = 0;
+    al[3] = len1;
+    if (score_cutoff > 100) {
+        *score = 0;
+        return;
+    }
+    if (!len1 || !len2) {
+        *score = (len1 == len2) ? 100.0 : 0.0;
+        return;
+    }
+    size_t ds, de;
+    const double sc = partial_ratio_impl_at(s1, len1, s2, len2, score_cutoff, &ds, &de);
+    *score = sc;
+    al[2] = ds;
+    al[3] = de;
+    if (sc != 100 && len1 == len2) {
+        score_cutoff = (score_cutoff < sc) ? sc : score_cutoff; /* std::max */
+        size_t ds2, de2;
+        const double sc2 = partial_ratio_impl_at(s2, len2, s1, len1, score_cutoff, &ds2, &de2);
+        if (sc2 > sc) {
+            /* the second pass's source is s2: swap its fields back */
+            *score = sc2;
+            al[0] = ds2;
+            al[1] = de2;
+            al[2] = 0;
+            al[3] = len2;
+        }
+    }
 }
 
 int orc_cached_score(int metric, int kind, const uint32_t* s1, size_t l1, const uint32_t* s2,

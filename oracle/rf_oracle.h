@@ -80,6 +80,9 @@ size_t orc_sorted_join(const uint32_t* s, size_t n, uint32_t* out);
 
 /* fuzz::partial_ratio(s1, s2, cutoff): the reference's window search, step by step */
 double orc_partial_ratio(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, double cutoff);
+/* fuzz::partial_ratio_alignment(s1, s2, cutoff): *score and al = {src_start, src_end, dest_start, dest_end} */
+void orc_partial_ratio_alignment(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, double cutoff,
+                                 double* score, size_t* al);
 
 /* ---- API layer: Cached<Metric>(s1).<kind>(s2, cutoff) ----
  * Returns 0 and writes *out_u for integer results, 1 and writes *out_d for
