@@ -987,6 +987,97 @@ def workload_editops(args, emit=True):
     return line
 
 
+PRA_PAIRS = 65536
+
+
+def gen_pralign(rng, n=PRA_PAIRS):
+    """Needles of 8-128 units (log-uniform) over lowercase letters and spaces; haystacks of the
+    needle's length plus 0-512 units, three in four holding a copy of the needle with up to
+    one edit per eight units at a random place; every eighth pair in the swapped order."""
+    a_strs, b_strs = [], []
+    l1 = np.exp(rng.uniform(np.log(8), np.log(129), size=n)).astype(np.int64)
+    for i in range(n):
+        nd = rng.integers(96, 123, size=int(l1[i])).astype(np.uint8)
+        nd[nd == 96] = 32
+        extra = int(rng.integers(0, 513))
+        hy = rng.integers(96, 123, size=int(l1[i]) + extra).astype(np.uint8)
+        hy[hy == 96] = 32
+        if i % 4 != 3:
+            u = list(nd)
+            for _ in range(int(rng.integers(0, 1 + len(u) // 8))):
+                op = int(rng.integers(0, 3))
+                if op == 0 and u:
+                    u[int(rng.integers(0, len(u)))] = int(rng.integers(97, 123))
+                elif op == 1:
+                    u.insert(int(rng.integers(0, len(u) + 1)), int(rng.integers(97, 123)))
+                elif u:
+                    del u[int(rng.integers(0, len(u)))]
+            at = int(rng.integers(0, extra + 1))
+            hy = np.concatenate([hy[:at], np.array(u, dtype=np.uint8), hy[at:]])[:int(l1[i]) + extra]
+        if i % 8 == 5:
+            nd, hy = hy, nd
+        a_strs.append(nd)
+        b_strs.append(hy)
+    return a_strs, b_strs
+
+
+def workload_pralign(args, emit=True):
+    """fuzz::partial_ratio_alignment over 65,536 pairs (fuzz_impl.hpp:196-230): uint8 needles
+    of 8-128 units located in haystacks of up to 640 units.  The call hands back host arrays,
+    so the timed region is the whole rfa_partial_ratio_alignment call: the host's grouping of
+    the pairs, the item upload, the search kernels and the results' copy back."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(SEED + 13)
+    a_strs, b_strs = gen_pralign(rng)
+    ac, ao = pack_u8(a_strs)
+    bc, bo = pack_u8(b_strs)
+    ctx = rfa.Context(0)
+    A = rfa.Strings(ctx, chars=ac, offsets=ao, width=1)
+    B = rfa.Strings(ctx, chars=bc, offsets=bo, width=1)
+    got = rfa.partial_ratio_alignment(ctx, A, B, 0.0)
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        got = rfa.partial_ratio_alignment(ctx, A, B, 0.0)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    print(f"# partial_ratio_alignment: {ms:.3f} ms per call (host-inclusive), "
+          f"{PRA_PAIRS / (ms / 1e3) / 1e6:.3f} Mpairs/s, mean score {float(np.mean(got['score'])):.2f}",
+          file=sys.stderr, flush=True)
+    line = {"metric": "partial_ratio_alignment 65536 pairs, Mpairs/s",
+            "value": round(PRA_PAIRS / (ms / 1e3) / 1e6, 4), "unit": "Mpairs/s", "n_gpus": 1, "steps": steps,
+            "warmup": 1, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+            "config": {"workload": f"fuzz::partial_ratio_alignment of {PRA_PAIRS} pairs, uint8 lowercase and "
+                                   "spaces, needles 8-128 units, haystacks needle + 0-512 units (3 in 4 holding "
+                                   "a mutated copy); whole call (host-inclusive)"},
+            "roofline": None}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        if oracle_py.reference_available():
+            ref = oracle_py.Reference()
+            cores = _cores()
+            t0 = time.perf_counter()
+            sc, al = ref.partial_ratio_alignment_u8_mt(ac, ao, bc, bo, 0.0, threads=cores)
+            dt = time.perf_counter() - t0
+            mine = np.stack([got["src_start"], got["src_end"], got["dest_start"], got["dest_end"]], axis=1)
+            bad = int(np.count_nonzero((sc.view(np.uint64) != got["score"].view(np.uint64)) |
+                                       np.any(al != mine, axis=1)))
+            line["cpu_baseline"] = {"value": round(PRA_PAIRS / dt / 1e6, 4), "unit": "Mpairs/s", "cores": cores,
+                                    "kind": "reference",
+                                    "sample": f"all {PRA_PAIRS} pairs, rapidfuzz::fuzz::partial_ratio_alignment "
+                                              f"({dt:.2f} s)"}
+            line["check"] = ("scores and windows of every pair equal the reference" if bad == 0
+                             else f"{bad} MISMATCHES")
+    A.close()
+    B.close()
+    ctx.close()
+    if emit:
+        print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
+    return line
+
+
 def workload_tokens(args, emit=True):
     """token_sort_ratio, partial_token_sort_ratio, QRatio, token_set_ratio, token_ratio, the partial
     token ratios and WRatio cdist on the partial_ratio data (1024
@@ -1297,7 +1388,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="cdist",
                     choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "partial", "tokens", "wlev",
-                             "dl", "editops", "copy", "scorer"],
+                             "dl", "editops", "pralign", "copy", "scorer"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
                          "uneven: 1M independent pairs; serve: the serving leg alone; latency: 1 x 10,000 "
                          "Levenshtein; dna: 256 x 16,384 long sequences with band cutoffs; copy: HBM counter "
@@ -1347,6 +1438,8 @@ def main():
         return workload_dl(args)
     if args.workload == "editops":
         return workload_editops(args)
+    if args.workload == "pralign":
+        return workload_pralign(args)
 
     import rapidfuzz_amd as rfa
     from rapidfuzz_amd import hip
@@ -1557,6 +1650,10 @@ def main():
         res["configs"]["levenshtein_editops_262144"] = {k: edo.get(k) for k in ("metric", "value", "unit",
                                                                                  "ms_per_step", "cpu_baseline",
                                                                                  "check")}
+        pra = workload_pralign(args, emit=False)
+        res["configs"]["partial_ratio_alignment_65536"] = {k: pra.get(k) for k in ("metric", "value", "unit",
+                                                                                    "ms_per_step", "cpu_baseline",
+                                                                                    "check")}
         une = workload_uneven(args, emit=False)
         res["configs"]["uneven_1M_pairs"] = {**{k: une.get(k) for k in keep}, "plain": une["plain"],
                                              "speedup_vs_plain": une["speedup_vs_plain"],

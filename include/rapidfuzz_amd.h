@@ -283,6 +283,23 @@ typedef struct {
 rfa_status rfa_editops(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, int metric, uint64_t* offsets,
                        rfa_editop* ops, uint64_t capacity);
 
+/* ---- partial ratio alignment -------------------------------------------------
+ * rfa_partial_ratio_alignment: fuzz::partial_ratio_alignment(a[i], b[i], score_cutoff)
+ * (fuzz.hpp:129-135, fuzz_impl.hpp:196-230) of every pair: the partial ratio and the
+ * windows of a[i] (src) and b[i] (dest) it was found at, with the reference's choice
+ * among equal windows.  The layout is ScoreAlignment<double>'s (details/types.hpp:567-594).
+ * out: count entries in host memory.  A pair whose shorter string has more than 1024
+ * code units returns RFA_ERR_UNSUPPORTED. */
+typedef struct {
+    double score;
+    uint64_t src_start;
+    uint64_t src_end;
+    uint64_t dest_start;
+    uint64_t dest_end;
+} rfa_score_alignment;
+rfa_status rfa_partial_ratio_alignment(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b,
+                                       double score_cutoff, rfa_score_alignment* out);
+
 /* ---- request server (continuous batching) -------------------------------
  *
  * No reference counterpart: the reference leaves the extract() loop to the

@@ -270,6 +270,9 @@ class Reference:
                                           ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
         L.ref_editops_u8_mt.restype = ctypes.c_int
         L.ref_editops_u8_mt.argtypes = [ctypes.c_int, _u8p, _u64p, _u8p, _u64p, ctypes.c_size_t, _u64p, ctypes.c_int]
+        L.ref_partial_ratio_alignment_u8_mt.restype = ctypes.c_int
+        L.ref_partial_ratio_alignment_u8_mt.argtypes = [_u8p, _u64p, _u8p, _u64p, ctypes.c_size_t, ctypes.c_double,
+                                                        _f64p, _u64p, ctypes.c_int]
         L.ref_pairs_u32_mt.restype = ctypes.c_int
         L.ref_pairs_u32_mt.argtypes = [_i32p, _u32p, _u64p, _u32p, _u64p, ctypes.c_size_t, _u64p,
                                        ctypes.c_int]
@@ -381,6 +384,16 @@ class Reference:
         self.lib.ref_editops_u8_mt(metric, _p(achars, _u8p), _p(aoff, _u64p), _p(bchars, _u8p), _p(boff, _u64p), n,
                                    _p(out, _u64p), threads or os.cpu_count() or 1)
         return out
+
+    def partial_ratio_alignment_u8_mt(self, achars, aoff, bchars, boff, cutoff=0.0, threads=None):
+        """Threaded fuzz::partial_ratio_alignment over 8-bit pairs: (scores, (n, 4) windows)."""
+        n = len(aoff) - 1
+        score = np.zeros(n, dtype=np.float64)
+        al = np.zeros((n, 4), dtype=np.uint64)
+        self.lib.ref_partial_ratio_alignment_u8_mt(_p(achars, _u8p), _p(aoff, _u64p), _p(bchars, _u8p),
+                                                   _p(boff, _u64p), n, float(cutoff), _p(score, _f64p),
+                                                   _p(al, _u64p), threads or os.cpu_count() or 1)
+        return score, al
 
     def pairs_mt(self, metrics, achars, aoff, bchars, boff, threads=None):
         n = len(aoff) - 1

@@ -803,4 +803,30 @@ int ref_partial_ratio_alignment(const uint32_t* s1, size_t l1, const uint32_t* s
     }
 }
 
+// partial_ratio_alignment over 8-bit pairs on nthreads threads: score[i], al[4 i .. 4 i + 3]
+int ref_partial_ratio_alignment_u8_mt(const uint8_t* a, const uint64_t* aoff, const uint8_t* b, const uint64_t* boff,
+                                      size_t n, double cutoff, double* score, uint64_t* al, int nthreads)
+{
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+        for (;;) {
+            size_t i0 = next.fetch_add(64);
+            if (i0 >= n) break;
+            for (size_t i = i0; i < std::min(n, i0 + 64); ++i) {
+                std::basic_string<uint8_t> s1(a + aoff[i], a + aoff[i + 1]), s2(b + boff[i], b + boff[i + 1]);
+                const rapidfuzz::ScoreAlignment<double> r = rapidfuzz::fuzz::partial_ratio_alignment(s1, s2, cutoff);
+                score[i] = r.score;
+                al[4 * i] = r.src_start;
+                al[4 * i + 1] = r.src_end;
+                al[4 * i + 2] = r.dest_start;
+                al[4 * i + 3] = r.dest_end;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < std::max(1, nthreads); ++t) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    return 0;
+}
+
 } // extern "C"

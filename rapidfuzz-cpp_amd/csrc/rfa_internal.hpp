@@ -387,6 +387,35 @@ hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, bool w
                           hipStream_t s);
 hipError_t launch_editops_hamming(int char_width, const EditopsArgs& a, bool walk, uint32_t nblocks, hipStream_t s);
 
+// fuzz::partial_ratio_alignment of pairs (rfa_pralign.hip): one wave per pair whose
+// shorter string (the needle) has 1..PRALIGN_MAX units.  Per item a slice of scratch
+// (pralign_scratch_bytes) holds the needle's match masks over the haystack, the LCS of
+// every candidate window and the search's window lists.
+constexpr uint32_t PRALIGN_MAX = 1024;
+struct PrAlignArgs {
+    const void* achars;
+    const uint64_t* aoff;
+    const void* bchars;
+    const uint64_t* boff;
+    const uint32_t* items;       // pair of each item
+    const uint64_t* scratch_off; // byte offset of each item's scratch slice
+    uint8_t* scratch;
+    uint32_t nitems;
+    double cutoff;
+    double* score;               // per pair
+    uint64_t* al;                // per pair: src_start, src_end, dest_start, dest_end
+};
+__host__ __device__ inline uint64_t pralign_scratch_bytes(uint64_t l1, uint64_t l2)
+{
+    const uint64_t w = (l1 + 63) / 64, f = l2 - l1;
+    const uint64_t xtab = w * (l2 + (l1 == l2 ? l1 : 0)) * 8;
+    const uint64_t lcs = ((f + 2 * l1) * 4 + 7) / 8 * 8;
+    const uint64_t lists = (2 * f + 2) * 2 * 8;
+    return xtab + lcs + lists;
+}
+// words: 1, 4 or 16 (the needle's 64-unit words, rounded up)
+hipError_t launch_pralign(int char_width, uint32_t words, const PrAlignArgs& a, hipStream_t s);
+
 // Widest shorter string (in 64-unit words) the long path takes.
 constexpr uint32_t PARTIAL_MAX_WORDS = 16;
 hipError_t launch_plong(int char_width, uint32_t needle_words, const PartialLongArgs& a, uint32_t nblocks,

@@ -733,6 +733,25 @@ RFA_FREE_DOUBLE(jaro_winkler, normalized_similarity, 0.0, CachedJaroWinkler, RFA
 #undef RFA_COMMA_PW_DECL
 #undef RFA_COMMA_PW
 
+// ---- the windows of a partial match (details/types.hpp:567-592) ----
+template <typename T>
+struct ScoreAlignment {
+    T score = T();
+    size_t src_start = 0; // window of the first string
+    size_t src_end = 0;
+    size_t dest_start = 0; // window of the second string
+    size_t dest_end = 0;
+    ScoreAlignment() = default;
+    ScoreAlignment(T s, size_t ss, size_t se, size_t ds, size_t de)
+        : score(s), src_start(ss), src_end(se), dest_start(ds), dest_end(de)
+    {}
+    friend bool operator==(const ScoreAlignment& a, const ScoreAlignment& b)
+    {
+        return a.score == b.score && a.src_start == b.src_start && a.src_end == b.src_end &&
+               a.dest_start == b.dest_start && a.dest_end == b.dest_end;
+    }
+};
+
 // ---- edit operations (details/types.hpp:28-60, 167-355: the parts a caller reads) ----
 enum class EditType { None = RFA_EDIT_NONE, Replace = RFA_EDIT_REPLACE, Insert = RFA_EDIT_INSERT, Delete = RFA_EDIT_DELETE };
 
@@ -1377,6 +1396,31 @@ RFA_FREE_FUZZ(partial_token_ratio, CachedPartialTokenRatio)
 RFA_FREE_FUZZ(QRatio, CachedQRatio)
 RFA_FREE_FUZZ(WRatio, CachedWRatio)
 #undef RFA_FREE_FUZZ
+
+// partial_ratio_alignment (fuzz.hpp:128-134): the best partial ratio and the windows of both
+// strings it was found at (src: the first string); the shorter string may hold at most 1024
+// code units
+template <typename InputIt1, typename InputIt2>
+ScoreAlignment<double> partial_ratio_alignment(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2,
+                                               double score_cutoff = 0)
+{
+    rapidfuzz_amd::detail::Packed a, b;
+    a.add(first1, last1);
+    b.add(first2, last2);
+    const int w = std::max(a.width, b.width);
+    rfa_context* ctx = rapidfuzz_amd::detail::context();
+    rapidfuzz_amd::detail::DeviceStrings da(ctx, a, w), db(ctx, b, w);
+    rfa_score_alignment r{};
+    rapidfuzz_amd::detail::check(rfa_partial_ratio_alignment(ctx, da.get(), db.get(), score_cutoff, &r));
+    return ScoreAlignment<double>(r.score, r.src_start, r.src_end, r.dest_start, r.dest_end);
+}
+
+template <typename Sentence1, typename Sentence2>
+ScoreAlignment<double> partial_ratio_alignment(const Sentence1& s1, const Sentence2& s2, double score_cutoff = 0)
+{
+    return partial_ratio_alignment(detail::to_begin(s1), detail::to_end(s1), detail::to_begin(s2), detail::to_end(s2),
+                                   score_cutoff);
+}
 
 } // namespace fuzz
 

@@ -33,7 +33,7 @@ EXPORTED = ["rfa_abi_version", "rfa_last_error", "rfa_context_create", "rfa_cont
             "rfa_context_synchronize", "rfa_context_stream", "rfa_context_set_option", "rfa_context_counter", "rfa_strings_create", "rfa_strings_destroy",
             "rfa_strings_count", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_scorer_set_option", "rfa_cdist", "rfa_cdist_cutoffs",
             "rfa_pairs", "rfa_pairs_noalloc", "rfa_pairs_autotune", "rfa_cdist_autotune", "rfa_topk", "rfa_editops",
-            "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
+            "rfa_partial_ratio_alignment", "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait", "rfa_server_replay",
             "rfa_server_get_stats"]
 
 
@@ -102,6 +102,7 @@ def load_library(path: str = LIB_PATH):
     L.rfa_cdist_autotune.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_uint64, _vp, _vp]
     L.rfa_topk.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _vp, _vp]
     L.rfa_editops.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64]
+    L.rfa_partial_ratio_alignment.argtypes = [_vp, _vp, _vp, ctypes.c_double, _vp]
     L.rfa_server_create.argtypes = [_vp, _vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
     L.rfa_server_destroy.argtypes = [_vp]
     L.rfa_server_submit.argtypes = [_vp, _vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
@@ -112,7 +113,7 @@ def load_library(path: str = LIB_PATH):
                  "rfa_strings_destroy", "rfa_scorer_create", "rfa_scorer_destroy", "rfa_cdist", "rfa_cdist_cutoffs",
                  "rfa_pairs", "rfa_pairs_autotune", "rfa_context_set_option", "rfa_scorer_set_option",
                  "rfa_topk", "rfa_server_create", "rfa_server_destroy", "rfa_server_submit", "rfa_server_wait",
-                 "rfa_server_replay", "rfa_server_get_stats", "rfa_editops"):
+                 "rfa_server_replay", "rfa_server_get_stats", "rfa_editops", "rfa_partial_ratio_alignment"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -394,6 +395,18 @@ def editops(ctx: Context, a: Strings, b: Strings, metric=LEVENSHTEIN):
     ops = np.empty(cap, dtype=EDITOP_DTYPE)
     _check(_lib.rfa_editops(ctx.h, a.h, b.h, metric, off.ctypes.data, ops.ctypes.data, cap))
     return off, ops[:int(off[-1])]
+
+
+ALIGNMENT_DTYPE = np.dtype([("score", np.float64), ("src_start", np.uint64), ("src_end", np.uint64),
+                            ("dest_start", np.uint64), ("dest_end", np.uint64)])
+
+
+def partial_ratio_alignment(ctx: Context, a: Strings, b: Strings, score_cutoff=0.0):
+    """fuzz::partial_ratio_alignment(a[i], b[i], score_cutoff) of every pair, as a structured
+    array (score, src_start, src_end, dest_start, dest_end)."""
+    out = np.zeros(a.count, dtype=ALIGNMENT_DTYPE)
+    _check(_lib.rfa_partial_ratio_alignment(ctx.h, a.h, b.h, float(score_cutoff), out.ctypes.data))
+    return out
 
 
 def pairs(ctx: Context, a: Strings, b: Strings, metrics, kind=DISTANCE, cutoff_u=INT64_MAX, cutoff_d=1.0,

@@ -122,6 +122,22 @@ int main()
             CHECK(throws_invalid([&] { rfa::hamming_editops(s, d, false); }));
         }
         {
+            // tests-fuzz.cpp:187-200 (issue 231) and the swap of the arguments
+            const std::string s1 = "er merkantilismus f/rderte handel und verkehr mit teils marktkonformen, teils "
+                                   "dirigistischen ma_nahmen.";
+            const std::string s2 = "ils marktkonformen, teils dirigistischen ma_nahmen. an der schwelle zum 19. "
+                                   "jahrhundert entstand ein neu";
+            const auto al = rfa::fuzz::partial_ratio_alignment(s1, s2);
+            CHECK(std::fabs(al.score - 66.2337662) < 1e-6);
+            CHECK(al.src_start == 0 && al.src_end == 103 && al.dest_start == 0 && al.dest_end == 51);
+            const auto sw = rfa::fuzz::partial_ratio_alignment(s2, s1);
+            CHECK(sw == rfa::ScoreAlignment<double>(al.score, 0, 51, 0, 103));
+            CHECK(al.score == rfa::fuzz::partial_ratio(s1, s2));
+            CHECK(rfa::fuzz::partial_ratio_alignment(std::string(""), std::string("")) ==
+                  rfa::ScoreAlignment<double>(100.0, 0, 0, 0, 0));
+            CHECK(rfa::fuzz::partial_ratio_alignment(s1, s2, 101.0) == rfa::ScoreAlignment<double>(0.0, 0, 103, 0, 103));
+        }
+        {
             // tests-DamerauLevenshtein.cpp:106-130
             rfa::experimental::CachedDamerauLevenshtein dl(std::string("abaa"));
             CHECK(dl.distance(std::string("baaa")) == 1);
