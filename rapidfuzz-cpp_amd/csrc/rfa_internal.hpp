@@ -388,32 +388,42 @@ hipError_t launch_editops(int char_width, const EditopsArgs& a, bool lev, bool w
 hipError_t launch_editops_hamming(int char_width, const EditopsArgs& a, bool walk, uint32_t nblocks, hipStream_t s);
 
 // fuzz::partial_ratio_alignment of pairs (rfa_pralign.hip): one wave per pair whose
-// shorter string (the needle) has 1..PRALIGN_MAX units.  Per item a slice of scratch
-// (pralign_scratch_bytes) holds the needle's match masks over the haystack, the LCS of
-// every candidate window and the search's window lists.
+// shorter string (the needle, l1 units) has 1..PRALIGN_MAX units.  Each search works in
+// a workspace (pralign_ws_bytes): the needle's match masks over the haystack (l2 units)
+// and over the reversed last l1 units and the LCS of every candidate window, with the
+// 8-bit symbol tables of short needles in the space the LCS values take later.  A workspace lives in LDS when it fits the launch's
+// per-wave LDS size, else in the item's slice of scratch.
 constexpr uint32_t PRALIGN_MAX = 1024;
+constexpr uint32_t PRALIGN_WAVES = 1;       // waves (pairs) per workgroup: LDS is taken per wave
+constexpr uint32_t PRALIGN_LDS_CAP = 16384; // per-wave LDS, bytes
 struct PrAlignArgs {
     const void* achars;
     const uint64_t* aoff;
     const void* bchars;
     const uint64_t* boff;
     const uint32_t* items;       // pair of each item
-    const uint64_t* scratch_off; // byte offset of each item's scratch slice
+    const uint64_t* scratch_off; // byte offset of each item's scratch slice (items outside LDS)
     uint8_t* scratch;
     uint32_t nitems;
+    uint32_t lds_bytes;          // per-wave LDS of the launch (multiple of 8, at most PRALIGN_LDS_CAP)
     double cutoff;
-    double* score;               // per pair
-    uint64_t* al;                // per pair: src_start, src_end, dest_start, dest_end
+    uint64_t* out;               // per pair, rfa_score_alignment: score bits, src_start, src_end,
+                                 // dest_start, dest_end
 };
-__host__ __device__ inline uint64_t pralign_scratch_bytes(uint64_t l1, uint64_t l2)
+// 8-bit needles of at most two words build their masks from per-symbol tables (in LDS only)
+__host__ __device__ inline bool pralign_symbol_tables(int char_width, uint64_t l1)
+{
+    return char_width == 1 && l1 <= 128;
+}
+__host__ __device__ inline uint64_t pralign_ws_bytes(uint64_t l1, uint64_t l2, bool tables)
 {
     const uint64_t w = (l1 + 63) / 64, f = l2 - l1;
-    const uint64_t xtab = w * (l2 + (l1 == l2 ? l1 : 0)) * 8;
-    const uint64_t lcs = ((f + 2 * l1) * 4 + 7) / 8 * 8;
-    const uint64_t lists = (2 * f + 2) * 2 * 8;
-    return xtab + lcs + lists;
+    const uint64_t masks = 8 * w * (l2 + l1);
+    const uint64_t rest = (4 * (f + 2 * l1) + 7) / 8 * 8;
+    const uint64_t sym = tables ? 256 * 8 * w : 0;
+    return masks + (rest > sym ? rest : sym);
 }
-// words: 1, 4 or 16 (the needle's 64-unit words, rounded up)
+// words: 1, 2, 4, 8 or 16 (the needle's 64-unit words, rounded up)
 hipError_t launch_pralign(int char_width, uint32_t words, const PrAlignArgs& a, hipStream_t s);
 
 // Widest shorter string (in 64-unit words) the long path takes.

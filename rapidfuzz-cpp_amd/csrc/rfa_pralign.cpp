@@ -1,9 +1,11 @@
 // rfa_pralign.cpp -- rfa_partial_ratio_alignment: fuzz::partial_ratio_alignment for
 // batches of pairs (fuzz_impl.hpp:196-230).  The host settles the pairs the reference
 // answers before searching (a cutoff above 100, an empty string), groups the rest by
-// the needle's word count and gives each pair a slice of scratch; the device searches
-// (rfa_pralign.hip).
+// the needle's word count, sizes each class's per-wave LDS and gives the pairs whose
+// workspace exceeds it a slice of scratch; the device searches (rfa_pralign.hip) and
+// writes the results in the caller's layout, copied back in one piece.
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "rfa_host.hpp"
@@ -13,6 +15,8 @@ using namespace rfa;
 
 namespace {
 
+static_assert(sizeof(rfa_score_alignment) == 40, "rfa_score_alignment is five 8-byte fields");
+
 void pralign_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, double cutoff,
                   rfa_score_alignment* out)
 {
@@ -21,72 +25,88 @@ void pralign_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
     const uint64_t n = a->count;
     if (n == 0) return;
     if (n > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 pairs");
-    // items by class (needle words 1, 2-4, 5-16), each with its scratch slice
-    std::vector<uint32_t> items[3];
-    std::vector<uint64_t> offs[3];
-    uint64_t total = 0;
+    // classes by needle words: 1, 2, 3-4, 5-8, 9-16
+    constexpr int NC = 5;
+    std::vector<uint8_t> cls(n);
+    std::vector<uint32_t> wsb(n); // workspace bytes with symbol tables (LDS candidates)
+    uint64_t cnt[NC] = {0, 0, 0, 0, 0};
+    uint32_t lds[NC] = {0, 0, 0, 0, 0};
+    std::vector<uint64_t> trivial;
     for (uint64_t i = 0; i < n; ++i) {
         const uint64_t la = a->len(i), lb = b->len(i);
         const uint64_t l1 = std::min(la, lb), l2 = std::max(la, lb);
-        rfa_score_alignment& r = out[i];
-        // partial_ratio_alignment: the swap for a longer first string leaves these as they are
-        if (cutoff > 100) {
-            r = rfa_score_alignment{0.0, 0, l1, 0, l1};
-            continue;
-        }
-        if (l1 == 0) {
-            r = rfa_score_alignment{la == lb ? 100.0 : 0.0, 0, 0, 0, 0};
+        if (cutoff > 100 || l1 == 0) { // settled here (the swap for a longer first string keeps them)
+            cls[i] = 0xff;
+            trivial.push_back(i);
             continue;
         }
         if (l1 > PRALIGN_MAX)
             throw ApiError(RFA_ERR_UNSUPPORTED, "partial_ratio_alignment of two strings longer than 1024 code units");
         const uint64_t w = (l1 + 63) / 64;
-        const int c = w <= 1 ? 0 : (w <= 4 ? 1 : 2);
-        items[c].push_back((uint32_t)i);
-        offs[c].push_back(total);
-        total += pralign_scratch_bytes(l1, l2);
+        const int c = w <= 1 ? 0 : w <= 2 ? 1 : w <= 4 ? 2 : w <= 8 ? 3 : 4;
+        cls[i] = (uint8_t)c;
+        ++cnt[c];
+        const uint64_t ws = pralign_ws_bytes(l1, l2, pralign_symbol_tables((int)a->width, l1));
+        wsb[i] = (uint32_t)std::min<uint64_t>(ws, 0xffffffffull);
+        // wider needles never fit the LDS cap
+        if (c <= 2 && ws <= PRALIGN_LDS_CAP) lds[c] = std::max(lds[c], (uint32_t)ws);
     }
-    uint64_t nitems = items[0].size() + items[1].size() + items[2].size();
-    if (nitems == 0) return;
+    const uint64_t nitems = n - trivial.size();
     hipStream_t s = ctx->stream;
-    // device: items + offsets of every class, scores and windows of every pair, scratch
-    std::vector<uint32_t> all_items;
-    std::vector<uint64_t> all_offs;
-    for (int c = 0; c < 3; ++c) {
-        all_items.insert(all_items.end(), items[c].begin(), items[c].end());
-        all_offs.insert(all_offs.end(), offs[c].begin(), offs[c].end());
+    if (nitems) {
+        // items (class-major) and scratch offsets in one pinned upload; the device takes LDS
+        // exactly when the workspace fits the class's size (the same test)
+        uint64_t first[NC + 1] = {0};
+        for (int c = 0; c < NC; ++c) first[c + 1] = first[c] + cnt[c];
+        const size_t o_offs = (nitems * 4 + 7) / 8 * 8, up = o_offs + nitems * 8;
+        int slot = 0;
+        uint8_t* h = (uint8_t*)ctx->staging.acquire(up, &slot);
+        uint32_t* hitems = (uint32_t*)h;
+        uint64_t* hoffs = (uint64_t*)(h + o_offs);
+        uint64_t pos[NC];
+        for (int c = 0; c < NC; ++c) pos[c] = first[c];
+        uint64_t total = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            const int c = cls[i];
+            if (c == 0xff) continue;
+            const uint64_t k = pos[c]++;
+            hitems[k] = (uint32_t)i;
+            hoffs[k] = total;
+            if (wsb[i] > lds[c]) {
+                const uint64_t la = a->len(i), lb = b->len(i);
+                total += pralign_ws_bytes(std::min(la, lb), std::max(la, lb), false);
+            }
+        }
+        const size_t o_out = (up + 255) / 256 * 256;
+        uint8_t* d = (uint8_t*)ctx->ed_items.get(o_out + n * 40);
+        ctx->staging.copy(slot, d, up, s);
+        PrAlignArgs pa{};
+        pa.achars = a->d_chars;
+        pa.aoff = a->d_off;
+        pa.bchars = b->d_chars;
+        pa.boff = b->d_off;
+        pa.scratch = (uint8_t*)ctx->ed_rows.get(std::max<uint64_t>(total, 8));
+        pa.cutoff = cutoff;
+        pa.out = (uint64_t*)(d + o_out);
+        const uint32_t words[NC] = {1, 2, 4, 8, 16};
+        for (int c = 0; c < NC; ++c) {
+            pa.items = (const uint32_t*)d + first[c];
+            pa.scratch_off = (const uint64_t*)(d + o_offs) + first[c];
+            pa.nitems = (uint32_t)cnt[c];
+            pa.lds_bytes = lds[c];
+            HIP_CHECK(launch_pralign(a->width, words[c], pa, s));
+        }
+        HIP_CHECK(hipMemcpyAsync(out, pa.out, n * 40, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
     }
-    const size_t o_items = 0, o_offs = (nitems * 4 + 7) / 8 * 8, o_score = o_offs + nitems * 8,
-                 o_al = o_score + n * 8, o_end = o_al + n * 32;
-    uint8_t* d = (uint8_t*)ctx->ed_items.get(o_end);
-    HIP_CHECK(hipMemcpyAsync(d + o_items, all_items.data(), nitems * 4, hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(d + o_offs, all_offs.data(), nitems * 8, hipMemcpyHostToDevice, s));
-    PrAlignArgs pa{};
-    pa.achars = a->d_chars;
-    pa.aoff = a->d_off;
-    pa.bchars = b->d_chars;
-    pa.boff = b->d_off;
-    pa.scratch = (uint8_t*)ctx->ed_rows.get(std::max<uint64_t>(total, 8));
-    pa.cutoff = cutoff;
-    pa.score = (double*)(d + o_score);
-    pa.al = (uint64_t*)(d + o_al);
-    uint64_t first = 0;
-    const uint32_t words[3] = {1, 4, 16};
-    for (int c = 0; c < 3; ++c) {
-        pa.items = (const uint32_t*)(d + o_items) + first;
-        pa.scratch_off = (const uint64_t*)(d + o_offs) + first;
-        pa.nitems = (uint32_t)items[c].size();
-        HIP_CHECK(launch_pralign(a->width, words[c], pa, s));
-        first += items[c].size();
+    for (uint64_t i : trivial) {
+        const uint64_t la = a->len(i), lb = b->len(i);
+        const uint64_t l1 = std::min(la, lb);
+        if (cutoff > 100)
+            out[i] = rfa_score_alignment{0.0, 0, l1, 0, l1};
+        else
+            out[i] = rfa_score_alignment{la == lb ? 100.0 : 0.0, 0, 0, 0, 0};
     }
-    std::vector<double> score(n);
-    std::vector<uint64_t> al(4 * n);
-    HIP_CHECK(hipMemcpyAsync(score.data(), pa.score, n * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(al.data(), pa.al, n * 32, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    for (int c = 0; c < 3; ++c)
-        for (uint32_t i : items[c])
-            out[i] = rfa_score_alignment{score[i], al[4 * i], al[4 * i + 1], al[4 * i + 2], al[4 * i + 3]};
 }
 
 } // namespace

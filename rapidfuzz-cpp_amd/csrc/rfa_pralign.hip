@@ -2,25 +2,39 @@
 // fuzz_impl.hpp:68-230): the best partial ratio and the windows it was found at.
 //
 // One wave per pair.  The needle N (the shorter string, l1 units; the first string
-// when the lengths are equal) sits one unit per lane per 64-unit word.  For every
-// haystack position t one ballot per word gives the positions of N holding H[t]
-// (X[k][t]); the table goes to the pair's scratch, word-major, so both its writes
-// and the lanes' later reads are coalesced.  Every candidate window of H -- the
-// full windows [p, p + l1) for p < l2 - l1, the prefixes [0, i) for 0 < i < l1 and
-// the suffixes [i, l2) for l2 - l1 <= i < l2 -- gets its LCS with N from the
-// bit-parallel LCS recurrence (Hyyro 2004), one window per lane.
+// when the lengths are equal) is searched in the haystack H (l2 units, F = l2 - l1).
+// The wave works in a workspace (LDS when it fits the launch's per-wave size, else
+// the pair's slice of scratch):
+//  * match masks: X[k][t], the positions of N's word k holding H[t], for every t, and
+//    R[k][j], the same for the reversed needle against H[l2 - 1 - j], j < l1.  8-bit
+//    needles of at most two words read them from per-symbol tables (R bit-reversed);
+//    the others take one ballot per word per haystack unit;
+//  * the LCS with N of every candidate window, from the bit-parallel LCS recurrence
+//    (Hyyro 2004), one window per lane: the full windows [p, p + l1), p < F, each
+//    l1 steps; all prefixes [0, i) in one pass of one lane over H (the recurrence's
+//    state after i units is the prefix's LCS) and all suffixes [i, l2) in one pass of
+//    another lane over the reversed strings;
+//  * the reference's choice among them.
 //
-// Then the reference's choice among them:
-//  * full windows: its bisection of [0, l2 - l1 - 1] (the bound on the distances
-//    between two evaluated windows, the running cutoff), replayed by lane 0 on the
-//    stored LCS values with window lists in scratch: the result is the first window
-//    visited whose distance beats the cutoff at the time;
-//  * prefixes, then suffixes: a window replaces the result when its ratio (0 below
-//    the running cutoff) is strictly larger, so the phase keeps the first window of
-//    the largest ratio among those whose boundary unit occurs in N and whose ratio
-//    reaches the cutoff -- one max / first-index reduction over the lanes;
-//  * equal lengths: the same search with the strings' roles exchanged, kept when
-//    strictly better (its windows lie in the first string).
+// Full windows: the reference bisects [0, F - 1] level by level, visiting each range
+// at both ends and splitting it (at the floor of its middle) while the bound
+// min(d_f, d_sc) - improvement on the distances inside can beat the running cutoff.
+// Window distances are even and change by at most 2 per unit of shift, so the bound
+// never exceeds a distance inside: every window below the running cutoff lies in a
+// range that is split, the search reaches the smallest distance g over all full
+// windows (when g beats the entry cutoff), and until it does every range around a
+// window of distance g is split.  The first window of distance g it visits is
+// therefore the one that first becomes a range end in the complete halving tree of
+// [0, F - 1] -- the smallest depth, then the smallest position (a level's ends are
+// visited left to right).  The kernel takes g from a reduction of the windows' LCS
+// and that window from a depth computation over the windows of distance g.
+//
+// Prefixes, then suffixes: a window replaces the result when its ratio (0 below the
+// running cutoff) is strictly larger, so a phase keeps the first window of the
+// largest ratio among those whose boundary unit occurs in N and whose ratio reaches
+// the cutoff -- one max / first-index reduction over the lanes.
+// Equal lengths: the same search with the strings' roles exchanged, kept when
+// strictly better (its windows lie in the first string).
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
 #include "rfa_semantics.hpp"
@@ -41,176 +55,260 @@ struct PrResult {
     uint32_t ds, de; // the window of the haystack
 };
 
-// One search (fuzz_detail::partial_ratio_impl, fuzz_impl.hpp:68-179) of needle nd
-// (l1 <= WM * 64 units) in haystack hy (l2 >= l1 units).  xtab: WM * l2 words;
-// lcs: l2 - l1 + 2 * l1 entries; lists: 2 * (2 * (l2 - l1) + 2) windows.  The result
-// is uniform across the wave.
-template <uint32_t WM, typename CharT>
-__device__ PrResult pr_search(const CharT* __restrict__ nd, uint32_t l1, const CharT* __restrict__ hy, uint32_t l2,
-                              double cutoff, uint64_t* __restrict__ xtab, uint32_t* __restrict__ lcs,
-                              uint2* __restrict__ lists)
+// The needle's match masks over the haystack: xtab[k * l2 + t] and, for the reversed
+// needle against the haystack read backwards, xrev[k * l1 + j].
+template <uint32_t WM, typename CharT, bool LDS>
+__device__ __forceinline__ void pr_masks(const CharT* __restrict__ nd, uint32_t l1, const CharT* __restrict__ hy,
+                                         uint32_t l2, uint32_t W, bool tables, uint64_t* xtab, uint64_t* xrev,
+                                         uint64_t* sym)
 {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t W = (l1 + 63) / 64;
-    // ---- the needle's match masks over the haystack ----
-    uint32_t nu[WM];
+    if constexpr (LDS && WM <= 4 && sizeof(CharT) == 1) {
+        if (tables) { // W <= 2
+            // per-symbol masks of the needle: sym[c * W + k]
+            for (uint32_t i = lane; i < 256 * W; i += 64) sym[i] = 0;
+            __threadfence_block();
+            for (uint32_t k = 0; k < W; ++k) {
+                const uint32_t i = 64 * k + lane;
+                if (i < l1) atomicOr((unsigned long long*)&sym[(uint32_t)nd[i] * W + k], 1ull << lane);
+            }
+            __threadfence_block();
+            // 512 haystack units per round, their loads issued together
+            for (uint32_t t0 = 0; t0 < l2; t0 += 512) {
+                uint32_t c[8];
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q) {
+                    const uint32_t t = t0 + 64 * q + lane;
+                    c[q] = t < l2 ? (uint32_t)hy[t] : 0u;
+                }
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q) {
+                    const uint32_t t = t0 + 64 * q + lane;
+                    if (t < l2)
+                        for (uint32_t k = 0; k < W; ++k) xtab[(uint64_t)k * l2 + t] = sym[c[q] * W + k];
+                }
+            }
+            // the reversed needle's masks: the needle's, bit-reversed within l1 bits
+            for (uint32_t j = lane; j < l1; j += 64) {
+                const uint32_t c = (uint32_t)hy[l2 - 1 - j];
+                if (W == 1) {
+                    xrev[j] = __builtin_bitreverse64(sym[c]) >> (64 - l1);
+                }
+                else {
+                    const uint64_t hr = __builtin_bitreverse64(sym[2 * c]), lr = __builtin_bitreverse64(sym[2 * c + 1]);
+                    const uint32_t sh = 128 - l1; // 0..63
+                    xrev[j] = sh ? (lr >> sh) | (hr << (64 - sh)) : lr;
+                    xrev[l1 + j] = hr >> sh;
+                }
+            }
+            __threadfence_block();
+            return;
+        }
+    }
+    uint32_t nu[WM], nr[WM];
     bool nv[WM];
 #pragma unroll
     for (uint32_t k = 0; k < WM; ++k) {
-        nv[k] = k < W && 64 * k + lane < l1;
-        nu[k] = nv[k] ? (uint32_t)nd[64 * k + lane] : 0u;
+        const uint32_t i = 64 * k + lane;
+        nv[k] = k < W && i < l1;
+        nu[k] = nv[k] ? (uint32_t)nd[i] : 0u;
+        nr[k] = nv[k] ? (uint32_t)nd[l1 - 1 - i] : 0u;
     }
-    for (uint32_t t0 = 0; t0 < l2; t0 += 64) {
-        const uint32_t cnt = min(64u, l2 - t0);
-        const uint32_t h = lane < cnt ? (uint32_t)hy[t0 + lane] : 0u;
-        uint64_t acc[WM];
+    for (int dir = 0; dir < 2; ++dir) {
+        const uint32_t n = dir == 0 ? l2 : l1;
+        uint64_t* out = dir == 0 ? xtab : xrev;
+        for (uint32_t t0 = 0; t0 < n; t0 += 64) {
+            const uint32_t cnt = min(64u, n - t0);
+            const uint32_t h = lane < cnt ? (uint32_t)hy[dir == 0 ? t0 + lane : l2 - 1 - (t0 + lane)] : 0u;
+            uint64_t acc[WM];
 #pragma unroll
-        for (uint32_t k = 0; k < WM; ++k) acc[k] = 0;
-        for (uint32_t i = 0; i < cnt; ++i) {
-            const uint32_t c = readlane(h, i);
+            for (uint32_t k = 0; k < WM; ++k) acc[k] = 0;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const uint32_t c = readlane(h, i);
 #pragma unroll
-            for (uint32_t k = 0; k < WM; ++k) {
-                const uint64_t m = ballot(nv[k] && nu[k] == c);
-                acc[k] = lane == i ? m : acc[k];
+                for (uint32_t k = 0; k < WM; ++k) {
+                    if (k < W) {
+                        const uint64_t m = ballot(nv[k] && (dir == 0 ? nu[k] : nr[k]) == c);
+                        acc[k] = lane == i ? m : acc[k];
+                    }
+                }
             }
-        }
-        if (lane < cnt) {
+            if (lane < cnt) {
 #pragma unroll
-            for (uint32_t k = 0; k < WM; ++k)
-                if (k < W) xtab[(uint64_t)k * l2 + t0 + lane] = acc[k];
+                for (uint32_t k = 0; k < WM; ++k)
+                    if (k < W) out[(uint64_t)k * n + t0 + lane] = acc[k];
+            }
         }
     }
     __threadfence_block();
+}
 
-    // ---- LCS of every candidate window, one per lane ----
-    const uint32_t F = l2 - l1; // full windows [p, p + l1), p < F
-    const uint32_t nwin = F + 2 * l1 - 1;
+// l1 steps of the LCS recurrence over p[k * stride + j] (every lane steps; the lanes
+// past their windows run on in-bounds words and are not read); a recorder lane stores
+// the LCS after each of its first len steps at lcs[recbase + recdir * j].  The next
+// step's words are loaded before the current step's arithmetic.
+template <uint32_t WM, bool REC>
+__device__ __forceinline__ void pr_lcs_run(const uint64_t* p, uint32_t stride, uint32_t len, uint32_t l1, uint32_t W,
+                                           const uint64_t (&lastm)[WM], uint64_t (&S)[WM], uint32_t* lcs,
+                                           int recbase, int recdir, bool isrec)
+{
+    uint64_t Xn[WM];
+#pragma unroll
+    for (uint32_t k = 0; k < WM; ++k) Xn[k] = k < W ? p[(uint64_t)k * stride] : 0ull;
+    for (uint32_t j = 0; j < l1; ++j) {
+        uint64_t X[WM];
+#pragma unroll
+        for (uint32_t k = 0; k < WM; ++k) X[k] = Xn[k];
+        const uint32_t jn = j + 1 < l1 ? j + 1 : j;
+#pragma unroll
+        for (uint32_t k = 0; k < WM; ++k)
+            if (k < W) Xn[k] = p[(uint64_t)k * stride + jn];
+        unsigned long long carry = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < WM; ++k) {
+            if (k < W) {
+                const uint64_t u = S[k] & X[k];
+                unsigned long long co;
+                const uint64_t x = __builtin_addcll(S[k], u, carry, &co);
+                carry = co;
+                S[k] = x | (S[k] & ~u);
+            }
+        }
+        if constexpr (REC) {
+            if (isrec && j < len) {
+                uint32_t v = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < WM; ++k)
+                    if (k < W) v += (uint32_t)__popcll(~S[k] & lastm[k]);
+                lcs[recbase + recdir * (int)j] = v;
+            }
+        }
+    }
+}
+
+// One search (fuzz_detail::partial_ratio_impl, fuzz_impl.hpp:68-179) of needle nd
+// (l1 <= WM * 64 units) in haystack hy (l2 >= l1 units) in workspace ws
+// (pralign_ws_bytes(l1, l2, tables) bytes).  The result is uniform across the wave.
+template <uint32_t WM, typename CharT, bool LDS>
+__device__ __forceinline__ PrResult pr_search(const CharT* __restrict__ nd, uint32_t l1, const CharT* __restrict__ hy,
+                                              uint32_t l2, double cutoff, uint64_t* ws, bool tables)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t W = WM <= 2 ? WM : (l1 + 63) / 64; // the classes of 1 and 2 words are exact
+    const uint32_t F = l2 - l1;
+    uint64_t* xtab = ws;
+    uint64_t* xrev = ws + (uint64_t)W * l2;
+    uint64_t* rest = xrev + (uint64_t)W * l1;
+    uint32_t* lcs = (uint32_t*)rest; // F full windows, l1 - 1 prefixes, l1 suffixes
+
+    pr_masks<WM, CharT, LDS>(nd, l1, hy, l2, W, tables, xtab, xrev, rest);
+
+    // ---- LCS of every candidate window ----
     uint64_t lastm[WM]; // the needle's bits of each word
 #pragma unroll
     for (uint32_t k = 0; k < WM; ++k) {
         const uint32_t lo = 64 * k;
         lastm[k] = k >= W ? 0ull : (l1 - lo >= 64 ? ~0ull : ((1ull << (l1 - lo)) - 1));
     }
-    for (uint32_t r0 = 0; r0 < nwin; r0 += 64) {
+    uint32_t gmax = 0; // the lane's largest LCS of a full window
+    for (uint32_t r0 = 0; r0 < F + 2; r0 += 64) {
         const uint32_t r = r0 + lane;
-        uint32_t s = 0, e = 0;
+        const uint64_t* p = xtab;
+        uint32_t stride = l2, len = 0;
+        int recbase = 0, recdir = 0;
+        bool isrec = false;
         if (r < F) {
-            s = r;
-            e = r + l1;
+            p = xtab + r;
+            len = l1;
         }
-        else if (r < F + l1 - 1) {
-            e = r - F + 1; // prefix [0, e)
+        else if (r == F) { // prefixes [0, j + 1)
+            len = l1 - 1;
+            isrec = true;
+            recbase = (int)F;
+            recdir = 1;
         }
-        else if (r < nwin) {
-            s = (r - F - (l1 - 1)) + F; // suffix [s, l2)
-            e = l2;
+        else if (r == F + 1) { // suffixes [l2 - 1 - j, l2)
+            p = xrev;
+            stride = l1;
+            len = l1;
+            isrec = true;
+            recbase = (int)(F + 2 * l1 - 2);
+            recdir = -1;
         }
-        uint32_t len = e - s, lmax = len;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor(lmax, off, 64));
-        lmax = uniform(lmax);
         uint64_t S[WM];
 #pragma unroll
         for (uint32_t k = 0; k < WM; ++k) S[k] = ~0ull;
-        for (uint32_t j = 0; j < lmax; ++j) {
-            const bool act = j < len;
-            const uint32_t t = act ? s + j : 0u;
-            uint64_t carry = 0;
-#pragma unroll
-            for (uint32_t k = 0; k < WM; ++k) {
-                if (k < W) {
-                    const uint64_t X = xtab[(uint64_t)k * l2 + t];
-                    const uint64_t u = S[k] & X;
-                    const uint64_t x = S[k] + u + carry;
-                    carry = (x < S[k] || (carry && x == S[k])) ? 1ull : 0ull;
-                    const uint64_t ns = x | (S[k] - u);
-                    S[k] = act ? ns : S[k];
-                }
-            }
-        }
-        if (r < nwin) {
+        if (r0 + 64 > F)
+            pr_lcs_run<WM, true>(p, stride, len, l1, W, lastm, S, lcs, recbase, recdir, isrec);
+        else
+            pr_lcs_run<WM, false>(p, stride, len, l1, W, lastm, S, lcs, recbase, recdir, isrec);
+        if (r < F) {
             uint32_t v = 0;
 #pragma unroll
-            for (uint32_t k = 0; k < WM; ++k) v += (uint32_t)__popcll(~S[k] & lastm[k]);
+            for (uint32_t k = 0; k < WM; ++k)
+                if (k < W) v += (uint32_t)__popcll(~S[k] & lastm[k]);
             lcs[r] = v;
+            gmax = max(gmax, v);
         }
     }
     __threadfence_block();
 
-    // ---- full windows: the reference's bisection (lane 0) ----
+    // ---- full windows: the window the reference's bisection settles on ----
     PrResult res{0.0, 0u, l1};
-    if (l2 > l1) {
-        double score = 0.0;
-        bool perfect = false;
-        uint32_t ds = 0, de = l1;
-        if (lane == 0) {
-            const uint64_t maximum = 2ull * l1;
-            uint64_t cutoff_dist = (uint64_t)ceil((double)maximum * normsim_to_normdist(cutoff / 100));
-            uint64_t best = ~0ull;
-            constexpr uint32_t SEEN = 0x80000000u;
-            auto dist_at = [&](uint32_t p, bool& fresh) -> uint64_t {
-                const uint32_t v = lcs[p];
-                fresh = (v & SEEN) == 0;
-                if (fresh) lcs[p] = v | SEEN;
-                return maximum - 2ull * (v & ~SEEN);
-            };
-            uint2* cur = lists;
-            uint2* nxt = lists + (2 * F + 2);
-            uint32_t ncur = 1, nnext = 0;
-            cur[0] = make_uint2(0u, F - 1);
-            while (ncur && !perfect) {
-                for (uint32_t k = 0; k < ncur && !perfect; ++k) {
-                    const uint32_t f = cur[k].x, sc = cur[k].y;
-                    bool fresh;
-                    const uint64_t df = dist_at(f, fresh);
-                    if (fresh && df < cutoff_dist) {
-                        cutoff_dist = best = df;
-                        ds = f;
-                        de = f + l1;
-                        if (best == 0) perfect = true;
-                    }
-                    if (perfect) break;
-                    const uint64_t dsc = dist_at(sc, fresh);
-                    if (fresh && dsc < cutoff_dist) {
-                        cutoff_dist = best = dsc;
-                        ds = sc;
-                        de = sc + l1;
-                        if (best == 0) perfect = true;
-                    }
-                    if (perfect) break;
-                    const uint32_t cell_diff = sc - f;
-                    if (cell_diff == 1) continue;
-                    const uint64_t known = df > dsc ? df - dsc : dsc - df;
-                    const uint64_t improve = (cell_diff - known / 2) / 2 * 2;
-                    const int64_t min_score = (int64_t)(df < dsc ? df : dsc) - (int64_t)improve;
-                    if (min_score < (int64_t)cutoff_dist) {
-                        const uint32_t center = cell_diff / 2;
-                        nxt[nnext++] = make_uint2(f, f + center);
-                        nxt[nnext++] = make_uint2(f + center, sc);
+    if (F > 0) {
+        const uint32_t maximum = 2 * l1;
+        // normsim_to_normdist is at most 1: the cutoff distance is at most maximum
+        const uint32_t cut = (uint32_t)ceil((double)maximum * normsim_to_normdist(cutoff / 100));
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) gmax = max(gmax, (uint32_t)__shfl_xor((int)gmax, off, 64));
+        const uint32_t G = uniform(gmax), g = maximum - 2 * G;
+        uint64_t best = ~0ull; // no window beat the cutoff
+        if (g < cut) {
+            // among the windows of distance g: smallest depth in the halving tree, then position
+            uint32_t bdep = ~0u, bpos = 0;
+            for (uint32_t r0 = 0; r0 < F; r0 += 64) {
+                const uint32_t p = r0 + lane;
+                const bool cand = p < F && lcs[p] == G;
+                if (ballot(cand) == 0) continue;
+                uint32_t dep = ~0u;
+                if (cand) {
+                    dep = 0;
+                    if (p != 0 && p != F - 1) {
+                        uint32_t lo = 0, hi = F - 1; // lo < p < hi throughout
+                        for (;;) {
+                            const uint32_t mid = lo + ((hi - lo) >> 1);
+                            ++dep;
+                            if (p == mid) break;
+                            if (p < mid)
+                                hi = mid;
+                            else
+                                lo = mid;
+                        }
                     }
                 }
-                uint2* tsw = cur;
-                cur = nxt;
-                nxt = tsw;
-                ncur = nnext;
-                nnext = 0;
+                uint32_t md = dep;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) md = min(md, (uint32_t)__shfl_xor((int)md, off, 64));
+                md = uniform(md);
+                if (md < bdep) { // later chunks hold later positions: ties keep the earlier
+                    bdep = md;
+                    bpos = r0 + (uint32_t)__builtin_ctzll(ballot(dep == md));
+                }
             }
-            if (perfect) {
-                score = 100.0;
+            res.ds = bpos;
+            res.de = bpos + l1;
+            if (g == 0) {
+                res.score = 100.0;
+                return res;
             }
-            else {
-                double sv = 1.0 - ((double)best / (double)maximum);
-                sv *= 100;
-                if (sv >= cutoff) score = sv;
-            }
+            best = g;
         }
-        // lane 0's values to the wave (the SEEN flags stay in scratch only)
-        res.score = __shfl(score, 0, 64);
-        res.ds = (uint32_t)__shfl((int)ds, 0, 64);
-        res.de = (uint32_t)__shfl((int)de, 0, 64);
-        perfect = __shfl((int)perfect, 0, 64) != 0;
-        if (perfect) return res;
-        if (res.score >= cutoff) cutoff = res.score; // score_cutoff = res.score when it was set
+        double sv = 1.0 - ((double)best / (double)maximum);
+        sv *= 100;
+        if (sv >= cutoff) {
+            res.score = sv;
+            cutoff = sv;
+        }
     }
 
     // ---- prefixes, then suffixes: first window of the largest ratio ----
@@ -227,7 +325,7 @@ __device__ PrResult pr_search(const CharT* __restrict__ nd, uint32_t l1, const C
                 bool in = false;
                 for (uint32_t k = 0; k < W; ++k) in = in || xtab[(uint64_t)k * l2 + t] != 0;
                 if (in) {
-                    const uint32_t v = lcs[(phase == 0 ? F : F + l1 - 1) + i] & 0x7fffffffu;
+                    const uint32_t v = lcs[(phase == 0 ? F : F + l1 - 1) + i];
                     const double ls = pr_ratio(l1, wl, v, cutoff);
                     if (ls > best) {
                         best = ls;
@@ -263,78 +361,100 @@ __device__ PrResult pr_search(const CharT* __restrict__ nd, uint32_t l1, const C
     return res;
 }
 
-template <uint32_t WM, typename CharT>
-__global__ __launch_bounds__(256) void k_pralign(PrAlignArgs a)
+// partial_ratio_alignment of one pair (fuzz_impl.hpp:196-230) in workspace ws
+template <uint32_t WM, typename CharT, bool LDS>
+__device__ __forceinline__ void pr_pair(const PrAlignArgs& a, uint32_t pair, const CharT* A, uint32_t la,
+                                        const CharT* B, uint32_t lb, uint64_t* ws, bool tables)
 {
-    const uint32_t item = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (item >= a.nitems) return;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t pair = a.items[item];
-    const CharT* A = (const CharT*)a.achars + a.aoff[pair];
-    const CharT* B = (const CharT*)a.bchars + a.boff[pair];
-    const uint32_t la = (uint32_t)(a.aoff[pair + 1] - a.aoff[pair]), lb = (uint32_t)(a.boff[pair + 1] - a.boff[pair]);
     const bool swap = la > lb; // partial_ratio_alignment(s2, s1) with the fields exchanged
     const CharT* nd = swap ? B : A;
     const CharT* hy = swap ? A : B;
     const uint32_t l1 = swap ? lb : la, l2 = swap ? la : lb;
-    uint8_t* base = a.scratch + a.scratch_off[item];
-    const uint64_t W = (l1 + 63) / 64;
-    uint64_t* xtab = (uint64_t*)base;
-    uint64_t* xtab2 = xtab + W * l2;
-    uint32_t* lcs = (uint32_t*)(xtab2 + (l1 == l2 ? W * l1 : 0));
-    uint2* lists = (uint2*)(base + (pralign_scratch_bytes(l1, l2) - (2ull * (l2 - l1) + 2) * 2 * 8));
-
-    PrResult r = pr_search<WM, CharT>(nd, l1, hy, l2, a.cutoff, xtab, lcs, lists);
-    uint32_t ss = 0, se = l1, dst = r.ds, den = r.de;
-    double score = r.score;
-    if (score != 100.0 && l1 == l2) {
-        const double c2 = a.cutoff < score ? score : a.cutoff; // std::max(score_cutoff, score)
-        const PrResult r2 = pr_search<WM, CharT>(hy, l2, nd, l1, c2, xtab2, lcs, lists);
-        if (r2.score > score) {
-            score = r2.score;
-            ss = r2.ds;
-            se = r2.de;
+    uint32_t ss = 0, se = l1, dst = 0, den = l1;
+    double score = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+        double c = a.cutoff;
+        if (pass == 1) {
+            if (score == 100.0 || l1 != l2) break;
+            c = a.cutoff < score ? score : a.cutoff; // std::max(score_cutoff, score)
+            __threadfence_block();
+        }
+        // the second pass searches the first string for the second (equal lengths)
+        const PrResult r = pr_search<WM, CharT, LDS>(pass ? hy : nd, l1, pass ? nd : hy, l2, c, ws, tables);
+        if (pass == 0) {
+            score = r.score;
+            dst = r.ds;
+            den = r.de;
+        }
+        else if (r.score > score) {
+            score = r.score;
+            ss = r.ds;
+            se = r.de;
             dst = 0;
             den = l2;
         }
     }
-    if (lane == 0) {
-        a.score[pair] = score;
-        uint64_t* o = a.al + 4ull * pair;
-        if (swap) {
-            o[0] = dst;
-            o[1] = den;
-            o[2] = ss;
-            o[3] = se;
-        }
-        else {
-            o[0] = ss;
-            o[1] = se;
-            o[2] = dst;
-            o[3] = den;
+    // lanes 0-4 write the pair's rfa_score_alignment
+    if (lane < 5) {
+        uint64_t v = (uint64_t)__double_as_longlong(score);
+        if (lane == 1) v = swap ? dst : ss;
+        if (lane == 2) v = swap ? den : se;
+        if (lane == 3) v = swap ? ss : dst;
+        if (lane == 4) v = swap ? se : den;
+        a.out[5ull * pair + lane] = v;
+    }
+}
+
+template <uint32_t WM, typename CharT>
+__global__ __launch_bounds__(64 * PRALIGN_WAVES) void k_pralign(PrAlignArgs a)
+{
+    extern __shared__ uint64_t pr_lds[];
+    // the pair and its lengths are wave-uniform: keep them (and the loops they bound) scalar
+    const uint32_t wave = uniform(threadIdx.x >> 6);
+    const uint32_t item = blockIdx.x * PRALIGN_WAVES + wave;
+    if (item >= a.nitems) return;
+    const uint32_t pair = uniform(a.items[item]);
+    const uint64_t ao = a.aoff[pair], bo = a.boff[pair];
+    const CharT* A = (const CharT*)a.achars + ao;
+    const CharT* B = (const CharT*)a.bchars + bo;
+    const uint32_t la = uniform((uint32_t)(a.aoff[pair + 1] - ao)), lb = uniform((uint32_t)(a.boff[pair + 1] - bo));
+    const uint32_t l1 = min(la, lb), l2 = max(la, lb);
+    const bool tables = pralign_symbol_tables((int)sizeof(CharT), l1);
+    if constexpr (WM <= 4) { // wider needles never fit the LDS cap
+        if (pralign_ws_bytes(l1, l2, tables) <= a.lds_bytes) {
+            pr_pair<WM, CharT, true>(a, pair, A, la, B, lb, pr_lds + (uint64_t)wave * (a.lds_bytes / 8), tables);
+            return;
         }
     }
+    pr_pair<WM, CharT, false>(a, pair, A, la, B, lb, (uint64_t*)(a.scratch + a.scratch_off[item]), false);
 }
 
 template <uint32_t WM>
 static void launch_w(int char_width, const PrAlignArgs& a, uint32_t blocks, hipStream_t s)
 {
+    const size_t lds = (size_t)PRALIGN_WAVES * a.lds_bytes;
     if (char_width == 1)
-        hipLaunchKernelGGL((k_pralign<WM, uint8_t>), dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_pralign<WM, uint8_t>), dim3(blocks), dim3(64 * PRALIGN_WAVES), lds, s, a);
     else if (char_width == 2)
-        hipLaunchKernelGGL((k_pralign<WM, uint16_t>), dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_pralign<WM, uint16_t>), dim3(blocks), dim3(64 * PRALIGN_WAVES), lds, s, a);
     else
-        hipLaunchKernelGGL((k_pralign<WM, uint32_t>), dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_pralign<WM, uint32_t>), dim3(blocks), dim3(64 * PRALIGN_WAVES), lds, s, a);
 }
 
 hipError_t launch_pralign(int char_width, uint32_t words, const PrAlignArgs& a, hipStream_t s)
 {
     if (a.nitems == 0) return hipSuccess;
-    const uint32_t blocks = (a.nitems + 3) / 4;
+    if (a.lds_bytes > PRALIGN_LDS_CAP || a.lds_bytes % 8) return hipErrorInvalidValue;
+    const uint32_t blocks = (a.nitems + PRALIGN_WAVES - 1) / PRALIGN_WAVES;
     if (words <= 1)
         launch_w<1>(char_width, a, blocks, s);
+    else if (words <= 2)
+        launch_w<2>(char_width, a, blocks, s);
     else if (words <= 4)
         launch_w<4>(char_width, a, blocks, s);
+    else if (words <= 8)
+        launch_w<8>(char_width, a, blocks, s);
     else
         launch_w<16>(char_width, a, blocks, s);
     return hipGetLastError();

@@ -122,6 +122,26 @@ def test_long_needles(ctx, oracle, lens, cutoff):
     _check(ctx, oracle, _long_pairs(rng, list(lens) * 2), cutoff)
 
 
+@pytest.mark.parametrize("l1", [5, 40, 64, 100, 200, 300])
+def test_many_equal_windows(ctx, oracle, l1):
+    """Haystacks of runs and repeats, where many full windows share the smallest distance and
+    the search's visiting order decides the reported window."""
+    rng = np.random.default_rng(460 + l1)
+    pairs = []
+    for i in range(24):
+        nd = rng.integers(97, 99, size=l1).astype(np.uint32)
+        if i % 3 == 0:
+            hy = np.full(l1 + int(rng.integers(1, 400)), 97, np.uint32)
+        elif i % 3 == 1:
+            hy = np.tile(nd, 2 + i % 5)[: l1 + int(rng.integers(1, 3 * l1 + 2))]
+        else:
+            hy = np.concatenate([np.full(int(rng.integers(0, 150)), 98, np.uint32), nd,
+                                 np.full(int(rng.integers(1, 150)), 97, np.uint32)])
+        pairs.append((nd, hy) if i % 2 else (hy, nd))
+    for c in (0.0, 60.0):
+        _check(ctx, oracle, pairs, c)
+
+
 def test_trivial_pairs(ctx, oracle):
     e = np.zeros(0, np.uint32)
     pairs = [(e, e), (e, _u("abc")), (_u("abc"), e), (_u("abc"), _u("abc")), (_u("a"), _u("b")),
