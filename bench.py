@@ -1053,6 +1053,31 @@ def workload_pralign(args, emit=True):
                                    "spaces, needles 8-128 units, haystacks needle + 0-512 units (3 in 4 holding "
                                    "a mutated copy); whole call (host-inclusive)"},
             "roofline": None}
+    # algorithmic LCS word-steps: every full window (l1 steps) plus the prefix and suffix passes
+    # (l1 - 1 and l1 steps), times the needle's words; equal lengths below 100 search twice
+    la = (ao[1:] - ao[:-1]).astype(np.float64)
+    lb = (bo[1:] - bo[:-1]).astype(np.float64)
+    l1, l2 = np.minimum(la, lb), np.maximum(la, lb)
+    w = np.ceil(l1 / 64)
+    steps_pair = w * ((l2 - l1) * l1 + np.maximum(2 * l1 - 1, 0))
+    steps_pair += np.where((l1 == l2) & (l1 > 0) & (got["score"] != 100.0), w * (2 * l1 - 1), 0)
+    wsteps = float(steps_pair.sum())
+    ops = wsteps * 10.0  # 10 INT32 ops per 64-bit LCS word-step
+    peak = 39.322
+    tops = ops / (ms / 1e3) / 1e12
+    prof = read_profile_facts(os.path.join(ROOT, "profiles", "partial_alignment_65536_pairs.md"))
+    kus = prof.get("kernel_us_per_call")
+    line["roofline"] = {"bound": "valu", "achieved": round(tops, 3), "peak": peak, "unit": "TOP/s",
+                        "frac": round(tops / peak, 4), "traffic": None,
+                        "time": "whole call (host-inclusive: grouping, upload, kernels, results' copy back)",
+                        "ops_model": f"LCS word-steps of every candidate window x 10 INT32 ops; {wsteps:.4g} "
+                                     "word-steps per call",
+                        "kernel": "k_pralign<1 word>, k_pralign<2 words>",
+                        "kernel_only": ({"achieved": round(ops / (kus * 1e-6) / 1e12, 3),
+                                         "frac": round(ops / (kus * 1e-6) / 1e12 / peak, 4),
+                                         "kernel_us_per_call": kus,
+                                         "valu_issue_utilization": prof.get("valu_issue_utilization"),
+                                         "source": "profiles/partial_alignment_65536_pairs.md"} if kus else None)}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py
