@@ -903,7 +903,7 @@ void cdist_partial(rfa_scorer* sc, const rfa_strings* ch, double cd, double* dou
 // Levenshtein with general weights: tile-planned queries (<= 64 units) through the
 // tile kernel, the rest one wave per pair.  The long kernel holds up to
 // WLEV_LONG_MAX pattern rows across a wave: the query, or the choice when the
-// query is longer.
+// query is longer; when both are longer, the query in strips of WLEV_LONG_MAX rows.
 void cdist_wlev(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, void* dout, const uint64_t* d_qcut)
 {
     rfa_context* ctx = sc->ctx;
@@ -949,13 +949,10 @@ void cdist_wlev(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, vo
         }
     }
     if (!sc->block_queries.empty()) {
-        uint64_t pattern = sc->block_max_len;
-        if (pattern > WLEV_LONG_MAX) {
-            if (ch->max_len > WLEV_LONG_MAX)
-                throw ApiError(RFA_ERR_UNSUPPORTED,
-                               "Levenshtein with general weights of two strings longer than 1024 code units");
-            pattern = WLEV_LONG_MAX;
-        }
+        // a query over WLEV_LONG_MAX units holds the choice across the lanes, or, when the
+        // choice is that long too, runs in strips of WLEV_LONG_MAX rows
+        const bool strips = sc->block_max_len > WLEV_LONG_MAX && ch->max_len > WLEV_LONG_MAX;
+        const uint64_t pattern = std::min<uint64_t>(sc->block_max_len, WLEV_LONG_MAX);
         WlevLongArgs wa{};
         wa.qchars = sc->q->d_chars;
         wa.qoff = sc->q->d_off;
@@ -970,8 +967,15 @@ void cdist_wlev(rfa_scorer* sc, const rfa_strings* ch, const ScoreSpec& spec, vo
         wa.qcut = d_qcut;
         const uint32_t cells = (uint32_t)std::max<uint64_t>(1, (pattern + 63) / 64);
         const uint64_t items = (uint64_t)wa.nql * nc;
-        const uint32_t nblocks = (uint32_t)std::max<uint64_t>(
+        uint32_t nblocks = (uint32_t)std::max<uint64_t>(
             1, std::min<uint64_t>((items + 3) / 4, (uint64_t)ctx->num_cus * 8));
+        if (strips && items) {
+            // one boundary row (the longest choice + 1 values) per wave, at most 1 GiB in all
+            wa.bnd_stride = (ch->max_len + 1 + 63) / 64 * 64;
+            const uint64_t per_block = 4 * wa.bnd_stride * sizeof(int32_t);
+            nblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nblocks, (1ull << 30) / per_block));
+            wa.bnd = (int32_t*)ctx->wl_bnd.get(nblocks * per_block);
+        }
         if (items) HIP_CHECK(launch_wlong(width, cells, wa, nblocks, s));
     }
 }
@@ -1664,7 +1668,8 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
                           &ctx->plan_vals, &ctx->plan_temp, &ctx->topk_scores, &ctx->topk_run[0],
                           &ctx->topk_run[1], &ctx->topk_cand_s, &ctx->topk_cand_i, &ctx->topk_counts, &ctx->topk_q,
                           &ctx->topk_out, &ctx->qcut, &ctx->ed_items, &ctx->ed_rows, &ctx->ed_ops, &ctx->tset_cols,
-                          &ctx->tset_rows})
+                          &ctx->tset_rows, &ctx->band_q, &ctx->band_w, &ctx->wl_bnd, &ctx->wr_buf[0],
+                          &ctx->wr_buf[1], &ctx->wr_buf[2]})
             b->release();
         ctx->staging.release();
         ctx->pairs_graph.release();

@@ -214,6 +214,7 @@ struct rfa_context {
         plan_temp;
     rfa::DevBuf topk_scores, topk_run[2], topk_cand_s, topk_cand_i, topk_counts, topk_q, topk_out, qcut;
     rfa::DevBuf ed_items, ed_rows, ed_ops, band_q, band_w;
+    rfa::DevBuf wl_bnd; // weighted Levenshtein strips: a boundary row per wave
     rfa::DevBuf tset_cols, tset_rows; // partial_token_ratio: the repeat columns and rows
     size_t lane_arena_bytes_zeroed = 0;
     uint32_t gen_next = 1;

@@ -319,8 +319,10 @@ struct WlevLongArgs {
     ScoreSpec spec;
     void* out;            // out[q * out_stride + c]
     const uint64_t* qcut; // integer cutoff per query (nullptr: spec.cutoff_u)
+    int32_t* bnd;         // pairs of two strings over WLEV_LONG_MAX units: a boundary row of
+    uint64_t bnd_stride;  // bnd_stride ints (> the longest choice) per wave of the launch
 };
-constexpr uint32_t WLEV_LONG_MAX = 1024;
+constexpr uint32_t WLEV_LONG_MAX = 1024; // pattern rows a wave holds (longer patterns run in strips)
 hipError_t launch_wlong(int char_width, uint32_t cells_per_lane, const WlevLongArgs& a, uint32_t nblocks,
                         hipStream_t s);
 

@@ -177,6 +177,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_wtile(TileArgs a)
 }
 
 // ---- long kernel: one wave per pair, the pattern's rows across the lanes ------
+//
+// The pattern's rows sit K per lane (64 K rows); a column step takes the text unit
+// wave-uniform, forms each cell's candidates and closes the column's vertical
+// (insertion along the pattern) chain with a min-plus scan across the lanes.  A
+// pattern longer than 64 K rows (both strings over 1,024 units) runs in strips of
+// 64 K rows: each strip reads its top boundary, the previous strip's bottom row,
+// from the wave's boundary row in scratch and writes its own bottom row there, 64
+// columns per coalesced load and store.
 
 template <typename CharT, int K>
 __global__ __launch_bounds__(256) void k_wlong(WlevLongArgs a)
@@ -188,6 +196,8 @@ __global__ __launch_bounds__(256) void k_wlong(WlevLongArgs a)
     const uint64_t nwaves = (uint64_t)gridDim.x * 4;
     const uint64_t items = (uint64_t)a.nql * a.nc;
     constexpr int32_t INF = 0x7fffffff;
+    constexpr uint32_t SROWS = 64 * K; // pattern rows per strip
+    int32_t* __restrict__ bnd = a.bnd ? a.bnd + wave0 * a.bnd_stride : nullptr;
 
     for (uint64_t it = wave0; it < items; it += nwaves) {
         const uint32_t qi = a.qlist[it / a.nc];
@@ -197,52 +207,70 @@ __global__ __launch_bounds__(256) void k_wlong(WlevLongArgs a)
         const uint32_t n = uniform((uint32_t)(a.coff[ci + 1] - co));
         ScoreSpec spec = a.spec;
         if (a.qcut) spec.cutoff_u = a.qcut[qi];
-        // the pattern (rows) is the query unless it is longer than the lanes hold
-        const bool swap = m > (uint32_t)(64 * K);
+        // the pattern (rows) is the query unless it is longer than a strip and the choice is not
+        const bool swap = m > SROWS && n <= SROWS;
         const CharT* P = swap ? cs + co : qs + qo;
         const CharT* T = swap ? qs + qo : cs + co;
         const uint32_t lp = swap ? n : m, lt = swap ? m : n;
         const int32_t rc = (int32_t)(swap ? spec.ins : spec.del); // cost of a pattern row
         const int32_t cc = (int32_t)(swap ? spec.del : spec.ins); // cost of a text column
         const int32_t rep = (int32_t)u64min(spec.rep, spec.ins + spec.del);
+        const uint32_t nstrip = lp > SROWS ? (lp + SROWS - 1) / SROWS : 1u; // > 1 only with a boundary row
 
-        uint32_t pu[K];
-        int32_t V[K], rr[K];
+        int32_t V[K];
+        for (uint32_t strip = 0; strip < nstrip; ++strip) {
+            const uint32_t row0 = strip * SROWS; // rows above the strip
+            const bool last = strip + 1 == nstrip;
+            uint32_t pu[K];
+            int32_t rr[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint32_t r = lane * K + k + 1; // row of cell k
-            pu[k] = r <= lp ? (uint32_t)P[r - 1] : 0u;
-            rr[k] = (int32_t)r * rc;
-            V[k] = rr[k];
-        }
-        for (uint32_t j0 = 0; j0 < lt; j0 += 64) {
-            const uint32_t tu = j0 + lane < lt ? (uint32_t)T[j0 + lane] : 0u;
-            const uint32_t steps = min(64u, lt - j0);
-            for (uint32_t s = 0; s < steps; ++s) {
-                const uint32_t t = readlane(tu, s);
-                const int32_t top_old = (int32_t)(j0 + s) * cc, top_new = top_old + cc;
-                int32_t prev = __shfl_up(V[K - 1], 1, 64);
-                if (lane == 0) prev = top_old;
-                int32_t run = INF;
-                int32_t Pk[K];
+            for (int k = 0; k < K; ++k) {
+                const uint32_t r = row0 + lane * K + k + 1; // row of cell k
+                pu[k] = r <= lp ? (uint32_t)P[r - 1] : 0u;
+                rr[k] = (int32_t)r * rc;
+                V[k] = rr[k];
+            }
+            int32_t top = (int32_t)row0 * rc; // D[row0][j], j = 0 first
+            for (uint32_t j0 = 0; j0 < lt; j0 += 64) {
+                const uint32_t tu = j0 + lane < lt ? (uint32_t)T[j0 + lane] : 0u;
+                const uint32_t steps = min(64u, lt - j0);
+                // the top boundary at columns j0 + 1 .. j0 + 64
+                int32_t tb = (int32_t)(j0 + lane + 1) * cc;
+                if (strip > 0) tb = j0 + lane < lt ? bnd[j0 + lane + 1] : 0;
+                int32_t nb = 0; // this strip's bottom row at column j0 + lane + 1
+                for (uint32_t s = 0; s < steps; ++s) {
+                    const uint32_t t = readlane(tu, s);
+                    const int32_t top_old = top, top_new = (int32_t)readlane((uint32_t)tb, s);
+                    top = top_new;
+                    int32_t prev = __shfl_up(V[K - 1], 1, 64);
+                    if (lane == 0) prev = top_old;
+                    int32_t run = INF;
+                    int32_t Pk[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int32_t A = min(V[k] + cc, prev + (pu[k] == t ? 0 : rep));
-                    prev = V[k];
-                    run = min(run, A - rr[k]);
-                    Pk[k] = run;
+                    for (int k = 0; k < K; ++k) {
+                        const int32_t A = min(V[k] + cc, prev + (pu[k] == t ? 0 : rep));
+                        prev = V[k];
+                        run = min(run, A - rr[k]);
+                        Pk[k] = run;
+                    }
+                    int32_t incl = run;
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const int32_t o = __shfl_up(incl, off, 64);
+                        if ((int)lane >= off) incl = min(incl, o);
+                    }
+                    int32_t carry = __shfl_up(incl, 1, 64);
+                    if (lane == 0) carry = INF;
+                    // the scan runs on values less their row's rr: the top boundary less row0's
+                    carry = min(carry, top_new - (int32_t)row0 * rc);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) V[k] = min(carry, Pk[k]) + rr[k];
+                    if (!last) { // a full strip: its bottom row is lane 63's last cell
+                        const int32_t bv = (int32_t)readlane((uint32_t)V[K - 1], 63);
+                        nb = lane == s ? bv : nb;
+                    }
                 }
-                int32_t incl = run;
-#pragma unroll
-                for (int off = 1; off < 64; off <<= 1) {
-                    const int32_t o = __shfl_up(incl, off, 64);
-                    if ((int)lane >= off) incl = min(incl, o);
-                }
-                int32_t carry = __shfl_up(incl, 1, 64);
-                if (lane == 0) carry = INF;
-                carry = min(carry, top_new);
-#pragma unroll
-                for (int k = 0; k < K; ++k) V[k] = min(carry, Pk[k]) + rr[k];
+                if (!last && lane < steps) bnd[j0 + lane + 1] = nb;
             }
         }
         uint32_t res;
@@ -250,11 +278,12 @@ __global__ __launch_bounds__(256) void k_wlong(WlevLongArgs a)
             res = lt * (uint32_t)cc;
         }
         else {
-            const uint32_t kk = (lp - 1) % K;
+            const uint32_t lr = (lp - 1) % SROWS; // the last row's place in the last strip
+            const uint32_t kk = lr % K;
             int32_t v = V[0];
 #pragma unroll
             for (int k = 1; k < K; ++k) v = ((uint32_t)k == kk) ? V[k] : v;
-            res = (uint32_t)readlane((uint32_t)v, (lp - 1) / K);
+            res = (uint32_t)readlane((uint32_t)v, lr / K);
         }
         RawResult r{};
         r.v = res;

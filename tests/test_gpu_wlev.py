@@ -1,8 +1,9 @@
 """GPU parity of Levenshtein with general weights (generalized Wagner-Fischer,
 Levenshtein_impl.hpp:85-155) against the oracle, bit for bit: the tile kernel
 (queries of at most 64 units), the long kernel (one wave per pair, the query's
-rows across the lanes) and its swapped form (queries over 1024 units against
-choices of at most 1024), every kind and cutoff, per-query cutoffs and top-k."""
+rows across the lanes), its swapped form (queries over 1024 units against
+choices of at most 1024) and its strips (both strings over 1024 units), every
+kind and cutoff, per-query cutoffs and top-k."""
 import numpy as np
 import pytest
 
@@ -88,6 +89,19 @@ def test_long_path_swapped(ctx, oracle, weights):
     check(ctx, oracle, queries, choices, weights)
 
 
+@pytest.mark.parametrize("weights", [(1, 2, 3), (3, 1, 2), (2, 2, 1), (1, 1, 5)])
+def test_long_path_strips(ctx, oracle, weights):
+    """Both strings over 1024 units: the query in strips of 1024 rows (2 and 3 strips, a strip
+    boundary exactly at the end), against choices on either side of 1024 units."""
+    rng = np.random.default_rng(520 + weights[0] + 3 * weights[2])
+    queries = [rng.integers(97, 100, size=n).astype(np.uint32) for n in (1025, 2048, 2100)]
+    queries.append(rng.integers(97, 100, size=300).astype(np.uint32))  # one strip beside them
+    choices = [rng.integers(97, 100, size=n).astype(np.uint32) for n in (1025, 1500, 2600)]
+    choices += [_gen.mutate(rng, queries[1], 40, alpha_n=3), queries[2].copy(), queries[0][:1024],
+                rng.integers(97, 100, size=200).astype(np.uint32), np.zeros(0, np.uint32)]
+    check(ctx, oracle, queries, choices, weights, kinds=(0, 3))
+
+
 def test_per_query_cutoffs(ctx, oracle):
     import rapidfuzz_amd as rfa
 
@@ -127,8 +141,7 @@ def test_limits(ctx):
 
     long_a = np.full(1100, 97, np.uint32)
     sc = rfa.Scorer(ctx, 0, ctx.strings([long_a], width=1), weights=(1, 2, 3))
-    with pytest.raises(rfa.RfaError):
-        sc.cdist(ctx.strings([long_a], width=1), 0)  # both strings longer than 1024 units
+    assert int(sc.cdist(ctx.strings([long_a], width=1), 0)[0, 0]) == 0  # both longer than 1024 units
     sc.close()
     sc = rfa.Scorer(ctx, 0, ctx.strings([b"abc"]), weights=(1 << 24, 1, 3))
     with pytest.raises(rfa.RfaError):
