@@ -94,6 +94,7 @@ class Oracle:
         L.orc_set_units.argtypes = [ctypes.c_int, ctypes.c_int]
         L.orc_editops.restype = ctypes.c_size_t
         L.orc_editops.argtypes = [ctypes.c_int, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _i32p, _u64p, _u64p]
+        L.orc_set_lev_block.argtypes = [ctypes.c_int]
         L.orc_sorted_join.restype = ctypes.c_size_t
         L.orc_sorted_join.argtypes = [_u32p, ctypes.c_size_t, _u32p]
         L.orc_jaro_raw.restype = None
@@ -174,7 +175,14 @@ class Oracle:
         dp = np.zeros(max(1, n), np.uint64)
         k = self.lib.orc_editops(metric, _p(a, _u32p), len(a), _p(b, _u32p), len(b), _p(t, _i32p), _p(sp, _u64p),
                                  _p(dp, _u64p))
+        if k == ctypes.c_size_t(-1).value:
+            raise ValueError("the block band vanished (distance above the bound)")
         return t[:k], sp[:k], dp[:k]
+
+    def set_lev_block(self, on):
+        """Levenshtein editops past 64 stripped units: the reference's block band (True, the
+        default) or the full-matrix walk (False)."""
+        self.lib.orc_set_lev_block(int(bool(on)))
 
     def sorted_join(self, s, char_width=4, signed8=False):
         """detail::sorted_split(s).join() as uint32 code units."""

@@ -412,6 +412,166 @@ static uint64_t lev_cached_distance(const Pair* p, uint64_t c)
  * ones record banded or Hirschberg matrices); LCS's holds for every length. */
 enum { ED_REPLACE = 1, ED_INSERT = 2, ED_DELETE = 3 };
 
+/* levenshtein_align for a stripped first string of more than 64 units below the
+ * Hirschberg split (Levenshtein_impl.hpp:1023-1045, 1185-1200): the block recurrence
+ * with RecordMatrix and max = the longer length (Levenshtein_impl.hpp:641-850),
+ * Ukkonen's band of blocks moving per row, then recover_alignment
+ * (Levenshtein_impl.hpp:959-1015) over the ShiftedBitMatrix rows
+ * (details/Matrix.hpp:121-180: bits left of a row's first block read clear; a row's
+ * words past the band keep the initial VP = ~0, VN = 0).  Returns the operation
+ * count, or SIZE_MAX if the band vanishes. */
+static size_t lev_block_editops(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, size_t pre,
+                                int32_t* type, uint64_t* src_pos, uint64_t* dest_pos)
+{
+    const size_t words = (la + 63) / 64;
+    size_t max = la > lb ? la : lb;
+    const size_t full_band = la < 2 * max + 1 ? la : 2 * max + 1;
+    size_t fbw = full_band / 64 + 2;
+    if (fbw > words) fbw = words;
+    uint64_t* VPm = (uint64_t*)malloc(lb * fbw * sizeof(uint64_t));
+    uint64_t* VNm = (uint64_t*)malloc(lb * fbw * sizeof(uint64_t));
+    size_t* off = (size_t*)malloc(lb * sizeof(size_t));
+    uint64_t* VP = (uint64_t*)malloc(words * sizeof(uint64_t));
+    uint64_t* VN = (uint64_t*)malloc(words * sizeof(uint64_t));
+    ptrdiff_t* scores = (ptrdiff_t*)malloc(words * sizeof(ptrdiff_t));
+    for (size_t i = 0; i < lb * fbw; ++i) VPm[i] = ~0ull, VNm[i] = 0;
+    for (size_t w = 0; w < words; ++w) VP[w] = ~0ull, VN[w] = 0, scores[w] = (ptrdiff_t)((w + 1) * 64);
+    scores[words - 1] = (ptrdiff_t)la;
+    const uint64_t Last = 1ull << ((la - 1) % 64);
+    ptrdiff_t first = 0, last;
+    {
+        size_t t = (max + la - lb) / 2;
+        if (max < t) t = max;
+        size_t nb = (t + 1 + 63) / 64;
+        if (nb > words) nb = words;
+        last = (ptrdiff_t)nb - 1;
+    }
+    size_t result = 0;
+    int failed = 0;
+    for (size_t row = 0; row < lb && !failed; ++row) {
+        uint64_t HP_carry = 1, HN_carry = 0;
+        off[row] = (size_t)first * 64;
+#define ADVANCE(word, delta)                                                                                  \
+    do {                                                                                                     \
+        uint64_t PMj = 0;                                                                                    \
+        for (size_t i = 0; i < 64 && (word) * 64 + i < la; ++i)                                              \
+            if (a[(word) * 64 + i] == b[row]) PMj |= 1ull << i;                                              \
+        const uint64_t X = PMj | HN_carry;                                                                   \
+        const uint64_t D0 = (((X & VP[word]) + VP[word]) ^ VP[word]) | X | VN[word];                         \
+        uint64_t HP = VN[word] | ~(D0 | VP[word]);                                                           \
+        uint64_t HN = D0 & VP[word];                                                                         \
+        const uint64_t HPt = HP_carry, HNt = HN_carry;                                                       \
+        if ((size_t)(word) < words - 1) {                                                                    \
+            HP_carry = HP >> 63;                                                                             \
+            HN_carry = HN >> 63;                                                                             \
+        }                                                                                                    \
+        else {                                                                                               \
+            HP_carry = (HP & Last) != 0;                                                                     \
+            HN_carry = (HN & Last) != 0;                                                                     \
+        }                                                                                                    \
+        HP = (HP << 1) | HPt;                                                                                \
+        HN = (HN << 1) | HNt;                                                                                \
+        VP[word] = HN | ~(D0 | HP);                                                                          \
+        VN[word] = HP & D0;                                                                                  \
+        VPm[row * fbw + (size_t)((word) - first)] = VP[word];                                                \
+        VNm[row * fbw + (size_t)((word) - first)] = VN[word];                                                \
+        (delta) = (ptrdiff_t)HP_carry - (ptrdiff_t)HN_carry;                                                 \
+    } while (0)
+        for (ptrdiff_t w = first; w <= last; ++w) {
+            ptrdiff_t d;
+            ADVANCE(w, d);
+            scores[w] += d;
+        }
+        {
+            const ptrdiff_t r1 = (ptrdiff_t)lb - (ptrdiff_t)row - 1;
+            const ptrdiff_t r2 = (ptrdiff_t)la - ((ptrdiff_t)((1 + last) * 64 - 1) - 1);
+            const ptrdiff_t m2 = scores[last] + (r1 > r2 ? r1 : r2);
+            if (m2 < (ptrdiff_t)max) max = (size_t)m2;
+        }
+#define ROW_NUM(w) ((size_t)(w) + 1 == words ? la - 1 : ((size_t)(w) + 1) * 64 - 1)
+        if ((size_t)last + 1 < words) {
+            const ptrdiff_t cond = (ptrdiff_t)(max + 128 + row + la) - (ptrdiff_t)(scores[last] + 2 + lb);
+            if ((ptrdiff_t)ROW_NUM(last) < cond) {
+                ++last;
+                VP[last] = ~0ull;
+                VN[last] = 0;
+                const size_t chars = (size_t)last + 1 == words ? ((la - 1) % 64 + 1) : 64;
+                scores[last] = scores[last - 1] + (ptrdiff_t)chars - (ptrdiff_t)HP_carry + (ptrdiff_t)HN_carry;
+                ptrdiff_t d;
+                ADVANCE(last, d);
+                scores[last] += d;
+            }
+        }
+        for (; last >= first; --last) {
+            const int c1 = scores[last] < (ptrdiff_t)(max + 64);
+            const ptrdiff_t cond = (ptrdiff_t)(max + 128 + row + la + 1) - (ptrdiff_t)(scores[last] + 2 + lb);
+            const int c2 = (ptrdiff_t)ROW_NUM(last) <= cond;
+            if (c1 && c2) break;
+        }
+        for (; first <= last; ++first) {
+            const int c1 = scores[first] < (ptrdiff_t)(max + 64);
+            const ptrdiff_t cond = (ptrdiff_t)(scores[first] + la + row) - (ptrdiff_t)(max + lb);
+            const int c2 = (ptrdiff_t)ROW_NUM(first) >= cond;
+            if (c1 && c2) break;
+        }
+        if (last < first) failed = 1;
+#undef ROW_NUM
+#undef ADVANCE
+    }
+    if (!failed) {
+        size_t dist = (size_t)scores[words - 1];
+        if (dist > max) failed = 1;
+        result = dist;
+        size_t col = la, row = lb;
+#define TEST_BIT(M, r, c)                                                                                     \
+    ((c) < off[r] ? 0 : (int)(((M)[(r) * fbw + ((c) - off[r]) / 64] >> (((c) - off[r]) % 64)) & 1))
+        while (!failed && row && col) {
+            if (TEST_BIT(VPm, row - 1, col - 1)) {
+                --col;
+                --dist;
+                type[dist] = ED_DELETE, src_pos[dist] = col + pre, dest_pos[dist] = row + pre;
+            }
+            else {
+                --row;
+                if (row && TEST_BIT(VNm, row - 1, col - 1)) {
+                    --dist;
+                    type[dist] = ED_INSERT, src_pos[dist] = col + pre, dest_pos[dist] = row + pre;
+                }
+                else {
+                    --col;
+                    if (a[col] != b[row]) {
+                        --dist;
+                        type[dist] = ED_REPLACE, src_pos[dist] = col + pre, dest_pos[dist] = row + pre;
+                    }
+                }
+            }
+        }
+#undef TEST_BIT
+        while (!failed && col) {
+            --col;
+            --dist;
+            type[dist] = ED_DELETE, src_pos[dist] = col + pre, dest_pos[dist] = row + pre;
+        }
+        while (!failed && row) {
+            --row;
+            --dist;
+            type[dist] = ED_INSERT, src_pos[dist] = col + pre, dest_pos[dist] = row + pre;
+        }
+    }
+    free(VPm);
+    free(VNm);
+    free(off);
+    free(VP);
+    free(VN);
+    free(scores);
+    return failed ? (size_t)-1 : result;
+}
+
+/* 1: Levenshtein pairs past 64 stripped units follow the reference's block band (the
+ * default); 0: the full-matrix walk, kept to show where the two differ */
+static int g_lev_block = 1;
+void orc_set_lev_block(int on) { g_lev_block = on; }
+
 size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, int32_t* type,
                    uint64_t* src_pos, uint64_t* dest_pos)
 {
@@ -433,6 +593,8 @@ size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2
     const uint32_t* a = s1 + pre;
     const uint32_t* b = s2 + pre;
     const size_t la = l1 - pre - suf, lb = l2 - pre - suf;
+    if (metric == ORC_LEVENSHTEIN && la > 64 && lb > 0 && g_lev_block)
+        return lev_block_editops(a, la, b, lb, pre, type, src_pos, dest_pos);
     const size_t W = la + 1;
     uint64_t* T = (uint64_t*)malloc((la + 1) * (lb + 1) * sizeof(uint64_t)); /* T[r * W + c] */
     const int lev = metric == ORC_LEVENSHTEIN;

@@ -60,6 +60,9 @@ uint64_t orc_damerau_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size
 /* Editops of levenshtein_editops (metric 0) / lcs_seq_editops = indel_editops (metric 1
  * or 2) / hamming_editops with pad (metric 7) for one pair; writes at most l1 + l2
  * operations, returns their count. */
+/* Levenshtein editops past 64 stripped units: 1 the reference's block band (default), 0 the
+ * full-matrix walk */
+void orc_set_lev_block(int on);
 size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, int32_t* type,
                    uint64_t* src_pos, uint64_t* dest_pos);
 /* positional metrics: Hamming distance with padding, common prefix / suffix length */

@@ -1,8 +1,9 @@
 // rfa_editops.cpp -- rfa_editops: levenshtein_editops and lcs_seq_editops /
 // indel_editops for batches of pairs (Levenshtein_impl.hpp:959-1015 and 1229-1257,
 // LCSseq_impl.hpp:440-534).  The device strips the common affixes, records the bit
-// rows and walks them back (rfa_editops.hip); the host turns the per-pair counts
-// into offsets and moves the operations.
+// rows and walks them back (rfa_editops.hip; Levenshtein pairs past 64 units through
+// the block kernels, their rows in per-wave scratch); the host turns the per-pair
+// counts into offsets and moves the operations.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -54,40 +55,47 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
         }
     }
     ea.rows = (uint64_t*)ctx->ed_rows.get(std::max<uint64_t>(8, row_words * 8));
-    // items: info, counts (n + 1), offsets (n + 1), row bases (LCS), the error word, scan temp
+    // items: info, counts (n + 1), offsets (n + 1), row bases (LCS), the error word, the
+    // block pairs' largest row area, scan temp
     const size_t scan_bytes = editops_scan_bytes(n);
     const size_t o_info = 0, o_counts = n * sizeof(EdInfo), o_offsets = o_counts + (n + 1) * 8,
-                 o_base = o_offsets + (n + 1) * 8, o_err = o_base + (lev || ham ? 0 : n * 8),
-                 o_temp = (o_err + 4 + 255) / 256 * 256;
+                 o_base = o_offsets + (n + 1) * 8, o_err = o_base + (lev || ham ? 0 : n * 8), o_need = o_err + 8,
+                 o_temp = (o_need + 8 + 255) / 256 * 256;
     uint8_t* items = (uint8_t*)ctx->ed_items.get(o_temp + scan_bytes);
     ea.info = (EdInfo*)(items + o_info);
     ea.counts = (uint64_t*)(items + o_counts);
     uint64_t* d_offsets = (uint64_t*)(items + o_offsets);
     ea.offsets = d_offsets;
     ea.err = (uint32_t*)(items + o_err);
+    ea.need = (uint64_t*)(items + o_need);
     ea.n = n;
     if (!lev && !ham) {
         ea.row_base = (const uint64_t*)(items + o_base);
         HIP_CHECK(hipMemcpyAsync((void*)ea.row_base, row_base.data(), n * 8, hipMemcpyHostToDevice, s));
     }
     HIP_CHECK(hipMemsetAsync(ea.counts + n, 0, 8, s));
-    HIP_CHECK(hipMemsetAsync(ea.err, 0, 4, s));
+    HIP_CHECK(hipMemsetAsync(ea.err, 0, 16, s)); // the error word and the row area
     const uint32_t nblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 3) / 4, (uint64_t)ctx->num_cus * 16));
+    // Levenshtein pairs past 64 units: one wave per 64 pairs scans for them
+    const uint32_t lblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 8));
     if (ham)
         HIP_CHECK(launch_editops_hamming(a->width, ea, false, nblocks, s));
     else
         HIP_CHECK(launch_editops(a->width, ea, lev, false, nblocks, s));
+    if (lev) HIP_CHECK(launch_editops_lev_block(a->width, ea, false, lblocks, s));
     HIP_CHECK(editops_offsets(ea.counts, d_offsets, n, items + o_temp, scan_bytes, s));
-    uint32_t err = 0;
-    HIP_CHECK(hipMemcpyAsync(&err, ea.err, 4, hipMemcpyDeviceToHost, s));
+    uint64_t errneed[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(errneed, ea.err, 16, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(offsets, d_offsets, (n + 1) * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    if (err)
+    if ((uint32_t)errneed[0])
         throw ApiError(RFA_ERR_UNSUPPORTED,
-                       lev ? "Levenshtein editops of a pair whose first string has more than 64 code units "
-                             "without the common prefix and suffix"
+                       lev ? "Levenshtein editops of a pair whose first string, without the common prefix and "
+                             "suffix, has more than 4096 code units, or whose matrix the reference splits "
+                             "(Hirschberg: 64 < |s1| and |s1| * |s2| >= 2^22, |s2| >= 10)"
                            : "LCS editops of a pair whose first string has more than 4096 code units without "
                              "the common prefix and suffix");
+    const uint64_t need = errneed[1];
     if (!ops) return;
     const uint64_t total = offsets[n];
     if (capacity < total)
@@ -99,6 +107,14 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
         HIP_CHECK(launch_editops_hamming(a->width, ea, true, nblocks, s));
     else
         HIP_CHECK(launch_editops(a->width, ea, lev, true, 0, s));
+    if (lev && need) {
+        // the block pairs' rows: one pair's area per wave, at most 1 GiB in all
+        ea.long_stride = (need + 31) / 32 * 32;
+        const uint64_t per_block = 4 * ea.long_stride * 8;
+        const uint32_t wb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(lblocks, (1ull << 30) / per_block));
+        ea.long_rows = (uint64_t*)ctx->ed_long.get(wb * per_block);
+        HIP_CHECK(launch_editops_lev_block(a->width, ea, true, wb, s));
+    }
     HIP_CHECK(hipMemcpyAsync(ops, ea.ops, total * sizeof(EdOp), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
 }

@@ -26,8 +26,9 @@
 //     are written at the pair's offset (the prefix sum of the counts).
 // Levenshtein's script is the reference's when the stripped first string has at
 // most 64 units: the reference then records the full single-word matrix, which
-// depends only on the strings (longer ones use banded or Hirschberg matrices and
-// are flagged here).  LCS records the full matrix at every length; stripped first
+// depends only on the strings.  Longer ones (up to 4096 units, where the reference
+// records its block band rather than splitting by Hirschberg) go to k_ed_levl below;
+// the rest are flagged.  LCS records the full matrix at every length; stripped first
 // strings of up to 4096 units run here, over several words (lane l holding
 // s1[64 k + l], so a word's match mask is one ballot; the words' additions carry
 // across lanes by carry-lookahead on two ballots).
@@ -96,13 +97,26 @@ __global__ __launch_bounds__(256) void k_ed_rows(EditopsArgs a)
         info.pre = pre;
         info.la = (uint32_t)la;
         info.lb = (uint32_t)lb;
-        info.flag = la > (LEV ? 64u : EDITOPS_LCS_MAX) ? 1u : 0u;
+        info.flag = 0;
+        if constexpr (LEV) {
+            // past 64 units the reference records its block band, unless it splits the
+            // problem (Hirschberg) first: those pairs are not taken
+            if (la > 64 && lb > 0) {
+                const bool whole = (2 * la * lb) / 8 < 1048576 || lb < 10;
+                info.flag = la <= EDITOPS_LEV_BLOCK_MAX && whole ? 2u : 1u;
+            }
+        }
+        else {
+            info.flag = la > EDITOPS_LCS_MAX ? 1u : 0u;
+        }
         info.count = (uint32_t)(la + lb);
         if (la == 0 || lb == 0 || info.flag) {
             if (lane == 0) {
                 a.info[it] = info;
                 a.counts[it] = info.flag ? 0 : info.count;
-                if (info.flag) atomicOr(a.err, 1u);
+                if (info.flag == 1) atomicOr(a.err, 1u);
+                if (info.flag == 2)
+                    atomicMax((unsigned long long*)a.need, (unsigned long long)((2 * ((la + 63) / 64) + 1) * lb));
             }
             continue;
         }
@@ -220,7 +234,7 @@ __global__ __launch_bounds__(256) void k_ed_walk(EditopsArgs a)
     if (i >= a.n) return;
     const EdInfo info = a.info[i];
     uint64_t dist = info.count;
-    if (dist == 0) return;
+    if (dist == 0 || info.flag == 2) return; // block pairs: k_ed_levl
     EdOp* out = a.ops + a.offsets[i];
     const uint64_t pre = info.pre;
     uint64_t col = info.la, row = info.lb;
@@ -264,6 +278,272 @@ __global__ __launch_bounds__(256) void k_ed_walk(EditopsArgs a)
         --row;
         put(RFA_EDIT_INSERT);
     }
+}
+
+// ---- Levenshtein editops over several words (Levenshtein_impl.hpp:641-850, 959-1045) ----
+//
+// A stripped first string of 65..4096 units: the reference records the band of its
+// block recurrence (max = the longer stripped length, tightened row by row), with
+// Ukkonen's band of blocks [first_block, last_block] moving per row, and walks it back
+// through ShiftedBitMatrix: row r keeps the words from its first block on (bits left
+// of them read as clear), words the row did not advance keep the matrix's initial
+// values (VP all set, VN clear).  The same steps here: one wave per pair, word w in
+// lane w.  A row advances the band's words from first_block with the carries
+// HP = 1, HN = 0 entering it; a word's D0 depends on the HN carry from the word
+// below it, so every lane forms D0 for both carry values, the chain of HN carries is
+// resolved over the band by scalar steps on two ballots, and each lane then takes its
+// D0.  The band bookkeeping (the running max, a block added after the last one, the
+// shrinking and advancing ends) reads the words' scores with v_readlane.
+//   RECORD = false: the pair's distance (its operation count).
+//   RECORD = true: the rows into the wave's scratch, then lane 0 walks them back and
+//   writes the operations at the pair's offset.
+
+__device__ __forceinline__ uint32_t lev_row_num(uint32_t w, uint32_t W, uint32_t la)
+{
+    return w + 1 == W ? la - 1 : (w + 1) * 64 - 1;
+}
+
+template <typename CharT, bool RECORD>
+__global__ __launch_bounds__(256) void k_ed_levl(EditopsArgs a)
+{
+    const CharT* __restrict__ A = (const CharT*)a.achars;
+    const CharT* __restrict__ B = (const CharT*)a.bchars;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave0 = (uint64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * 4;
+    uint64_t* __restrict__ rows = RECORD ? a.long_rows + wave0 * a.long_stride : nullptr;
+    constexpr uint32_t WMAX = EDITOPS_LEV_BLOCK_MAX / 64;
+    for (uint64_t base = wave0 * 64; base < a.n; base += nwaves * 64) {
+        const uint64_t idx = base + lane;
+        uint64_t todo = ballot(idx < a.n && a.info[idx].flag == 2);
+        while (todo) {
+            const uint64_t it = base + (uint64_t)__builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint64_t pre = a.info[it].pre;
+            const uint32_t la = uniform(a.info[it].la), lb = uniform(a.info[it].lb);
+            const uint64_t as = a.aoff[it] + pre, bs = a.boff[it] + pre;
+            const uint32_t W = (la + 63) / 64; // 2..WMAX
+            uint32_t su[WMAX];
+#pragma unroll
+            for (uint32_t k = 0; k < WMAX; ++k)
+                su[k] = (k < W && k * 64 + lane < la) ? (uint32_t)A[as + k * 64 + lane] : 0u;
+            const uint64_t Last = 1ull << ((la - 1) % 64);
+            const bool top = lane == W - 1;
+            uint64_t VP = ~0ull, VN = 0;
+            int64_t score = lane + 1 < W ? (int64_t)(lane + 1) * 64 : (int64_t)la;
+            int64_t maxv = la > lb ? la : lb;
+            uint32_t fb = 0;
+            uint32_t lbk;
+            {
+                const int64_t t = min(maxv, (maxv + (int64_t)la - (int64_t)lb) / 2) + 1;
+                lbk = (uint32_t)min<int64_t>((int64_t)W, (t + 63) / 64) - 1;
+            }
+            bool bad = false;
+            const uint32_t rw = 2 * W + 1; // row: first block, W VP words, W VN words
+            for (uint32_t j0 = 0; j0 < lb && !bad; j0 += 64) {
+                const uint32_t tb = j0 + lane < lb ? (uint32_t)B[bs + j0 + lane] : 0u;
+                const uint32_t steps = min(64u, lb - j0);
+                for (uint32_t s = 0; s < steps; ++s) {
+                    const uint32_t row = j0 + s;
+                    const uint32_t c = readlane(tb, s);
+                    uint64_t PM = 0; // lane k: the match mask of word k
+#pragma unroll
+                    for (uint32_t k = 0; k < WMAX; ++k) {
+                        if (k < W) {
+                            const uint64_t m = ballot(k * 64 + lane < la && su[k] == c);
+                            PM = lane == k ? m : PM;
+                        }
+                    }
+                    const uint32_t fb0 = fb, lb0 = lbk;
+                    const bool inb = lane >= fb0 && lane <= lb0;
+                    // D0 for an HN carry of 0 and of 1 into the word
+                    const uint64_t X0 = PM, X1 = PM | 1ull;
+                    const uint64_t D00 = (((X0 & VP) + VP) ^ VP) | X0 | VN;
+                    const uint64_t D01 = (((X1 & VP) + VP) ^ VP) | X1 | VN;
+                    const bool h0 = top ? ((D00 & VP) & Last) != 0 : ((D00 & VP) >> 63) != 0;
+                    const bool h1 = top ? ((D01 & VP) & Last) != 0 : ((D01 & VP) >> 63) != 0;
+                    const uint64_t M0 = ballot(inb && h0), M1 = ballot(inb && h1);
+                    uint64_t H = 0; // bit w: the HN carry into word w
+                    {
+                        uint32_t h = 0;
+                        for (uint32_t w = fb0; w <= lb0; ++w) {
+                            H |= (uint64_t)h << w;
+                            h = (uint32_t)(((h ? M1 : M0) >> w) & 1u);
+                        }
+                    }
+                    const uint64_t hn_in = (H >> lane) & 1u;
+                    const uint64_t D0 = hn_in ? D01 : D00;
+                    uint64_t HP = VN | ~(D0 | VP);
+                    uint64_t HN = D0 & VP;
+                    const bool hpo = top ? (HP & Last) != 0 : (HP >> 63) != 0;
+                    const bool hno = top ? (HN & Last) != 0 : (HN >> 63) != 0;
+                    const uint64_t PO = ballot(inb && hpo), NO = ballot(inb && hno);
+                    const uint64_t hp_in = lane == fb0 ? 1ull : (lane ? (PO >> (lane - 1)) & 1u : 0ull);
+                    HP = (HP << 1) | hp_in;
+                    HN = (HN << 1) | hn_in;
+                    if (inb) {
+                        VP = HN | ~(D0 | HP);
+                        VN = HP & D0;
+                        score += (int64_t)hpo - (int64_t)hno;
+                    }
+                    // the running bound on the distance
+                    {
+                        const int64_t sl = (int64_t)(int32_t)readlane((uint32_t)score, lb0);
+                        const int64_t r1 = (int64_t)lb - (int64_t)row - 1;
+                        const int64_t r2 = (int64_t)la - ((int64_t)(1 + lb0) * 64 - 2);
+                        maxv = min(maxv, sl + max(r1, r2));
+                    }
+                    uint32_t amax = lb0; // the last word this row advanced
+                    if (lb0 + 1 < W) {
+                        const int64_t sl = (int64_t)(int32_t)readlane((uint32_t)score, lb0);
+                        const int64_t cond = (maxv + 128 + row + la) - (sl + 2 + lb);
+                        if ((int64_t)lev_row_num(lb0, W, la) < cond) {
+                            // the next block joins with the carries out of the last one
+                            const uint32_t nw = lb0 + 1;
+                            const uint64_t hpc = (PO >> lb0) & 1u, hnc = (NO >> lb0) & 1u;
+                            if (lane == nw) {
+                                const int64_t chars = nw + 1 == W ? (int64_t)((la - 1) % 64 + 1) : 64;
+                                score = sl + chars - (int64_t)hpc + (int64_t)hnc;
+                                VP = ~0ull;
+                                VN = 0;
+                                const uint64_t X = PM | hnc;
+                                const uint64_t D = (((X & VP) + VP) ^ VP) | X | VN;
+                                uint64_t hp = VN | ~(D | VP);
+                                uint64_t hn = D & VP;
+                                const bool po = top ? (hp & Last) != 0 : (hp >> 63) != 0;
+                                const bool no = top ? (hn & Last) != 0 : (hn >> 63) != 0;
+                                hp = (hp << 1) | hpc;
+                                hn = (hn << 1) | hnc;
+                                VP = hn | ~(D | hp);
+                                VN = hp & D;
+                                score += (int64_t)po - (int64_t)no;
+                            }
+                            amax = nw;
+                            lbk = nw;
+                        }
+                    }
+                    if constexpr (RECORD) {
+                        // row: the first block, then VP and VN of the words from it on
+                        uint64_t* r = rows + (uint64_t)row * rw;
+                        const uint32_t w = fb0 + lane;
+                        const bool have = w <= amax;
+                        const uint32_t src = have ? w : lane;
+                        const uint64_t vp = __shfl(VP, (int)src, 64), vn = __shfl(VN, (int)src, 64);
+                        if (lane < W) {
+                            r[1 + lane] = have ? vp : ~0ull;
+                            r[1 + W + lane] = have ? vn : 0ull;
+                        }
+                        if (lane == 0) r[0] = fb0;
+                    }
+                    // Ukkonen's band for the next row
+                    for (;;) {
+                        const int64_t sl = (int64_t)(int32_t)readlane((uint32_t)score, lbk);
+                        const bool c1 = sl < maxv + 64;
+                        const int64_t cond = (maxv + 128 + row + la + 1) - (sl + 2 + lb);
+                        const bool c2 = (int64_t)lev_row_num(lbk, W, la) <= cond;
+                        if (c1 && c2) break;
+                        if (lbk == fb) {
+                            bad = true;
+                            break;
+                        }
+                        --lbk;
+                    }
+                    if (bad) break;
+                    for (; fb <= lbk; ++fb) {
+                        const int64_t sf = (int64_t)(int32_t)readlane((uint32_t)score, fb);
+                        const bool c1 = sf < maxv + 64;
+                        const int64_t cond = (sf + la + row) - (maxv + lb);
+                        const bool c2 = (int64_t)lev_row_num(fb, W, la) >= cond;
+                        if (c1 && c2) break;
+                    }
+                    if (lbk < fb) {
+                        bad = true;
+                        break;
+                    }
+                }
+            }
+            const uint32_t dist = readlane((uint32_t)score, W - 1);
+            if constexpr (!RECORD) {
+                if (lane == 0) {
+                    if (bad) {
+                        a.info[it].flag = 1;
+                        atomicOr(a.err, 1u);
+                    }
+                    else {
+                        a.info[it].count = dist;
+                        a.counts[it] = dist;
+                    }
+                }
+            }
+            else {
+                __threadfence_block();
+                if (lane == 0 && !bad && dist) {
+                    EdOp* out = a.ops + a.offsets[it];
+                    uint64_t d = dist, col = la, row = lb;
+                    auto put = [&](int32_t type) {
+                        --d;
+                        EdOp o;
+                        o.type = type;
+                        o.reserved = 0;
+                        o.src_pos = col + pre;
+                        o.dest_pos = row + pre;
+                        out[d] = o;
+                    };
+                    // ShiftedBitMatrix::test_bit: clear left of the row's first block
+                    auto bit = [&](uint64_t r, uint64_t cc, uint32_t which) -> bool {
+                        const uint64_t* rr = rows + r * rw;
+                        const uint64_t f = rr[0], wd = cc / 64;
+                        if (wd < f) return false;
+                        return (rr[1 + which * W + (wd - f)] >> (cc % 64)) & 1u;
+                    };
+                    while (row && col) {
+                        if (bit(row - 1, col - 1, 0)) {
+                            --col;
+                            put(RFA_EDIT_DELETE);
+                            continue;
+                        }
+                        --row;
+                        if (row && bit(row - 1, col - 1, 1)) {
+                            put(RFA_EDIT_INSERT);
+                            continue;
+                        }
+                        --col;
+                        if (A[as + col] != B[bs + row]) put(RFA_EDIT_REPLACE);
+                    }
+                    while (col) {
+                        --col;
+                        put(RFA_EDIT_DELETE);
+                    }
+                    while (row) {
+                        --row;
+                        put(RFA_EDIT_INSERT);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <typename CharT>
+static void launch_ed_levl_c(const EditopsArgs& a, bool walk, uint32_t nblocks, hipStream_t s)
+{
+    if (walk)
+        hipLaunchKernelGGL((k_ed_levl<CharT, true>), dim3(nblocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_ed_levl<CharT, false>), dim3(nblocks), dim3(256), 0, s, a);
+}
+
+hipError_t launch_editops_lev_block(int char_width, const EditopsArgs& a, bool walk, uint32_t nblocks,
+                                    hipStream_t s)
+{
+    if (a.n == 0) return hipSuccess;
+    if (char_width == 1)
+        launch_ed_levl_c<uint8_t>(a, walk, nblocks, s);
+    else if (char_width == 2)
+        launch_ed_levl_c<uint16_t>(a, walk, nblocks, s);
+    else
+        launch_ed_levl_c<uint32_t>(a, walk, nblocks, s);
+    return hipGetLastError();
 }
 
 // hamming_editops with pad (Hamming_impl.hpp:40-59): a replacement at every differing

@@ -357,7 +357,7 @@ struct EdInfo {
     uint32_t la;    // stripped lengths
     uint32_t lb;
     uint32_t count; // operations
-    uint32_t flag;  // 1: the stripped first string exceeds 64 units
+    uint32_t flag;  // 1: past the supported lengths; 2: a Levenshtein pair for the block kernels
 };
 struct EdOp { // the layout of rfa_editop
     int32_t type;
@@ -379,7 +379,16 @@ struct EditopsArgs {
     const uint64_t* row_base; // LCS: first row word of each pair (rows of ceil(la / 64) words)
     uint64_t* counts;         // n + 1 operation counts (the last one zero), scanned into offsets
     uint32_t* err;            // set when a pair exceeds the length limit
+    uint64_t* need;           // Levenshtein block pairs: the largest row area (words) of one pair
+    uint64_t* long_rows;      // per wave of k_ed_levl: long_stride words of rows
+    uint64_t long_stride;
 };
+// Levenshtein editops of a stripped first string of 65..EDITOPS_LEV_BLOCK_MAX units run the
+// reference's block recurrence with its Ukkonen band (Levenshtein_impl.hpp:641-850) when the
+// reference records the whole band (no Hirschberg split, Levenshtein_impl.hpp:1185-1200)
+constexpr uint32_t EDITOPS_LEV_BLOCK_MAX = 4096;
+hipError_t launch_editops_lev_block(int char_width, const EditopsArgs& a, bool walk, uint32_t nblocks,
+                                    hipStream_t s);
 size_t editops_scan_bytes(uint64_t n);
 // offsets[0..n] = exclusive prefix sum of counts[0..n] (counts[n] = 0)
 hipError_t editops_offsets(const uint64_t* counts, uint64_t* offsets, uint64_t n, void* temp, size_t bytes,

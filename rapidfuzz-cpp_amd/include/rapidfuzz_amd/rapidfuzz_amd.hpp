@@ -1096,9 +1096,10 @@ Editops editops_of(int metric, It1 first1, It1 last1, It2 first2, It2 last2)
 
 } // namespace detail
 
-// levenshtein_editops (Levenshtein.hpp:282-295): pairs whose first string has at most 64
-// code units once the common prefix and suffix are removed; score_hint does not change
-// the result
+// levenshtein_editops (Levenshtein.hpp:282-295): pairs whose first string has at most 4096
+// code units once the common prefix and suffix are removed, and that the reference does
+// not split by Hirschberg (|s1| * |s2| < 2^22 or |s2| < 10 past 64 units); score_hint
+// does not change the result
 template <typename InputIt1, typename InputIt2>
 Editops levenshtein_editops(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2,
                             size_t score_hint = std::numeric_limits<size_t>::max())

@@ -4,8 +4,10 @@ Hamming_impl.hpp:40-59), pinned:
 
 * against the compiled reference, operation for operation: Levenshtein pairs whose
   first string has at most 64 units after the common affix is removed (where the
-  reference walks the full single-word matrix), LCS pairs up to 4096 units (its
-  single-word, unrolled and blockwise matrices);
+  reference walks the full single-word matrix) and longer ones up to 4096 units below
+  its Hirschberg split (where it walks the band of its block recurrence; the oracle
+  restates that band, and its walk is also checked against the full-matrix walk),
+  LCS pairs up to 4096 units (its single-word, unrolled and blockwise matrices);
 * by what any correct script satisfies: applied to s1 it gives s2 (the reference's
   own editops tests, tests-Levenshtein.cpp:235-316 and tests-Indel.cpp:275-285, are
   of this form and their strings are used here), its length is the distance, and
@@ -85,6 +87,46 @@ def test_levenshtein_vs_reference(oracle, reference):
             pairs.append((s1, s2))
             pairs.append((s1, _gen.mutate(rng, s1, int(rng.integers(0, 8)), alpha_n=alpha)))
     _check_vs_reference(oracle, reference, LEV, pairs)
+
+
+def _long_lev_pairs(rng, count):
+    out = []
+    for it in range(count):
+        alpha = int(rng.choice([2, 3, 4, 26]))
+        n1 = int(rng.choice([int(rng.integers(65, 200)), int(rng.integers(200, 1500)), int(rng.integers(1500, 4097))]))
+        s1 = rng.integers(97, 97 + alpha, size=n1).astype(np.uint32)
+        kind = it % 4
+        if kind == 0:
+            s2 = _gen.mutate(rng, s1, int(rng.integers(0, max(1, n1 // 3))), alpha_n=alpha)
+        elif kind == 1:
+            s2 = rng.integers(97, 97 + alpha, size=int(rng.integers(1, 1000))).astype(np.uint32)
+        elif kind == 2:
+            pre = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 300))).astype(np.uint32)
+            s2 = np.concatenate([pre, _gen.mutate(rng, s1, 5, alpha_n=alpha)])
+        else:  # a second string under 10 units: never split
+            s2 = rng.integers(97, 97 + alpha, size=int(rng.integers(1, 10))).astype(np.uint32)
+        if (2 * len(s1) * len(s2)) // 8 < 1048576 or len(s2) < 10:  # below the Hirschberg split
+            out.append((s1, s2))
+    return out
+
+
+def test_levenshtein_long_vs_reference(oracle, reference):
+    """Stripped first strings of 65-4096 units: the block band restated, against the reference,
+    and the full-matrix walk (independent of any band) giving the same script."""
+    rng = np.random.default_rng(23)
+    pairs = _long_lev_pairs(rng, 160)
+    assert len(pairs) > 100
+    _check_vs_reference(oracle, reference, LEV, pairs)
+    oracle.set_lev_block(False)
+    try:
+        for s1, s2 in pairs[:60]:
+            full = oracle.editops(LEV, s1, s2)
+            oracle.set_lev_block(True)
+            band = oracle.editops(LEV, s1, s2)
+            oracle.set_lev_block(False)
+            assert all(np.array_equal(x, y) for x, y in zip(full, band))
+    finally:
+        oracle.set_lev_block(True)
 
 
 def test_hamming_vs_reference(oracle, reference):

@@ -1,6 +1,7 @@
 """GPU parity of rfa_editops (levenshtein_editops, lcs_seq_editops / indel_editops)
-against the oracle, operation for operation: random pairs around the 64-unit limit
-of the stripped first string (Levenshtein; LCS up to 4096 units over several words),
+against the oracle, operation for operation: random pairs around the 64-unit single
+word of the stripped first string (Levenshtein past it up to 4096 units through the
+block band, LCS up to 4096 units over several words),
 long second strings (several 64-unit chunks of rows),
 common affixes, empty strings, 8/16/32-bit units, and the reference's editops test
 strings; every script also rebuilds the second string."""
@@ -79,6 +80,28 @@ def test_lcs_long_first_strings(ctx, oracle, metric):
     check(ctx, oracle, metric, pairs)
 
 
+@pytest.mark.parametrize("width", [1, 2])
+def test_levenshtein_long_first_strings(ctx, oracle, width):
+    """Levenshtein editops past 64 stripped units: the reference's block band (first strings
+    of 65-4096 units below its Hirschberg split, second strings under 10 units at any length)."""
+    from test_editops_oracle import _long_lev_pairs
+
+    rng = np.random.default_rng(70 + width)
+    pairs = _long_lev_pairs(rng, 120)
+    # word boundaries, common affixes around a long core, both orders
+    for n1 in (65, 127, 128, 129, 191, 192, 1023, 1024, 1025):
+        s1 = rng.integers(97, 100, size=n1).astype(np.uint32)
+        pairs.append((s1, _gen.mutate(rng, s1, int(rng.integers(1, 12)), alpha_n=3)))
+        pairs.append((_gen.mutate(rng, s1, 3, alpha_n=3), s1))
+    core = rng.integers(97, 100, size=300).astype(np.uint32)
+    pairs.append((np.concatenate([_codes("prefix-"), core, _codes("-suffix")]),
+                  np.concatenate([_codes("prefix-"), _gen.mutate(rng, core, 20, alpha_n=3), _codes("-suffix")])))
+    pairs += _pairs(rng, 4, n=30)  # short pairs in the same batch
+    if width == 2:
+        pairs = [(x + 0x4E00 - 97, y + 0x4E00 - 97) for x, y in pairs]
+    check(ctx, oracle, LEV, pairs, width=width)
+
+
 def test_hamming_long(ctx, oracle):
     """Hamming editops have no length limit: strings of up to 5000 units."""
     rng = np.random.default_rng(61)
@@ -98,10 +121,14 @@ def test_sizes_then_ops_and_limits(ctx, oracle):
     assert rfa._lib.rfa_editops(ctx.h, a.h, b.h, 0, off.ctypes.data, small.ctypes.data, 1) == 1  # too small
     assert int(off[2]) == len(oracle.editops(0, b"Lorem ipsum.", b"XYZLorem ABC iPsum")[0]) + \
         len(oracle.editops(0, b"aa", b"abb")[0])
-    long_a = ctx.strings([np.full(70, 97, np.uint32)], width=1)
-    long_b = ctx.strings([np.full(70, 98, np.uint32)], width=1)
+    long_a = ctx.strings([np.full(2100, 97, np.uint32)], width=1)
+    long_b = ctx.strings([np.full(2100, 98, np.uint32)], width=1)
     with pytest.raises(rfa.RfaError):
-        rfa.editops(ctx, long_a, long_b, 0)  # Levenshtein: more than 64 units after the affixes
+        rfa.editops(ctx, long_a, long_b, 0)  # Levenshtein: the reference splits this (Hirschberg)
+    over_a = ctx.strings([np.full(4097, 97, np.uint32)], width=1)
+    over_b = ctx.strings([np.full(5, 98, np.uint32)], width=1)
+    with pytest.raises(rfa.RfaError):
+        rfa.editops(ctx, over_a, over_b, 0)  # Levenshtein: more than 4096 units after the affixes
     huge_a = ctx.strings([np.full(4097, 97, np.uint32)], width=1)
     huge_b = ctx.strings([np.full(10, 98, np.uint32)], width=1)
     with pytest.raises(rfa.RfaError):
