@@ -615,10 +615,9 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "token_set_ratio and token_ratio have no Multi* scorer");
     if (raw->width != sc->q->width)
         throw ApiError(RFA_ERR_INVALID_ARGUMENT, "queries and choices must have the same char width");
-    if (!sc->block_queries.empty())
+    if (!sc->block_queries.empty() && sc->block_max_len > TSET_LONG_MAX)
         throw ApiError(RFA_ERR_UNSUPPORTED,
-                       "the token-set ratios take queries whose deduplicated token join has at most 64 code units "
-                       "(and tiles of at most 254 distinct symbols)");
+                       "the token-set ratios take queries whose deduplicated token join has at most 4096 code units");
     const uint64_t nq = sc->q->count, nc = raw->count;
     if (nq == 0 || nc == 0) return;
     if (nc > 0xffffffffull || nq > 0xffffffffull) throw ApiError(RFA_ERR_UNSUPPORTED, "more than 2^32 strings");
@@ -760,6 +759,51 @@ void cdist_tset(rfa_scorer* sc, const rfa_strings* raw, int kind, int semantics,
             nbx = (ta.ncsel + cpb - 1) / cpb;
             HIP_CHECK(launch_tset(sc->q->width, tc.word_bits, a, tc.ntiles, nbx, s));
         }
+    }
+    if (!sc->block_queries.empty()) {
+        // the queries the tile plan does not take: one wave per (query, choice)
+        TsetLongArgs la{};
+        PartialLongArgs& pa = la.p;
+        pa.qchars = sc->q->d_chars;
+        pa.qoff = sc->q->d_off;
+        pa.qlist = sc->d_block_queries;
+        pa.nql = (uint32_t)sc->block_queries.size();
+        pa.cchars = ch->d_chars;
+        pa.coff = ch->d_off;
+        pa.nc = (uint32_t)nc;
+        pa.out_stride = nc;
+        pa.out = (double*)dout;
+        pa.cutoff = cd;
+        pa.pcut = ctx->pcut;
+        la.q_set_off = sc->q_tokens->d_off;
+        la.q_set_start = sc->q_tokens->d_start;
+        la.q_set_len = sc->q_tokens->d_len;
+        la.q_set_hash = sc->q_tokens->d_hash;
+        la.c_set_off = ct.d_off;
+        la.c_set_start = ct.d_start;
+        la.c_set_len = ct.d_len;
+        la.c_set_hash = ct.d_hash;
+        la.mode = mode;
+        la.aux = aux;
+        const int width = sc->q->width;
+        constexpr uint64_t LDS_BUDGET = 40 * 1024;
+        const uint64_t need = block_arena_bytes(width, (uint32_t)sc->block_max_len, sc->block_u8_rows);
+        const uint64_t items = (uint64_t)pa.nql * nc;
+        uint32_t nblocks;
+        if (need <= LDS_BUDGET) {
+            pa.lds_bytes = (uint32_t)need;
+            const uint32_t waves_per_cu =
+                (uint32_t)std::min<uint64_t>(16, std::max<uint64_t>(1, ctx->lds_per_cu / (need + 1024)));
+            nblocks = (uint32_t)std::min<uint64_t>(items, (uint64_t)ctx->num_cus * waves_per_cu);
+        }
+        else {
+            constexpr uint64_t ARENA_TOTAL = 2ull << 30;
+            const uint64_t cap = std::max<uint64_t>((uint64_t)ctx->num_cus, ARENA_TOTAL / need);
+            nblocks = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(items, (uint64_t)ctx->num_cus * 8), cap);
+            pa.pm_scratch_words = (need + 7) / 8;
+            pa.pm_scratch = (uint64_t*)ctx->pm_scratch.get(pa.pm_scratch_words * 8 * std::max<uint32_t>(1, nblocks));
+        }
+        if (items) HIP_CHECK(launch_tset_long(width, la, std::max<uint32_t>(1, nblocks), s));
     }
     if (!out_on_device) {
         HIP_CHECK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));

@@ -284,6 +284,24 @@ hipError_t launch_scatter_rows(double* dst, uint64_t stride, const double* src, 
                                hipStream_t s);
 hipError_t launch_tset(int char_width, int word_bits, TsetArgs a, uint32_t ntiles, uint32_t nblocks_x,
                        hipStream_t s);
+// The token-set ratios for queries the tile plan does not take (deduplicated joins of
+// 65..TSET_LONG_MAX units, or tiles of too many symbols): one wave per (query, choice),
+// the query's join as a pattern of up to 64 words across the lanes (k_tset_long).
+constexpr uint32_t TSET_LONG_MAX = 4096;
+struct TsetLongArgs {
+    PartialLongArgs p; // the long queries, the choices, out, cutoffs and the pattern arenas
+    const uint64_t* q_set_off;
+    const uint32_t* q_set_start;
+    const uint32_t* q_set_len;
+    const uint64_t* q_set_hash;
+    const uint64_t* c_set_off;
+    const uint32_t* c_set_start;
+    const uint32_t* c_set_len;
+    const uint64_t* c_set_hash;
+    int mode;
+    const double* aux;
+};
+hipError_t launch_tset_long(int char_width, const TsetLongArgs& a, uint32_t nblocks, hipStream_t s);
 
 // fuzz::WRatio's combining steps over an nq x nc matrix (rfa_wratio.hip): stage 0 turns the
 // ratio R into the first raised cutoffs C; stage 1 folds partial_ratio X into R and C for

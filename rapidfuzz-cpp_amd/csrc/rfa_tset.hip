@@ -25,6 +25,7 @@
 // Bloom bit is set, and tokens whose hashes match are compared unit by unit.
 #include <cmath>
 
+#include "rfa_block_impl.hpp"
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
 #include "rfa_semantics.hpp"
@@ -224,6 +225,234 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tset(TsetArgs a)
             *dst = r;
         }
     }
+}
+
+// ---- long queries: one wave per (query, choice) -------------------------------
+//
+// The query's join U_q (65..4096 units) is a pattern of W <= 64 words, word w in
+// lane w, its PM rows built once per query into the workgroup's arena (LDS or
+// scratch, build_pattern).  Per choice:
+//   1. the query's tokens, 64 at a time one per lane, looked up among the choice's
+//      (hash, length, then units): sect and ab sums, and the mask of join(ab) in U_q
+//      (a kept token's units and, after the first kept token, the space before it),
+//      set bit by bit into an LDS word per lane;
+//   2. the choice's tokens the same way against the query's: ba sums, kept counts;
+//   3. (token_set_ratio / token_ratio, unless settled) LCS(join(ab), join(ba)): U_c
+//      streamed unit by unit, removed units stepping with no match, the kept units'
+//      PM words masked; the words' additions carry across lanes by lookahead on two
+//      ballots;
+//   4. the ratios as the tile kernel forms them.
+
+template <typename CharT>
+__device__ __forceinline__ bool ts_tok_eq(const CharT* __restrict__ x, const CharT* __restrict__ y, uint32_t len)
+{
+    for (uint32_t i = 0; i < len; ++i)
+        if (x[i] != y[i]) return false;
+    return true;
+}
+
+template <typename CharT, bool LDS, int MODE>
+__global__ __launch_bounds__(64) void k_tset_long(TsetLongArgs a)
+{
+    extern __shared__ uint64_t smem_u64[];
+    __shared__ uint64_t abmask[64]; // join(ab)'s units in U_q, word w at [w]
+    const PartialLongArgs& pa = a.p;
+    uint8_t* arena = LDS ? (uint8_t*)smem_u64 : (uint8_t*)(pa.pm_scratch + (size_t)blockIdx.x * pa.pm_scratch_words);
+    const CharT* __restrict__ Q = (const CharT*)pa.qchars;
+    const CharT* __restrict__ C = (const CharT*)pa.cchars;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t total = (uint64_t)pa.nql * pa.nc;
+    const uint64_t lo = total * blockIdx.x / gridDim.x, hi = total * (blockIdx.x + 1) / gridDim.x;
+    uint32_t prev = 0xffffffffu;
+    Pattern<CharT, LDS> P{};
+    for (uint64_t it = lo; it < hi; ++it) {
+        const uint32_t qk = (uint32_t)(it / pa.nc), ci = (uint32_t)(it % pa.nc);
+        const uint32_t qi = pa.qlist[qk];
+        const uint64_t qo = pa.qoff[qi];
+        const uint32_t m = uniform((uint32_t)(pa.qoff[qi + 1] - qo));
+        if (qi != prev) {
+            P = build_pattern<CharT, LDS>(Q + qo, m, arena, pattern_hash_bits<CharT>(m));
+            prev = qi;
+        }
+        const uint64_t co = pa.coff[ci];
+        const double cut = pa.pcut ? pa.pcut[(uint64_t)qi * pa.out_stride + ci] : pa.cutoff;
+        const uint64_t tq0 = a.q_set_off[qi], tc0 = a.c_set_off[ci];
+        const uint32_t Tq = uniform((uint32_t)(a.q_set_off[qi + 1] - tq0));
+        const uint32_t Tc = uniform((uint32_t)(a.c_set_off[ci + 1] - tc0));
+        double v = 0.0;
+        bool perfect = false, common = false;
+        if (cut <= 100) {
+            // ---- 1. the query's tokens ----
+            abmask[lane] = 0;
+            __syncthreads();
+            uint64_t ab_units = 0, sect_units = 0;
+            uint32_t n_kept_q = 0, n_inter = 0, first_kept = 0xffffffffu;
+            for (uint32_t k0 = 0; k0 < Tq; k0 += 64) {
+                const uint32_t k = k0 + lane;
+                const bool in = k < Tq;
+                const uint32_t st = in ? a.q_set_start[tq0 + k] : 0u, ln = in ? a.q_set_len[tq0 + k] : 0u;
+                bool found = false;
+                if (in) {
+                    const uint64_t h = a.q_set_hash[tq0 + k];
+                    for (uint32_t t = 0; t < Tc && !found; ++t)
+                        found = a.c_set_hash[tc0 + t] == h && a.c_set_len[tc0 + t] == ln &&
+                                ts_tok_eq(Q + qo + st, C + co + a.c_set_start[tc0 + t], ln);
+                }
+                const bool kept = in && !found;
+                const uint64_t km = ballot(kept);
+                if (first_kept == 0xffffffffu && km) first_kept = k0 + (uint32_t)__builtin_ctzll(km);
+                n_kept_q += (uint32_t)__popcll(km);
+                n_inter += (uint32_t)__popcll(ballot(found));
+                if (kept) {
+                    ab_units += ln;
+                    // the token's units, and the space before it after the first kept token
+                    const uint32_t b0 = k != first_kept ? st - 1 : st, b1 = st + ln;
+                    for (uint32_t b = b0; b < b1;) {
+                        const uint32_t w = b / 64, lo_b = b % 64, nb = min(64u - lo_b, b1 - b);
+                        const uint64_t bits = (nb == 64 ? ~0ull : ((1ull << nb) - 1)) << lo_b;
+                        atomicOr((unsigned long long*)&abmask[w], (unsigned long long)bits);
+                        b += nb;
+                    }
+                }
+                else if (found) {
+                    sect_units += ln;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                ab_units += (uint64_t)__shfl_xor((long long)ab_units, off, 64);
+                sect_units += (uint64_t)__shfl_xor((long long)sect_units, off, 64);
+            }
+            // ---- 2. the choice's tokens ----
+            uint64_t ba_units = 0;
+            uint32_t n_kept_c = 0;
+            for (uint32_t t0 = 0; t0 < Tc; t0 += 64) {
+                const uint32_t t = t0 + lane;
+                const bool in = t < Tc;
+                bool found = false;
+                uint32_t ln = 0;
+                if (in) {
+                    const uint64_t h = a.c_set_hash[tc0 + t];
+                    const uint32_t st = a.c_set_start[tc0 + t];
+                    ln = a.c_set_len[tc0 + t];
+                    for (uint32_t k = 0; k < Tq && !found; ++k)
+                        found = a.q_set_hash[tq0 + k] == h && a.q_set_len[tq0 + k] == ln &&
+                                ts_tok_eq(C + co + st, Q + qo + a.q_set_start[tq0 + k], ln);
+                }
+                const bool kept = in && !found;
+                n_kept_c += (uint32_t)__popcll(ballot(kept));
+                if (kept) ba_units += ln;
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) ba_units += (uint64_t)__shfl_xor((long long)ba_units, off, 64);
+            const uint64_t ab_len = ab_units + (n_kept_q ? n_kept_q - 1 : 0);
+            const uint64_t ba_len = ba_units + (n_kept_c ? n_kept_c - 1 : 0);
+            const uint64_t sect_len = sect_units + (n_inter ? n_inter - 1 : 0);
+            perfect = n_inter > 0 && (n_kept_q == 0 || n_kept_c == 0);
+            common = n_inter > 0;
+            // ---- 3. LCS(join(ab), join(ba)) ----
+            if (MODE < 2 && !perfect) {
+                __syncthreads();
+                const uint32_t W = P.W;
+                const bool act = lane < W;
+                const uint64_t mw = act ? abmask[lane] : 0ull;
+                uint64_t S = ~0ull;
+                bool seen = false;
+                auto step = [&](uint32_t unit, bool use) {
+                    uint64_t X = 0;
+                    if (use) {
+                        const uint32_t r = P.map.find(unit);
+                        X = act ? (P.pm[(size_t)r * P.Wp + lane] & mw) : 0ull;
+                    }
+                    const uint64_t u = S & X;
+                    uint64_t x = S + u;
+                    const bool g = act && (x < S);
+                    const bool pr = act && !g && (x == ~0ull);
+                    const uint64_t G = ballot(g), Pp = ballot(pr);
+                    const uint64_t sum = (G | Pp) + G;
+                    x += ((sum ^ Pp) >> lane) & 1u;
+                    if (act) S = x | (S - u);
+                };
+                for (uint32_t t0 = 0; t0 < Tc; t0 += 64) {
+                    const uint32_t t = t0 + lane;
+                    const bool in = t < Tc;
+                    bool found = false;
+                    uint32_t st = 0, ln = 0;
+                    if (in) {
+                        const uint64_t h = a.c_set_hash[tc0 + t];
+                        st = a.c_set_start[tc0 + t];
+                        ln = a.c_set_len[tc0 + t];
+                        for (uint32_t k = 0; k < Tq && !found; ++k)
+                            found = a.q_set_hash[tq0 + k] == h && a.q_set_len[tq0 + k] == ln &&
+                                    ts_tok_eq(C + co + st, Q + qo + a.q_set_start[tq0 + k], ln);
+                    }
+                    const uint64_t keptm = ballot(in && !found);
+                    const uint32_t cnt = min(64u, Tc - t0);
+                    for (uint32_t l = 0; l < cnt; ++l) {
+                        const bool kept = (keptm >> l) & 1u;
+                        const uint32_t tst = readlane(st, l), tln = readlane(ln, l);
+                        if (t0 + l > 0) step((uint32_t)C[co + tst - 1], kept && seen);
+                        for (uint32_t i = 0; i < tln; ++i) step((uint32_t)C[co + tst + i], kept);
+                        seen = seen || kept;
+                    }
+                }
+                const uint64_t valid = !act ? 0ull : (m - 64ull * lane >= 64 ? ~0ull : ((1ull << (m - 64ull * lane)) - 1));
+                uint32_t lcs = (uint32_t)__popcll(~S & valid);
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) lcs += (uint32_t)__shfl_xor((int)lcs, off, 64);
+                const uint64_t dist = ab_len + ba_len - 2 * (uint64_t)lcs;
+                const uint64_t sect_ab_len = sect_len + (sect_len != 0) + ab_len;
+                const uint64_t sect_ba_len = sect_len + (sect_len != 0) + ba_len;
+                const uint64_t lensum = sect_ab_len + sect_ba_len;
+                const uint64_t cutoff_distance = (uint64_t)ceil((double)lensum * (1.0 - cut / 100));
+                if (dist <= cutoff_distance) v = fz_norm_distance(dist, lensum, cut);
+                if (sect_len) {
+                    v = dmax(v, fz_norm_distance((sect_len != 0) + ab_len, sect_len + sect_ab_len, cut));
+                    v = dmax(v, fz_norm_distance((sect_len != 0) + ba_len, sect_len + sect_ba_len, cut));
+                }
+            }
+            __syncthreads(); // abmask is cleared for the next item
+        }
+        if (lane == 0) {
+            double* dst = pa.out + (uint64_t)qi * pa.out_stride + ci;
+            double r;
+            if (cut > 100)
+                r = 0.0;
+            else if (MODE == 0)
+                r = (Tq == 0 || Tc == 0) ? 0.0 : (perfect ? 100.0 : v);
+            else if (MODE == 1)
+                r = perfect ? 100.0 : dmax(*dst, v);
+            else if (MODE == 2)
+                r = (Tq == 0 || Tc == 0) ? 0.0 : (common ? 100.0 : *dst);
+            else
+                r = common ? 100.0 : dmax(*dst, a.aux[(uint64_t)qi * pa.out_stride + ci]);
+            *dst = r;
+        }
+    }
+}
+
+template <typename CharT, bool LDS>
+static void launch_tset_long_l(const TsetLongArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    const size_t lds = LDS ? a.p.lds_bytes : 0;
+    switch (a.mode) {
+    case 0: hipLaunchKernelGGL((k_tset_long<CharT, LDS, 0>), dim3(nblocks), dim3(64), lds, s, a); break;
+    case 1: hipLaunchKernelGGL((k_tset_long<CharT, LDS, 1>), dim3(nblocks), dim3(64), lds, s, a); break;
+    case 2: hipLaunchKernelGGL((k_tset_long<CharT, LDS, 2>), dim3(nblocks), dim3(64), lds, s, a); break;
+    default: hipLaunchKernelGGL((k_tset_long<CharT, LDS, 3>), dim3(nblocks), dim3(64), lds, s, a); break;
+    }
+}
+
+hipError_t launch_tset_long(int char_width, const TsetLongArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    const bool lds = a.p.lds_bytes != 0;
+    if (char_width == 1)
+        lds ? launch_tset_long_l<uint8_t, true>(a, nblocks, s) : launch_tset_long_l<uint8_t, false>(a, nblocks, s);
+    else if (char_width == 2)
+        lds ? launch_tset_long_l<uint16_t, true>(a, nblocks, s) : launch_tset_long_l<uint16_t, false>(a, nblocks, s);
+    else
+        lds ? launch_tset_long_l<uint32_t, true>(a, nblocks, s) : launch_tset_long_l<uint32_t, false>(a, nblocks, s);
+    return hipGetLastError();
 }
 
 size_t tset_lds_bytes(int char_width, int word_bits, uint32_t rows_cap, uint32_t hash_bits, uint32_t* tok_lds_u64)

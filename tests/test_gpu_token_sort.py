@@ -2,7 +2,7 @@
 token_set_ratio, token_ratio, partial_token_set_ratio and partial_token_ratio (rfa_cdist)
 against the oracle, bit for bit: the device
 token sort (plain and deduplicated) for 8/16/32-bit units and signed 8-bit order, the
-cached token-sorted choice sets and the masked-LCS set kernel."""
+cached token-sorted choice sets, the masked-LCS set kernel and its long-query form."""
 import numpy as np
 import pytest
 
@@ -161,12 +161,61 @@ def test_partial_token_ratio_few_repeats(ctx, oracle, signed8):
     _check(ctx, oracle, PTRATIO, q, c, 1, signed8=signed8, cuts=(0.0, 60.0, 95.0))
 
 
+def _vocab_sentences(rng, width, nwords_q, nwords_c, nq, nc, base=97):
+    """Sentences over a vocabulary of random words: long deduplicated joins, shared tokens."""
+    hi = base + 6
+    vocab = [rng.integers(base, hi, size=int(rng.integers(1, 7))).astype(np.uint32) for _ in range(900)]
+    if width > 1:
+        vocab += [np.array([0x4E2D + i % 5, 0x0101], np.uint32) for i in range(40)]
+
+    def sentence(n):
+        idx = rng.integers(0, len(vocab), size=n)
+        out = []
+        for i, t in enumerate(idx):
+            if i:
+                out.append(32 if i % 7 else 0x09)
+            out.extend(int(x) for x in vocab[int(t)])
+        return np.array(out, np.uint32)
+    q = [sentence(int(rng.integers(*nwords_q))) for _ in range(nq)]
+    c = [sentence(int(rng.integers(*nwords_c))) for _ in range(nc)]
+    # choices sharing many of a query's tokens, in another order, plus some of their own
+    for s in q[:6]:
+        toks = "".join(chr(x) for x in s).split()
+        rng.shuffle(toks)
+        keep = toks[: max(1, len(toks) * 2 // 3)] + ["zz", "qq"]
+        c.append(np.array([ord(ch) for ch in " ".join(keep)], np.uint32))
+    c += [q[1], np.zeros(0, np.uint32)]
+    return q, c
+
+
+@pytest.mark.parametrize("width", [1, 2, 4])
+@pytest.mark.parametrize("metric", [TSET, TRATIO, PTSET, PTRATIO])
+def test_token_sets_long_queries(ctx, oracle, metric, width):
+    """Queries whose deduplicated join has 65-1000 units (the long set kernel: the join as a
+    pattern of several words across the lanes), with short queries in the same scorer."""
+    rng = np.random.default_rng(700 + 7 * metric + width)
+    q, c = _vocab_sentences(rng, width, (20, 250), (0, 60), 10, 40)
+    q = [x for x in q if 64 < len(x) <= 1000] + [q[0][:30]]
+    _check(ctx, oracle, metric, q, c, width, cuts=(0.0, 70.0, 100.5))
+
+
+def test_token_set_ratio_very_long_queries(ctx, oracle):
+    """token_set_ratio / token_ratio with deduplicated joins of 1,000-4,000 units (up to 64
+    pattern words), against choices of up to 2,000 units."""
+    rng = np.random.default_rng(720)
+    q, c = _vocab_sentences(rng, 1, (300, 900), (0, 500), 5, 20)
+    q = [x for x in q if len(x) <= 4096]
+    for metric in (TSET, TRATIO):
+        _check(ctx, oracle, metric, q, c, 1, cuts=(0.0, 80.0))
+
+
 def test_token_set_limits(ctx):
     import rapidfuzz_amd as rfa
 
-    long_q = rfa.Scorer(ctx, TSET, ctx.strings([b"alpha beta gamma delta epsilon zeta eta theta iota kappa lambda mu"]))
+    words = " ".join(f"w{i:04d}" for i in range(900)).encode()  # a join of 5,399 units
+    long_q = rfa.Scorer(ctx, TSET, ctx.strings([words]))
     with pytest.raises(rfa.RfaError):
-        long_q.cdist(ctx.strings([b"beta"]), SIM)  # deduplicated join over 64 units
+        long_q.cdist(ctx.strings([b"w0001"]), SIM)  # deduplicated join over 4096 units
     sc = rfa.Scorer(ctx, TRATIO, ctx.strings([b"a b"]))
     with pytest.raises(rfa.InvalidArgument):
         sc.topk(ctx.strings([b"b a"]), 1, kind=3)  # similarity only
