@@ -1258,9 +1258,9 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         }
         return;
     }
-    // per-pair cutoffs (WRatio's sub-scores) run on the tile kernels only
-    if (ctx->pcut && (algo != ALGO_LCS || !sc->block_queries.empty() || semantics != RFA_CACHED))
-        throw ApiError(RFA_ERR_UNSUPPORTED, "WRatio takes queries of at most 64 code units");
+    // per-pair cutoffs (WRatio's sub-scores): the LCS kinds, which apply them after an exact LCS
+    if (ctx->pcut && (algo != ALGO_LCS || semantics != RFA_CACHED))
+        throw ApiError(RFA_ERR_UNSUPPORTED, "per-pair cutoffs are for the ratio scorers");
     // a few queries against a set: one pair per lane (choices across lanes)
     if (!ctx->pcut && cdist_few_queries(sc, ch, spec, algo, dout, d_qcut)) {
         if (!out_on_device) {
@@ -1352,6 +1352,7 @@ void cdist_impl(rfa_scorer* sc, const rfa_strings* ch, int kind, int semantics, 
         ba.spec = spec;
         ba.out = dout;
         ba.qcut = d_qcut;
+        ba.pcut = ctx->pcut; // indexed like out: query * nc + choice
         // the q-gram filter can reject pairs only where 7k + 7 < m
         if (kind == RFA_DISTANCE)
             for (uint32_t qi : rest_queries) {

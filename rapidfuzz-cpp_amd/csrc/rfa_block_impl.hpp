@@ -900,6 +900,7 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a)
             spec.cutoff_u = a.qcut[ia];
             spec.band_k = band_from_cutoff(spec);
         }
+        if (a.pcut) spec.cutoff_d = a.pcut[oidx]; // the LCS kinds use it only in the filter and epilogue
         const uint64_t oa = a.aoff[ia], ob = a.boff[ib];
         const uint32_t la = (uint32_t)(a.aoff[ia + 1] - oa), lb = (uint32_t)(a.boff[ib + 1] - ob);
         const CharT* pp = swap ? B + ob : A + oa;

@@ -79,6 +79,7 @@ struct BlockArgs {
     const uint32_t* nitems_dev;
     const uint8_t* pair_metric; // metric per pair (RFA_LEVENSHTEIN .. RFA_OSA)
     const uint64_t* qcut; // cross mode: integer cutoff per string of set A (nullptr: spec.cutoff_u)
+    const double* pcut;   // cross mode: double cutoff per item at its output index (nullptr: spec.cutoff_d)
 };
 
 // Largest item count of one block-kernel launch: the 32-bit queue head runs past the

@@ -1,6 +1,6 @@
 """GPU parity of fuzz::WRatio (rfa_cdist, rfa_topk) against the oracle, bit for bit: the
 ratio, the token ratio or the partial ratio and partial token ratio launched with the
-cutoffs WRatio raises per pair, and the combining steps."""
+cutoffs WRatio raises per pair, and the combining steps; short and long queries."""
 import numpy as np
 import pytest
 
@@ -70,6 +70,20 @@ def test_sentences_and_branches(ctx, oracle):
     cuts = [0.0] + [float(v) for v in vals[:: max(1, len(vals) // 6)]]
     cuts += [np.nextafter(x, 0.0) for x in cuts[1:3]] + [np.nextafter(x, 200.0) for x in cuts[1:3]]
     _check(ctx, oracle, q, c, 1, cuts=tuple(cuts))
+
+
+@pytest.mark.parametrize("width", [1, 2])
+def test_long_queries(ctx, oracle, width):
+    """Queries of 65-900 units (the sub-scores' long paths: the block kernel's ratio and the
+    long token-set kernel with per-pair cutoffs, the long partial_ratio path) against choices
+    on both sides of 1.5 and 8 times their length, and a few short queries beside them."""
+    from test_gpu_token_sort import _vocab_sentences
+
+    rng = np.random.default_rng(195 + width)
+    q, c = _vocab_sentences(rng, width, (15, 180), (1, 150), 10, 40)
+    q = [x for x in q if 64 < len(x) <= 900] + [q[0][:20], q[1][:50]]
+    c += [x[: len(x) // 3] for x in q[:4]] + [np.concatenate([x, x[:40]]) for x in q[:3]]
+    _check(ctx, oracle, q, c, width, cuts=(0.0, 60.0, 90.0, 100.5))
 
 
 def test_topk(ctx, oracle):
