@@ -61,7 +61,7 @@ typedef enum {
     RFA_PARTIAL_TOKEN_SORT_RATIO = 12, /* fuzz::CachedPartialTokenSortRatio (fuzz.hpp:342-364) */
     RFA_QRATIO = 13,                   /* fuzz::CachedQRatio (fuzz.hpp:776-796): ratio, 0 if either string is empty */
     /* token sets (deduplicated sorted tokens): rfa_cdist only; queries whose deduplicated token join
-     * has at most 64 code units */
+     * has at most 4096 code units (1024 for the two partial ones) */
     RFA_TOKEN_SET_RATIO = 14,          /* fuzz::CachedTokenSetRatio (fuzz.hpp:412-434) */
     RFA_TOKEN_RATIO = 15,              /* fuzz::CachedTokenRatio (fuzz.hpp:530-557): max of the token-sort and token-set ratios */
     RFA_PARTIAL_TOKEN_SET_RATIO = 16,  /* fuzz::CachedPartialTokenSetRatio (fuzz.hpp:471-493) */
@@ -70,7 +70,8 @@ typedef enum {
                                           Cached semantics, every kind; strings below 2^29 code units */
     RFA_WRATIO = 19                    /* fuzz::CachedWRatio (fuzz.hpp:659-682, fuzz_impl.hpp:790-876): ratio, then
                                           token_ratio or partial_ratio and partial_token_ratio with cutoffs raised per
-                                          pair; similarity, Cached only; the token ratios' query limit applies */
+                                          pair; similarity, Cached only; the token ratios' and partial_ratio's
+                                          limits apply */
 } rfa_metric;
 
 /* Scorer method (details/distance.hpp:211-301, 326-356). */
