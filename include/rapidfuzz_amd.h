@@ -61,7 +61,7 @@ typedef enum {
     RFA_PARTIAL_TOKEN_SORT_RATIO = 12, /* fuzz::CachedPartialTokenSortRatio (fuzz.hpp:342-364) */
     RFA_QRATIO = 13,                   /* fuzz::CachedQRatio (fuzz.hpp:776-796): ratio, 0 if either string is empty */
     /* token sets (deduplicated sorted tokens): rfa_cdist only; queries whose deduplicated token join
-     * has at most 4096 code units (1024 for the two partial ones) */
+     * has at most 4096 code units */
     RFA_TOKEN_SET_RATIO = 14,          /* fuzz::CachedTokenSetRatio (fuzz.hpp:412-434) */
     RFA_TOKEN_RATIO = 15,              /* fuzz::CachedTokenRatio (fuzz.hpp:530-557): max of the token-sort and token-set ratios */
     RFA_PARTIAL_TOKEN_SET_RATIO = 16,  /* fuzz::CachedPartialTokenSetRatio (fuzz.hpp:471-493) */
@@ -290,7 +290,7 @@ rfa_status rfa_editops(rfa_context* ctx, const rfa_strings* a, const rfa_strings
  * (fuzz.hpp:129-135, fuzz_impl.hpp:196-230) of every pair: the partial ratio and the
  * windows of a[i] (src) and b[i] (dest) it was found at, with the reference's choice
  * among equal windows.  The layout is ScoreAlignment<double>'s (details/types.hpp:567-594).
- * out: count entries in host memory.  A pair whose shorter string has more than 1024
+ * out: count entries in host memory.  A pair whose shorter string has more than 4096
  * code units returns RFA_ERR_UNSUPPORTED. */
 typedef struct {
     double score;

@@ -103,7 +103,7 @@ class Oracle:
             getattr(L, name).restype = ctypes.c_uint64
             getattr(L, name).argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t, ctypes.c_uint64,
                                          ctypes.c_uint64, ctypes.c_uint64]
-        for name in ("orc_lcs_raw", "orc_osa_raw"):
+        for name in ("orc_lcs_raw", "orc_lcs_bits", "orc_osa_raw"):
             getattr(L, name).restype = ctypes.c_uint64
             getattr(L, name).argtypes = [_u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t]
         L.orc_partial_ratio_alignment.restype = None
@@ -119,6 +119,12 @@ class Oracle:
         a = np.ascontiguousarray(np.asarray(s1, dtype=np.uint32))
         b = np.ascontiguousarray(np.asarray(s2, dtype=np.uint32))
         return int(self.lib.orc_lcs_raw(a.ctypes.data_as(_u32p), len(a), b.ctypes.data_as(_u32p), len(b)))
+
+    def lcs_bits(self, s1, s2):
+        """Length of the longest common subsequence (bit-parallel recurrence)."""
+        a = np.ascontiguousarray(np.asarray(s1, dtype=np.uint32))
+        b = np.ascontiguousarray(np.asarray(s2, dtype=np.uint32))
+        return int(self.lib.orc_lcs_bits(a.ctypes.data_as(_u32p), len(a), b.ctypes.data_as(_u32p), len(b)))
 
     def cdist(self, metric, kind, queries, choices, weights=(1, 1, 1), prefix_weight=0.1,
               cutoff_u=None, cutoff_d=None, multi=False, char_width=4, signed8=False):

@@ -352,7 +352,7 @@ static uint64_t hamming_cached_distance(const Pair* p, uint64_t c)
 static uint64_t lcs_cached_similarity(const Pair* p, uint64_t c)
 {
     if (c > p->l1 || c > p->l2) return 0;
-    uint64_t lcs = orc_lcs_raw(p->s1, p->l1, p->s2, p->l2);
+    uint64_t lcs = orc_lcs_bits(p->s1, p->l1, p->s2, p->l2); /* equals orc_lcs_raw's DP */
     return lcs >= c ? lcs : 0;
 }
 
@@ -835,9 +835,62 @@ static double ratio_similarity(const uint32_t* s1, size_t l1, const uint32_t* s2
 }
 
 /* CachedIndel(s1).distance(s2) with the default (unbounded) cutoff */
+static int cmp_u32(const void* x, const void* y)
+{
+    const uint32_t a = *(const uint32_t*)x, b = *(const uint32_t*)y;
+    return a < b ? -1 : (a > b);
+}
+
+/* The LCS length by the bit-parallel recurrence (Hyyro 2004; the reference's
+ * LCSseq_impl.hpp:205-250): S starts all ones, each unit of s2 with match words X
+ * (bit i set where s1[i] equals it) does U = S & X, S = (S + U) | (S - U) over the
+ * words with the addition's carries, and the LCS is the number of clear bits.  The
+ * partial ratios' many windows use it; tests/test_oracle_vs_reference.py checks it
+ * against the DP of orc_lcs_raw. */
+uint64_t orc_lcs_bits(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2)
+{
+    if (!l1 || !l2) return 0;
+    const size_t W = (l1 + 63) / 64;
+    uint32_t* sym = (uint32_t*)malloc(l1 * sizeof(uint32_t));
+    memcpy(sym, s1, l1 * sizeof(uint32_t));
+    qsort(sym, l1, sizeof(uint32_t), cmp_u32);
+    size_t d = 0;
+    for (size_t i = 0; i < l1; ++i)
+        if (i == 0 || sym[i] != sym[d - 1]) sym[d++] = sym[i];
+    uint64_t* pm = (uint64_t*)calloc(d * W, sizeof(uint64_t));
+    for (size_t i = 0; i < l1; ++i) {
+        const uint32_t* k = (const uint32_t*)bsearch(&s1[i], sym, d, sizeof(uint32_t), cmp_u32);
+        pm[(size_t)(k - sym) * W + i / 64] |= 1ull << (i % 64);
+    }
+    uint64_t* S = (uint64_t*)malloc(W * sizeof(uint64_t));
+    for (size_t w = 0; w < W; ++w) S[w] = ~0ull;
+    for (size_t j = 0; j < l2; ++j) {
+        const uint32_t* k = (const uint32_t*)bsearch(&s2[j], sym, d, sizeof(uint32_t), cmp_u32);
+        if (!k) continue; /* no match: S is unchanged */
+        const uint64_t* X = pm + (size_t)(k - sym) * W;
+        uint64_t carry = 0;
+        for (size_t w = 0; w < W; ++w) {
+            const uint64_t u = S[w] & X[w];
+            const uint64_t x = S[w] + u + carry;
+            carry = (x < S[w] || (carry && x == S[w])) ? 1 : 0;
+            S[w] = x | (S[w] - u);
+        }
+    }
+    uint64_t lcs = 0;
+    for (size_t w = 0; w < W; ++w) {
+        const size_t bits = l1 - 64 * w >= 64 ? 64 : l1 - 64 * w;
+        const uint64_t mask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
+        lcs += (uint64_t)__builtin_popcountll(~S[w] & mask);
+    }
+    free(sym);
+    free(pm);
+    free(S);
+    return lcs;
+}
+
 static size_t indel_distance_full(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2)
 {
-    return l1 + l2 - 2 * (size_t)orc_lcs_raw(s1, l1, s2, l2);
+    return l1 + l2 - 2 * (size_t)orc_lcs_bits(s1, l1, s2, l2);
 }
 
 /* CharSet::find of s1's units */

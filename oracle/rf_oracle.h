@@ -54,6 +54,8 @@ enum { ORC_DISTANCE = 0, ORC_SIMILARITY = 1, ORC_NORM_DISTANCE = 2, ORC_NORM_SIM
 /* ---- raw values (naive DP) ---- */
 uint64_t orc_levenshtein_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2,
                              uint64_t ins, uint64_t del, uint64_t rep);
+/* The LCS length by the bit-parallel recurrence (the Indel / LCS scorers' raw value) */
+uint64_t orc_lcs_bits(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 uint64_t orc_lcs_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 uint64_t orc_osa_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);
 uint64_t orc_damerau_raw(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2);

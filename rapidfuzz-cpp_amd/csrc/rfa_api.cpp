@@ -878,8 +878,8 @@ void cdist_partial(rfa_scorer* sc, const rfa_strings* ch, double cd, double* dou
     spec.band_k = ~0ull;
     if (!sc->block_queries.empty()) {
         const uint64_t needle = std::min<uint64_t>(sc->block_max_len, ch->max_len);
-        if ((needle + 63) / 64 > PARTIAL_MAX_WORDS)
-            throw ApiError(RFA_ERR_UNSUPPORTED, "partial_ratio of two strings longer than 1024 code units");
+        if (needle > PBIG_MAX)
+            throw ApiError(RFA_ERR_UNSUPPORTED, "partial_ratio of two strings longer than 4096 code units");
     }
     if (!sc->classes.empty()) {
         TileArgs ta{};
@@ -921,8 +921,8 @@ void cdist_partial(rfa_scorer* sc, const rfa_strings* ch, double cd, double* dou
         pa.out = dout;
         pa.cutoff = cd;
         pa.pcut = ctx->pcut;
-        const uint32_t needle_words =
-            (uint32_t)std::max<uint64_t>(1, (std::min<uint64_t>(sc->block_max_len, ch->max_len) + 63) / 64);
+        const uint32_t needle_words = (uint32_t)std::min<uint64_t>(
+            PARTIAL_MAX_WORDS, std::max<uint64_t>(1, (std::min<uint64_t>(sc->block_max_len, ch->max_len) + 63) / 64));
         constexpr uint64_t LDS_BUDGET = 40 * 1024;
         const uint64_t need = block_arena_bytes(width, (uint32_t)sc->block_max_len, sc->block_u8_rows);
         const uint64_t items = (uint64_t)pa.nql * nc;
@@ -941,6 +941,29 @@ void cdist_partial(rfa_scorer* sc, const rfa_strings* ch, double cd, double* dou
             pa.pm_scratch = (uint64_t*)ctx->pm_scratch.get(pa.pm_scratch_words * 8 * nblocks);
         }
         if (items) HIP_CHECK(launch_plong(width, needle_words, pa, std::max<uint32_t>(1, nblocks), s));
+        if (items && std::min<uint64_t>(sc->block_max_len, ch->max_len) > PRALIGN_MAX) {
+            // pairs whose shorter string has more than 1024 units: one wave per pair
+            PbigArgs ba{};
+            ba.achars = sc->q->d_chars;
+            ba.aoff = sc->q->d_off;
+            ba.bchars = ch->d_chars;
+            ba.boff = ch->d_off;
+            ba.items = sc->d_block_queries;
+            ba.nb = (uint32_t)nc;
+            ba.nitems = items;
+            ba.out_stride = nc;
+            ba.cdist = 1;
+            ba.cutoff = cd;
+            ba.pcut = ctx->pcut;
+            ba.score_out = (double*)dout;
+            const uint64_t l1max = std::min<uint64_t>(sc->block_max_len, ch->max_len);
+            const uint64_t l2max = std::max<uint64_t>(sc->block_max_len, ch->max_len);
+            ba.ws_stride = pbig_ws_bytes(width, l1max, l1max + l2max);
+            const uint32_t nb = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(std::min<uint64_t>(items, (uint64_t)ctx->num_cus * 4), (1ull << 30) / ba.ws_stride));
+            ba.ws = (uint8_t*)ctx->pb_ws.get(nb * ba.ws_stride);
+            HIP_CHECK(launch_pbig(width, ba, nb, s));
+        }
     }
 }
 
@@ -1713,7 +1736,7 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
                           &ctx->plan_vals, &ctx->plan_temp, &ctx->topk_scores, &ctx->topk_run[0],
                           &ctx->topk_run[1], &ctx->topk_cand_s, &ctx->topk_cand_i, &ctx->topk_counts, &ctx->topk_q,
                           &ctx->topk_out, &ctx->qcut, &ctx->ed_items, &ctx->ed_rows, &ctx->ed_ops, &ctx->tset_cols,
-                          &ctx->tset_rows, &ctx->band_q, &ctx->band_w, &ctx->wl_bnd, &ctx->ed_long, &ctx->wr_buf[0],
+                          &ctx->tset_rows, &ctx->band_q, &ctx->band_w, &ctx->wl_bnd, &ctx->ed_long, &ctx->pb_ws, &ctx->wr_buf[0],
                           &ctx->wr_buf[1], &ctx->wr_buf[2]})
             b->release();
         ctx->staging.release();

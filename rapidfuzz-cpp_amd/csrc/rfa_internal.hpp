@@ -456,6 +456,65 @@ __host__ __device__ inline uint64_t pralign_ws_bytes(uint64_t l1, uint64_t l2, b
 // words: 1, 2, 4, 8 or 16 (the needle's 64-unit words, rounded up)
 hipError_t launch_pralign(int char_width, uint32_t words, const PrAlignArgs& a, hipStream_t s);
 
+// Needles of PRALIGN_MAX + 1 .. PBIG_MAX units (k_pbig, one wave per pair): the needle and
+// its reversal as multi-word patterns (one word per lane), a row index per haystack unit,
+// the reference's bisection replayed window by window, prefixes and suffixes recorded in
+// one pass each.  Per-wave workspace in scratch (pbig_ws_bytes).
+constexpr uint32_t PBIG_MAX = 4096;
+__host__ __device__ inline uint64_t pbig_arena_bytes(int char_width, uint64_t m)
+{
+    const uint64_t alpha = char_width == 1 ? 256 : (char_width == 2 ? 65536 : 0xffffffffull);
+    const uint64_t d = m < alpha ? m : alpha;
+    uint64_t map = 1024;
+    if (char_width != 1) {
+        uint32_t b = 6;
+        while ((1ull << b) < 2 * d) ++b;
+        map = (1ull << b) * 12 + 8;
+    }
+    return (map + (d + 1) * ((m + 63) / 64) * 8 + 16 + 255) / 256 * 256;
+}
+// offsets (bytes) of the workspace parts: arena, reversed arena, reversed needle, row index,
+// reversed row index, window distances, two range lists, prefix and suffix LCS; [9] = total
+__host__ __device__ inline void pbig_layout(int char_width, uint64_t l1, uint64_t l2, uint64_t* o)
+{
+    const uint64_t F = l2 - l1, ab = pbig_arena_bytes(char_width, l1);
+    auto r = [](uint64_t x) { return (x + 255) / 256 * 256; };
+    o[0] = 0;
+    o[1] = ab;
+    o[2] = 2 * ab;
+    o[3] = o[2] + r(l1 * (uint64_t)char_width);
+    o[4] = o[3] + r(l2 * 4);
+    o[5] = o[4] + r(l1 * 4);
+    o[6] = o[5] + r(F * 4);
+    o[7] = o[6] + r((2 * F + 2) * 2 * 8);
+    o[8] = o[7] + r(l1 * 4);
+    o[9] = o[8] + r(l1 * 4);
+}
+__host__ __device__ inline uint64_t pbig_ws_bytes(int char_width, uint64_t l1, uint64_t l2)
+{
+    uint64_t o[10];
+    pbig_layout(char_width, l1, l2, o);
+    return o[9];
+}
+struct PbigArgs {
+    const void* achars; // alignment: pairs (a[i], b[i]); cdist: queries (a) x choices (b)
+    const uint64_t* aoff;
+    const void* bchars;
+    const uint64_t* boff;
+    const uint32_t* items; // alignment: the pairs; cdist: the long queries (item = qk * nb + choice)
+    uint64_t nitems;
+    uint32_t nb;           // cdist: choices per query
+    uint64_t out_stride;
+    int cdist;             // 1: out[q * out_stride + c] = partial_ratio score (double)
+    double cutoff;
+    const double* pcut;    // cdist: per-pair cutoff (nullptr: cutoff)
+    double* score_out;     // cdist output
+    uint64_t* al_out;      // alignment output: 5 words per pair (rfa_score_alignment)
+    uint8_t* ws;           // per-wave workspace of ws_stride bytes
+    uint64_t ws_stride;
+};
+hipError_t launch_pbig(int char_width, const PbigArgs& a, uint32_t nblocks, hipStream_t s);
+
 // Widest shorter string (in 64-unit words) the long path takes.
 constexpr uint32_t PARTIAL_MAX_WORDS = 16;
 hipError_t launch_plong(int char_width, uint32_t needle_words, const PartialLongArgs& a, uint32_t nblocks,

@@ -458,6 +458,7 @@ __global__ __launch_bounds__(64) void k_plong(PartialLongArgs a)
         }
         const uint64_t co = a.coff[ci];
         const uint32_t n = (uint32_t)(a.coff[ci + 1] - co);
+        if (min(m, n) > 64u * WMAX) continue; // a longer needle: k_pbig's pair
         const double cut = a.pcut ? a.pcut[(uint64_t)qi * a.out_stride + ci] : a.cutoff;
         double v;
         if (cut > 100) {

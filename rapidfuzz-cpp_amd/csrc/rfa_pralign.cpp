@@ -32,6 +32,8 @@ void pralign_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
     uint64_t cnt[NC] = {0, 0, 0, 0, 0};
     uint32_t lds[NC] = {0, 0, 0, 0, 0};
     std::vector<uint64_t> trivial;
+    std::vector<uint32_t> big;
+    uint64_t big_ws = 0;
     for (uint64_t i = 0; i < n; ++i) {
         const uint64_t la = a->len(i), lb = b->len(i);
         const uint64_t l1 = std::min(la, lb), l2 = std::max(la, lb);
@@ -40,8 +42,14 @@ void pralign_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
             trivial.push_back(i);
             continue;
         }
-        if (l1 > PRALIGN_MAX)
-            throw ApiError(RFA_ERR_UNSUPPORTED, "partial_ratio_alignment of two strings longer than 1024 code units");
+        if (l1 > PBIG_MAX)
+            throw ApiError(RFA_ERR_UNSUPPORTED, "partial_ratio_alignment of two strings longer than 4096 code units");
+        if (l1 > PRALIGN_MAX) { // one wave per pair, k_pbig
+            cls[i] = 0xfe;
+            big.push_back((uint32_t)i);
+            big_ws = std::max<uint64_t>(big_ws, pbig_ws_bytes((int)a->width, l1, l2));
+            continue;
+        }
         const uint64_t w = (l1 + 63) / 64;
         const int c = w <= 1 ? 0 : w <= 2 ? 1 : w <= 4 ? 2 : w <= 8 ? 3 : 4;
         cls[i] = (uint8_t)c;
@@ -51,9 +59,9 @@ void pralign_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
         // wider needles never fit the LDS cap
         if (c <= 2 && ws <= PRALIGN_LDS_CAP) lds[c] = std::max(lds[c], (uint32_t)ws);
     }
-    const uint64_t nitems = n - trivial.size();
+    const uint64_t nitems = n - trivial.size() - big.size();
     hipStream_t s = ctx->stream;
-    if (nitems) {
+    if (nitems || !big.empty()) {
         // items (class-major) and scratch offsets in one pinned upload; the device takes LDS
         // exactly when the workspace fits the class's size (the same test)
         uint64_t first[NC + 1] = {0};
@@ -68,7 +76,7 @@ void pralign_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
         uint64_t total = 0;
         for (uint64_t i = 0; i < n; ++i) {
             const int c = cls[i];
-            if (c == 0xff) continue;
+            if (c >= 0xfe) continue;
             const uint64_t k = pos[c]++;
             hitems[k] = (uint32_t)i;
             hoffs[k] = total;
@@ -77,7 +85,7 @@ void pralign_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
                 total += pralign_ws_bytes(std::min(la, lb), std::max(la, lb), false);
             }
         }
-        const size_t o_out = (up + 255) / 256 * 256;
+        const size_t o_big = (up + 255) / 256 * 256, o_out = o_big + (big.size() * 4 + 255) / 256 * 256;
         uint8_t* d = (uint8_t*)ctx->ed_items.get(o_out + n * 40);
         ctx->staging.copy(slot, d, up, s);
         PrAlignArgs pa{};
@@ -95,6 +103,24 @@ void pralign_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
             pa.nitems = (uint32_t)cnt[c];
             pa.lds_bytes = lds[c];
             HIP_CHECK(launch_pralign(a->width, words[c], pa, s));
+        }
+        if (!big.empty()) {
+            HIP_CHECK(hipMemcpyAsync(d + o_big, big.data(), big.size() * 4, hipMemcpyHostToDevice, s));
+            PbigArgs ba{};
+            ba.achars = a->d_chars;
+            ba.aoff = a->d_off;
+            ba.bchars = b->d_chars;
+            ba.boff = b->d_off;
+            ba.items = (const uint32_t*)(d + o_big);
+            ba.nitems = big.size();
+            ba.cutoff = cutoff;
+            ba.al_out = pa.out;
+            ba.ws_stride = (big_ws + 255) / 256 * 256;
+            const uint32_t nb = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(std::min<uint64_t>(big.size(), (uint64_t)ctx->num_cus * 4), (1ull << 30) / ba.ws_stride));
+            ba.ws = (uint8_t*)ctx->pb_ws.get(nb * ba.ws_stride);
+            HIP_CHECK(launch_pbig(a->width, ba, nb, s));
+            HIP_CHECK(hipStreamSynchronize(s)); // big is a host vector
         }
         HIP_CHECK(hipMemcpyAsync(out, pa.out, n * 40, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));

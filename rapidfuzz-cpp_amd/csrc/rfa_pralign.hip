@@ -35,6 +35,7 @@
 // the cutoff -- one max / first-index reduction over the lanes.
 // Equal lengths: the same search with the strings' roles exchanged, kept when
 // strictly better (its windows lie in the first string).
+#include "rfa_block_impl.hpp"
 #include "rfa_device.hpp"
 #include "rfa_internal.hpp"
 #include "rfa_semantics.hpp"
@@ -428,6 +429,305 @@ __global__ __launch_bounds__(64 * PRALIGN_WAVES) void k_pralign(PrAlignArgs a)
         }
     }
     pr_pair<WM, CharT, false>(a, pair, A, la, B, lb, (uint64_t*)(a.scratch + a.scratch_off[item]), false);
+}
+
+// ---- needles of 1025..4096 units (k_pbig) ---------------------------------------
+//
+// One wave per pair, the needle's words across the lanes (word w in lane w), its PM rows
+// and its reversal's in the wave's workspace (build_pattern); every haystack unit's row
+// index is looked up once.  A window's LCS runs the bit-parallel recurrence over the
+// words with the additions carried across lanes by lookahead on two ballots, the match
+// words of eight steps loaded ahead of their use.  The full windows follow the
+// reference's bisection itself (fuzz_impl.hpp:68-146), one window at a time with its
+// distance kept for revisits; the prefixes and the suffixes are recorded in one pass
+// each (the reversed needle against the haystack read backwards) and chosen like the
+// one-word-per-lane kernel above.
+
+template <typename CharT, bool REC>
+__device__ uint32_t pb_lcs(const Pattern<CharT, false>& P, uint32_t l1, const uint32_t* __restrict__ rowseq,
+                           uint32_t n, uint32_t* __restrict__ rec)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t W = P.W;
+    const bool act = lane < W;
+    const uint64_t valid = !act ? 0ull : (l1 - 64ull * lane >= 64 ? ~0ull : ((1ull << (l1 - 64ull * lane)) - 1));
+    const uint64_t* __restrict__ pm = P.pm;
+    const uint32_t Wp = P.Wp;
+    uint64_t S = ~0ull;
+    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+        const uint32_t rr = j0 + lane < n ? rowseq[j0 + lane] : 0u;
+        const uint32_t steps = min(64u, n - j0);
+        uint64_t Xc[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q) {
+            const uint32_t r = q < steps ? readlane(rr, q) : 0u;
+            Xc[q] = (act && q < steps) ? pm[(size_t)r * Wp + lane] : 0ull;
+        }
+        for (uint32_t g0 = 0; g0 < steps; g0 += 8) {
+            uint64_t Xn[8];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                const uint32_t s2 = g0 + 8 + q;
+                const uint32_t r = s2 < steps ? readlane(rr, s2) : 0u;
+                Xn[q] = (act && s2 < steps) ? pm[(size_t)r * Wp + lane] : 0ull;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                if (g0 + q < steps) {
+                    const uint64_t u = S & Xc[q];
+                    uint64_t x = S + u;
+                    const bool g = act && (x < S);
+                    const bool pr = act && !g && (x == ~0ull);
+                    const uint64_t G = ballot(g), Pp = ballot(pr);
+                    const uint64_t sum = (G | Pp) + G;
+                    x += ((sum ^ Pp) >> lane) & 1u;
+                    if (act) S = x | (S - u);
+                    if constexpr (REC) {
+                        uint32_t v = (uint32_t)__popcll(~S & valid);
+#pragma unroll
+                        for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+                        if (lane == 0) rec[j0 + g0 + q] = v;
+                    }
+                }
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) Xc[q] = Xn[q];
+        }
+    }
+    uint32_t v = (uint32_t)__popcll(~S & valid);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+    return v;
+}
+
+// fuzz_detail::partial_ratio_impl (fuzz_impl.hpp:68-179) of needle nd in haystack hy
+template <typename CharT>
+__device__ PrResult pb_search(const CharT* __restrict__ nd, uint32_t l1, const CharT* __restrict__ hy, uint32_t l2,
+                              double cutoff, uint8_t* ws)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t o[10];
+    pbig_layout((int)sizeof(CharT), l1, l2, o);
+    CharT* rev = (CharT*)(ws + o[2]);
+    for (uint32_t i = lane; i < l1; i += 64) rev[i] = nd[l1 - 1 - i];
+    arena_sync<false>();
+    const Pattern<CharT, false> P = build_pattern<CharT, false>(nd, l1, ws + o[0], pattern_hash_bits<CharT>(l1));
+    const Pattern<CharT, false> R = build_pattern<CharT, false>(rev, l1, ws + o[1], pattern_hash_bits<CharT>(l1));
+    uint32_t* rows = (uint32_t*)(ws + o[3]);
+    uint32_t* rrows = (uint32_t*)(ws + o[4]);
+    for (uint32_t t = lane; t < l2; t += 64) rows[t] = P.map.find((uint32_t)hy[t]);
+    for (uint32_t j = lane; j < l1; j += 64) rrows[j] = R.map.find((uint32_t)hy[l2 - 1 - j]);
+    arena_sync<false>();
+
+    PrResult res{0.0, 0u, l1};
+    const uint32_t F = l2 - l1;
+    if (F > 0) {
+        uint32_t* scores = (uint32_t*)(ws + o[5]);
+        uint2* cur = (uint2*)(ws + o[6]);
+        uint2* nxt = cur + (2 * F + 2);
+        constexpr uint32_t UNSET = 0xffffffffu;
+        for (uint32_t i = lane; i < F; i += 64) scores[i] = UNSET;
+        if (lane == 0) cur[0] = make_uint2(0u, F - 1);
+        arena_sync<false>();
+        const uint64_t maximum = 2ull * l1;
+        uint64_t cutoff_dist = (uint64_t)ceil((double)maximum * normsim_to_normdist(cutoff / 100));
+        uint64_t best = ~0ull;
+        uint32_t ncur = 1, ds = 0, de = l1;
+        bool perfect = false;
+        // distance of window p: evaluated once, kept for revisits (the last one in registers:
+        // adjacent ranges of a level share an end; earlier levels' are read after a barrier)
+        uint32_t last_p = UNSET, last_d = 0;
+        auto dist_of = [&](uint32_t p, bool& fresh) -> uint64_t {
+            if (p == last_p) {
+                fresh = false;
+                return last_d;
+            }
+            const uint32_t known = uniform(scores[p]);
+            fresh = known == UNSET;
+            if (!fresh) return known;
+            const uint32_t d = (uint32_t)(maximum - 2ull * pb_lcs<CharT, false>(P, l1, rows + p, l1, nullptr));
+            if (lane == 0) scores[p] = d;
+            last_p = p;
+            last_d = d;
+            return d;
+        };
+        while (ncur && !perfect) {
+            uint32_t nnext = 0;
+            for (uint32_t k = 0; k < ncur && !perfect; ++k) {
+                const uint32_t f = uniform(cur[k].x), sc = uniform(cur[k].y);
+                bool fresh;
+                const uint64_t df = dist_of(f, fresh);
+                if (fresh && df < cutoff_dist) {
+                    cutoff_dist = best = df;
+                    ds = f;
+                    de = f + l1;
+                    if (best == 0) perfect = true;
+                }
+                if (perfect) break;
+                const uint64_t dsc = dist_of(sc, fresh);
+                if (fresh && dsc < cutoff_dist) {
+                    cutoff_dist = best = dsc;
+                    ds = sc;
+                    de = sc + l1;
+                    if (best == 0) perfect = true;
+                }
+                if (perfect) break;
+                const uint32_t cell_diff = sc - f;
+                if (cell_diff == 1) continue;
+                const uint64_t known = df > dsc ? df - dsc : dsc - df;
+                const uint64_t improve = ((uint64_t)cell_diff - known / 2) / 2 * 2;
+                const int64_t min_score = (int64_t)(df < dsc ? df : dsc) - (int64_t)improve;
+                if (min_score < (int64_t)cutoff_dist) {
+                    const uint32_t center = cell_diff / 2;
+                    if (lane == 0) {
+                        nxt[nnext] = make_uint2(f, f + center);
+                        nxt[nnext + 1] = make_uint2(f + center, sc);
+                    }
+                    nnext += 2;
+                }
+            }
+            arena_sync<false>(); // the next level's list and the distances
+            uint2* t = cur;
+            cur = nxt;
+            nxt = t;
+            ncur = perfect ? 0 : nnext;
+        }
+        res.ds = ds;
+        res.de = de;
+        if (perfect) {
+            res.score = 100.0;
+            return res;
+        }
+        double sv = 1.0 - ((double)best / (double)maximum);
+        sv *= 100;
+        if (sv >= cutoff) {
+            res.score = sv;
+            cutoff = sv;
+        }
+    }
+
+    // ---- prefixes [0, i + 1) and suffixes [F + i, l2): recorded, then chosen ----
+    uint32_t* rec_pre = (uint32_t*)(ws + o[7]);
+    uint32_t* rec_suf = (uint32_t*)(ws + o[8]);
+    if (l1 > 1) pb_lcs<CharT, true>(P, l1, rows, l1 - 1, rec_pre);
+    pb_lcs<CharT, true>(R, l1, rrows, l1, rec_suf); // after step j: the suffix [l2 - 1 - j, l2)
+    arena_sync<false>();
+    for (int phase = 0; phase < 2; ++phase) {
+        const uint32_t n = phase == 0 ? l1 - 1 : l1;
+        double bestv = -1.0;
+        uint32_t bidx = ~0u;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            if (i < n) {
+                const uint32_t t = phase == 0 ? i : F + i;
+                const uint32_t wl = phase == 0 ? i + 1 : l2 - (F + i);
+                if (rows[t] != 0) { // the boundary unit occurs in the needle
+                    const uint32_t v = phase == 0 ? rec_pre[i] : rec_suf[l1 - 1 - i];
+                    const double ls = pr_ratio(l1, wl, v, cutoff);
+                    if (ls > bestv) {
+                        bestv = ls;
+                        bidx = i;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(bestv, off, 64);
+            const uint32_t oi = (uint32_t)__shfl_xor((int)bidx, off, 64);
+            if (ob > bestv || (ob == bestv && oi < bidx)) {
+                bestv = ob;
+                bidx = oi;
+            }
+        }
+        if (bidx != ~0u && bestv > res.score) {
+            res.score = bestv;
+            if (phase == 0) {
+                res.ds = 0;
+                res.de = bidx + 1;
+            }
+            else {
+                res.ds = F + bidx;
+                res.de = l2;
+            }
+            cutoff = bestv;
+            if (res.score == 100.0) return res;
+        }
+    }
+    return res;
+}
+
+template <typename CharT>
+__global__ __launch_bounds__(64) void k_pbig(PbigArgs a)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint8_t* ws = a.ws + (uint64_t)blockIdx.x * a.ws_stride;
+    for (uint64_t it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+        uint32_t ia, ib;
+        if (a.cdist) {
+            ia = a.items[it / a.nb];
+            ib = (uint32_t)(it % a.nb);
+        }
+        else {
+            ia = ib = a.items[it];
+        }
+        ia = uniform(ia);
+        ib = uniform(ib);
+        const uint64_t ao = a.aoff[ia], bo = a.boff[ib];
+        const CharT* A = (const CharT*)a.achars + ao;
+        const CharT* B = (const CharT*)a.bchars + bo;
+        const uint32_t la = uniform((uint32_t)(a.aoff[ia + 1] - ao)), lb = uniform((uint32_t)(a.boff[ib + 1] - bo));
+        const uint32_t l1 = min(la, lb), l2 = max(la, lb);
+        if (a.cdist && l1 <= PRALIGN_MAX) continue; // the long partial kernel's pairs
+        const double cut = a.cdist && a.pcut ? a.pcut[(uint64_t)ia * a.out_stride + ib] : a.cutoff;
+        const bool swap = la > lb;
+        const CharT* nd = swap ? B : A;
+        const CharT* hy = swap ? A : B;
+        uint32_t ss = 0, se = l1, dst = 0, den = l1;
+        double score = 0.0;
+        if (cut <= 100) {
+            const PrResult r = pb_search<CharT>(nd, l1, hy, l2, cut, ws);
+            score = r.score;
+            dst = r.ds;
+            den = r.de;
+            if (score != 100.0 && l1 == l2) {
+                const double c2 = cut < score ? score : cut;
+                arena_sync<false>();
+                const PrResult r2 = pb_search<CharT>(hy, l2, nd, l1, c2, ws);
+                if (r2.score > score) {
+                    score = r2.score;
+                    ss = r2.ds;
+                    se = r2.de;
+                    dst = 0;
+                    den = l2;
+                }
+            }
+        }
+        if (a.cdist) {
+            if (lane == 0) a.score_out[(uint64_t)ia * a.out_stride + ib] = score;
+        }
+        else if (lane < 5) {
+            uint64_t v = (uint64_t)__double_as_longlong(score);
+            if (lane == 1) v = swap ? dst : ss;
+            if (lane == 2) v = swap ? den : se;
+            if (lane == 3) v = swap ? ss : dst;
+            if (lane == 4) v = swap ? se : den;
+            a.al_out[5ull * ia + lane] = v;
+        }
+        arena_sync<false>(); // the workspace is reused by the next item
+    }
+}
+
+hipError_t launch_pbig(int char_width, const PbigArgs& a, uint32_t nblocks, hipStream_t s)
+{
+    if (a.nitems == 0) return hipSuccess;
+    if (char_width == 1)
+        hipLaunchKernelGGL((k_pbig<uint8_t>), dim3(nblocks), dim3(64), 0, s, a);
+    else if (char_width == 2)
+        hipLaunchKernelGGL((k_pbig<uint16_t>), dim3(nblocks), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_pbig<uint32_t>), dim3(nblocks), dim3(64), 0, s, a);
+    return hipGetLastError();
 }
 
 template <uint32_t WM>

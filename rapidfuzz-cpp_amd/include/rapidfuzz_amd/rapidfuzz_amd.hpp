@@ -1399,7 +1399,7 @@ RFA_FREE_FUZZ(WRatio, CachedWRatio)
 #undef RFA_FREE_FUZZ
 
 // partial_ratio_alignment (fuzz.hpp:128-134): the best partial ratio and the windows of both
-// strings it was found at (src: the first string); the shorter string may hold at most 1024
+// strings it was found at (src: the first string); the shorter string may hold at most 4096
 // code units
 template <typename InputIt1, typename InputIt2>
 ScoreAlignment<double> partial_ratio_alignment(InputIt1 first1, InputIt1 last1, InputIt2 first2, InputIt2 last2,

@@ -73,6 +73,20 @@ def test_long_queries(ctx, oracle, width):
     _check(ctx, oracle, q, c, width=width, cuts=(0.0, 60.0, 95.0))
 
 
+@pytest.mark.parametrize("width", [1, 4])
+def test_needles_past_1024(ctx, oracle, width):
+    """Pairs whose shorter string has 1,025-3,000 units (one wave per pair) beside pairs the
+    long kernel takes, in one cdist: queries of 1,100-3,000 units against choices of 0-3,500."""
+    rng = np.random.default_rng(960 + width)
+    base = {1: 97, 4: 1000000}[width]
+    q = [x + (base - 97) for x in _gen.random_strings(rng, 3, 1100, 3000, alpha_n=4)]
+    q.append(_gen.random_strings(rng, 1, 100, 200, alpha_n=4)[0] + (base - 97))
+    c = [x + (base - 97) for x in _gen.random_strings(rng, 6, 0, 3500, alpha_n=4)]
+    c += [np.concatenate([c[0][:300], _gen.mutate(rng, q[0] - (base - 97), 20) + (base - 97), c[1][:100]]),
+          q[1][50:1500].copy(), q[2].copy()]
+    _check(ctx, oracle, q, c, width=width, cuts=(0.0, 80.0))
+
+
 def test_large_alphabet_tiles(ctx, oracle):
     """Short queries whose tile has more symbols than the LDS table holds take the long path."""
     rng = np.random.default_rng(990)
@@ -91,9 +105,9 @@ def test_rejected_calls(ctx):
         sc.cdist(ctx.strings([b"abcd"]), 0)  # partial_ratio only has similarity
     with pytest.raises(rfa.InvalidArgument):
         sc.cdist(ctx.strings([b"abcd"]), SIM, semantics=1)  # no Multi* scorer
-    long_q = rfa.Scorer(ctx, PARTIAL, ctx.strings([b"a" * 1100]))
+    long_q = rfa.Scorer(ctx, PARTIAL, ctx.strings([b"a" * 4100]))
     with pytest.raises(rfa.RfaError):
-        long_q.cdist(ctx.strings([b"b" * 1100]), SIM)  # both strings beyond the long path's needle limit
+        long_q.cdist(ctx.strings([b"b" * 4100]), SIM)  # both strings beyond the 4096-unit needle limit
 
 
 def test_topk_partial(ctx, oracle):

@@ -2,8 +2,8 @@
 the source / destination windows) against the oracle, field for field and bit for bit:
 the reference's issue-231 known answer, random pairs over small alphabets (ties between
 windows), both length orders, equal lengths (the swapped second pass), empty strings,
-needles of 1, 2-4 and 5-16 words up to the 1024-unit limit, 8/16/32-bit units, and
-cutoffs at and next to the scores the pairs reach."""
+needles of 1, 2-4 and 5-16 words and past 1024 units up to the 4096-unit limit,
+8/16/32-bit units, and cutoffs at and next to the scores the pairs reach."""
 import numpy as np
 import pytest
 
@@ -150,14 +150,26 @@ def test_trivial_pairs(ctx, oracle):
         _check(ctx, oracle, pairs, c)
 
 
+@pytest.mark.parametrize("cutoff", [0.0, 85.0])
+def test_needles_past_1024(ctx, oracle, cutoff):
+    """Shorter strings of 1,025-4,096 units (one wave per pair, the reference's bisection window
+    by window), in a batch with short pairs; both orders, equal lengths, 8/16-bit units."""
+    rng = np.random.default_rng(480 + int(cutoff))
+    pairs = _long_pairs(rng, [1025, 1300, 2048, 3000, 4096, 1100], alpha=4)
+    pairs += _small_pairs(rng, 40)
+    _check(ctx, oracle, pairs, cutoff)
+    wide = [(x + 0x4E00 - 97, y + 0x4E00 - 97) for x, y in pairs[:6]]
+    _check(ctx, oracle, wide, cutoff, width=2)
+
+
 def test_length_limit(ctx):
     import rapidfuzz_amd as rfa
 
-    a = ctx.strings([np.full(1025, 97, np.uint32)])
-    b = ctx.strings([np.full(1030, 97, np.uint32)])
+    a = ctx.strings([np.full(4097, 97, np.uint32)])
+    b = ctx.strings([np.full(4100, 97, np.uint32)])
     with pytest.raises(rfa.RfaError):
         rfa.partial_ratio_alignment(ctx, a, b, 0.0)
     # a longer haystack is fine while the shorter string fits
-    c = ctx.strings([np.full(1024, 97, np.uint32)])
+    c = ctx.strings([np.full(4096, 97, np.uint32)])
     got = rfa.partial_ratio_alignment(ctx, c, a, 0.0)
     assert float(got[0]["score"]) == 100.0

@@ -132,3 +132,18 @@ def test_positional_metrics_brute_force(oracle):
         assert oracle.score(7, 0, a, b) == ham
         assert oracle.score(8, 1, a, b) == pre
         assert oracle.score(9, 1, a, b) == post
+
+
+def test_lcs_bits_equals_dp(oracle):
+    """The oracle's bit-parallel LCS (the Indel scorers' raw value) against its plain DP:
+    random strings over small and large alphabets, lengths across word boundaries."""
+    import numpy as np
+
+    rng = np.random.default_rng(77)
+    for _ in range(400):
+        alpha = int(rng.choice([2, 3, 26, 70000]))
+        n1 = int(rng.choice([0, 1, 63, 64, 65, 127, 128, 129, int(rng.integers(0, 400))]))
+        n2 = int(rng.integers(0, 400))
+        a = rng.integers(0, alpha, size=n1).astype(np.uint32)
+        b = rng.integers(0, alpha, size=n2).astype(np.uint32)
+        assert oracle.lcs_bits(a, b) == oracle.lcs_raw(a, b), (n1, n2, alpha)
