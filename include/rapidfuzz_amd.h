@@ -269,9 +269,11 @@ rfa_status rfa_topk(rfa_scorer* scorer, const rfa_strings* choices, int kind, do
  * sizes only; otherwise capacity (in operations) must be at least offsets[count]
  * (else RFA_ERR_INVALID_ARGUMENT, with offsets written).  A pair whose first string
  * has more than 4096 code units once the common prefix and suffix are removed
- * returns RFA_ERR_UNSUPPORTED, and so does a Levenshtein pair the reference splits
- * by Hirschberg (stripped |s1| > 64, |s2| >= 10 and |s1| * |s2| >= 2^22,
- * Levenshtein_impl.hpp:1185-1200); Hamming has no limit. */
+ * returns RFA_ERR_UNSUPPORTED.  Levenshtein pairs the reference splits by
+ * Hirschberg (stripped |s1| > 64, |s2| >= 10 and |s1| * |s2| >= 2^22,
+ * Levenshtein_impl.hpp:1185-1212) are split the same way; they need string sets
+ * that hold a host copy (views return RFA_ERR_UNSUPPORTED for them).  Hamming has
+ * no limit. */
 #define RFA_EDIT_NONE 0
 #define RFA_EDIT_REPLACE 1
 #define RFA_EDIT_INSERT 2

@@ -572,29 +572,12 @@ static size_t lev_block_editops(const uint32_t* a, size_t la, const uint32_t* b,
 static int g_lev_block = 1;
 void orc_set_lev_block(int on) { g_lev_block = on; }
 
-size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, int32_t* type,
-                   uint64_t* src_pos, uint64_t* dest_pos)
+/* The walk of the full DP table of a stripped pair (Levenshtein or LCS): the bits the
+ * reference records read from the table's differences; operations at positions
+ * (col + src_off, row + dst_off). */
+static size_t ed_table_walk(int metric, const uint32_t* a, size_t la, const uint32_t* b, size_t lb, size_t src_off,
+                            size_t dst_off, int32_t* type, uint64_t* src_pos, uint64_t* dest_pos)
 {
-    if (metric == ORC_HAMMING) {
-        /* hamming_editops with pad (Hamming_impl.hpp:40-59): a replacement at every
-         * differing position of the common length, then the longer string's tail */
-        size_t k = 0, i = 0;
-        const size_t mn = l1 < l2 ? l1 : l2;
-        for (; i < mn; ++i)
-            if (s1[i] != s2[i]) type[k] = ED_REPLACE, src_pos[k] = i, dest_pos[k] = i, ++k;
-        for (; i < l1; ++i) type[k] = ED_DELETE, src_pos[k] = i, dest_pos[k] = l2, ++k;
-        for (; i < l2; ++i) type[k] = ED_INSERT, src_pos[k] = l1, dest_pos[k] = i, ++k;
-        return k;
-    }
-    size_t pre = 0;
-    while (pre < l1 && pre < l2 && s1[pre] == s2[pre]) ++pre;
-    size_t suf = 0;
-    while (suf < l1 - pre && suf < l2 - pre && s1[l1 - 1 - suf] == s2[l2 - 1 - suf]) ++suf;
-    const uint32_t* a = s1 + pre;
-    const uint32_t* b = s2 + pre;
-    const size_t la = l1 - pre - suf, lb = l2 - pre - suf;
-    if (metric == ORC_LEVENSHTEIN && la > 64 && lb > 0 && g_lev_block)
-        return lev_block_editops(a, la, b, lb, pre, type, src_pos, dest_pos);
     const size_t W = la + 1;
     uint64_t* T = (uint64_t*)malloc((la + 1) * (lb + 1) * sizeof(uint64_t)); /* T[r * W + c] */
     const int lev = metric == ORC_LEVENSHTEIN;
@@ -624,8 +607,8 @@ size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2
     do {                                                                                                         \
         --dist;                                                                                                  \
         type[dist] = (t);                                                                                        \
-        src_pos[dist] = col + pre;                                                                               \
-        dest_pos[dist] = row + pre;                                                                              \
+        src_pos[dist] = col + src_off;                                                                           \
+        dest_pos[dist] = row + dst_off;                                                                           \
     } while (0)
     if (lev) {
         while (row && col) {
@@ -677,6 +660,102 @@ size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2
 #undef ED_PUT
     free(T);
     return total;
+}
+
+/* Levenshtein rows: D[i] = distance(a[0, i), b) for i = 0..la (the DP's last row) */
+static void lev_last_row(const uint32_t* a, size_t la, const uint32_t* b, size_t lb, uint64_t* D)
+{
+    for (size_t i = 0; i <= la; ++i) D[i] = i;
+    for (size_t r = 1; r <= lb; ++r) {
+        uint64_t diag = D[0];
+        D[0] = r;
+        for (size_t i = 1; i <= la; ++i) {
+            const uint64_t up = D[i];
+            uint64_t v = diag + (a[i - 1] != b[r - 1]);
+            if (up + 1 < v) v = up + 1;
+            if (D[i - 1] + 1 < v) v = D[i - 1] + 1;
+            diag = up;
+            D[i] = v;
+        }
+    }
+}
+
+/* levenshtein_align_hirschberg (Levenshtein_impl.hpp:1185-1212): strip the common affixes;
+ * when the band's matrix stays under 1 MiB (or |s1| < 65 or |s2| < 10) walk the pair's
+ * matrix, else split s2 in the middle at the s1 position find_hirschberg_pos
+ * (Levenshtein_impl.hpp:1098-1182) picks -- the first position of the smallest sum of the
+ * forward and backward distances (exact rows here; the reference's banded rows hold
+ * every position that can reach that minimum) -- and recurse on both halves.
+ * Returns the operations written at ops + opos. */
+static size_t lev_hirschberg(const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, size_t max, size_t src,
+                             size_t dst, int32_t* type, uint64_t* src_pos, uint64_t* dest_pos, size_t opos)
+{
+    size_t pre = 0;
+    while (pre < l1 && pre < l2 && s1[pre] == s2[pre]) ++pre;
+    size_t suf = 0;
+    while (suf < l1 - pre && suf < l2 - pre && s1[l1 - 1 - suf] == s2[l2 - 1 - suf]) ++suf;
+    const uint32_t* a = s1 + pre;
+    const uint32_t* b = s2 + pre;
+    const size_t la = l1 - pre - suf, lb = l2 - pre - suf;
+    src += pre;
+    dst += pre;
+    const size_t mx = la > lb ? la : lb;
+    if (mx < max) max = mx;
+    const size_t full_band = la < 2 * max + 1 ? la : 2 * max + 1;
+    if ((2 * full_band * lb) / 8 < 1024 * 1024 || la < 65 || lb < 10)
+        return ed_table_walk(ORC_LEVENSHTEIN, a, la, b, lb, src, dst, type + opos, src_pos + opos, dest_pos + opos);
+    const size_t ls = lb / 2;
+    uint64_t* L = (uint64_t*)malloc((la + 1) * sizeof(uint64_t));
+    uint64_t* R = (uint64_t*)malloc((la + 1) * sizeof(uint64_t));
+    uint32_t* ra = (uint32_t*)malloc((la + 1) * sizeof(uint32_t));
+    uint32_t* rb = (uint32_t*)malloc((lb - ls + 1) * sizeof(uint32_t));
+    for (size_t i = 0; i < la; ++i) ra[i] = a[la - 1 - i];
+    for (size_t i = 0; i < lb - ls; ++i) rb[i] = b[lb - 1 - i];
+    lev_last_row(a, la, b, ls, L);
+    lev_last_row(ra, la, rb, lb - ls, R); /* R[j] = distance(a[la - j, la), b[ls, lb)) */
+    size_t mid = 0;
+    uint64_t best = ~0ull;
+    for (size_t i = 0; i <= la; ++i) {
+        const uint64_t t = L[i] + R[la - i];
+        if (t < best) best = t, mid = i;
+    }
+    const size_t left = (size_t)L[mid], right = (size_t)R[la - mid];
+    free(L);
+    free(R);
+    free(ra);
+    free(rb);
+    const size_t k1 = lev_hirschberg(a, mid, b, ls, left, src, dst, type, src_pos, dest_pos, opos);
+    const size_t k2 =
+        lev_hirschberg(a + mid, la - mid, b + ls, lb - ls, right, src + mid, dst + ls, type, src_pos, dest_pos, opos + k1);
+    return k1 + k2;
+}
+
+size_t orc_editops(int metric, const uint32_t* s1, size_t l1, const uint32_t* s2, size_t l2, int32_t* type,
+                   uint64_t* src_pos, uint64_t* dest_pos)
+{
+    if (metric == ORC_HAMMING) {
+        /* hamming_editops with pad (Hamming_impl.hpp:40-59): a replacement at every
+         * differing position of the common length, then the longer string's tail */
+        size_t k = 0, i = 0;
+        const size_t mn = l1 < l2 ? l1 : l2;
+        for (; i < mn; ++i)
+            if (s1[i] != s2[i]) type[k] = ED_REPLACE, src_pos[k] = i, dest_pos[k] = i, ++k;
+        for (; i < l1; ++i) type[k] = ED_DELETE, src_pos[k] = i, dest_pos[k] = l2, ++k;
+        for (; i < l2; ++i) type[k] = ED_INSERT, src_pos[k] = l1, dest_pos[k] = i, ++k;
+        return k;
+    }
+    size_t pre = 0;
+    while (pre < l1 && pre < l2 && s1[pre] == s2[pre]) ++pre;
+    size_t suf = 0;
+    while (suf < l1 - pre && suf < l2 - pre && s1[l1 - 1 - suf] == s2[l2 - 1 - suf]) ++suf;
+    const uint32_t* a = s1 + pre;
+    const uint32_t* b = s2 + pre;
+    const size_t la = l1 - pre - suf, lb = l2 - pre - suf;
+    if (metric == ORC_LEVENSHTEIN && la > 64 && lb >= 10 && (2 * la * lb) / 8 >= 1024 * 1024)
+        return lev_hirschberg(s1, l1, s2, l2, l1 > l2 ? l1 : l2, 0, 0, type, src_pos, dest_pos, 0);
+    if (metric == ORC_LEVENSHTEIN && la > 64 && lb > 0 && g_lev_block)
+        return lev_block_editops(a, la, b, lb, pre, type, src_pos, dest_pos);
+    return ed_table_walk(metric, a, la, b, lb, pre, pre, type, src_pos, dest_pos);
 }
 
 /* damerau_levenshtein_distance (DamerauLevenshtein_impl.hpp:105-121): the length

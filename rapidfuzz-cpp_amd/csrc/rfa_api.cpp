@@ -1736,7 +1736,7 @@ rfa_status rfa_context_destroy(rfa_context* ctx)
                           &ctx->plan_vals, &ctx->plan_temp, &ctx->topk_scores, &ctx->topk_run[0],
                           &ctx->topk_run[1], &ctx->topk_cand_s, &ctx->topk_cand_i, &ctx->topk_counts, &ctx->topk_q,
                           &ctx->topk_out, &ctx->qcut, &ctx->ed_items, &ctx->ed_rows, &ctx->ed_ops, &ctx->tset_cols,
-                          &ctx->tset_rows, &ctx->band_q, &ctx->band_w, &ctx->wl_bnd, &ctx->ed_long, &ctx->pb_ws, &ctx->wr_buf[0],
+                          &ctx->tset_rows, &ctx->band_q, &ctx->band_w, &ctx->wl_bnd, &ctx->ed_long, &ctx->ed_hnodes, &ctx->ed_htmp, &ctx->pb_ws, &ctx->wr_buf[0],
                           &ctx->wr_buf[1], &ctx->wr_buf[2]})
             b->release();
         ctx->staging.release();

@@ -216,6 +216,7 @@ struct rfa_context {
     rfa::DevBuf ed_items, ed_rows, ed_ops, band_q, band_w;
     rfa::DevBuf wl_bnd; // weighted Levenshtein strips: a boundary row per wave
     rfa::DevBuf ed_long; // Levenshtein editops past 64 units: a pair's rows per wave
+    rfa::DevBuf ed_hnodes, ed_htmp; // Hirschberg splits: the pieces of one level, two rows per wave
     rfa::DevBuf pb_ws;   // partial ratios of needles over 1024 units: a workspace per wave
     rfa::DevBuf tset_cols, tset_rows; // partial_token_ratio: the repeat columns and rows
     size_t lane_arena_bytes_zeroed = 0;

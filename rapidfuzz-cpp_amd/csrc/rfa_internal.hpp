@@ -384,6 +384,18 @@ struct EdOp { // the layout of rfa_editop
     uint64_t src_pos;
     uint64_t dest_pos;
 };
+// A piece of a Levenshtein pair split by Hirschberg: s1[a0, a0 + la) against s2[b0, b0 + lb)
+// of pair `pair`, its `dist` operations at ops[opos], positions offset by src / dst.
+struct EdLeaf {
+    uint32_t pair, la, lb, dist;
+    uint64_t a0, b0, src, dst, opos;
+};
+// A split of such a piece (find_hirschberg_pos): in a0 / la / b0 / lb, out mid / left / right
+struct EdSplit {
+    uint32_t pair, la, lb, mid;
+    uint64_t a0, b0;
+    uint32_t left, right;
+};
 struct EditopsArgs {
     const void* achars;
     const void* bchars;
@@ -401,6 +413,10 @@ struct EditopsArgs {
     uint64_t* need;           // Levenshtein block pairs: the largest row area (words) of one pair
     uint64_t* long_rows;      // per wave of k_ed_levl: long_stride words of rows
     uint64_t long_stride;
+    uint64_t* hcount;         // Levenshtein pairs the reference splits by Hirschberg: their number
+    uint32_t* hlist;          // and indices
+    const EdLeaf* leaves;     // k_ed_levl's leaf mode (nullptr: the pairs of flag 2)
+    uint64_t nleaves;
 };
 // Levenshtein editops of a stripped first string of 65..EDITOPS_LEV_BLOCK_MAX units run the
 // reference's block recurrence with its Ukkonen band (Levenshtein_impl.hpp:641-850) when the
@@ -408,6 +424,9 @@ struct EditopsArgs {
 constexpr uint32_t EDITOPS_LEV_BLOCK_MAX = 4096;
 hipError_t launch_editops_lev_block(int char_width, const EditopsArgs& a, bool walk, uint32_t nblocks,
                                     hipStream_t s);
+// find_hirschberg_pos of every split (one wave each): tmp holds 2 * (PBIG-sized) rows per wave
+hipError_t launch_editops_split(int char_width, const EditopsArgs& a, EdSplit* splits, uint64_t nsplits,
+                                uint32_t* tmp, uint64_t tmp_stride, uint32_t nblocks, hipStream_t s);
 size_t editops_scan_bytes(uint64_t n);
 // offsets[0..n] = exclusive prefix sum of counts[0..n] (counts[n] = 0)
 hipError_t editops_offsets(const uint64_t* counts, uint64_t* offsets, uint64_t n, void* temp, size_t bytes,

@@ -129,6 +129,28 @@ def test_levenshtein_long_vs_reference(oracle, reference):
         oracle.set_lev_block(True)
 
 
+def test_levenshtein_hirschberg_vs_reference(oracle, reference):
+    """Pairs the reference splits by Hirschberg (stripped |s1| > 64, |s2| >= 10 and
+    |s1| * |s2| >= 2^22): the oracle's split at the first smallest sum of exact forward and
+    backward rows, its leaves walked over full tables, against the reference."""
+    rng = np.random.default_rng(25)
+    pairs = []
+    for it in range(24):
+        alpha = int(rng.choice([2, 4, 26]))
+        n1 = int(rng.integers(2050, 3600))
+        s1 = rng.integers(97, 97 + alpha, size=n1).astype(np.uint32)
+        kind = it % 3
+        if kind == 0:
+            s2 = _gen.mutate(rng, s1, int(rng.integers(1, n1 // 4)), alpha_n=alpha)
+        elif kind == 1:
+            s2 = rng.integers(97, 97 + alpha, size=int(rng.integers(2050, 3600))).astype(np.uint32)
+        else:
+            pre = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 500))).astype(np.uint32)
+            s2 = np.concatenate([pre, _gen.mutate(rng, s1, 30, alpha_n=alpha)])
+        pairs.append((s1, s2))
+    _check_vs_reference(oracle, reference, LEV, pairs)
+
+
 def test_hamming_vs_reference(oracle, reference):
     """hamming_editops with pad (Hamming_impl.hpp:40-59), tests-Hamming.cpp:112-127's strings included."""
     rng = np.random.default_rng(23)

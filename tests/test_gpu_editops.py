@@ -1,7 +1,7 @@
 """GPU parity of rfa_editops (levenshtein_editops, lcs_seq_editops / indel_editops)
 against the oracle, operation for operation: random pairs around the 64-unit single
 word of the stripped first string (Levenshtein past it up to 4096 units through the
-block band, LCS up to 4096 units over several words),
+block band and the Hirschberg split, LCS up to 4096 units over several words),
 long second strings (several 64-unit chunks of rows),
 common affixes, empty strings, 8/16/32-bit units, and the reference's editops test
 strings; every script also rebuilds the second string."""
@@ -102,6 +102,42 @@ def test_levenshtein_long_first_strings(ctx, oracle, width):
     check(ctx, oracle, LEV, pairs, width=width)
 
 
+@pytest.mark.parametrize("width", [1, 2])
+def test_levenshtein_hirschberg(ctx, oracle, width):
+    """Pairs the reference splits by Hirschberg (stripped |s1| > 64, |s2| >= 10 and
+    |s1| * |s2| >= 2^22; Levenshtein_impl.hpp:1185-1212): mutated copies, independent
+    strings, shifted copies, short first strings against long second ones (leaves under
+    65 units), first strings up to the 4096-unit limit against longer second strings, and
+    short pairs in the same batch."""
+    rng = np.random.default_rng(90 + width)
+    pairs = []
+    for it in range(12):
+        alpha = int(rng.choice([2, 4, 26]))
+        n1 = int(rng.integers(2050, 4000))
+        s1 = rng.integers(97, 97 + alpha, size=n1).astype(np.uint32)
+        kind = it % 3
+        if kind == 0:
+            s2 = _gen.mutate(rng, s1, int(rng.integers(1, n1 // 4)), alpha_n=alpha)
+        elif kind == 1:
+            s2 = rng.integers(97, 97 + alpha, size=int(rng.integers(2050, 3600))).astype(np.uint32)
+        else:
+            pre = rng.integers(97, 97 + alpha, size=int(rng.integers(0, 500))).astype(np.uint32)
+            s2 = np.concatenate([pre, _gen.mutate(rng, s1, 30, alpha_n=alpha)])
+        pairs.append((s1, s2))
+    s1 = rng.integers(97, 101, size=100).astype(np.uint32)
+    pairs.append((s1, rng.integers(97, 101, size=45000).astype(np.uint32)))
+    s1 = rng.integers(97, 101, size=4096).astype(np.uint32)
+    pairs.append((s1, np.concatenate([_gen.mutate(rng, s1, 200, alpha_n=4),
+                                      rng.integers(97, 101, size=4000).astype(np.uint32)])))
+    core = rng.integers(97, 99, size=3000).astype(np.uint32)
+    pairs.append((np.concatenate([_codes("prefix-"), core, _codes("-suffix")]),
+                  np.concatenate([_codes("prefix-"), _gen.mutate(rng, core, 300, alpha_n=2), _codes("-suffix")])))
+    pairs += _pairs(rng, 4, n=20)
+    if width == 2:
+        pairs = [(x + 0x4E00 - 97, y + 0x4E00 - 97) for x, y in pairs]
+    check(ctx, oracle, LEV, pairs, width=width)
+
+
 def test_hamming_long(ctx, oracle):
     """Hamming editops have no length limit: strings of up to 5000 units."""
     rng = np.random.default_rng(61)
@@ -121,10 +157,8 @@ def test_sizes_then_ops_and_limits(ctx, oracle):
     assert rfa._lib.rfa_editops(ctx.h, a.h, b.h, 0, off.ctypes.data, small.ctypes.data, 1) == 1  # too small
     assert int(off[2]) == len(oracle.editops(0, b"Lorem ipsum.", b"XYZLorem ABC iPsum")[0]) + \
         len(oracle.editops(0, b"aa", b"abb")[0])
-    long_a = ctx.strings([np.full(2100, 97, np.uint32)], width=1)
-    long_b = ctx.strings([np.full(2100, 98, np.uint32)], width=1)
-    with pytest.raises(rfa.RfaError):
-        rfa.editops(ctx, long_a, long_b, 0)  # Levenshtein: the reference splits this (Hirschberg)
+    # Levenshtein pairs the reference splits (Hirschberg) are supported
+    check(ctx, oracle, LEV, [(np.full(2100, 97, np.uint32), np.full(2100, 98, np.uint32))])
     over_a = ctx.strings([np.full(4097, 97, np.uint32)], width=1)
     over_b = ctx.strings([np.full(5, 98, np.uint32)], width=1)
     with pytest.raises(rfa.RfaError):
