@@ -987,6 +987,80 @@ def workload_editops(args, emit=True):
     return line
 
 
+ED_LONG_PAIRS = 512
+
+
+def workload_editops_long(args):
+    """levenshtein_editops of 512 pairs the reference splits by Hirschberg
+    (Levenshtein_impl.hpp:1185-1212): lowercase first strings of 3,000 units, each against a
+    copy with 300 random substitutions, insertions and deletions.  The timed region is the whole
+    rfa_editops call (the count pass, the level-by-level splits and the leaves' walk)."""
+    import rapidfuzz_amd as rfa
+
+    rng = np.random.default_rng(SEED + 12)
+    a_strs, b_strs = [], []
+    for _ in range(ED_LONG_PAIRS):
+        t = rng.integers(97, 123, size=3000).astype(np.uint8)
+        u = list(t)
+        for _ in range(300):
+            op = int(rng.integers(0, 3))
+            if op == 0:
+                u[int(rng.integers(0, len(u)))] = int(rng.integers(97, 123))
+            elif op == 1:
+                u.insert(int(rng.integers(0, len(u) + 1)), int(rng.integers(97, 123)))
+            else:
+                del u[int(rng.integers(0, len(u)))]
+        a_strs.append(t)
+        b_strs.append(np.array(u, dtype=np.uint8))
+    ac, ao = pack_u8(a_strs)
+    bc, bo = pack_u8(b_strs)
+    ctx = rfa.Context(0)
+    A = rfa.Strings(ctx, chars=ac, offsets=ao, width=1)
+    B = rfa.Strings(ctx, chars=bc, offsets=bo, width=1)
+    for _ in range(max(1, args.warmup)):
+        off, ops = rfa.editops(ctx, A, B, rfa.LEVENSHTEIN)
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        off, ops = rfa.editops(ctx, A, B, rfa.LEVENSHTEIN)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    print(f"# editops_long: {ms:.3f} ms per call (host-inclusive), {ED_LONG_PAIRS / (ms / 1e3):.0f} pairs/s, "
+          f"{int(off[-1])} operations", file=sys.stderr, flush=True)
+    line = {"metric": "levenshtein_editops 512 Hirschberg pairs, pairs/s", "value": round(ED_LONG_PAIRS / (ms / 1e3), 1),
+            "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": max(1, args.warmup), "ms_per_step": round(ms, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+            "config": {"workload": f"levenshtein_editops of {ED_LONG_PAIRS} pairs, uint8 lowercase, first strings of "
+                                   "3000 units, second = first with 300 random edits (every pair split by Hirschberg); "
+                                   "whole call (host-inclusive)", "operations": int(off[-1])},
+            "roofline": None}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        if oracle_py.reference_available():
+            ref = oracle_py.Reference()
+            cores = _cores()
+            t0 = time.perf_counter()
+            counts = ref.editops_counts_u8_mt(0, ac, ao, bc, bo, threads=cores)
+            dt = time.perf_counter() - t0
+            bad = int(np.count_nonzero(counts != (off[1:] - off[:-1])))
+            for i in range(0, ED_LONG_PAIRS, ED_LONG_PAIRS // 16):
+                exp = ref.editops(0, a_strs[i].astype(np.uint32), b_strs[i].astype(np.uint32))
+                got = ops[int(off[i]):int(off[i + 1])]
+                if not (np.array_equal(got["type"], exp[0]) and np.array_equal(got["src_pos"], exp[1]) and
+                        np.array_equal(got["dest_pos"], exp[2])):
+                    bad += 1
+            line["cpu_baseline"] = {"value": round(ED_LONG_PAIRS / dt, 1), "unit": "pairs/s", "cores": cores,
+                                    "kind": "reference",
+                                    "sample": f"all {ED_LONG_PAIRS} pairs, rapidfuzz::levenshtein_editops ({dt:.2f} s)"}
+            line["check"] = ("operation counts of every pair and 16 whole scripts equal the reference"
+                             if bad == 0 else f"{bad} MISMATCHES")
+    A.close()
+    B.close()
+    ctx.close()
+    print(json.dumps(annotate_cpu(line, host_cpu())), flush=True)
+    return line
+
+
 PRA_PAIRS = 65536
 
 
@@ -1419,7 +1493,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="cdist",
                     choices=["cdist", "uneven", "serve", "latency", "dna", "positional", "partial", "tokens", "wlev",
-                             "dl", "editops", "pralign", "copy", "scorer"],
+                             "dl", "editops", "editops_long", "pralign", "copy", "scorer"],
                     help="cdist: the headline 4096x65536 configuration (default; also runs the serving leg); "
                          "uneven: 1M independent pairs; serve: the serving leg alone; latency: 1 x 10,000 "
                          "Levenshtein; dna: 256 x 16,384 long sequences with band cutoffs; copy: HBM counter "
@@ -1469,6 +1543,8 @@ def main():
         return workload_dl(args)
     if args.workload == "editops":
         return workload_editops(args)
+    if args.workload == "editops_long":
+        return workload_editops_long(args)
     if args.workload == "pralign":
         return workload_pralign(args)
 

@@ -190,8 +190,8 @@ void editops_impl(rfa_context* ctx, const rfa_strings* a, const rfa_strings* b, 
     HIP_CHECK(hipMemsetAsync(ea.counts + n, 0, 8, s));
     HIP_CHECK(hipMemsetAsync(ea.err, 0, 24, s)); // the error word, the row area, the split count
     const uint32_t nblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 3) / 4, (uint64_t)ctx->num_cus * 16));
-    // Levenshtein pairs past 64 units: one wave per 64 pairs scans for them
-    const uint32_t lblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 8));
+    // Levenshtein pairs past 64 units: the waves scan for them round robin, up to a wave per pair
+    const uint32_t lblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 3) / 4, (uint64_t)ctx->num_cus * 8));
     if (ham)
         HIP_CHECK(launch_editops_hamming(a->width, ea, false, nblocks, s));
     else

@@ -547,12 +547,13 @@ __global__ __launch_bounds__(256) void k_ed_levl(EditopsArgs a)
         }
         return;
     }
-    for (uint64_t base = wave0 * 64; base < a.n; base += nwaves * 64) {
-        const uint64_t idx = base + lane;
+    // the pairs round robin over the waves: wave w, lane l of chunk k looks at w + (64 k + l) * nwaves
+    for (uint64_t base = 0; wave0 + base * nwaves < a.n; base += 64) {
+        const uint64_t idx = wave0 + (base + lane) * nwaves;
         // 2: walked here; 3 (Hirschberg): counted here, walked as leaves
         uint64_t todo = ballot(idx < a.n && (a.info[idx].flag == 2 || (!RECORD && a.info[idx].flag == 3)));
         while (todo) {
-            const uint64_t it = base + (uint64_t)__builtin_ctzll(todo);
+            const uint64_t it = wave0 + (base + (uint64_t)__builtin_ctzll(todo)) * nwaves;
             todo &= todo - 1;
             const uint64_t pre = a.info[it].pre;
             const uint32_t la = uniform(a.info[it].la), lb = uniform(a.info[it].lb);
